@@ -1,0 +1,148 @@
+/* rsh.h -- native C ABI of the MI355X ray-space-hierarchy engine (gfx950).
+ *
+ * The engine owns device-resident scenes and per-pass scratch and runs a whole
+ * frame (primary rays, shadow rays, gated mirror bounces, shading) on one HIP
+ * stream.  It replaces the per-frame driver of the reference
+ * (projects/CUDA-Engine/source/main.cpp:630-1054 createShadowRays / castRays /
+ * colorShadowRays / createReflectionRays / colorReflectionRays, and the frame
+ * loop main.cpp:1234-1303), and the OpenGL G-buffer (main.cpp:1082-1108) by a
+ * primary-ray stage.  The stage-by-stage drop-in for the reference's
+ * extern "C" wrappers is include/rsh_engine.h.
+ *
+ * Plain C types only: device pointers are void*, streams are void*
+ * (hipStream_t), sizes are explicit.  Every call returns an rsh_status; the
+ * message of the last failure is rsh_last_error().
+ */
+#ifndef RSH_H_
+#define RSH_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum rsh_status {
+    RSH_OK = 0,
+    RSH_INVALID_ARG = 1,
+    RSH_CAPACITY = 2,
+    RSH_HIP_ERROR = 3,
+    RSH_INTERNAL = 4
+} rsh_status;
+
+/* Scene in the reference's upload layout (main.cpp:2375-2470, :2596-2656).
+ * Host pointers; the engine copies them to device memory. */
+typedef struct rsh_scene_desc {
+    uint32_t triangle_total;        /* T */
+    const float* tri_pos;           /* float4[3T] per-vertex position          */
+    const float* tri_nrm;           /* float4[3T] per-vertex normal            */
+    const int32_t* tri_mat;         /* int32[3T]  material id per vertex        */
+    uint32_t material_total;        /* M */
+    const float* mat_diffuse;       /* float4[M]  (Kd.rgb, reflectivity)       */
+    const float* mat_specular;      /* float4[M]  (Ks.rgb, Ns)                 */
+    uint32_t object_total;          /* K */
+    const float* object_spheres;    /* float3[2K] center | (radius, first, last); objects tile [0,T) in order */
+    uint32_t light_total;           /* L */
+    const float* light_pos;         /* float4[L] */
+    const float* light_col;         /* float4[L] */
+    const float* light_int;         /* float2[L] (diffuse, specular intensity) */
+} rsh_scene_desc;
+
+/* Pinhole basis of Camera::loadView (projects/CUDA-Engine/source/Camera.cpp:108-133);
+ * primary ray (x, y) = normalize(direction + right*(x/W-0.5) + up*(y/H-0.5)). */
+typedef struct rsh_camera {
+    float eye[3];
+    float direction[3];
+    float right[3];
+    float up[3];
+} rsh_camera;
+
+enum { RSH_CRSH = 0, RSH_RAH = 1 };
+enum { RSH_KEY_REF = 0, RSH_KEY_OCT = 1 };
+enum { RSH_HIT_EXACT = 0, RSH_HIT_REF = 1 };
+enum { RSH_CONE_REF = 0, RSH_CONE_GUARDED = 1 };
+
+/* Frame configuration.  Defaults of the reference (Constants.h:9-29):
+ * leaf = branch = 8 (HIERARCHY_SUBDIVISION), depth = 2, triangle_batch = 5000. */
+typedef struct rsh_frame_config {
+    uint32_t width, height;
+    uint32_t algorithm;             /* RSH_CRSH sorted hierarchy, RSH_RAH unsorted       */
+    uint32_t key_mode;              /* RSH_KEY_REF spherical bins, RSH_KEY_OCT octahedral */
+    uint32_t hit_mode;              /* RSH_HIT_EXACT 64-bit record, RSH_HIT_REF 32-bit   */
+    uint32_t cone_mode;             /* RSH_CONE_REF reference union, RSH_CONE_GUARDED    */
+    uint32_t leaf;                  /* rays per leaf node                                */
+    uint32_t branch;                /* children per inner node                           */
+    uint32_t depth;                 /* hierarchy levels (1..4)                           */
+    uint32_t triangle_batch;        /* batch size whose boundary case the culling keeps  */
+    uint32_t bounces;               /* mirror bounces after the primary hit              */
+    uint32_t reflect_gate;          /* 1: only diffuse.w > 0 materials reflect           */
+    uint32_t bounce_shadows;        /* 1: shadow rays at bounce hits                     */
+    uint32_t primary_rays;          /* must be 1 (the G-buffer comes from primary rays)  */
+    uint32_t cell_bits, dir_bits;   /* octahedral key: 3*cell_bits + 2*dir_bits + 2 <= 32 */
+    uint32_t key_layout;            /* 0 type|cell|dir, 1 type|dir|cell                  */
+    float scene_min[3];
+    float cell_scale[3];            /* cells per world unit                              */
+    uint32_t segment_shift;         /* reserved: segment id position (batched frames)    */
+    float initial_radius, initial_spread;
+} rsh_frame_config;
+
+#define RSH_MAX_PASSES 16
+
+typedef struct rsh_frame_stats {
+    uint32_t pass_count;
+    uint32_t pass_kind[RSH_MAX_PASSES];      /* 0 primary, 1 shadow, 2 reflection */
+    uint32_t pass_slots[RSH_MAX_PASSES];
+    uint32_t pass_rays[RSH_MAX_PASSES];      /* live rays after trimming */
+    uint32_t pass_chunks[RSH_MAX_PASSES];    /* packets after compression */
+    uint64_t pass_pairs_top[RSH_MAX_PASSES]; /* (top node, triangle) pairs tested */
+    uint64_t pass_pairs_leaf[RSH_MAX_PASSES];/* surviving (leaf, triangle) pairs */
+    uint64_t pass_ray_tests[RSH_MAX_PASSES]; /* ray-triangle tests */
+    float stage_ms[8];                       /* create, trim, compress, sort, decompress,
+                                                hierarchy, traversal, shading (summed) */
+    uint32_t rays_total;
+} rsh_frame_stats;
+
+typedef struct rsh_engine rsh_engine;
+
+/* Engine lifetime.  `device` is the HIP device ordinal. */
+rsh_status rsh_engine_create(rsh_engine** out, int device);
+void rsh_engine_destroy(rsh_engine* e);
+const char* rsh_last_error(const rsh_engine* e);
+rsh_status rsh_engine_set_stream(rsh_engine* e, void* hip_stream);
+
+/* Upload a scene (host arrays) and precompute per-triangle data. */
+rsh_status rsh_engine_load_scene(rsh_engine* e, const rsh_scene_desc* scene);
+
+/* Render one frame into the engine's pixel buffer.  `flags`:
+ *   bit 0  keep every pass's per-slot hit records for rsh_engine_read_pass
+ *   bit 1  collect traversal counters (pairs / tests) into the stats
+ *   bit 2  time stages with HIP events into stats->stage_ms */
+rsh_status rsh_engine_render(rsh_engine* e, const rsh_camera* cam, const rsh_frame_config* cfg,
+                             uint32_t flags, rsh_frame_stats* stats);
+
+rsh_status rsh_engine_read_pixels(rsh_engine* e, uint32_t* host_rgba, size_t count);
+/* Primary G-buffer: diffuse, specular, position, normal, each float4[W*H]. */
+rsh_status rsh_engine_read_gbuffer(rsh_engine* e, float* host_dif, float* host_spec,
+                                   float* host_pos, float* host_nrm, size_t count);
+/* Per-slot records of pass i (closest: exact or reference packing; shadow:
+ * 0 or 0xFFFFFFFF), widened to uint64.  Needs flag bit 0 at render. */
+rsh_status rsh_engine_read_pass(rsh_engine* e, uint32_t pass, uint64_t* host, size_t count);
+/* Device pointer of the pixel buffer (uint32[W*H], RGBA8 R|G<<8|B<<16). */
+void* rsh_engine_pixels_device(rsh_engine* e);
+
+/* Device memory helpers (plain HIP calls, for callers without a HIP runtime
+ * binding of their own). */
+void* rsh_device_alloc(size_t bytes);
+void rsh_device_free(void* p);
+int rsh_memcpy_htod(void* dst, const void* src, size_t bytes);
+int rsh_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int rsh_memset_d32(void* dst, uint32_t value, size_t count);
+int rsh_device_synchronize(void);
+int rsh_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RSH_H_ */

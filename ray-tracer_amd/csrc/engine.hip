@@ -1,0 +1,556 @@
+// engine.hip -- native frame driver of the RSH pipeline on one HIP stream.
+//
+// Restates the reference's per-frame orchestration
+// (projects/CUDA-Engine/source/main.cpp:630-1054 createShadowRays, castRays,
+// colorShadowRays, createReflectionRays, colorReflectionRays; frame loop
+// main.cpp:1234-1303) with these MI355X-side changes:
+//  * the scene stays resident; per-triangle bounding spheres and edge
+//    vectors are computed once at load instead of once per candidate pair;
+//  * a pass reads back two scalars (live rays, packets) to size its
+//    launches, where the reference blocks on a readback after every scan
+//    of every triangle batch (RayTracer.cu:2316,2364,2649,2720,2832);
+//  * traversal and intersection run as one fused kernel per pass, with no
+//    triangle batching, no candidate arrays and no atomics on results;
+//  * the OpenGL G-buffer is produced by a primary-ray pass through the same
+//    pipeline.
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rsh.h"
+#include "rsh_kernels.hpp"
+
+using namespace rsh;
+
+struct rsh_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+
+    // scene
+    uint32_t T = 0, K = 0, M = 0, L = 0;
+    DevBuf tri_pos, tri_nrm, tri_mat, mat_dif, mat_spec, light_pos, light_col, light_int;
+    DevBuf tri_sphere, tri_mt, obj_sphere, obj_range;
+    std::vector<float> host_spheres;   // float3[2K]
+    uint32_t ranges_tb = 0xFFFFFFFFu;  // triangle batch the exclusion ranges were built for
+
+    // frame buffers
+    uint32_t W = 0, H = 0, Lmax = 1;
+    size_t cap_slots = 0;
+    DevBuf rays_a, rays_b, flags, keys, values, tkeys, tvalues, bases, sizes, ckeys, cvalues, skeys, svalues;
+    DevBuf rkeys, rvalues, skeleton, scan, rec0, rec1, nodes, aux;
+    DevBuf closest, shadow, pixels, counters, dcounts;
+    DevBuf g0[4], gb[4];
+    ScanScratch sc;
+
+    // kept passes (flag bit 0)
+    struct Kept { DevBuf buf; uint32_t kind; uint32_t slots; };
+    std::vector<Kept> kept;
+
+    // timing (flag bit 2)
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> timed;
+    size_t ev_used = 0;
+};
+
+namespace {
+
+bool fail(rsh_engine* e, const std::string& msg) {
+    e->err = msg;
+    return false;
+}
+
+#define HIP_OK(call)                                                                       \
+    do {                                                                                   \
+        hipError_t _e = (call);                                                            \
+        if (_e != hipSuccess) {                                                            \
+            e->err = std::string(#call) + ": " + hipGetErrorString(_e);                    \
+            return RSH_HIP_ERROR;                                                          \
+        }                                                                                  \
+    } while (0)
+
+inline float4* f4p(const DevBuf& b) { return b.as<float4>(); }
+inline uint32_t* u32p(const DevBuf& b) { return b.as<uint32_t>(); }
+inline unsigned long long* u64p(const DevBuf& b) { return b.as<unsigned long long>(); }
+
+GBufView view(rsh_engine* e, DevBuf* g) { return GBufView{f4p(g[0]), f4p(g[1]), f4p(g[2]), f4p(g[3])}; }
+
+SceneView scene_view(rsh_engine* e) {
+    SceneView s;
+    s.tri_pos = f4p(e->tri_pos); s.tri_nrm = f4p(e->tri_nrm); s.tri_mat = e->tri_mat.as<int32_t>();
+    s.mat_dif = f4p(e->mat_dif); s.mat_spec = f4p(e->mat_spec);
+    s.lights = Lights{f4p(e->light_pos), f4p(e->light_col), e->light_int.as<float2>(), e->L};
+    return s;
+}
+
+struct StageTimer {
+    rsh_engine* e;
+    int stage;
+    bool on;
+    hipEvent_t a = nullptr, b = nullptr;
+    StageTimer(rsh_engine* en, int st, bool enabled) : e(en), stage(st), on(enabled) {
+        if (!on) return;
+        if (e->ev_used + 2 > e->ev_pool.size()) {
+            for (int i = 0; i < 64; i++) {
+                hipEvent_t ev;
+                if (hipEventCreate(&ev) == hipSuccess) e->ev_pool.push_back(ev);
+            }
+        }
+        if (e->ev_used + 2 > e->ev_pool.size()) { on = false; return; }
+        a = e->ev_pool[e->ev_used++];
+        b = e->ev_pool[e->ev_used++];
+        (void)hipEventRecord(a, e->stream);
+    }
+    ~StageTimer() {
+        if (!on) return;
+        (void)hipEventRecord(b, e->stream);
+        e->timed.push_back({stage, {a, b}});
+    }
+};
+
+enum { ST_CREATE = 0, ST_TRIM, ST_COMPRESS, ST_SORT, ST_DECOMPRESS, ST_HIER, ST_TRAVERSE, ST_SHADE };
+
+rsh_status ensure_frame(rsh_engine* e, uint32_t W, uint32_t H, uint32_t Lmax) {
+    size_t S = (size_t)W * H * Lmax, P = (size_t)W * H;
+    e->W = W; e->H = H; e->Lmax = Lmax;
+    if (S <= e->cap_slots) return RSH_OK;
+    size_t u = S * 4;
+    HIP_OK(e->rays_a.reserve(S * 24)); HIP_OK(e->rays_b.reserve(S * 24));
+    for (DevBuf* b : {&e->flags, &e->keys, &e->values, &e->tkeys, &e->tvalues, &e->bases, &e->sizes, &e->ckeys,
+                      &e->cvalues, &e->skeys, &e->svalues, &e->rkeys, &e->rvalues, &e->skeleton, &e->scan, &e->shadow})
+        HIP_OK(b->reserve(u));
+    HIP_OK(e->closest.reserve(S * 8));
+    HIP_OK(e->rec0.reserve(S * 16)); HIP_OK(e->rec1.reserve(S * 16));
+    HIP_OK(e->pixels.reserve(P * 4));
+    for (int i = 0; i < 4; i++) { HIP_OK(e->g0[i].reserve(P * 16)); HIP_OK(e->gb[i].reserve(P * 16)); }
+    HIP_OK(e->counters.reserve(64)); HIP_OK(e->dcounts.reserve(64));
+    HIP_OK(e->sc.reserve(S));
+    e->cap_slots = S;
+    return RSH_OK;
+}
+
+rsh_status build_ranges(rsh_engine* e, uint32_t Tb) {
+    if (Tb == e->ranges_tb) return RSH_OK;
+    std::vector<uint32_t> r(4 * (size_t)e->K);
+    for (uint32_t k = 0; k < e->K; k++) {
+        uint32_t y = (uint32_t)e->host_spheres[6 * k + 4], z = (uint32_t)e->host_spheres[6 * k + 5];
+        uint32_t lo = 0, hi = 0;
+        // RayTracer.cu:1192-1244 has no branch for an object that starts before a
+        // batch and ends exactly at the batch's upper bound: the batch's pairs of
+        // that object are never candidates.
+        if (Tb && z != 0 && z % Tb == 0 && y < z - Tb) { lo = z - Tb; hi = z; }
+        r[4 * k] = y; r[4 * k + 1] = z; r[4 * k + 2] = lo; r[4 * k + 3] = hi;
+    }
+    HIP_OK(hipMemcpyAsync(e->obj_range.p, r.data(), r.size() * 4, hipMemcpyHostToDevice, e->stream));
+    HIP_OK(hipStreamSynchronize(e->stream));
+    e->ranges_tb = Tb;
+    return RSH_OK;
+}
+
+uint32_t key_bits(const rsh_frame_config& c) {
+    if (c.key_mode == RSH_KEY_REF) return 32;
+    uint32_t b = 2 + 3 * c.cell_bits + 2 * c.dir_bits;
+    if (c.segment_shift) b = 32;
+    return b > 32 ? 32 : b;
+}
+
+KeyCfg key_cfg(const rsh_frame_config& c) {
+    KeyCfg k{};
+    k.mode = c.key_mode;
+    k.h.cell_bits = c.cell_bits; k.h.dir_bits = c.dir_bits; k.h.layout = c.key_layout;
+    k.h.segment_shift = c.segment_shift;
+    k.h.min_x = c.scene_min[0]; k.h.min_y = c.scene_min[1]; k.h.min_z = c.scene_min[2];
+    k.h.scale_x = c.cell_scale[0]; k.h.scale_y = c.cell_scale[1]; k.h.scale_z = c.cell_scale[2];
+    k.segment = 0;
+    return k;
+}
+
+struct PassOut { uint32_t N = 0, C = 0; uint64_t top = 0, leaf = 0, tests = 0; };
+
+// One ray batch: trim -> (compress -> sort -> decompress) -> gather ->
+// hierarchy -> fused traversal + intersection.  kind 0: closest hit into
+// e->closest; kind 1: shadow flags into e->shadow.
+rsh_status trace_pass(rsh_engine* e, const rsh_frame_config& cfg, const float* rays, uint32_t S, uint32_t kind,
+                      const float4* shadow_pos, uint32_t flags_mask, PassOut& out) {
+    hipStream_t s = e->stream;
+    const bool timing = flags_mask & 4u, counting = flags_mask & 2u;
+    uint32_t* dc = u32p(e->dcounts);
+    {
+        StageTimer t(e, ST_TRIM, timing);
+        HIP_OK(run_trim(u32p(e->flags), u32p(e->keys), u32p(e->values), S, u32p(e->tkeys), u32p(e->tvalues), dc,
+                        e->sc, s));
+    }
+    uint32_t hc[2] = {0, 0};
+    HIP_OK(hipMemcpyAsync(hc, dc, 4, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    const uint32_t N = hc[0];
+    out.N = N;
+    if (kind == 0) launch_fill_u64(u64p(e->closest), cfg.hit_mode == 0 ? MISS64 : (unsigned long long)MISS32, S, s);
+    else launch_fill_u32(u32p(e->shadow), 0u, S, s);
+    if (N == 0) return RSH_OK;
+    const uint32_t* sorted_values = u32p(e->tvalues);
+    if (cfg.algorithm == RSH_CRSH) {
+        {
+            StageTimer t(e, ST_COMPRESS, timing);
+            HIP_OK(run_compress(u32p(e->tkeys), N, u32p(e->bases), u32p(e->sizes), u32p(e->ckeys), u32p(e->cvalues),
+                                dc + 1, e->sc, s));
+        }
+        HIP_OK(hipMemcpyAsync(hc + 1, dc + 1, 4, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        const uint32_t C = hc[1];
+        out.C = C;
+        {
+            StageTimer t(e, ST_SORT, timing);
+            HIP_OK(run_radix_sort(u32p(e->ckeys), u32p(e->cvalues), C, key_bits(cfg), u32p(e->skeys),
+                                  u32p(e->svalues), e->sc, s));
+        }
+        {
+            StageTimer t(e, ST_DECOMPRESS, timing);
+            HIP_OK(run_decompress(u32p(e->bases), u32p(e->sizes), u32p(e->svalues), u32p(e->tkeys), u32p(e->tvalues),
+                                  u32p(e->skeleton), u32p(e->scan), C, N, u32p(e->rkeys), u32p(e->rvalues), e->sc, s));
+        }
+        sorted_values = u32p(e->rvalues);
+    } else {
+        out.C = N;
+    }
+    uint32_t counts[4] = {0, 0, 0, 0}, offs[4] = {0, 0, 0, 0};
+    level_sizes(N, cfg.leaf, cfg.branch, cfg.depth, counts, offs);
+    {
+        StageTimer t(e, ST_HIER, timing);
+        launch_gather(rays, sorted_values, N, kind == 1 ? shadow_pos : nullptr, e->W, e->Lmax, f4p(e->rec0),
+                      f4p(e->rec1), s);
+        launch_hierarchy(f4p(e->rec0), f4p(e->rec1), nullptr, nullptr, N, cfg.leaf, cfg.branch, cfg.depth,
+                         cfg.initial_radius, cfg.initial_spread, cfg.cone_mode, f4p(e->nodes), e->aux.as<float2>(),
+                         counts, offs, s);
+        HIP_OK(hipGetLastError());
+    }
+    TravParams p{};
+    p.nodes = f4p(e->nodes); p.aux = e->aux.as<float2>();
+    for (int k = 0; k < 4; k++) { p.level_offset[k] = offs[k]; p.level_count[k] = counts[k]; }
+    p.depth = cfg.depth; p.leaf = cfg.leaf; p.branch = cfg.branch;
+    uint32_t rpt = cfg.leaf;
+    for (uint32_t k = 1; k < cfg.depth; k++) rpt *= cfg.branch;
+    p.rays_per_top = rpt;
+    p.waves_per_top = (rpt + 63) / 64;
+    p.rec0 = f4p(e->rec0); p.rec1 = f4p(e->rec1); p.N = N;
+    p.tri_sphere = f4p(e->tri_sphere); p.tri_mt = f4p(e->tri_mt); p.T = e->T;
+    p.obj_sphere = f4p(e->obj_sphere); p.obj_range = e->obj_range.as<uint4>(); p.K = e->K;
+    p.algorithm = cfg.algorithm; p.kind = kind; p.hit_mode = cfg.hit_mode;
+    p.closest = u64p(e->closest); p.shadow = u32p(e->shadow);
+    p.counters = counting ? u64p(e->counters) : nullptr;
+    if (counting) launch_fill_u64(u64p(e->counters), 0ull, 3, s);
+    {
+        StageTimer t(e, ST_TRAVERSE, timing);
+        HIP_OK(launch_traverse(p, s));
+    }
+    if (counting) {
+        unsigned long long c[3];
+        HIP_OK(hipMemcpyAsync(c, e->counters.p, sizeof(c), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        out.top = c[0]; out.leaf = c[1]; out.tests = c[2];
+    }
+    return RSH_OK;
+}
+
+rsh_status record(rsh_engine* e, rsh_frame_stats* st, uint32_t kind, uint32_t S, const PassOut& po, uint32_t fl) {
+    if (st && st->pass_count < RSH_MAX_PASSES) {
+        uint32_t i = st->pass_count++;
+        st->pass_kind[i] = kind; st->pass_slots[i] = S; st->pass_rays[i] = po.N; st->pass_chunks[i] = po.C;
+        st->pass_pairs_top[i] = po.top; st->pass_pairs_leaf[i] = po.leaf; st->pass_ray_tests[i] = po.tests;
+        st->rays_total += po.N;
+    }
+    if (fl & 1u) {
+        rsh_engine::Kept k;
+        k.kind = kind; k.slots = S;
+        size_t bytes = kind == 1 ? (size_t)S * 4 : (size_t)S * 8;
+        HIP_OK(k.buf.reserve(bytes));
+        HIP_OK(hipMemcpyAsync(k.buf.p, kind == 1 ? e->shadow.p : e->closest.p, bytes, hipMemcpyDeviceToDevice,
+                              e->stream));
+        e->kept.push_back(k);
+    }
+    return RSH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+rsh_status rsh_engine_create(rsh_engine** out, int device) {
+    if (!out) return RSH_INVALID_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return RSH_HIP_ERROR;
+    if (hipSetDevice(device) != hipSuccess) return RSH_HIP_ERROR;
+    rsh_engine* e = new rsh_engine();
+    e->device = device;
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) { delete e; return RSH_HIP_ERROR; }
+    e->own_stream = true;
+    *out = e;
+    return RSH_OK;
+}
+
+void rsh_engine_destroy(rsh_engine* e) {
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    for (DevBuf* b : {&e->tri_pos, &e->tri_nrm, &e->tri_mat, &e->mat_dif, &e->mat_spec, &e->light_pos, &e->light_col,
+                      &e->light_int, &e->tri_sphere, &e->tri_mt, &e->obj_sphere, &e->obj_range, &e->rays_a,
+                      &e->rays_b, &e->flags, &e->keys, &e->values, &e->tkeys, &e->tvalues, &e->bases, &e->sizes,
+                      &e->ckeys, &e->cvalues, &e->skeys, &e->svalues, &e->rkeys, &e->rvalues, &e->skeleton, &e->scan,
+                      &e->rec0, &e->rec1, &e->nodes, &e->aux, &e->closest, &e->shadow, &e->pixels, &e->counters,
+                      &e->dcounts})
+        b->release();
+    for (int i = 0; i < 4; i++) { e->g0[i].release(); e->gb[i].release(); }
+    for (auto& k : e->kept) k.buf.release();
+    e->sc.release();
+    for (hipEvent_t ev : e->ev_pool) (void)hipEventDestroy(ev);
+    if (e->own_stream && e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+const char* rsh_last_error(const rsh_engine* e) { return e ? e->err.c_str() : "null engine"; }
+
+rsh_status rsh_engine_set_stream(rsh_engine* e, void* stream) {
+    if (!e) return RSH_INVALID_ARG;
+    if (e->own_stream && e->stream) (void)hipStreamDestroy(e->stream);
+    e->stream = static_cast<hipStream_t>(stream);
+    e->own_stream = false;
+    return RSH_OK;
+}
+
+rsh_status rsh_engine_load_scene(rsh_engine* e, const rsh_scene_desc* s) {
+    if (!e || !s || !s->tri_pos || !s->tri_nrm || !s->tri_mat || !s->mat_diffuse || !s->mat_specular ||
+        !s->object_spheres || !s->light_pos || !s->light_col || !s->light_int)
+        return e ? (fail(e, "load_scene: null argument"), RSH_INVALID_ARG) : RSH_INVALID_ARG;
+    if (s->triangle_total == 0 || s->object_total == 0 || s->light_total == 0 || s->material_total == 0) {
+        e->err = "load_scene: empty scene";
+        return RSH_INVALID_ARG;
+    }
+    // Objects must tile [0, T) in order (the culling closed form relies on it).
+    uint32_t next = 0;
+    for (uint32_t k = 0; k < s->object_total; k++) {
+        float fy = s->object_spheres[6 * k + 4], fz = s->object_spheres[6 * k + 5];
+        if (!(fy >= 0.0f) || !(fz >= fy) || (uint32_t)fy != next || fz >= (float)s->triangle_total) {
+            e->err = "load_scene: object triangle ranges must tile [0, T) in order";
+            return RSH_INVALID_ARG;
+        }
+        next = (uint32_t)fz + 1;
+    }
+    if (next != s->triangle_total) { e->err = "load_scene: objects do not cover every triangle"; return RSH_INVALID_ARG; }
+    for (uint32_t i = 0; i < 3 * s->triangle_total; i++) {
+        if (s->tri_mat[i] < 0 || (uint32_t)s->tri_mat[i] >= s->material_total) {
+            e->err = "load_scene: material id out of range";
+            return RSH_INVALID_ARG;
+        }
+    }
+    HIP_OK(hipSetDevice(e->device));
+    const uint32_t T = s->triangle_total, K = s->object_total, M = s->material_total, L = s->light_total;
+    e->T = T; e->K = K; e->M = M; e->L = L;
+    HIP_OK(e->tri_pos.reserve((size_t)T * 48)); HIP_OK(e->tri_nrm.reserve((size_t)T * 48));
+    HIP_OK(e->tri_mat.reserve((size_t)T * 12));
+    HIP_OK(e->mat_dif.reserve((size_t)M * 16)); HIP_OK(e->mat_spec.reserve((size_t)M * 16));
+    HIP_OK(e->light_pos.reserve((size_t)L * 16)); HIP_OK(e->light_col.reserve((size_t)L * 16));
+    HIP_OK(e->light_int.reserve((size_t)L * 8));
+    HIP_OK(e->tri_sphere.reserve((size_t)T * 16)); HIP_OK(e->tri_mt.reserve((size_t)T * 48));
+    HIP_OK(e->obj_sphere.reserve((size_t)K * 16)); HIP_OK(e->obj_range.reserve((size_t)K * 16));
+    hipStream_t st = e->stream;
+    HIP_OK(hipMemcpyAsync(e->tri_pos.p, s->tri_pos, (size_t)T * 48, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(e->tri_nrm.p, s->tri_nrm, (size_t)T * 48, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(e->tri_mat.p, s->tri_mat, (size_t)T * 12, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(e->mat_dif.p, s->mat_diffuse, (size_t)M * 16, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(e->mat_spec.p, s->mat_specular, (size_t)M * 16, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(e->light_pos.p, s->light_pos, (size_t)L * 16, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(e->light_col.p, s->light_col, (size_t)L * 16, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(e->light_int.p, s->light_int, (size_t)L * 8, hipMemcpyHostToDevice, st));
+    e->host_spheres.assign(s->object_spheres, s->object_spheres + 6 * (size_t)K);
+    std::vector<float> s4(4 * (size_t)K);
+    for (uint32_t k = 0; k < K; k++)
+        for (int i = 0; i < 4; i++) s4[4 * k + i] = s->object_spheres[6 * k + i];
+    HIP_OK(hipMemcpyAsync(e->obj_sphere.p, s4.data(), s4.size() * 4, hipMemcpyHostToDevice, st));
+    launch_triangle_prep(f4p(e->tri_pos), T, f4p(e->tri_sphere), f4p(e->tri_mt), st);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(st));
+    e->ranges_tb = 0xFFFFFFFFu;
+    return RSH_OK;
+}
+
+rsh_status rsh_engine_render(rsh_engine* e, const rsh_camera* cam, const rsh_frame_config* cfgp, uint32_t fl,
+                             rsh_frame_stats* st) {
+    if (!e || !cam || !cfgp) return RSH_INVALID_ARG;
+    const rsh_frame_config cfg = *cfgp;
+    if (e->T == 0) { e->err = "render: no scene loaded"; return RSH_INVALID_ARG; }
+    if (cfg.width == 0 || cfg.height == 0 || cfg.leaf == 0 || cfg.branch == 0 || cfg.depth == 0 || cfg.depth > 4 ||
+        cfg.primary_rays != 1 || cfg.algorithm > 1 || cfg.key_mode > 1 || cfg.hit_mode > 1 || cfg.cone_mode > 1) {
+        e->err = "render: invalid configuration";
+        return RSH_INVALID_ARG;
+    }
+    if (cfg.key_mode == RSH_KEY_OCT && (cfg.cell_bits > 8 || cfg.dir_bits > 10 || cfg.cell_bits == 0 ||
+                                        cfg.dir_bits == 0 || 2 + 3 * cfg.cell_bits + 2 * cfg.dir_bits > 32)) {
+        e->err = "render: octahedral key does not fit 32 bits";
+        return RSH_INVALID_ARG;
+    }
+    if (cfg.hit_mode == RSH_HIT_REF && e->T > (1u << 20)) {
+        e->err = "render: reference hit packing holds 20-bit triangle ids";
+        return RSH_CAPACITY;
+    }
+    const uint64_t S64 = (uint64_t)cfg.width * cfg.height * e->L;
+    if (S64 > (uint64_t)MAX_TILES * TILE || S64 > 0xFFFFFFFFull) { e->err = "render: frame exceeds slot capacity"; return RSH_CAPACITY; }
+    HIP_OK(hipSetDevice(e->device));
+    rsh_status rs = ensure_frame(e, cfg.width, cfg.height, e->L);
+    if (rs != RSH_OK) return rs;
+    rs = build_ranges(e, cfg.triangle_batch);
+    if (rs != RSH_OK) return rs;
+    {
+        uint32_t cnt[4], off[4];
+        size_t total = level_sizes((uint32_t)S64, cfg.leaf, cfg.branch, cfg.depth, cnt, off);
+        HIP_OK(e->nodes.reserve(total * 32 + 64));
+        HIP_OK(e->aux.reserve(total * 8 + 64));
+    }
+    for (auto& k : e->kept) k.buf.release();
+    e->kept.clear();
+    e->timed.clear();
+    e->ev_used = 0;
+    if (st) std::memset(st, 0, sizeof(*st));
+    hipStream_t s = e->stream;
+    const uint32_t W = cfg.width, H = cfg.height, Lm = e->L, P = W * H, S = W * H * Lm;
+    const KeyCfg kc = key_cfg(cfg);
+    const SceneView sv = scene_view(e);
+    const v3 eye{cam->eye[0], cam->eye[1], cam->eye[2]};
+    const bool timing = fl & 4u;
+    GBufView G0 = view(e, e->g0), GB = view(e, e->gb);
+    launch_fill_u32(u32p(e->pixels), 0u, P, s);
+
+    // primary pass -> G-buffer
+    {
+        StageTimer t(e, ST_CREATE, timing);
+        launch_create_primary(W, H, eye, v3{cam->direction[0], cam->direction[1], cam->direction[2]},
+                              v3{cam->right[0], cam->right[1], cam->right[2]}, v3{cam->up[0], cam->up[1], cam->up[2]},
+                              kc, e->rays_b.as<float>(), u32p(e->flags), u32p(e->keys), u32p(e->values), s);
+    }
+    PassOut po;
+    rs = trace_pass(e, cfg, e->rays_b.as<float>(), P, 0, nullptr, fl, po);
+    if (rs != RSH_OK) return rs;
+    if ((rs = record(e, st, 0, P, po, fl)) != RSH_OK) return rs;
+    {
+        StageTimer t(e, ST_SHADE, timing);
+        launch_surface(e->rays_b.as<float>(), u64p(e->closest), cfg.hit_mode, W, H, 1, 1, sv, G0, s);
+    }
+    // shadow pass + primary shading
+    {
+        StageTimer t(e, ST_CREATE, timing);
+        launch_create_shadow(G0.pos, G0.nrm, W, H, Lm, f4p(e->light_pos), kc, e->rays_b.as<float>(), u32p(e->flags),
+                             u32p(e->keys), u32p(e->values), s);
+    }
+    rs = trace_pass(e, cfg, e->rays_b.as<float>(), S, 1, G0.pos, fl, po);
+    if (rs != RSH_OK) return rs;
+    if ((rs = record(e, st, 1, S, po, fl)) != RSH_OK) return rs;
+    if (po.N > 0) {
+        StageTimer t(e, ST_SHADE, timing);
+        launch_shade_primary(G0, W, H, Lm, sv.lights, eye, u32p(e->shadow), u32p(e->pixels), s);
+    }
+    // mirror bounces
+    for (uint32_t b = 1; b <= cfg.bounces; b++) {
+        {
+            StageTimer t(e, ST_CREATE, timing);
+            if (b == 1)
+                launch_create_reflection(G0.pos, G0.nrm, G0.dif, W, H, Lm, eye, cfg.reflect_gate, 1, kc,
+                                         e->rays_a.as<float>(), u32p(e->flags), u32p(e->keys), u32p(e->values), s);
+            else
+                launch_create_bounce(GB, W, H, Lm, u64p(e->closest), cfg.hit_mode, cfg.reflect_gate, kc,
+                                     e->rays_a.as<float>(), u32p(e->flags), u32p(e->keys), u32p(e->values), s);
+        }
+        rs = trace_pass(e, cfg, e->rays_a.as<float>(), S, 0, nullptr, fl, po);
+        if (rs != RSH_OK) return rs;
+        if ((rs = record(e, st, 2, S, po, fl)) != RSH_OK) return rs;
+        if (po.N == 0) break;
+        {
+            StageTimer t(e, ST_SHADE, timing);
+            launch_surface(e->rays_a.as<float>(), u64p(e->closest), cfg.hit_mode, W, H, Lm, 0, sv, GB, s);
+        }
+        const uint32_t* sflags = nullptr;
+        if (cfg.bounce_shadows) {
+            {
+                StageTimer t(e, ST_CREATE, timing);
+                launch_create_shadow(GB.pos, GB.nrm, W, H, Lm, f4p(e->light_pos), kc, e->rays_b.as<float>(),
+                                     u32p(e->flags), u32p(e->keys), u32p(e->values), s);
+            }
+            PassOut sp;
+            rs = trace_pass(e, cfg, e->rays_b.as<float>(), S, 1, GB.pos, fl, sp);
+            if (rs != RSH_OK) return rs;
+            if ((rs = record(e, st, 1, S, sp, fl)) != RSH_OK) return rs;
+            sflags = u32p(e->shadow);
+        }
+        {
+            StageTimer t(e, ST_SHADE, timing);
+            launch_shade_reflection(GB, W, H, Lm, sv.lights, eye, u64p(e->closest), cfg.hit_mode, sflags,
+                                    u32p(e->pixels), s);
+        }
+    }
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(s));
+    if (st && timing) {
+        for (auto& t : e->timed) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, t.second.first, t.second.second) == hipSuccess) st->stage_ms[t.first] += ms;
+        }
+    }
+    return RSH_OK;
+}
+
+rsh_status rsh_engine_read_pixels(rsh_engine* e, uint32_t* host, size_t count) {
+    if (!e || !host || count > (size_t)e->W * e->H) return RSH_INVALID_ARG;
+    HIP_OK(hipMemcpy(host, e->pixels.p, count * 4, hipMemcpyDeviceToHost));
+    return RSH_OK;
+}
+
+rsh_status rsh_engine_read_gbuffer(rsh_engine* e, float* dif, float* spec, float* pos, float* nrm, size_t count) {
+    if (!e || count > (size_t)e->W * e->H) return RSH_INVALID_ARG;
+    float* dst[4] = {pos, nrm, dif, spec};
+    for (int i = 0; i < 4; i++)
+        if (dst[i]) HIP_OK(hipMemcpy(dst[i], e->g0[i].p, count * 16, hipMemcpyDeviceToHost));
+    return RSH_OK;
+}
+
+rsh_status rsh_engine_read_pass(rsh_engine* e, uint32_t pass, uint64_t* host, size_t count) {
+    if (!e || !host || pass >= e->kept.size()) return RSH_INVALID_ARG;
+    const auto& k = e->kept[pass];
+    if (count > k.slots) return RSH_INVALID_ARG;
+    if (k.kind == 1) {
+        std::vector<uint32_t> tmp(count);
+        HIP_OK(hipMemcpy(tmp.data(), k.buf.p, count * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < count; i++) host[i] = tmp[i];
+    } else {
+        HIP_OK(hipMemcpy(host, k.buf.p, count * 8, hipMemcpyDeviceToHost));
+    }
+    return RSH_OK;
+}
+
+void* rsh_engine_pixels_device(rsh_engine* e) { return e ? e->pixels.p : nullptr; }
+
+void* rsh_device_alloc(size_t bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) return nullptr;
+    return p;
+}
+void rsh_device_free(void* p) { if (p) (void)hipFree(p); }
+int rsh_memcpy_htod(void* dst, const void* src, size_t bytes) {
+    return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? 0 : -1;
+}
+int rsh_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+    return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -1;
+}
+int rsh_memset_d32(void* dst, uint32_t value, size_t count) {
+    launch_fill_u32(static_cast<uint32_t*>(dst), value, count, nullptr);
+    if (hipGetLastError() != hipSuccess) return -1;
+    return hipDeviceSynchronize() == hipSuccess ? 0 : -1;
+}
+int rsh_device_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : -1; }
+int rsh_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+}  // extern "C"

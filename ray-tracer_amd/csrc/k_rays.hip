@@ -1,0 +1,384 @@
+// k_rays.hip -- ray creation and hashing, surface records and shading (gfx950).
+//
+// Reference: projects/CUDA-Engine/source/RayTracer.cu
+//   UpdateVertex :448-511, UpdateBoundingSphere :513-570
+//   CreateShadowRays :682-740, CreateReflectionRays :742-791
+//   ColorPrimaryShadowRay :1616-1709, ColorReflectionRay :1776-1882,
+//   ColorAndCreateReflectionRay :1884-2017
+// plus the primary-ray stage that replaces the OpenGL G-buffer
+// (main.cpp:1082-1108; formula of RayTracer.cu:667).
+//
+// All kernels are one thread per slot or pixel with coalesced row-major
+// access; the ray buffer keeps the reference's interleaved layout
+// (origin at 2*slot, direction at 2*slot+1, float3), written as float2 pairs.
+#include "rsh_kernels.hpp"
+
+namespace rsh {
+
+namespace {
+
+inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap = 1u << 20) {
+    uint64_t g = (n + block - 1) / block;
+    if (g == 0) g = 1;
+    return (uint32_t)(g < cap ? g : cap);
+}
+
+__device__ __forceinline__ void store_ray(float* rays, size_t slot, v3 o, v3 d) {
+    float2* r = reinterpret_cast<float2*>(rays + 6 * slot);
+    r[0] = make_float2(o.x, o.y);
+    r[1] = make_float2(o.z, d.x);
+    r[2] = make_float2(d.y, d.z);
+}
+__device__ __forceinline__ void load_ray(const float* rays, size_t slot, v3& o, v3& d) {
+    const float2* r = reinterpret_cast<const float2*>(rays + 6 * slot);
+    float2 a = r[0], b = r[1], c = r[2];
+    o = mk(a.x, a.y, b.x);
+    d = mk(b.y, c.x, c.y);
+}
+
+// type: 0 primary, 1 shadow (light l), 2 mirror
+__device__ __forceinline__ uint32_t make_key(const KeyCfg& kc, v3 o, v3 d, uint32_t type, uint32_t l) {
+    if (kc.mode == 0) return type == 1 ? shadow_key_ref(d, l) : reflection_key_ref(o, d);
+    uint32_t k = key_oct(kc.h, o, d, type);
+    if (kc.h.segment_shift) k |= kc.segment << kc.h.segment_shift;
+    return k;
+}
+
+__global__ void __launch_bounds__(256) k_create_primary(uint32_t W, uint32_t H, v3 eye, v3 D, v3 R, v3 U, KeyCfg kc,
+                                                        float* rays, uint32_t* flags, uint32_t* keys,
+                                                        uint32_t* values) {
+    const uint32_t n = W * H;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        uint32_t x = p % W, y = p / W;
+        float a = (float)x / (float)W - 0.5f;
+        float c = (float)y / (float)H - 0.5f;
+        v3 d = normalize((D + R * a) + U * c);
+        store_ray(rays, p, eye, d);
+        keys[p] = make_key(kc, eye, d, 0, 0);
+        values[p] = p;
+        flags[p] = 0;
+    }
+}
+
+// RayTracer.cu:682-740; slot i = x + W*(l + Lmax*y).
+__global__ void __launch_bounds__(256) k_create_shadow(const float4* gpos, const float4* gnrm, uint32_t W, uint32_t H,
+                                                       uint32_t Lmax, const float4* light_pos, KeyCfg kc, float* rays,
+                                                       uint32_t* flags, uint32_t* keys, uint32_t* values) {
+    const uint32_t n = W * H * Lmax;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint32_t x = i % W, y = i / (W * Lmax), l = (i / W) % Lmax;
+        size_t px = (size_t)x + (size_t)y * W;
+        v3 P = xyz(gpos[px]);
+        v3 N = normalize(xyz(gnrm[px]));
+        uint32_t flag = 1;
+        if (len(P) != 0.0f) {
+            v3 o = xyz(light_pos[l]);
+            v3 d = normalize(P - o);
+            float diffuse = fmaxf(dot(d, -N), 0.0f);
+            if (diffuse > 0.0f) {
+                store_ray(rays, i, o, d);
+                keys[i] = make_key(kc, o, d, 1, l);
+                values[i] = i;
+                flag = 0;
+            }
+        }
+        flags[i] = flag;
+    }
+}
+
+// RayTracer.cu:742-791; slot x + y*W*Lmax.  gate: only diffuse.w > 0 reflects.
+// mark_gaps: also flag the unused slots between rows when Lmax > 1.
+__global__ void __launch_bounds__(256) k_create_reflection(const float4* gpos, const float4* gnrm, const float4* gdif,
+                                                           uint32_t W, uint32_t H, uint32_t Lmax, v3 eye,
+                                                           uint32_t gate, uint32_t mark_gaps, KeyCfg kc, float* rays,
+                                                           uint32_t* flags, uint32_t* keys, uint32_t* values) {
+    const uint32_t n = W * H;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        uint32_t x = p % W, y = p / W;
+        uint32_t slot = x + y * W * Lmax;
+        v3 P = xyz(gpos[p]);
+        v3 N = normalize(xyz(gnrm[p]));
+        bool live = len(P) != 0.0f && (!gate || gdif[p].w > 0.0f);
+        if (live) {
+            v3 d = reflect(normalize(P - eye), normalize(N));
+            store_ray(rays, slot, P, d);
+            keys[slot] = make_key(kc, P, d, 2, 0);
+            values[slot] = slot;
+            flags[slot] = 0;
+        } else {
+            flags[slot] = 1;
+        }
+        if (mark_gaps)
+            for (uint32_t j = 1; j < Lmax; j++) flags[slot + j * W] = 1;
+    }
+}
+
+// Next bounce (RayTracer.cu:1995-2010): origin = hit point, direction =
+// reflect(previous direction, barycentric normal), in place.
+__global__ void __launch_bounds__(256) k_create_bounce(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax,
+                                                       const unsigned long long* rec, uint32_t hit_mode,
+                                                       uint32_t gate, KeyCfg kc, float* rays, uint32_t* flags,
+                                                       uint32_t* keys, uint32_t* values) {
+    const uint32_t n = W * H;
+    const unsigned long long miss = hit_mode == 0 ? MISS64 : (unsigned long long)MISS32;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        uint32_t x = p % W, y = p / W;
+        uint32_t slot = x + y * W * Lmax;
+        bool live = rec[slot] != miss && (!gate || g.dif[p].w > 0.0f);
+        if (live) {
+            v3 o, d;
+            load_ray(rays, slot, o, d);
+            v3 P = xyz(g.pos[p]), N = xyz(g.nrm[p]);
+            v3 nd = reflect(d, N);
+            store_ray(rays, slot, P, nd);
+            keys[slot] = make_key(kc, P, nd, 2, 0);
+            values[slot] = slot;
+            flags[slot] = 0;
+        } else {
+            flags[slot] = 1;
+        }
+        for (uint32_t j = 1; j < Lmax; j++) flags[slot + j * W] = 1;
+    }
+}
+
+// Surface record at a hit (RayTracer.cu:1807-1843): point o + d*(eps + t),
+// area-ratio barycentric normal, material of the triangle's first vertex.
+__global__ void __launch_bounds__(256) k_surface(const float* rays, const unsigned long long* rec, uint32_t hit_mode,
+                                                 uint32_t W, uint32_t H, uint32_t slot_stride, uint32_t normalize_n,
+                                                 SceneView sc, GBufView g) {
+    const uint32_t n = W * H;
+    const unsigned long long miss = hit_mode == 0 ? MISS64 : (unsigned long long)MISS32;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        uint32_t x = p % W, y = p / W;
+        uint32_t slot = x + y * W * slot_stride;
+        unsigned long long r = rec[slot];
+        if (r == miss) {
+            float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            g.pos[p] = z; g.nrm[p] = z; g.dif[p] = z; g.spec[p] = z;
+            continue;
+        }
+        uint32_t t = hit_mode == 0 ? (uint32_t)(r & 0xFFFFFFFFull) : (uint32_t)(r & 0x000FFFFFull);
+        v3 v0 = xyz(sc.tri_pos[3 * (size_t)t]), v1 = xyz(sc.tri_pos[3 * (size_t)t + 1]);
+        v3 v2 = xyz(sc.tri_pos[3 * (size_t)t + 2]);
+        v3 o, d;
+        load_ray(rays, slot, o, d);
+        float th = ray_triangle(o + d * EPS, d, v0, v1 - v0, v2 - v0);
+        v3 P = o + d * (EPS + th);
+        float aABC = len(cross(v1 - v0, v2 - v0));
+        float aPBC = len(cross(v1 - P, v2 - P));
+        float aPCA = len(cross(v0 - P, v2 - P));
+        v3 N = ((aPBC / aABC) * xyz(sc.tri_nrm[3 * (size_t)t]) + (aPCA / aABC) * xyz(sc.tri_nrm[3 * (size_t)t + 1])) +
+               ((1.0f - (aPBC / aABC)) - (aPCA / aABC)) * xyz(sc.tri_nrm[3 * (size_t)t + 2]);
+        if (normalize_n) N = normalize(N);
+        int32_t m = sc.tri_mat[3 * (size_t)t];
+        g.pos[p] = f4(P, 1.0f);
+        g.nrm[p] = f4(N, 0.0f);
+        g.dif[p] = sc.mat_dif[m];
+        g.spec[p] = sc.mat_spec[m];
+    }
+}
+
+// RayTracer.cu:1616-1709 ColorPrimaryShadowRay
+__global__ void __launch_bounds__(256) k_shade_primary(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L,
+                                                       v3 eye, const uint32_t* sflags, uint32_t* pixels) {
+    const uint32_t n = W * H;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        uint32_t x = p % W, y = p / W;
+        v3 color = mk(0.0f, 0.0f, 0.0f);
+        v3 P = xyz(g.pos[p]), N = xyz(g.nrm[p]);
+        if (len(P) != 0.0f) {
+            float4 Kd = g.dif[p], Ks = g.spec[p];
+            for (uint32_t l = 0; l < L.total; l++) {
+                if (sflags[(size_t)(x + y * W * Lmax) + (size_t)W * l] != MISS32)
+                    color = color + light_term(L, l, P, N, eye, Kd, Ks, false);
+            }
+        }
+        pixels[p] = rgb_to_int(color.x * 255.0f, color.y * 255.0f, color.z * 255.0f);
+    }
+}
+
+// RayTracer.cu:1776-1882 shading from the surface record; with bounce shadow
+// flags a light whose shadow ray is occluded is skipped.
+__global__ void __launch_bounds__(256) k_shade_reflection(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L,
+                                                          v3 eye, const unsigned long long* rec, uint32_t hit_mode,
+                                                          const uint32_t* sflags, uint32_t* pixels) {
+    const uint32_t n = W * H;
+    const unsigned long long miss = hit_mode == 0 ? MISS64 : (unsigned long long)MISS32;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        uint32_t x = p % W, y = p / W;
+        uint32_t slot = x + y * W * Lmax;
+        if (rec[slot] == miss) continue;
+        v3 P = xyz(g.pos[p]), N = xyz(g.nrm[p]);
+        float4 Kd = g.dif[p], Ks = g.spec[p];
+        v3 color = mk(0.0f, 0.0f, 0.0f);
+        for (uint32_t l = 0; l < L.total; l++) {
+            if (sflags && sflags[(size_t)slot + (size_t)W * l] == MISS32) continue;
+            color = color + light_term(L, l, P, N, eye, Kd, Ks, true);
+        }
+        pixels[p] += rgb_to_int(color.x * Ks.w, color.y * Ks.w, color.z * Ks.w);
+    }
+}
+
+// Per-triangle data of the fused traversal: bounding sphere (RayTracer.cu:301-376
+// evaluated once per triangle instead of once per candidate pair) and the
+// vertex / edge vectors of the Moller-Trumbore test.
+__global__ void __launch_bounds__(256) k_triangle_prep(const float4* tp, uint32_t T, float4* tri_sphere, float4* tri_mt) {
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < T; t += gridDim.x * blockDim.x) {
+        v3 a = xyz(tp[3 * (size_t)t]), b = xyz(tp[3 * (size_t)t + 1]), c = xyz(tp[3 * (size_t)t + 2]);
+        tri_sphere[t] = triangle_sphere(a, b, c);
+        tri_mt[3 * (size_t)t] = f4(a, 0.0f);
+        tri_mt[3 * (size_t)t + 1] = f4(b - a, 0.0f);
+        tri_mt[3 * (size_t)t + 2] = f4(c - a, 0.0f);
+    }
+}
+
+// RayTracer.cu:448-511
+__global__ void __launch_bounds__(256) k_update_vertex(const float* model, const float* normal, uint32_t n,
+                                                       const float4* pos_in, const float4* nrm_in,
+                                                       const int32_t* obj, float4* pos_out, float4* nrm_out) {
+    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < n; x += gridDim.x * blockDim.x) {
+        uint32_t m = (uint32_t)obj[x];
+        const float* M = model + 16 * (size_t)m;
+        float4 v = pos_in[x];
+        float u[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            u[i] = 0.0f;
+            u[i] += M[i * 4 + 0] * v.x; u[i] += M[i * 4 + 1] * v.y;
+            u[i] += M[i * 4 + 2] * v.z; u[i] += M[i * 4 + 3] * v.w;
+        }
+        pos_out[x] = make_float4(u[0], u[1], u[2], (float)m);
+        const float* Nm = normal + 16 * (size_t)m;
+        float4 q = nrm_in[x];
+        float w[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            w[i] = 0.0f;
+            w[i] += Nm[i * 4 + 0] * q.x; w[i] += Nm[i * 4 + 1] * q.y;
+            w[i] += Nm[i * 4 + 2] * q.z; w[i] += Nm[i * 4 + 3] * q.w;
+        }
+        nrm_out[x] = f4(normalize(mk(w[0], w[1], w[2])), 0.0f);
+    }
+}
+
+// RayTracer.cu:513-570
+__global__ void __launch_bounds__(256) k_update_sphere(const float* translation, const float* scale, uint32_t K,
+                                                       const float4* sin4, float* sout3) {
+    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < K; x += gridDim.x * blockDim.x) {
+        const float* Tm = translation + 16 * (size_t)x;
+        const float* Sm = scale + 16 * (size_t)x;
+        float4 c = sin4[2 * (size_t)x], rb = sin4[2 * (size_t)x + 1];
+        float u[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            u[i] = 0.0f;
+            u[i] += Tm[i * 4 + 0] * c.x; u[i] += Tm[i * 4 + 1] * c.y;
+            u[i] += Tm[i * 4 + 2] * c.z; u[i] += Tm[i * 4 + 3] * 1.0f;
+        }
+        float ms = 1.17549435e-38f;
+        for (int i = 0; i < 3; i++)
+            if (fabsf(Sm[i * 4 + i]) > ms) ms = fabsf(Sm[i * 4 + i]);
+        float* o = sout3 + 6 * (size_t)x;
+        o[0] = u[0]; o[1] = u[1]; o[2] = u[2];
+        o[3] = rb.x * ms; o[4] = rb.y; o[5] = rb.z;
+    }
+}
+
+// RayTracer.cu:1776-2017 (reference 32-bit records, recomputed hit, optional
+// in-place creation of the next bounce).
+__global__ void __launch_bounds__(256) k_color_reflection_ref(const float4* tp, const float4* tn, uint32_t W, uint32_t H,
+                                                              uint32_t Lmax, SceneView sc, v3 eye, uint32_t create,
+                                                              const uint32_t* times, float* rays, uint32_t* flags,
+                                                              uint32_t* keys, uint32_t* values, uint32_t* pixels) {
+    const uint32_t n = W * H;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        uint32_t x = p % W, y = p / W;
+        uint32_t r = x + y * W * Lmax;
+        uint32_t rec = times[r], t = rec & 0x000FFFFFu;
+        if (rec == MISS32) { if (create) flags[r] = 1; continue; }
+        int32_t m = sc.tri_mat[3 * (size_t)t];
+        v3 v0 = xyz(tp[3 * (size_t)t]), v1 = xyz(tp[3 * (size_t)t + 1]), v2 = xyz(tp[3 * (size_t)t + 2]);
+        v3 o, d;
+        load_ray(rays, r, o, d);
+        float th = ray_triangle(o + d * EPS, d, v0, v1 - v0, v2 - v0);
+        v3 P = o + d * (EPS + th);
+        float aABC = len(cross(v1 - v0, v2 - v0));
+        float aPBC = len(cross(v1 - P, v2 - P));
+        float aPCA = len(cross(v0 - P, v2 - P));
+        v3 N = ((aPBC / aABC) * xyz(tn[3 * (size_t)t]) + (aPCA / aABC) * xyz(tn[3 * (size_t)t + 1])) +
+               ((1.0f - (aPBC / aABC)) - (aPCA / aABC)) * xyz(tn[3 * (size_t)t + 2]);
+        float4 Kd = sc.mat_dif[m], Ks = sc.mat_spec[m];
+        v3 color = mk(0.0f, 0.0f, 0.0f);
+        for (uint32_t l = 0; l < sc.lights.total; l++) color = color + light_term(sc.lights, l, P, N, eye, Kd, Ks, true);
+        pixels[p] += rgb_to_int(color.x * Ks.w, color.y * Ks.w, color.z * Ks.w);
+        if (create) {
+            v3 nd = reflect(d, N);
+            store_ray(rays, r, P, nd);
+            keys[r] = reflection_key_ref(P, nd);
+            values[r] = r;
+            flags[r] = 0;
+        }
+    }
+}
+
+}  // namespace
+
+void launch_create_primary(uint32_t W, uint32_t H, v3 eye, v3 D, v3 R, v3 U, KeyCfg kc, float* rays, uint32_t* flags,
+                           uint32_t* keys, uint32_t* values, hipStream_t s) {
+    k_create_primary<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(W, H, eye, D, R, U, kc, rays, flags, keys,
+                                                                            values);
+}
+void launch_create_shadow(const float4* gpos, const float4* gnrm, uint32_t W, uint32_t H, uint32_t Lmax,
+                          const float4* light_pos, KeyCfg kc, float* rays, uint32_t* flags, uint32_t* keys,
+                          uint32_t* values, hipStream_t s) {
+    k_create_shadow<<<grid_for((uint64_t)W * H * Lmax, 256, 65536), 256, 0, s>>>(gpos, gnrm, W, H, Lmax, light_pos, kc,
+                                                                                   rays, flags, keys, values);
+}
+void launch_create_reflection(const float4* gpos, const float4* gnrm, const float4* gdif, uint32_t W, uint32_t H,
+                              uint32_t Lmax, v3 eye, uint32_t gate, uint32_t mark_gaps, KeyCfg kc, float* rays,
+                              uint32_t* flags, uint32_t* keys, uint32_t* values, hipStream_t s) {
+    k_create_reflection<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(gpos, gnrm, gdif, W, H, Lmax, eye, gate,
+                                                                               mark_gaps, kc, rays, flags, keys, values);
+}
+void launch_create_bounce(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, const unsigned long long* rec,
+                          uint32_t hit_mode, uint32_t gate, KeyCfg kc, float* rays, uint32_t* flags, uint32_t* keys,
+                          uint32_t* values, hipStream_t s) {
+    k_create_bounce<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(g, W, H, Lmax, rec, hit_mode, gate, kc, rays,
+                                                                           flags, keys, values);
+}
+void launch_surface(const float* rays, const unsigned long long* rec, uint32_t hit_mode, uint32_t W, uint32_t H,
+                    uint32_t slot_stride, uint32_t normalize_n, SceneView sc, GBufView g, hipStream_t s) {
+    k_surface<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(rays, rec, hit_mode, W, H, slot_stride,
+                                                                     normalize_n, sc, g);
+}
+void launch_shade_primary(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L, v3 eye, const uint32_t* sflags,
+                          uint32_t* pixels, hipStream_t s) {
+    k_shade_primary<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(g, W, H, Lmax, L, eye, sflags, pixels);
+}
+void launch_shade_reflection(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L, v3 eye,
+                             const unsigned long long* rec, uint32_t hit_mode, const uint32_t* sflags,
+                             uint32_t* pixels, hipStream_t s) {
+    k_shade_reflection<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(g, W, H, Lmax, L, eye, rec, hit_mode,
+                                                                              sflags, pixels);
+}
+void launch_triangle_prep(const float4* tri_pos, uint32_t T, float4* tri_sphere, float4* tri_mt, hipStream_t s) {
+    if (T) k_triangle_prep<<<grid_for(T, 256), 256, 0, s>>>(tri_pos, T, tri_sphere, tri_mt);
+}
+void launch_update_vertex(const float* model, const float* normal, uint32_t n, const float4* pos_in,
+                          const float4* nrm_in, const int32_t* obj_ids, float4* pos_out, float4* nrm_out,
+                          hipStream_t s) {
+    if (n) k_update_vertex<<<grid_for(n, 256), 256, 0, s>>>(model, normal, n, pos_in, nrm_in, obj_ids, pos_out, nrm_out);
+}
+void launch_update_sphere(const float* translation, const float* scale, uint32_t K, const float4* spheres_in,
+                          float* spheres_out, hipStream_t s) {
+    if (K) k_update_sphere<<<grid_for(K, 256), 256, 0, s>>>(translation, scale, K, spheres_in, spheres_out);
+}
+void launch_color_reflection_ref(const float4* tp, const float4* tn, uint32_t W, uint32_t H, uint32_t Lmax,
+                                 SceneView sc, v3 eye, uint32_t create, const uint32_t* times, float* rays,
+                                 uint32_t* flags, uint32_t* keys, uint32_t* values, uint32_t* pixels, hipStream_t s) {
+    k_color_reflection_ref<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(tp, tn, W, H, Lmax, sc, eye, create,
+                                                                                  times, rays, flags, keys, values,
+                                                                                  pixels);
+}
+
+}  // namespace rsh

@@ -1,0 +1,425 @@
+// k_trav.hip -- hierarchy construction and traversal + intersection (gfx950).
+//
+// Reference: projects/CUDA-Engine/source/RayTracer.cu
+//   CreateHierarchyLevel0 :1061-1108, CreateHierarchyLevelN :1110-1150
+//   CalculateBoundingSpheresIntersections :1152-1280, CreateHierarchyBoundingSphereHits :1282-1313
+//   CreateCRSH/RAHHierarchyLevel0Hits :1315-1405, CreateHierarchyLevelNHits :1407-1462
+//   CreateTrimmedHierarchyHits :1464-1496
+//   CalculateShadowRayIntersections :1498-1614, CalculateReflectionRayIntersections :1711-1774
+//
+// The native path fuses traversal and intersection: one wave64 owns the 64
+// sorted rays under one top-level node (8 leaves of 8 rays for the reference
+// fan-out), keeps each lane's ray and best hit in registers, and streams the
+// node's candidate triangles -- object spheres tested 64 at a time, then each
+// surviving object's triangles 64 at a time against the top node, then every
+// survivor against the lane's own chain of lower nodes before the
+// Moller-Trumbore test.  Triangle data for the per-survivor step is
+// wave-uniform and read with scalar loads.  Nothing is materialised: no flag
+// arrays, no scans, no host round trips, no atomics on results.  The
+// candidate set per (leaf, triangle) is the reference's; only the order of
+// tests differs, and both any-hit and closest-hit are order independent.
+#include "rsh_internal.hpp"
+#include "rsh_kernels.hpp"
+
+namespace rsh {
+
+namespace {
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+
+inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap = 1u << 20) {
+    uint64_t g = (n + block - 1) / block;
+    if (g == 0) g = 1;
+    return (uint32_t)(g < cap ? g : cap);
+}
+
+// -------------------------------------------------------------- gather ----
+// Sorted ray records: rec0 = (origin, slot bits), rec1 = (direction, tmax).
+// tmax = |o - P| - 2 eps for shadow rays (RayTracer.cu:1610), P the surface
+// point of the slot's pixel; unused for closest-hit passes.
+__global__ void __launch_bounds__(256) k_gather(const float* rays, const uint32_t* sorted_values, uint32_t N,
+                                                const float4* shadow_pos, uint32_t W, uint32_t Lmax,
+                                                float4* rec0, float4* rec1) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
+        uint32_t slot = sorted_values[p];
+        const float2* r = reinterpret_cast<const float2*>(rays + 6 * (size_t)slot);
+        float2 a = r[0], b = r[1], c = r[2];
+        v3 o = mk(a.x, a.y, b.x), d = mk(b.y, c.x, c.y);
+        float tmax = 0.0f;
+        if (shadow_pos) {
+            uint32_t x = slot % W, y = slot / (W * Lmax);
+            v3 P = xyz(shadow_pos[(size_t)x + (size_t)y * W]);
+            tmax = len(o - P) - EPS * 2.0f;
+        }
+        rec0[p] = f4(o, __uint_as_float(slot));
+        rec1[p] = f4(d, tmax);
+    }
+}
+
+// ----------------------------------------------------------- hierarchy ----
+// Level 0: node j over sorted rays [j*leaf, min((j+1)*leaf, N)).  The
+// reference's loop bound reads one slot past the last ray (RayTracer.cu:1090);
+// here the node stops at the last live ray.
+template <bool FROM_RECORDS>
+__global__ void __launch_bounds__(256) k_level0(const float4* rec0, const float4* rec1, const float* rays,
+                                                const uint32_t* sorted_values, uint32_t N, uint32_t leaf,
+                                                float r0, float s0, uint32_t mode, float4* nodes, float2* aux) {
+    const uint32_t n0 = (N + leaf - 1) / leaf;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n0; j += gridDim.x * blockDim.x) {
+        auto fetch = [&](uint32_t p, v3& o, v3& d) {
+            if (FROM_RECORDS) {
+                float4 a = rec0[p], b = rec1[p];
+                o = xyz(a); d = xyz(b);
+            } else {
+                const float* r = rays + 6 * (size_t)sorted_values[p];
+                o = mk(r[0], r[1], r[2]); d = mk(r[3], r[4], r[5]);
+            }
+        };
+        v3 o, d;
+        fetch(j * leaf, o, d);
+        float4 sphere = f4(o, r0);
+        float4 cone = f4(mode == 0 ? d : normalize(d), s0);
+        for (uint32_t i = 1; i < leaf; i++) {
+            uint32_t p = j * leaf + i;
+            if (p >= N) break;
+            fetch(p, o, d);
+            sphere = sphere_union(sphere, f4(o, r0));
+            cone = cone0(cone, d, s0, mode);
+        }
+        nodes[2 * (size_t)j] = sphere;
+        nodes[2 * (size_t)j + 1] = cone;
+        if (aux) aux[j] = make_float2(cos_s(cone.w), tan_s(cone.w));
+    }
+}
+
+__global__ void __launch_bounds__(256) k_levelN(float4* nodes, float2* aux, uint32_t read_off, uint32_t read_count,
+                                                uint32_t write_off, uint32_t write_count, uint32_t branch,
+                                                uint32_t mode) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < write_count; j += gridDim.x * blockDim.x) {
+        size_t c0 = read_off + (size_t)j * branch;
+        float4 sphere = nodes[2 * c0], cone = nodes[2 * c0 + 1];
+        for (uint32_t i = 1; i < branch; i++) {
+            if (j * branch + i >= read_count) break;
+            size_t c = c0 + i;
+            sphere = sphere_union(sphere, nodes[2 * c]);
+            cone = coneN(cone, nodes[2 * c + 1], mode);
+        }
+        nodes[2 * (size_t)(write_off + j)] = sphere;
+        nodes[2 * (size_t)(write_off + j) + 1] = cone;
+        if (aux) aux[write_off + j] = make_float2(cos_s(cone.w), tan_s(cone.w));
+    }
+}
+
+// ---------------------------------------------------- fused traversal ----
+template <int DEPTH>
+__global__ void __launch_bounds__(256) k_traverse(TravParams P) {
+    const uint32_t lane = lane_id();
+    const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const uint32_t top = wave / P.waves_per_top;
+    const uint32_t sub = wave - top * P.waves_per_top;
+    const uint32_t topLevel = DEPTH - 1;
+    if (top >= P.level_count[topLevel]) return;
+    const uint32_t local = sub * 64 + lane;
+    const uint32_t pos = top * P.rays_per_top + local;
+    const bool valid = local < P.rays_per_top && pos < P.N;
+    if (__ballot(valid) == 0ull) return;
+
+    v3 o = mk(0, 0, 0), d = mk(0, 0, 0);
+    uint32_t slot = 0;
+    float tmax = 0.0f;
+    if (valid) {
+        float4 a = P.rec0[pos], b = P.rec1[pos];
+        o = xyz(a); d = xyz(b);
+        slot = __float_as_uint(a.w);
+        tmax = b.w;
+    }
+    const v3 oe = o + d * EPS;
+
+    // top node (wave-uniform)
+    const uint32_t gt = P.level_offset[topLevel] + top;
+    const float4 tS = P.nodes[2 * (size_t)gt], tC = P.nodes[2 * (size_t)gt + 1];
+    const float2 tA = P.aux[gt];
+
+    // per-lane chain of lower nodes, level DEPTH-2 down to 0
+    float4 cS[DEPTH > 1 ? DEPTH - 1 : 1], cC[DEPTH > 1 ? DEPTH - 1 : 1];
+    float2 cA[DEPTH > 1 ? DEPTH - 1 : 1];
+    bool leaf_head = false;
+    if (DEPTH > 1) {
+        uint32_t span = P.leaf;
+        uint32_t idx0 = valid ? pos / P.leaf : 0;
+        leaf_head = valid && (pos % P.leaf) == 0;
+#pragma unroll
+        for (int k = 0; k < DEPTH - 1; k++) {
+            uint32_t idx = valid ? pos / span : 0;
+            (void)idx0;
+            uint32_t g = P.level_offset[k] + idx;
+            cS[k] = P.nodes[2 * (size_t)g];
+            cC[k] = P.nodes[2 * (size_t)g + 1];
+            cA[k] = P.aux[g];
+            span *= P.branch;
+        }
+    } else {
+        leaf_head = valid && (pos % P.leaf) == 0;
+    }
+
+    unsigned long long best = P.hit_mode == 0 ? MISS64 : (unsigned long long)MISS32;
+    bool occluded = false;
+    uint64_t alive = __ballot(valid);
+    const bool count = P.counters != nullptr;
+    uint64_t n_top = 0, n_leaf = 0, n_tests = 0;
+    bool done = false;
+
+    auto process_range = [&](uint32_t first, uint32_t last, uint32_t exlo, uint32_t exhi) {
+        for (uint32_t tb = first; tb <= last && !done; tb += 64) {
+            const uint32_t t = tb + lane;
+            const bool inr = t <= last && !(t >= exlo && t < exhi);
+            bool p = false;
+            if (inr) p = sphere_node(tS, tC, P.tri_sphere[t], tA.x, tA.y);
+            if (count && sub == 0) n_top += (uint64_t)__popcll(__ballot(inr));
+            uint64_t m = __ballot(p);
+            while (m) {
+                const uint32_t tt = __builtin_amdgcn_readfirstlane(tb + (uint32_t)__builtin_ctzll(m));
+                m &= m - 1;
+                const float4 ts = P.tri_sphere[tt];
+                bool lp = valid && !occluded;
+                if (DEPTH > 1) {
+#pragma unroll
+                    for (int k = DEPTH - 2; k >= 0; k--) lp = lp && sphere_node(cS[k], cC[k], ts, cA[k].x, cA[k].y);
+                }
+                const uint64_t lm = __ballot(lp);
+                if (count) {
+                    n_leaf += (uint64_t)__popcll(__ballot(lp && leaf_head));
+                    n_tests += (uint64_t)__popcll(lm);
+                }
+                if (lm == 0ull) continue;
+                const float4 A = P.tri_mt[3 * (size_t)tt], B = P.tri_mt[3 * (size_t)tt + 1], Cc = P.tri_mt[3 * (size_t)tt + 2];
+                if (lp) {
+                    float th = ray_triangle(oe, d, xyz(A), xyz(B), xyz(Cc));
+                    if (P.kind == 1) {
+                        if (th > EPS && th < tmax) occluded = true;
+                    } else if (th > EPS) {
+                        unsigned long long rec = pack_hit(th, tt, P.hit_mode);
+                        best = rec < best ? rec : best;
+                    }
+                }
+                if (P.kind == 1 && (__ballot(occluded) & alive) == alive) { done = true; break; }
+            }
+        }
+    };
+
+    if (P.algorithm == 1) {
+        if (P.T) process_range(0, P.T - 1, 0, 0);
+    } else {
+        for (uint32_t ob = 0; ob < P.K && !done; ob += 64) {
+            const uint32_t oi = ob + lane;
+            bool po = false;
+            if (oi < P.K) po = sphere_node(tS, tC, P.obj_sphere[oi], tA.x, tA.y);
+            uint64_t m = __ballot(po);
+            while (m && !done) {
+                const uint32_t ok = __builtin_amdgcn_readfirstlane(ob + (uint32_t)__builtin_ctzll(m));
+                m &= m - 1;
+                const uint4 r = P.obj_range[ok];
+                process_range(r.x, r.y, r.z, r.w);
+            }
+        }
+    }
+
+    if (valid) {
+        if (P.kind == 1) { if (occluded) P.shadow[slot] = MISS32; }
+        else P.closest[slot] = best;
+    }
+    if (count) {
+        if (lane == 0) {
+            atomicAdd(&P.counters[0], (unsigned long long)n_top);
+            atomicAdd(&P.counters[1], (unsigned long long)n_leaf);
+            atomicAdd(&P.counters[2], (unsigned long long)n_tests);
+        }
+    }
+}
+
+// ------------------------------------------- materialised traversal ------
+__device__ __forceinline__ uint32_t create_hit(uint32_t node, uint32_t tri) {
+    return ((node << 14) & 0xFFFFC000u) + (tri & 0x00003FFFu);
+}
+__device__ __forceinline__ uint32_t hit_node(uint32_t h) { return (h & 0xFFFFC000u) >> 14; }
+__device__ __forceinline__ uint32_t hit_tri(uint32_t h) { return h & 0x00003FFFu; }
+
+__device__ __forceinline__ float4 tri_sphere_at(const float4* tp, uint32_t t) {
+    return triangle_sphere(xyz(tp[3 * (size_t)t]), xyz(tp[3 * (size_t)t + 1]), xyz(tp[3 * (size_t)t + 2]));
+}
+__device__ __forceinline__ bool node_test(const float4* h, uint32_t g, float4 ts) {
+    float4 s = h[2 * (size_t)g], c = h[2 * (size_t)g + 1];
+    return sphere_node(s, c, ts, cos_s(c.w), tan_s(c.w));
+}
+
+// RayTracer.cu:1152-1280 (run-length culling flags by atomicAdd)
+__global__ void __launch_bounds__(256) k_bsphere_intersections(const float4* h, const float* spheres3, uint32_t K,
+                                                               uint32_t Tb, uint32_t offset, uint32_t nodeOffset,
+                                                               uint32_t nodeTotal, uint32_t* flags) {
+    const uint64_t n = (uint64_t)nodeTotal * K;
+    for (uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t node = (uint32_t)(x / K), b = (uint32_t)(x % K);
+        float4 s = h[2 * (size_t)(nodeOffset + node)], c = h[2 * (size_t)(nodeOffset + node) + 1];
+        const float* p = spheres3 + 6 * (size_t)b;
+        float ry = p[4], rz = p[5];
+        bool res = sphere_node(s, c, make_float4(p[0], p[1], p[2], p[3]), cos_s(c.w), tan_s(c.w));
+        const float lower = (float)offset, upper = (float)(offset + Tb);
+        uint32_t* f = flags + (size_t)node * Tb;
+        if (res) {
+            if (ry >= lower && ry < upper && rz >= lower && rz < upper)
+                atomicAdd(&f[(uint32_t)ry - offset], (uint32_t)rz - (uint32_t)ry + 1);
+            else if (ry < lower && rz >= lower && rz < upper)
+                atomicAdd(&f[0], (uint32_t)rz - offset + 1);
+            else if (ry >= lower && ry < upper && rz >= upper)
+                atomicAdd(&f[(uint32_t)ry - offset], Tb);
+            else if (ry < lower && rz > upper)
+                atomicAdd(&f[0], (uint32_t)rz - (uint32_t)ry + 1 - ((uint32_t)rz - (offset + Tb) + 1));
+        } else {
+            if (ry >= lower && ry < upper && rz >= lower && rz + 1 < upper)
+                atomicAdd(&f[(uint32_t)rz - offset + 1], (uint32_t)rz - (uint32_t)ry + 1);
+            else if (ry < lower && rz >= lower && (rz + 1) < upper)
+                atomicAdd(&f[(uint32_t)rz - offset + 1], (uint32_t)rz - offset + 1);
+        }
+    }
+}
+
+// RayTracer.cu:1282-1313
+__global__ void __launch_bounds__(256) k_bsphere_hits(uint32_t Tb, uint32_t nodeTotal, const uint32_t* scan,
+                                                      uint32_t* flags, uint32_t* hits) {
+    const uint64_t n = (uint64_t)nodeTotal * Tb;
+    for (uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < n; x += (uint64_t)gridDim.x * blockDim.x) {
+        uint32_t node = (uint32_t)(x / Tb), t = (uint32_t)(x % Tb);
+        int off = (int)scan[x];
+        if (node > 0) off -= (int)scan[(size_t)node * Tb - 1];
+        bool hit = (uint32_t)off > t;
+        hits[x] = hit ? create_hit(node, t) : 0u;
+        flags[x] = hit ? 0u : 1u;
+    }
+}
+
+// RayTracer.cu:1315-1405 (rah = 1: every (node, triangle) pair)
+__global__ void __launch_bounds__(256) k_level0_hits(const float4* h, const float4* tp, uint32_t Tb, uint32_t offset,
+                                                     uint32_t hitTotal, uint32_t nodeOffset, uint32_t rah,
+                                                     uint32_t* flags, uint32_t* hits, const uint32_t* thits) {
+    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < hitTotal; x += gridDim.x * blockDim.x) {
+        uint32_t hit = rah ? create_hit(x / Tb, x % Tb) : thits[x];
+        uint32_t node = hit_node(hit), t = hit_tri(hit);
+        float4 ts = tri_sphere_at(tp, offset + t);
+        flags[x] = node_test(h, nodeOffset + node, ts) ? 0u : 1u;
+        hits[x] = hit;
+    }
+}
+
+// RayTracer.cu:1407-1462
+__global__ void __launch_bounds__(256) k_levelN_hits(const float4* h, const float4* tp, uint32_t offset,
+                                                     uint32_t hitTotal, uint32_t nodeOffset, uint32_t nodeWriteTotal,
+                                                     uint32_t branch, uint32_t* flags, uint32_t* hits,
+                                                     const uint32_t* thits) {
+    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < hitTotal; x += gridDim.x * blockDim.x) {
+        uint32_t hit = thits[x];
+        uint32_t node = hit_node(hit), t = hit_tri(hit);
+        float4 ts = tri_sphere_at(tp, offset + t);
+        for (uint32_t i = 0; i < branch; i++) {
+            uint32_t c = node * branch + i;
+            uint32_t f = 1u;
+            if (c < nodeWriteTotal) f = node_test(h, nodeOffset + c, ts) ? 0u : 1u;
+            flags[(size_t)x * branch + i] = f;
+            hits[(size_t)x * branch + i] = create_hit(c, t);
+        }
+    }
+}
+
+// RayTracer.cu:1498-1614 (hard shadows) / :1711-1774 (reference packing)
+__global__ void __launch_bounds__(256) k_hits_intersect(const float* rays, const uint32_t* svalues, const uint32_t* hits,
+                                                        const float4* tp, uint32_t hitTotal, uint32_t N,
+                                                        uint32_t offset, uint32_t W, uint32_t Lmax, uint32_t leaf,
+                                                        const float4* gpos, uint32_t shadow, uint32_t* out) {
+    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < hitTotal; x += gridDim.x * blockDim.x) {
+        uint32_t h = hits[x], node = hit_node(h), t = hit_tri(h);
+        v3 v0 = xyz(tp[3 * (size_t)(offset + t)]);
+        v3 e1 = xyz(tp[3 * (size_t)(offset + t) + 1]) - v0;
+        v3 e2 = xyz(tp[3 * (size_t)(offset + t) + 2]) - v0;
+        for (uint32_t i = 0; i < leaf; i++) {
+            if (node * leaf + i >= N) break;
+            uint32_t r = svalues[node * leaf + i];
+            const float* rp = rays + 6 * (size_t)r;
+            v3 o = mk(rp[0], rp[1], rp[2]), d = mk(rp[3], rp[4], rp[5]);
+            float th = ray_triangle(o + d * EPS, d, v0, e1, e2);
+            if (shadow) {
+                v3 P = xyz(gpos[(size_t)(r % W) + (size_t)(r / (W * Lmax)) * W]);
+                if (th > EPS && th < len(o - P) - EPS * 2.0f) out[r] = MISS32;
+            } else if (th > EPS) {
+                uint32_t rec = (uint32_t)pack_hit(th, offset + t, 1);
+                atomicMin(&out[r], rec);
+            }
+        }
+    }
+}
+
+}  // namespace
+
+// ================================================================ launchers
+void launch_gather(const float* rays, const uint32_t* sorted_values, uint32_t N, const float4* shadow_pos, uint32_t W,
+                   uint32_t Lmax, float4* rec0, float4* rec1, hipStream_t s) {
+    if (N) k_gather<<<grid_for(N, 256, 65536), 256, 0, s>>>(rays, sorted_values, N, shadow_pos, W, Lmax, rec0, rec1);
+}
+
+void launch_hierarchy(const float4* rec0, const float4* rec1, const float* rays, const uint32_t* sorted_values,
+                      uint32_t N, uint32_t leaf, uint32_t branch, uint32_t depth, float r0, float s0, uint32_t mode,
+                      float4* nodes, float2* aux, const uint32_t* counts, const uint32_t* offsets, hipStream_t s) {
+    if (!N) return;
+    if (rec0) k_level0<true><<<grid_for(counts[0], 256), 256, 0, s>>>(rec0, rec1, nullptr, nullptr, N, leaf, r0, s0,
+                                                                       mode, nodes, aux);
+    else k_level0<false><<<grid_for(counts[0], 256), 256, 0, s>>>(nullptr, nullptr, rays, sorted_values, N, leaf, r0,
+                                                                   s0, mode, nodes, aux);
+    for (uint32_t k = 1; k < depth; k++)
+        k_levelN<<<grid_for(counts[k], 256), 256, 0, s>>>(nodes, aux, offsets[k - 1], counts[k - 1], offsets[k],
+                                                           counts[k], branch, mode);
+}
+
+hipError_t launch_traverse(const TravParams& p, hipStream_t s) {
+    const uint32_t top = p.level_count[p.depth - 1];
+    const uint64_t waves = (uint64_t)top * p.waves_per_top;
+    if (waves == 0) return hipSuccess;
+    const uint32_t blocks = (uint32_t)((waves + 3) / 4);
+    switch (p.depth) {
+        case 1: k_traverse<1><<<blocks, 256, 0, s>>>(p); break;
+        case 2: k_traverse<2><<<blocks, 256, 0, s>>>(p); break;
+        case 3: k_traverse<3><<<blocks, 256, 0, s>>>(p); break;
+        case 4: k_traverse<4><<<blocks, 256, 0, s>>>(p); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+void launch_bsphere_intersections(const float4* h, const float* spheres3, uint32_t K, uint32_t Tb, uint32_t offset,
+                                  uint32_t nodeOffset, uint32_t nodeTotal, uint32_t* flags, hipStream_t s) {
+    uint64_t n = (uint64_t)nodeTotal * K;
+    if (n) k_bsphere_intersections<<<grid_for(n, 256), 256, 0, s>>>(h, spheres3, K, Tb, offset, nodeOffset, nodeTotal,
+                                                                     flags);
+}
+void launch_bsphere_hits(uint32_t Tb, uint32_t nodeTotal, const uint32_t* scan, uint32_t* flags, uint32_t* hits,
+                         hipStream_t s) {
+    uint64_t n = (uint64_t)nodeTotal * Tb;
+    if (n) k_bsphere_hits<<<grid_for(n, 256), 256, 0, s>>>(Tb, nodeTotal, scan, flags, hits);
+}
+void launch_level0_hits(const float4* h, const float4* tp, uint32_t Tb, uint32_t offset, uint32_t hitTotal,
+                        uint32_t nodeOffset, uint32_t rah, uint32_t* flags, uint32_t* hits, const uint32_t* thits,
+                        hipStream_t s) {
+    if (hitTotal) k_level0_hits<<<grid_for(hitTotal, 256), 256, 0, s>>>(h, tp, Tb, offset, hitTotal, nodeOffset, rah,
+                                                                         flags, hits, thits);
+}
+void launch_levelN_hits(const float4* h, const float4* tp, uint32_t offset, uint32_t hitTotal, uint32_t nodeOffset,
+                        uint32_t nodeWriteTotal, uint32_t branch, uint32_t* flags, uint32_t* hits,
+                        const uint32_t* thits, hipStream_t s) {
+    if (hitTotal) k_levelN_hits<<<grid_for(hitTotal, 256), 256, 0, s>>>(h, tp, offset, hitTotal, nodeOffset,
+                                                                         nodeWriteTotal, branch, flags, hits, thits);
+}
+void launch_hits_intersect(const float* rays, const uint32_t* svalues, const uint32_t* hits, const float4* tp,
+                           uint32_t hitTotal, uint32_t N, uint32_t offset, uint32_t W, uint32_t Lmax, uint32_t leaf,
+                           const float4* gpos, uint32_t shadow, uint32_t* out, hipStream_t s) {
+    if (hitTotal) k_hits_intersect<<<grid_for(hitTotal, 256), 256, 0, s>>>(rays, svalues, hits, tp, hitTotal, N,
+                                                                            offset, W, Lmax, leaf, gpos, shadow, out);
+}
+
+}  // namespace rsh

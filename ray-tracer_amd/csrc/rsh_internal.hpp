@@ -1,0 +1,82 @@
+// rsh_internal.hpp -- host-side declarations shared by the library's
+// translation units: kernel launchers and small device-memory utilities.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "rsh_device.hpp"
+
+namespace rsh {
+
+// Tile geometry of the scan / compaction / sort kernels: 256 threads x 16
+// items.  The single-block scan of tile totals bounds a pass to
+// MAX_TILES * TILE items (67M).
+constexpr uint32_t SCAN_BLOCK = 256;
+constexpr uint32_t SCAN_ITEMS = 16;
+constexpr uint32_t TILE = SCAN_BLOCK * SCAN_ITEMS;
+constexpr uint32_t MAX_TILES = 16384;
+
+inline uint32_t tiles_for(uint64_t n) { return (uint32_t)((n + TILE - 1) / TILE); }
+
+// Growable device buffer.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t reserve(size_t b) {
+        if (b <= bytes) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr; bytes = 0;
+        hipError_t e = hipMalloc(&p, b ? b : 1);
+        if (e == hipSuccess) bytes = b;
+        return e;
+    }
+    void release() { if (p) (void)hipFree(p); p = nullptr; bytes = 0; }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Scratch used by scans and the radix sort.
+struct ScanScratch {
+    DevBuf tile_counts;   // uint32[MAX_TILES + 1]
+    DevBuf hist;          // uint32[256 * MAX_TILES]
+    DevBuf digit_totals;  // uint32[256]
+    DevBuf total;         // uint32[4]
+    DevBuf ping_k, ping_v;
+    hipError_t reserve(uint64_t n);
+    void release();
+};
+
+struct PassParams;  // defined in the kernel translation units
+
+// -------------------------------------------------------------- launchers --
+void launch_fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
+void launch_fill_u64(unsigned long long* p, unsigned long long v, uint64_t n, hipStream_t s);
+
+// Stable compaction of (keys, values) where flags == 0.  Writes the live count
+// to *d_total (device).  Returns hipSuccess or the launch error.
+hipError_t run_trim(const uint32_t* flags, const uint32_t* keys, const uint32_t* values, uint64_t n,
+                    uint32_t* out_keys, uint32_t* out_values, uint32_t* d_total, ScanScratch& sc,
+                    hipStream_t s);
+// Inclusive scan of flags (the reference's scanArray by-product).
+hipError_t run_inclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, ScanScratch& sc, hipStream_t s);
+// Exclusive scan of u32 values.
+hipError_t run_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, ScanScratch& sc, hipStream_t s);
+// Run-length compression of equal consecutive keys: chunk bases, sizes, keys,
+// values (= chunk index).  Chunk count -> *d_total.
+hipError_t run_compress(const uint32_t* keys, uint64_t n, uint32_t* bases, uint32_t* sizes,
+                        uint32_t* ckeys, uint32_t* cvalues, uint32_t* d_total, ScanScratch& sc,
+                        hipStream_t s);
+// Stable LSD radix sort of (keys, values) over the low `bits` bits; result
+// in out_keys / out_values (inputs preserved).
+hipError_t run_radix_sort(const uint32_t* keys, const uint32_t* values, uint64_t n, uint32_t bits,
+                          uint32_t* out_keys, uint32_t* out_values, ScanScratch& sc, hipStream_t s);
+// Expand sorted chunks back to rays: for chunk x in sorted order, rays
+// trimmed[bases[sv[x]] .. + sizes[sv[x]]) in order.  `scan` receives the
+// exclusive scan of the sorted chunk sizes, `skeleton` the sizes themselves.
+hipError_t run_decompress(const uint32_t* bases, const uint32_t* sizes, const uint32_t* sorted_cvalues,
+                          const uint32_t* tkeys, const uint32_t* tvalues, uint32_t* skeleton,
+                          uint32_t* scan, uint64_t C, uint64_t N, uint32_t* out_keys,
+                          uint32_t* out_values, ScanScratch& sc, hipStream_t s);
+
+}  // namespace rsh

@@ -1,0 +1,117 @@
+// rsh_kernels.hpp -- kernel parameter blocks and launchers of the ray,
+// surface, shading, hierarchy and traversal stages.
+#pragma once
+#include "rsh_internal.hpp"
+
+namespace rsh {
+
+struct KeyCfg {
+    uint32_t mode;        // 0 reference spherical bins, 1 octahedral
+    HashParams h;
+    uint32_t segment;     // OR-ed into the key at h.segment_shift (0: none)
+};
+
+struct SceneView {
+    const float4* tri_pos;    // [3T] per-vertex positions
+    const float4* tri_nrm;    // [3T] per-vertex normals
+    const int32_t* tri_mat;   // [3T] material per vertex
+    const float4* mat_dif;    // [M]
+    const float4* mat_spec;   // [M]
+    Lights lights;
+};
+
+struct GBufView {
+    float4* pos;
+    float4* nrm;
+    float4* dif;
+    float4* spec;
+};
+
+struct TravParams {
+    const float4* nodes;          // hierarchy, sphere at 2g, cone at 2g+1
+    const float2* aux;            // (cos, tan) of each node's spread
+    uint32_t level_offset[4];
+    uint32_t level_count[4];
+    uint32_t depth, leaf, branch;
+    uint32_t rays_per_top, waves_per_top;
+    const float4* rec0;           // sorted rays: (origin, slot bits)
+    const float4* rec1;           //              (direction, tmax)
+    uint32_t N;
+    const float4* tri_sphere;     // [T]
+    const float4* tri_mt;         // [3T] v0, e1, e2
+    uint32_t T;
+    const float4* obj_sphere;     // [K] (center, radius)
+    const uint4* obj_range;       // [K] (first, last, excluded lo, excluded hi)
+    uint32_t K;
+    uint32_t algorithm;           // 0 CRSH (object culling), 1 RAH (all triangles)
+    uint32_t kind;                // 0 closest hit, 1 any hit (shadow)
+    uint32_t hit_mode;            // 0 exact, 1 reference packing
+    unsigned long long* closest;  // [S] per slot
+    uint32_t* shadow;             // [S] per slot
+    unsigned long long* counters; // optional [3]: top pairs, leaf pairs, ray tests
+};
+
+// --- rays ---
+void launch_create_primary(uint32_t W, uint32_t H, v3 eye, v3 D, v3 R, v3 U, KeyCfg kc, float* rays, uint32_t* flags,
+                           uint32_t* keys, uint32_t* values, hipStream_t s);
+void launch_create_shadow(const float4* gpos, const float4* gnrm, uint32_t W, uint32_t H, uint32_t Lmax,
+                          const float4* light_pos, KeyCfg kc, float* rays, uint32_t* flags, uint32_t* keys,
+                          uint32_t* values, hipStream_t s);
+void launch_create_reflection(const float4* gpos, const float4* gnrm, const float4* gdif, uint32_t W, uint32_t H,
+                              uint32_t Lmax, v3 eye, uint32_t gate, uint32_t mark_gaps, KeyCfg kc, float* rays,
+                              uint32_t* flags, uint32_t* keys, uint32_t* values, hipStream_t s);
+void launch_create_bounce(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, const unsigned long long* rec,
+                          uint32_t hit_mode, uint32_t gate, KeyCfg kc, float* rays, uint32_t* flags, uint32_t* keys,
+                          uint32_t* values, hipStream_t s);
+void launch_surface(const float* rays, const unsigned long long* rec, uint32_t hit_mode, uint32_t W, uint32_t H,
+                    uint32_t slot_stride, uint32_t normalize_n, SceneView sc, GBufView g, hipStream_t s);
+void launch_shade_primary(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L, v3 eye, const uint32_t* sflags,
+                          uint32_t* pixels, hipStream_t s);
+void launch_shade_reflection(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L, v3 eye,
+                             const unsigned long long* rec, uint32_t hit_mode, const uint32_t* sflags,
+                             uint32_t* pixels, hipStream_t s);
+void launch_triangle_prep(const float4* tri_pos, uint32_t T, float4* tri_sphere, float4* tri_mt, hipStream_t s);
+// reference-only stages
+void launch_update_vertex(const float* model, const float* normal, uint32_t vertexTotal, const float4* pos_in,
+                          const float4* nrm_in, const int32_t* obj_ids, float4* pos_out, float4* nrm_out,
+                          hipStream_t s);
+void launch_update_sphere(const float* translation, const float* scale, uint32_t K, const float4* spheres_in,
+                          float* spheres_out, hipStream_t s);
+void launch_color_reflection_ref(const float4* tp, const float4* tn, uint32_t W, uint32_t H, uint32_t Lmax,
+                                 SceneView sc, v3 eye, uint32_t create, const uint32_t* times, float* rays,
+                                 uint32_t* flags, uint32_t* keys, uint32_t* values, uint32_t* pixels, hipStream_t s);
+
+// --- hierarchy / traversal ---
+void launch_gather(const float* rays, const uint32_t* sorted_values, uint32_t N, const float4* shadow_pos, uint32_t W,
+                   uint32_t Lmax, float4* rec0, float4* rec1, hipStream_t s);
+void launch_hierarchy(const float4* rec0, const float4* rec1, const float* rays, const uint32_t* sorted_values,
+                      uint32_t N, uint32_t leaf, uint32_t branch, uint32_t depth, float r0, float s0, uint32_t mode,
+                      float4* nodes, float2* aux, const uint32_t* counts, const uint32_t* offsets, hipStream_t s);
+hipError_t launch_traverse(const TravParams& p, hipStream_t s);
+void launch_bsphere_intersections(const float4* h, const float* spheres3, uint32_t K, uint32_t Tb, uint32_t offset,
+                                  uint32_t nodeOffset, uint32_t nodeTotal, uint32_t* flags, hipStream_t s);
+void launch_bsphere_hits(uint32_t Tb, uint32_t nodeTotal, const uint32_t* scan, uint32_t* flags, uint32_t* hits,
+                         hipStream_t s);
+void launch_level0_hits(const float4* h, const float4* tp, uint32_t Tb, uint32_t offset, uint32_t hitTotal,
+                        uint32_t nodeOffset, uint32_t rah, uint32_t* flags, uint32_t* hits, const uint32_t* thits,
+                        hipStream_t s);
+void launch_levelN_hits(const float4* h, const float4* tp, uint32_t offset, uint32_t hitTotal, uint32_t nodeOffset,
+                        uint32_t nodeWriteTotal, uint32_t branch, uint32_t* flags, uint32_t* hits,
+                        const uint32_t* thits, hipStream_t s);
+void launch_hits_intersect(const float* rays, const uint32_t* svalues, const uint32_t* hits, const float4* tp,
+                           uint32_t hitTotal, uint32_t N, uint32_t offset, uint32_t W, uint32_t Lmax, uint32_t leaf,
+                           const float4* gpos, uint32_t shadow, uint32_t* out, hipStream_t s);
+
+// Hierarchy level sizes: level 0 has ceil(N/leaf) nodes, level k
+// ceil(count[k-1]/branch); offsets are prefix sums.  Returns the node total.
+inline uint32_t level_sizes(uint32_t N, uint32_t leaf, uint32_t branch, uint32_t depth, uint32_t* counts,
+                            uint32_t* offsets) {
+    uint32_t n = (N + leaf - 1) / leaf, total = 0;
+    for (uint32_t k = 0; k < depth; k++) {
+        counts[k] = n; offsets[k] = total; total += n;
+        n = (n + branch - 1) / branch;
+    }
+    return total;
+}
+
+}  // namespace rsh

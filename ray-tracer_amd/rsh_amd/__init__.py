@@ -1,0 +1,332 @@
+"""Python access to the gfx950 ray-space-hierarchy library (lib/librsh.so).
+
+Only ctypes plumbing lives here; every stage runs in the HIP library.  If the
+library is missing, :func:`lib` raises -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import config as config  # noqa: F401
+from . import scenes as scenes  # noqa: F401
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+LIB_PATH = os.path.join(ROOT, "lib", "librsh.so")
+
+u32, i32, f32, u64 = C.c_uint32, C.c_int32, C.c_float, C.c_uint64
+PF, PU, PI, PU64, PV = (C.POINTER(C.c_float), C.POINTER(C.c_uint32), C.POINTER(C.c_int32),
+                        C.POINTER(C.c_uint64), C.c_void_p)
+
+MAX_PASSES = 16
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("triangle_total", u32), ("tri_pos", PF), ("tri_nrm", PF), ("tri_mat", PI),
+                ("material_total", u32), ("mat_diffuse", PF), ("mat_specular", PF),
+                ("object_total", u32), ("object_spheres", PF),
+                ("light_total", u32), ("light_pos", PF), ("light_col", PF), ("light_int", PF)]
+
+
+class CameraDesc(C.Structure):
+    _fields_ = [("eye", f32 * 3), ("direction", f32 * 3), ("right", f32 * 3), ("up", f32 * 3)]
+
+
+class FrameConfigDesc(C.Structure):
+    _fields_ = [("width", u32), ("height", u32), ("algorithm", u32), ("key_mode", u32),
+                ("hit_mode", u32), ("cone_mode", u32), ("leaf", u32), ("branch", u32),
+                ("depth", u32), ("triangle_batch", u32), ("bounces", u32), ("reflect_gate", u32),
+                ("bounce_shadows", u32), ("primary_rays", u32), ("cell_bits", u32), ("dir_bits", u32),
+                ("key_layout", u32), ("scene_min", f32 * 3), ("cell_scale", f32 * 3),
+                ("segment_shift", u32), ("initial_radius", f32), ("initial_spread", f32)]
+
+
+class FrameStats(C.Structure):
+    _fields_ = [("pass_count", u32), ("pass_kind", u32 * MAX_PASSES), ("pass_slots", u32 * MAX_PASSES),
+                ("pass_rays", u32 * MAX_PASSES), ("pass_chunks", u32 * MAX_PASSES),
+                ("pass_pairs_top", u64 * MAX_PASSES), ("pass_pairs_leaf", u64 * MAX_PASSES),
+                ("pass_ray_tests", u64 * MAX_PASSES), ("stage_ms", f32 * 8), ("rays_total", u32)]
+
+
+class Float3(C.Structure):
+    _fields_ = [("x", f32), ("y", f32), ("z", f32)]
+
+
+STAGES = ("create", "trim", "compress", "sort", "decompress", "hierarchy", "traversal", "shading")
+
+# Reference wrapper signatures (include/rsh_engine.h); P = device pointer.
+_P = PV
+_U = C.c_uint
+_F3 = Float3
+COMPAT_SIGNATURES = {
+    "TriangleUpdateWrapper": [_P, _P, _U, _P, _P],
+    "BoundingSphereUpdateWrapper": [_P, _P, _U, _P],
+    "MemoryPreparationWrapper": [_U, _U, _U, _P, _P, _P, _P, _P, _P],
+    "ShadowRayCreationWrapper": [_U, _U, _U, _P, _P, _P, _P],
+    "ReflectionRayCreationWrapper": [_U, _U, _F3, _P, _P, _P, _P],
+    "RayTrimmingWrapper": [_P, _P, _U, _U, _U, _P, _P, _P, _P, PU],
+    "RayCompressionWrapper": [_P, _P, _U, _P, _P, _P, _P, _P, _P, PU],
+    "RaySortingWrapper": [_P, _P, _U, _P, _P],
+    "RayDecompressionWrapper": [_P, _P, _P, _P, _P, _P, _P, _P, _U, _P, _P],
+    "HierarchyCreationWrapper": [_P, _P, _P, _U, f32, f32, _P],
+    "HierarchyTraversalWarmUpWrapper": [_P, _P, _P, _U, _U, _U, _U, _P, _P, _P, _P, PU, PU],
+    "HierarchyTraversalWrapper": [_P, _P, _U, _U, _U, _P, _P, _P, _P, PU, PU],
+    "ShadowRayPreparationWrapper": [_U, _U, _U, _P],
+    "ShadowRayIntersectionWrapper": [_P, _P, _P, _P, _P, _U, _U, _U, _U, _U, _F3, _P],
+    "ShadowRayColoringWrapper": [_U, _U, _U, _F3, _P, _P],
+    "ReflectionRayPreparationWrapper": [_U, _U, _P],
+    "ReflectionRayIntersectionWrapper": [_P, _P, _P, _P, _P, _U, _U, _U, _U, _U, _U, _F3, _P],
+    "ReflectionRayColoringWrapper": [_P, _P, _U, _U, _U, _F3, C.c_int, _P, _P, _P, _P, _P, _P],
+}
+BINDERS = ["bindDiffuseTextureArray", "bindSpecularTextureArray", "bindFragmentPositionArray",
+           "bindFragmentNormalArray", "bindTrianglePositions", "bindTriangleNormals",
+           "bindTriangleTextureCoordinates", "bindTriangleObjectIDs", "bindTriangleMaterialIDs",
+           "bindMaterialDiffuseProperties", "bindMaterialSpecularProperties", "bindBoundingSpheres",
+           "bindLightPositions", "bindLightColors", "bindLightIntensities"]
+NATIVE_SYMBOLS = ["rsh_engine_create", "rsh_engine_destroy", "rsh_last_error", "rsh_engine_set_stream",
+                  "rsh_engine_load_scene", "rsh_engine_render", "rsh_engine_read_pixels",
+                  "rsh_engine_read_gbuffer", "rsh_engine_read_pass", "rsh_engine_pixels_device",
+                  "rsh_device_alloc", "rsh_device_free", "rsh_memcpy_htod", "rsh_memcpy_dtoh",
+                  "rsh_memset_d32", "rsh_device_synchronize", "rsh_device_count",
+                  "rsh_compat_configure", "rsh_compat_set_stream", "rsh_compat_set_exit_on_error",
+                  "rsh_compat_status", "rsh_compat_error", "rsh_compat_reset_status"]
+
+_lib = None
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+def lib():
+    """Load lib/librsh.so (raises LibraryMissing if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LibraryMissing(f"{LIB_PATH} not built: run __graft_entry__.build() or rsh_amd/build.py")
+    L = C.CDLL(LIB_PATH)
+    L.rsh_engine_create.restype = C.c_int
+    L.rsh_engine_create.argtypes = [C.POINTER(PV), C.c_int]
+    L.rsh_engine_destroy.restype = None
+    L.rsh_engine_destroy.argtypes = [PV]
+    L.rsh_last_error.restype = C.c_char_p
+    L.rsh_last_error.argtypes = [PV]
+    L.rsh_engine_set_stream.argtypes = [PV, PV]
+    L.rsh_engine_load_scene.restype = C.c_int
+    L.rsh_engine_load_scene.argtypes = [PV, C.POINTER(SceneDesc)]
+    L.rsh_engine_render.restype = C.c_int
+    L.rsh_engine_render.argtypes = [PV, C.POINTER(CameraDesc), C.POINTER(FrameConfigDesc), u32,
+                                    C.POINTER(FrameStats)]
+    L.rsh_engine_read_pixels.restype = C.c_int
+    L.rsh_engine_read_pixels.argtypes = [PV, PU, C.c_size_t]
+    L.rsh_engine_read_gbuffer.restype = C.c_int
+    L.rsh_engine_read_gbuffer.argtypes = [PV, PF, PF, PF, PF, C.c_size_t]
+    L.rsh_engine_read_pass.restype = C.c_int
+    L.rsh_engine_read_pass.argtypes = [PV, u32, PU64, C.c_size_t]
+    L.rsh_engine_pixels_device.restype = PV
+    L.rsh_engine_pixels_device.argtypes = [PV]
+    L.rsh_device_alloc.restype = PV
+    L.rsh_device_alloc.argtypes = [C.c_size_t]
+    L.rsh_device_free.argtypes = [PV]
+    L.rsh_memcpy_htod.restype = C.c_int
+    L.rsh_memcpy_htod.argtypes = [PV, PV, C.c_size_t]
+    L.rsh_memcpy_dtoh.restype = C.c_int
+    L.rsh_memcpy_dtoh.argtypes = [PV, PV, C.c_size_t]
+    L.rsh_memset_d32.restype = C.c_int
+    L.rsh_memset_d32.argtypes = [PV, u32, C.c_size_t]
+    L.rsh_device_synchronize.restype = C.c_int
+    L.rsh_device_count.restype = C.c_int
+    L.rsh_compat_configure.argtypes = [C.c_uint, C.c_uint, C.c_uint, C.c_int]
+    L.rsh_compat_set_stream.argtypes = [PV]
+    L.rsh_compat_set_exit_on_error.argtypes = [C.c_int]
+    L.rsh_compat_status.restype = C.c_int
+    L.rsh_compat_error.restype = C.c_char_p
+    for name, args in COMPAT_SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = None
+        fn.argtypes = args
+    for name in BINDERS:
+        fn = getattr(L, name)
+        fn.restype = None
+        fn.argtypes = [PV] if "TextureArray" in name or name in ("bindFragmentPositionArray",
+                                                                   "bindFragmentNormalArray") else [PV, C.c_uint]
+    _lib = L
+    return L
+
+
+# ------------------------------------------------------------ device memory --
+class DeviceArray:
+    """A device allocation holding a numpy-shaped array (owned; freed on close)."""
+
+    def __init__(self, shape, dtype):
+        self.shape = tuple(np.atleast_1d(shape)) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = lib().rsh_device_alloc(max(self.nbytes, 4))
+        if not self.ptr:
+            raise MemoryError(f"device allocation of {self.nbytes} bytes failed")
+
+    @classmethod
+    def from_host(cls, arr):
+        arr = np.ascontiguousarray(arr)
+        d = cls(arr.shape, arr.dtype)
+        d.upload(arr)
+        return d
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr, dtype=self.dtype)
+        assert arr.nbytes <= self.nbytes
+        if lib().rsh_memcpy_htod(self.ptr, arr.ctypes.data, arr.nbytes) != 0:
+            raise RuntimeError("host->device copy failed")
+
+    def download(self, count=None):
+        out = np.empty(self.shape, self.dtype)
+        if count is not None:
+            out = out.reshape(-1)[:count]
+        if lib().rsh_memcpy_dtoh(out.ctypes.data, self.ptr, out.nbytes) != 0:
+            raise RuntimeError("device->host copy failed")
+        return out
+
+    def fill_u32(self, value):
+        if lib().rsh_memset_d32(self.ptr, value, self.nbytes // 4) != 0:
+            raise RuntimeError("fill failed")
+
+    def close(self):
+        if self.ptr:
+            lib().rsh_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_count() -> int:
+    return lib().rsh_device_count()
+
+
+# ------------------------------------------------------------------ engine --
+def _fp(a):
+    return a.ctypes.data_as(PF)
+
+
+def scene_desc(s):
+    arrs = [np.ascontiguousarray(x) for x in (s.tri_pos, s.tri_nrm, s.tri_mat, s.mat_diffuse, s.mat_specular,
+                                               s.object_spheres, s.light_pos, s.light_col, s.light_int)]
+    d = SceneDesc(arrs[0].shape[0] // 3, _fp(arrs[0]), _fp(arrs[1]), arrs[2].ctypes.data_as(PI), arrs[3].shape[0],
+                  _fp(arrs[3]), _fp(arrs[4]), arrs[5].shape[0], _fp(arrs[5]), arrs[6].shape[0], _fp(arrs[6]),
+                  _fp(arrs[7]), _fp(arrs[8]))
+    d.keep = arrs
+    return d
+
+
+def camera_desc(cam):
+    c = CameraDesc()
+    c.eye[:] = [float(v) for v in cam.eye]
+    c.direction[:] = [float(v) for v in cam.direction]
+    c.right[:] = [float(v) for v in cam.right]
+    c.up[:] = [float(v) for v in cam.up]
+    return c
+
+
+def config_desc(cfg):
+    o = FrameConfigDesc()
+    for name, _ in FrameConfigDesc._fields_:
+        v = getattr(cfg, name)
+        if name in ("scene_min", "cell_scale"):
+            getattr(o, name)[:] = [float(x) for x in v]
+        else:
+            setattr(o, name, v)
+    return o
+
+
+class RshError(RuntimeError):
+    pass
+
+
+class Engine:
+    """Device-resident scene + frame renderer (include/rsh.h)."""
+
+    KEEP_PASSES, COUNTERS, TIMING = 1, 2, 4
+
+    def __init__(self, device: int = 0):
+        self.L = lib()
+        h = PV()
+        rc = self.L.rsh_engine_create(C.byref(h), device)
+        if rc != 0:
+            raise RshError(f"rsh_engine_create failed ({rc})")
+        self.h = h
+        self.cfg = None
+        self.stats = None
+
+    def _check(self, rc, what):
+        if rc != 0:
+            raise RshError(f"{what} failed ({rc}): {self.L.rsh_last_error(self.h).decode()}")
+
+    def load_scene(self, scene):
+        self._scene = scene
+        self._check(self.L.rsh_engine_load_scene(self.h, C.byref(scene_desc(scene))), "load_scene")
+
+    def render(self, camera, cfg, flags=0) -> FrameStats:
+        self.cfg = cfg
+        st = FrameStats()
+        self._check(self.L.rsh_engine_render(self.h, C.byref(camera_desc(camera)), C.byref(config_desc(cfg)),
+                                             flags, C.byref(st)), "render")
+        self.stats = st
+        return st
+
+    def pixels(self):
+        W, H = self.cfg.width, self.cfg.height
+        out = np.empty(W * H, np.uint32)
+        self._check(self.L.rsh_engine_read_pixels(self.h, out.ctypes.data_as(PU), out.size), "read_pixels")
+        return out.reshape(H, W)
+
+    def gbuffer(self):
+        n = self.cfg.width * self.cfg.height
+        g = {k: np.empty((n, 4), np.float32) for k in ("dif", "spec", "pos", "nrm")}
+        self._check(self.L.rsh_engine_read_gbuffer(self.h, _fp(g["dif"]), _fp(g["spec"]), _fp(g["pos"]),
+                                                   _fp(g["nrm"]), n), "read_gbuffer")
+        return g
+
+    def pass_hits(self, i):
+        n = self.stats.pass_slots[i]
+        out = np.empty(n, np.uint64)
+        self._check(self.L.rsh_engine_read_pass(self.h, i, out.ctypes.data_as(PU64), n), "read_pass")
+        return out
+
+    def passes(self):
+        st = self.stats
+        return [dict(kind=st.pass_kind[i], slots=st.pass_slots[i], rays=st.pass_rays[i], chunks=st.pass_chunks[i],
+                     pairs_top=st.pass_pairs_top[i], pairs_leaf=st.pass_pairs_leaf[i],
+                     ray_tests=st.pass_ray_tests[i]) for i in range(st.pass_count)]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.rsh_engine_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def image_rgb(pixels: np.ndarray) -> np.ndarray:
+    """RGBA8-packed pixels (R | G<<8 | B<<16) -> uint8 [H, W, 3], top row first."""
+    p = pixels.astype(np.uint32)
+    rgb = np.stack([p & 255, (p >> 8) & 255, (p >> 16) & 255], -1).astype(np.uint8)
+    return rgb[::-1]
+
+
+def write_ppm(path: str, pixels: np.ndarray):
+    rgb = image_rgb(pixels)
+    with open(path, "wb") as f:
+        f.write(b"P6 %d %d 255\n" % (rgb.shape[1], rgb.shape[0]))
+        f.write(rgb.tobytes())

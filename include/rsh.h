@@ -85,6 +85,8 @@ typedef struct rsh_frame_config {
     float cell_scale[3];            /* cells per world unit                              */
     uint32_t segment_shift;         /* reserved: segment id position (batched frames)    */
     float initial_radius, initial_spread;
+    uint32_t traversal_kernel;      /* 0 automatic, 1 generic kernel for every shape     */
+    uint32_t traversal_splits;      /* waves per top node; 0 = sized to fill the GPU      */
 } rsh_frame_config;
 
 #define RSH_MAX_PASSES 16
@@ -101,6 +103,9 @@ typedef struct rsh_frame_stats {
     float stage_ms[8];                       /* create, trim, compress, sort, decompress,
                                                 hierarchy, traversal, shading (summed) */
     uint32_t rays_total;
+    uint64_t pass_top_survivors[RSH_MAX_PASSES]; /* top-level (node, triangle) pairs passing */
+    float pass_traverse_ms[RSH_MAX_PASSES];      /* traversal kernel time per pass (bit 2)   */
+    uint32_t pass_splits[RSH_MAX_PASSES];        /* waves per top node used by the pass      */
 } rsh_frame_stats;
 
 typedef struct rsh_engine rsh_engine;

@@ -53,6 +53,7 @@ struct rsh_engine {
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> timed;
     size_t ev_used = 0;
+    uint32_t pass_index = 0;   // passes traced so far in the current frame
 };
 
 namespace {
@@ -111,6 +112,8 @@ struct StageTimer {
 };
 
 enum { ST_CREATE = 0, ST_TRIM, ST_COMPRESS, ST_SORT, ST_DECOMPRESS, ST_HIER, ST_TRAVERSE, ST_SHADE };
+constexpr int ST_PASS_TRAVERSE = 100;   // + pass index: per-pass traversal time
+constexpr uint64_t TRAVERSE_TARGET_WAVES = 16384;
 
 rsh_status ensure_frame(rsh_engine* e, uint32_t W, uint32_t H, uint32_t Lmax) {
     size_t S = (size_t)W * H * Lmax, P = (size_t)W * H;
@@ -167,7 +170,7 @@ KeyCfg key_cfg(const rsh_frame_config& c) {
     return k;
 }
 
-struct PassOut { uint32_t N = 0, C = 0; uint64_t top = 0, leaf = 0, tests = 0; };
+struct PassOut { uint32_t N = 0, C = 0, splits = 0; uint64_t top = 0, leaf = 0, tests = 0, surv = 0; };
 
 // One ray batch: trim -> (compress -> sort -> decompress) -> gather ->
 // hierarchy -> fused traversal + intersection.  kind 0: closest hit into
@@ -240,16 +243,29 @@ rsh_status trace_pass(rsh_engine* e, const rsh_frame_config& cfg, const float* r
     p.algorithm = cfg.algorithm; p.kind = kind; p.hit_mode = cfg.hit_mode;
     p.closest = u64p(e->closest); p.shadow = u32p(e->shadow);
     p.counters = counting ? u64p(e->counters) : nullptr;
-    if (counting) launch_fill_u64(u64p(e->counters), 0ull, 3, s);
+    p.force_generic = cfg.traversal_kernel == 1 ? 1u : 0u;
+    // Enough waves to fill the GPU: passes with few top nodes split each top
+    // node's triangle range across waves (at least 64 triangles per wave).
     {
-        StageTimer t(e, ST_TRAVERSE, timing);
+        const uint64_t base = (uint64_t)counts[cfg.depth - 1] * p.waves_per_top;
+        uint64_t sp = (TRAVERSE_TARGET_WAVES + base - 1) / (base ? base : 1);
+        const uint64_t max_sp = e->T / 64 ? e->T / 64 : 1;
+        if (sp > max_sp) sp = max_sp;
+        if (sp > 1024) sp = 1024;
+        if (cfg.traversal_splits) sp = cfg.traversal_splits;
+        p.splits = (uint32_t)(sp ? sp : 1);
+        out.splits = p.splits;
+    }
+    if (counting) launch_fill_u64(u64p(e->counters), 0ull, 4, s);
+    {
+        StageTimer t(e, ST_PASS_TRAVERSE + (int)e->pass_index, timing);
         HIP_OK(launch_traverse(p, s));
     }
     if (counting) {
-        unsigned long long c[3];
+        unsigned long long c[4];
         HIP_OK(hipMemcpyAsync(c, e->counters.p, sizeof(c), hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
-        out.top = c[0]; out.leaf = c[1]; out.tests = c[2];
+        out.top = c[0]; out.leaf = c[1]; out.tests = c[2]; out.surv = c[3];
     }
     return RSH_OK;
 }
@@ -257,10 +273,13 @@ rsh_status trace_pass(rsh_engine* e, const rsh_frame_config& cfg, const float* r
 rsh_status record(rsh_engine* e, rsh_frame_stats* st, uint32_t kind, uint32_t S, const PassOut& po, uint32_t fl) {
     if (st && st->pass_count < RSH_MAX_PASSES) {
         uint32_t i = st->pass_count++;
+        st->pass_splits[i] = po.splits;
         st->pass_kind[i] = kind; st->pass_slots[i] = S; st->pass_rays[i] = po.N; st->pass_chunks[i] = po.C;
         st->pass_pairs_top[i] = po.top; st->pass_pairs_leaf[i] = po.leaf; st->pass_ray_tests[i] = po.tests;
+        st->pass_top_survivors[i] = po.surv;
         st->rays_total += po.N;
     }
+    e->pass_index++;
     if (fl & 1u) {
         rsh_engine::Kept k;
         k.kind = kind; k.slots = S;
@@ -412,6 +431,7 @@ rsh_status rsh_engine_render(rsh_engine* e, const rsh_camera* cam, const rsh_fra
     e->kept.clear();
     e->timed.clear();
     e->ev_used = 0;
+    e->pass_index = 0;
     if (st) std::memset(st, 0, sizeof(*st));
     hipStream_t s = e->stream;
     const uint32_t W = cfg.width, H = cfg.height, Lm = e->L, P = W * H, S = W * H * Lm;
@@ -493,7 +513,13 @@ rsh_status rsh_engine_render(rsh_engine* e, const rsh_camera* cam, const rsh_fra
     if (st && timing) {
         for (auto& t : e->timed) {
             float ms = 0.0f;
-            if (hipEventElapsedTime(&ms, t.second.first, t.second.second) == hipSuccess) st->stage_ms[t.first] += ms;
+            if (hipEventElapsedTime(&ms, t.second.first, t.second.second) != hipSuccess) continue;
+            if (t.first >= ST_PASS_TRAVERSE) {
+                st->stage_ms[ST_TRAVERSE] += ms;
+                if (t.first - ST_PASS_TRAVERSE < RSH_MAX_PASSES) st->pass_traverse_ms[t.first - ST_PASS_TRAVERSE] += ms;
+            } else {
+                st->stage_ms[t.first] += ms;
+            }
         }
     }
     return RSH_OK;

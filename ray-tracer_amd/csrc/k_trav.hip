@@ -111,12 +111,26 @@ __global__ void __launch_bounds__(256) k_levelN(float4* nodes, float2* aux, uint
 }
 
 // ---------------------------------------------------- fused traversal ----
+// Object whose triangle range contains t (objects tile [0, T) in order);
+// wave-uniform binary search over the first-triangle column.
+__device__ __forceinline__ uint32_t object_of(const uint4* ranges, uint32_t K, uint32_t t) {
+    uint32_t a = 0, b = K;      // ranges[a].x <= t < ranges[b].x (b = K: end)
+    while (b - a > 1) {
+        uint32_t m = (a + b) >> 1;
+        if (ranges[m].x <= t) a = m; else b = m;
+    }
+    return a;
+}
+
 template <int DEPTH>
 __global__ void __launch_bounds__(256) k_traverse(TravParams P) {
     const uint32_t lane = lane_id();
     const uint32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const uint32_t top = wave / P.waves_per_top;
-    const uint32_t sub = wave - top * P.waves_per_top;
+    const uint32_t per_top = P.waves_per_top * P.splits;
+    const uint32_t top = wave / per_top;
+    const uint32_t rem = wave - top * per_top;
+    const uint32_t sub = rem / P.splits;
+    const uint32_t split = rem - sub * P.splits;
     const uint32_t topLevel = DEPTH - 1;
     if (top >= P.level_count[topLevel]) return;
     const uint32_t local = sub * 64 + lane;
@@ -166,7 +180,7 @@ __global__ void __launch_bounds__(256) k_traverse(TravParams P) {
     bool occluded = false;
     uint64_t alive = __ballot(valid);
     const bool count = P.counters != nullptr;
-    uint64_t n_top = 0, n_leaf = 0, n_tests = 0;
+    uint64_t n_top = 0, n_leaf = 0, n_tests = 0, n_surv = 0;
     bool done = false;
 
     auto process_range = [&](uint32_t first, uint32_t last, uint32_t exlo, uint32_t exhi) {
@@ -175,8 +189,11 @@ __global__ void __launch_bounds__(256) k_traverse(TravParams P) {
             const bool inr = t <= last && !(t >= exlo && t < exhi);
             bool p = false;
             if (inr) p = sphere_node(tS, tC, P.tri_sphere[t], tA.x, tA.y);
-            if (count && sub == 0) n_top += (uint64_t)__popcll(__ballot(inr));
             uint64_t m = __ballot(p);
+            if (count && sub == 0) {
+                n_top += (uint64_t)__popcll(__ballot(inr));
+                n_surv += (uint64_t)__popcll(m);
+            }
             while (m) {
                 const uint32_t tt = __builtin_amdgcn_readfirstlane(tb + (uint32_t)__builtin_ctzll(m));
                 m &= m - 1;
@@ -207,32 +224,208 @@ __global__ void __launch_bounds__(256) k_traverse(TravParams P) {
         }
     };
 
-    if (P.algorithm == 1) {
-        if (P.T) process_range(0, P.T - 1, 0, 0);
-    } else {
-        for (uint32_t ob = 0; ob < P.K && !done; ob += 64) {
-            const uint32_t oi = ob + lane;
-            bool po = false;
-            if (oi < P.K) po = sphere_node(tS, tC, P.obj_sphere[oi], tA.x, tA.y);
-            uint64_t m = __ballot(po);
-            while (m && !done) {
-                const uint32_t ok = __builtin_amdgcn_readfirstlane(ob + (uint32_t)__builtin_ctzll(m));
-                m &= m - 1;
-                const uint4 r = P.obj_range[ok];
-                process_range(r.x, r.y, r.z, r.w);
+    const uint32_t lo = (uint32_t)(((uint64_t)split * P.T) / P.splits);
+    const uint32_t hi = (uint32_t)(((uint64_t)(split + 1) * P.T) / P.splits);
+    if (hi > lo) {
+        if (P.algorithm == 1) {
+            process_range(lo, hi - 1, 0, 0);
+        } else {
+            const uint32_t k0 = object_of(P.obj_range, P.K, lo);
+            const uint32_t k1 = object_of(P.obj_range, P.K, hi - 1);
+            for (uint32_t ob = k0; ob <= k1 && !done; ob += 64) {
+                const uint32_t oi = ob + lane;
+                bool po = false;
+                if (oi <= k1) po = sphere_node(tS, tC, P.obj_sphere[oi], tA.x, tA.y);
+                uint64_t m = __ballot(po);
+                while (m && !done) {
+                    const uint32_t ok = __builtin_amdgcn_readfirstlane(ob + (uint32_t)__builtin_ctzll(m));
+                    m &= m - 1;
+                    const uint4 r = P.obj_range[ok];
+                    process_range(r.x > lo ? r.x : lo, r.y < hi - 1 ? r.y : hi - 1, r.z, r.w);
+                }
             }
         }
     }
 
     if (valid) {
         if (P.kind == 1) { if (occluded) P.shadow[slot] = MISS32; }
-        else P.closest[slot] = best;
+        else if (P.splits == 1) P.closest[slot] = best;
+        else if (best != (P.hit_mode == 0 ? MISS64 : (unsigned long long)MISS32)) atomicMin(&P.closest[slot], best);
     }
     if (count) {
         if (lane == 0) {
             atomicAdd(&P.counters[0], (unsigned long long)n_top);
             atomicAdd(&P.counters[1], (unsigned long long)n_leaf);
             atomicAdd(&P.counters[2], (unsigned long long)n_tests);
+            atomicAdd(&P.counters[3], (unsigned long long)n_surv);
+        }
+    }
+}
+
+// ------------------------------------- fused traversal, 64-ray top nodes ----
+// Specialisation for depth 2 with leaf * branch == 64: one wave = one top
+// node = BRANCH leaves of LEAF rays.  Three stages per block of 64 candidate
+// triangles:
+//  A  lane j tests triangle tb+j against the top node; survivors' bounding
+//     sphere, index and Moller-Trumbore vectors go to the wave's LDS slice in
+//     survivor order (one parallel vector load instead of a dependent load
+//     per survivor);
+//  B  (survivor, leaf) pairs are tested 64 at a time: lane i takes leaf
+//     i % BRANCH (its node is held in registers) and survivor i / BRANCH;
+//  C  each leaf's LEAF lanes (lane i holds ray i of the top node) walk the
+//     survivors that passed their leaf, all leaf groups in parallel.
+template <int LEAF>
+__global__ void __launch_bounds__(256) k_traverse64(TravParams P) {
+    constexpr int BRANCH = 64 / LEAF;
+    constexpr int GROUP = 64 / BRANCH;          // survivors per stage-B step
+    __shared__ float4 s_sph[4][64];
+    __shared__ float4 s_mt[4][64][3];
+    __shared__ uint32_t s_idx[4][64];
+    const uint32_t lane = lane_id();
+    const uint32_t w = threadIdx.x >> 6;
+    const uint32_t wave = blockIdx.x * 4 + w;
+    const uint32_t top = wave / P.splits;
+    const uint32_t split = wave - top * P.splits;
+    if (top >= P.level_count[1]) return;
+    const uint32_t pos = top * 64 + lane;
+    const bool valid = pos < P.N;
+
+    v3 o = mk(0, 0, 0), d = mk(0, 0, 0);
+    uint32_t slot = 0;
+    float tmax = 0.0f;
+    if (valid) {
+        float4 a = P.rec0[pos], b = P.rec1[pos];
+        o = xyz(a); d = xyz(b);
+        slot = __float_as_uint(a.w);
+        tmax = b.w;
+    }
+    const v3 oe = o + d * EPS;
+    const uint32_t gt = P.level_offset[1] + top;
+    const float4 tS = P.nodes[2 * (size_t)gt], tC = P.nodes[2 * (size_t)gt + 1];
+    const float2 tA = P.aux[gt];
+    // stage-B leaf of this lane: leaf (top * BRANCH + lane % BRANCH)
+    const uint32_t bl = top * BRANCH + (lane % BRANCH);
+    const bool bl_valid = bl < P.level_count[0];
+    float4 lS = make_float4(0, 0, 0, 0), lC = make_float4(0, 0, 1, 0);
+    float2 lA = make_float2(1, 0);
+    if (bl_valid) {
+        lS = P.nodes[2 * (size_t)bl];
+        lC = P.nodes[2 * (size_t)bl + 1];
+        lA = P.aux[bl];
+    }
+    const uint32_t my_leaf = lane / LEAF;       // stage-C leaf of this lane's ray
+    const bool leaf_head = valid && (lane % LEAF) == 0;
+    (void)leaf_head;
+
+    unsigned long long best = P.hit_mode == 0 ? MISS64 : (unsigned long long)MISS32;
+    bool occluded = false;
+    const uint64_t alive = __ballot(valid);
+    const bool count = P.counters != nullptr;
+    uint64_t n_top = 0, n_surv = 0, n_leaf = 0, n_tests = 0;
+    bool done = false;
+
+    auto process_range = [&](uint32_t first, uint32_t last, uint32_t exlo, uint32_t exhi) {
+        for (uint32_t tb = first; tb <= last && !done; tb += 64) {
+            // ---- stage A
+            const uint32_t t = tb + lane;
+            const bool inr = t <= last && !(t >= exlo && t < exhi);
+            float4 ts = make_float4(0, 0, 0, 0);
+            bool p = false;
+            if (inr) {
+                ts = P.tri_sphere[t];
+                p = sphere_node(tS, tC, ts, tA.x, tA.y);
+            }
+            const uint64_t m = __ballot(p);
+            if (count) { n_top += (uint64_t)__popcll(__ballot(inr)); n_surv += (uint64_t)__popcll(m); }
+            if (m == 0ull) continue;
+            const uint32_t k = (uint32_t)__popcll(m);
+            if (p) {
+                const uint32_t r = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                s_sph[w][r] = ts;
+                s_idx[w][r] = t;
+                s_mt[w][r][0] = P.tri_mt[3 * (size_t)t];
+                s_mt[w][r][1] = P.tri_mt[3 * (size_t)t + 1];
+                s_mt[w][r][2] = P.tri_mt[3 * (size_t)t + 2];
+            }
+            __builtin_amdgcn_wave_barrier();
+            // ---- stages B and C, GROUP survivors at a time
+            for (uint32_t g = 0; g < k && !done; g += GROUP) {
+                const uint32_t s = g + lane / BRANCH;
+                bool lp = false;
+                if (s < k && bl_valid) lp = sphere_node(lS, lC, s_sph[w][s], lA.x, lA.y);
+                const uint64_t pm = __ballot(lp);
+                if (count) n_leaf += (uint64_t)__popcll(pm);
+                if (pm == 0ull) continue;
+                // pending survivors of this lane's leaf: bit j <=> pair (g + j, my_leaf)
+                uint32_t q = 0;
+#pragma unroll
+                for (int j = 0; j < GROUP; j++) q |= (uint32_t)((pm >> (j * BRANCH + my_leaf)) & 1ull) << j;
+                if (!valid || occluded) q = 0;
+                while (__ballot(q != 0u)) {
+                    if (q) {
+                        const uint32_t j = (uint32_t)__builtin_ctz(q);
+                        q &= q - 1u;
+                        const uint32_t r = g + j;
+                        const float4 A = s_mt[w][r][0], B = s_mt[w][r][1], Cc = s_mt[w][r][2];
+                        const float th = ray_triangle(oe, d, xyz(A), xyz(B), xyz(Cc));
+                        if (count) n_tests++;
+                        if (P.kind == 1) {
+                            if (th > EPS && th < tmax) { occluded = true; q = 0; }
+                        } else if (th > EPS) {
+                            const unsigned long long rec = pack_hit(th, s_idx[w][r], P.hit_mode);
+                            best = rec < best ? rec : best;
+                        }
+                    }
+                }
+                if (P.kind == 1 && (__ballot(occluded) & alive) == alive) done = true;
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    };
+
+    // This wave's share of the triangle range (a top node may be split across
+    // waves when the pass has few top nodes).
+    const uint32_t lo = (uint32_t)(((uint64_t)split * P.T) / P.splits);
+    const uint32_t hi = (uint32_t)(((uint64_t)(split + 1) * P.T) / P.splits);
+    if (hi > lo) {
+        if (P.algorithm == 1) {
+            process_range(lo, hi - 1, 0, 0);
+        } else {
+            const uint32_t k0 = object_of(P.obj_range, P.K, lo);
+            const uint32_t k1 = object_of(P.obj_range, P.K, hi - 1);
+            for (uint32_t ob = k0; ob <= k1 && !done; ob += 64) {
+                const uint32_t oi = ob + lane;
+                bool po = false;
+                if (oi <= k1) po = sphere_node(tS, tC, P.obj_sphere[oi], tA.x, tA.y);
+                uint64_t m = __ballot(po);
+                while (m && !done) {
+                    const uint32_t ok = __builtin_amdgcn_readfirstlane(ob + (uint32_t)__builtin_ctzll(m));
+                    m &= m - 1;
+                    const uint4 r = P.obj_range[ok];
+                    process_range(r.x > lo ? r.x : lo, r.y < hi - 1 ? r.y : hi - 1, r.z, r.w);
+                }
+            }
+        }
+    }
+
+    if (valid) {
+        if (P.splits == 1) {
+            if (P.kind == 1) { if (occluded) P.shadow[slot] = MISS32; }
+            else P.closest[slot] = best;
+        } else {
+            if (P.kind == 1) { if (occluded) P.shadow[slot] = MISS32; }
+            else if (best != (P.hit_mode == 0 ? MISS64 : (unsigned long long)MISS32)) atomicMin(&P.closest[slot], best);
+        }
+    }
+    if (count) {
+        // per-lane test counts -> wave sum
+        unsigned long long tsum = n_tests;
+        for (int dlt = 32; dlt > 0; dlt >>= 1) tsum += __shfl_xor(tsum, dlt, 64);
+        if (lane == 0) {
+            atomicAdd(&P.counters[0], (unsigned long long)n_top);
+            atomicAdd(&P.counters[1], (unsigned long long)n_leaf);
+            atomicAdd(&P.counters[2], tsum);
+            atomicAdd(&P.counters[3], (unsigned long long)n_surv);
         }
     }
 }
@@ -379,9 +572,21 @@ void launch_hierarchy(const float4* rec0, const float4* rec1, const float* rays,
 
 hipError_t launch_traverse(const TravParams& p, hipStream_t s) {
     const uint32_t top = p.level_count[p.depth - 1];
-    const uint64_t waves = (uint64_t)top * p.waves_per_top;
+    if (p.splits == 0) return hipErrorInvalidValue;
+    const uint64_t waves = (uint64_t)top * p.waves_per_top * p.splits;
     if (waves == 0) return hipSuccess;
+    if (waves > 0xFFFFFFFFull) return hipErrorInvalidValue;
     const uint32_t blocks = (uint32_t)((waves + 3) / 4);
+    if (p.depth == 2 && p.leaf * p.branch == 64 && !p.force_generic) {
+        switch (p.leaf) {
+            case 2: k_traverse64<2><<<blocks, 256, 0, s>>>(p); return hipGetLastError();
+            case 4: k_traverse64<4><<<blocks, 256, 0, s>>>(p); return hipGetLastError();
+            case 8: k_traverse64<8><<<blocks, 256, 0, s>>>(p); return hipGetLastError();
+            case 16: k_traverse64<16><<<blocks, 256, 0, s>>>(p); return hipGetLastError();
+            case 32: k_traverse64<32><<<blocks, 256, 0, s>>>(p); return hipGetLastError();
+            default: break;
+        }
+    }
     switch (p.depth) {
         case 1: k_traverse<1><<<blocks, 256, 0, s>>>(p); break;
         case 2: k_traverse<2><<<blocks, 256, 0, s>>>(p); break;

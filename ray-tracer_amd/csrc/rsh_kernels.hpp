@@ -48,7 +48,9 @@ struct TravParams {
     uint32_t hit_mode;            // 0 exact, 1 reference packing
     unsigned long long* closest;  // [S] per slot
     uint32_t* shadow;             // [S] per slot
-    unsigned long long* counters; // optional [3]: top pairs, leaf pairs, ray tests
+    unsigned long long* counters; // optional [4]: top pairs, leaf pairs, ray tests, top survivors
+    uint32_t force_generic;       // 1: use the generic kernel even for 64-ray top nodes
+    uint32_t splits;              // waves sharing one top node's triangle range (>= 1)
 };
 
 // --- rays ---

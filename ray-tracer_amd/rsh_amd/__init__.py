@@ -41,14 +41,17 @@ class FrameConfigDesc(C.Structure):
                 ("depth", u32), ("triangle_batch", u32), ("bounces", u32), ("reflect_gate", u32),
                 ("bounce_shadows", u32), ("primary_rays", u32), ("cell_bits", u32), ("dir_bits", u32),
                 ("key_layout", u32), ("scene_min", f32 * 3), ("cell_scale", f32 * 3),
-                ("segment_shift", u32), ("initial_radius", f32), ("initial_spread", f32)]
+                ("segment_shift", u32), ("initial_radius", f32), ("initial_spread", f32),
+                ("traversal_kernel", u32), ("traversal_splits", u32)]
 
 
 class FrameStats(C.Structure):
     _fields_ = [("pass_count", u32), ("pass_kind", u32 * MAX_PASSES), ("pass_slots", u32 * MAX_PASSES),
                 ("pass_rays", u32 * MAX_PASSES), ("pass_chunks", u32 * MAX_PASSES),
                 ("pass_pairs_top", u64 * MAX_PASSES), ("pass_pairs_leaf", u64 * MAX_PASSES),
-                ("pass_ray_tests", u64 * MAX_PASSES), ("stage_ms", f32 * 8), ("rays_total", u32)]
+                ("pass_ray_tests", u64 * MAX_PASSES), ("stage_ms", f32 * 8), ("rays_total", u32),
+                ("pass_top_survivors", u64 * MAX_PASSES), ("pass_traverse_ms", f32 * MAX_PASSES),
+                ("pass_splits", u32 * MAX_PASSES)]
 
 
 class Float3(C.Structure):
@@ -304,7 +307,9 @@ class Engine:
         st = self.stats
         return [dict(kind=st.pass_kind[i], slots=st.pass_slots[i], rays=st.pass_rays[i], chunks=st.pass_chunks[i],
                      pairs_top=st.pass_pairs_top[i], pairs_leaf=st.pass_pairs_leaf[i],
-                     ray_tests=st.pass_ray_tests[i]) for i in range(st.pass_count)]
+                     ray_tests=st.pass_ray_tests[i], top_survivors=st.pass_top_survivors[i],
+                     traverse_ms=st.pass_traverse_ms[i], splits=st.pass_splits[i])
+                for i in range(st.pass_count)]
 
     def close(self):
         if getattr(self, "h", None):

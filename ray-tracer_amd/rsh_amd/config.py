@@ -43,6 +43,8 @@ class FrameConfig:
     segment_shift: int = 0
     initial_radius: float = 0.0
     initial_spread: float = 0.0
+    traversal_kernel: int = 0       # engine-only: 0 automatic, 1 generic kernel
+    traversal_splits: int = 0       # engine-only: waves per top node, 0 = fill the GPU
 
     def with_scene_bounds(self, scene: Scene) -> "FrameConfig":
         """Cell grid over the scene AABB: 2^cell_bits cells per axis."""

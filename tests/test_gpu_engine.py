@@ -53,6 +53,15 @@ def test_config0_default(engine, oracle, rsh, w):
     dict(depth=1),
     dict(leaf=16, branch=4),
     dict(leaf=4, branch=16),
+    dict(leaf=32, branch=2),
+    dict(leaf=2, branch=32),
+    dict(traversal_kernel=1),                # generic kernel on the 64-ray shape
+    dict(traversal_kernel=1, algorithm=1),
+    dict(algorithm=1, bounces=2, bounce_shadows=1),
+    dict(traversal_splits=3),                # top nodes split across waves, merged by atomicMin
+    dict(traversal_splits=7, algorithm=1),
+    dict(traversal_splits=5, traversal_kernel=1, bounces=2, bounce_shadows=1),
+    dict(traversal_splits=4, depth=3, branch=4),
     dict(key_layout=1),
     dict(triangle_batch=512),
     dict(bounces=2, bounce_shadows=1),

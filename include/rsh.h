@@ -86,7 +86,9 @@ typedef struct rsh_frame_config {
     uint32_t segment_shift;         /* reserved: segment id position (batched frames)    */
     float initial_radius, initial_spread;
     uint32_t traversal_kernel;      /* 0 automatic, 1 generic kernel for every shape     */
-    uint32_t traversal_splits;      /* waves per top node; 0 = sized to fill the GPU      */
+    uint32_t traversal_splits;      /* generic kernel: waves per top node; 0 = fill the GPU */
+    uint32_t traversal_chunk;       /* 64-ray kernel: candidate triangles per work item;
+                                       0 = sized per pass                                */
 } rsh_frame_config;
 
 #define RSH_MAX_PASSES 16
@@ -105,7 +107,9 @@ typedef struct rsh_frame_stats {
     uint32_t rays_total;
     uint64_t pass_top_survivors[RSH_MAX_PASSES]; /* top-level (node, triangle) pairs passing */
     float pass_traverse_ms[RSH_MAX_PASSES];      /* traversal kernel time per pass (bit 2)   */
-    uint32_t pass_splits[RSH_MAX_PASSES];        /* waves per top node used by the pass      */
+    uint32_t pass_splits[RSH_MAX_PASSES];        /* generic kernel: waves per top node       */
+    uint32_t pass_items[RSH_MAX_PASSES];         /* 64-ray kernel: work items launched       */
+    uint32_t pass_chunk[RSH_MAX_PASSES];         /* 64-ray kernel: candidates per work item  */
 } rsh_frame_stats;
 
 typedef struct rsh_engine rsh_engine;

@@ -42,6 +42,8 @@ struct rsh_engine {
     DevBuf rays_a, rays_b, flags, keys, values, tkeys, tvalues, bases, sizes, ckeys, cvalues, skeys, svalues;
     DevBuf rkeys, rvalues, skeleton, scan, rec0, rec1, nodes, aux;
     DevBuf closest, shadow, pixels, counters, dcounts;
+    // work list of the 64-ray traversal kernel
+    DevBuf w_npass, w_cand, w_nitems, w_off, w_first, w_list, w_cum, w_items;
     DevBuf g0[4], gb[4];
     ScanScratch sc;
 
@@ -114,6 +116,12 @@ struct StageTimer {
 enum { ST_CREATE = 0, ST_TRIM, ST_COMPRESS, ST_SORT, ST_DECOMPRESS, ST_HIER, ST_TRAVERSE, ST_SHADE };
 constexpr int ST_PASS_TRAVERSE = 100;   // + pass index: per-pass traversal time
 constexpr uint64_t TRAVERSE_TARGET_WAVES = 16384;
+// 64-ray kernel work items: a pass starts at ITEM_CHUNK candidates per item
+// and shrinks the chunk (not below ITEM_CHUNK_MIN) until it has at least
+// ITEM_TARGET items, several per resident wave slot of the GPU.
+constexpr uint32_t ITEM_CHUNK = 4096;
+constexpr uint32_t ITEM_CHUNK_MIN = 256;
+constexpr uint32_t ITEM_TARGET = 32768;
 
 rsh_status ensure_frame(rsh_engine* e, uint32_t W, uint32_t H, uint32_t Lmax) {
     size_t S = (size_t)W * H * Lmax, P = (size_t)W * H;
@@ -170,7 +178,51 @@ KeyCfg key_cfg(const rsh_frame_config& c) {
     return k;
 }
 
-struct PassOut { uint32_t N = 0, C = 0, splits = 0; uint64_t top = 0, leaf = 0, tests = 0, surv = 0; };
+struct PassOut {
+    uint32_t N = 0, C = 0, splits = 0, items = 0, chunk = 0;
+    uint64_t top = 0, leaf = 0, tests = 0, surv = 0;
+};
+
+// Work list of the 64-ray kernel (see uses_items): cull, size, scan, fill.
+rsh_status build_items(rsh_engine* e, const rsh_frame_config& cfg, TravParams& p, PassOut& out) {
+    hipStream_t s = e->stream;
+    const uint32_t ntop = p.level_count[1];
+    const size_t nb = ((size_t)ntop + 1) * 4;
+    for (DevBuf* b : {&e->w_npass, &e->w_cand, &e->w_nitems, &e->w_off, &e->w_first}) HIP_OK(b->reserve(nb));
+    uint32_t* npass = u32p(e->w_npass);
+    uint32_t* cand = u32p(e->w_cand);
+    uint32_t* nitems = u32p(e->w_nitems);
+    launch_cull_count(p, npass, cand, s);
+    if (p.algorithm != 1) HIP_OK(run_exclusive_scan(npass, u32p(e->w_off), (uint64_t)ntop + 1, e->sc, s));
+    uint32_t chunk = cfg.traversal_chunk ? cfg.traversal_chunk : ITEM_CHUNK;
+    uint32_t tot[2] = {0, 0};   // list entries, items
+    for (;;) {
+        launch_item_count(cand, chunk, ntop, nitems, s);
+        HIP_OK(run_exclusive_scan(nitems, u32p(e->w_first), (uint64_t)ntop + 1, e->sc, s));
+        if (p.algorithm != 1) HIP_OK(hipMemcpyAsync(tot, u32p(e->w_off) + ntop, 4, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(tot + 1, u32p(e->w_first) + ntop, 4, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        if (cfg.traversal_chunk || tot[1] >= ITEM_TARGET || chunk <= ITEM_CHUNK_MIN || tot[1] == 0) break;
+        const uint64_t want = ((uint64_t)chunk * tot[1] / ITEM_TARGET) & ~63ull;
+        chunk = want < ITEM_CHUNK_MIN ? ITEM_CHUNK_MIN : (uint32_t)want;
+    }
+    if (tot[1] > MAX_TRAVERSE_ITEMS) {
+        e->err = "traversal work list of " + std::to_string(tot[1]) + " items exceeds " +
+                 std::to_string(MAX_TRAVERSE_ITEMS) + " (raise traversal_chunk)";
+        return RSH_CAPACITY;
+    }
+    HIP_OK(e->w_list.reserve((size_t)tot[0] * 4 + 4));
+    HIP_OK(e->w_cum.reserve((size_t)tot[0] * 4 + 4));
+    HIP_OK(e->w_items.reserve((size_t)tot[1] * 4 + 4));
+    p.item_top = u32p(e->w_items); p.item_first = u32p(e->w_first); p.cand = cand;
+    p.obj_off = u32p(e->w_off); p.obj_list = u32p(e->w_list); p.obj_cum = u32p(e->w_cum);
+    p.chunk = chunk; p.items = tot[1];
+    launch_cull_list(p, u32p(e->w_list), u32p(e->w_cum), s);     // writes at p.obj_off
+    launch_item_fill(u32p(e->w_first), nitems, ntop, u32p(e->w_items), s);
+    HIP_OK(hipGetLastError());
+    out.items = tot[1]; out.chunk = chunk;
+    return RSH_OK;
+}
 
 // One ray batch: trim -> (compress -> sort -> decompress) -> gather ->
 // hierarchy -> fused traversal + intersection.  kind 0: closest hit into
@@ -244,9 +296,15 @@ rsh_status trace_pass(rsh_engine* e, const rsh_frame_config& cfg, const float* r
     p.closest = u64p(e->closest); p.shadow = u32p(e->shadow);
     p.counters = counting ? u64p(e->counters) : nullptr;
     p.force_generic = cfg.traversal_kernel == 1 ? 1u : 0u;
-    // Enough waves to fill the GPU: passes with few top nodes split each top
-    // node's triangle range across waves (at least 64 triangles per wave).
-    {
+    p.splits = 1;
+    if (uses_items(p)) {
+        // cull + work list; its time (host read-backs included) counts as traversal
+        StageTimer t(e, ST_TRAVERSE, timing);
+        rsh_status r = build_items(e, cfg, p, out);
+        if (r != RSH_OK) return r;
+    } else {
+        // Enough waves to fill the GPU: passes with few top nodes split each top
+        // node's triangle range across waves (at least 64 triangles per wave).
         const uint64_t base = (uint64_t)counts[cfg.depth - 1] * p.waves_per_top;
         uint64_t sp = (TRAVERSE_TARGET_WAVES + base - 1) / (base ? base : 1);
         const uint64_t max_sp = e->T / 64 ? e->T / 64 : 1;
@@ -274,6 +332,7 @@ rsh_status record(rsh_engine* e, rsh_frame_stats* st, uint32_t kind, uint32_t S,
     if (st && st->pass_count < RSH_MAX_PASSES) {
         uint32_t i = st->pass_count++;
         st->pass_splits[i] = po.splits;
+        st->pass_items[i] = po.items; st->pass_chunk[i] = po.chunk;
         st->pass_kind[i] = kind; st->pass_slots[i] = S; st->pass_rays[i] = po.N; st->pass_chunks[i] = po.C;
         st->pass_pairs_top[i] = po.top; st->pass_pairs_leaf[i] = po.leaf; st->pass_ray_tests[i] = po.tests;
         st->pass_top_survivors[i] = po.surv;
@@ -319,7 +378,8 @@ void rsh_engine_destroy(rsh_engine* e) {
                       &e->rays_b, &e->flags, &e->keys, &e->values, &e->tkeys, &e->tvalues, &e->bases, &e->sizes,
                       &e->ckeys, &e->cvalues, &e->skeys, &e->svalues, &e->rkeys, &e->rvalues, &e->skeleton, &e->scan,
                       &e->rec0, &e->rec1, &e->nodes, &e->aux, &e->closest, &e->shadow, &e->pixels, &e->counters,
-                      &e->dcounts})
+                      &e->dcounts, &e->w_npass, &e->w_cand, &e->w_nitems, &e->w_off, &e->w_first, &e->w_list,
+                      &e->w_cum, &e->w_items})
         b->release();
     for (int i = 0; i < 4; i++) { e->g0[i].release(); e->gb[i].release(); }
     for (auto& k : e->kept) k.buf.release();

@@ -262,30 +262,112 @@ __global__ void __launch_bounds__(256) k_traverse(TravParams P) {
     }
 }
 
+// ------------------------------------------- top-level object culling ------
+// One wave per top node (depth 2, 64 rays): the object-sphere tests of
+// RayTracer.cu:1152-1280, once per node.  Passing objects are listed in
+// triangle order with the number of candidate triangles before each, so the
+// node's candidates form one stream that is cut into equal chunks (work
+// items) for the traversal kernel.  The count pass sizes the compact lists
+// (npass, cand per node; entry n_top is written as 0 so an exclusive scan
+// over n_top + 1 entries ends with the total), the list pass fills them.
+// RAH (no object culling) streams every triangle and has no list.
+__device__ __forceinline__ uint32_t wave_incl_add(uint32_t x) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    return x;
+}
+
+template <bool LIST>
+__global__ void __launch_bounds__(256) k_cull(TravParams P, uint32_t* npass, uint32_t* cand, uint32_t* obj_list,
+                                              uint32_t* obj_cum) {
+    const uint32_t lane = lane_id();
+    const uint32_t top = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t ntop = P.level_count[1];
+    if (top > ntop || (LIST && top == ntop)) return;
+    if (top == ntop) {
+        if (lane == 0) { npass[top] = 0; cand[top] = 0; }
+        return;
+    }
+    if (P.algorithm == 1) {
+        if (!LIST && lane == 0) { npass[top] = 0; cand[top] = P.T; }
+        return;
+    }
+    const uint32_t gt = P.level_offset[1] + top;
+    const float4 tS = P.nodes[2 * (size_t)gt], tC = P.nodes[2 * (size_t)gt + 1];
+    const float2 tA = P.aux[gt];
+    const size_t base = LIST ? P.obj_off[top] : 0;
+    uint32_t n = 0, cum = 0;
+    for (uint32_t ob = 0; ob < P.K; ob += 64) {
+        const uint32_t oi = ob + lane;
+        bool po = false;
+        uint32_t len = 0;
+        if (oi < P.K) {
+            po = sphere_node(tS, tC, P.obj_sphere[oi], tA.x, tA.y);
+            if (po) { const uint4 r = P.obj_range[oi]; len = r.y - r.x + 1; }
+        }
+        const uint64_t m = __ballot(po);
+        const uint32_t incl = wave_incl_add(len);
+        if (LIST && po) {
+            const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            obj_list[base + n + rank] = oi;
+            obj_cum[base + n + rank] = cum + incl - len;
+        }
+        n += (uint32_t)__popcll(m);
+        cum += __shfl(incl, 63, 64);
+    }
+    if (!LIST && lane == 0) { npass[top] = n; cand[top] = cum; }
+}
+
+// Work items per top node for a chunk size; entry n_top is 0.
+__global__ void __launch_bounds__(256) k_item_count(const uint32_t* cand, uint32_t chunk, uint32_t ntop,
+                                                    uint32_t* nitems) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i > ntop) return;
+    nitems[i] = i == ntop ? 0u : (uint32_t)(((uint64_t)cand[i] + chunk - 1) / chunk);
+}
+
+__global__ void __launch_bounds__(256) k_item_fill(const uint32_t* item_first, const uint32_t* nitems, uint32_t ntop,
+                                                   uint32_t* item_top) {
+    const uint32_t top = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (top >= ntop) return;
+    const uint32_t f = item_first[top], k = nitems[top];
+    for (uint32_t i = lane_id(); i < k; i += 64) item_top[f + i] = top;
+}
+
 // ------------------------------------- fused traversal, 64-ray top nodes ----
-// Specialisation for depth 2 with leaf * branch == 64: one wave = one top
-// node = BRANCH leaves of LEAF rays.  Three stages per block of 64 candidate
-// triangles:
-//  A  lane j tests triangle tb+j against the top node; survivors' bounding
-//     sphere, index and Moller-Trumbore vectors go to the wave's LDS slice in
-//     survivor order (one parallel vector load instead of a dependent load
-//     per survivor);
-//  B  (survivor, leaf) pairs are tested 64 at a time: lane i takes leaf
-//     i % BRANCH (its node is held in registers) and survivor i / BRANCH;
-//  C  each leaf's LEAF lanes (lane i holds ray i of the top node) walk the
-//     survivors that passed their leaf, all leaf groups in parallel.
+// Specialisation for depth 2 with leaf * branch == 64: one wave = one work
+// item = one chunk of one top node's candidate stream (k_cull); the top node
+// covers BRANCH leaves of LEAF rays (lane i holds sorted ray i of the node).
+//  A  The wave walks its chunk 64 candidate triangles at a time with a cursor
+//     over the listed objects; NB blocks' bounding spheres are loaded back to
+//     back and tested against the top node in order.  Survivors (index +
+//     sphere) are appended to an LDS queue.
+//  B  When 64 survivors are queued their Moller-Trumbore vectors are fetched
+//     with one parallel load, then (survivor, leaf) pairs are tested 64 at a
+//     time: lane i takes leaf i % BRANCH (held in registers), survivor
+//     i / BRANCH.  Each lane collects a 64-bit mask of the survivors that
+//     passed its own ray's leaf.
+//  C  All leaf groups walk their masks in parallel; a lane runs the ray test
+//     of its ray against each pending survivor.
+// The set of (leaf, triangle) pairs and ray tests is the reference's; only
+// the order differs, and any-hit / closest-hit are order independent.
 template <int LEAF>
 __global__ void __launch_bounds__(256) k_traverse64(TravParams P) {
-    constexpr int BRANCH = 64 / LEAF;
-    constexpr int GROUP = 64 / BRANCH;          // survivors per stage-B step
-    __shared__ float4 s_sph[4][64];
+    constexpr uint32_t BRANCH = 64 / LEAF;
+    constexpr uint32_t GROUP = 64 / BRANCH;     // survivors per stage-B step
+    constexpr uint32_t QCAP = 128;
+    __shared__ float4 s_qsph[4][QCAP];
+    __shared__ uint32_t s_qidx[4][QCAP];
     __shared__ float4 s_mt[4][64][3];
-    __shared__ uint32_t s_idx[4][64];
     const uint32_t lane = lane_id();
     const uint32_t w = threadIdx.x >> 6;
-    const uint32_t wave = blockIdx.x * 4 + w;
-    const uint32_t top = wave / P.splits;
-    const uint32_t split = wave - top * P.splits;
+    const uint32_t item = blockIdx.x * 4 + w;
+    if (item >= P.items) return;
+    const uint32_t top = P.item_top[item];
     if (top >= P.level_count[1]) return;
     const uint32_t pos = top * 64 + lane;
     const bool valid = pos < P.N;
@@ -303,8 +385,7 @@ __global__ void __launch_bounds__(256) k_traverse64(TravParams P) {
     const uint32_t gt = P.level_offset[1] + top;
     const float4 tS = P.nodes[2 * (size_t)gt], tC = P.nodes[2 * (size_t)gt + 1];
     const float2 tA = P.aux[gt];
-    // stage-B leaf of this lane: leaf (top * BRANCH + lane % BRANCH)
-    const uint32_t bl = top * BRANCH + (lane % BRANCH);
+    const uint32_t bl = top * BRANCH + (lane % BRANCH);     // stage-B leaf of this lane
     const bool bl_valid = bl < P.level_count[0];
     float4 lS = make_float4(0, 0, 0, 0), lC = make_float4(0, 0, 1, 0);
     float2 lA = make_float2(1, 0);
@@ -313,9 +394,7 @@ __global__ void __launch_bounds__(256) k_traverse64(TravParams P) {
         lC = P.nodes[2 * (size_t)bl + 1];
         lA = P.aux[bl];
     }
-    const uint32_t my_leaf = lane / LEAF;       // stage-C leaf of this lane's ray
-    const bool leaf_head = valid && (lane % LEAF) == 0;
-    (void)leaf_head;
+    const uint32_t my_leaf = lane / LEAF;                   // stage-C leaf of this lane's ray
 
     unsigned long long best = P.hit_mode == 0 ? MISS64 : (unsigned long long)MISS32;
     bool occluded = false;
@@ -323,102 +402,155 @@ __global__ void __launch_bounds__(256) k_traverse64(TravParams P) {
     const bool count = P.counters != nullptr;
     uint64_t n_top = 0, n_surv = 0, n_leaf = 0, n_tests = 0;
     bool done = false;
+    uint32_t qlen = 0;
 
-    auto process_range = [&](uint32_t first, uint32_t last, uint32_t exlo, uint32_t exhi) {
-        for (uint32_t tb = first; tb <= last && !done; tb += 64) {
-            // ---- stage A
-            const uint32_t t = tb + lane;
-            const bool inr = t <= last && !(t >= exlo && t < exhi);
-            float4 ts = make_float4(0, 0, 0, 0);
-            bool p = false;
-            if (inr) {
-                ts = P.tri_sphere[t];
-                p = sphere_node(tS, tC, ts, tA.x, tA.y);
-            }
-            const uint64_t m = __ballot(p);
-            if (count) { n_top += (uint64_t)__popcll(__ballot(inr)); n_surv += (uint64_t)__popcll(m); }
-            if (m == 0ull) continue;
-            const uint32_t k = (uint32_t)__popcll(m);
-            if (p) {
-                const uint32_t r = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-                s_sph[w][r] = ts;
-                s_idx[w][r] = t;
-                s_mt[w][r][0] = P.tri_mt[3 * (size_t)t];
-                s_mt[w][r][1] = P.tri_mt[3 * (size_t)t + 1];
-                s_mt[w][r][2] = P.tri_mt[3 * (size_t)t + 2];
+    // ---- stages B + C over the queued survivors
+    auto flush = [&]() {
+        for (uint32_t c0 = 0; c0 < qlen && !done; c0 += 64) {
+            const uint32_t n = qlen - c0 < 64 ? qlen - c0 : 64;
+            if (lane < n) {
+                size_t t = s_qidx[w][c0 + lane];
+                if (t >= P.T) t = 0;     // queue entries are < T by construction
+                s_mt[w][lane][0] = P.tri_mt[3 * t];
+                s_mt[w][lane][1] = P.tri_mt[3 * t + 1];
+                s_mt[w][lane][2] = P.tri_mt[3 * t + 2];
             }
             __builtin_amdgcn_wave_barrier();
-            // ---- stages B and C, GROUP survivors at a time
-            for (uint32_t g = 0; g < k && !done; g += GROUP) {
+            uint64_t mine = 0;   // survivors (within this chunk) that passed my ray's leaf
+            for (uint32_t g = 0; g < n; g += GROUP) {
                 const uint32_t s = g + lane / BRANCH;
                 bool lp = false;
-                if (s < k && bl_valid) lp = sphere_node(lS, lC, s_sph[w][s], lA.x, lA.y);
+                if (s < n && bl_valid) lp = sphere_node(lS, lC, s_qsph[w][c0 + s], lA.x, lA.y);
                 const uint64_t pm = __ballot(lp);
                 if (count) n_leaf += (uint64_t)__popcll(pm);
-                if (pm == 0ull) continue;
-                // pending survivors of this lane's leaf: bit j <=> pair (g + j, my_leaf)
-                uint32_t q = 0;
 #pragma unroll
-                for (int j = 0; j < GROUP; j++) q |= (uint32_t)((pm >> (j * BRANCH + my_leaf)) & 1ull) << j;
-                if (!valid || occluded) q = 0;
-                while (__ballot(q != 0u)) {
-                    if (q) {
-                        const uint32_t j = (uint32_t)__builtin_ctz(q);
-                        q &= q - 1u;
-                        const uint32_t r = g + j;
-                        const float4 A = s_mt[w][r][0], B = s_mt[w][r][1], Cc = s_mt[w][r][2];
-                        const float th = ray_triangle(oe, d, xyz(A), xyz(B), xyz(Cc));
-                        if (count) n_tests++;
-                        if (P.kind == 1) {
-                            if (th > EPS && th < tmax) { occluded = true; q = 0; }
-                        } else if (th > EPS) {
-                            const unsigned long long rec = pack_hit(th, s_idx[w][r], P.hit_mode);
-                            best = rec < best ? rec : best;
-                        }
+                for (uint32_t j = 0; j < GROUP; j++)
+                    mine |= ((pm >> (j * BRANCH + my_leaf)) & 1ull) << (g + j);
+            }
+            if (!valid || occluded) mine = 0;
+            while (__ballot(mine != 0ull)) {
+                if (mine) {
+                    const uint32_t s = (uint32_t)__builtin_ctzll(mine);
+                    mine &= mine - 1ull;
+                    const float4 A = s_mt[w][s][0], B = s_mt[w][s][1], Cc = s_mt[w][s][2];
+                    const float th = ray_triangle(oe, d, xyz(A), xyz(B), xyz(Cc));
+                    if (count) n_tests++;
+                    if (P.kind == 1) {
+                        if (th > EPS && th < tmax) { occluded = true; mine = 0; }
+                    } else if (th > EPS) {
+                        const unsigned long long rec = pack_hit(th, s_qidx[w][c0 + s], P.hit_mode);
+                        best = rec < best ? rec : best;
                     }
                 }
-                if (P.kind == 1 && (__ballot(occluded) & alive) == alive) done = true;
             }
+            if (P.kind == 1 && (__ballot(occluded) & alive) == alive) done = true;
             __builtin_amdgcn_wave_barrier();
         }
+        qlen = 0;
     };
 
-    // This wave's share of the triangle range (a top node may be split across
-    // waves when the pass has few top nodes).
-    const uint32_t lo = (uint32_t)(((uint64_t)split * P.T) / P.splits);
-    const uint32_t hi = (uint32_t)(((uint64_t)(split + 1) * P.T) / P.splits);
-    if (hi > lo) {
-        if (P.algorithm == 1) {
-            process_range(lo, hi - 1, 0, 0);
+    // ---- cursor over this item's chunk [s0, s1) of the node's candidate stream
+    const bool rah = P.algorithm == 1;
+    const uint32_t chunk_ix = item - P.item_first[top];
+    const uint32_t ncand = P.cand[top];
+    const uint32_t s0 = chunk_ix * P.chunk;
+    const uint32_t s1 = ncand - s0 < P.chunk ? ncand : s0 + P.chunk;
+    const size_t lbase = rah ? 0 : P.obj_off[top];
+    const uint32_t np = rah ? 1u : P.obj_off[top + 1] - P.obj_off[top];
+    uint32_t rem = s1 > s0 ? s1 - s0 : 0;     // candidates not yet handed to a block
+    uint32_t j = 0;                            // current entry of the object list
+    uint32_t c_tb = 0, c_last = 0, c_exlo = 0, c_exhi = 0;
+    bool c_valid = false;
+    // Take the next `rem`-bounded portion of list entry jj, starting `off` candidates in.
+    auto load_entry = [&](uint32_t jj, uint32_t off) {
+        uint32_t first, last;
+        if (rah) { first = 0; last = P.T - 1; c_exlo = c_exhi = 0; }
+        else {
+            const uint4 r = P.obj_range[P.obj_list[lbase + jj]];
+            first = r.x; last = r.y; c_exlo = r.z; c_exhi = r.w;
+        }
+        c_tb = first + off;
+        const uint32_t avail = last - c_tb + 1;
+        const uint32_t take = avail < rem ? avail : rem;
+        c_last = c_tb + take - 1;
+        rem -= take;
+        c_valid = true;
+    };
+    if (rem) {
+        if (rah) {
+            load_entry(0, s0);
         } else {
-            const uint32_t k0 = object_of(P.obj_range, P.K, lo);
-            const uint32_t k1 = object_of(P.obj_range, P.K, hi - 1);
-            for (uint32_t ob = k0; ob <= k1 && !done; ob += 64) {
-                const uint32_t oi = ob + lane;
-                bool po = false;
-                if (oi <= k1) po = sphere_node(tS, tC, P.obj_sphere[oi], tA.x, tA.y);
-                uint64_t m = __ballot(po);
-                while (m && !done) {
-                    const uint32_t ok = __builtin_amdgcn_readfirstlane(ob + (uint32_t)__builtin_ctzll(m));
-                    m &= m - 1;
-                    const uint4 r = P.obj_range[ok];
-                    process_range(r.x > lo ? r.x : lo, r.y < hi - 1 ? r.y : hi - 1, r.z, r.w);
+            // last list entry whose stream offset is <= s0 (the offsets increase)
+            uint32_t j0 = 0;
+            for (uint32_t jb = 0; jb < np; jb += 64) {
+                const uint32_t jj = jb + lane;
+                const bool le = jj < np && P.obj_cum[lbase + jj] <= s0;
+                const uint64_t bm = __ballot(le);
+                if (bm) j0 = jb + 63u - (uint32_t)__builtin_clzll(bm);
+                if (bm != ~0ull) break;
+            }
+            j = __builtin_amdgcn_readfirstlane(j0);
+            load_entry(j, s0 - P.obj_cum[lbase + j]);
+        }
+    }
+    auto advance = [&]() {
+        if (c_valid && c_tb + 64 <= c_last) { c_tb += 64; return; }
+        if (!c_valid || rah || rem == 0 || j + 1 >= np) { c_valid = false; return; }
+        j++;
+        load_entry(j, 0);
+    };
+    auto fetch = [&](float4& ts, bool& inr) {
+        const uint32_t t = c_tb + lane;
+        inr = c_valid && t <= c_last && !(t >= c_exlo && t < c_exhi);
+        ts = make_float4(0, 0, 0, 0);
+        if (inr) ts = P.tri_sphere[t];
+    };
+
+    // NB blocks per iteration: their sphere loads are issued back to back and
+    // consumed in order, so only the first load's latency is exposed.
+    constexpr int NB = 4;
+    while (c_valid && !done) {    // the cursor starts on the item's first block
+        uint32_t tbk[NB];
+        bool ink[NB];
+        float4 tsk[NB];
+#pragma unroll
+        for (int k = 0; k < NB; k++) {
+            tbk[k] = c_tb;
+            fetch(tsk[k], ink[k]);
+            advance();
+        }
+#pragma unroll
+        for (int k = 0; k < NB; k++) {
+            if (done) break;             // every ray of the wave is occluded: stop queueing
+            // ---- stage A
+            bool p = false;
+            if (ink[k]) p = sphere_node(tS, tC, tsk[k], tA.x, tA.y);
+            const uint64_t m = __ballot(p);
+            if (count) { n_top += (uint64_t)__popcll(__ballot(ink[k])); n_surv += (uint64_t)__popcll(m); }
+            if (m) {
+                const uint32_t add = (uint32_t)__popcll(m);
+                if (qlen + add > QCAP) flush();   // never taken: qlen < 64 here
+                if (done) break;
+                if (p) {
+                    const uint32_t r = qlen + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                    s_qsph[w][r] = tsk[k];
+                    s_qidx[w][r] = tbk[k] + lane;
                 }
+                qlen += add;
+                __builtin_amdgcn_wave_barrier();
+                if (qlen >= 64) flush();
             }
         }
     }
+    if (qlen && !done) flush();
 
+    // Results start as misses; a node cut into several items merges by atomicMin.
     if (valid) {
-        if (P.splits == 1) {
-            if (P.kind == 1) { if (occluded) P.shadow[slot] = MISS32; }
-            else P.closest[slot] = best;
-        } else {
-            if (P.kind == 1) { if (occluded) P.shadow[slot] = MISS32; }
-            else if (best != (P.hit_mode == 0 ? MISS64 : (unsigned long long)MISS32)) atomicMin(&P.closest[slot], best);
-        }
+        if (P.kind == 1) { if (occluded) P.shadow[slot] = MISS32; }
+        else if (ncand <= P.chunk) P.closest[slot] = best;
+        else if (best != (P.hit_mode == 0 ? MISS64 : (unsigned long long)MISS32)) atomicMin(&P.closest[slot], best);
     }
     if (count) {
-        // per-lane test counts -> wave sum
         unsigned long long tsum = n_tests;
         for (int dlt = 32; dlt > 0; dlt >>= 1) tsum += __shfl_xor(tsum, dlt, 64);
         if (lane == 0) {
@@ -570,23 +702,47 @@ void launch_hierarchy(const float4* rec0, const float4* rec1, const float* rays,
                                                            counts[k], branch, mode);
 }
 
+void launch_cull_count(const TravParams& p, uint32_t* npass, uint32_t* cand, hipStream_t s) {
+    const uint32_t top = p.level_count[1];
+    k_cull<false><<<top / 4 + 1, 256, 0, s>>>(p, npass, cand, nullptr, nullptr);
+}
+
+void launch_cull_list(const TravParams& p, uint32_t* obj_list, uint32_t* obj_cum, hipStream_t s) {
+    const uint32_t top = p.level_count[1];
+    if (top && p.algorithm != 1) k_cull<true><<<(top + 3) / 4, 256, 0, s>>>(p, nullptr, nullptr, obj_list, obj_cum);
+}
+
+void launch_item_count(const uint32_t* cand, uint32_t chunk, uint32_t ntop, uint32_t* nitems, hipStream_t s) {
+    k_item_count<<<ntop / 256 + 1, 256, 0, s>>>(cand, chunk, ntop, nitems);
+}
+
+void launch_item_fill(const uint32_t* item_first, const uint32_t* nitems, uint32_t ntop, uint32_t* item_top,
+                      hipStream_t s) {
+    if (ntop) k_item_fill<<<(ntop + 3) / 4, 256, 0, s>>>(item_first, nitems, ntop, item_top);
+}
+
 hipError_t launch_traverse(const TravParams& p, hipStream_t s) {
     const uint32_t top = p.level_count[p.depth - 1];
-    if (p.splits == 0) return hipErrorInvalidValue;
-    const uint64_t waves = (uint64_t)top * p.waves_per_top * p.splits;
-    if (waves == 0) return hipSuccess;
-    if (waves > 0xFFFFFFFFull) return hipErrorInvalidValue;
-    const uint32_t blocks = (uint32_t)((waves + 3) / 4);
-    if (p.depth == 2 && p.leaf * p.branch == 64 && !p.force_generic) {
+    if (uses_items(p)) {
+        if (p.items == 0) return hipSuccess;
+        if (p.chunk == 0 || !p.item_top || !p.item_first || !p.cand) return hipErrorInvalidValue;
+        if (p.algorithm != 1 && (!p.obj_off || !p.obj_list || !p.obj_cum)) return hipErrorInvalidValue;
+        if (p.items > MAX_TRAVERSE_ITEMS) return hipErrorInvalidValue;
+        const uint32_t blocks = (p.items + 3) / 4;
         switch (p.leaf) {
             case 2: k_traverse64<2><<<blocks, 256, 0, s>>>(p); return hipGetLastError();
             case 4: k_traverse64<4><<<blocks, 256, 0, s>>>(p); return hipGetLastError();
             case 8: k_traverse64<8><<<blocks, 256, 0, s>>>(p); return hipGetLastError();
             case 16: k_traverse64<16><<<blocks, 256, 0, s>>>(p); return hipGetLastError();
             case 32: k_traverse64<32><<<blocks, 256, 0, s>>>(p); return hipGetLastError();
-            default: break;
+            default: return hipErrorInvalidValue;
         }
     }
+    if (p.splits == 0) return hipErrorInvalidValue;
+    const uint64_t waves = (uint64_t)top * p.waves_per_top * p.splits;
+    if (waves == 0) return hipSuccess;
+    if (waves > 0xFFFFFFFFull) return hipErrorInvalidValue;
+    const uint32_t blocks = (uint32_t)((waves + 3) / 4);
     switch (p.depth) {
         case 1: k_traverse<1><<<blocks, 256, 0, s>>>(p); break;
         case 2: k_traverse<2><<<blocks, 256, 0, s>>>(p); break;

@@ -50,8 +50,40 @@ struct TravParams {
     uint32_t* shadow;             // [S] per slot
     unsigned long long* counters; // optional [4]: top pairs, leaf pairs, ray tests, top survivors
     uint32_t force_generic;       // 1: use the generic kernel even for 64-ray top nodes
-    uint32_t splits;              // waves sharing one top node's triangle range (>= 1)
+    uint32_t splits;              // generic kernel: waves sharing one top node's triangle range
+    // work items of the 64-ray kernel: (top node, chunk of candidate triangles)
+    const uint32_t* item_top;     // [items] top node of each item
+    const uint32_t* item_first;   // [n_top + 1] first item of each top node
+    const uint32_t* cand;         // [n_top + 1] candidate triangles of each top node
+    const uint32_t* obj_off;      // [n_top + 1] first list entry of each top node (CRSH)
+    const uint32_t* obj_list;     // passing objects of each node, in triangle order
+    const uint32_t* obj_cum;      // candidates of the node before each listed object
+    uint32_t chunk;               // candidates per item
+    uint32_t items;
 };
+
+// Work items of one launch of the 64-ray kernel: 64 lanes each, so the grid
+// stays below 2^31 threads.
+constexpr uint32_t MAX_TRAVERSE_ITEMS = 1u << 24;
+
+// The 64-ray kernel (depth 2, leaf * branch == 64) consumes work items; every
+// other shape runs the generic kernel with `splits` waves per top node.
+// Work list of a pass with n_top top nodes:
+//   launch_cull_count  -> npass, cand            [n_top + 1], entry n_top = 0
+//   exclusive scans    -> obj_off (of npass), item_first (of nitems)
+//   launch_item_count  -> nitems = ceil(cand / chunk)
+//   launch_cull_list   -> obj_list, obj_cum at obj_off
+//   launch_item_fill   -> item_top
+inline bool uses_items(const TravParams& p) {
+    return p.depth == 2 && p.leaf * p.branch == 64 && !p.force_generic &&
+           (p.leaf == 2 || p.leaf == 4 || p.leaf == 8 || p.leaf == 16 || p.leaf == 32);
+}
+
+void launch_cull_count(const TravParams& p, uint32_t* npass, uint32_t* cand, hipStream_t s);
+void launch_cull_list(const TravParams& p, uint32_t* obj_list, uint32_t* obj_cum, hipStream_t s);
+void launch_item_count(const uint32_t* cand, uint32_t chunk, uint32_t ntop, uint32_t* nitems, hipStream_t s);
+void launch_item_fill(const uint32_t* item_first, const uint32_t* nitems, uint32_t ntop, uint32_t* item_top,
+                      hipStream_t s);
 
 // --- rays ---
 void launch_create_primary(uint32_t W, uint32_t H, v3 eye, v3 D, v3 R, v3 U, KeyCfg kc, float* rays, uint32_t* flags,

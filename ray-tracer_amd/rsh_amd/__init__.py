@@ -42,7 +42,7 @@ class FrameConfigDesc(C.Structure):
                 ("bounce_shadows", u32), ("primary_rays", u32), ("cell_bits", u32), ("dir_bits", u32),
                 ("key_layout", u32), ("scene_min", f32 * 3), ("cell_scale", f32 * 3),
                 ("segment_shift", u32), ("initial_radius", f32), ("initial_spread", f32),
-                ("traversal_kernel", u32), ("traversal_splits", u32)]
+                ("traversal_kernel", u32), ("traversal_splits", u32), ("traversal_chunk", u32)]
 
 
 class FrameStats(C.Structure):
@@ -51,7 +51,8 @@ class FrameStats(C.Structure):
                 ("pass_pairs_top", u64 * MAX_PASSES), ("pass_pairs_leaf", u64 * MAX_PASSES),
                 ("pass_ray_tests", u64 * MAX_PASSES), ("stage_ms", f32 * 8), ("rays_total", u32),
                 ("pass_top_survivors", u64 * MAX_PASSES), ("pass_traverse_ms", f32 * MAX_PASSES),
-                ("pass_splits", u32 * MAX_PASSES)]
+                ("pass_splits", u32 * MAX_PASSES), ("pass_items", u32 * MAX_PASSES),
+                ("pass_chunk", u32 * MAX_PASSES)]
 
 
 class Float3(C.Structure):
@@ -308,7 +309,8 @@ class Engine:
         return [dict(kind=st.pass_kind[i], slots=st.pass_slots[i], rays=st.pass_rays[i], chunks=st.pass_chunks[i],
                      pairs_top=st.pass_pairs_top[i], pairs_leaf=st.pass_pairs_leaf[i],
                      ray_tests=st.pass_ray_tests[i], top_survivors=st.pass_top_survivors[i],
-                     traverse_ms=st.pass_traverse_ms[i], splits=st.pass_splits[i])
+                     traverse_ms=st.pass_traverse_ms[i], splits=st.pass_splits[i], items=st.pass_items[i],
+                     chunk=st.pass_chunk[i])
                 for i in range(st.pass_count)]
 
     def close(self):

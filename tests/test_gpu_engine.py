@@ -58,8 +58,12 @@ def test_config0_default(engine, oracle, rsh, w):
     dict(traversal_kernel=1),                # generic kernel on the 64-ray shape
     dict(traversal_kernel=1, algorithm=1),
     dict(algorithm=1, bounces=2, bounce_shadows=1),
-    dict(traversal_splits=3),                # top nodes split across waves, merged by atomicMin
-    dict(traversal_splits=7, algorithm=1),
+    dict(traversal_chunk=64),                # many work items per top node, merged by atomicMin
+    dict(traversal_chunk=100),               # items that start and end inside 64-triangle blocks
+    dict(traversal_chunk=100, algorithm=1),
+    dict(traversal_chunk=64, bounces=2, bounce_shadows=1),
+    dict(traversal_chunk=1 << 30),           # one item per top node
+    dict(traversal_splits=3, traversal_kernel=1),
     dict(traversal_splits=5, traversal_kernel=1, bounces=2, bounce_shadows=1),
     dict(traversal_splits=4, depth=3, branch=4),
     dict(key_layout=1),
@@ -83,6 +87,19 @@ def test_config2_small(engine, oracle, rsh):
                                      arch_along=8)
     cam = rsh.scenes.config2_camera(160, 90)
     run_case(engine, oracle, rsh, scene, cam, rsh.config.config2(160, 90))
+
+
+@pytest.fixture(scope="module")
+def atrium(rsh):
+    return rsh.scenes.config2_scene()
+
+
+def test_config2_full_scene_low_res(engine, oracle, rsh, atrium):
+    """The full 262k-triangle atrium: many top nodes whose 64 shadow rays are
+    all occluded part-way through a batch of candidate blocks (regression:
+    the survivor queue must not grow past its capacity after the wave exits)."""
+    cam = rsh.scenes.config2_camera(192, 108)
+    run_case(engine, oracle, rsh, atrium, cam, rsh.config.config2(192, 108))
 
 
 def test_render_twice_is_deterministic(engine, rsh):

@@ -8,11 +8,21 @@ octahedral hashing, trimming, compression into packets, radix sort,
 decompression, hierarchy build, fused traversal + intersection, shading).
 Scene and camera are resident in HBM before timing starts.
 
-Prints one JSON line (see the "roofline" and "cpu_baseline" fields).
+Prints one JSON line.  "roofline" describes the dominant kernel, the fused
+traversal (k_traverse64, one launch per pass): its launch durations come from
+HIP events recorded on the engine's stream around every launch of the timed
+frames; its algorithmic FLOPs from one extra frame with the traversal
+counters on (pair and test counts are deterministic); its HBM traffic from the
+rocprofv3 FETCH_SIZE / WRITE_SIZE summary committed under profiles/.
+"cpu_baseline" is the C++ oracle on a band of rows of the same frame.
+
+--copy-probe runs a plain device copy of known size (the calibration kernel
+for the FETCH_SIZE / WRITE_SIZE counters) and exits.
 """
 import argparse
 import json
 import os
+import re
 import statistics
 import sys
 import time
@@ -22,12 +32,11 @@ for _p in (ROOT, os.path.join(ROOT, "ray-tracer_amd")):
     if _p not in sys.path:
         sys.path.insert(0, _p)
 
-import numpy as np  # noqa: E402
-
 PEAK_HBM_GBS = 8000.0
 PEAK_FP32_TFLOPS = 157.3
 FLOP_NODE_TEST = 25      # SphereNodeIntersection, RayTracer.cu:283-298
 FLOP_RAY_TEST = 46       # RayTriangleIntersection full path, RayTracer.cu:257-280
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_traverse64_atrium1080p_traffic.txt")
 
 
 def parse():
@@ -39,8 +48,33 @@ def parse():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the CPU sample (0: automatic)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="copy MIB MiB device-to-device 3 times with a plain copy kernel and exit")
     ap.add_argument("--out", default="")
     return ap.parse_args()
+
+
+def copy_probe(mib):
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.int32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    assert bool((dst[-4:] == src[-4:]).all())
+    print(json.dumps({"copy_probe_bytes": n * 4, "launches": 3}))
+
+
+def traffic_per_launch():
+    """HBM bytes per launch of the traversal kernel on this workload, from the
+    committed rocprofv3 summary (None when it is absent)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            m = re.search(r"^traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", f.read(), re.M)
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
 
 
 def cpu_baseline(scene, cam, cfg, rows):
@@ -68,6 +102,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only",
                           "n_gpus": a.gpus}))
         sys.exit(2)
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
     import rsh_amd
     from rsh_amd import config as cf, scenes
 
@@ -80,30 +117,41 @@ def main():
         eng.render(cam, cfg)
     rsh_amd.lib().rsh_device_synchronize()
     times, rays = [], []
+    stage_sum = [0.0] * len(rsh_amd.STAGES)
+    trav_ms, trav_launches = 0.0, 0
     t_all = time.perf_counter()
     for _ in range(a.steps):
         t0 = time.perf_counter()
-        st = eng.render(cam, cfg)
+        st = eng.render(cam, cfg, flags=eng.TIMING)
         times.append(time.perf_counter() - t0)
         rays.append(st.rays_total)
+        for i in range(len(stage_sum)):
+            stage_sum[i] += st.stage_ms[i]
+        for i in range(st.pass_count):
+            if st.pass_rays[i]:
+                trav_ms += st.pass_traverse_ms[i]
+                trav_launches += 1
     rsh_amd.lib().rsh_device_synchronize()
     total = time.perf_counter() - t_all
     value = sum(rays) / total / 1e6
+    stages = {name: stage_sum[i] / a.steps for i, name in enumerate(rsh_amd.STAGES)}
 
-    # Kernel-level figures from one instrumented frame (HIP events on the
-    # engine's stream around each stage; counters of the fused traversal).
-    st = eng.render(cam, cfg, flags=eng.COUNTERS | eng.TIMING)
-    stages = {name: st.stage_ms[i] for i, name in enumerate(rsh_amd.STAGES)}
+    # Algorithmic work of the traversal: one frame with the counters on.
+    eng.render(cam, cfg, flags=eng.COUNTERS)
     passes = eng.passes()
     n_tests = sum(p["ray_tests"] for p in passes)
     n_top = sum(p["pairs_top"] for p in passes)
-    n_leaf = sum(p["pairs_leaf"] for p in passes)
-    flops = FLOP_RAY_TEST * n_tests + FLOP_NODE_TEST * (n_top + 8 * n_leaf)
-    trav_ms = stages["traversal"]
-    achieved = flops / (trav_ms * 1e-3) / 1e12 if trav_ms > 0 else 0.0
-    roof = {"bound": "valu", "kernel": "k_traverse (fused traversal + intersection)",
+    n_surv = sum(p["top_survivors"] for p in passes)
+    launches_per_frame = sum(1 for p in passes if p["rays"])
+    flops_frame = FLOP_RAY_TEST * n_tests + FLOP_NODE_TEST * (n_top + cfg.branch * n_surv)
+    avg_launch_ms = trav_ms / trav_launches if trav_launches else 0.0
+    flops_launch = flops_frame / launches_per_frame if launches_per_frame else 0.0
+    achieved = flops_launch / (avg_launch_ms * 1e-3) / 1e12 if avg_launch_ms > 0 else 0.0
+    roof = {"bound": "valu", "kernel": "k_traverse64<8> (fused traversal + intersection, one launch per pass)",
             "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None}
+            "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic_per_launch(),
+            "avg_launch_ms": avg_launch_ms, "launches_per_frame": launches_per_frame,
+            "flops_per_launch": flops_launch}
     # hash + sort + compress bandwidth (algorithmic bytes / stage time)
     hsc_bytes = 0
     for p in passes:

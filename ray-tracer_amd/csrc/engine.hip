@@ -136,7 +136,7 @@ rsh_status ensure_frame(rsh_engine* e, uint32_t W, uint32_t H, uint32_t Lmax) {
     HIP_OK(e->rec0.reserve(S * 16)); HIP_OK(e->rec1.reserve(S * 16));
     HIP_OK(e->pixels.reserve(P * 4));
     for (int i = 0; i < 4; i++) { HIP_OK(e->g0[i].reserve(P * 16)); HIP_OK(e->gb[i].reserve(P * 16)); }
-    HIP_OK(e->counters.reserve(64)); HIP_OK(e->dcounts.reserve(64));
+    HIP_OK(e->counters.reserve(COUNTER_SLOTS * 4 * 8)); HIP_OK(e->dcounts.reserve(64));
     HIP_OK(e->sc.reserve(S));
     e->cap_slots = S;
     return RSH_OK;
@@ -314,16 +314,18 @@ rsh_status trace_pass(rsh_engine* e, const rsh_frame_config& cfg, const float* r
         p.splits = (uint32_t)(sp ? sp : 1);
         out.splits = p.splits;
     }
-    if (counting) launch_fill_u64(u64p(e->counters), 0ull, 4, s);
+    if (counting) launch_fill_u64(u64p(e->counters), 0ull, COUNTER_SLOTS * 4, s);
     {
         StageTimer t(e, ST_PASS_TRAVERSE + (int)e->pass_index, timing);
         HIP_OK(launch_traverse(p, s));
     }
     if (counting) {
-        unsigned long long c[4];
+        unsigned long long c[COUNTER_SLOTS * 4];
         HIP_OK(hipMemcpyAsync(c, e->counters.p, sizeof(c), hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
-        out.top = c[0]; out.leaf = c[1]; out.tests = c[2]; out.surv = c[3];
+        for (uint32_t k = 0; k < COUNTER_SLOTS; k++) {
+            out.top += c[4 * k]; out.leaf += c[4 * k + 1]; out.tests += c[4 * k + 2]; out.surv += c[4 * k + 3];
+        }
     }
     return RSH_OK;
 }

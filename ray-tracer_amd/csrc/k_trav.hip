@@ -10,14 +10,15 @@
 // The native path fuses traversal and intersection: one wave64 owns the 64
 // sorted rays under one top-level node (8 leaves of 8 rays for the reference
 // fan-out), keeps each lane's ray and best hit in registers, and streams the
-// node's candidate triangles -- object spheres tested 64 at a time, then each
-// surviving object's triangles 64 at a time against the top node, then every
-// survivor against the lane's own chain of lower nodes before the
-// Moller-Trumbore test.  Triangle data for the per-survivor step is
-// wave-uniform and read with scalar loads.  Nothing is materialised: no flag
-// arrays, no scans, no host round trips, no atomics on results.  The
-// candidate set per (leaf, triangle) is the reference's; only the order of
-// tests differs, and both any-hit and closest-hit are order independent.
+// node's candidate triangles -- the triangles of the objects whose spheres
+// pass the node -- 64 at a time against the top node, then every survivor
+// against the lane's own chain of lower nodes before the Moller-Trumbore
+// test.  No (node, triangle) pair is materialised.  A top node's candidate
+// stream is cut into equal chunks (work items, one wave each) so that nodes
+// with a wide cone over large objects do not serialise the pass; waves of
+// the same node merge closest hits with a 64-bit atomicMin.  The candidate
+// set per (leaf, triangle) is the reference's; only the order of tests
+// differs, and both any-hit and closest-hit are order independent.
 #include "rsh_internal.hpp"
 #include "rsh_kernels.hpp"
 
@@ -254,10 +255,11 @@ __global__ void __launch_bounds__(256) k_traverse(TravParams P) {
     }
     if (count) {
         if (lane == 0) {
-            atomicAdd(&P.counters[0], (unsigned long long)n_top);
-            atomicAdd(&P.counters[1], (unsigned long long)n_leaf);
-            atomicAdd(&P.counters[2], (unsigned long long)n_tests);
-            atomicAdd(&P.counters[3], (unsigned long long)n_surv);
+            unsigned long long* cs = P.counters + (size_t)(wave % COUNTER_SLOTS) * 4;
+            atomicAdd(&cs[0], (unsigned long long)n_top);
+            atomicAdd(&cs[1], (unsigned long long)n_leaf);
+            atomicAdd(&cs[2], (unsigned long long)n_tests);
+            atomicAdd(&cs[3], (unsigned long long)n_surv);
         }
     }
 }
@@ -554,10 +556,11 @@ __global__ void __launch_bounds__(256) k_traverse64(TravParams P) {
         unsigned long long tsum = n_tests;
         for (int dlt = 32; dlt > 0; dlt >>= 1) tsum += __shfl_xor(tsum, dlt, 64);
         if (lane == 0) {
-            atomicAdd(&P.counters[0], (unsigned long long)n_top);
-            atomicAdd(&P.counters[1], (unsigned long long)n_leaf);
-            atomicAdd(&P.counters[2], tsum);
-            atomicAdd(&P.counters[3], (unsigned long long)n_surv);
+            unsigned long long* cs = P.counters + (size_t)(item % COUNTER_SLOTS) * 4;
+            atomicAdd(&cs[0], (unsigned long long)n_top);
+            atomicAdd(&cs[1], (unsigned long long)n_leaf);
+            atomicAdd(&cs[2], tsum);
+            atomicAdd(&cs[3], (unsigned long long)n_surv);
         }
     }
 }

@@ -27,6 +27,10 @@ struct GBufView {
     float4* spec;
 };
 
+// Diagnostic traversal counters are spread over this many slots (summed on
+// the host) so that the per-wave atomics do not serialise on one address.
+constexpr uint32_t COUNTER_SLOTS = 64;
+
 struct TravParams {
     const float4* nodes;          // hierarchy, sphere at 2g, cone at 2g+1
     const float2* aux;            // (cos, tan) of each node's spread
@@ -48,7 +52,8 @@ struct TravParams {
     uint32_t hit_mode;            // 0 exact, 1 reference packing
     unsigned long long* closest;  // [S] per slot
     uint32_t* shadow;             // [S] per slot
-    unsigned long long* counters; // optional [4]: top pairs, leaf pairs, ray tests, top survivors
+    unsigned long long* counters; // optional [COUNTER_SLOTS][4]: top pairs, leaf pairs, ray tests,
+                                  // top survivors; waves add into slot wave % COUNTER_SLOTS
     uint32_t force_generic;       // 1: use the generic kernel even for 64-ray top nodes
     uint32_t splits;              // generic kernel: waves sharing one top node's triangle range
     // work items of the 64-ray kernel: (top node, chunk of candidate triangles)

@@ -87,8 +87,9 @@ typedef struct rsh_frame_config {
     float initial_radius, initial_spread;
     uint32_t traversal_kernel;      /* 0 automatic, 1 generic kernel for every shape     */
     uint32_t traversal_splits;      /* generic kernel: waves per top node; 0 = fill the GPU */
-    uint32_t traversal_chunk;       /* 64-ray kernel: candidate triangles per work item;
-                                       0 = sized per pass                                */
+    uint32_t traversal_chunk;       /* 64-ray kernel: candidate clusters (64 consecutive
+                                       triangles of one object) per work item; 0 = sized
+                                       per pass                                           */
 } rsh_frame_config;
 
 #define RSH_MAX_PASSES 16
@@ -109,7 +110,11 @@ typedef struct rsh_frame_stats {
     float pass_traverse_ms[RSH_MAX_PASSES];      /* traversal kernel time per pass (bit 2)   */
     uint32_t pass_splits[RSH_MAX_PASSES];        /* generic kernel: waves per top node       */
     uint32_t pass_items[RSH_MAX_PASSES];         /* 64-ray kernel: work items launched       */
-    uint32_t pass_chunk[RSH_MAX_PASSES];         /* 64-ray kernel: candidates per work item  */
+    uint32_t pass_chunk[RSH_MAX_PASSES];         /* 64-ray kernel: candidate clusters per item */
+    uint64_t pass_tri_tests[RSH_MAX_PASSES];     /* top-level triangle tests actually run (bit 1) */
+    uint64_t pass_cluster_tests[RSH_MAX_PASSES]; /* cluster pre-tests run (bit 1)            */
+    uint64_t pass_leaf_steps[RSH_MAX_PASSES];    /* 64-ray kernel: wave steps of leaf tests (bit 1) */
+    uint64_t pass_ray_steps[RSH_MAX_PASSES];     /* 64-ray kernel: wave steps of ray tests (bit 1)  */
 } rsh_frame_stats;
 
 typedef struct rsh_engine rsh_engine;

@@ -9,8 +9,10 @@
 //  * a pass reads back two scalars (live rays, packets) to size its
 //    launches, where the reference blocks on a readback after every scan
 //    of every triangle batch (RayTracer.cu:2316,2364,2649,2720,2832);
-//  * traversal and intersection run as one fused kernel per pass, with no
-//    triangle batching, no candidate arrays and no atomics on results;
+//  * traversal and intersection run as one fused kernel per pass over a
+//    work list of (top node, chunk of candidate clusters) items built by an
+//    object-culling kernel, with no triangle batching and no candidate pair
+//    arrays;
 //  * the OpenGL G-buffer is produced by a primary-ray pass through the same
 //    pipeline.
 #include <cstdio>
@@ -33,6 +35,9 @@ struct rsh_engine {
     uint32_t T = 0, K = 0, M = 0, L = 0;
     DevBuf tri_pos, tri_nrm, tri_mat, mat_dif, mat_spec, light_pos, light_col, light_int;
     DevBuf tri_sphere, tri_mt, obj_sphere, obj_range;
+    // cluster level: CLUSTER consecutive triangles of one object per cluster
+    uint32_t NC = 0;
+    DevBuf cl_sphere, cl_base, all_list;   // float4[NC], uint32[K] first cluster, uint32[K] identity
     std::vector<float> host_spheres;   // float3[2K]
     uint32_t ranges_tb = 0xFFFFFFFFu;  // triangle batch the exclusion ranges were built for
 
@@ -116,11 +121,12 @@ struct StageTimer {
 enum { ST_CREATE = 0, ST_TRIM, ST_COMPRESS, ST_SORT, ST_DECOMPRESS, ST_HIER, ST_TRAVERSE, ST_SHADE };
 constexpr int ST_PASS_TRAVERSE = 100;   // + pass index: per-pass traversal time
 constexpr uint64_t TRAVERSE_TARGET_WAVES = 16384;
-// 64-ray kernel work items: a pass starts at ITEM_CHUNK candidates per item
-// and shrinks the chunk (not below ITEM_CHUNK_MIN) until it has at least
-// ITEM_TARGET items, several per resident wave slot of the GPU.
-constexpr uint32_t ITEM_CHUNK = 4096;
-constexpr uint32_t ITEM_CHUNK_MIN = 256;
+// 64-ray kernel work items: a pass starts at ITEM_CHUNK candidate clusters
+// (of CLUSTER triangles) per item and shrinks the chunk (not below
+// ITEM_CHUNK_MIN) until it has at least ITEM_TARGET items, several per
+// resident wave slot of the GPU.
+constexpr uint32_t ITEM_CHUNK = 64;
+constexpr uint32_t ITEM_CHUNK_MIN = 4;
 constexpr uint32_t ITEM_TARGET = 32768;
 
 rsh_status ensure_frame(rsh_engine* e, uint32_t W, uint32_t H, uint32_t Lmax) {
@@ -136,7 +142,7 @@ rsh_status ensure_frame(rsh_engine* e, uint32_t W, uint32_t H, uint32_t Lmax) {
     HIP_OK(e->rec0.reserve(S * 16)); HIP_OK(e->rec1.reserve(S * 16));
     HIP_OK(e->pixels.reserve(P * 4));
     for (int i = 0; i < 4; i++) { HIP_OK(e->g0[i].reserve(P * 16)); HIP_OK(e->gb[i].reserve(P * 16)); }
-    HIP_OK(e->counters.reserve(COUNTER_SLOTS * 4 * 8)); HIP_OK(e->dcounts.reserve(64));
+    HIP_OK(e->counters.reserve(COUNTER_SLOTS * COUNTERS_PER_SLOT * 8)); HIP_OK(e->dcounts.reserve(64));
     HIP_OK(e->sc.reserve(S));
     e->cap_slots = S;
     return RSH_OK;
@@ -180,7 +186,7 @@ KeyCfg key_cfg(const rsh_frame_config& c) {
 
 struct PassOut {
     uint32_t N = 0, C = 0, splits = 0, items = 0, chunk = 0;
-    uint64_t top = 0, leaf = 0, tests = 0, surv = 0;
+    uint64_t top = 0, leaf = 0, tests = 0, surv = 0, tri_tests = 0, cl_tests = 0, b_steps = 0, c_steps = 0;
 };
 
 // Work list of the 64-ray kernel (see uses_items): cull, size, scan, fill.
@@ -203,7 +209,7 @@ rsh_status build_items(rsh_engine* e, const rsh_frame_config& cfg, TravParams& p
         HIP_OK(hipMemcpyAsync(tot + 1, u32p(e->w_first) + ntop, 4, hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
         if (cfg.traversal_chunk || tot[1] >= ITEM_TARGET || chunk <= ITEM_CHUNK_MIN || tot[1] == 0) break;
-        const uint64_t want = ((uint64_t)chunk * tot[1] / ITEM_TARGET) & ~63ull;
+        const uint64_t want = (uint64_t)chunk * tot[1] / ITEM_TARGET;
         chunk = want < ITEM_CHUNK_MIN ? ITEM_CHUNK_MIN : (uint32_t)want;
     }
     if (tot[1] > MAX_TRAVERSE_ITEMS) {
@@ -216,6 +222,7 @@ rsh_status build_items(rsh_engine* e, const rsh_frame_config& cfg, TravParams& p
     HIP_OK(e->w_items.reserve((size_t)tot[1] * 4 + 4));
     p.item_top = u32p(e->w_items); p.item_first = u32p(e->w_first); p.cand = cand;
     p.obj_off = u32p(e->w_off); p.obj_list = u32p(e->w_list); p.obj_cum = u32p(e->w_cum);
+    if (p.algorithm == 1) { p.obj_list = u32p(e->all_list); p.obj_cum = u32p(e->cl_base); }
     p.chunk = chunk; p.items = tot[1];
     launch_cull_list(p, u32p(e->w_list), u32p(e->w_cum), s);     // writes at p.obj_off
     launch_item_fill(u32p(e->w_first), nitems, ntop, u32p(e->w_items), s);
@@ -229,6 +236,7 @@ rsh_status build_items(rsh_engine* e, const rsh_frame_config& cfg, TravParams& p
 // e->closest; kind 1: shadow flags into e->shadow.
 rsh_status trace_pass(rsh_engine* e, const rsh_frame_config& cfg, const float* rays, uint32_t S, uint32_t kind,
                       const float4* shadow_pos, uint32_t flags_mask, PassOut& out) {
+    out = PassOut{};
     hipStream_t s = e->stream;
     const bool timing = flags_mask & 4u, counting = flags_mask & 2u;
     uint32_t* dc = u32p(e->dcounts);
@@ -292,11 +300,13 @@ rsh_status trace_pass(rsh_engine* e, const rsh_frame_config& cfg, const float* r
     p.rec0 = f4p(e->rec0); p.rec1 = f4p(e->rec1); p.N = N;
     p.tri_sphere = f4p(e->tri_sphere); p.tri_mt = f4p(e->tri_mt); p.T = e->T;
     p.obj_sphere = f4p(e->obj_sphere); p.obj_range = e->obj_range.as<uint4>(); p.K = e->K;
+    p.cl_sphere = f4p(e->cl_sphere); p.obj_cl_base = u32p(e->cl_base); p.n_clusters = e->NC;
     p.algorithm = cfg.algorithm; p.kind = kind; p.hit_mode = cfg.hit_mode;
     p.closest = u64p(e->closest); p.shadow = u32p(e->shadow);
     p.counters = counting ? u64p(e->counters) : nullptr;
     p.force_generic = cfg.traversal_kernel == 1 ? 1u : 0u;
     p.splits = 1;
+    if (counting) launch_fill_u64(u64p(e->counters), 0ull, COUNTER_SLOTS * COUNTERS_PER_SLOT, s);
     if (uses_items(p)) {
         // cull + work list; its time (host read-backs included) counts as traversal
         StageTimer t(e, ST_TRAVERSE, timing);
@@ -314,18 +324,20 @@ rsh_status trace_pass(rsh_engine* e, const rsh_frame_config& cfg, const float* r
         p.splits = (uint32_t)(sp ? sp : 1);
         out.splits = p.splits;
     }
-    if (counting) launch_fill_u64(u64p(e->counters), 0ull, COUNTER_SLOTS * 4, s);
     {
         StageTimer t(e, ST_PASS_TRAVERSE + (int)e->pass_index, timing);
         HIP_OK(launch_traverse(p, s));
     }
     if (counting) {
-        unsigned long long c[COUNTER_SLOTS * 4];
+        unsigned long long c[COUNTER_SLOTS * COUNTERS_PER_SLOT];
         HIP_OK(hipMemcpyAsync(c, e->counters.p, sizeof(c), hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
-        for (uint32_t k = 0; k < COUNTER_SLOTS; k++) {
-            out.top += c[4 * k]; out.leaf += c[4 * k + 1]; out.tests += c[4 * k + 2]; out.surv += c[4 * k + 3];
-        }
+        unsigned long long sum[COUNTERS_PER_SLOT] = {};
+        for (uint32_t k = 0; k < COUNTER_SLOTS; k++)
+            for (uint32_t i = 0; i < COUNTERS_PER_SLOT; i++) sum[i] += c[(size_t)k * COUNTERS_PER_SLOT + i];
+        out.top = sum[CNT_TOP_PAIRS]; out.leaf = sum[CNT_LEAF_PAIRS]; out.tests = sum[CNT_RAY_TESTS];
+        out.surv = sum[CNT_TOP_SURV]; out.tri_tests = sum[CNT_TRI_TESTS]; out.cl_tests = sum[CNT_CLUSTER_TESTS];
+        out.b_steps = sum[CNT_B_STEPS]; out.c_steps = sum[CNT_C_STEPS];
     }
     return RSH_OK;
 }
@@ -338,6 +350,8 @@ rsh_status record(rsh_engine* e, rsh_frame_stats* st, uint32_t kind, uint32_t S,
         st->pass_kind[i] = kind; st->pass_slots[i] = S; st->pass_rays[i] = po.N; st->pass_chunks[i] = po.C;
         st->pass_pairs_top[i] = po.top; st->pass_pairs_leaf[i] = po.leaf; st->pass_ray_tests[i] = po.tests;
         st->pass_top_survivors[i] = po.surv;
+        st->pass_tri_tests[i] = po.tri_tests; st->pass_cluster_tests[i] = po.cl_tests;
+        st->pass_leaf_steps[i] = po.b_steps; st->pass_ray_steps[i] = po.c_steps;
         st->rays_total += po.N;
     }
     e->pass_index++;
@@ -376,7 +390,8 @@ void rsh_engine_destroy(rsh_engine* e) {
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     for (DevBuf* b : {&e->tri_pos, &e->tri_nrm, &e->tri_mat, &e->mat_dif, &e->mat_spec, &e->light_pos, &e->light_col,
-                      &e->light_int, &e->tri_sphere, &e->tri_mt, &e->obj_sphere, &e->obj_range, &e->rays_a,
+                      &e->light_int, &e->tri_sphere, &e->tri_mt, &e->obj_sphere, &e->obj_range, &e->cl_sphere,
+                      &e->cl_base, &e->all_list, &e->rays_a,
                       &e->rays_b, &e->flags, &e->keys, &e->values, &e->tkeys, &e->tvalues, &e->bases, &e->sizes,
                       &e->ckeys, &e->cvalues, &e->skeys, &e->svalues, &e->rkeys, &e->rvalues, &e->skeleton, &e->scan,
                       &e->rec0, &e->rec1, &e->nodes, &e->aux, &e->closest, &e->shadow, &e->pixels, &e->counters,
@@ -451,8 +466,31 @@ rsh_status rsh_engine_load_scene(rsh_engine* e, const rsh_scene_desc* s) {
         for (int i = 0; i < 4; i++) s4[4 * k + i] = s->object_spheres[6 * k + i];
     HIP_OK(hipMemcpyAsync(e->obj_sphere.p, s4.data(), s4.size() * 4, hipMemcpyHostToDevice, st));
     launch_triangle_prep(f4p(e->tri_pos), T, f4p(e->tri_sphere), f4p(e->tri_mt), st);
+    // clusters of CLUSTER consecutive triangles per object, in object order
+    std::vector<uint32_t> base(K), ident(K);
+    std::vector<uint32_t> crange;
+    uint32_t nc = 0;
+    for (uint32_t k = 0; k < K; k++) {
+        const uint32_t first = (uint32_t)s->object_spheres[6 * k + 4], last = (uint32_t)s->object_spheres[6 * k + 5];
+        base[k] = nc; ident[k] = k;
+        for (uint32_t t = first; t <= last; t += CLUSTER) {
+            crange.push_back(t);
+            crange.push_back(t + CLUSTER - 1 < last ? t + CLUSTER - 1 : last);
+            nc++;
+        }
+    }
+    e->NC = nc;
+    HIP_OK(e->cl_sphere.reserve((size_t)nc * 16)); HIP_OK(e->cl_base.reserve((size_t)K * 4));
+    HIP_OK(e->all_list.reserve((size_t)K * 4));
+    DevBuf crange_d;
+    HIP_OK(crange_d.reserve(crange.size() * 4));
+    HIP_OK(hipMemcpyAsync(crange_d.p, crange.data(), crange.size() * 4, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(e->cl_base.p, base.data(), (size_t)K * 4, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(e->all_list.p, ident.data(), (size_t)K * 4, hipMemcpyHostToDevice, st));
+    launch_cluster_build(f4p(e->tri_sphere), crange_d.as<uint2>(), nc, f4p(e->cl_sphere), st);
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(st));
+    crange_d.release();
     e->ranges_tb = 0xFFFFFFFFu;
     return RSH_OK;
 }

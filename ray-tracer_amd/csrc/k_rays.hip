@@ -232,6 +232,35 @@ __global__ void __launch_bounds__(256) k_triangle_prep(const float4* tp, uint32_
     }
 }
 
+// Cluster spheres: one sphere over the triangle spheres of up to 64
+// consecutive triangles of one object (range[i] = first, last).  Center =
+// middle of the bounding box of the triangle spheres; radius = the largest
+// |c_i - q| + r_i, inflated by 1e-5 relative so that FP32 rounding cannot
+// leave a triangle sphere outside.
+__global__ void __launch_bounds__(256) k_cluster_build(const float4* tri_sphere, const uint2* range, uint32_t n,
+                                                       float4* cl_sphere) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint2 r = range[i];
+        v3 lo = mk(INFINITY, INFINITY, INFINITY), hi = mk(-INFINITY, -INFINITY, -INFINITY);
+        for (uint32_t t = r.x; t <= r.y; t++) {
+            const float4 s = tri_sphere[t];
+            lo = mk(fminf(lo.x, s.x - s.w), fminf(lo.y, s.y - s.w), fminf(lo.z, s.z - s.w));
+            hi = mk(fmaxf(hi.x, s.x + s.w), fmaxf(hi.y, s.y + s.w), fmaxf(hi.z, s.z + s.w));
+        }
+        const v3 q = (lo + hi) * 0.5f;
+        float rho = 0.0f;
+        bool finite = true;
+        for (uint32_t t = r.x; t <= r.y; t++) {
+            const float4 s = tri_sphere[t];
+            const float d = len(xyz(s) - q) + s.w;
+            finite = finite && isfinite(d);
+            rho = fmaxf(rho, d);
+        }
+        // A non-finite member makes the cluster sphere NaN: the pre-test then passes.
+        cl_sphere[i] = finite ? f4(q, rho * 1.00001f + 1e-6f) : make_float4(NAN, NAN, NAN, NAN);
+    }
+}
+
 // RayTracer.cu:448-511
 __global__ void __launch_bounds__(256) k_update_vertex(const float* model, const float* normal, uint32_t n,
                                                        const float4* pos_in, const float4* nrm_in,
@@ -361,6 +390,11 @@ void launch_shade_reflection(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, 
     k_shade_reflection<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(g, W, H, Lmax, L, eye, rec, hit_mode,
                                                                               sflags, pixels);
 }
+void launch_cluster_build(const float4* tri_sphere, const uint2* range, uint32_t n, float4* cl_sphere,
+                          hipStream_t s) {
+    if (n) k_cluster_build<<<grid_for(n, 256), 256, 0, s>>>(tri_sphere, range, n, cl_sphere);
+}
+
 void launch_triangle_prep(const float4* tri_pos, uint32_t T, float4* tri_sphere, float4* tri_mt, hipStream_t s) {
     if (T) k_triangle_prep<<<grid_for(T, 256), 256, 0, s>>>(tri_pos, T, tri_sphere, tri_mt);
 }

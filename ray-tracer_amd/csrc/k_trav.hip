@@ -255,11 +255,12 @@ __global__ void __launch_bounds__(256) k_traverse(TravParams P) {
     }
     if (count) {
         if (lane == 0) {
-            unsigned long long* cs = P.counters + (size_t)(wave % COUNTER_SLOTS) * 4;
-            atomicAdd(&cs[0], (unsigned long long)n_top);
-            atomicAdd(&cs[1], (unsigned long long)n_leaf);
-            atomicAdd(&cs[2], (unsigned long long)n_tests);
-            atomicAdd(&cs[3], (unsigned long long)n_surv);
+            unsigned long long* cs = P.counters + (size_t)(wave % COUNTER_SLOTS) * COUNTERS_PER_SLOT;
+            atomicAdd(&cs[CNT_TOP_PAIRS], (unsigned long long)n_top);
+            atomicAdd(&cs[CNT_TRI_TESTS], (unsigned long long)n_top);
+            atomicAdd(&cs[CNT_LEAF_PAIRS], (unsigned long long)n_leaf);
+            atomicAdd(&cs[CNT_RAY_TESTS], (unsigned long long)n_tests);
+            atomicAdd(&cs[CNT_TOP_SURV], (unsigned long long)n_surv);
         }
     }
 }
@@ -267,12 +268,15 @@ __global__ void __launch_bounds__(256) k_traverse(TravParams P) {
 // ------------------------------------------- top-level object culling ------
 // One wave per top node (depth 2, 64 rays): the object-sphere tests of
 // RayTracer.cu:1152-1280, once per node.  Passing objects are listed in
-// triangle order with the number of candidate triangles before each, so the
-// node's candidates form one stream that is cut into equal chunks (work
-// items) for the traversal kernel.  The count pass sizes the compact lists
-// (npass, cand per node; entry n_top is written as 0 so an exclusive scan
-// over n_top + 1 entries ends with the total), the list pass fills them.
-// RAH (no object culling) streams every triangle and has no list.
+// triangle order with the number of candidate clusters (CLUSTER consecutive
+// triangles of one object) before each, so the node's candidates form one
+// stream of clusters that is cut into equal chunks (work items) for the
+// traversal kernel.  The count pass sizes the compact lists (npass, cand per
+// node; entry n_top is written as 0 so an exclusive scan over n_top + 1
+// entries ends with the total), the list pass fills them.  RAH (no object
+// culling) streams every cluster through one shared list of all objects.
+// With counters on, the count pass adds the reference's candidate pair count
+// (triangles of passing objects outside the excluded batch) per node.
 __device__ __forceinline__ uint32_t wave_incl_add(uint32_t x) {
     const uint32_t lane = lane_id();
 #pragma unroll
@@ -294,8 +298,13 @@ __global__ void __launch_bounds__(256) k_cull(TravParams P, uint32_t* npass, uin
         if (lane == 0) { npass[top] = 0; cand[top] = 0; }
         return;
     }
+    unsigned long long* cs =
+        P.counters ? P.counters + (size_t)(top % COUNTER_SLOTS) * COUNTERS_PER_SLOT : nullptr;
     if (P.algorithm == 1) {
-        if (!LIST && lane == 0) { npass[top] = 0; cand[top] = P.T; }
+        if (!LIST && lane == 0) {
+            npass[top] = P.K; cand[top] = P.n_clusters;
+            if (cs) atomicAdd(&cs[CNT_TOP_PAIRS], (unsigned long long)P.T);
+        }
         return;
     }
     const uint32_t gt = P.level_offset[1] + top;
@@ -303,23 +312,33 @@ __global__ void __launch_bounds__(256) k_cull(TravParams P, uint32_t* npass, uin
     const float2 tA = P.aux[gt];
     const size_t base = LIST ? P.obj_off[top] : 0;
     uint32_t n = 0, cum = 0;
+    unsigned long long pairs = 0;
     for (uint32_t ob = 0; ob < P.K; ob += 64) {
         const uint32_t oi = ob + lane;
         bool po = false;
-        uint32_t len = 0;
+        uint32_t ncl = 0, npair = 0;
         if (oi < P.K) {
             po = sphere_node(tS, tC, P.obj_sphere[oi], tA.x, tA.y);
-            if (po) { const uint4 r = P.obj_range[oi]; len = r.y - r.x + 1; }
+            if (po) {
+                const uint4 r = P.obj_range[oi];
+                ncl = (r.y - r.x + CLUSTER) / CLUSTER;
+                npair = (r.y - r.x + 1) - (r.w - r.z);
+            }
         }
         const uint64_t m = __ballot(po);
-        const uint32_t incl = wave_incl_add(len);
+        const uint32_t incl = wave_incl_add(ncl);
         if (LIST && po) {
             const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
             obj_list[base + n + rank] = oi;
-            obj_cum[base + n + rank] = cum + incl - len;
+            obj_cum[base + n + rank] = cum + incl - ncl;
         }
         n += (uint32_t)__popcll(m);
         cum += __shfl(incl, 63, 64);
+        pairs += npair;
+    }
+    if (!LIST && cs) {
+        for (int dlt = 32; dlt > 0; dlt >>= 1) pairs += __shfl_xor(pairs, dlt, 64);
+        if (lane == 0) atomicAdd(&cs[CNT_TOP_PAIRS], pairs);
     }
     if (!LIST && lane == 0) { npass[top] = n; cand[top] = cum; }
 }
@@ -344,10 +363,13 @@ __global__ void __launch_bounds__(256) k_item_fill(const uint32_t* item_first, c
 // Specialisation for depth 2 with leaf * branch == 64: one wave = one work
 // item = one chunk of one top node's candidate stream (k_cull); the top node
 // covers BRANCH leaves of LEAF rays (lane i holds sorted ray i of the node).
-//  A  The wave walks its chunk 64 candidate triangles at a time with a cursor
-//     over the listed objects; NB blocks' bounding spheres are loaded back to
-//     back and tested against the top node in order.  Survivors (index +
-//     sphere) are appended to an LDS queue.
+//  A  The wave walks its chunk of clusters (CLUSTER consecutive triangles of
+//     a listed object): 64 clusters at a time are pre-tested, one per lane,
+//     with the conservative cluster_may_pass; the triangles of each passing
+//     cluster are then tested against the top node 64 at a time, NB blocks'
+//     bounding spheres loaded back to back.  Survivors (index + sphere) are
+//     appended to an LDS queue.  The pre-test only skips triangles whose own
+//     test would fail, so the survivors are exactly the reference's.
 //  B  When 64 survivors are queued their Moller-Trumbore vectors are fetched
 //     with one parallel load, then (survivor, leaf) pairs are tested 64 at a
 //     time: lane i takes leaf i % BRANCH (held in registers), survivor
@@ -360,11 +382,17 @@ __global__ void __launch_bounds__(256) k_item_fill(const uint32_t* item_first, c
 template <int LEAF>
 __global__ void __launch_bounds__(256) k_traverse64(TravParams P) {
     constexpr uint32_t BRANCH = 64 / LEAF;
-    constexpr uint32_t GROUP = 64 / BRANCH;     // survivors per stage-B step
-    constexpr uint32_t QCAP = 128;
+    constexpr uint32_t GROUP = 64 / BRANCH;       // survivors per stage-B step
+    constexpr uint32_t QCAP = 128;                // survivor queue
+    constexpr uint32_t PCAP = 512;                // (survivor, leaf) pair list
+    constexpr uint32_t SCH = PCAP / BRANCH < 64 ? PCAP / BRANCH : 64;   // survivors per B+C round
     __shared__ float4 s_qsph[4][QCAP];
     __shared__ uint32_t s_qidx[4][QCAP];
     __shared__ float4 s_mt[4][64][3];
+    __shared__ float4 s_ray0[4][64];              // (o + d*eps, tmax)
+    __shared__ float4 s_ray1[4][64];              // (d, valid)
+    __shared__ unsigned long long s_best[4][64];  // closest record / shadow flag per ray
+    __shared__ uint16_t s_pair[4][PCAP];          // survivor << 5 | leaf
     const uint32_t lane = lane_id();
     const uint32_t w = threadIdx.x >> 6;
     const uint32_t item = blockIdx.x * 4 + w;
@@ -374,16 +402,21 @@ __global__ void __launch_bounds__(256) k_traverse64(TravParams P) {
     const uint32_t pos = top * 64 + lane;
     const bool valid = pos < P.N;
 
-    v3 o = mk(0, 0, 0), d = mk(0, 0, 0);
     uint32_t slot = 0;
-    float tmax = 0.0f;
-    if (valid) {
-        float4 a = P.rec0[pos], b = P.rec1[pos];
-        o = xyz(a); d = xyz(b);
-        slot = __float_as_uint(a.w);
-        tmax = b.w;
+    {
+        v3 o = mk(0, 0, 0), d = mk(0, 0, 0);
+        float tmax = 0.0f;
+        if (valid) {
+            float4 a = P.rec0[pos], b = P.rec1[pos];
+            o = xyz(a); d = xyz(b);
+            slot = __float_as_uint(a.w);
+            tmax = b.w;
+        }
+        s_ray0[w][lane] = f4(o + d * EPS, tmax);
+        s_ray1[w][lane] = f4(d, valid ? 1.0f : 0.0f);
     }
-    const v3 oe = o + d * EPS;
+    const unsigned long long miss = P.kind == 1 ? 0ull : (P.hit_mode == 0 ? MISS64 : (unsigned long long)MISS32);
+    s_best[w][lane] = miss;
     const uint32_t gt = P.level_offset[1] + top;
     const float4 tS = P.nodes[2 * (size_t)gt], tC = P.nodes[2 * (size_t)gt + 1];
     const float2 tA = P.aux[gt];
@@ -396,21 +429,41 @@ __global__ void __launch_bounds__(256) k_traverse64(TravParams P) {
         lC = P.nodes[2 * (size_t)bl + 1];
         lA = P.aux[bl];
     }
-    const uint32_t my_leaf = lane / LEAF;                   // stage-C leaf of this lane's ray
-
-    unsigned long long best = P.hit_mode == 0 ? MISS64 : (unsigned long long)MISS32;
-    bool occluded = false;
     const uint64_t alive = __ballot(valid);
     const bool count = P.counters != nullptr;
-    uint64_t n_top = 0, n_surv = 0, n_leaf = 0, n_tests = 0;
+    uint64_t n_top = 0, n_surv = 0, n_leaf = 0, n_tests = 0, n_bsteps = 0, n_csteps = 0;
     bool done = false;
     uint32_t qlen = 0;
+    __builtin_amdgcn_wave_barrier();
 
-    // ---- stages B + C over the queued survivors
+    // ---- stages B + C over the queued survivors, SCH at a time:
+    //  B  (survivor, leaf) tests, lane = survivor g + lane / BRANCH, leaf lane % BRANCH;
+    //     passing pairs are appended to s_pair, and `used` marks survivors in a pair;
+    //  load the Moller-Trumbore vectors of the used survivors only;
+    //  C  the pair list is walked BRANCH pairs per step, lane = pair p0 + lane / LEAF,
+    //     ray lane % LEAF of the pair's leaf; hits merge per ray in LDS.
     auto flush = [&]() {
-        for (uint32_t c0 = 0; c0 < qlen && !done; c0 += 64) {
-            const uint32_t n = qlen - c0 < 64 ? qlen - c0 : 64;
-            if (lane < n) {
+        for (uint32_t c0 = 0; c0 < qlen && !done; c0 += SCH) {
+            const uint32_t n = qlen - c0 < SCH ? qlen - c0 : SCH;
+            uint32_t np_ = 0;
+            uint64_t used = 0;
+            for (uint32_t g = 0; g < n; g += GROUP) {
+                const uint32_t sv = g + lane / BRANCH;
+                bool lp = false;
+                if (sv < n && bl_valid) lp = sphere_node(lS, lC, s_qsph[w][c0 + sv], lA.x, lA.y);
+                const uint64_t pm = __ballot(lp);
+                if (lp) {
+                    const uint32_t r = np_ + (uint32_t)__popcll(pm & ((1ull << lane) - 1ull));
+                    s_pair[w][r] = (uint16_t)((sv << 5) | (lane % BRANCH));
+                }
+                np_ += (uint32_t)__popcll(pm);
+#pragma unroll
+                for (uint32_t j = 0; j < GROUP; j++)
+                    used |= (uint64_t)(((pm >> (j * BRANCH)) & ((1ull << BRANCH) - 1ull)) != 0ull) << (g + j);
+                if (count) { n_leaf += (uint64_t)__popcll(pm); n_bsteps++; }
+            }
+            if (np_ == 0) continue;
+            if (lane < n && ((used >> lane) & 1ull)) {
                 size_t t = s_qidx[w][c0 + lane];
                 if (t >= P.T) t = 0;     // queue entries are < T by construction
                 s_mt[w][lane][0] = P.tri_mt[3 * t];
@@ -418,108 +471,119 @@ __global__ void __launch_bounds__(256) k_traverse64(TravParams P) {
                 s_mt[w][lane][2] = P.tri_mt[3 * t + 2];
             }
             __builtin_amdgcn_wave_barrier();
-            uint64_t mine = 0;   // survivors (within this chunk) that passed my ray's leaf
-            for (uint32_t g = 0; g < n; g += GROUP) {
-                const uint32_t s = g + lane / BRANCH;
-                bool lp = false;
-                if (s < n && bl_valid) lp = sphere_node(lS, lC, s_qsph[w][c0 + s], lA.x, lA.y);
-                const uint64_t pm = __ballot(lp);
-                if (count) n_leaf += (uint64_t)__popcll(pm);
-#pragma unroll
-                for (uint32_t j = 0; j < GROUP; j++)
-                    mine |= ((pm >> (j * BRANCH + my_leaf)) & 1ull) << (g + j);
-            }
-            if (!valid || occluded) mine = 0;
-            while (__ballot(mine != 0ull)) {
-                if (mine) {
-                    const uint32_t s = (uint32_t)__builtin_ctzll(mine);
-                    mine &= mine - 1ull;
-                    const float4 A = s_mt[w][s][0], B = s_mt[w][s][1], Cc = s_mt[w][s][2];
-                    const float th = ray_triangle(oe, d, xyz(A), xyz(B), xyz(Cc));
-                    if (count) n_tests++;
-                    if (P.kind == 1) {
-                        if (th > EPS && th < tmax) { occluded = true; mine = 0; }
-                    } else if (th > EPS) {
-                        const unsigned long long rec = pack_hit(th, s_qidx[w][c0 + s], P.hit_mode);
-                        best = rec < best ? rec : best;
+            for (uint32_t p0 = 0; p0 < np_; p0 += BRANCH) {
+                const uint32_t pi = p0 + lane / LEAF;
+                if (count) n_csteps++;
+                if (pi < np_) {
+                    const uint32_t e = s_pair[w][pi];
+                    const uint32_t sv = e >> 5;
+                    const uint32_t ray = (e & 31u) * LEAF + lane % LEAF;
+                    const float4 r0 = s_ray0[w][ray], r1 = s_ray1[w][ray];
+                    const bool live = r1.w != 0.0f && (P.kind != 1 || s_best[w][ray] == 0ull);
+                    if (live) {
+                        const float4 A = s_mt[w][sv][0], B = s_mt[w][sv][1], Cc = s_mt[w][sv][2];
+                        const float th = ray_triangle(xyz(r0), xyz(r1), xyz(A), xyz(B), xyz(Cc));
+                        if (count) n_tests++;
+                        if (P.kind == 1) {
+                            if (th > EPS && th < r0.w) s_best[w][ray] = 1ull;
+                        } else if (th > EPS) {
+                            atomicMin(&s_best[w][ray], pack_hit(th, s_qidx[w][c0 + sv], P.hit_mode));
+                        }
                     }
                 }
             }
-            if (P.kind == 1 && (__ballot(occluded) & alive) == alive) done = true;
             __builtin_amdgcn_wave_barrier();
+            if (P.kind == 1 && (__ballot(s_best[w][lane] != 0ull) & alive) == alive) done = true;
         }
         qlen = 0;
     };
 
-    // ---- cursor over this item's chunk [s0, s1) of the node's candidate stream
+    // ---- cursor over this item's chunk [s0, s1) of the node's cluster stream
+    // (RAH: the engine passes the list of all objects with obj_cum = obj_cl_base)
     const bool rah = P.algorithm == 1;
     const uint32_t chunk_ix = item - P.item_first[top];
     const uint32_t ncand = P.cand[top];
     const uint32_t s0 = chunk_ix * P.chunk;
     const uint32_t s1 = ncand - s0 < P.chunk ? ncand : s0 + P.chunk;
     const size_t lbase = rah ? 0 : P.obj_off[top];
-    const uint32_t np = rah ? 1u : P.obj_off[top + 1] - P.obj_off[top];
-    uint32_t rem = s1 > s0 ? s1 - s0 : 0;     // candidates not yet handed to a block
+    const uint32_t np = rah ? P.K : P.obj_off[top + 1] - P.obj_off[top];
+    const NodeCull nc = node_cull(tS, tC, tA.x, tA.y);
+    uint64_t n_cl = 0;
+    uint32_t rem = s1 > s0 ? s1 - s0 : 0;     // clusters of the chunk not yet taken from an entry
     uint32_t j = 0;                            // current entry of the object list
-    uint32_t c_tb = 0, c_last = 0, c_exlo = 0, c_exhi = 0;
-    bool c_valid = false;
-    // Take the next `rem`-bounded portion of list entry jj, starting `off` candidates in.
+    uint32_t e_first = 0, e_last = 0, e_exlo = 0, e_exhi = 0, e_cbase = 0, e_ncl = 0;
+    uint32_t lc_cur = 0, lc_end = 0;           // local clusters of the entry still to pre-test
+    uint32_t g0 = 0;                           // first local cluster of the tested group
+    uint64_t gmask = 0;                        // clusters of the group that passed the pre-test
+    // Take the `rem`-bounded part of list entry jj, starting `off` clusters in.
     auto load_entry = [&](uint32_t jj, uint32_t off) {
-        uint32_t first, last;
-        if (rah) { first = 0; last = P.T - 1; c_exlo = c_exhi = 0; }
-        else {
-            const uint4 r = P.obj_range[P.obj_list[lbase + jj]];
-            first = r.x; last = r.y; c_exlo = r.z; c_exhi = r.w;
-        }
-        c_tb = first + off;
-        const uint32_t avail = last - c_tb + 1;
+        const uint32_t ob = P.obj_list[lbase + jj];
+        const uint4 r = P.obj_range[ob];
+        e_first = r.x; e_last = r.y;
+        e_exlo = rah ? 0u : r.z; e_exhi = rah ? 0u : r.w;
+        e_cbase = P.obj_cl_base[ob];
+        e_ncl = (r.y - r.x + CLUSTER) / CLUSTER;
+        const uint32_t avail = e_ncl - off;
         const uint32_t take = avail < rem ? avail : rem;
-        c_last = c_tb + take - 1;
+        lc_cur = off; lc_end = off + take;
         rem -= take;
-        c_valid = true;
     };
     if (rem) {
-        if (rah) {
-            load_entry(0, s0);
-        } else {
-            // last list entry whose stream offset is <= s0 (the offsets increase)
-            uint32_t j0 = 0;
-            for (uint32_t jb = 0; jb < np; jb += 64) {
-                const uint32_t jj = jb + lane;
-                const bool le = jj < np && P.obj_cum[lbase + jj] <= s0;
-                const uint64_t bm = __ballot(le);
-                if (bm) j0 = jb + 63u - (uint32_t)__builtin_clzll(bm);
-                if (bm != ~0ull) break;
-            }
-            j = __builtin_amdgcn_readfirstlane(j0);
-            load_entry(j, s0 - P.obj_cum[lbase + j]);
+        // last list entry whose stream offset is <= s0 (the offsets increase)
+        uint32_t j0 = 0;
+        for (uint32_t jb = 0; jb < np; jb += 64) {
+            const uint32_t jj = jb + lane;
+            const bool le = jj < np && P.obj_cum[lbase + jj] <= s0;
+            const uint64_t bm = __ballot(le);
+            if (bm) j0 = jb + 63u - (uint32_t)__builtin_clzll(bm);
+            if (bm != ~0ull) break;
         }
+        j = __builtin_amdgcn_readfirstlane(j0);
+        load_entry(j, s0 - P.obj_cum[lbase + j]);
     }
-    auto advance = [&]() {
-        if (c_valid && c_tb + 64 <= c_last) { c_tb += 64; return; }
-        if (!c_valid || rah || rem == 0 || j + 1 >= np) { c_valid = false; return; }
-        j++;
-        load_entry(j, 0);
-    };
-    auto fetch = [&](float4& ts, bool& inr) {
-        const uint32_t t = c_tb + lane;
-        inr = c_valid && t <= c_last && !(t >= c_exlo && t < c_exhi);
-        ts = make_float4(0, 0, 0, 0);
-        if (inr) ts = P.tri_sphere[t];
+    // Next block of one cluster's triangles [tb, tl] that passed the cluster
+    // pre-test; clusters are pre-tested 64 at a time, one per lane.
+    auto next_block = [&](uint32_t& tb, uint32_t& tl) -> bool {
+        while (gmask == 0ull) {
+            if (lc_cur >= lc_end) {
+                if (rem == 0 || j + 1 >= np) return false;
+                j++;
+                load_entry(j, 0);
+                continue;
+            }
+            g0 = lc_cur;
+            const uint32_t n = lc_end - lc_cur < 64 ? lc_end - lc_cur : 64;
+            lc_cur += n;
+            if (e_ncl == 1) { gmask = 1ull; break; }   // the object test already stands for it
+            bool cp = false;
+            if (lane < n) cp = cluster_may_pass(nc, P.cl_sphere[e_cbase + g0 + lane]);
+            gmask = __ballot(cp);
+            if (count) n_cl += n;
+        }
+        const uint32_t i = (uint32_t)__builtin_ctzll(gmask);
+        gmask &= gmask - 1ull;
+        tb = e_first + (g0 + i) * CLUSTER;
+        tl = tb + (CLUSTER - 1) < e_last ? tb + (CLUSTER - 1) : e_last;
+        return true;
     };
 
     // NB blocks per iteration: their sphere loads are issued back to back and
     // consumed in order, so only the first load's latency is exposed.
     constexpr int NB = 4;
-    while (c_valid && !done) {    // the cursor starts on the item's first block
+    bool have = rem != 0 || lc_cur < lc_end;
+    while (have && !done) {
         uint32_t tbk[NB];
         bool ink[NB];
         float4 tsk[NB];
 #pragma unroll
         for (int k = 0; k < NB; k++) {
-            tbk[k] = c_tb;
-            fetch(tsk[k], ink[k]);
-            advance();
+            uint32_t tb = 0, tl = 0;
+            have = have && next_block(tb, tl);
+            const uint32_t t = tb + lane;
+            ink[k] = have && t <= tl && !(t >= e_exlo && t < e_exhi);
+            tbk[k] = tb;
+            tsk[k] = make_float4(0, 0, 0, 0);
+            if (ink[k]) tsk[k] = P.tri_sphere[t];
         }
 #pragma unroll
         for (int k = 0; k < NB; k++) {
@@ -547,8 +611,10 @@ __global__ void __launch_bounds__(256) k_traverse64(TravParams P) {
     if (qlen && !done) flush();
 
     // Results start as misses; a node cut into several items merges by atomicMin.
+    __builtin_amdgcn_wave_barrier();
+    const unsigned long long best = s_best[w][lane];
     if (valid) {
-        if (P.kind == 1) { if (occluded) P.shadow[slot] = MISS32; }
+        if (P.kind == 1) { if (best != 0ull) P.shadow[slot] = MISS32; }
         else if (ncand <= P.chunk) P.closest[slot] = best;
         else if (best != (P.hit_mode == 0 ? MISS64 : (unsigned long long)MISS32)) atomicMin(&P.closest[slot], best);
     }
@@ -556,11 +622,14 @@ __global__ void __launch_bounds__(256) k_traverse64(TravParams P) {
         unsigned long long tsum = n_tests;
         for (int dlt = 32; dlt > 0; dlt >>= 1) tsum += __shfl_xor(tsum, dlt, 64);
         if (lane == 0) {
-            unsigned long long* cs = P.counters + (size_t)(item % COUNTER_SLOTS) * 4;
-            atomicAdd(&cs[0], (unsigned long long)n_top);
-            atomicAdd(&cs[1], (unsigned long long)n_leaf);
-            atomicAdd(&cs[2], tsum);
-            atomicAdd(&cs[3], (unsigned long long)n_surv);
+            unsigned long long* cs = P.counters + (size_t)(item % COUNTER_SLOTS) * COUNTERS_PER_SLOT;
+            atomicAdd(&cs[CNT_LEAF_PAIRS], (unsigned long long)n_leaf);
+            atomicAdd(&cs[CNT_RAY_TESTS], tsum);
+            atomicAdd(&cs[CNT_TOP_SURV], (unsigned long long)n_surv);
+            atomicAdd(&cs[CNT_TRI_TESTS], (unsigned long long)n_top);
+            atomicAdd(&cs[CNT_CLUSTER_TESTS], (unsigned long long)n_cl);
+            atomicAdd(&cs[CNT_B_STEPS], (unsigned long long)n_bsteps);
+            atomicAdd(&cs[CNT_C_STEPS], (unsigned long long)n_csteps);
         }
     }
 }

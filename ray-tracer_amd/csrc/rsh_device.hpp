@@ -244,6 +244,46 @@ __device__ __forceinline__ bool sphere_node(float4 sphere, float4 cone, float4 t
     return (pr * tangent + (sphere.w + tri.w) / cosine) >= len(s2t - proj);
 }
 
+// Conservative cluster pre-test for sphere_node.  With J = I - cd cd^T the
+// triangle test passes iff f(c) = |J (c - s)| - ((c - s).cd) tan <= (r + rt)/cos.
+// f is Lipschitz with constant L = sqrt(1 + (tan k)^2 + 2 tan k |1 - k^2|),
+// k = |cd| <= sqrt(2).  For a triangle sphere (c, rt) inside a cluster sphere
+// (q, rho), |c - q| <= rho - rt, so a passing triangle gives
+// f(q) <= r/cos + rho * max(L, 1/cos).  The cluster is rejected only when
+// f(q) exceeds that bound by E, a margin of 1e-4 of the magnitudes involved
+// (the FP32 rounding of either evaluation is ~1e-6 of them).  Nodes outside
+// the argument's assumptions (the HALF_PI shortcut, cos <= 0, tan < 0,
+// non-finite values, |cd| far from 1) never reject; a NaN anywhere passes.
+struct NodeCull {
+    v3 c, cd;
+    float r, cosv, tanv, lmax;
+    bool ok;
+};
+
+__device__ __forceinline__ NodeCull node_cull(float4 sphere, float4 cone, float cosine, float tangent) {
+    NodeCull n;
+    n.c = xyz(sphere); n.cd = xyz(cone); n.r = sphere.w; n.cosv = cosine; n.tanv = tangent;
+    const float k = len(n.cd);
+    const float tk = tangent * k;
+    const float L = sqrtf(1.0f + tk * tk + 2.0f * tk * fabsf(1.0f - k * k));
+    n.lmax = fmaxf(L, 1.0f / cosine) * 1.00001f;
+    n.ok = cone.w != HALF_PI_F && cosine > 0.0f && tangent >= 0.0f && k > 0.5f && k < 1.4f &&
+           isfinite(cosine) && isfinite(tangent) && isfinite(n.lmax) && isfinite(sphere.w) &&
+           isfinite(n.c.x) && isfinite(n.c.y) && isfinite(n.c.z);
+    return n;
+}
+
+__device__ __forceinline__ bool cluster_may_pass(const NodeCull& n, float4 cl) {
+    if (!n.ok) return true;
+    const v3 s2q = xyz(cl) - n.c;
+    const float pr = dot(s2q, n.cd);
+    const float f = len(s2q - pr * n.cd) - pr * n.tanv;
+    const float rho = cl.w;
+    const float allow = n.r / n.cosv + rho * n.lmax;
+    const float E = 1e-4f * (len(s2q) + rho + (fabsf(pr) + rho) * n.tanv + (n.r + rho) * n.lmax) + 1e-4f;
+    return !(f > allow + E);
+}
+
 // RayTracer.cu:301-376 CreateTriangleBoundingSphere
 __device__ inline float4 triangle_sphere(v3 a, v3 b, v3 c) {
     float dotABAB = dot(b - a, b - a);

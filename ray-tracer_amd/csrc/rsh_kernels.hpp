@@ -29,7 +29,21 @@ struct GBufView {
 
 // Diagnostic traversal counters are spread over this many slots (summed on
 // the host) so that the per-wave atomics do not serialise on one address.
+// Each slot holds COUNTERS_PER_SLOT values (see CounterIndex).
 constexpr uint32_t COUNTER_SLOTS = 64;
+constexpr uint32_t COUNTERS_PER_SLOT = 8;
+enum CounterIndex : uint32_t {
+    CNT_TOP_PAIRS = 0,     // (top node, triangle) candidate pairs of the reference's culling
+    CNT_LEAF_PAIRS = 1,    // (leaf, triangle) pairs passing the leaf test
+    CNT_RAY_TESTS = 2,     // ray-triangle tests run
+    CNT_TOP_SURV = 3,      // (top node, triangle) pairs passing the top-level test
+    CNT_TRI_TESTS = 4,     // top-level triangle-sphere tests actually run
+    CNT_CLUSTER_TESTS = 5, // cluster pre-tests run
+    CNT_C_STEPS = 6,       // 64-ray kernel: wave steps of the ray-test stage (lane use = tests / 64 steps)
+    CNT_B_STEPS = 7        // 64-ray kernel: wave steps of the leaf-test stage
+};
+// Triangles per cluster of the traversal's cluster level.
+constexpr uint32_t CLUSTER = 64;
 
 struct TravParams {
     const float4* nodes;          // hierarchy, sphere at 2g, cone at 2g+1
@@ -52,8 +66,8 @@ struct TravParams {
     uint32_t hit_mode;            // 0 exact, 1 reference packing
     unsigned long long* closest;  // [S] per slot
     uint32_t* shadow;             // [S] per slot
-    unsigned long long* counters; // optional [COUNTER_SLOTS][4]: top pairs, leaf pairs, ray tests,
-                                  // top survivors; waves add into slot wave % COUNTER_SLOTS
+    unsigned long long* counters; // optional [COUNTER_SLOTS][COUNTERS_PER_SLOT] (CounterIndex);
+                                  // waves add into slot wave % COUNTER_SLOTS
     uint32_t force_generic;       // 1: use the generic kernel even for 64-ray top nodes
     uint32_t splits;              // generic kernel: waves sharing one top node's triangle range
     // work items of the 64-ray kernel: (top node, chunk of candidate triangles)
@@ -61,10 +75,14 @@ struct TravParams {
     const uint32_t* item_first;   // [n_top + 1] first item of each top node
     const uint32_t* cand;         // [n_top + 1] candidate triangles of each top node
     const uint32_t* obj_off;      // [n_top + 1] first list entry of each top node (CRSH)
-    const uint32_t* obj_list;     // passing objects of each node, in triangle order
-    const uint32_t* obj_cum;      // candidates of the node before each listed object
-    uint32_t chunk;               // candidates per item
+    const uint32_t* obj_list;     // passing objects of each node, in triangle order (RAH: all, shared)
+    const uint32_t* obj_cum;      // candidate clusters of the node before each listed object
+    uint32_t chunk;               // candidate clusters per item
     uint32_t items;
+    // cluster level: clusters of CLUSTER consecutive triangles of one object
+    const float4* cl_sphere;      // [clusters] conservative sphere over the member triangle spheres
+    const uint32_t* obj_cl_base;  // [K] first cluster of each object (RAH: also the list's obj_cum)
+    uint32_t n_clusters;
 };
 
 // Work items of one launch of the 64-ray kernel: 64 lanes each, so the grid
@@ -110,6 +128,8 @@ void launch_shade_reflection(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, 
                              const unsigned long long* rec, uint32_t hit_mode, const uint32_t* sflags,
                              uint32_t* pixels, hipStream_t s);
 void launch_triangle_prep(const float4* tri_pos, uint32_t T, float4* tri_sphere, float4* tri_mt, hipStream_t s);
+void launch_cluster_build(const float4* tri_sphere, const uint2* range, uint32_t n, float4* cl_sphere,
+                          hipStream_t s);
 // reference-only stages
 void launch_update_vertex(const float* model, const float* normal, uint32_t vertexTotal, const float4* pos_in,
                           const float4* nrm_in, const int32_t* obj_ids, float4* pos_out, float4* nrm_out,

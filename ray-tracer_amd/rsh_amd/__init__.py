@@ -52,7 +52,9 @@ class FrameStats(C.Structure):
                 ("pass_ray_tests", u64 * MAX_PASSES), ("stage_ms", f32 * 8), ("rays_total", u32),
                 ("pass_top_survivors", u64 * MAX_PASSES), ("pass_traverse_ms", f32 * MAX_PASSES),
                 ("pass_splits", u32 * MAX_PASSES), ("pass_items", u32 * MAX_PASSES),
-                ("pass_chunk", u32 * MAX_PASSES)]
+                ("pass_chunk", u32 * MAX_PASSES), ("pass_tri_tests", u64 * MAX_PASSES),
+                ("pass_cluster_tests", u64 * MAX_PASSES), ("pass_leaf_steps", u64 * MAX_PASSES),
+                ("pass_ray_steps", u64 * MAX_PASSES)]
 
 
 class Float3(C.Structure):
@@ -310,7 +312,9 @@ class Engine:
                      pairs_top=st.pass_pairs_top[i], pairs_leaf=st.pass_pairs_leaf[i],
                      ray_tests=st.pass_ray_tests[i], top_survivors=st.pass_top_survivors[i],
                      traverse_ms=st.pass_traverse_ms[i], splits=st.pass_splits[i], items=st.pass_items[i],
-                     chunk=st.pass_chunk[i])
+                     chunk=st.pass_chunk[i], tri_tests=st.pass_tri_tests[i],
+                     cluster_tests=st.pass_cluster_tests[i], leaf_steps=st.pass_leaf_steps[i],
+                     ray_steps=st.pass_ray_steps[i])
                 for i in range(st.pass_count)]
 
     def close(self):

@@ -58,10 +58,10 @@ def test_config0_default(engine, oracle, rsh, w):
     dict(traversal_kernel=1),                # generic kernel on the 64-ray shape
     dict(traversal_kernel=1, algorithm=1),
     dict(algorithm=1, bounces=2, bounce_shadows=1),
-    dict(traversal_chunk=64),                # many work items per top node, merged by atomicMin
-    dict(traversal_chunk=100),               # items that start and end inside 64-triangle blocks
-    dict(traversal_chunk=100, algorithm=1),
-    dict(traversal_chunk=64, bounces=2, bounce_shadows=1),
+    dict(traversal_chunk=1),                 # one cluster per work item, merged by atomicMin
+    dict(traversal_chunk=3),                 # items that start and end inside objects
+    dict(traversal_chunk=3, algorithm=1),
+    dict(traversal_chunk=1, bounces=2, bounce_shadows=1),
     dict(traversal_chunk=1 << 30),           # one item per top node
     dict(traversal_splits=3, traversal_kernel=1),
     dict(traversal_splits=5, traversal_kernel=1, bounces=2, bounce_shadows=1),
@@ -100,6 +100,52 @@ def test_config2_full_scene_low_res(engine, oracle, rsh, atrium):
     the survivor queue must not grow past its capacity after the wave exits)."""
     cam = rsh.scenes.config2_camera(192, 108)
     run_case(engine, oracle, rsh, atrium, cam, rsh.config.config2(192, 108))
+
+
+@pytest.mark.parametrize("variant", [
+    dict(),
+    dict(algorithm=1),
+    dict(traversal_chunk=1, hit_mode=1),
+    dict(depth=3, branch=4),                 # generic kernel
+    dict(leaf=16, branch=4, bounces=2, bounce_shadows=1),
+])
+def test_traversal_counts_match_oracle(engine, oracle, rsh, variant):
+    """Candidate (top node, triangle) pairs, passing (leaf, triangle) pairs and
+    ray-triangle tests equal the oracle's for every closest-hit pass (the
+    cluster pre-test and the work split must not change the candidate sets).
+    Shadow passes stop early once every ray of a wave is occluded, so their
+    leaf pairs and tests may only be fewer."""
+    scene, cam = c0(rsh, 96)
+    cfg = dataclasses.replace(rsh.config.config0(96, 96), **variant).with_scene_bounds(scene)
+    engine.load_scene(scene)
+    engine.render(cam, cfg, flags=engine.COUNTERS)
+    gp = engine.passes()
+    op = oracle.render(scene, cam, cfg, threads=0, keep_hits=False).passes
+    assert len(gp) == len(op)
+    for g, o in zip(gp, op):
+        assert g["pairs_top"] == o["pairs_top"], (g, o)
+        if g["kind"] == 1:
+            assert g["pairs_leaf"] <= o["pairs_leaf"] and g["ray_tests"] <= o["ray_tests"], (g, o)
+        else:
+            assert (g["pairs_leaf"], g["ray_tests"]) == (o["pairs_leaf"], o["ray_tests"]), (g, o)
+        assert g["tri_tests"] <= g["pairs_top"]
+
+
+def test_config2_counts_match_oracle(engine, oracle, rsh):
+    """Same on an atrium slice, where clusters do cull."""
+    scene = rsh.scenes.config2_scene(grid=3, col_slices=16, col_stacks=12, tiles=8, tile_sub=2, arch_around=6,
+                                     arch_along=8)
+    cam = rsh.scenes.config2_camera(160, 90)
+    cfg = rsh.config.config2(160, 90).with_scene_bounds(scene)
+    engine.load_scene(scene)
+    engine.render(cam, cfg, flags=engine.COUNTERS)
+    gp = engine.passes()
+    op = oracle.render(scene, cam, cfg, threads=0, keep_hits=False).passes
+    for g, o in zip(gp, op):
+        assert g["pairs_top"] == o["pairs_top"]
+        if g["kind"] != 1:
+            assert (g["pairs_leaf"], g["ray_tests"]) == (o["pairs_leaf"], o["ray_tests"])
+    assert sum(g["tri_tests"] for g in gp) < sum(g["pairs_top"] for g in gp)
 
 
 def test_render_twice_is_deterministic(engine, rsh):

@@ -186,7 +186,7 @@ def main():
     }
     eng.close()
     if not a.no_cpu:
-        rows = a.cpu_rows or max(8, a.height // 32)
+        rows = a.cpu_rows or max(8, a.height // 4)     # about 10-15 s of oracle work at 1080p
         out["cpu_baseline"] = cpu_baseline(scene, cam, cfg, rows)
     line = json.dumps(out)
     print(line)

@@ -379,20 +379,27 @@ __global__ void __launch_bounds__(256) k_item_fill(const uint32_t* item_first, c
 //     of its ray against each pending survivor.
 // The set of (leaf, triangle) pairs and ray tests is the reference's; only
 // the order differs, and any-hit / closest-hit are order independent.
+// Four waves per SIMD (the LDS footprint allows four 256-thread blocks per CU):
+// the register budget is capped at 128 VGPRs, which costs a 4-VGPR spill and
+// measures faster than three waves without it.
 template <int LEAF>
-__global__ void __launch_bounds__(256) k_traverse64(TravParams P) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_traverse64(TravParams P) {
     constexpr uint32_t BRANCH = 64 / LEAF;
-    constexpr uint32_t GROUP = 64 / BRANCH;       // survivors per stage-B step
     constexpr uint32_t QCAP = 128;                // survivor queue
-    constexpr uint32_t PCAP = 512;                // (survivor, leaf) pair list
+    constexpr uint32_t PCAP = 512;                // (survivor, leaf) candidate / pair list
     constexpr uint32_t SCH = PCAP / BRANCH < 64 ? PCAP / BRANCH : 64;   // survivors per B+C round
     __shared__ float4 s_qsph[4][QCAP];
     __shared__ uint32_t s_qidx[4][QCAP];
+    __shared__ uint32_t s_qmask[4][QCAP];         // leaves the survivor's cluster may pass
     __shared__ float4 s_mt[4][64][3];
     __shared__ float4 s_ray0[4][64];              // (o + d*eps, tmax)
     __shared__ float4 s_ray1[4][64];              // (d, valid)
     __shared__ unsigned long long s_best[4][64];  // closest record / shadow flag per ray
     __shared__ uint16_t s_pair[4][PCAP];          // survivor << 5 | leaf
+    __shared__ float4 s_lsph[4][BRANCH];          // leaf nodes of the top node
+    __shared__ float4 s_lcone[4][BRANCH];
+    __shared__ float4 s_laux[4][BRANCH];          // (cos, tan, lmax, exists)
+    __shared__ uint8_t s_used[4][64];             // survivor of the round appears in a pair
     const uint32_t lane = lane_id();
     const uint32_t w = threadIdx.x >> 6;
     const uint32_t item = blockIdx.x * 4 + w;
@@ -417,18 +424,20 @@ __global__ void __launch_bounds__(256) k_traverse64(TravParams P) {
     }
     const unsigned long long miss = P.kind == 1 ? 0ull : (P.hit_mode == 0 ? MISS64 : (unsigned long long)MISS32);
     s_best[w][lane] = miss;
+    if (lane < BRANCH) {
+        const uint32_t bl = top * BRANCH + lane;
+        float4 ls = make_float4(0, 0, 0, 0), lc = make_float4(0, 0, 1, 0), la = make_float4(0, 0, 0, 0);
+        if (bl < P.level_count[0]) {
+            ls = P.nodes[2 * (size_t)bl];
+            lc = P.nodes[2 * (size_t)bl + 1];
+            const float2 a = P.aux[bl];
+            la = make_float4(a.x, a.y, node_lmax(ls, lc, a.x, a.y), 1.0f);
+        }
+        s_lsph[w][lane] = ls; s_lcone[w][lane] = lc; s_laux[w][lane] = la;
+    }
     const uint32_t gt = P.level_offset[1] + top;
     const float4 tS = P.nodes[2 * (size_t)gt], tC = P.nodes[2 * (size_t)gt + 1];
     const float2 tA = P.aux[gt];
-    const uint32_t bl = top * BRANCH + (lane % BRANCH);     // stage-B leaf of this lane
-    const bool bl_valid = bl < P.level_count[0];
-    float4 lS = make_float4(0, 0, 0, 0), lC = make_float4(0, 0, 1, 0);
-    float2 lA = make_float2(1, 0);
-    if (bl_valid) {
-        lS = P.nodes[2 * (size_t)bl];
-        lC = P.nodes[2 * (size_t)bl + 1];
-        lA = P.aux[bl];
-    }
     const uint64_t alive = __ballot(valid);
     const bool count = P.counters != nullptr;
     uint64_t n_top = 0, n_surv = 0, n_leaf = 0, n_tests = 0, n_bsteps = 0, n_csteps = 0;
@@ -437,33 +446,49 @@ __global__ void __launch_bounds__(256) k_traverse64(TravParams P) {
     __builtin_amdgcn_wave_barrier();
 
     // ---- stages B + C over the queued survivors, SCH at a time:
-    //  B  (survivor, leaf) tests, lane = survivor g + lane / BRANCH, leaf lane % BRANCH;
-    //     passing pairs are appended to s_pair, and `used` marks survivors in a pair;
+    //  B  the (survivor, leaf) candidates allowed by each survivor's cluster
+    //     leaf mask are listed in s_pair and tested 64 per step; passing pairs
+    //     are compacted in place and their survivors marked used;
     //  load the Moller-Trumbore vectors of the used survivors only;
     //  C  the pair list is walked BRANCH pairs per step, lane = pair p0 + lane / LEAF,
     //     ray lane % LEAF of the pair's leaf; hits merge per ray in LDS.
     auto flush = [&]() {
         for (uint32_t c0 = 0; c0 < qlen && !done; c0 += SCH) {
             const uint32_t n = qlen - c0 < SCH ? qlen - c0 : SCH;
+            uint32_t lm = lane < n ? s_qmask[w][c0 + lane] : 0u;
+            const uint32_t kc = (uint32_t)__popc(lm);
+            const uint32_t incl = wave_incl_add(kc);
+            const uint32_t ncand_ = __shfl(incl, 63, 64);
+            uint32_t o = incl - kc;
+            while (lm) {
+                const uint32_t b = (uint32_t)__builtin_ctz(lm);
+                lm &= lm - 1u;
+                s_pair[w][o++] = (uint16_t)((lane << 5) | b);
+            }
+            s_used[w][lane] = 0;
+            __builtin_amdgcn_wave_barrier();
             uint32_t np_ = 0;
-            uint64_t used = 0;
-            for (uint32_t g = 0; g < n; g += GROUP) {
-                const uint32_t sv = g + lane / BRANCH;
+            for (uint32_t c = 0; c < ncand_; c += 64) {
+                const uint32_t ci = c + lane;
                 bool lp = false;
-                if (sv < n && bl_valid) lp = sphere_node(lS, lC, s_qsph[w][c0 + sv], lA.x, lA.y);
+                uint32_t e = 0;
+                if (ci < ncand_) {
+                    e = s_pair[w][ci];
+                    const uint32_t b = e & 31u;
+                    const float4 la = s_laux[w][b];
+                    lp = sphere_node(s_lsph[w][b], s_lcone[w][b], s_qsph[w][c0 + (e >> 5)], la.x, la.y);
+                }
                 const uint64_t pm = __ballot(lp);
-                if (lp) {
-                    const uint32_t r = np_ + (uint32_t)__popcll(pm & ((1ull << lane) - 1ull));
-                    s_pair[w][r] = (uint16_t)((sv << 5) | (lane % BRANCH));
+                if (lp) {     // in place: entries c.. were read above, writes land below c + 64
+                    s_pair[w][np_ + (uint32_t)__popcll(pm & ((1ull << lane) - 1ull))] = (uint16_t)e;
+                    s_used[w][e >> 5] = 1;
                 }
                 np_ += (uint32_t)__popcll(pm);
-#pragma unroll
-                for (uint32_t j = 0; j < GROUP; j++)
-                    used |= (uint64_t)(((pm >> (j * BRANCH)) & ((1ull << BRANCH) - 1ull)) != 0ull) << (g + j);
                 if (count) { n_leaf += (uint64_t)__popcll(pm); n_bsteps++; }
             }
+            __builtin_amdgcn_wave_barrier();
             if (np_ == 0) continue;
-            if (lane < n && ((used >> lane) & 1ull)) {
+            if (lane < n && s_used[w][lane]) {
                 size_t t = s_qidx[w][c0 + lane];
                 if (t >= P.T) t = 0;     // queue entries are < T by construction
                 s_mt[w][lane][0] = P.tri_mt[3 * t];
@@ -487,7 +512,8 @@ __global__ void __launch_bounds__(256) k_traverse64(TravParams P) {
                         if (P.kind == 1) {
                             if (th > EPS && th < r0.w) s_best[w][ray] = 1ull;
                         } else if (th > EPS) {
-                            atomicMin(&s_best[w][ray], pack_hit(th, s_qidx[w][c0 + sv], P.hit_mode));
+                            const unsigned long long rec = pack_hit(th, s_qidx[w][c0 + sv], P.hit_mode);
+                            if (rec < s_best[w][ray]) atomicMin(&s_best[w][ray], rec);
                         }
                     }
                 }
@@ -507,7 +533,7 @@ __global__ void __launch_bounds__(256) k_traverse64(TravParams P) {
     const uint32_t s1 = ncand - s0 < P.chunk ? ncand : s0 + P.chunk;
     const size_t lbase = rah ? 0 : P.obj_off[top];
     const uint32_t np = rah ? P.K : P.obj_off[top + 1] - P.obj_off[top];
-    const NodeCull nc = node_cull(tS, tC, tA.x, tA.y);
+    const float t_lmax = node_lmax(tS, tC, tA.x, tA.y);
     uint64_t n_cl = 0;
     uint32_t rem = s1 > s0 ? s1 - s0 : 0;     // clusters of the chunk not yet taken from an entry
     uint32_t j = 0;                            // current entry of the object list
@@ -541,9 +567,9 @@ __global__ void __launch_bounds__(256) k_traverse64(TravParams P) {
         j = __builtin_amdgcn_readfirstlane(j0);
         load_entry(j, s0 - P.obj_cum[lbase + j]);
     }
-    // Next block of one cluster's triangles [tb, tl] that passed the cluster
-    // pre-test; clusters are pre-tested 64 at a time, one per lane.
-    auto next_block = [&](uint32_t& tb, uint32_t& tl) -> bool {
+    // Next block of one cluster's triangles [tb, tl] (cluster cid) that passed
+    // the cluster pre-test; clusters are pre-tested 64 at a time, one per lane.
+    auto next_block = [&](uint32_t& tb, uint32_t& tl, uint32_t& cid) -> bool {
         while (gmask == 0ull) {
             if (lc_cur >= lc_end) {
                 if (rem == 0 || j + 1 >= np) return false;
@@ -556,12 +582,13 @@ __global__ void __launch_bounds__(256) k_traverse64(TravParams P) {
             lc_cur += n;
             if (e_ncl == 1) { gmask = 1ull; break; }   // the object test already stands for it
             bool cp = false;
-            if (lane < n) cp = cluster_may_pass(nc, P.cl_sphere[e_cbase + g0 + lane]);
+            if (lane < n) cp = cluster_may_pass(tS, tC, tA.x, tA.y, t_lmax, P.cl_sphere[e_cbase + g0 + lane]);
             gmask = __ballot(cp);
             if (count) n_cl += n;
         }
         const uint32_t i = (uint32_t)__builtin_ctzll(gmask);
         gmask &= gmask - 1ull;
+        cid = e_cbase + g0 + i;
         tb = e_first + (g0 + i) * CLUSTER;
         tl = tb + (CLUSTER - 1) < e_last ? tb + (CLUSTER - 1) : e_last;
         return true;
@@ -572,16 +599,17 @@ __global__ void __launch_bounds__(256) k_traverse64(TravParams P) {
     constexpr int NB = 4;
     bool have = rem != 0 || lc_cur < lc_end;
     while (have && !done) {
-        uint32_t tbk[NB];
+        uint32_t tbk[NB], cidk[NB];
         bool ink[NB];
         float4 tsk[NB];
 #pragma unroll
         for (int k = 0; k < NB; k++) {
-            uint32_t tb = 0, tl = 0;
-            have = have && next_block(tb, tl);
+            uint32_t tb = 0, tl = 0, cid = 0;
+            have = have && next_block(tb, tl, cid);
             const uint32_t t = tb + lane;
             ink[k] = have && t <= tl && !(t >= e_exlo && t < e_exhi);
             tbk[k] = tb;
+            cidk[k] = cid;
             tsk[k] = make_float4(0, 0, 0, 0);
             if (ink[k]) tsk[k] = P.tri_sphere[t];
         }
@@ -593,19 +621,29 @@ __global__ void __launch_bounds__(256) k_traverse64(TravParams P) {
             if (ink[k]) p = sphere_node(tS, tC, tsk[k], tA.x, tA.y);
             const uint64_t m = __ballot(p);
             if (count) { n_top += (uint64_t)__popcll(__ballot(ink[k])); n_surv += (uint64_t)__popcll(m); }
-            if (m) {
-                const uint32_t add = (uint32_t)__popcll(m);
-                if (qlen + add > QCAP) flush();   // never taken: qlen < 64 here
-                if (done) break;
-                if (p) {
-                    const uint32_t r = qlen + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-                    s_qsph[w][r] = tsk[k];
-                    s_qidx[w][r] = tbk[k] + lane;
-                }
-                qlen += add;
-                __builtin_amdgcn_wave_barrier();
-                if (qlen >= 64) flush();
+            if (m == 0ull) continue;
+            // leaves the block's cluster may pass (conservative, as the cluster pre-test);
+            // a cluster that no leaf may pass contributes no pair
+            bool lpass = false;
+            if (lane < BRANCH) {
+                const float4 la = s_laux[w][lane];
+                lpass = la.w != 0.0f &&
+                        cluster_may_pass(s_lsph[w][lane], s_lcone[w][lane], la.x, la.y, la.z, P.cl_sphere[cidk[k]]);
             }
+            const uint32_t cm = (uint32_t)__ballot(lpass);
+            if (cm == 0u) continue;
+            const uint32_t add = (uint32_t)__popcll(m);
+            if (qlen + add > QCAP) flush();   // never taken: qlen < 64 here
+            if (done) break;
+            if (p) {
+                const uint32_t r = qlen + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                s_qsph[w][r] = tsk[k];
+                s_qidx[w][r] = tbk[k] + lane;
+                s_qmask[w][r] = cm;
+            }
+            qlen += add;
+            __builtin_amdgcn_wave_barrier();
+            if (qlen >= 64) flush();
         }
     }
     if (qlen && !done) flush();

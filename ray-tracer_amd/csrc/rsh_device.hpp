@@ -254,33 +254,29 @@ __device__ __forceinline__ bool sphere_node(float4 sphere, float4 cone, float4 t
 // (the FP32 rounding of either evaluation is ~1e-6 of them).  Nodes outside
 // the argument's assumptions (the HALF_PI shortcut, cos <= 0, tan < 0,
 // non-finite values, |cd| far from 1) never reject; a NaN anywhere passes.
-struct NodeCull {
-    v3 c, cd;
-    float r, cosv, tanv, lmax;
-    bool ok;
-};
-
-__device__ __forceinline__ NodeCull node_cull(float4 sphere, float4 cone, float cosine, float tangent) {
-    NodeCull n;
-    n.c = xyz(sphere); n.cd = xyz(cone); n.r = sphere.w; n.cosv = cosine; n.tanv = tangent;
-    const float k = len(n.cd);
+// node_lmax returns max(L, 1/cos) (slightly inflated), or -1 for a node the
+// argument does not cover; cluster_may_pass then never rejects.
+__device__ __forceinline__ float node_lmax(float4 sphere, float4 cone, float cosine, float tangent) {
+    const float k = len(xyz(cone));
     const float tk = tangent * k;
     const float L = sqrtf(1.0f + tk * tk + 2.0f * tk * fabsf(1.0f - k * k));
-    n.lmax = fmaxf(L, 1.0f / cosine) * 1.00001f;
-    n.ok = cone.w != HALF_PI_F && cosine > 0.0f && tangent >= 0.0f && k > 0.5f && k < 1.4f &&
-           isfinite(cosine) && isfinite(tangent) && isfinite(n.lmax) && isfinite(sphere.w) &&
-           isfinite(n.c.x) && isfinite(n.c.y) && isfinite(n.c.z);
-    return n;
+    const float lmax = fmaxf(L, 1.0f / cosine) * 1.00001f;
+    const bool ok = cone.w != HALF_PI_F && cosine > 0.0f && tangent >= 0.0f && k > 0.5f && k < 1.4f &&
+                    isfinite(cosine) && isfinite(tangent) && isfinite(lmax) && isfinite(sphere.w) &&
+                    isfinite(sphere.x) && isfinite(sphere.y) && isfinite(sphere.z);
+    return ok ? lmax : -1.0f;
 }
 
-__device__ __forceinline__ bool cluster_may_pass(const NodeCull& n, float4 cl) {
-    if (!n.ok) return true;
-    const v3 s2q = xyz(cl) - n.c;
-    const float pr = dot(s2q, n.cd);
-    const float f = len(s2q - pr * n.cd) - pr * n.tanv;
+__device__ __forceinline__ bool cluster_may_pass(float4 sphere, float4 cone, float cosine, float tangent, float lmax,
+                                                 float4 cl) {
+    if (!(lmax > 0.0f)) return true;
+    const v3 cd = xyz(cone);
+    const v3 s2q = xyz(cl) - xyz(sphere);
+    const float pr = dot(s2q, cd);
+    const float f = len(s2q - pr * cd) - pr * tangent;
     const float rho = cl.w;
-    const float allow = n.r / n.cosv + rho * n.lmax;
-    const float E = 1e-4f * (len(s2q) + rho + (fabsf(pr) + rho) * n.tanv + (n.r + rho) * n.lmax) + 1e-4f;
+    const float allow = sphere.w / cosine + rho * lmax;
+    const float E = 1e-4f * (len(s2q) + rho + (fabsf(pr) + rho) * tangent + (sphere.w + rho) * lmax) + 1e-4f;
     return !(f > allow + E);
 }
 

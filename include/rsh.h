@@ -83,7 +83,7 @@ typedef struct rsh_frame_config {
     uint32_t key_layout;            /* 0 type|cell|dir, 1 type|dir|cell                  */
     float scene_min[3];
     float cell_scale[3];            /* cells per world unit                              */
-    uint32_t segment_shift;         /* reserved: segment id position (batched frames)    */
+    uint32_t segment_shift;         /* batches: key bit of the view id (0 = not in key) */
     float initial_radius, initial_spread;
     uint32_t traversal_kernel;      /* 0 automatic, 1 generic kernel for every shape     */
     uint32_t traversal_splits;      /* generic kernel: waves per top node; 0 = fill the GPU */
@@ -115,6 +115,7 @@ typedef struct rsh_frame_stats {
     uint64_t pass_cluster_tests[RSH_MAX_PASSES]; /* cluster pre-tests run (bit 1)            */
     uint64_t pass_leaf_steps[RSH_MAX_PASSES];    /* 64-ray kernel: wave steps of leaf tests (bit 1) */
     uint64_t pass_ray_steps[RSH_MAX_PASSES];     /* 64-ray kernel: wave steps of ray tests (bit 1)  */
+    uint32_t graph_replayed;                     /* 1: the call replayed a captured graph (bit 3) */
 } rsh_frame_stats;
 
 typedef struct rsh_engine rsh_engine;
@@ -131,9 +132,34 @@ rsh_status rsh_engine_load_scene(rsh_engine* e, const rsh_scene_desc* scene);
 /* Render one frame into the engine's pixel buffer.  `flags`:
  *   bit 0  keep every pass's per-slot hit records for rsh_engine_read_pass
  *   bit 1  collect traversal counters (pairs / tests) into the stats
- *   bit 2  time stages with HIP events into stats->stage_ms */
+ *   bit 2  time stages with HIP events into stats->stage_ms
+ *   bit 3  graph replay (alone, without bits 0-2): the first call for a given
+ *          set of views, configuration and scene renders eagerly and captures
+ *          a HIP graph of the frame (every pass's sizes fixed to the eager
+ *          frame's); later identical calls replay that graph with no host
+ *          read-backs.  The graph is dropped when any device buffer of the
+ *          library is reallocated, the scene is reloaded or the inputs change.
+ *          Needs a non-null stream (the engine's own by default). */
 rsh_status rsh_engine_render(rsh_engine* e, const rsh_camera* cam, const rsh_frame_config* cfg,
                              uint32_t flags, rsh_frame_stats* stats);
+
+/* One view of a batch. */
+typedef struct rsh_view {
+    rsh_camera camera;
+    uint32_t width, height;
+} rsh_view;
+
+/* Render n views as one batch: every pass traces the rays of all views
+ * together (one trim, one sort, one hierarchy, one traversal launch).  The
+ * pixels of view i follow those of views 0..i-1 (rsh_engine_read_pixels
+ * reads sum(w*h) values).  cfg->width / height are ignored.  With
+ * cfg->segment_shift = s > 0 the view id is placed in key bits s..31 (the
+ * octahedral key must fit below s, and n <= 2^(32-s)), so the sort orders
+ * rays by view first and hierarchy nodes rarely mix views.  Flags as for
+ * rsh_engine_render.  Replaces running the frame driver once per view
+ * (main.cpp:1234-1303). */
+rsh_status rsh_engine_render_views(rsh_engine* e, const rsh_view* views, uint32_t n,
+                                   const rsh_frame_config* cfg, uint32_t flags, rsh_frame_stats* stats);
 
 rsh_status rsh_engine_read_pixels(rsh_engine* e, uint32_t* host_rgba, size_t count);
 /* Primary G-buffer: diffuse, specular, position, normal, each float4[W*H]. */

@@ -86,6 +86,15 @@ typedef struct oracle_frame_out {
 int oracle_render(const oracle_scene* scene, const oracle_camera* camera,
                   const oracle_config* config, const float* gbuffer_in /* 4*W*H*4 or NULL */,
                   oracle_frame_out* out, int threads);
+// A batch of views rendered as one image of sum(w*h) pixels (view i after
+// views 0..i-1); config->width / height are ignored.  The view id is placed
+// in the octahedral key at config->segment_shift when that is non-zero.
+typedef struct oracle_view {
+    oracle_camera camera;
+    uint32_t width, height;
+} oracle_view;
+int oracle_render_views(const oracle_scene* scene, const oracle_view* views, uint32_t n,
+                        const oracle_config* config, oracle_frame_out* out, int threads);
 
 // Math and keys.
 float oracle_acos(float x);

@@ -29,6 +29,10 @@ class OCamera(C.Structure):
     _fields_ = [("eye", f32 * 3), ("direction", f32 * 3), ("right", f32 * 3), ("up", f32 * 3)]
 
 
+class OView(C.Structure):
+    _fields_ = [("camera", OCamera), ("width", u32), ("height", u32)]
+
+
 class OConfig(C.Structure):
     _fields_ = [("width", u32), ("height", u32), ("algorithm", u32), ("key_mode", u32),
                 ("hit_mode", u32), ("cone_mode", u32), ("leaf", u32), ("branch", u32),
@@ -88,6 +92,9 @@ def lib():
         L.oracle_render.restype = C.c_int
         L.oracle_render.argtypes = [C.POINTER(OScene), C.POINTER(OCamera), C.POINTER(OConfig), PF,
                                     C.POINTER(OFrameOut), C.c_int]
+        L.oracle_render_views.restype = C.c_int
+        L.oracle_render_views.argtypes = [C.POINTER(OScene), C.POINTER(OView), u32, C.POINTER(OConfig),
+                                          C.POINTER(OFrameOut), C.c_int]
         for n in ("oracle_leaf_pairs_literal", "oracle_leaf_pairs_closed"):
             getattr(L, n).restype = u64
             getattr(L, n).argtypes = [PF, PU, u32, u32, C.POINTER(OScene), u32, u32, PU64, u64]
@@ -208,6 +215,46 @@ def render(scene, camera, cfg, gbuffer=None, threads=0, keep_hits=True, keep_gbu
                          C.byref(out), threads)
     if rc != 0:
         raise RuntimeError("oracle_render failed: %d" % rc)
+    passes = []
+    for i in range(out.pass_count):
+        passes.append(dict(kind=out.pass_kind[i], slots=out.pass_slots[i], rays=out.pass_rays[i],
+                           chunks=out.pass_chunks[i], pairs_top=out.pass_pairs_top[i],
+                           pairs_leaf=out.pass_pairs_leaf[i], ray_tests=out.pass_ray_tests[i],
+                           hits=hits[i][:out.pass_slots[i]].copy() if keep_hits else None))
+    return Frame(pixels.reshape(H, W), gb, passes)
+
+
+def render_views(scene, views, cfg, threads=0, keep_hits=True, keep_gbuffer=True) -> Frame:
+    """A batch of views, each (camera, width, height), rendered as one image of
+    sum(w*h) pixels (returned as a 1 x P frame; a single view keeps H x W)."""
+    L = lib()
+    n = len(views)
+    P = sum(int(w) * int(h) for _, w, h in views)
+    W, H = (int(views[0][1]), int(views[0][2])) if n == 1 else (P, 1)
+    S = P * max(1, scene.light_pos.shape[0])
+    pixels = np.zeros(P, np.uint32)
+    out = OFrameOut()
+    out.pixels = up(pixels)
+    gb = None
+    if keep_gbuffer:
+        gb = {k: np.zeros((P, 4), np.float32) for k in ("pos", "nrm", "dif", "spec")}
+        out.gbuffer_pos, out.gbuffer_nrm = fp(gb["pos"]), fp(gb["nrm"])
+        out.gbuffer_dif, out.gbuffer_spec = fp(gb["dif"]), fp(gb["spec"])
+    hits = []
+    if keep_hits:
+        for i in range(16):
+            h = np.zeros(S, np.uint64)
+            hits.append(h)
+            out.pass_hits[i] = u64p(h)
+    ov = (OView * n)()
+    for i, (cam, w, h) in enumerate(views):
+        ov[i].camera = make_camera(cam)
+        ov[i].width, ov[i].height = int(w), int(h)
+    os_ = make_scene(scene)
+    ocfg = make_config(cfg)
+    rc = L.oracle_render_views(C.byref(os_), ov, n, C.byref(ocfg), C.byref(out), threads)
+    if rc != 0:
+        raise RuntimeError("oracle_render_views failed: %d" % rc)
     passes = []
     for i in range(out.pass_count):
         passes.append(dict(kind=out.pass_kind[i], slots=out.pass_slots[i], rays=out.pass_rays[i],

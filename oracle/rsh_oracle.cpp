@@ -485,9 +485,33 @@ void trace(const Scene& sc, const oracle_config& cfg, const Batch& b, uint32_t k
     st.top = top; st.leaf = lf; st.tests = tests;
 }
 
+// ---------------------------------------------------------- views --------
+// Views of a batch laid out as one image of P = sum(w*h) pixels (view i from
+// pixel off[i]); a single frame is one view of W x H.  The view id goes into
+// the octahedral key at cfg.segment_shift when that is set.
+struct ViewSet {
+    std::vector<uint32_t> off, w, h;
+    std::vector<f3> eye, D, R, U;
+    uint32_t view_of(uint32_t p) const {
+        uint32_t a = 0, b = (uint32_t)off.size();
+        while (b - a > 1) {
+            uint32_t m = (a + b) >> 1;
+            if (off[m] <= p) a = m; else b = m;
+        }
+        return a;
+    }
+};
+
+uint32_t view_key(const oracle_config& cfg, f3 o, f3 d, uint32_t type, uint32_t view) {
+    uint32_t k = key_oct(cfg, o, d, type);
+    if (cfg.segment_shift) k |= view << cfg.segment_shift;
+    return k;
+}
+
 // ---------------------------------------------------- ray creation --------
 // RT:682-740 CreateShadowRays over slots i = x + W*(l + L*y).
-void create_shadow(const Scene& sc, const oracle_config& cfg, const GBuf& g, uint32_t Lmax, Batch& b) {
+void create_shadow(const Scene& sc, const oracle_config& cfg, const GBuf& g, uint32_t Lmax, const ViewSet& vs,
+                   Batch& b) {
     const uint32_t W = g.W, H = g.H;
     b.init(W * H * Lmax);
 #pragma omp parallel for schedule(static)
@@ -504,7 +528,8 @@ void create_shadow(const Scene& sc, const oracle_config& cfg, const GBuf& g, uin
             if (diffuse > 0.0f) {
                 b.rays[2 * (size_t)i] = o;
                 b.rays[2 * (size_t)i + 1] = d;
-                b.keys[i] = cfg.key_mode == 0 ? shadow_key_ref(o, d, l) : key_oct(cfg, o, d, 1);
+                b.keys[i] = cfg.key_mode == 0 ? shadow_key_ref(o, d, l)
+                                              : view_key(cfg, o, d, 1, vs.view_of(x + y * W));
                 b.values[i] = i;
                 b.flags[i] = 0;
             }
@@ -514,7 +539,7 @@ void create_shadow(const Scene& sc, const oracle_config& cfg, const GBuf& g, uin
 
 // RT:742-791 CreateReflectionRays, slot x + y*W*L; gap slots (L > 1) are
 // marked empty.  With reflect_gate only materials whose diffuse.w > 0 reflect.
-void create_reflection(const oracle_config& cfg, const GBuf& g, f3 eye, uint32_t Lmax, Batch& b) {
+void create_reflection(const oracle_config& cfg, const GBuf& g, const ViewSet& vs, uint32_t Lmax, Batch& b) {
     const uint32_t W = g.W, H = g.H;
     b.init(W * H * Lmax);
 #pragma omp parallel for schedule(static)
@@ -525,10 +550,11 @@ void create_reflection(const oracle_config& cfg, const GBuf& g, f3 eye, uint32_t
         f3 N = normalize(xyz(g.nrm[pp]));
         bool live = length(P) != 0.0f && (!cfg.reflect_gate || g.dif[pp].w > 0.0f);
         if (live) {
-            f3 d = reflect(normalize(P - eye), normalize(N));
+            const uint32_t v = vs.view_of((uint32_t)pp);
+            f3 d = reflect(normalize(P - vs.eye[v]), normalize(N));
             b.rays[2 * (size_t)slot] = P;
             b.rays[2 * (size_t)slot + 1] = d;
-            b.keys[slot] = cfg.key_mode == 0 ? reflection_key_ref(P, d) : key_oct(cfg, P, d, 2);
+            b.keys[slot] = cfg.key_mode == 0 ? reflection_key_ref(P, d) : view_key(cfg, P, d, 2, v);
             b.values[slot] = slot;
             b.flags[slot] = 0;
         }
@@ -538,7 +564,7 @@ void create_reflection(const oracle_config& cfg, const GBuf& g, f3 eye, uint32_t
 // Next bounce (RT:1995-2010): origin P, direction reflect(d, N) with the
 // barycentric (unnormalised) normal; gated by material reflectivity.
 void create_bounce(const oracle_config& cfg, const GBuf& g, const Batch& prev,
-                   const std::vector<uint64_t>& rec, uint32_t Lmax, Batch& b) {
+                   const std::vector<uint64_t>& rec, uint32_t Lmax, const ViewSet& vs, Batch& b) {
     const uint32_t W = g.W, H = g.H;
     b.init(W * H * Lmax);
 #pragma omp parallel for schedule(static)
@@ -552,26 +578,29 @@ void create_bounce(const oracle_config& cfg, const GBuf& g, const Batch& prev,
         f3 d = reflect(prev.rays[2 * (size_t)slot + 1], N);
         b.rays[2 * (size_t)slot] = P;
         b.rays[2 * (size_t)slot + 1] = d;
-        b.keys[slot] = cfg.key_mode == 0 ? reflection_key_ref(P, d) : key_oct(cfg, P, d, 2);
+        b.keys[slot] = cfg.key_mode == 0 ? reflection_key_ref(P, d) : view_key(cfg, P, d, 2, vs.view_of((uint32_t)pp));
         b.values[slot] = slot;
         b.flags[slot] = 0;
     }
 }
 
-// Primary rays (configuration stage; formula of RT:667).
-void create_primary(const oracle_config& cfg, const oracle_camera& cam, Batch& b) {
-    const uint32_t W = cfg.width, H = cfg.height;
-    b.init(W * H);
-    f3 eye = ld3(cam.eye), D = ld3(cam.direction), R = ld3(cam.right), U = ld3(cam.up);
+// Primary rays (configuration stage; formula of RT:667), per view: pixel p of
+// view v is (x, y) within it, dir = D + R*(x/w - 0.5) + U*(y/h - 0.5).
+void create_primary(const oracle_config& cfg, const ViewSet& vs, Batch& b) {
+    const uint32_t P = cfg.width * cfg.height;
+    b.init(P);
 #pragma omp parallel for schedule(static)
-    for (int64_t pp = 0; pp < (int64_t)W * H; pp++) {
-        uint32_t x = (uint32_t)(pp % W), y = (uint32_t)(pp / W);
-        float a = (float)x / (float)W - 0.5f;
-        float c = (float)y / (float)H - 0.5f;
-        f3 d = normalize((D + R * a) + U * c);
+    for (int64_t pp = 0; pp < (int64_t)P; pp++) {
+        const uint32_t v = vs.view_of((uint32_t)pp);
+        const uint32_t q = (uint32_t)pp - vs.off[v];
+        uint32_t x = q % vs.w[v], y = q / vs.w[v];
+        float a = (float)x / (float)vs.w[v] - 0.5f;
+        float c = (float)y / (float)vs.h[v] - 0.5f;
+        const f3 eye = vs.eye[v];
+        f3 d = normalize((vs.D[v] + vs.R[v] * a) + vs.U[v] * c);
         b.rays[2 * pp] = eye;
         b.rays[2 * pp + 1] = d;
-        b.keys[pp] = cfg.key_mode == 0 ? reflection_key_ref(eye, d) : key_oct(cfg, eye, d, 0);
+        b.keys[pp] = cfg.key_mode == 0 ? reflection_key_ref(eye, d) : view_key(cfg, eye, d, 0, v);
         b.values[pp] = (uint32_t)pp;
         // Optional row band (a bounded sample of the frame for timing).
         bool in_band = cfg.row_end == 0 || (y >= cfg.row_begin && y < cfg.row_end);
@@ -639,7 +668,7 @@ inline f3 light_term(const oracle_scene* s, uint32_t l, f3 P, f3 N, f3 eye, f4 K
 
 // RT:1616-1709 ColorPrimaryShadowRay
 void shade_primary(const Scene& sc, const GBuf& g, const std::vector<uint64_t>& sflags, uint32_t Lmax,
-                   uint32_t Lt, f3 eye, uint32_t* pixels) {
+                   uint32_t Lt, const ViewSet& vs, uint32_t* pixels) {
     const uint32_t W = g.W, H = g.H;
 #pragma omp parallel for schedule(static)
     for (int64_t pp = 0; pp < (int64_t)W * H; pp++) {
@@ -647,6 +676,7 @@ void shade_primary(const Scene& sc, const GBuf& g, const std::vector<uint64_t>& 
         f3 color{0, 0, 0};
         f3 P = xyz(g.pos[pp]), N = xyz(g.nrm[pp]);
         if (length(P) != 0.0f) {
+            const f3 eye = vs.eye[vs.view_of((uint32_t)pp)];
             for (uint32_t l = 0; l < Lt; l++) {
                 uint64_t rec = sflags[(size_t)(x + y * W * Lmax) + (size_t)W * l];
                 if (rec != MISS32)
@@ -661,13 +691,14 @@ void shade_primary(const Scene& sc, const GBuf& g, const std::vector<uint64_t>& 
 // bounce shadows a light whose shadow flag is set is skipped.
 void shade_reflection(const Scene& sc, const oracle_config& cfg, const GBuf& g,
                       const std::vector<uint64_t>& rec, const std::vector<uint64_t>* sflags,
-                      uint32_t Lmax, uint32_t Lt, f3 eye, uint32_t* pixels) {
+                      uint32_t Lmax, uint32_t Lt, const ViewSet& vs, uint32_t* pixels) {
     const uint32_t W = g.W, H = g.H;
 #pragma omp parallel for schedule(static)
     for (int64_t pp = 0; pp < (int64_t)W * H; pp++) {
         uint32_t x = (uint32_t)(pp % W), y = (uint32_t)(pp / W);
         uint32_t slot = x + y * W * Lmax;
         if (is_miss(rec[slot], cfg.hit_mode)) continue;
+        const f3 eye = vs.eye[vs.view_of((uint32_t)pp)];
         f3 P = xyz(g.pos[pp]), N = xyz(g.nrm[pp]);
         f4 Kd = g.dif[pp], Ks = g.spec[pp];
         f3 color{0, 0, 0};
@@ -824,24 +855,28 @@ void oracle_cone0(const float* cone, const float* ray, uint32_t mode, float* out
 void oracle_coneN(const float* c1, const float* c2, float* out4) { st4(out4, coneN(ld4(c1), ld4(c2), 0)); }
 void oracle_sphere_union(const float* s1, const float* s2, float* out4) { st4(out4, sphere_union(ld4(s1), ld4(s2))); }
 
-int oracle_render(const oracle_scene* scene, const oracle_camera* camera, const oracle_config* config,
-                  const float* gbuffer_in, oracle_frame_out* out, int threads) {
+}  // extern "C"
+
+namespace {
+
+// The frame driver (MAIN:630-1054, 1234-1303) over a view set; cfg.width /
+// height give the image (a batch: P x 1).
+int render_views(const oracle_scene* scene, const ViewSet& vs, const oracle_config& cfg, const float* gbuffer_in,
+                 oracle_frame_out* out, int threads) {
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);
 #else
     (void)threads;
 #endif
-    const oracle_config& cfg = *config;
     if (cfg.leaf == 0 || cfg.branch == 0 || cfg.depth == 0 || cfg.width == 0 || cfg.height == 0) return -1;
     Scene sc(scene);
     const uint32_t W = cfg.width, H = cfg.height, Lt = scene->light_total, Lmax = Lt;
-    const f3 eye = ld3(camera->eye);
     if (out) out->pass_count = 0;
     std::vector<uint32_t> pixels((size_t)W * H, 0);
     GBuf g0;
     if (cfg.primary_rays) {
         Batch b;
-        create_primary(cfg, *camera, b);
+        create_primary(cfg, vs, b);
         std::vector<uint64_t> rec;
         PassStats st;
         trace(sc, cfg, b, 0, nullptr, 1, rec, st, threads);
@@ -865,20 +900,20 @@ int oracle_render(const oracle_scene* scene, const oracle_camera* camera, const 
     }
     {
         Batch b;
-        create_shadow(sc, cfg, g0, Lmax, b);
+        create_shadow(sc, cfg, g0, Lmax, vs, b);
         std::vector<uint64_t> flags;
         PassStats st;
         trace(sc, cfg, b, 1, &g0, Lmax, flags, st, threads);
         record_pass(out, 1, b.S, st, flags);
-        if (st.N > 0) shade_primary(sc, g0, flags, Lmax, Lt, eye, pixels.data());
+        if (st.N > 0) shade_primary(sc, g0, flags, Lmax, Lt, vs, pixels.data());
     }
     Batch prev;
     GBuf gprev;
     std::vector<uint64_t> prev_rec;
     for (uint32_t bounce = 1; bounce <= cfg.bounces; bounce++) {
         Batch b;
-        if (bounce == 1) create_reflection(cfg, g0, eye, Lmax, b);
-        else create_bounce(cfg, gprev, prev, prev_rec, Lmax, b);
+        if (bounce == 1) create_reflection(cfg, g0, vs, Lmax, b);
+        else create_bounce(cfg, gprev, prev, prev_rec, Lmax, vs, b);
         std::vector<uint64_t> rec;
         PassStats st;
         trace(sc, cfg, b, 0, nullptr, Lmax, rec, st, threads);
@@ -890,19 +925,53 @@ int oracle_render(const oracle_scene* scene, const oracle_camera* camera, const 
         bool have_flags = false;
         if (cfg.bounce_shadows) {
             Batch sbt;
-            create_shadow(sc, cfg, gb, Lmax, sbt);
+            create_shadow(sc, cfg, gb, Lmax, vs, sbt);
             PassStats sst;
             trace(sc, cfg, sbt, 1, &gb, Lmax, sflags, sst, threads);
             record_pass(out, 1, sbt.S, sst, sflags);
             have_flags = true;
         }
-        shade_reflection(sc, cfg, gb, rec, have_flags ? &sflags : nullptr, Lmax, Lt, eye, pixels.data());
+        shade_reflection(sc, cfg, gb, rec, have_flags ? &sflags : nullptr, Lmax, Lt, vs, pixels.data());
         prev = std::move(b);
         gprev = std::move(gb);
         prev_rec = std::move(rec);
     }
     if (out && out->pixels) std::copy(pixels.begin(), pixels.end(), out->pixels);
     return 0;
+}
+
+void add_view(ViewSet& vs, const oracle_camera& c, uint32_t off, uint32_t w, uint32_t h) {
+    vs.off.push_back(off); vs.w.push_back(w); vs.h.push_back(h);
+    vs.eye.push_back(ld3(c.eye)); vs.D.push_back(ld3(c.direction));
+    vs.R.push_back(ld3(c.right)); vs.U.push_back(ld3(c.up));
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_render(const oracle_scene* scene, const oracle_camera* camera, const oracle_config* config,
+                  const float* gbuffer_in, oracle_frame_out* out, int threads) {
+    ViewSet vs;
+    add_view(vs, *camera, 0, config->width, config->height);
+    return render_views(scene, vs, *config, gbuffer_in, out, threads);
+}
+
+int oracle_render_views(const oracle_scene* scene, const oracle_view* views, uint32_t n,
+                        const oracle_config* config, oracle_frame_out* out, int threads) {
+    if (n == 0) return -1;
+    ViewSet vs;
+    uint64_t off = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        add_view(vs, views[i].camera, (uint32_t)off, views[i].width, views[i].height);
+        off += (uint64_t)views[i].width * views[i].height;
+    }
+    if (off == 0 || off > 0xFFFFFFFFull) return -1;
+    oracle_config cfg = *config;
+    cfg.width = n == 1 ? views[0].width : (uint32_t)off;
+    cfg.height = n == 1 ? views[0].height : 1u;
+    cfg.row_begin = cfg.row_end = 0;
+    return render_views(scene, vs, cfg, nullptr, out, threads);
 }
 
 uint64_t oracle_leaf_pairs_literal(const float* hierarchy, const uint32_t* level_counts, uint32_t depth,

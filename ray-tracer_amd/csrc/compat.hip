@@ -182,8 +182,8 @@ void ShadowRayCreationWrapper(const unsigned int W, const unsigned int H, const 
         raise(RSH_INVALID_ARG, "ShadowRayCreationWrapper: lightTotal exceeds the bound light count");
         return;
     }
-    launch_create_shadow(c.g_pos, c.g_nrm, W, H, L, c.light_pos, ref_keys(), reinterpret_cast<float*>(rays), flags,
-                         keys, values, c.stream);
+    launch_create_shadow(c.g_pos, c.g_nrm, W, H, L, c.light_pos, ref_keys(), single_view(W, H, v3{0, 0, 0}),
+                         reinterpret_cast<float*>(rays), flags, keys, values, c.stream);
     check(hipGetLastError(), "ShadowRayCreationWrapper");
 }
 
@@ -191,8 +191,9 @@ void ReflectionRayCreationWrapper(const unsigned int W, const unsigned int H, co
                                   unsigned int* flags, unsigned int* keys, unsigned int* values) {
     Compat& c = C();
     if (!need(c.g_pos, "bindFragmentPositionArray") || !need(c.g_nrm, "bindFragmentNormalArray")) return;
-    launch_create_reflection(c.g_pos, c.g_nrm, c.g_dif ? c.g_dif : c.g_pos, W, H, c.light_max, V(eye), 0, 0,
-                             ref_keys(), reinterpret_cast<float*>(rays), flags, keys, values, c.stream);
+    launch_create_reflection(c.g_pos, c.g_nrm, c.g_dif ? c.g_dif : c.g_pos, W, H, c.light_max,
+                             single_view(W, H, V(eye)), 0, 0, ref_keys(), reinterpret_cast<float*>(rays), flags, keys,
+                             values, c.stream);
     check(hipGetLastError(), "ReflectionRayCreationWrapper");
 }
 
@@ -345,8 +346,8 @@ void ShadowRayColoringWrapper(const unsigned int W, const unsigned int H, const 
     }
     GBufView g{const_cast<float4*>(c.g_pos), const_cast<float4*>(c.g_nrm), const_cast<float4*>(c.g_dif),
                const_cast<float4*>(c.g_spec)};
-    launch_shade_primary(g, W, H, c.light_max, Lights{c.light_pos, c.light_col, c.light_int, L}, V(eye), sflags, pixels,
-                         c.stream);
+    launch_shade_primary(g, W, H, c.light_max, Lights{c.light_pos, c.light_col, c.light_int, L},
+                         single_view(W, H, V(eye)), sflags, pixels, c.stream);
     check(hipGetLastError(), "ShadowRayColoringWrapper");
 }
 

@@ -49,6 +49,21 @@ struct rsh_engine {
     DevBuf closest, shadow, pixels, counters, dcounts;
     // work list of the 64-ray traversal kernel
     DevBuf w_npass, w_cand, w_nitems, w_off, w_first, w_list, w_cum, w_items;
+    DevBuf views_d;                    // view table of a batch (more than one view)
+    uint64_t scene_epoch = 0;          // counts scene loads
+
+    // Graph replay (render flag bit 3).  An eager frame records every pass's
+    // sizes (the values the host otherwise reads back); a second run replays
+    // those sizes without read-backs or synchronisation under stream capture.
+    struct PassPlan { uint32_t N = 0, C = 0, chunk = 0, list_total = 0, items = 0; };
+    std::vector<PassPlan> plan;
+    bool recording = false, replaying = false;
+    size_t plan_pos = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<uint8_t> gsig;         // views + config + scene epoch the graph was captured for
+    uint64_t gepoch = 0;               // allocation epoch at capture
+    rsh_frame_stats gstats{};          // statistics of the captured frame
     DevBuf g0[4], gb[4];
     ScanScratch sc;
 
@@ -180,7 +195,6 @@ KeyCfg key_cfg(const rsh_frame_config& c) {
     k.h.segment_shift = c.segment_shift;
     k.h.min_x = c.scene_min[0]; k.h.min_y = c.scene_min[1]; k.h.min_z = c.scene_min[2];
     k.h.scale_x = c.cell_scale[0]; k.h.scale_y = c.cell_scale[1]; k.h.scale_z = c.cell_scale[2];
-    k.segment = 0;
     return k;
 }
 
@@ -202,7 +216,13 @@ rsh_status build_items(rsh_engine* e, const rsh_frame_config& cfg, TravParams& p
     if (p.algorithm != 1) HIP_OK(run_exclusive_scan(npass, u32p(e->w_off), (uint64_t)ntop + 1, e->sc, s));
     uint32_t chunk = cfg.traversal_chunk ? cfg.traversal_chunk : ITEM_CHUNK;
     uint32_t tot[2] = {0, 0};   // list entries, items
-    for (;;) {
+    if (e->replaying) {
+        const rsh_engine::PassPlan& pp = e->plan[e->plan_pos - 1];
+        chunk = pp.chunk; tot[0] = pp.list_total; tot[1] = pp.items;
+        launch_item_count(cand, chunk, ntop, nitems, s);
+        HIP_OK(run_exclusive_scan(nitems, u32p(e->w_first), (uint64_t)ntop + 1, e->sc, s));
+    }
+    for (; !e->replaying;) {
         launch_item_count(cand, chunk, ntop, nitems, s);
         HIP_OK(run_exclusive_scan(nitems, u32p(e->w_first), (uint64_t)ntop + 1, e->sc, s));
         if (p.algorithm != 1) HIP_OK(hipMemcpyAsync(tot, u32p(e->w_off) + ntop, 4, hipMemcpyDeviceToHost, s));
@@ -228,6 +248,10 @@ rsh_status build_items(rsh_engine* e, const rsh_frame_config& cfg, TravParams& p
     launch_item_fill(u32p(e->w_first), nitems, ntop, u32p(e->w_items), s);
     HIP_OK(hipGetLastError());
     out.items = tot[1]; out.chunk = chunk;
+    if (e->recording) {
+        rsh_engine::PassPlan& pp = e->plan.back();
+        pp.chunk = chunk; pp.list_total = tot[0]; pp.items = tot[1];
+    }
     return RSH_OK;
 }
 
@@ -246,8 +270,13 @@ rsh_status trace_pass(rsh_engine* e, const rsh_frame_config& cfg, const float* r
                         e->sc, s));
     }
     uint32_t hc[2] = {0, 0};
-    HIP_OK(hipMemcpyAsync(hc, dc, 4, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
+    if (e->replaying) {
+        hc[0] = e->plan[e->plan_pos++].N;
+    } else {
+        HIP_OK(hipMemcpyAsync(hc, dc, 4, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        if (e->recording) { e->plan.emplace_back(); e->plan.back().N = hc[0]; }
+    }
     const uint32_t N = hc[0];
     out.N = N;
     if (kind == 0) launch_fill_u64(u64p(e->closest), cfg.hit_mode == 0 ? MISS64 : (unsigned long long)MISS32, S, s);
@@ -260,8 +289,13 @@ rsh_status trace_pass(rsh_engine* e, const rsh_frame_config& cfg, const float* r
             HIP_OK(run_compress(u32p(e->tkeys), N, u32p(e->bases), u32p(e->sizes), u32p(e->ckeys), u32p(e->cvalues),
                                 dc + 1, e->sc, s));
         }
-        HIP_OK(hipMemcpyAsync(hc + 1, dc + 1, 4, hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
+        if (e->replaying) {
+            hc[1] = e->plan[e->plan_pos - 1].C;
+        } else {
+            HIP_OK(hipMemcpyAsync(hc + 1, dc + 1, 4, hipMemcpyDeviceToHost, s));
+            HIP_OK(hipStreamSynchronize(s));
+            if (e->recording) e->plan.back().C = hc[1];
+        }
         const uint32_t C = hc[1];
         out.C = C;
         {
@@ -396,8 +430,10 @@ void rsh_engine_destroy(rsh_engine* e) {
                       &e->ckeys, &e->cvalues, &e->skeys, &e->svalues, &e->rkeys, &e->rvalues, &e->skeleton, &e->scan,
                       &e->rec0, &e->rec1, &e->nodes, &e->aux, &e->closest, &e->shadow, &e->pixels, &e->counters,
                       &e->dcounts, &e->w_npass, &e->w_cand, &e->w_nitems, &e->w_off, &e->w_first, &e->w_list,
-                      &e->w_cum, &e->w_items})
+                      &e->w_cum, &e->w_items, &e->views_d})
         b->release();
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
     for (int i = 0; i < 4; i++) { e->g0[i].release(); e->gb[i].release(); }
     for (auto& k : e->kept) k.buf.release();
     e->sc.release();
@@ -492,15 +528,23 @@ rsh_status rsh_engine_load_scene(rsh_engine* e, const rsh_scene_desc* s) {
     HIP_OK(hipStreamSynchronize(st));
     crange_d.release();
     e->ranges_tb = 0xFFFFFFFFu;
+    e->scene_epoch++;
     return RSH_OK;
 }
 
-rsh_status rsh_engine_render(rsh_engine* e, const rsh_camera* cam, const rsh_frame_config* cfgp, uint32_t fl,
-                             rsh_frame_stats* st) {
-    if (!e || !cam || !cfgp) return RSH_INVALID_ARG;
-    const rsh_frame_config cfg = *cfgp;
-    if (e->T == 0) { e->err = "render: no scene loaded"; return RSH_INVALID_ARG; }
-    if (cfg.width == 0 || cfg.height == 0 || cfg.leaf == 0 || cfg.branch == 0 || cfg.depth == 0 || cfg.depth > 4 ||
+}  // extern "C"
+
+namespace {
+
+// One frame over n views (n == 1: a W x H image; n > 1: a batch laid out as
+// one image of P = sum(w*h) pixels, W = P, H = 1).  hv[i].off must be the
+// prefix sums of w*h.
+rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rsh_frame_config& cfg, uint32_t fl,
+                        rsh_frame_stats* st) {
+    uint64_t P64 = 0;
+    for (uint32_t i = 0; i < nv; i++) P64 += (uint64_t)hv[i].w * hv[i].h;
+    const uint32_t width = nv == 1 ? hv[0].w : (uint32_t)P64, height = nv == 1 ? hv[0].h : 1u;
+    if (width == 0 || height == 0 || cfg.leaf == 0 || cfg.branch == 0 || cfg.depth == 0 || cfg.depth > 4 ||
         cfg.primary_rays != 1 || cfg.algorithm > 1 || cfg.key_mode > 1 || cfg.hit_mode > 1 || cfg.cone_mode > 1) {
         e->err = "render: invalid configuration";
         return RSH_INVALID_ARG;
@@ -514,10 +558,18 @@ rsh_status rsh_engine_render(rsh_engine* e, const rsh_camera* cam, const rsh_fra
         e->err = "render: reference hit packing holds 20-bit triangle ids";
         return RSH_CAPACITY;
     }
-    const uint64_t S64 = (uint64_t)cfg.width * cfg.height * e->L;
+    if (cfg.segment_shift) {
+        const uint32_t hb = cfg.key_mode == RSH_KEY_OCT ? 2 + 3 * cfg.cell_bits + 2 * cfg.dir_bits : 32;
+        if (cfg.key_mode != RSH_KEY_OCT || cfg.segment_shift > 31 || hb > cfg.segment_shift ||
+            (nv > 1 && ((uint64_t)(nv - 1) >> (32 - cfg.segment_shift)) != 0)) {
+            e->err = "render: the view id does not fit above the octahedral key at segment_shift";
+            return RSH_INVALID_ARG;
+        }
+    }
+    const uint64_t S64 = (uint64_t)width * height * e->L;
     if (S64 > (uint64_t)MAX_TILES * TILE || S64 > 0xFFFFFFFFull) { e->err = "render: frame exceeds slot capacity"; return RSH_CAPACITY; }
     HIP_OK(hipSetDevice(e->device));
-    rsh_status rs = ensure_frame(e, cfg.width, cfg.height, e->L);
+    rsh_status rs = ensure_frame(e, width, height, e->L);
     if (rs != RSH_OK) return rs;
     rs = build_ranges(e, cfg.triangle_batch);
     if (rs != RSH_OK) return rs;
@@ -527,17 +579,28 @@ rsh_status rsh_engine_render(rsh_engine* e, const rsh_camera* cam, const rsh_fra
         HIP_OK(e->nodes.reserve(total * 32 + 64));
         HIP_OK(e->aux.reserve(total * 8 + 64));
     }
-    for (auto& k : e->kept) k.buf.release();
-    e->kept.clear();
+    Views V{};
+    if (nv == 1) {
+        V.n = 0; V.one = hv[0];
+    } else {
+        if (!e->replaying) {
+            HIP_OK(e->views_d.reserve((size_t)nv * sizeof(ViewDesc)));
+            HIP_OK(hipMemcpyAsync(e->views_d.p, hv, (size_t)nv * sizeof(ViewDesc), hipMemcpyHostToDevice, e->stream));
+        }
+        V.d = e->views_d.as<ViewDesc>(); V.n = nv;
+    }
+    if (!e->replaying) {
+        for (auto& k : e->kept) k.buf.release();
+        e->kept.clear();
+    }
     e->timed.clear();
     e->ev_used = 0;
     e->pass_index = 0;
     if (st) std::memset(st, 0, sizeof(*st));
     hipStream_t s = e->stream;
-    const uint32_t W = cfg.width, H = cfg.height, Lm = e->L, P = W * H, S = W * H * Lm;
+    const uint32_t W = width, H = height, Lm = e->L, P = W * H, S = W * H * Lm;
     const KeyCfg kc = key_cfg(cfg);
     const SceneView sv = scene_view(e);
-    const v3 eye{cam->eye[0], cam->eye[1], cam->eye[2]};
     const bool timing = fl & 4u;
     GBufView G0 = view(e, e->g0), GB = view(e, e->gb);
     launch_fill_u32(u32p(e->pixels), 0u, P, s);
@@ -545,9 +608,7 @@ rsh_status rsh_engine_render(rsh_engine* e, const rsh_camera* cam, const rsh_fra
     // primary pass -> G-buffer
     {
         StageTimer t(e, ST_CREATE, timing);
-        launch_create_primary(W, H, eye, v3{cam->direction[0], cam->direction[1], cam->direction[2]},
-                              v3{cam->right[0], cam->right[1], cam->right[2]}, v3{cam->up[0], cam->up[1], cam->up[2]},
-                              kc, e->rays_b.as<float>(), u32p(e->flags), u32p(e->keys), u32p(e->values), s);
+        launch_create_primary(V, P, kc, e->rays_b.as<float>(), u32p(e->flags), u32p(e->keys), u32p(e->values), s);
     }
     PassOut po;
     rs = trace_pass(e, cfg, e->rays_b.as<float>(), P, 0, nullptr, fl, po);
@@ -560,25 +621,25 @@ rsh_status rsh_engine_render(rsh_engine* e, const rsh_camera* cam, const rsh_fra
     // shadow pass + primary shading
     {
         StageTimer t(e, ST_CREATE, timing);
-        launch_create_shadow(G0.pos, G0.nrm, W, H, Lm, f4p(e->light_pos), kc, e->rays_b.as<float>(), u32p(e->flags),
-                             u32p(e->keys), u32p(e->values), s);
+        launch_create_shadow(G0.pos, G0.nrm, W, H, Lm, f4p(e->light_pos), kc, V, e->rays_b.as<float>(),
+                             u32p(e->flags), u32p(e->keys), u32p(e->values), s);
     }
     rs = trace_pass(e, cfg, e->rays_b.as<float>(), S, 1, G0.pos, fl, po);
     if (rs != RSH_OK) return rs;
     if ((rs = record(e, st, 1, S, po, fl)) != RSH_OK) return rs;
     if (po.N > 0) {
         StageTimer t(e, ST_SHADE, timing);
-        launch_shade_primary(G0, W, H, Lm, sv.lights, eye, u32p(e->shadow), u32p(e->pixels), s);
+        launch_shade_primary(G0, W, H, Lm, sv.lights, V, u32p(e->shadow), u32p(e->pixels), s);
     }
     // mirror bounces
     for (uint32_t b = 1; b <= cfg.bounces; b++) {
         {
             StageTimer t(e, ST_CREATE, timing);
             if (b == 1)
-                launch_create_reflection(G0.pos, G0.nrm, G0.dif, W, H, Lm, eye, cfg.reflect_gate, 1, kc,
+                launch_create_reflection(G0.pos, G0.nrm, G0.dif, W, H, Lm, V, cfg.reflect_gate, 1, kc,
                                          e->rays_a.as<float>(), u32p(e->flags), u32p(e->keys), u32p(e->values), s);
             else
-                launch_create_bounce(GB, W, H, Lm, u64p(e->closest), cfg.hit_mode, cfg.reflect_gate, kc,
+                launch_create_bounce(GB, W, H, Lm, u64p(e->closest), cfg.hit_mode, cfg.reflect_gate, kc, V,
                                      e->rays_a.as<float>(), u32p(e->flags), u32p(e->keys), u32p(e->values), s);
         }
         rs = trace_pass(e, cfg, e->rays_a.as<float>(), S, 0, nullptr, fl, po);
@@ -593,7 +654,7 @@ rsh_status rsh_engine_render(rsh_engine* e, const rsh_camera* cam, const rsh_fra
         if (cfg.bounce_shadows) {
             {
                 StageTimer t(e, ST_CREATE, timing);
-                launch_create_shadow(GB.pos, GB.nrm, W, H, Lm, f4p(e->light_pos), kc, e->rays_b.as<float>(),
+                launch_create_shadow(GB.pos, GB.nrm, W, H, Lm, f4p(e->light_pos), kc, V, e->rays_b.as<float>(),
                                      u32p(e->flags), u32p(e->keys), u32p(e->values), s);
             }
             PassOut sp;
@@ -604,11 +665,12 @@ rsh_status rsh_engine_render(rsh_engine* e, const rsh_camera* cam, const rsh_fra
         }
         {
             StageTimer t(e, ST_SHADE, timing);
-            launch_shade_reflection(GB, W, H, Lm, sv.lights, eye, u64p(e->closest), cfg.hit_mode, sflags,
+            launch_shade_reflection(GB, W, H, Lm, sv.lights, V, u64p(e->closest), cfg.hit_mode, sflags,
                                     u32p(e->pixels), s);
         }
     }
     HIP_OK(hipGetLastError());
+    if (e->replaying) return RSH_OK;     // under stream capture: no synchronisation
     HIP_OK(hipStreamSynchronize(s));
     if (st && timing) {
         for (auto& t : e->timed) {
@@ -623,6 +685,101 @@ rsh_status rsh_engine_render(rsh_engine* e, const rsh_camera* cam, const rsh_fra
         }
     }
     return RSH_OK;
+}
+
+ViewDesc view_desc(const rsh_camera& c, uint32_t off, uint32_t w, uint32_t h) {
+    ViewDesc v{};
+    v.off = off; v.w = w; v.h = h;
+    for (int i = 0; i < 3; i++) {
+        v.eye[i] = c.eye[i]; v.dir[i] = c.direction[i]; v.right[i] = c.right[i]; v.up[i] = c.up[i];
+    }
+    return v;
+}
+
+void drop_graph(rsh_engine* e) {
+    if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->gexec = nullptr; e->graph = nullptr;
+    e->gsig.clear();
+}
+
+// Flag bit 3: replay a captured graph of this exact frame when one exists
+// for the same views, configuration and scene and no device buffer has been
+// reallocated since; otherwise render eagerly (recording the pass sizes),
+// then capture a replay of the frame for the next call.  A frame's sizes
+// depend only on these inputs, so the replay repeats the eager frame's work
+// without host read-backs.
+rsh_status render(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rsh_frame_config& cfg, uint32_t fl,
+                  rsh_frame_stats* st) {
+    if (!(fl & 8u)) return render_frame(e, hv, nv, cfg, fl, st);
+    if (fl & 7u) { e->err = "render: graph replay keeps no passes, counters or timers (flag bits 0-2)"; return RSH_INVALID_ARG; }
+    if (e->stream == nullptr) { e->err = "render: graph replay needs a non-null stream"; return RSH_INVALID_ARG; }
+    std::vector<uint8_t> sig(sizeof(cfg) + sizeof(e->scene_epoch) + (size_t)nv * sizeof(ViewDesc));
+    std::memcpy(sig.data(), &cfg, sizeof(cfg));
+    std::memcpy(sig.data() + sizeof(cfg), &e->scene_epoch, sizeof(e->scene_epoch));
+    std::memcpy(sig.data() + sizeof(cfg) + sizeof(e->scene_epoch), hv, (size_t)nv * sizeof(ViewDesc));
+    if (e->gexec && sig == e->gsig && alloc_epoch().load() == e->gepoch) {
+        HIP_OK(hipSetDevice(e->device));
+        HIP_OK(hipGraphLaunch(e->gexec, e->stream));
+        HIP_OK(hipStreamSynchronize(e->stream));
+        if (st) { *st = e->gstats; st->graph_replayed = 1; }
+        return RSH_OK;
+    }
+    drop_graph(e);
+    rsh_frame_stats local{};
+    e->plan.clear();
+    e->recording = true;
+    rsh_status rs = render_frame(e, hv, nv, cfg, 0, &local);
+    e->recording = false;
+    if (rs != RSH_OK) return rs;
+    if (st) *st = local;
+    const uint64_t epoch = alloc_epoch().load();
+    e->replaying = true;
+    e->plan_pos = 0;
+    HIP_OK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
+    rs = render_frame(e, hv, nv, cfg, 0, nullptr);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(e->stream, &g);
+    e->replaying = false;
+    if (rs != RSH_OK) { if (g) (void)hipGraphDestroy(g); return rs; }
+    HIP_OK(ce);
+    if (alloc_epoch().load() != epoch || e->plan_pos != e->plan.size()) {
+        (void)hipGraphDestroy(g);
+        e->err = "render: the captured frame did not repeat the recorded one";
+        return RSH_INTERNAL;
+    }
+    const hipError_t ie = hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0);
+    if (ie != hipSuccess) { (void)hipGraphDestroy(g); e->gexec = nullptr; HIP_OK(ie); }
+    e->graph = g; e->gsig = sig; e->gepoch = epoch; e->gstats = local;
+    return RSH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+rsh_status rsh_engine_render(rsh_engine* e, const rsh_camera* cam, const rsh_frame_config* cfgp, uint32_t fl,
+                             rsh_frame_stats* st) {
+    if (!e || !cam || !cfgp) return RSH_INVALID_ARG;
+    if (e->T == 0) { e->err = "render: no scene loaded"; return RSH_INVALID_ARG; }
+    const ViewDesc v = view_desc(*cam, 0, cfgp->width, cfgp->height);
+    return render(e, &v, 1, *cfgp, fl, st);
+}
+
+rsh_status rsh_engine_render_views(rsh_engine* e, const rsh_view* views, uint32_t n, const rsh_frame_config* cfgp,
+                                   uint32_t fl, rsh_frame_stats* st) {
+    if (!e || !views || !cfgp || n == 0) return RSH_INVALID_ARG;
+    if (e->T == 0) { e->err = "render_views: no scene loaded"; return RSH_INVALID_ARG; }
+    std::vector<ViewDesc> hv(n);
+    uint64_t off = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        if (views[i].width == 0 || views[i].height == 0) { e->err = "render_views: empty view"; return RSH_INVALID_ARG; }
+        if (off > 0xFFFFFFFFull) { e->err = "render_views: batch exceeds 2^32 pixels"; return RSH_CAPACITY; }
+        hv[i] = view_desc(views[i].camera, (uint32_t)off, views[i].width, views[i].height);
+        off += (uint64_t)views[i].width * views[i].height;
+    }
+    if (off > 0xFFFFFFFFull) { e->err = "render_views: batch exceeds 2^32 pixels"; return RSH_CAPACITY; }
+    return render(e, hv.data(), n, *cfgp, fl, st);
 }
 
 rsh_status rsh_engine_read_pixels(rsh_engine* e, uint32_t* host, size_t count) {

@@ -37,24 +37,33 @@ __device__ __forceinline__ void load_ray(const float* rays, size_t slot, v3& o, 
 }
 
 // type: 0 primary, 1 shadow (light l), 2 mirror
-__device__ __forceinline__ uint32_t make_key(const KeyCfg& kc, v3 o, v3 d, uint32_t type, uint32_t l) {
+// seg: the segment (view) id OR-ed in at kc.h.segment_shift when that is set.
+__device__ __forceinline__ uint32_t make_key(const KeyCfg& kc, v3 o, v3 d, uint32_t type, uint32_t l, uint32_t seg) {
     if (kc.mode == 0) return type == 1 ? shadow_key_ref(d, l) : reflection_key_ref(o, d);
     uint32_t k = key_oct(kc.h, o, d, type);
-    if (kc.h.segment_shift) k |= kc.segment << kc.h.segment_shift;
+    if (kc.h.segment_shift) k |= seg << kc.h.segment_shift;
     return k;
 }
+__device__ __forceinline__ uint32_t seg_of(const KeyCfg& kc, const Views& V, uint32_t p) {
+    return kc.h.segment_shift ? view_of(V, p) : 0u;
+}
 
-__global__ void __launch_bounds__(256) k_create_primary(uint32_t W, uint32_t H, v3 eye, v3 D, v3 R, v3 U, KeyCfg kc,
-                                                        float* rays, uint32_t* flags, uint32_t* keys,
-                                                        uint32_t* values) {
-    const uint32_t n = W * H;
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-        uint32_t x = p % W, y = p / W;
-        float a = (float)x / (float)W - 0.5f;
-        float c = (float)y / (float)H - 0.5f;
+// Pixel p of view v: (x, y) within the view, dir = D + R*(x/w - 0.5) + U*(y/h - 0.5).
+__global__ void __launch_bounds__(256) k_create_primary(Views V, uint32_t P, KeyCfg kc, float* rays, uint32_t* flags,
+                                                        uint32_t* keys, uint32_t* values) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
+        const uint32_t v = view_of(V, p);
+        const ViewDesc& vd = V.n ? V.d[v] : V.one;
+        const uint32_t q = p - vd.off;
+        const uint32_t x = q % vd.w, y = q / vd.w;
+        const float a = (float)x / (float)vd.w - 0.5f;
+        const float c = (float)y / (float)vd.h - 0.5f;
+        const v3 eye = mk(vd.eye[0], vd.eye[1], vd.eye[2]);
+        const v3 D = mk(vd.dir[0], vd.dir[1], vd.dir[2]), R = mk(vd.right[0], vd.right[1], vd.right[2]);
+        const v3 U = mk(vd.up[0], vd.up[1], vd.up[2]);
         v3 d = normalize((D + R * a) + U * c);
         store_ray(rays, p, eye, d);
-        keys[p] = make_key(kc, eye, d, 0, 0);
+        keys[p] = make_key(kc, eye, d, 0, 0, v);
         values[p] = p;
         flags[p] = 0;
     }
@@ -62,8 +71,9 @@ __global__ void __launch_bounds__(256) k_create_primary(uint32_t W, uint32_t H, 
 
 // RayTracer.cu:682-740; slot i = x + W*(l + Lmax*y).
 __global__ void __launch_bounds__(256) k_create_shadow(const float4* gpos, const float4* gnrm, uint32_t W, uint32_t H,
-                                                       uint32_t Lmax, const float4* light_pos, KeyCfg kc, float* rays,
-                                                       uint32_t* flags, uint32_t* keys, uint32_t* values) {
+                                                       uint32_t Lmax, const float4* light_pos, KeyCfg kc, Views V,
+                                                       float* rays, uint32_t* flags, uint32_t* keys,
+                                                       uint32_t* values) {
     const uint32_t n = W * H * Lmax;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         uint32_t x = i % W, y = i / (W * Lmax), l = (i / W) % Lmax;
@@ -77,7 +87,7 @@ __global__ void __launch_bounds__(256) k_create_shadow(const float4* gpos, const
             float diffuse = fmaxf(dot(d, -N), 0.0f);
             if (diffuse > 0.0f) {
                 store_ray(rays, i, o, d);
-                keys[i] = make_key(kc, o, d, 1, l);
+                keys[i] = make_key(kc, o, d, 1, l, seg_of(kc, V, (uint32_t)px));
                 values[i] = i;
                 flag = 0;
             }
@@ -89,7 +99,7 @@ __global__ void __launch_bounds__(256) k_create_shadow(const float4* gpos, const
 // RayTracer.cu:742-791; slot x + y*W*Lmax.  gate: only diffuse.w > 0 reflects.
 // mark_gaps: also flag the unused slots between rows when Lmax > 1.
 __global__ void __launch_bounds__(256) k_create_reflection(const float4* gpos, const float4* gnrm, const float4* gdif,
-                                                           uint32_t W, uint32_t H, uint32_t Lmax, v3 eye,
+                                                           uint32_t W, uint32_t H, uint32_t Lmax, Views V,
                                                            uint32_t gate, uint32_t mark_gaps, KeyCfg kc, float* rays,
                                                            uint32_t* flags, uint32_t* keys, uint32_t* values) {
     const uint32_t n = W * H;
@@ -100,9 +110,10 @@ __global__ void __launch_bounds__(256) k_create_reflection(const float4* gpos, c
         v3 N = normalize(xyz(gnrm[p]));
         bool live = len(P) != 0.0f && (!gate || gdif[p].w > 0.0f);
         if (live) {
-            v3 d = reflect(normalize(P - eye), normalize(N));
+            const uint32_t v = view_of(V, p);
+            v3 d = reflect(normalize(P - view_eye(V, v)), normalize(N));
             store_ray(rays, slot, P, d);
-            keys[slot] = make_key(kc, P, d, 2, 0);
+            keys[slot] = make_key(kc, P, d, 2, 0, v);
             values[slot] = slot;
             flags[slot] = 0;
         } else {
@@ -117,8 +128,8 @@ __global__ void __launch_bounds__(256) k_create_reflection(const float4* gpos, c
 // reflect(previous direction, barycentric normal), in place.
 __global__ void __launch_bounds__(256) k_create_bounce(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax,
                                                        const unsigned long long* rec, uint32_t hit_mode,
-                                                       uint32_t gate, KeyCfg kc, float* rays, uint32_t* flags,
-                                                       uint32_t* keys, uint32_t* values) {
+                                                       uint32_t gate, KeyCfg kc, Views V, float* rays,
+                                                       uint32_t* flags, uint32_t* keys, uint32_t* values) {
     const uint32_t n = W * H;
     const unsigned long long miss = hit_mode == 0 ? MISS64 : (unsigned long long)MISS32;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
@@ -131,7 +142,7 @@ __global__ void __launch_bounds__(256) k_create_bounce(GBufView g, uint32_t W, u
             v3 P = xyz(g.pos[p]), N = xyz(g.nrm[p]);
             v3 nd = reflect(d, N);
             store_ray(rays, slot, P, nd);
-            keys[slot] = make_key(kc, P, nd, 2, 0);
+            keys[slot] = make_key(kc, P, nd, 2, 0, seg_of(kc, V, p));
             values[slot] = slot;
             flags[slot] = 0;
         } else {
@@ -180,13 +191,14 @@ __global__ void __launch_bounds__(256) k_surface(const float* rays, const unsign
 
 // RayTracer.cu:1616-1709 ColorPrimaryShadowRay
 __global__ void __launch_bounds__(256) k_shade_primary(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L,
-                                                       v3 eye, const uint32_t* sflags, uint32_t* pixels) {
+                                                       Views V, const uint32_t* sflags, uint32_t* pixels) {
     const uint32_t n = W * H;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
         uint32_t x = p % W, y = p / W;
         v3 color = mk(0.0f, 0.0f, 0.0f);
         v3 P = xyz(g.pos[p]), N = xyz(g.nrm[p]);
         if (len(P) != 0.0f) {
+            const v3 eye = view_eye(V, view_of(V, p));
             float4 Kd = g.dif[p], Ks = g.spec[p];
             for (uint32_t l = 0; l < L.total; l++) {
                 if (sflags[(size_t)(x + y * W * Lmax) + (size_t)W * l] != MISS32)
@@ -200,7 +212,7 @@ __global__ void __launch_bounds__(256) k_shade_primary(GBufView g, uint32_t W, u
 // RayTracer.cu:1776-1882 shading from the surface record; with bounce shadow
 // flags a light whose shadow ray is occluded is skipped.
 __global__ void __launch_bounds__(256) k_shade_reflection(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L,
-                                                          v3 eye, const unsigned long long* rec, uint32_t hit_mode,
+                                                          Views V, const unsigned long long* rec, uint32_t hit_mode,
                                                           const uint32_t* sflags, uint32_t* pixels) {
     const uint32_t n = W * H;
     const unsigned long long miss = hit_mode == 0 ? MISS64 : (unsigned long long)MISS32;
@@ -208,6 +220,7 @@ __global__ void __launch_bounds__(256) k_shade_reflection(GBufView g, uint32_t W
         uint32_t x = p % W, y = p / W;
         uint32_t slot = x + y * W * Lmax;
         if (rec[slot] == miss) continue;
+        const v3 eye = view_eye(V, view_of(V, p));
         v3 P = xyz(g.pos[p]), N = xyz(g.nrm[p]);
         float4 Kd = g.dif[p], Ks = g.spec[p];
         v3 color = mk(0.0f, 0.0f, 0.0f);
@@ -352,42 +365,41 @@ __global__ void __launch_bounds__(256) k_color_reflection_ref(const float4* tp, 
 
 }  // namespace
 
-void launch_create_primary(uint32_t W, uint32_t H, v3 eye, v3 D, v3 R, v3 U, KeyCfg kc, float* rays, uint32_t* flags,
-                           uint32_t* keys, uint32_t* values, hipStream_t s) {
-    k_create_primary<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(W, H, eye, D, R, U, kc, rays, flags, keys,
-                                                                            values);
+void launch_create_primary(const Views& V, uint32_t P, KeyCfg kc, float* rays, uint32_t* flags, uint32_t* keys,
+                           uint32_t* values, hipStream_t s) {
+    k_create_primary<<<grid_for(P, 256, 65536), 256, 0, s>>>(V, P, kc, rays, flags, keys, values);
 }
 void launch_create_shadow(const float4* gpos, const float4* gnrm, uint32_t W, uint32_t H, uint32_t Lmax,
-                          const float4* light_pos, KeyCfg kc, float* rays, uint32_t* flags, uint32_t* keys,
-                          uint32_t* values, hipStream_t s) {
+                          const float4* light_pos, KeyCfg kc, const Views& V, float* rays, uint32_t* flags,
+                          uint32_t* keys, uint32_t* values, hipStream_t s) {
     k_create_shadow<<<grid_for((uint64_t)W * H * Lmax, 256, 65536), 256, 0, s>>>(gpos, gnrm, W, H, Lmax, light_pos, kc,
-                                                                                   rays, flags, keys, values);
+                                                                                   V, rays, flags, keys, values);
 }
 void launch_create_reflection(const float4* gpos, const float4* gnrm, const float4* gdif, uint32_t W, uint32_t H,
-                              uint32_t Lmax, v3 eye, uint32_t gate, uint32_t mark_gaps, KeyCfg kc, float* rays,
-                              uint32_t* flags, uint32_t* keys, uint32_t* values, hipStream_t s) {
-    k_create_reflection<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(gpos, gnrm, gdif, W, H, Lmax, eye, gate,
+                              uint32_t Lmax, const Views& V, uint32_t gate, uint32_t mark_gaps, KeyCfg kc,
+                              float* rays, uint32_t* flags, uint32_t* keys, uint32_t* values, hipStream_t s) {
+    k_create_reflection<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(gpos, gnrm, gdif, W, H, Lmax, V, gate,
                                                                                mark_gaps, kc, rays, flags, keys, values);
 }
 void launch_create_bounce(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, const unsigned long long* rec,
-                          uint32_t hit_mode, uint32_t gate, KeyCfg kc, float* rays, uint32_t* flags, uint32_t* keys,
-                          uint32_t* values, hipStream_t s) {
-    k_create_bounce<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(g, W, H, Lmax, rec, hit_mode, gate, kc, rays,
-                                                                           flags, keys, values);
+                          uint32_t hit_mode, uint32_t gate, KeyCfg kc, const Views& V, float* rays, uint32_t* flags,
+                          uint32_t* keys, uint32_t* values, hipStream_t s) {
+    k_create_bounce<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(g, W, H, Lmax, rec, hit_mode, gate, kc, V,
+                                                                           rays, flags, keys, values);
 }
 void launch_surface(const float* rays, const unsigned long long* rec, uint32_t hit_mode, uint32_t W, uint32_t H,
                     uint32_t slot_stride, uint32_t normalize_n, SceneView sc, GBufView g, hipStream_t s) {
     k_surface<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(rays, rec, hit_mode, W, H, slot_stride,
                                                                      normalize_n, sc, g);
 }
-void launch_shade_primary(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L, v3 eye, const uint32_t* sflags,
-                          uint32_t* pixels, hipStream_t s) {
-    k_shade_primary<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(g, W, H, Lmax, L, eye, sflags, pixels);
+void launch_shade_primary(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L, const Views& V,
+                          const uint32_t* sflags, uint32_t* pixels, hipStream_t s) {
+    k_shade_primary<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(g, W, H, Lmax, L, V, sflags, pixels);
 }
-void launch_shade_reflection(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L, v3 eye,
+void launch_shade_reflection(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L, const Views& V,
                              const unsigned long long* rec, uint32_t hit_mode, const uint32_t* sflags,
                              uint32_t* pixels, hipStream_t s) {
-    k_shade_reflection<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(g, W, H, Lmax, L, eye, rec, hit_mode,
+    k_shade_reflection<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(g, W, H, Lmax, L, V, rec, hit_mode,
                                                                               sflags, pixels);
 }
 void launch_cluster_build(const float4* tri_sphere, const uint2* range, uint32_t n, float4* cl_sphere,

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <string>
 
 #include "rsh_device.hpp"
@@ -20,19 +21,31 @@ constexpr uint32_t MAX_TILES = 16384;
 
 inline uint32_t tiles_for(uint64_t n) { return (uint32_t)((n + TILE - 1) / TILE); }
 
+// Process-wide count of device (re)allocations made through DevBuf.  A
+// captured HIP graph holds raw pointers, so it stays valid only while this
+// count is unchanged.
+inline std::atomic<uint64_t>& alloc_epoch() {
+    static std::atomic<uint64_t> n{0};
+    return n;
+}
+
 // Growable device buffer.
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
     hipError_t reserve(size_t b) {
         if (b <= bytes) return hipSuccess;
+        alloc_epoch()++;
         if (p) (void)hipFree(p);
         p = nullptr; bytes = 0;
         hipError_t e = hipMalloc(&p, b ? b : 1);
         if (e == hipSuccess) bytes = b;
         return e;
     }
-    void release() { if (p) (void)hipFree(p); p = nullptr; bytes = 0; }
+    void release() {
+        if (p) { alloc_epoch()++; (void)hipFree(p); }
+        p = nullptr; bytes = 0;
+    }
     template <class T> T* as() const { return static_cast<T*>(p); }
 };
 

@@ -7,8 +7,7 @@ namespace rsh {
 
 struct KeyCfg {
     uint32_t mode;        // 0 reference spherical bins, 1 octahedral
-    HashParams h;
-    uint32_t segment;     // OR-ed into the key at h.segment_shift (0: none)
+    HashParams h;         // h.segment_shift != 0: the pixel's view id is OR-ed in at that bit
 };
 
 struct SceneView {
@@ -19,6 +18,45 @@ struct SceneView {
     const float4* mat_spec;   // [M]
     Lights lights;
 };
+
+// Views of one batch.  A batch renders n views as one image of P = sum(w*h)
+// pixels laid out view after view (view i starts at pixel off); slots then
+// follow the single-image formulas with W = P, H = 1.  n == 0 is a single
+// view held inline in `one` (off 0, w = W, h = H).
+struct ViewDesc {
+    uint32_t off, w, h, pad;
+    float eye[4], dir[4], right[4], up[4];
+};
+struct Views {
+    const ViewDesc* d;   // [n] device table, sorted by off
+    uint32_t n;
+    ViewDesc one;
+};
+__device__ __forceinline__ uint32_t view_of(const Views& V, uint32_t p) {
+    if (V.n <= 1) return 0;
+    uint32_t a = 0, b = V.n;        // d[a].off <= p < d[b].off
+    while (b - a > 1) {
+        const uint32_t m = (a + b) >> 1;
+        if (V.d[m].off <= p) a = m; else b = m;
+    }
+    return a;
+}
+__device__ __forceinline__ v3 view_eye(const Views& V, uint32_t v) {
+    const float* e = V.n ? V.d[v].eye : V.one.eye;
+    return mk(e[0], e[1], e[2]);
+}
+
+// One view held inline (eye only, or a full pinhole camera of W x H).
+inline Views single_view(uint32_t W, uint32_t H, v3 eye, v3 D = v3{0, 0, 1}, v3 R = v3{1, 0, 0},
+                         v3 U = v3{0, 1, 0}) {
+    Views v{};
+    v.d = nullptr; v.n = 0;
+    v.one.off = 0; v.one.w = W; v.one.h = H;
+    const v3 src[4] = {eye, D, R, U};
+    float* dst[4] = {v.one.eye, v.one.dir, v.one.right, v.one.up};
+    for (int i = 0; i < 4; i++) { dst[i][0] = src[i].x; dst[i][1] = src[i].y; dst[i][2] = src[i].z; dst[i][3] = 0.0f; }
+    return v;
+}
 
 struct GBufView {
     float4* pos;
@@ -70,10 +108,10 @@ struct TravParams {
                                   // waves add into slot wave % COUNTER_SLOTS
     uint32_t force_generic;       // 1: use the generic kernel even for 64-ray top nodes
     uint32_t splits;              // generic kernel: waves sharing one top node's triangle range
-    // work items of the 64-ray kernel: (top node, chunk of candidate triangles)
+    // work items of the 64-ray kernel: (top node, chunk of candidate clusters)
     const uint32_t* item_top;     // [items] top node of each item
     const uint32_t* item_first;   // [n_top + 1] first item of each top node
-    const uint32_t* cand;         // [n_top + 1] candidate triangles of each top node
+    const uint32_t* cand;         // [n_top + 1] candidate clusters of each top node
     const uint32_t* obj_off;      // [n_top + 1] first list entry of each top node (CRSH)
     const uint32_t* obj_list;     // passing objects of each node, in triangle order (RAH: all, shared)
     const uint32_t* obj_cum;      // candidate clusters of the node before each listed object
@@ -109,22 +147,22 @@ void launch_item_fill(const uint32_t* item_first, const uint32_t* nitems, uint32
                       hipStream_t s);
 
 // --- rays ---
-void launch_create_primary(uint32_t W, uint32_t H, v3 eye, v3 D, v3 R, v3 U, KeyCfg kc, float* rays, uint32_t* flags,
-                           uint32_t* keys, uint32_t* values, hipStream_t s);
+void launch_create_primary(const Views& V, uint32_t P, KeyCfg kc, float* rays, uint32_t* flags, uint32_t* keys,
+                           uint32_t* values, hipStream_t s);
 void launch_create_shadow(const float4* gpos, const float4* gnrm, uint32_t W, uint32_t H, uint32_t Lmax,
-                          const float4* light_pos, KeyCfg kc, float* rays, uint32_t* flags, uint32_t* keys,
-                          uint32_t* values, hipStream_t s);
+                          const float4* light_pos, KeyCfg kc, const Views& V, float* rays, uint32_t* flags,
+                          uint32_t* keys, uint32_t* values, hipStream_t s);
 void launch_create_reflection(const float4* gpos, const float4* gnrm, const float4* gdif, uint32_t W, uint32_t H,
-                              uint32_t Lmax, v3 eye, uint32_t gate, uint32_t mark_gaps, KeyCfg kc, float* rays,
-                              uint32_t* flags, uint32_t* keys, uint32_t* values, hipStream_t s);
+                              uint32_t Lmax, const Views& V, uint32_t gate, uint32_t mark_gaps, KeyCfg kc,
+                              float* rays, uint32_t* flags, uint32_t* keys, uint32_t* values, hipStream_t s);
 void launch_create_bounce(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, const unsigned long long* rec,
-                          uint32_t hit_mode, uint32_t gate, KeyCfg kc, float* rays, uint32_t* flags, uint32_t* keys,
-                          uint32_t* values, hipStream_t s);
+                          uint32_t hit_mode, uint32_t gate, KeyCfg kc, const Views& V, float* rays, uint32_t* flags,
+                          uint32_t* keys, uint32_t* values, hipStream_t s);
 void launch_surface(const float* rays, const unsigned long long* rec, uint32_t hit_mode, uint32_t W, uint32_t H,
                     uint32_t slot_stride, uint32_t normalize_n, SceneView sc, GBufView g, hipStream_t s);
-void launch_shade_primary(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L, v3 eye, const uint32_t* sflags,
-                          uint32_t* pixels, hipStream_t s);
-void launch_shade_reflection(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L, v3 eye,
+void launch_shade_primary(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L, const Views& V,
+                          const uint32_t* sflags, uint32_t* pixels, hipStream_t s);
+void launch_shade_reflection(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L, const Views& V,
                              const unsigned long long* rec, uint32_t hit_mode, const uint32_t* sflags,
                              uint32_t* pixels, hipStream_t s);
 void launch_triangle_prep(const float4* tri_pos, uint32_t T, float4* tri_sphere, float4* tri_mt, hipStream_t s);

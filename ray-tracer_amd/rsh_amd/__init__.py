@@ -6,6 +6,7 @@ library is missing, :func:`lib` raises -- there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
 import os
 
 import numpy as np
@@ -35,6 +36,10 @@ class CameraDesc(C.Structure):
     _fields_ = [("eye", f32 * 3), ("direction", f32 * 3), ("right", f32 * 3), ("up", f32 * 3)]
 
 
+class ViewDesc(C.Structure):
+    _fields_ = [("camera", CameraDesc), ("width", u32), ("height", u32)]
+
+
 class FrameConfigDesc(C.Structure):
     _fields_ = [("width", u32), ("height", u32), ("algorithm", u32), ("key_mode", u32),
                 ("hit_mode", u32), ("cone_mode", u32), ("leaf", u32), ("branch", u32),
@@ -54,7 +59,7 @@ class FrameStats(C.Structure):
                 ("pass_splits", u32 * MAX_PASSES), ("pass_items", u32 * MAX_PASSES),
                 ("pass_chunk", u32 * MAX_PASSES), ("pass_tri_tests", u64 * MAX_PASSES),
                 ("pass_cluster_tests", u64 * MAX_PASSES), ("pass_leaf_steps", u64 * MAX_PASSES),
-                ("pass_ray_steps", u64 * MAX_PASSES)]
+                ("pass_ray_steps", u64 * MAX_PASSES), ("graph_replayed", u32)]
 
 
 class Float3(C.Structure):
@@ -93,7 +98,7 @@ BINDERS = ["bindDiffuseTextureArray", "bindSpecularTextureArray", "bindFragmentP
            "bindMaterialDiffuseProperties", "bindMaterialSpecularProperties", "bindBoundingSpheres",
            "bindLightPositions", "bindLightColors", "bindLightIntensities"]
 NATIVE_SYMBOLS = ["rsh_engine_create", "rsh_engine_destroy", "rsh_last_error", "rsh_engine_set_stream",
-                  "rsh_engine_load_scene", "rsh_engine_render", "rsh_engine_read_pixels",
+                  "rsh_engine_load_scene", "rsh_engine_render", "rsh_engine_render_views", "rsh_engine_read_pixels",
                   "rsh_engine_read_gbuffer", "rsh_engine_read_pass", "rsh_engine_pixels_device",
                   "rsh_device_alloc", "rsh_device_free", "rsh_memcpy_htod", "rsh_memcpy_dtoh",
                   "rsh_memset_d32", "rsh_device_synchronize", "rsh_device_count",
@@ -127,6 +132,9 @@ def lib():
     L.rsh_engine_render.restype = C.c_int
     L.rsh_engine_render.argtypes = [PV, C.POINTER(CameraDesc), C.POINTER(FrameConfigDesc), u32,
                                     C.POINTER(FrameStats)]
+    L.rsh_engine_render_views.restype = C.c_int
+    L.rsh_engine_render_views.argtypes = [PV, C.POINTER(ViewDesc), u32, C.POINTER(FrameConfigDesc), u32,
+                                          C.POINTER(FrameStats)]
     L.rsh_engine_read_pixels.restype = C.c_int
     L.rsh_engine_read_pixels.argtypes = [PV, PU, C.c_size_t]
     L.rsh_engine_read_gbuffer.restype = C.c_int
@@ -259,7 +267,7 @@ class RshError(RuntimeError):
 class Engine:
     """Device-resident scene + frame renderer (include/rsh.h)."""
 
-    KEEP_PASSES, COUNTERS, TIMING = 1, 2, 4
+    KEEP_PASSES, COUNTERS, TIMING, GRAPH = 1, 2, 4, 8
 
     def __init__(self, device: int = 0):
         self.L = lib()
@@ -286,6 +294,34 @@ class Engine:
                                              flags, C.byref(st)), "render")
         self.stats = st
         return st
+
+    def render_views(self, views, cfg, flags=0) -> FrameStats:
+        """Render a batch of (camera, width, height) views as one image of
+        sum(w*h) pixels; pixels() / gbuffer() / pass_hits() then address that
+        1 x P image (a single view keeps its H x W shape)."""
+        n = len(views)
+        vd = (ViewDesc * n)()
+        for i, (cam, w, h) in enumerate(views):
+            vd[i].camera = camera_desc(cam)
+            vd[i].width, vd[i].height = int(w), int(h)
+        P = sum(int(w) * int(h) for _, w, h in views)
+        W, H = (int(views[0][1]), int(views[0][2])) if n == 1 else (P, 1)
+        self.cfg = dataclasses.replace(cfg, width=W, height=H)
+        st = FrameStats()
+        self._check(self.L.rsh_engine_render_views(self.h, vd, n, C.byref(config_desc(cfg)), flags, C.byref(st)),
+                    "render_views")
+        self.stats = st
+        return st
+
+    @staticmethod
+    def split_views(pixels, views):
+        """Per-view H x W images from the pixels of a batch."""
+        flat = np.asarray(pixels).reshape(-1)
+        out, off = [], 0
+        for _, w, h in views:
+            out.append(flat[off:off + w * h].reshape(h, w))
+            off += w * h
+        return out
 
     def pixels(self):
         W, H = self.cfg.width, self.cfg.height

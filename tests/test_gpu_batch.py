@@ -1,0 +1,123 @@
+"""Batched views (rsh_engine_render_views) and graph replay (render flag bit 3).
+
+A batch traces every view's rays together, with the view id in the key bits
+above the octahedral hash (segment_shift), so it is checked bit for bit
+against the oracle's batch (oracle_render_views), and per view against
+single-view frames.  Graph replays are checked against the eager frame, with
+other eager work and synchronisation between the replays.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from gpu_common import compare_frames
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def engine(rsh):
+    e = rsh.Engine(0)
+    yield e
+    e.close()
+
+
+def batch_cfg(rsh, scene, base, **kw):
+    cfg = dataclasses.replace(base, cell_bits=4, dir_bits=6, segment_shift=26, **kw)
+    return cfg.with_scene_bounds(scene)
+
+
+def random_views(rsh, seed, n, lo, hi):
+    rng = np.random.default_rng(seed)
+    views = []
+    for _ in range(n):
+        w, h = int(rng.integers(lo, hi + 1)), int(rng.integers(lo, hi + 1))
+        views.append((rsh.scenes.random_view(rng, w, h), w, h))
+    return views
+
+
+@pytest.mark.parametrize("variant", [dict(), dict(algorithm=1), dict(bounces=2, bounce_shadows=1), dict(leaf=4, branch=16)])
+def test_batch_matches_oracle(engine, oracle, rsh, variant):
+    scene = rsh.scenes.config0_scene()
+    views = random_views(rsh, 11, 7, 24, 80)
+    cfg = batch_cfg(rsh, scene, rsh.config.config0(64, 64), **variant)
+    engine.load_scene(scene)
+    engine.render_views(views, cfg, flags=engine.KEEP_PASSES)
+    ofr = oracle.render_views(scene, views, cfg, threads=0)
+    compare_frames(engine, ofr, engine.cfg)
+
+
+def test_batch_config1_matches_oracle(engine, oracle, rsh):
+    scene = rsh.scenes.config1_scene(boxes=200)
+    views = random_views(rsh, 12, 5, 32, 72)
+    cfg = batch_cfg(rsh, scene, rsh.config.config1(64, 64))
+    engine.load_scene(scene)
+    engine.render_views(views, cfg, flags=engine.KEEP_PASSES)
+    compare_frames(engine, oracle.render_views(scene, views, cfg, threads=0), engine.cfg)
+
+
+def test_batch_equals_per_view_frames(engine, rsh):
+    """Views traced together give the images of views traced alone (up to
+    rays where the different hierarchy grouping moves a culling decision;
+    none are expected, at most 1e-4 of pixels allowed)."""
+    scene = rsh.scenes.config1_scene(boxes=300)
+    views = random_views(rsh, 13, 9, 48, 160)
+    cfg = batch_cfg(rsh, scene, rsh.config.config1(64, 64))
+    engine.load_scene(scene)
+    engine.render_views(views, cfg)
+    parts = engine.split_views(engine.pixels(), views)
+    bad = total = 0
+    for (cam, w, h), img in zip(views, parts):
+        engine.render(cam, dataclasses.replace(cfg, width=w, height=h))
+        bad += int((engine.pixels() != img).sum())
+        total += w * h
+    assert bad <= 1e-4 * total, (bad, total)
+
+
+def test_single_view_batch_equals_render(engine, rsh):
+    scene = rsh.scenes.config0_scene()
+    cam = rsh.scenes.config0_camera(96, 80)
+    cfg = rsh.config.config0(96, 80).with_scene_bounds(scene)
+    engine.load_scene(scene)
+    engine.render(cam, cfg)
+    a = engine.pixels()
+    engine.render_views([(cam, 96, 80)], cfg)
+    np.testing.assert_array_equal(engine.pixels(), a)
+
+
+def test_graph_replay_matches_eager(engine, rsh):
+    scene = rsh.scenes.config1_scene(boxes=300)
+    views = random_views(rsh, 14, 6, 64, 128)
+    cfg = batch_cfg(rsh, scene, rsh.config.config1(64, 64))
+    engine.load_scene(scene)
+    engine.render_views(views, cfg)
+    eager = engine.pixels().copy()
+    st = engine.render_views(views, cfg, flags=engine.GRAPH)          # eager frame + capture
+    assert st.graph_replayed == 0
+    np.testing.assert_array_equal(engine.pixels(), eager)
+    small = rsh.scenes.config1_camera(32, 32)
+    for i in range(3):
+        engine.render(small, dataclasses.replace(cfg, width=32, height=32, segment_shift=0))   # eager work between
+        engine.render_views(views, cfg)                               # eager batch: no reallocation
+        np.testing.assert_array_equal(engine.pixels(), eager)
+        st = engine.render_views(views, cfg, flags=engine.GRAPH)
+        assert st.graph_replayed == 1, i
+        np.testing.assert_array_equal(engine.pixels(), eager, err_msg="replay %d" % i)
+
+
+def test_graph_rejects_other_flags(engine, rsh):
+    scene = rsh.scenes.config0_scene()
+    engine.load_scene(scene)
+    cam = rsh.scenes.config0_camera(32, 32)
+    with pytest.raises(rsh.RshError):
+        engine.render(cam, rsh.config.config0(32, 32), flags=engine.GRAPH | engine.TIMING)
+
+
+def test_batch_rejects_view_ids_that_do_not_fit(engine, rsh):
+    scene = rsh.scenes.config0_scene()
+    engine.load_scene(scene)
+    views = random_views(rsh, 15, 3, 16, 24)
+    bad = dataclasses.replace(rsh.config.config0(64, 64), segment_shift=26).with_scene_bounds(scene)   # 29-bit hash
+    with pytest.raises(rsh.RshError):
+        engine.render_views(views, bad)

@@ -394,3 +394,15 @@ def random_view(rng: np.random.Generator, width: int, height: int) -> Camera:
     while np.linalg.norm(target - eye) < 0.2:
         target = rng.uniform(0.2, 0.8, size=3)
     return camera_basis(world(eye), world(target), (0, 1, 0), 60.0, width, height)
+
+
+def batch_views(n: int = 64, seed: int = SEED_BASE + 3, lo: int = 128, hi: int = 1024):
+    """Batched-views workload: n square views of side uniform in [lo, hi]
+    (E[pixels] = 398,677 for 128..1024) at random poses in the unit room.
+    Returns [(camera, width, height)]."""
+    rng = np.random.default_rng(seed)
+    views = []
+    for _ in range(n):
+        side = int(rng.integers(lo, hi + 1))
+        views.append((random_view(rng, side, side), side, side))
+    return views

@@ -161,6 +161,22 @@ typedef struct rsh_view {
 rsh_status rsh_engine_render_views(rsh_engine* e, const rsh_view* views, uint32_t n,
                                    const rsh_frame_config* cfg, uint32_t flags, rsh_frame_stats* stats);
 
+/* Ray query: trace n caller rays against the loaded scene through the same
+ * pipeline (octahedral or reference key, trim, compress, sort, decompress,
+ * hierarchy, traversal).  Device buffers:
+ *   rays      float[6n]: origin xyz, direction xyz per ray (the T1 layout)
+ *   tmax      float[n], any hit only: a hit counts when eps < t < tmax
+ *   segments  optional uint32[n]: id placed in key bits cfg->segment_shift..31
+ *             (merged requests are sorted apart; octahedral key below the shift)
+ *   out       kind 0 closest hit: uint64[n] (hit_mode packing, miss = all ones
+ *             in EXACT mode, 0xFFFFFFFF in REF mode); kind 1 any hit: uint32[n],
+ *             0xFFFFFFFF occluded, 0 clear.
+ * Only cfg's hierarchy, key, hit and culling fields are used; t is measured
+ * from origin + direction * 0.01 as for frame rays.  flags: bits 1, 2. */
+rsh_status rsh_engine_trace(rsh_engine* e, const float* rays, const float* tmax, const uint32_t* segments,
+                            uint32_t n, uint32_t kind, const rsh_frame_config* cfg, void* out,
+                            uint32_t flags, rsh_frame_stats* stats);
+
 rsh_status rsh_engine_read_pixels(rsh_engine* e, uint32_t* host_rgba, size_t count);
 /* Primary G-buffer: diffuse, specular, position, normal, each float4[W*H]. */
 rsh_status rsh_engine_read_gbuffer(rsh_engine* e, float* host_dif, float* host_spec,
@@ -170,6 +186,51 @@ rsh_status rsh_engine_read_gbuffer(rsh_engine* e, float* host_dif, float* host_s
 rsh_status rsh_engine_read_pass(rsh_engine* e, uint32_t pass, uint64_t* host, size_t count);
 /* Device pointer of the pixel buffer (uint32[W*H], RGBA8 R|G<<8|B<<16). */
 void* rsh_engine_pixels_device(rsh_engine* e);
+
+/* ------------------------------------------------------------- serving --
+ * A request loop over device-resident scenes (no counterpart in the
+ * reference, whose GLUT loop renders one frame at a time, main.cpp:1234-1303).
+ * Each scene gets its own engine and HIP stream and one worker thread.
+ * Requests (ray queries, as rsh_engine_trace) are admitted in arrival order
+ * while the sum of in-flight requests' device bytes (256 per ray) stays within
+ * the budget; a scene's worker merges every admitted request of the head
+ * request's kind (up to max_batch_rays rays and 64 requests) into one traced
+ * batch, each request's index in the batch placed above the 26-bit octahedral
+ * key so the sort keeps requests apart.  Requests arriving while a batch runs
+ * join the next one.  Host buffers passed to submit must stay valid until
+ * rsh_server_wait returns for that request (pinned memory from
+ * rsh_host_alloc copies fastest). */
+typedef struct rsh_server rsh_server;
+typedef struct rsh_request_info {
+    double queue_ms;         /* submit -> start of the batch that traced it */
+    double latency_ms;       /* submit -> results in the caller's buffer     */
+    uint32_t batch_requests; /* requests traced in that batch                */
+} rsh_request_info;
+typedef struct rsh_server_info {
+    uint64_t batches, requests, rays;
+    double busy_ms;          /* summed over scenes (scenes overlap) */
+    uint32_t pending;        /* waiting for budget */
+    uint64_t admitted_bytes;
+} rsh_server_info;
+
+rsh_status rsh_server_create(rsh_server** out, int device, uint64_t hbm_budget_bytes, uint32_t max_batch_rays);
+/* cfg: hierarchy / key / hit fields used for the scene's queries; the
+ * octahedral key must fit 26 bits.  Before the first submit. */
+rsh_status rsh_server_add_scene(rsh_server* s, const rsh_scene_desc* scene, const rsh_frame_config* cfg,
+                                uint32_t* scene_id);
+/* kind 0 closest hit (host_out uint64[n]); kind 1 any hit with host_tmax
+ * (host_out uint32[n]).  Non-blocking. */
+rsh_status rsh_server_submit(rsh_server* s, uint32_t scene_id, uint32_t kind, const float* host_rays,
+                             const float* host_tmax, uint32_t n, void* host_out, uint64_t* request_id);
+/* Block until the request has completed; info may be NULL.  Each request is
+ * waited for once. */
+rsh_status rsh_server_wait(rsh_server* s, uint64_t request_id, rsh_request_info* info);
+rsh_status rsh_server_stats(rsh_server* s, rsh_server_info* info);
+const char* rsh_server_last_error(const rsh_server* s);
+/* Stops the workers after the admitted requests; wait for every request first. */
+void rsh_server_destroy(rsh_server* s);
+void* rsh_host_alloc(size_t bytes);
+void rsh_host_free(void* p);
 
 /* Device memory helpers (plain HIP calls, for callers without a HIP runtime
  * binding of their own). */

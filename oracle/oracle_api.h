@@ -95,6 +95,10 @@ typedef struct oracle_view {
 } oracle_view;
 int oracle_render_views(const oracle_scene* scene, const oracle_view* views, uint32_t n,
                         const oracle_config* config, oracle_frame_out* out, int threads);
+// Ray query (rsh_engine_trace): out = uint64 per ray (closest-hit record, or
+// 0xFFFFFFFF occluded / 0 clear for kind 1).
+int oracle_trace_rays(const oracle_scene* scene, const float* rays, const float* tmax, const uint32_t* segs,
+                      uint32_t n, uint32_t kind, const oracle_config* config, uint64_t* out, int threads);
 
 // Math and keys.
 float oracle_acos(float x);

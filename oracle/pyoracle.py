@@ -92,6 +92,8 @@ def lib():
         L.oracle_render.restype = C.c_int
         L.oracle_render.argtypes = [C.POINTER(OScene), C.POINTER(OCamera), C.POINTER(OConfig), PF,
                                     C.POINTER(OFrameOut), C.c_int]
+        L.oracle_trace_rays.restype = C.c_int
+        L.oracle_trace_rays.argtypes = [C.POINTER(OScene), PF, PF, PU, u32, u32, C.POINTER(OConfig), PU64, C.c_int]
         L.oracle_render_views.restype = C.c_int
         L.oracle_render_views.argtypes = [C.POINTER(OScene), C.POINTER(OView), u32, C.POINTER(OConfig),
                                           C.POINTER(OFrameOut), C.c_int]
@@ -262,3 +264,20 @@ def render_views(scene, views, cfg, threads=0, keep_hits=True, keep_gbuffer=True
                            pairs_leaf=out.pass_pairs_leaf[i], ray_tests=out.pass_ray_tests[i],
                            hits=hits[i][:out.pass_slots[i]].copy() if keep_hits else None))
     return Frame(pixels.reshape(H, W), gb, passes)
+
+
+def trace_rays(scene, rays, kind, cfg, tmax=None, segments=None, threads=0):
+    """Ray query: rays float32 [n, 6]; returns uint64 [n] (closest-hit records,
+    or 0xFFFFFFFF occluded / 0 clear for kind 1)."""
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+    n = rays.shape[0]
+    tm = None if tmax is None else np.ascontiguousarray(tmax, np.float32)
+    sg = None if segments is None else np.ascontiguousarray(segments, np.uint32)
+    out = np.zeros(n, np.uint64)
+    os_ = make_scene(scene)
+    rc = lib().oracle_trace_rays(C.byref(os_), fp(rays), fp(tm) if tm is not None else None,
+                                 up(sg) if sg is not None else None, n, kind, C.byref(make_config(cfg)),
+                                 u64p(out), threads)
+    if rc != 0:
+        raise RuntimeError("oracle_trace_rays failed: %d" % rc)
+    return out

@@ -437,9 +437,10 @@ struct Traverser {
 
 // Build one batch's hierarchy and run traversal + intersection.
 // kind 0: closest hit (RT:1711-1774), kind 1: shadow any-hit (RT:1498-1614).
+// tmax (any hit, optional): the caller's per-slot limit instead of |o - P| - 2 eps.
 void trace(const Scene& sc, const oracle_config& cfg, const Batch& b, uint32_t kind,
            const GBuf* shadow_g, uint32_t Lmax, std::vector<uint64_t>& result, PassStats& st,
-           int threads) {
+           int threads, const float* tmax = nullptr) {
     (void)threads;
     std::vector<uint32_t> tk, tv, sk, sv;
     trim(b, tk, tv);
@@ -470,9 +471,15 @@ void trace(const Scene& sc, const oracle_config& cfg, const Batch& b, uint32_t k
                 float th = ray_triangle(o + d * EPS, d, v0, e1, e2);
                 ltests++;
                 if (kind == 1) {
-                    uint32_t x = slot % W, y = slot / (W * Lmax);
-                    f3 P = xyz(shadow_g->pos[(size_t)x + (size_t)y * W]);
-                    if (th > EPS && th < length(o - P) - EPS * 2.0f)
+                    float lim;
+                    if (tmax) {
+                        lim = tmax[slot];
+                    } else {
+                        uint32_t x = slot % W, y = slot / (W * Lmax);
+                        f3 P = xyz(shadow_g->pos[(size_t)x + (size_t)y * W]);
+                        lim = length(o - P) - EPS * 2.0f;
+                    }
+                    if (th > EPS && th < lim)
                         __atomic_store_n(&res[slot], (uint64_t)MISS32, __ATOMIC_RELAXED);
                 } else if (th > EPS) {
                     atomic_min64(&res[slot], pack_hit(th, t, cfg.hit_mode));
@@ -972,6 +979,36 @@ int oracle_render_views(const oracle_scene* scene, const oracle_view* views, uin
     cfg.height = n == 1 ? views[0].height : 1u;
     cfg.row_begin = cfg.row_end = 0;
     return render_views(scene, vs, cfg, nullptr, out, threads);
+}
+
+// Ray query (rsh_engine_trace): n rays float[6n], keyed as frame rays of type
+// kind (0 closest -> type 0, 1 any -> type 1) with segs at segment_shift.
+int oracle_trace_rays(const oracle_scene* scene, const float* rays, const float* tmax, const uint32_t* segs,
+                      uint32_t n, uint32_t kind, const oracle_config* config, uint64_t* out, int threads) {
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#endif
+    const oracle_config& cfg = *config;
+    if (n == 0 || kind > 1 || (kind == 1 && !tmax) || cfg.leaf == 0 || cfg.branch == 0 || cfg.depth == 0) return -1;
+    Scene sc(scene);
+    Batch b;
+    b.init(n);
+    for (uint32_t i = 0; i < n; i++) {
+        const f3 o = ld3(rays + 6 * (size_t)i), d = ld3(rays + 6 * (size_t)i + 3);
+        b.rays[2 * (size_t)i] = o;
+        b.rays[2 * (size_t)i + 1] = d;
+        if (cfg.key_mode == 0) b.keys[i] = kind == 1 ? shadow_key_ref(o, d, 0) : reflection_key_ref(o, d);
+        else b.keys[i] = view_key(cfg, o, d, kind == 1 ? 1u : 0u, segs ? segs[i] : 0u);
+        b.values[i] = i;
+        b.flags[i] = 0;
+    }
+    std::vector<uint64_t> res;
+    PassStats st;
+    oracle_config c1 = cfg;
+    c1.width = n; c1.height = 1;
+    trace(sc, c1, b, kind, nullptr, 1, res, st, threads, tmax);
+    std::copy(res.begin(), res.end(), out);
+    return 0;
 }
 
 uint64_t oracle_leaf_pairs_literal(const float* hierarchy, const uint32_t* level_counts, uint32_t depth,

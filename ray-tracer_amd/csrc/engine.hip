@@ -259,7 +259,7 @@ rsh_status build_items(rsh_engine* e, const rsh_frame_config& cfg, TravParams& p
 // hierarchy -> fused traversal + intersection.  kind 0: closest hit into
 // e->closest; kind 1: shadow flags into e->shadow.
 rsh_status trace_pass(rsh_engine* e, const rsh_frame_config& cfg, const float* rays, uint32_t S, uint32_t kind,
-                      const float4* shadow_pos, uint32_t flags_mask, PassOut& out) {
+                      const float4* shadow_pos, uint32_t flags_mask, PassOut& out, const float* tmax = nullptr) {
     out = PassOut{};
     hipStream_t s = e->stream;
     const bool timing = flags_mask & 4u, counting = flags_mask & 2u;
@@ -316,7 +316,8 @@ rsh_status trace_pass(rsh_engine* e, const rsh_frame_config& cfg, const float* r
     level_sizes(N, cfg.leaf, cfg.branch, cfg.depth, counts, offs);
     {
         StageTimer t(e, ST_HIER, timing);
-        launch_gather(rays, sorted_values, N, kind == 1 ? shadow_pos : nullptr, e->W, e->Lmax, f4p(e->rec0),
+        launch_gather(rays, sorted_values, N, kind == 1 ? shadow_pos : nullptr, kind == 1 ? tmax : nullptr, e->W,
+                      e->Lmax, f4p(e->rec0),
                       f4p(e->rec1), s);
         launch_hierarchy(f4p(e->rec0), f4p(e->rec1), nullptr, nullptr, N, cfg.leaf, cfg.branch, cfg.depth,
                          cfg.initial_radius, cfg.initial_spread, cfg.cone_mode, f4p(e->nodes), e->aux.as<float2>(),
@@ -780,6 +781,83 @@ rsh_status rsh_engine_render_views(rsh_engine* e, const rsh_view* views, uint32_
     }
     if (off > 0xFFFFFFFFull) { e->err = "render_views: batch exceeds 2^32 pixels"; return RSH_CAPACITY; }
     return render(e, hv.data(), n, *cfgp, fl, st);
+}
+
+rsh_status rsh_engine_trace(rsh_engine* e, const float* rays, const float* tmax, const uint32_t* segments, uint32_t n,
+                            uint32_t kind, const rsh_frame_config* cfgp, void* out, uint32_t fl, rsh_frame_stats* st) {
+    if (!e || !rays || !cfgp || !out || n == 0 || kind > 1 || (kind == 1 && !tmax)) {
+        if (e) e->err = "trace: null buffer, empty query, kind > 1 or any-hit without tmax";
+        return RSH_INVALID_ARG;
+    }
+    if (fl & 9u) { e->err = "trace: flags take counters (bit 1) and timing (bit 2) only"; return RSH_INVALID_ARG; }
+    if (e->T == 0) { e->err = "trace: no scene loaded"; return RSH_INVALID_ARG; }
+    const rsh_frame_config cfg = *cfgp;
+    if (cfg.leaf == 0 || cfg.branch == 0 || cfg.depth == 0 || cfg.depth > 4 || cfg.algorithm > 1 ||
+        cfg.key_mode > 1 || cfg.hit_mode > 1 || cfg.cone_mode > 1) {
+        e->err = "trace: invalid configuration";
+        return RSH_INVALID_ARG;
+    }
+    if (cfg.key_mode == RSH_KEY_OCT && (cfg.cell_bits > 8 || cfg.dir_bits > 10 || cfg.cell_bits == 0 ||
+                                        cfg.dir_bits == 0 || 2 + 3 * cfg.cell_bits + 2 * cfg.dir_bits > 32)) {
+        e->err = "trace: octahedral key does not fit 32 bits";
+        return RSH_INVALID_ARG;
+    }
+    if (segments && (cfg.key_mode != RSH_KEY_OCT || cfg.segment_shift == 0 || cfg.segment_shift > 31 ||
+                     2 + 3 * cfg.cell_bits + 2 * cfg.dir_bits > cfg.segment_shift)) {
+        e->err = "trace: segment ids need an octahedral key below segment_shift";
+        return RSH_INVALID_ARG;
+    }
+    if (cfg.hit_mode == RSH_HIT_REF && e->T > (1u << 20)) {
+        e->err = "trace: reference hit packing holds 20-bit triangle ids";
+        return RSH_CAPACITY;
+    }
+    if ((uint64_t)n > (uint64_t)MAX_TILES * TILE) { e->err = "trace: query exceeds slot capacity"; return RSH_CAPACITY; }
+    HIP_OK(hipSetDevice(e->device));
+    rsh_status rs = ensure_frame(e, n, 1, 1);
+    if (rs != RSH_OK) return rs;
+    if ((rs = build_ranges(e, cfg.triangle_batch)) != RSH_OK) return rs;
+    {
+        uint32_t cnt[4], off[4];
+        size_t total = level_sizes(n, cfg.leaf, cfg.branch, cfg.depth, cnt, off);
+        HIP_OK(e->nodes.reserve(total * 32 + 64));
+        HIP_OK(e->aux.reserve(total * 8 + 64));
+    }
+    e->timed.clear();
+    e->ev_used = 0;
+    e->pass_index = 0;
+    if (st) std::memset(st, 0, sizeof(*st));
+    hipStream_t s = e->stream;
+    const bool timing = fl & 4u;
+    {
+        StageTimer t(e, ST_CREATE, timing);
+        launch_query_keys(rays, segments, n, key_cfg(cfg), kind == 1 ? 1u : 0u, u32p(e->flags), u32p(e->keys),
+                          u32p(e->values), s);
+        HIP_OK(hipGetLastError());
+    }
+    PassOut po;
+    if ((rs = trace_pass(e, cfg, rays, n, kind, nullptr, fl, po, tmax)) != RSH_OK) return rs;
+    if (st && st->pass_count < RSH_MAX_PASSES) {
+        const uint32_t i = st->pass_count++;
+        st->pass_kind[i] = kind; st->pass_slots[i] = n; st->pass_rays[i] = po.N; st->pass_chunks[i] = po.C;
+        st->pass_pairs_top[i] = po.top; st->pass_pairs_leaf[i] = po.leaf; st->pass_ray_tests[i] = po.tests;
+        st->pass_top_survivors[i] = po.surv; st->pass_items[i] = po.items; st->pass_chunk[i] = po.chunk;
+        st->pass_splits[i] = po.splits; st->pass_tri_tests[i] = po.tri_tests; st->pass_cluster_tests[i] = po.cl_tests;
+        st->pass_leaf_steps[i] = po.b_steps; st->pass_ray_steps[i] = po.c_steps;
+        st->rays_total = po.N;
+    }
+    HIP_OK(hipMemcpyAsync(out, kind == 1 ? e->shadow.p : e->closest.p, (size_t)n * (kind == 1 ? 4 : 8),
+                          hipMemcpyDeviceToDevice, s));
+    HIP_OK(hipStreamSynchronize(s));
+    if (st && timing) {
+        for (auto& t : e->timed) {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, t.second.first, t.second.second) != hipSuccess) continue;
+            const int k = t.first >= ST_PASS_TRAVERSE ? ST_TRAVERSE : t.first;
+            st->stage_ms[k] += ms;
+            if (t.first >= ST_PASS_TRAVERSE) st->pass_traverse_ms[0] += ms;
+        }
+    }
+    return RSH_OK;
 }
 
 rsh_status rsh_engine_read_pixels(rsh_engine* e, uint32_t* host, size_t count) {

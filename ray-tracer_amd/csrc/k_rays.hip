@@ -69,6 +69,19 @@ __global__ void __launch_bounds__(256) k_create_primary(Views V, uint32_t P, Key
     }
 }
 
+// Ray queries: the caller's rays, keyed like frame rays of the given type.
+__global__ void __launch_bounds__(256) k_query_keys(const float* rays, const uint32_t* segs, uint32_t n, KeyCfg kc,
+                                                    uint32_t type, uint32_t* flags, uint32_t* keys,
+                                                    uint32_t* values) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        v3 o, d;
+        load_ray(rays, i, o, d);
+        keys[i] = make_key(kc, o, d, type, 0, segs ? segs[i] : 0u);
+        values[i] = i;
+        flags[i] = 0;
+    }
+}
+
 // RayTracer.cu:682-740; slot i = x + W*(l + Lmax*y).
 __global__ void __launch_bounds__(256) k_create_shadow(const float4* gpos, const float4* gnrm, uint32_t W, uint32_t H,
                                                        uint32_t Lmax, const float4* light_pos, KeyCfg kc, Views V,
@@ -365,6 +378,10 @@ __global__ void __launch_bounds__(256) k_color_reflection_ref(const float4* tp, 
 
 }  // namespace
 
+void launch_query_keys(const float* rays, const uint32_t* segs, uint32_t n, KeyCfg kc, uint32_t type,
+                       uint32_t* flags, uint32_t* keys, uint32_t* values, hipStream_t s) {
+    if (n) k_query_keys<<<grid_for(n, 256, 65536), 256, 0, s>>>(rays, segs, n, kc, type, flags, keys, values);
+}
 void launch_create_primary(const Views& V, uint32_t P, KeyCfg kc, float* rays, uint32_t* flags, uint32_t* keys,
                            uint32_t* values, hipStream_t s) {
     k_create_primary<<<grid_for(P, 256, 65536), 256, 0, s>>>(V, P, kc, rays, flags, keys, values);

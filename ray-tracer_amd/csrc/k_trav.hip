@@ -37,17 +37,20 @@ inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap = 1u << 20) {
 // -------------------------------------------------------------- gather ----
 // Sorted ray records: rec0 = (origin, slot bits), rec1 = (direction, tmax).
 // tmax = |o - P| - 2 eps for shadow rays (RayTracer.cu:1610), P the surface
-// point of the slot's pixel; unused for closest-hit passes.
+// point of the slot's pixel, or the caller's per-slot tmax for ray queries;
+// unused for closest-hit passes.
 __global__ void __launch_bounds__(256) k_gather(const float* rays, const uint32_t* sorted_values, uint32_t N,
-                                                const float4* shadow_pos, uint32_t W, uint32_t Lmax,
-                                                float4* rec0, float4* rec1) {
+                                                const float4* shadow_pos, const float* tmax_in, uint32_t W,
+                                                uint32_t Lmax, float4* rec0, float4* rec1) {
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
         uint32_t slot = sorted_values[p];
         const float2* r = reinterpret_cast<const float2*>(rays + 6 * (size_t)slot);
         float2 a = r[0], b = r[1], c = r[2];
         v3 o = mk(a.x, a.y, b.x), d = mk(b.y, c.x, c.y);
         float tmax = 0.0f;
-        if (shadow_pos) {
+        if (tmax_in) {
+            tmax = tmax_in[slot];
+        } else if (shadow_pos) {
             uint32_t x = slot % W, y = slot / (W * Lmax);
             v3 P = xyz(shadow_pos[(size_t)x + (size_t)y * W]);
             tmax = len(o - P) - EPS * 2.0f;
@@ -794,9 +797,10 @@ __global__ void __launch_bounds__(256) k_hits_intersect(const float* rays, const
 }  // namespace
 
 // ================================================================ launchers
-void launch_gather(const float* rays, const uint32_t* sorted_values, uint32_t N, const float4* shadow_pos, uint32_t W,
-                   uint32_t Lmax, float4* rec0, float4* rec1, hipStream_t s) {
-    if (N) k_gather<<<grid_for(N, 256, 65536), 256, 0, s>>>(rays, sorted_values, N, shadow_pos, W, Lmax, rec0, rec1);
+void launch_gather(const float* rays, const uint32_t* sorted_values, uint32_t N, const float4* shadow_pos,
+                   const float* tmax, uint32_t W, uint32_t Lmax, float4* rec0, float4* rec1, hipStream_t s) {
+    if (N) k_gather<<<grid_for(N, 256, 65536), 256, 0, s>>>(rays, sorted_values, N, shadow_pos, tmax, W, Lmax, rec0,
+                                                            rec1);
 }
 
 void launch_hierarchy(const float4* rec0, const float4* rec1, const float* rays, const uint32_t* sorted_values,

@@ -179,8 +179,12 @@ void launch_color_reflection_ref(const float4* tp, const float4* tn, uint32_t W,
                                  uint32_t* flags, uint32_t* keys, uint32_t* values, uint32_t* pixels, hipStream_t s);
 
 // --- hierarchy / traversal ---
-void launch_gather(const float* rays, const uint32_t* sorted_values, uint32_t N, const float4* shadow_pos, uint32_t W,
-                   uint32_t Lmax, float4* rec0, float4* rec1, hipStream_t s);
+void launch_gather(const float* rays, const uint32_t* sorted_values, uint32_t N, const float4* shadow_pos,
+                   const float* tmax, uint32_t W, uint32_t Lmax, float4* rec0, float4* rec1, hipStream_t s);
+// Ray queries: key, value (= index) and live flag of n caller rays (float[6n]);
+// segs (optional) places a per-ray segment id at kc.h.segment_shift.
+void launch_query_keys(const float* rays, const uint32_t* segs, uint32_t n, KeyCfg kc, uint32_t type,
+                       uint32_t* flags, uint32_t* keys, uint32_t* values, hipStream_t s);
 void launch_hierarchy(const float4* rec0, const float4* rec1, const float* rays, const uint32_t* sorted_values,
                       uint32_t N, uint32_t leaf, uint32_t branch, uint32_t depth, float r0, float s0, uint32_t mode,
                       float4* nodes, float2* aux, const uint32_t* counts, const uint32_t* offsets, hipStream_t s);

@@ -98,7 +98,8 @@ BINDERS = ["bindDiffuseTextureArray", "bindSpecularTextureArray", "bindFragmentP
            "bindMaterialDiffuseProperties", "bindMaterialSpecularProperties", "bindBoundingSpheres",
            "bindLightPositions", "bindLightColors", "bindLightIntensities"]
 NATIVE_SYMBOLS = ["rsh_engine_create", "rsh_engine_destroy", "rsh_last_error", "rsh_engine_set_stream",
-                  "rsh_engine_load_scene", "rsh_engine_render", "rsh_engine_render_views", "rsh_engine_read_pixels",
+                  "rsh_engine_load_scene", "rsh_engine_render", "rsh_engine_render_views", "rsh_engine_trace",
+                  "rsh_engine_read_pixels",
                   "rsh_engine_read_gbuffer", "rsh_engine_read_pass", "rsh_engine_pixels_device",
                   "rsh_device_alloc", "rsh_device_free", "rsh_memcpy_htod", "rsh_memcpy_dtoh",
                   "rsh_memset_d32", "rsh_device_synchronize", "rsh_device_count",
@@ -135,6 +136,9 @@ def lib():
     L.rsh_engine_render_views.restype = C.c_int
     L.rsh_engine_render_views.argtypes = [PV, C.POINTER(ViewDesc), u32, C.POINTER(FrameConfigDesc), u32,
                                           C.POINTER(FrameStats)]
+    L.rsh_engine_trace.restype = C.c_int
+    L.rsh_engine_trace.argtypes = [PV, PV, PV, PV, u32, u32, C.POINTER(FrameConfigDesc), PV, u32,
+                                   C.POINTER(FrameStats)]
     L.rsh_engine_read_pixels.restype = C.c_int
     L.rsh_engine_read_pixels.argtypes = [PV, PU, C.c_size_t]
     L.rsh_engine_read_gbuffer.restype = C.c_int
@@ -312,6 +316,33 @@ class Engine:
                     "render_views")
         self.stats = st
         return st
+
+    def trace(self, rays, kind, cfg, tmax=None, segments=None, flags=0):
+        """Ray query through the GPU pipeline: rays float32 [n, 6] (origin,
+        direction); kind 0 closest hit -> uint64 records, kind 1 any hit with
+        per-ray tmax -> uint32 (0xFFFFFFFF occluded)."""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        n = rays.shape[0]
+        bufs = [DeviceArray.from_host(rays)]
+        tm = sg = None
+        if tmax is not None:
+            tm = DeviceArray.from_host(np.ascontiguousarray(tmax, np.float32))
+            bufs.append(tm)
+        if segments is not None:
+            sg = DeviceArray.from_host(np.ascontiguousarray(segments, np.uint32))
+            bufs.append(sg)
+        out = DeviceArray((n,), np.uint32 if kind == 1 else np.uint64)
+        bufs.append(out)
+        try:
+            st = FrameStats()
+            self._check(self.L.rsh_engine_trace(self.h, bufs[0].ptr, tm.ptr if tm else None, sg.ptr if sg else None,
+                                                n, kind, C.byref(config_desc(cfg)), out.ptr, flags, C.byref(st)),
+                        "trace")
+            self.stats = st
+            return out.download()
+        finally:
+            for b in bufs:
+                b.close()
 
     @staticmethod
     def split_views(pixels, views):

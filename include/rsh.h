@@ -80,7 +80,8 @@ typedef struct rsh_frame_config {
     uint32_t bounce_shadows;        /* 1: shadow rays at bounce hits                     */
     uint32_t primary_rays;          /* must be 1 (the G-buffer comes from primary rays)  */
     uint32_t cell_bits, dir_bits;   /* octahedral key: 3*cell_bits + 2*dir_bits + 2 <= 32 */
-    uint32_t key_layout;            /* 0 type|cell|dir, 1 type|dir|cell                  */
+    uint32_t key_layout;            /* 0 type|cell|dir, 1 type|dir|cell, 2 type|bits of
+                                       cell z,y,x and direction u,v interleaved          */
     float scene_min[3];
     float cell_scale[3];            /* cells per world unit                              */
     uint32_t segment_shift;         /* batches: key bit of the view id (0 = not in key) */

@@ -57,7 +57,7 @@ typedef struct oracle_config {
     uint32_t bounce_shadows;        // 1: shadow rays at bounce hits
     uint32_t primary_rays;          // 1: trace primary rays (else G-buffer given)
     uint32_t cell_bits, dir_bits;   // octahedral key split
-    uint32_t key_layout;            // 0 type|cell|dir, 1 type|dir|cell
+    uint32_t key_layout;            // 0 type|cell|dir, 1 type|dir|cell, 2 interleaved z y x u v bits
     float scene_min[3];
     float cell_scale[3];            // cells per world unit along each axis
     uint32_t segment_shift;         // segment (view/request) id << shift

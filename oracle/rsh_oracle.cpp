@@ -78,6 +78,7 @@ uint32_t reflection_key_ref(f3 o, f3 d) {
 
 // Octahedral key (configuration hash; no counterpart in the reference):
 //   [type:2][cell z|y|x : 3*cb][u|v : 2*db]   (layout 0)
+//   [type:2][z y x u v bits interleaved, most significant level first] (layout 2)
 //   [type:2][u|v : 2*db][cell z|y|x : 3*cb]   (layout 1)
 // cell = min(sat((o - scene_min) * cell_scale), 2^cb - 1) per axis;
 // (u, v) = octahedral map of d onto [0, 2^db)^2.
@@ -100,7 +101,18 @@ uint32_t key_oct(const oracle_config& c, f3 o, f3 d, uint32_t type) {
     uint32_t v = std::min(f2u_sat((py * 0.5f + 0.5f) * fs), dmax);
     uint32_t cell = (cz << (2 * cb)) | (cy << cb) | cx;
     uint32_t dir = (u << db) | v;
-    uint32_t body = c.key_layout == 0 ? ((cell << (2 * db)) | dir) : ((dir << (3 * cb)) | cell);
+    uint32_t body;
+    if (c.key_layout == 2) {
+        // bit-interleaved, most significant level first: z y x u v at each level
+        body = 0;
+        const uint32_t top = cb > db ? cb : db;
+        for (uint32_t i = top; i-- > 0;) {
+            if (i < cb) body = (body << 3) | (((cz >> i) & 1u) << 2) | (((cy >> i) & 1u) << 1) | ((cx >> i) & 1u);
+            if (i < db) body = (body << 2) | (((u >> i) & 1u) << 1) | ((v >> i) & 1u);
+        }
+    } else {
+        body = c.key_layout == 0 ? ((cell << (2 * db)) | dir) : ((dir << (3 * cb)) | cell);
+    }
     return (type << (3 * cb + 2 * db)) | body;
 }
 

@@ -546,6 +546,7 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
     for (uint32_t i = 0; i < nv; i++) P64 += (uint64_t)hv[i].w * hv[i].h;
     const uint32_t width = nv == 1 ? hv[0].w : (uint32_t)P64, height = nv == 1 ? hv[0].h : 1u;
     if (width == 0 || height == 0 || cfg.leaf == 0 || cfg.branch == 0 || cfg.depth == 0 || cfg.depth > 4 ||
+        cfg.key_layout > 2 ||
         cfg.primary_rays != 1 || cfg.algorithm > 1 || cfg.key_mode > 1 || cfg.hit_mode > 1 || cfg.cone_mode > 1) {
         e->err = "render: invalid configuration";
         return RSH_INVALID_ARG;
@@ -793,7 +794,7 @@ rsh_status rsh_engine_trace(rsh_engine* e, const float* rays, const float* tmax,
     if (e->T == 0) { e->err = "trace: no scene loaded"; return RSH_INVALID_ARG; }
     const rsh_frame_config cfg = *cfgp;
     if (cfg.leaf == 0 || cfg.branch == 0 || cfg.depth == 0 || cfg.depth > 4 || cfg.algorithm > 1 ||
-        cfg.key_mode > 1 || cfg.hit_mode > 1 || cfg.cone_mode > 1) {
+        cfg.key_mode > 1 || cfg.hit_mode > 1 || cfg.cone_mode > 1 || cfg.key_layout > 2) {
         e->err = "trace: invalid configuration";
         return RSH_INVALID_ARG;
     }

@@ -216,7 +216,18 @@ __device__ inline uint32_t key_oct(const HashParams& h, v3 o, v3 d, uint32_t typ
     uint32_t v = min(f2u((py * 0.5f + 0.5f) * fs), dmax);
     uint32_t cell = (cz << (2 * cb)) | (cy << cb) | cx;
     uint32_t dir = (u << db) | v;
-    uint32_t body = h.layout == 0 ? ((cell << (2 * db)) | dir) : ((dir << (3 * cb)) | cell);
+    uint32_t body;
+    if (h.layout == 2) {
+        // bit-interleaved, most significant level first: z y x u v at each level
+        body = 0;
+        const uint32_t top = cb > db ? cb : db;
+        for (uint32_t i = top; i-- > 0;) {
+            if (i < cb) body = (body << 3) | (((cz >> i) & 1u) << 2) | (((cy >> i) & 1u) << 1) | ((cx >> i) & 1u);
+            if (i < db) body = (body << 2) | (((u >> i) & 1u) << 1) | ((v >> i) & 1u);
+        }
+    } else {
+        body = h.layout == 0 ? ((cell << (2 * db)) | dir) : ((dir << (3 * cb)) | cell);
+    }
     return (type << (3 * cb + 2 * db)) | body;
 }
 

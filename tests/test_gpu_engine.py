@@ -67,6 +67,7 @@ def test_config0_default(engine, oracle, rsh, w):
     dict(traversal_splits=5, traversal_kernel=1, bounces=2, bounce_shadows=1),
     dict(traversal_splits=4, depth=3, branch=4),
     dict(key_layout=1),
+    dict(key_layout=2),                      # bit-interleaved cell / direction key
     dict(triangle_batch=512),
     dict(bounces=2, bounce_shadows=1),
 ])

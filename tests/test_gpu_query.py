@@ -38,7 +38,8 @@ def test_query_matches_oracle(engine, oracle, rsh, mix, scene_name):
         assert (want != np.uint64(0xFFFFFFFFFFFFFFFF)).mean() > 0.5      # most queries hit something
 
 
-@pytest.mark.parametrize("variant", [dict(), dict(algorithm=1), dict(leaf=4, branch=16), dict(hit_mode=1)])
+@pytest.mark.parametrize("variant", [dict(), dict(algorithm=1), dict(leaf=4, branch=16), dict(hit_mode=1),
+                                     dict(key_layout=2)])
 def test_segmented_query_matches_oracle(engine, oracle, rsh, variant):
     scene = rsh.scenes.config0_scene()
     rng = np.random.default_rng(7)

@@ -162,3 +162,34 @@ def test_oct_key_edges(oracle):
     assert (got[1] >> 12) & 0x7FFF == 0x7FFF
     assert got[0] & 0xFFF == (32 << 6) | 32          # +z maps to the centre
     assert got[2] & 0xFFF == 0                       # NaN direction -> bin 0
+
+
+def test_oct_interleaved_layout_is_a_bit_shuffle_of_layout0(oracle, rsh):
+    """Layout 2 interleaves the z, y, x cell bits and u, v direction bits of
+    layout 0's fields, most significant level first, under the same type."""
+    import dataclasses
+    rng = np.random.default_rng(21)
+    o = rng.uniform(-40, 160, size=(300, 3)).astype(np.float32)
+    d = unit_dirs(rng, 300)
+    for cb, db in ((4, 6), (5, 5), (3, 7), (6, 3)):
+        base = dataclasses.replace(rsh.config.FrameConfig(), cell_bits=cb, dir_bits=db, scene_min=(0.0, 0.0, 0.0),
+                                   cell_scale=(float(np.float32((1 << cb) / 120.0)),) * 3)
+        L = oracle.lib()
+        for t in range(4):
+            c0 = oracle.make_config(dataclasses.replace(base, key_layout=0))
+            c2 = oracle.make_config(dataclasses.replace(base, key_layout=2))
+            k0 = keys_of(lambda a, b: L.oracle_key_oct(C.byref(c0), a, b, t), o, d)
+            k2 = keys_of(lambda a, b: L.oracle_key_oct(C.byref(c2), a, b, t), o, d)
+            nb = 3 * cb + 2 * db
+            assert np.all((k0 >> nb) == t) and np.all((k2 >> nb) == t)
+            v = (k0 & ((1 << db) - 1)).astype(np.int64)
+            u = ((k0 >> db) & ((1 << db) - 1)).astype(np.int64)
+            cell = (k0 >> (2 * db)).astype(np.int64)
+            cx, cy, cz = cell & ((1 << cb) - 1), (cell >> cb) & ((1 << cb) - 1), cell >> (2 * cb)
+            want = np.zeros(len(k0), np.int64)
+            for i in range(max(cb, db) - 1, -1, -1):
+                if i < cb:
+                    want = (want << 3) | (((cz >> i) & 1) << 2) | (((cy >> i) & 1) << 1) | ((cx >> i) & 1)
+                if i < db:
+                    want = (want << 2) | (((u >> i) & 1) << 1) | ((v >> i) & 1)
+            np.testing.assert_array_equal(k2 & ((1 << nb) - 1), want)

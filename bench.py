@@ -8,6 +8,15 @@ octahedral hashing, trimming, compression into packets, radix sort,
 decompression, hierarchy build, fused traversal + intersection, shading).
 Scene and camera are resident in HBM before timing starts.
 
+Beside the headline it measures, on the same GPU:
+  "batching"  64 views (square, side uniform 128..1024, random poses) of the
+              office-scale and atrium scenes: the plain pipeline (one frame
+              per view) against the batched form (all views in one pass per
+              stage, view id in the key) eager and as a replayed HIP graph;
+  "serving"   Poisson-arriving ray-query requests (1k-4M rays, primary /
+              shadow / diffuse 40/30/30) against 4 resident scenes at 70% of
+              the measured capacity: p50/p99 latency and Mrays/s.
+
 Prints one JSON line.  "roofline" describes the dominant kernel, the fused
 traversal (k_traverse64, one launch per pass): its launch durations come from
 HIP events recorded on the engine's stream around every launch of the timed
@@ -48,6 +57,9 @@ def parse():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the CPU sample (0: automatic)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-views leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-seconds", type=float, default=6.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="copy MIB MiB device-to-device 3 times with a plain copy kernel and exit")
     ap.add_argument("--out", default="")
@@ -77,23 +89,81 @@ def traffic_per_launch():
         return None
 
 
+def progress(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(scene, cam, cfg, rows):
     """The oracle (C++ restatement, OpenMP over top nodes and pixels) on a
-    band of rows of the same frame at full resolution."""
+    band of rows of the same frame at full resolution, on every core the
+    process may use and on one core (a narrower band)."""
     from oracle import pyoracle
     threads = len(os.sched_getaffinity(0))
     omp = os.environ.get("OMP_NUM_THREADS")
     if omp and omp.isdigit():
         threads = min(threads, int(omp))
-    y0 = (cfg.height - rows) // 2
-    t0 = time.perf_counter()
-    fr = pyoracle.render(scene, cam, cfg, threads=threads, keep_hits=False, keep_gbuffer=False,
-                         row_range=(y0, y0 + rows))
-    dt = time.perf_counter() - t0
-    rays = sum(p["rays"] for p in fr.passes)
+
+    def band(r, th):
+        y0 = (cfg.height - r) // 2
+        t0 = time.perf_counter()
+        fr = pyoracle.render(scene, cam, cfg, threads=th, keep_hits=False, keep_gbuffer=False, row_range=(y0, y0 + r))
+        dt = time.perf_counter() - t0
+        return y0, sum(p["rays"] for p in fr.passes), dt
+
+    y0, rays, dt = band(rows, threads)
+    r1 = max(4, rows // max(1, threads))
+    y1, rays1, dt1 = band(r1, 1)
     return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "cpu": cpu_model(),
             "sample": f"rows {y0}-{y0 + rows} of the 1920x1080 atrium frame (all passes), "
-                      f"{rays} rays in {dt:.1f} s"}
+                      f"{rays} rays in {dt:.1f} s",
+            "one_core": {"value": rays1 / dt1 / 1e6, "cores": 1,
+                         "sample": f"rows {y1}-{y1 + r1}, {rays1} rays in {dt1:.1f} s"}}
+
+
+def batching_leg(rsh_amd, eng, scene2):
+    """Plain per-view frames against the batched form on the 64-view batch."""
+    import dataclasses
+    from rsh_amd import autotune, config as cf, scenes
+    views = scenes.batch_views()
+    out = {"views": len(views), "pixels": int(sum(w * h for _, w, h in views)),
+           "key": "2 type + 3x4 cell + 2x6 direction bits interleaved, view id in bits 26-31"}
+    for name, scene, base in (("office", scenes.config1_scene(), cf.config1()),
+                              ("atrium", scene2, cf.config2())):
+        eng.load_scene(scene)
+        cfg = dataclasses.replace(base, cell_bits=4, dir_bits=6, segment_shift=26).with_scene_bounds(scene)
+        for cam, w, h in views[:2]:
+            eng.render(cam, dataclasses.replace(cfg, width=w, height=h))
+        t0 = time.perf_counter()
+        rays, imgs = 0, []
+        for cam, w, h in views:
+            rays += eng.render(cam, dataclasses.replace(cfg, width=w, height=h)).rays_total
+            imgs.append(eng.pixels().copy())
+        plain = time.perf_counter() - t0
+        eng.render_views(views, cfg)
+        t0 = time.perf_counter()
+        st = eng.render_views(views, cfg)
+        eager = time.perf_counter() - t0
+        parts = eng.split_views(eng.pixels(), views)
+        same = sum(int((a == b).sum()) for a, b in zip(parts, imgs)) / out["pixels"]
+        graph_ms, brays = autotune.time_batch(eng, views, cfg, reps=3)
+        out[name] = {"plain_mrays": rays / plain / 1e6, "plain_ms": plain * 1e3,
+                     "batched_mrays": st.rays_total / eager / 1e6, "batched_ms": eager * 1e3,
+                     "graph_mrays": brays / graph_ms / 1e3, "graph_ms": graph_ms,
+                     "speedup": (brays / graph_ms / 1e3) / (rays / plain / 1e6),
+                     "rays": int(brays), "identical_pixels_vs_plain": same}
+    return out
 
 
 def main():
@@ -162,6 +232,10 @@ def main():
            "unit": "GB/s"}
     hsc["frac"] = hsc["achieved"] / PEAK_HBM_GBS
 
+    progress(f"frames: {value:.1f} Mrays/s")
+    if not a.no_batch:
+        batching = batching_leg(rsh_amd, eng, scene)
+        progress("batching leg done")
     out = {
         "metric": "Mrays/s end-to-end at 262k tris 1080p",
         "value": value,
@@ -178,16 +252,24 @@ def main():
         "config": {"workload": f"atrium {a.width}x{a.height}, {scene.triangle_total} tris, "
                                f"{scene.object_total} objects, 3 bounces + shadows, 30% reflective",
                    "rays_per_frame": int(statistics.median(rays)),
-                   "hash": "octahedral 3x5+2x6+2 bits", "hierarchy": "leaf 8, branch 8, depth 2"},
+                   "hash": "octahedral 3x5+2x6+2 bits, cell and direction bits interleaved",
+                   "hierarchy": "leaf 8, branch 8, depth 2"},
         "roofline": roof,
         "hash_sort_compress": hsc,
         "stage_ms": stages,
         "frame_ms_median": 1e3 * statistics.median(times),
     }
     eng.close()
+    if not a.no_batch:
+        out["batching"] = batching
+    if not a.no_serve:
+        from rsh_amd import serving
+        out["serving"] = serving.run(duration_s=a.serve_seconds)
+        progress("serving leg done")
     if not a.no_cpu:
         rows = a.cpu_rows or max(8, a.height // 4)     # about 10-15 s of oracle work at 1080p
         out["cpu_baseline"] = cpu_baseline(scene, cam, cfg, rows)
+        progress("cpu baseline done")
     line = json.dumps(out)
     print(line)
     if a.out:

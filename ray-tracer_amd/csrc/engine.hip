@@ -37,7 +37,8 @@ struct rsh_engine {
     DevBuf tri_sphere, tri_mt, obj_sphere, obj_range;
     // cluster level: CLUSTER consecutive triangles of one object per cluster
     uint32_t NC = 0;
-    DevBuf cl_sphere, cl_base, all_list;   // float4[NC], uint32[K] first cluster, uint32[K] identity
+    DevBuf cl_sphere, cl_base, all_list;   // float4[NC], uint32[K] first cluster, uint4[K] all objects as
+                                           // list entries (first, last, first cluster, 0) for RAH
     std::vector<float> host_spheres;   // float3[2K]
     uint32_t ranges_tb = 0xFFFFFFFFu;  // triangle batch the exclusion ranges were built for
 
@@ -237,15 +238,15 @@ rsh_status build_items(rsh_engine* e, const rsh_frame_config& cfg, TravParams& p
                  std::to_string(MAX_TRAVERSE_ITEMS) + " (raise traversal_chunk)";
         return RSH_CAPACITY;
     }
-    HIP_OK(e->w_list.reserve((size_t)tot[0] * 4 + 4));
+    HIP_OK(e->w_list.reserve((size_t)tot[0] * 16 + 16));
     HIP_OK(e->w_cum.reserve((size_t)tot[0] * 4 + 4));
-    HIP_OK(e->w_items.reserve((size_t)tot[1] * 4 + 4));
-    p.item_top = u32p(e->w_items); p.item_first = u32p(e->w_first); p.cand = cand;
-    p.obj_off = u32p(e->w_off); p.obj_list = u32p(e->w_list); p.obj_cum = u32p(e->w_cum);
-    if (p.algorithm == 1) { p.obj_list = u32p(e->all_list); p.obj_cum = u32p(e->cl_base); }
+    HIP_OK(e->w_items.reserve((size_t)tot[1] * 16 + 16));
+    p.item_desc = e->w_items.as<uint4>(); p.item_first = u32p(e->w_first); p.cand = cand;
+    p.obj_off = u32p(e->w_off); p.obj_ent = e->w_list.as<uint4>(); p.obj_cum = u32p(e->w_cum);
+    if (p.algorithm == 1) { p.obj_ent = e->all_list.as<uint4>(); p.obj_cum = u32p(e->cl_base); }
     p.chunk = chunk; p.items = tot[1];
-    launch_cull_list(p, u32p(e->w_list), u32p(e->w_cum), s);     // writes at p.obj_off
-    launch_item_fill(u32p(e->w_first), nitems, ntop, u32p(e->w_items), s);
+    launch_cull_list(p, e->w_list.as<uint4>(), u32p(e->w_cum), s);     // writes at p.obj_off
+    launch_item_desc(p, nitems, e->w_items.as<uint4>(), s);
     HIP_OK(hipGetLastError());
     out.items = tot[1]; out.chunk = chunk;
     if (e->recording) {
@@ -504,12 +505,13 @@ rsh_status rsh_engine_load_scene(rsh_engine* e, const rsh_scene_desc* s) {
     HIP_OK(hipMemcpyAsync(e->obj_sphere.p, s4.data(), s4.size() * 4, hipMemcpyHostToDevice, st));
     launch_triangle_prep(f4p(e->tri_pos), T, f4p(e->tri_sphere), f4p(e->tri_mt), st);
     // clusters of CLUSTER consecutive triangles per object, in object order
-    std::vector<uint32_t> base(K), ident(K);
+    std::vector<uint32_t> base(K), ident(4 * (size_t)K);
     std::vector<uint32_t> crange;
     uint32_t nc = 0;
     for (uint32_t k = 0; k < K; k++) {
         const uint32_t first = (uint32_t)s->object_spheres[6 * k + 4], last = (uint32_t)s->object_spheres[6 * k + 5];
-        base[k] = nc; ident[k] = k;
+        base[k] = nc;
+        ident[4 * k] = first; ident[4 * k + 1] = last; ident[4 * k + 2] = nc; ident[4 * k + 3] = 0;
         for (uint32_t t = first; t <= last; t += CLUSTER) {
             crange.push_back(t);
             crange.push_back(t + CLUSTER - 1 < last ? t + CLUSTER - 1 : last);
@@ -518,12 +520,12 @@ rsh_status rsh_engine_load_scene(rsh_engine* e, const rsh_scene_desc* s) {
     }
     e->NC = nc;
     HIP_OK(e->cl_sphere.reserve((size_t)nc * 16)); HIP_OK(e->cl_base.reserve((size_t)K * 4));
-    HIP_OK(e->all_list.reserve((size_t)K * 4));
+    HIP_OK(e->all_list.reserve((size_t)K * 16));
     DevBuf crange_d;
     HIP_OK(crange_d.reserve(crange.size() * 4));
     HIP_OK(hipMemcpyAsync(crange_d.p, crange.data(), crange.size() * 4, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(e->cl_base.p, base.data(), (size_t)K * 4, hipMemcpyHostToDevice, st));
-    HIP_OK(hipMemcpyAsync(e->all_list.p, ident.data(), (size_t)K * 4, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(e->all_list.p, ident.data(), (size_t)K * 16, hipMemcpyHostToDevice, st));
     launch_cluster_build(f4p(e->tri_sphere), crange_d.as<uint2>(), nc, f4p(e->cl_sphere), st);
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(st));

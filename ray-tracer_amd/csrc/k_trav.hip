@@ -291,7 +291,7 @@ __device__ __forceinline__ uint32_t wave_incl_add(uint32_t x) {
 }
 
 template <bool LIST>
-__global__ void __launch_bounds__(256) k_cull(TravParams P, uint32_t* npass, uint32_t* cand, uint32_t* obj_list,
+__global__ void __launch_bounds__(256) k_cull(TravParams P, uint32_t* npass, uint32_t* cand, uint4* obj_ent,
                                               uint32_t* obj_cum) {
     const uint32_t lane = lane_id();
     const uint32_t top = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -332,7 +332,8 @@ __global__ void __launch_bounds__(256) k_cull(TravParams P, uint32_t* npass, uin
         const uint32_t incl = wave_incl_add(ncl);
         if (LIST && po) {
             const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-            obj_list[base + n + rank] = oi;
+            const uint4 r = P.obj_range[oi];
+            obj_ent[base + n + rank] = make_uint4(r.x, r.y, P.obj_cl_base[oi], r.z);
             obj_cum[base + n + rank] = cum + incl - ncl;
         }
         n += (uint32_t)__popcll(m);
@@ -354,12 +355,29 @@ __global__ void __launch_bounds__(256) k_item_count(const uint32_t* cand, uint32
     nitems[i] = i == ntop ? 0u : (uint32_t)(((uint64_t)cand[i] + chunk - 1) / chunk);
 }
 
-__global__ void __launch_bounds__(256) k_item_fill(const uint32_t* item_first, const uint32_t* nitems, uint32_t ntop,
-                                                   uint32_t* item_top) {
+// Work-item descriptors, one wave per top node, one lane per item:
+//   x top node, y absolute list entry holding the item's first cluster,
+//   z first cluster within that entry, w clusters of the item | (1 << 31)
+//   when the item is the node's only one (no merge needed).
+// The entry is found by binary search of the node's candidate offsets (RAH:
+// the shared list of all objects with the cluster bases as offsets).
+__global__ void __launch_bounds__(256) k_item_desc(TravParams P, const uint32_t* nitems, uint4* desc) {
     const uint32_t top = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (top >= ntop) return;
-    const uint32_t f = item_first[top], k = nitems[top];
-    for (uint32_t i = lane_id(); i < k; i += 64) item_top[f + i] = top;
+    if (top >= P.level_count[1]) return;
+    const bool rah = P.algorithm == 1;
+    const uint32_t f = P.item_first[top], k = nitems[top], ncand = P.cand[top];
+    const uint32_t lbase = rah ? 0u : P.obj_off[top];
+    const uint32_t np = rah ? P.K : P.obj_off[top + 1] - lbase;
+    for (uint32_t c = lane_id(); c < k; c += 64) {
+        const uint32_t s0 = c * P.chunk;
+        const uint32_t rem = ncand - s0 < P.chunk ? ncand - s0 : P.chunk;
+        uint32_t a = 0, b = np;       // last entry with offset <= s0: cum[a] <= s0 < cum[b]
+        while (b - a > 1) {
+            const uint32_t m = (a + b) >> 1;
+            if (P.obj_cum[lbase + m] <= s0) a = m; else b = m;
+        }
+        desc[f + c] = make_uint4(top, lbase + a, s0 - P.obj_cum[lbase + a], rem | (k == 1 ? 0x80000000u : 0u));
+    }
 }
 
 // ------------------------------------- fused traversal, 64-ray top nodes ----
@@ -407,7 +425,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     const uint32_t w = threadIdx.x >> 6;
     const uint32_t item = blockIdx.x * 4 + w;
     if (item >= P.items) return;
-    const uint32_t top = P.item_top[item];
+    const uint4 desc = P.item_desc[item];
+    const uint32_t top = desc.x;
     if (top >= P.level_count[1]) return;
     const uint32_t pos = top * 64 + lane;
     const bool valid = pos < P.N;
@@ -527,55 +546,36 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
         qlen = 0;
     };
 
-    // ---- cursor over this item's chunk [s0, s1) of the node's cluster stream
-    // (RAH: the engine passes the list of all objects with obj_cum = obj_cl_base)
+    // ---- cursor over this item's clusters of the node's cluster stream, from the
+    // descriptor's list entry (RAH: the list of all objects, offsets obj_cl_base)
     const bool rah = P.algorithm == 1;
-    const uint32_t chunk_ix = item - P.item_first[top];
-    const uint32_t ncand = P.cand[top];
-    const uint32_t s0 = chunk_ix * P.chunk;
-    const uint32_t s1 = ncand - s0 < P.chunk ? ncand : s0 + P.chunk;
-    const size_t lbase = rah ? 0 : P.obj_off[top];
-    const uint32_t np = rah ? P.K : P.obj_off[top + 1] - P.obj_off[top];
+    const bool single = (desc.w >> 31) != 0u;
     const float t_lmax = node_lmax(tS, tC, tA.x, tA.y);
     uint64_t n_cl = 0;
-    uint32_t rem = s1 > s0 ? s1 - s0 : 0;     // clusters of the chunk not yet taken from an entry
-    uint32_t j = 0;                            // current entry of the object list
+    uint32_t rem = desc.w & 0x7FFFFFFFu;      // clusters of the item not yet taken from an entry
+    uint32_t j = desc.y;                       // current entry (absolute index) of the object list
     uint32_t e_first = 0, e_last = 0, e_exlo = 0, e_exhi = 0, e_cbase = 0, e_ncl = 0;
     uint32_t lc_cur = 0, lc_end = 0;           // local clusters of the entry still to pre-test
     uint32_t g0 = 0;                           // first local cluster of the tested group
     uint64_t gmask = 0;                        // clusters of the group that passed the pre-test
     // Take the `rem`-bounded part of list entry jj, starting `off` clusters in.
     auto load_entry = [&](uint32_t jj, uint32_t off) {
-        const uint32_t ob = P.obj_list[lbase + jj];
-        const uint4 r = P.obj_range[ob];
-        e_first = r.x; e_last = r.y;
-        e_exlo = rah ? 0u : r.z; e_exhi = rah ? 0u : r.w;
-        e_cbase = P.obj_cl_base[ob];
-        e_ncl = (r.y - r.x + CLUSTER) / CLUSTER;
+        const uint4 en = P.obj_ent[jj];          // (first, last, first cluster, excluded from)
+        e_first = en.x; e_last = en.y; e_cbase = en.z;
+        e_exlo = en.w; e_exhi = en.w ? en.y : 0u;  // an excluded batch ends at the object's last triangle
+        e_ncl = (en.y - en.x + CLUSTER) / CLUSTER;
         const uint32_t avail = e_ncl - off;
         const uint32_t take = avail < rem ? avail : rem;
         lc_cur = off; lc_end = off + take;
         rem -= take;
     };
-    if (rem) {
-        // last list entry whose stream offset is <= s0 (the offsets increase)
-        uint32_t j0 = 0;
-        for (uint32_t jb = 0; jb < np; jb += 64) {
-            const uint32_t jj = jb + lane;
-            const bool le = jj < np && P.obj_cum[lbase + jj] <= s0;
-            const uint64_t bm = __ballot(le);
-            if (bm) j0 = jb + 63u - (uint32_t)__builtin_clzll(bm);
-            if (bm != ~0ull) break;
-        }
-        j = __builtin_amdgcn_readfirstlane(j0);
-        load_entry(j, s0 - P.obj_cum[lbase + j]);
-    }
+    if (rem) load_entry(j, desc.z);
     // Next block of one cluster's triangles [tb, tl] (cluster cid) that passed
     // the cluster pre-test; clusters are pre-tested 64 at a time, one per lane.
     auto next_block = [&](uint32_t& tb, uint32_t& tl, uint32_t& cid) -> bool {
         while (gmask == 0ull) {
             if (lc_cur >= lc_end) {
-                if (rem == 0 || j + 1 >= np) return false;
+                if (rem == 0) return false;     // the item's cluster count is exact
                 j++;
                 load_entry(j, 0);
                 continue;
@@ -656,7 +656,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     const unsigned long long best = s_best[w][lane];
     if (valid) {
         if (P.kind == 1) { if (best != 0ull) P.shadow[slot] = MISS32; }
-        else if (ncand <= P.chunk) P.closest[slot] = best;
+        else if (single) P.closest[slot] = best;
         else if (best != (P.hit_mode == 0 ? MISS64 : (unsigned long long)MISS32)) atomicMin(&P.closest[slot], best);
     }
     if (count) {
@@ -821,26 +821,26 @@ void launch_cull_count(const TravParams& p, uint32_t* npass, uint32_t* cand, hip
     k_cull<false><<<top / 4 + 1, 256, 0, s>>>(p, npass, cand, nullptr, nullptr);
 }
 
-void launch_cull_list(const TravParams& p, uint32_t* obj_list, uint32_t* obj_cum, hipStream_t s) {
+void launch_cull_list(const TravParams& p, uint4* obj_ent, uint32_t* obj_cum, hipStream_t s) {
     const uint32_t top = p.level_count[1];
-    if (top && p.algorithm != 1) k_cull<true><<<(top + 3) / 4, 256, 0, s>>>(p, nullptr, nullptr, obj_list, obj_cum);
+    if (top && p.algorithm != 1) k_cull<true><<<(top + 3) / 4, 256, 0, s>>>(p, nullptr, nullptr, obj_ent, obj_cum);
 }
 
 void launch_item_count(const uint32_t* cand, uint32_t chunk, uint32_t ntop, uint32_t* nitems, hipStream_t s) {
     k_item_count<<<ntop / 256 + 1, 256, 0, s>>>(cand, chunk, ntop, nitems);
 }
 
-void launch_item_fill(const uint32_t* item_first, const uint32_t* nitems, uint32_t ntop, uint32_t* item_top,
-                      hipStream_t s) {
-    if (ntop) k_item_fill<<<(ntop + 3) / 4, 256, 0, s>>>(item_first, nitems, ntop, item_top);
+void launch_item_desc(const TravParams& p, const uint32_t* nitems, uint4* desc, hipStream_t s) {
+    const uint32_t ntop = p.level_count[1];
+    if (ntop) k_item_desc<<<(ntop + 3) / 4, 256, 0, s>>>(p, nitems, desc);
 }
 
 hipError_t launch_traverse(const TravParams& p, hipStream_t s) {
     const uint32_t top = p.level_count[p.depth - 1];
     if (uses_items(p)) {
         if (p.items == 0) return hipSuccess;
-        if (p.chunk == 0 || !p.item_top || !p.item_first || !p.cand) return hipErrorInvalidValue;
-        if (p.algorithm != 1 && (!p.obj_off || !p.obj_list || !p.obj_cum)) return hipErrorInvalidValue;
+        if (p.chunk == 0 || !p.item_desc) return hipErrorInvalidValue;
+        if (!p.obj_ent || !p.obj_cum || (p.algorithm != 1 && !p.obj_off)) return hipErrorInvalidValue;
         if (p.items > MAX_TRAVERSE_ITEMS) return hipErrorInvalidValue;
         const uint32_t blocks = (p.items + 3) / 4;
         switch (p.leaf) {

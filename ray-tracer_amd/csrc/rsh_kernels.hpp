@@ -109,11 +109,12 @@ struct TravParams {
     uint32_t force_generic;       // 1: use the generic kernel even for 64-ray top nodes
     uint32_t splits;              // generic kernel: waves sharing one top node's triangle range
     // work items of the 64-ray kernel: (top node, chunk of candidate clusters)
-    const uint32_t* item_top;     // [items] top node of each item
+    const uint4* item_desc;       // [items] (top, list entry, first cluster in it, clusters | single << 31)
     const uint32_t* item_first;   // [n_top + 1] first item of each top node
     const uint32_t* cand;         // [n_top + 1] candidate clusters of each top node
     const uint32_t* obj_off;      // [n_top + 1] first list entry of each top node (CRSH)
-    const uint32_t* obj_list;     // passing objects of each node, in triangle order (RAH: all, shared)
+    const uint4* obj_ent;         // passing objects of each node, in triangle order, as (first, last,
+                                  // first cluster, excluded-batch start or 0) (RAH: all objects, shared)
     const uint32_t* obj_cum;      // candidate clusters of the node before each listed object
     uint32_t chunk;               // candidate clusters per item
     uint32_t items;
@@ -133,18 +134,17 @@ constexpr uint32_t MAX_TRAVERSE_ITEMS = 1u << 24;
 //   launch_cull_count  -> npass, cand            [n_top + 1], entry n_top = 0
 //   exclusive scans    -> obj_off (of npass), item_first (of nitems)
 //   launch_item_count  -> nitems = ceil(cand / chunk)
-//   launch_cull_list   -> obj_list, obj_cum at obj_off
-//   launch_item_fill   -> item_top
+//   launch_cull_list   -> obj_ent, obj_cum at obj_off
+//   launch_item_desc   -> item_desc
 inline bool uses_items(const TravParams& p) {
     return p.depth == 2 && p.leaf * p.branch == 64 && !p.force_generic &&
            (p.leaf == 2 || p.leaf == 4 || p.leaf == 8 || p.leaf == 16 || p.leaf == 32);
 }
 
 void launch_cull_count(const TravParams& p, uint32_t* npass, uint32_t* cand, hipStream_t s);
-void launch_cull_list(const TravParams& p, uint32_t* obj_list, uint32_t* obj_cum, hipStream_t s);
+void launch_cull_list(const TravParams& p, uint4* obj_ent, uint32_t* obj_cum, hipStream_t s);
 void launch_item_count(const uint32_t* cand, uint32_t chunk, uint32_t ntop, uint32_t* nitems, hipStream_t s);
-void launch_item_fill(const uint32_t* item_first, const uint32_t* nitems, uint32_t ntop, uint32_t* item_top,
-                      hipStream_t s);
+void launch_item_desc(const TravParams& p, const uint32_t* nitems, uint4* desc, hipStream_t s);
 
 // --- rays ---
 void launch_create_primary(const Views& V, uint32_t P, KeyCfg kc, float* rays, uint32_t* flags, uint32_t* keys,

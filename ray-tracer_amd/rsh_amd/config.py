@@ -3,7 +3,10 @@
 Fields mirror ``rsh_frame_config`` in include/rsh.h.  Defaults reproduce the
 reference's compile-time constants (projects/CUDA-Engine/source/Constants.h:9-29):
 depth 2, subdivision 8 (split here into ``leaf`` and ``branch``), triangle
-batch 5000.
+batch 5000.  The octahedral key holds 2 type + 3 x 5 cell + 2 x 6 direction
+bits with the cell and direction bits interleaved (layout 2), which groups
+incoherent bounce rays into tighter hierarchy nodes than the concatenated
+layouts.
 """
 from __future__ import annotations
 
@@ -37,7 +40,7 @@ class FrameConfig:
     primary_rays: int = 1
     cell_bits: int = 5
     dir_bits: int = 6
-    key_layout: int = 0
+    key_layout: int = 2             # 0 cell|dir, 1 dir|cell, 2 bits interleaved (default: best grouping)
     scene_min: tuple = (0.0, 0.0, 0.0)
     cell_scale: tuple = (1.0, 1.0, 1.0)
     segment_shift: int = 0

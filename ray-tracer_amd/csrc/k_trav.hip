@@ -381,49 +381,52 @@ __global__ void __launch_bounds__(256) k_item_desc(TravParams P, const uint32_t*
 }
 
 // ------------------------------------- fused traversal, 64-ray top nodes ----
-// Specialisation for depth 2 with leaf * branch == 64: one wave = one work
-// item = one chunk of one top node's candidate stream (k_cull); the top node
-// covers BRANCH leaves of LEAF rays (lane i holds sorted ray i of the node).
-//  A  The wave walks its chunk of clusters (CLUSTER consecutive triangles of
-//     a listed object): 64 clusters at a time are pre-tested, one per lane,
-//     with the conservative cluster_may_pass; the triangles of each passing
-//     cluster are then tested against the top node 64 at a time, NB blocks'
-//     bounding spheres loaded back to back.  Survivors (index + sphere) are
-//     appended to an LDS queue.  The pre-test only skips triangles whose own
-//     test would fail, so the survivors are exactly the reference's.
-//  B  When 64 survivors are queued their Moller-Trumbore vectors are fetched
-//     with one parallel load, then (survivor, leaf) pairs are tested 64 at a
-//     time: lane i takes leaf i % BRANCH (held in registers), survivor
-//     i / BRANCH.  Each lane collects a 64-bit mask of the survivors that
-//     passed its own ray's leaf.
-//  C  All leaf groups walk their masks in parallel; a lane runs the ray test
-//     of its ray against each pending survivor.
+// Specialisation for depth 2 with leaf * branch == 64: one workgroup = one
+// wave = one work item = one chunk of one top node's candidate cluster stream
+// (k_cull, k_item_desc); the top node covers BRANCH leaves of LEAF rays (lane
+// i holds sorted ray i of the node).
+//  A  The wave walks its chunk: 64 clusters (CLUSTER consecutive triangles of
+//     a listed object) at a time are pre-tested, one per lane, with the
+//     conservative cluster_may_pass; the triangles of each passing cluster
+//     are tested against the top node 64 at a time, NB blocks' bounding
+//     spheres loaded back to back.  Survivors (index, sphere and the mask of
+//     leaves the cluster may pass) are appended to an LDS queue.  The
+//     pre-tests only skip pairs whose own test would fail, so the survivors
+//     and the passing (leaf, triangle) pairs are exactly the reference's.
+//  B  Per round of up to SCH queued survivors, the (survivor, leaf)
+//     candidates allowed by the masks are listed and tested 64 per step;
+//     passing pairs are compacted in place.
+//  C  The Moller-Trumbore vectors of the survivors that appear in a pair are
+//     loaded, then the pair list is walked BRANCH pairs per step, lane =
+//     pair p0 + lane / LEAF, ray lane % LEAF of the pair's leaf; closest hits
+//     merge per ray in LDS (atomicMin), shadow rays set a flag.
 // The set of (leaf, triangle) pairs and ray tests is the reference's; only
 // the order differs, and any-hit / closest-hit are order independent.
-// Four waves per SIMD (the LDS footprint allows four 256-thread blocks per CU):
-// the register budget is capped at 128 VGPRs, which costs a 4-VGPR spill and
-// measures faster than three waves without it.
+// One wave per workgroup: items differ widely in length, and a workgroup's LDS
+// is held until its last wave ends, so a finished item must not wait for
+// others (measured: 2x faster than four-wave workgroups on the 1080p atrium).
+// The register budget is capped at 128 VGPRs for four waves per SIMD; the
+// 10 KB of LDS per wave allows sixteen waves per CU.
 template <int LEAF>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_traverse64(TravParams P) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_traverse64(TravParams P) {
     constexpr uint32_t BRANCH = 64 / LEAF;
     constexpr uint32_t QCAP = 128;                // survivor queue
     constexpr uint32_t PCAP = 512;                // (survivor, leaf) candidate / pair list
     constexpr uint32_t SCH = PCAP / BRANCH < 64 ? PCAP / BRANCH : 64;   // survivors per B+C round
-    __shared__ float4 s_qsph[4][QCAP];
-    __shared__ uint32_t s_qidx[4][QCAP];
-    __shared__ uint32_t s_qmask[4][QCAP];         // leaves the survivor's cluster may pass
-    __shared__ float4 s_mt[4][64][3];
-    __shared__ float4 s_ray0[4][64];              // (o + d*eps, tmax)
-    __shared__ float4 s_ray1[4][64];              // (d, valid)
-    __shared__ unsigned long long s_best[4][64];  // closest record / shadow flag per ray
-    __shared__ uint16_t s_pair[4][PCAP];          // survivor << 5 | leaf
-    __shared__ float4 s_lsph[4][BRANCH];          // leaf nodes of the top node
-    __shared__ float4 s_lcone[4][BRANCH];
-    __shared__ float4 s_laux[4][BRANCH];          // (cos, tan, lmax, exists)
-    __shared__ uint8_t s_used[4][64];             // survivor of the round appears in a pair
+    __shared__ float4 s_qsph[QCAP];
+    __shared__ uint32_t s_qidx[QCAP];
+    __shared__ uint32_t s_qmask[QCAP];         // leaves the survivor's cluster may pass
+    __shared__ float4 s_mt[64][3];
+    __shared__ float4 s_ray0[64];              // (o + d*eps, tmax)
+    __shared__ float4 s_ray1[64];              // (d, valid)
+    __shared__ unsigned long long s_best[64];  // closest record / shadow flag per ray
+    __shared__ uint16_t s_pair[PCAP];          // survivor << 5 | leaf
+    __shared__ float4 s_lsph[BRANCH];          // leaf nodes of the top node
+    __shared__ float4 s_lcone[BRANCH];
+    __shared__ float4 s_laux[BRANCH];          // (cos, tan, lmax, r / cos)
+    __shared__ uint8_t s_used[64];             // survivor of the round appears in a pair
     const uint32_t lane = lane_id();
-    const uint32_t w = threadIdx.x >> 6;
-    const uint32_t item = blockIdx.x * 4 + w;
+    const uint32_t item = blockIdx.x;
     if (item >= P.items) return;
     const uint4 desc = P.item_desc[item];
     const uint32_t top = desc.x;
@@ -441,11 +444,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             slot = __float_as_uint(a.w);
             tmax = b.w;
         }
-        s_ray0[w][lane] = f4(o + d * EPS, tmax);
-        s_ray1[w][lane] = f4(d, valid ? 1.0f : 0.0f);
+        s_ray0[lane] = f4(o + d * EPS, tmax);
+        s_ray1[lane] = f4(d, valid ? 1.0f : 0.0f);
     }
     const unsigned long long miss = P.kind == 1 ? 0ull : (P.hit_mode == 0 ? MISS64 : (unsigned long long)MISS32);
-    s_best[w][lane] = miss;
+    s_best[lane] = miss;
     if (lane < BRANCH) {
         const uint32_t bl = top * BRANCH + lane;
         float4 ls = make_float4(0, 0, 0, 0), lc = make_float4(0, 0, 1, 0), la = make_float4(0, 0, 0, 0);
@@ -453,9 +456,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             ls = P.nodes[2 * (size_t)bl];
             lc = P.nodes[2 * (size_t)bl + 1];
             const float2 a = P.aux[bl];
-            la = make_float4(a.x, a.y, node_lmax(ls, lc, a.x, a.y), 1.0f);
+            const float2 nb = node_bound(ls, lc, a.x, a.y);
+            la = make_float4(a.x, a.y, nb.x, nb.y);
         }
-        s_lsph[w][lane] = ls; s_lcone[w][lane] = lc; s_laux[w][lane] = la;
+        s_lsph[lane] = ls; s_lcone[lane] = lc; s_laux[lane] = la;
     }
     const uint32_t gt = P.level_offset[1] + top;
     const float4 tS = P.nodes[2 * (size_t)gt], tC = P.nodes[2 * (size_t)gt + 1];
@@ -477,7 +481,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     auto flush = [&]() {
         for (uint32_t c0 = 0; c0 < qlen && !done; c0 += SCH) {
             const uint32_t n = qlen - c0 < SCH ? qlen - c0 : SCH;
-            uint32_t lm = lane < n ? s_qmask[w][c0 + lane] : 0u;
+            uint32_t lm = lane < n ? s_qmask[c0 + lane] : 0u;
             const uint32_t kc = (uint32_t)__popc(lm);
             const uint32_t incl = wave_incl_add(kc);
             const uint32_t ncand_ = __shfl(incl, 63, 64);
@@ -485,9 +489,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             while (lm) {
                 const uint32_t b = (uint32_t)__builtin_ctz(lm);
                 lm &= lm - 1u;
-                s_pair[w][o++] = (uint16_t)((lane << 5) | b);
+                s_pair[o++] = (uint16_t)((lane << 5) | b);
             }
-            s_used[w][lane] = 0;
+            s_used[lane] = 0;
             __builtin_amdgcn_wave_barrier();
             uint32_t np_ = 0;
             for (uint32_t c = 0; c < ncand_; c += 64) {
@@ -495,53 +499,53 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                 bool lp = false;
                 uint32_t e = 0;
                 if (ci < ncand_) {
-                    e = s_pair[w][ci];
+                    e = s_pair[ci];
                     const uint32_t b = e & 31u;
-                    const float4 la = s_laux[w][b];
-                    lp = sphere_node(s_lsph[w][b], s_lcone[w][b], s_qsph[w][c0 + (e >> 5)], la.x, la.y);
+                    const float4 la = s_laux[b];
+                    lp = sphere_node(s_lsph[b], s_lcone[b], s_qsph[c0 + (e >> 5)], la.x, la.y);
                 }
                 const uint64_t pm = __ballot(lp);
                 if (lp) {     // in place: entries c.. were read above, writes land below c + 64
-                    s_pair[w][np_ + (uint32_t)__popcll(pm & ((1ull << lane) - 1ull))] = (uint16_t)e;
-                    s_used[w][e >> 5] = 1;
+                    s_pair[np_ + (uint32_t)__popcll(pm & ((1ull << lane) - 1ull))] = (uint16_t)e;
+                    s_used[e >> 5] = 1;
                 }
                 np_ += (uint32_t)__popcll(pm);
                 if (count) { n_leaf += (uint64_t)__popcll(pm); n_bsteps++; }
             }
             __builtin_amdgcn_wave_barrier();
             if (np_ == 0) continue;
-            if (lane < n && s_used[w][lane]) {
-                size_t t = s_qidx[w][c0 + lane];
+            if (lane < n && s_used[lane]) {
+                size_t t = s_qidx[c0 + lane];
                 if (t >= P.T) t = 0;     // queue entries are < T by construction
-                s_mt[w][lane][0] = P.tri_mt[3 * t];
-                s_mt[w][lane][1] = P.tri_mt[3 * t + 1];
-                s_mt[w][lane][2] = P.tri_mt[3 * t + 2];
+                s_mt[lane][0] = P.tri_mt[3 * t];
+                s_mt[lane][1] = P.tri_mt[3 * t + 1];
+                s_mt[lane][2] = P.tri_mt[3 * t + 2];
             }
             __builtin_amdgcn_wave_barrier();
             for (uint32_t p0 = 0; p0 < np_; p0 += BRANCH) {
                 const uint32_t pi = p0 + lane / LEAF;
                 if (count) n_csteps++;
                 if (pi < np_) {
-                    const uint32_t e = s_pair[w][pi];
+                    const uint32_t e = s_pair[pi];
                     const uint32_t sv = e >> 5;
                     const uint32_t ray = (e & 31u) * LEAF + lane % LEAF;
-                    const float4 r0 = s_ray0[w][ray], r1 = s_ray1[w][ray];
-                    const bool live = r1.w != 0.0f && (P.kind != 1 || s_best[w][ray] == 0ull);
+                    const float4 r0 = s_ray0[ray], r1 = s_ray1[ray];
+                    const bool live = r1.w != 0.0f && (P.kind != 1 || s_best[ray] == 0ull);
                     if (live) {
-                        const float4 A = s_mt[w][sv][0], B = s_mt[w][sv][1], Cc = s_mt[w][sv][2];
+                        const float4 A = s_mt[sv][0], B = s_mt[sv][1], Cc = s_mt[sv][2];
                         const float th = ray_triangle(xyz(r0), xyz(r1), xyz(A), xyz(B), xyz(Cc));
                         if (count) n_tests++;
                         if (P.kind == 1) {
-                            if (th > EPS && th < r0.w) s_best[w][ray] = 1ull;
+                            if (th > EPS && th < r0.w) s_best[ray] = 1ull;
                         } else if (th > EPS) {
-                            const unsigned long long rec = pack_hit(th, s_qidx[w][c0 + sv], P.hit_mode);
-                            if (rec < s_best[w][ray]) atomicMin(&s_best[w][ray], rec);
+                            const unsigned long long rec = pack_hit(th, s_qidx[c0 + sv], P.hit_mode);
+                            if (rec < s_best[ray]) atomicMin(&s_best[ray], rec);
                         }
                     }
                 }
             }
             __builtin_amdgcn_wave_barrier();
-            if (P.kind == 1 && (__ballot(s_best[w][lane] != 0ull) & alive) == alive) done = true;
+            if (P.kind == 1 && (__ballot(s_best[lane] != 0ull) & alive) == alive) done = true;
         }
         qlen = 0;
     };
@@ -550,7 +554,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     // descriptor's list entry (RAH: the list of all objects, offsets obj_cl_base)
     const bool rah = P.algorithm == 1;
     const bool single = (desc.w >> 31) != 0u;
-    const float t_lmax = node_lmax(tS, tC, tA.x, tA.y);
+    const float2 t_bound = node_bound(tS, tC, tA.x, tA.y);
     uint64_t n_cl = 0;
     uint32_t rem = desc.w & 0x7FFFFFFFu;      // clusters of the item not yet taken from an entry
     uint32_t j = desc.y;                       // current entry (absolute index) of the object list
@@ -585,7 +589,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             lc_cur += n;
             if (e_ncl == 1) { gmask = 1ull; break; }   // the object test already stands for it
             bool cp = false;
-            if (lane < n) cp = cluster_may_pass(tS, tC, tA.x, tA.y, t_lmax, P.cl_sphere[e_cbase + g0 + lane]);
+            if (lane < n) cp = cluster_may_pass(tS, tC, tA.y, t_bound, P.cl_sphere[e_cbase + g0 + lane]);
             gmask = __ballot(cp);
             if (count) n_cl += n;
         }
@@ -629,9 +633,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             // a cluster that no leaf may pass contributes no pair
             bool lpass = false;
             if (lane < BRANCH) {
-                const float4 la = s_laux[w][lane];
-                lpass = la.w != 0.0f &&
-                        cluster_may_pass(s_lsph[w][lane], s_lcone[w][lane], la.x, la.y, la.z, P.cl_sphere[cidk[k]]);
+                const float4 la = s_laux[lane];
+                lpass = top * BRANCH + lane < P.level_count[0] &&
+                        cluster_may_pass(s_lsph[lane], s_lcone[lane], la.y, make_float2(la.z, la.w),
+                                         P.cl_sphere[cidk[k]]);
             }
             const uint32_t cm = (uint32_t)__ballot(lpass);
             if (cm == 0u) continue;
@@ -640,9 +645,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
             if (done) break;
             if (p) {
                 const uint32_t r = qlen + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-                s_qsph[w][r] = tsk[k];
-                s_qidx[w][r] = tbk[k] + lane;
-                s_qmask[w][r] = cm;
+                s_qsph[r] = tsk[k];
+                s_qidx[r] = tbk[k] + lane;
+                s_qmask[r] = cm;
             }
             qlen += add;
             __builtin_amdgcn_wave_barrier();
@@ -653,7 +658,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 
     // Results start as misses; a node cut into several items merges by atomicMin.
     __builtin_amdgcn_wave_barrier();
-    const unsigned long long best = s_best[w][lane];
+    const unsigned long long best = s_best[lane];
     if (valid) {
         if (P.kind == 1) { if (best != 0ull) P.shadow[slot] = MISS32; }
         else if (single) P.closest[slot] = best;
@@ -842,13 +847,14 @@ hipError_t launch_traverse(const TravParams& p, hipStream_t s) {
         if (p.chunk == 0 || !p.item_desc) return hipErrorInvalidValue;
         if (!p.obj_ent || !p.obj_cum || (p.algorithm != 1 && !p.obj_off)) return hipErrorInvalidValue;
         if (p.items > MAX_TRAVERSE_ITEMS) return hipErrorInvalidValue;
-        const uint32_t blocks = (p.items + 3) / 4;
+        // one wave per workgroup: a finished item releases its LDS at once
+        const uint32_t blocks = p.items;
         switch (p.leaf) {
-            case 2: k_traverse64<2><<<blocks, 256, 0, s>>>(p); return hipGetLastError();
-            case 4: k_traverse64<4><<<blocks, 256, 0, s>>>(p); return hipGetLastError();
-            case 8: k_traverse64<8><<<blocks, 256, 0, s>>>(p); return hipGetLastError();
-            case 16: k_traverse64<16><<<blocks, 256, 0, s>>>(p); return hipGetLastError();
-            case 32: k_traverse64<32><<<blocks, 256, 0, s>>>(p); return hipGetLastError();
+            case 2: k_traverse64<2><<<blocks, 64, 0, s>>>(p); return hipGetLastError();
+            case 4: k_traverse64<4><<<blocks, 64, 0, s>>>(p); return hipGetLastError();
+            case 8: k_traverse64<8><<<blocks, 64, 0, s>>>(p); return hipGetLastError();
+            case 16: k_traverse64<16><<<blocks, 64, 0, s>>>(p); return hipGetLastError();
+            case 32: k_traverse64<32><<<blocks, 64, 0, s>>>(p); return hipGetLastError();
             default: return hipErrorInvalidValue;
         }
     }

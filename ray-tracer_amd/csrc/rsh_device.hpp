@@ -261,33 +261,43 @@ __device__ __forceinline__ bool sphere_node(float4 sphere, float4 cone, float4 t
 // k = |cd| <= sqrt(2).  For a triangle sphere (c, rt) inside a cluster sphere
 // (q, rho), |c - q| <= rho - rt, so a passing triangle gives
 // f(q) <= r/cos + rho * max(L, 1/cos).  The cluster is rejected only when
-// f(q) exceeds that bound by E, a margin of 1e-4 of the magnitudes involved
-// (the FP32 rounding of either evaluation is ~1e-6 of them).  Nodes outside
-// the argument's assumptions (the HALF_PI shortcut, cos <= 0, tan < 0,
+// f(q) exceeds that bound by E, a margin of 1e-4 of the magnitudes involved.
+// Both evaluations use the hardware square root and reciprocal (v_sqrt_f32,
+// v_rcp_f32, within 1 ulp): their error, like the FP32 rounding of either
+// evaluation, is ~1e-6 of those magnitudes, far inside E.  Nodes outside the
+// argument's assumptions (the HALF_PI shortcut, cos <= 0, tan < 0,
 // non-finite values, |cd| far from 1) never reject; a NaN anywhere passes.
-// node_lmax returns max(L, 1/cos) (slightly inflated), or -1 for a node the
-// argument does not cover; cluster_may_pass then never rejects.
-__device__ __forceinline__ float node_lmax(float4 sphere, float4 cone, float cosine, float tangent) {
-    const float k = len(xyz(cone));
+// node_bound returns (max(L, 1/cos) slightly inflated, r/cos), with -1 in the
+// first for a node the argument does not cover; cluster_may_pass then never
+// rejects.
+__device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+__device__ __forceinline__ float2 node_bound(float4 sphere, float4 cone, float cosine, float tangent) {
+    const float k = hw_sqrt(dot(xyz(cone), xyz(cone)));
     const float tk = tangent * k;
-    const float L = sqrtf(1.0f + tk * tk + 2.0f * tk * fabsf(1.0f - k * k));
-    const float lmax = fmaxf(L, 1.0f / cosine) * 1.00001f;
+    const float L = hw_sqrt(1.0f + tk * tk + 2.0f * tk * fabsf(1.0f - k * k));
+    const float rc = hw_rcp(cosine);
+    const float lmax = fmaxf(L, rc) * 1.00001f;
     const bool ok = cone.w != HALF_PI_F && cosine > 0.0f && tangent >= 0.0f && k > 0.5f && k < 1.4f &&
                     isfinite(cosine) && isfinite(tangent) && isfinite(lmax) && isfinite(sphere.w) &&
                     isfinite(sphere.x) && isfinite(sphere.y) && isfinite(sphere.z);
-    return ok ? lmax : -1.0f;
+    return make_float2(ok ? lmax : -1.0f, sphere.w * rc);
 }
 
-__device__ __forceinline__ bool cluster_may_pass(float4 sphere, float4 cone, float cosine, float tangent, float lmax,
-                                                 float4 cl) {
+// bound = node_bound(sphere, cone, cos, tan)
+__device__ __forceinline__ bool cluster_may_pass(float4 sphere, float4 cone, float tangent, float2 bound, float4 cl) {
+    const float lmax = bound.x;
     if (!(lmax > 0.0f)) return true;
     const v3 cd = xyz(cone);
     const v3 s2q = xyz(cl) - xyz(sphere);
     const float pr = dot(s2q, cd);
-    const float f = len(s2q - pr * cd) - pr * tangent;
+    const v3 v = s2q - pr * cd;
+    const float f = hw_sqrt(dot(v, v)) - pr * tangent;
     const float rho = cl.w;
-    const float allow = sphere.w / cosine + rho * lmax;
-    const float E = 1e-4f * (len(s2q) + rho + (fabsf(pr) + rho) * tangent + (sphere.w + rho) * lmax) + 1e-4f;
+    const float allow = bound.y + rho * lmax;
+    const float E = 1e-4f * (hw_sqrt(dot(s2q, s2q)) + rho + (fabsf(pr) + rho) * tangent + (sphere.w + rho) * lmax) +
+                    1e-4f;
     return !(f > allow + E);
 }
 

@@ -3,7 +3,10 @@ ray-tracer_amd/csrc/rsh_device.hpp) must never reject a cluster holding a
 triangle whose own FP32 node test (sphere_node, RayTracer.cu:283-298) passes;
 otherwise hits would differ from the reference's.  Both predicates are
 restated here in numpy float32 with the kernel's operation order and checked
-on adversarial samples: cluster centres placed on the cone surface, member
+on adversarial samples.  The kernel evaluates the cluster test with the
+hardware square root and reciprocal (v_sqrt_f32, v_rcp_f32, within 1 ulp);
+the restatement perturbs each of those results by 2 ulp in the direction that
+rejects more clusters.  Samples: cluster centres placed on the cone surface, member
 triangle spheres pushed to the cluster boundary towards the axis, and
 triangle spheres equal to the cluster sphere on the exact test boundary.
 A planted mistake (Lipschitz constant 1 instead of max(L, 1/cos), or no
@@ -25,22 +28,24 @@ def node_test(S, Cn, T, cosv, tanv):
     return (Cn[:, 3] == HALF_PI) | (lhs >= l)
 
 
-def cluster_test(S, Cn, Q, cosv, tanv, lipschitz_one=False, margin=1e-4):
+def cluster_test(S, Cn, Q, cosv, tanv, lipschitz_one=False, margin=1e-4, hw_err=2.0 ** -22):
+    lo, hi = f32(1 - hw_err), f32(1 + hw_err)
     cd = Cn[:, :3].astype(f32)
-    k = np.sqrt((cd * cd).sum(1).astype(f32)).astype(f32)
+    k = (np.sqrt((cd * cd).sum(1).astype(f32)) * lo).astype(f32)
     tk = (tanv * k).astype(f32)
-    L = np.sqrt(f32(1) + tk * tk + f32(2) * tk * np.abs(f32(1) - k * k)).astype(f32)
-    lmax = (np.maximum(L, f32(1) / cosv) * f32(1.00001)).astype(f32)
+    L = (np.sqrt(f32(1) + tk * tk + f32(2) * tk * np.abs(f32(1) - k * k)) * lo).astype(f32)
+    rc = ((f32(1) / cosv) * lo).astype(f32)
+    lmax = (np.maximum(L, rc) * f32(1.00001)).astype(f32)
     if lipschitz_one:
         lmax = np.ones_like(lmax)
     ok = (Cn[:, 3] != HALF_PI) & (cosv > 0) & (tanv >= 0) & (k > 0.5) & (k < 1.4) & np.isfinite(lmax)
     s2q = (Q[:, :3] - S[:, :3]).astype(f32)
     pr = (s2q * cd).sum(1).astype(f32)
     v = (s2q - pr[:, None] * cd).astype(f32)
-    f = (np.sqrt((v * v).sum(1)) - pr * tanv).astype(f32)
+    f = (np.sqrt((v * v).sum(1)) * hi - pr * tanv).astype(f32)
     rho = Q[:, 3]
-    allow = (S[:, 3] / cosv + rho * lmax).astype(f32)
-    E = f32(margin) * (np.sqrt((s2q * s2q).sum(1)) + rho + (np.abs(pr) + rho) * tanv + (S[:, 3] + rho) * lmax) \
+    allow = (S[:, 3] * rc + rho * lmax).astype(f32)
+    E = f32(margin) * (np.sqrt((s2q * s2q).sum(1)) * lo + rho + (np.abs(pr) + rho) * tanv + (S[:, 3] + rho) * lmax) \
         + f32(margin)
     return ~ok | ~(f > allow + E)
 

@@ -132,6 +132,34 @@ def cpu_baseline(scene, cam, cfg, rows):
                          "sample": f"rows {y1}-{y1 + r1}, {rays1} rays in {dt1:.1f} s"}}
 
 
+def sort_passes(cfg):
+    """8-bit digit passes of the radix sort over the key width of cfg."""
+    bits = 32 if cfg.key_mode == 0 or cfg.segment_shift else min(32, cfg.hash_bits())
+    return (bits + 7) // 8
+
+
+def hsc_bytes(S, N, C, digit_passes):
+    """Algorithmic HBM bytes of trimming + compression, sorting and
+    decompression for one pass of S ray slots, N live rays, C chunks, at the
+    reference's buffer layout (uint32 flags / keys / values per slot):
+    trim+compress reads every flag and the key and value of each live slot and
+    writes the live values and the chunk heads (base, key, value); the sort
+    reads and writes every chunk's key and value once per digit pass;
+    decompression reads the sorted values, two chunk bases per chunk and the
+    live values, and writes the rays' values in sorted order."""
+    return (4 * S + 8 * N + 4 * N + 12 * C) + 16 * C * digit_passes + (4 * C + 8 * C + 4 * N + 4 * N)
+
+
+def hsc_figure(passes, stages, digit_passes):
+    """Hash + sort + compress bandwidth: algorithmic bytes over the summed
+    trim (compression fused in), sort and decompress stage times."""
+    nbytes = sum(hsc_bytes(p["slots"], p["rays"], p["chunks"], digit_passes) for p in passes)
+    ms = stages["trim"] + stages["compress"] + stages["sort"] + stages["decompress"]
+    gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return {"achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS,
+            "bytes": int(nbytes), "ms": ms}
+
+
 def batching_leg(rsh_amd, eng, scene2):
     """Plain per-view frames against the batched form on the 64-view batch."""
     import dataclasses
@@ -158,11 +186,17 @@ def batching_leg(rsh_amd, eng, scene2):
         parts = eng.split_views(eng.pixels(), views)
         same = sum(int((a == b).sum()) for a, b in zip(parts, imgs)) / out["pixels"]
         graph_ms, brays = autotune.time_batch(eng, views, cfg, reps=3)
+        st = eng.render_views(views, cfg, flags=eng.TIMING)
+        stages = {n: st.stage_ms[i] for i, n in enumerate(rsh_amd.STAGES)}
+        bp = [{"slots": st.pass_slots[i], "rays": st.pass_rays[i], "chunks": st.pass_chunks[i]}
+              for i in range(st.pass_count)]
+        hsc = hsc_figure(bp, stages, sort_passes(cfg))
         out[name] = {"plain_mrays": rays / plain / 1e6, "plain_ms": plain * 1e3,
                      "batched_mrays": st.rays_total / eager / 1e6, "batched_ms": eager * 1e3,
                      "graph_mrays": brays / graph_ms / 1e3, "graph_ms": graph_ms,
                      "speedup": (brays / graph_ms / 1e3) / (rays / plain / 1e6),
-                     "rays": int(brays), "identical_pixels_vs_plain": same}
+                     "rays": int(brays), "identical_pixels_vs_plain": same,
+                     "hash_sort_compress": hsc}
     return out
 
 
@@ -222,15 +256,8 @@ def main():
             "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic_per_launch(),
             "avg_launch_ms": avg_launch_ms, "launches_per_frame": launches_per_frame,
             "flops_per_launch": flops_launch}
-    # hash + sort + compress bandwidth (algorithmic bytes / stage time)
-    hsc_bytes = 0
-    for p in passes:
-        N, C, S = p["rays"], p["chunks"], p["slots"]
-        hsc_bytes += 12 * S + 8 * N + 4 * N + 16 * C + C * (4 + 16 * 4) + 16 * N + 16 * C
-    hsc_ms = stages["trim"] + stages["compress"] + stages["sort"] + stages["decompress"]
-    hsc = {"achieved": hsc_bytes / (hsc_ms * 1e-3) / 1e9 if hsc_ms > 0 else 0.0, "peak": PEAK_HBM_GBS,
-           "unit": "GB/s"}
-    hsc["frac"] = hsc["achieved"] / PEAK_HBM_GBS
+    hsc = hsc_figure(passes, stages, sort_passes(cfg))
+    hsc["workload"] = "the timed atrium frames (8 passes, %d rays per frame)" % int(statistics.median(rays))
 
     progress(f"frames: {value:.1f} Mrays/s")
     if not a.no_batch:

@@ -233,6 +233,12 @@ void rsh_server_destroy(rsh_server* s);
 void* rsh_host_alloc(size_t bytes);
 void rsh_host_free(void* p);
 
+/* Device self-check (no counterpart in the reference; used by the test
+ * suite): compares the kernels' correctly rounded reciprocal (one Newton step
+ * on the hardware estimate) with IEEE division over every float in its
+ * estimate range.  Returns a HIP error code; *mismatches must be 0. */
+int rsh_selfcheck_rcp(int device, uint64_t* tested, uint64_t* mismatches);
+
 /* Device memory helpers (plain HIP calls, for callers without a HIP runtime
  * binding of their own). */
 void* rsh_device_alloc(size_t bytes);

@@ -45,8 +45,8 @@ struct rsh_engine {
     // frame buffers
     uint32_t W = 0, H = 0, Lmax = 1;
     size_t cap_slots = 0;
-    DevBuf rays_a, rays_b, flags, keys, values, tkeys, tvalues, bases, sizes, ckeys, cvalues, skeys, svalues;
-    DevBuf rkeys, rvalues, skeleton, scan, rec0, rec1, nodes, aux;
+    DevBuf rays_a, rays_b, flags, keys, values, tvalues, bases, ckeys, cvalues, skeys, svalues;
+    DevBuf rvalues, soff, rec0, rec1, nodes, aux;
     DevBuf closest, shadow, pixels, counters, dcounts;
     // work list of the 64-ray traversal kernel
     DevBuf w_npass, w_cand, w_nitems, w_off, w_first, w_list, w_cum, w_items;
@@ -151,9 +151,11 @@ rsh_status ensure_frame(rsh_engine* e, uint32_t W, uint32_t H, uint32_t Lmax) {
     if (S <= e->cap_slots) return RSH_OK;
     size_t u = S * 4;
     HIP_OK(e->rays_a.reserve(S * 24)); HIP_OK(e->rays_b.reserve(S * 24));
-    for (DevBuf* b : {&e->flags, &e->keys, &e->values, &e->tkeys, &e->tvalues, &e->bases, &e->sizes, &e->ckeys,
-                      &e->cvalues, &e->skeys, &e->svalues, &e->rkeys, &e->rvalues, &e->skeleton, &e->scan, &e->shadow})
+    for (DevBuf* b : {&e->flags, &e->keys, &e->values, &e->tvalues, &e->ckeys, &e->cvalues, &e->skeys, &e->svalues,
+                      &e->rvalues, &e->shadow})
         HIP_OK(b->reserve(u));
+    HIP_OK(e->bases.reserve(u + 4));   // bases[C] = N closes the last chunk
+    HIP_OK(e->soff.reserve(u + 4));
     HIP_OK(e->closest.reserve(S * 8));
     HIP_OK(e->rec0.reserve(S * 16)); HIP_OK(e->rec1.reserve(S * 16));
     HIP_OK(e->pixels.reserve(P * 4));
@@ -266,17 +268,20 @@ rsh_status trace_pass(rsh_engine* e, const rsh_frame_config& cfg, const float* r
     const bool timing = flags_mask & 4u, counting = flags_mask & 2u;
     uint32_t* dc = u32p(e->dcounts);
     {
+        // trimming and compression in one walk over the slots (the compress
+        // stage time is part of the trim stage's)
         StageTimer t(e, ST_TRIM, timing);
-        HIP_OK(run_trim(u32p(e->flags), u32p(e->keys), u32p(e->values), S, u32p(e->tkeys), u32p(e->tvalues), dc,
-                        e->sc, s));
+        HIP_OK(run_pack(u32p(e->flags), u32p(e->keys), u32p(e->values), S, u32p(e->tvalues), u32p(e->bases),
+                        u32p(e->ckeys), u32p(e->cvalues), dc, e->sc, s));
     }
     uint32_t hc[2] = {0, 0};
     if (e->replaying) {
-        hc[0] = e->plan[e->plan_pos++].N;
+        const rsh_engine::PassPlan& pp = e->plan[e->plan_pos++];
+        hc[0] = pp.N; hc[1] = pp.C;
     } else {
-        HIP_OK(hipMemcpyAsync(hc, dc, 4, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(hc, dc, 8, hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
-        if (e->recording) { e->plan.emplace_back(); e->plan.back().N = hc[0]; }
+        if (e->recording) { e->plan.emplace_back(); e->plan.back().N = hc[0]; e->plan.back().C = hc[1]; }
     }
     const uint32_t N = hc[0];
     out.N = N;
@@ -285,18 +290,6 @@ rsh_status trace_pass(rsh_engine* e, const rsh_frame_config& cfg, const float* r
     if (N == 0) return RSH_OK;
     const uint32_t* sorted_values = u32p(e->tvalues);
     if (cfg.algorithm == RSH_CRSH) {
-        {
-            StageTimer t(e, ST_COMPRESS, timing);
-            HIP_OK(run_compress(u32p(e->tkeys), N, u32p(e->bases), u32p(e->sizes), u32p(e->ckeys), u32p(e->cvalues),
-                                dc + 1, e->sc, s));
-        }
-        if (e->replaying) {
-            hc[1] = e->plan[e->plan_pos - 1].C;
-        } else {
-            HIP_OK(hipMemcpyAsync(hc + 1, dc + 1, 4, hipMemcpyDeviceToHost, s));
-            HIP_OK(hipStreamSynchronize(s));
-            if (e->recording) e->plan.back().C = hc[1];
-        }
         const uint32_t C = hc[1];
         out.C = C;
         {
@@ -306,8 +299,9 @@ rsh_status trace_pass(rsh_engine* e, const rsh_frame_config& cfg, const float* r
         }
         {
             StageTimer t(e, ST_DECOMPRESS, timing);
-            HIP_OK(run_decompress(u32p(e->bases), u32p(e->sizes), u32p(e->svalues), u32p(e->tkeys), u32p(e->tvalues),
-                                  u32p(e->skeleton), u32p(e->scan), C, N, u32p(e->rkeys), u32p(e->rvalues), e->sc, s));
+            // the sorted keys are not needed after the sort: their buffer holds the chunk bases
+            HIP_OK(run_unpack(u32p(e->bases), u32p(e->svalues), u32p(e->tvalues), C, N, u32p(e->soff),
+                              u32p(e->skeys), u32p(e->rvalues), e->sc, s));
         }
         sorted_values = u32p(e->rvalues);
     } else {
@@ -428,8 +422,8 @@ void rsh_engine_destroy(rsh_engine* e) {
     for (DevBuf* b : {&e->tri_pos, &e->tri_nrm, &e->tri_mat, &e->mat_dif, &e->mat_spec, &e->light_pos, &e->light_col,
                       &e->light_int, &e->tri_sphere, &e->tri_mt, &e->obj_sphere, &e->obj_range, &e->cl_sphere,
                       &e->cl_base, &e->all_list, &e->rays_a,
-                      &e->rays_b, &e->flags, &e->keys, &e->values, &e->tkeys, &e->tvalues, &e->bases, &e->sizes,
-                      &e->ckeys, &e->cvalues, &e->skeys, &e->svalues, &e->rkeys, &e->rvalues, &e->skeleton, &e->scan,
+                      &e->rays_b, &e->flags, &e->keys, &e->values, &e->tvalues, &e->bases,
+                      &e->ckeys, &e->cvalues, &e->skeys, &e->svalues, &e->rvalues, &e->soff,
                       &e->rec0, &e->rec1, &e->nodes, &e->aux, &e->closest, &e->shadow, &e->pixels, &e->counters,
                       &e->dcounts, &e->w_npass, &e->w_cand, &e->w_nitems, &e->w_off, &e->w_first, &e->w_list,
                       &e->w_cum, &e->w_items, &e->views_d})

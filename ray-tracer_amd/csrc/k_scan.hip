@@ -205,6 +205,334 @@ __global__ void __launch_bounds__(256) k_chunk_sizes(const uint32_t* bases, cons
         sizes[x] = (x + 1 < C ? bases[x + 1] : n) - bases[x];
 }
 
+// ------------------------------------------ fused trim + compress (engine) --
+// One walk over the ray slots does the reference's trimming (stable compaction
+// of the live slots, flags == 0) and compression (one chunk per run of equal
+// keys among the live rays) together.  k_pack_tiles summarises each tile
+// (live count, chunk heads after its first live key, first and last live key),
+// k_pack_scan turns the summaries into offsets and the key carried into each
+// tile, and k_pack_write writes the live slot values and the chunk heads
+// (bases, keys, values = chunk index), staged in LDS so that every global
+// write is coalesced.  bases[C] = N closes the last chunk, so chunk sizes are
+// bases[v + 1] - bases[v] and are not stored.  The trimmed keys are not
+// written: after compression only the chunk keys are used.
+
+struct LastKey {
+    uint32_t valid, key;
+};
+
+// Block-wide scan of "the last live key so far" in thread order: returns the
+// value over the threads before this one, and the whole block's in *blk.
+// lds holds blockDim.x / 64 entries.
+__device__ __forceinline__ LastKey block_last_key(LastKey v, LastKey* lds, LastKey* blk) {
+    const uint32_t lane = lane_id(), w = wave_id(), nw = blockDim.x >> 6;
+    LastKey acc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t yv = __shfl_up(acc.valid, d, 64), yk = __shfl_up(acc.key, d, 64);
+        if (lane >= (uint32_t)d && !acc.valid) { acc.valid = yv; acc.key = yk; }
+    }
+    LastKey ex;
+    ex.valid = __shfl_up(acc.valid, 1, 64);
+    ex.key = __shfl_up(acc.key, 1, 64);
+    if (lane == 0) ex.valid = 0;
+    if (lane == 63) lds[w] = acc;
+    __syncthreads();
+    LastKey carry{0u, 0u}, tot{0u, 0u};
+    for (uint32_t k = 0; k < nw; k++) {
+        const LastKey t = lds[k];
+        if (t.valid) { if (k < w) carry = t; tot = t; }
+    }
+    __syncthreads();
+    *blk = tot;
+    return ex.valid ? ex : carry;
+}
+
+// Exclusive sum over a block of any multiple of 64 threads; lds holds
+// blockDim.x / 64 entries.
+__device__ __forceinline__ uint32_t block_excl_sum(uint32_t v, uint32_t* lds, uint32_t* total) {
+    const uint32_t w = wave_id(), nw = blockDim.x >> 6;
+    const uint32_t inc = wave_inclusive_add(v);
+    if (lane_id() == 63) lds[w] = inc;
+    __syncthreads();
+    uint32_t pre = 0, tot = 0;
+    for (uint32_t k = 0; k < nw; k++) {
+        const uint32_t t = lds[k];
+        pre += k < w ? t : 0u;
+        tot += t;
+    }
+    __syncthreads();
+    *total = tot;
+    return pre + inc - v;
+}
+
+// Load the 16 entries of a thread's slots in groups of 4, skipping groups
+// with no live slot (flags != 0 everywhere): sparse passes read little more
+// than their flags.  Skipped entries read as 0 and are never used.
+__device__ __forceinline__ void load16_live(const uint32_t* p, uint64_t base, uint64_t n, const uint32_t f[16],
+                                            uint32_t v[16]) {
+    if (base + 16 <= n) {
+        const uint4* q = reinterpret_cast<const uint4*>(p + base);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            uint4 a = make_uint4(0u, 0u, 0u, 0u);
+            if (f[4 * i] == 0u || f[4 * i + 1] == 0u || f[4 * i + 2] == 0u || f[4 * i + 3] == 0u) a = q[i];
+            v[4 * i] = a.x; v[4 * i + 1] = a.y; v[4 * i + 2] = a.z; v[4 * i + 3] = a.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 16; i++) v[i] = (base + i < n && f[i] == 0u) ? p[base + i] : 0u;
+    }
+}
+
+// A thread's 16 slots: live mask, heads after the thread's first live key,
+// the index and key of that first live key, and the last live key.
+struct PackThread {
+    uint32_t live, heads, first_i;
+    LastKey first, last;
+};
+
+__device__ __forceinline__ PackThread pack_thread(const uint32_t f[16], const uint32_t k[16], uint64_t base,
+                                                  uint64_t n) {
+    PackThread t{0u, 0u, 0u, {0u, 0u}, {0u, 0u}};
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        if (base + i < n && f[i] == 0u) {
+            if (t.last.valid) {
+                if (k[i] != t.last.key) t.heads |= 1u << i;
+            } else {
+                t.first.valid = 1u; t.first.key = k[i]; t.first_i = (uint32_t)i;
+            }
+            t.last.valid = 1u; t.last.key = k[i];
+            t.live |= 1u << i;
+        }
+    }
+    return t;
+}
+
+// meta[tile] = (live count, heads after the tile's first live key, first live key, last live key)
+__global__ void __launch_bounds__(256) k_pack_tiles(const uint32_t* flags, const uint32_t* keys, uint64_t n,
+                                                    uint4* meta) {
+    __shared__ LastKey lk[WAVES];
+    __shared__ uint32_t red[WAVES];
+    __shared__ uint32_t s_first;
+    const uint64_t base = (uint64_t)blockIdx.x * TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t f[16], k[16];
+    load16(flags, base, n, f, 1u);
+    load16_live(keys, base, n, f, k);
+    const PackThread t = pack_thread(f, k, base, n);
+    LastKey blk;
+    const LastKey carry = block_last_key(t.last, lk, &blk);
+    if (t.first.valid && !carry.valid) s_first = t.first.key;     // the tile's first live key
+    const uint32_t h = (uint32_t)__popc(t.heads) + (t.first.valid && carry.valid && t.first.key != carry.key);
+    uint32_t total;
+    (void)block_excl_sum((h << 16) | (uint32_t)__popc(t.live), red, &total);
+    if (threadIdx.x == 0) meta[blockIdx.x] = make_uint4(total & 0xFFFFu, total >> 16, blk.valid ? s_first : 0u, blk.key);
+}
+
+// Single block: meta[tile] becomes (live offset, chunk offset, carried key,
+// carried key valid); totals[0] = N live rays, totals[1] = C chunks,
+// bases[C] = N.
+__global__ void __launch_bounds__(1024) k_pack_scan(uint4* meta, uint32_t ntiles, uint32_t* totals, uint32_t* bases) {
+    __shared__ LastKey lk[16];
+    __shared__ uint32_t red[16];
+    constexpr uint32_t PER = MAX_TILES / 1024;      // tiles per thread, held in registers
+    const uint32_t b = threadIdx.x * PER;
+    uint4 m[PER];
+    uint32_t ls = 0;
+    LastKey last{0u, 0u};
+#pragma unroll
+    for (uint32_t i = 0; i < PER; i++) {
+        m[i] = b + i < ntiles ? meta[b + i] : make_uint4(0u, 0u, 0u, 0u);
+        ls += m[i].x;
+        if (m[i].x) { last.valid = 1u; last.key = m[i].w; }
+    }
+    uint32_t ntot, ctot;
+    const uint32_t lo0 = block_excl_sum(ls, red, &ntot);
+    LastKey blk;
+    const LastKey c0 = block_last_key(last, lk, &blk);
+    uint32_t hs = 0;
+    LastKey c = c0;
+#pragma unroll
+    for (uint32_t i = 0; i < PER; i++) {
+        if (m[i].x) {
+            hs += m[i].y + ((!c.valid || m[i].z != c.key) ? 1u : 0u);
+            c.valid = 1u; c.key = m[i].w;
+        }
+    }
+    const uint32_t ho0 = block_excl_sum(hs, red, &ctot);
+    uint32_t lo = lo0, ho = ho0;
+    c = c0;
+#pragma unroll
+    for (uint32_t i = 0; i < PER; i++) {
+        if (b + i < ntiles) meta[b + i] = make_uint4(lo, ho, c.key, c.valid);
+        if (m[i].x) {
+            ho += m[i].y + ((!c.valid || m[i].z != c.key) ? 1u : 0u);
+            c.valid = 1u; c.key = m[i].w;
+        }
+        lo += m[i].x;
+    }
+    if (threadIdx.x == 0) { totals[0] = ntot; totals[1] = ctot; bases[ctot] = ntot; }
+}
+
+__global__ void __launch_bounds__(256) k_pack_write(const uint32_t* flags, const uint32_t* keys,
+                                                    const uint32_t* values, uint64_t n, const uint4* meta,
+                                                    uint32_t* tvalues, uint32_t* bases, uint32_t* ckeys,
+                                                    uint32_t* cvalues) {
+    __shared__ uint32_t sa[TILE];
+    __shared__ uint32_t sb[TILE];
+    __shared__ LastKey lk[WAVES];
+    __shared__ uint32_t red[WAVES];
+    const uint64_t base = (uint64_t)blockIdx.x * TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    const uint4 m = meta[blockIdx.x];
+    uint32_t f[16], k[16], v[16];
+    load16(flags, base, n, f, 1u);
+    load16_live(keys, base, n, f, k);
+    load16_live(values, base, n, f, v);
+    PackThread t = pack_thread(f, k, base, n);
+    LastKey blk;
+    LastKey carry = block_last_key(t.last, lk, &blk);
+    if (!carry.valid && m.w) { carry.valid = 1u; carry.key = m.z; }
+    if (t.first.valid && (!carry.valid || t.first.key != carry.key)) t.heads |= 1u << t.first_i;
+    const uint32_t nl = (uint32_t)__popc(t.live), nh = (uint32_t)__popc(t.heads);
+    uint32_t total;
+    const uint32_t ex = block_excl_sum((nh << 16) | nl, red, &total);
+    const uint32_t tl = total & 0xFFFFu, th = total >> 16;
+    uint32_t lp = ex & 0xFFFFu;
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+        if (t.live & (1u << i)) sa[lp++] = v[i];
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < tl; i += blockDim.x) tvalues[m.x + i] = sa[i];
+    __syncthreads();
+    lp = ex & 0xFFFFu;
+    uint32_t hp = ex >> 16;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        if (t.live & (1u << i)) {
+            if (t.heads & (1u << i)) { sa[hp] = m.x + lp; sb[hp] = k[i]; hp++; }
+            lp++;
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < th; i += blockDim.x) {
+        bases[m.y + i] = sa[i];
+        ckeys[m.y + i] = sb[i];
+        cvalues[m.y + i] = m.y + i;
+    }
+}
+
+// ------------------------------------------- decompression (engine) ------
+// Sorted chunk x (value v = sv[x]) expands to the live rays
+// [bases[v], bases[v + 1]) in order.  k_unpack_sizes sums each tile's chunk
+// sizes; after the tile scan, k_unpack_offsets writes every sorted chunk's
+// output offset and source base.  k_unpack_expand then gives each block a
+// fixed range of TILE outputs (chunk sizes are very uneven, so output ranges
+// per chunk tile would be too): two binary searches over the offsets find the
+// chunks that cover the range, their offsets and bases are staged in LDS,
+// and each output finds its chunk by a binary search there.
+__global__ void __launch_bounds__(256) k_unpack_sizes(const uint32_t* sv, const uint32_t* bases, uint64_t C,
+                                                      uint32_t* tile_totals) {
+    __shared__ uint32_t red[WAVES];
+    const uint64_t base = (uint64_t)blockIdx.x * TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t v[16];
+    load16(sv, base, C, v, 0u);
+    uint32_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+        if (base + i < C) sum += bases[v[i] + 1] - bases[v[i]];
+    uint32_t total;
+    (void)block_excl_sum(sum, red, &total);
+    if (threadIdx.x == 0) tile_totals[blockIdx.x] = total;
+}
+
+// soff[x] = output offset of sorted chunk x, sbase[x] = its first live ray;
+// soff[C] = N.
+__global__ void __launch_bounds__(256) k_unpack_offsets(const uint32_t* sv, const uint32_t* bases, uint64_t C,
+                                                        const uint32_t* tile_offsets, uint32_t* soff,
+                                                        uint32_t* sbase) {
+    __shared__ uint32_t red[WAVES];
+    const uint64_t base = (uint64_t)blockIdx.x * TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    uint32_t v[16], b0[16], sz[16];
+    load16(sv, base, C, v, 0u);
+    uint32_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        b0[i] = 0u; sz[i] = 0u;
+        if (base + i < C) { b0[i] = bases[v[i]]; sz[i] = bases[v[i] + 1] - b0[i]; }
+        sum += sz[i];
+    }
+    uint32_t total;
+    uint32_t o = tile_offsets[blockIdx.x] + block_excl_sum(sum, red, &total);
+    if (base + 16 <= C) {
+        uint4* qo = reinterpret_cast<uint4*>(soff + base);
+        uint4* qb = reinterpret_cast<uint4*>(sbase + base);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            uint4 a, c;
+            a.x = o; o += sz[4 * i]; a.y = o; o += sz[4 * i + 1]; a.z = o; o += sz[4 * i + 2]; a.w = o; o += sz[4 * i + 3];
+            c.x = b0[4 * i]; c.y = b0[4 * i + 1]; c.z = b0[4 * i + 2]; c.w = b0[4 * i + 3];
+            qo[i] = a; qb[i] = c;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            if (base + i < C) { soff[base + i] = o; sbase[base + i] = b0[i]; }
+            o += sz[i];
+        }
+    }
+    if (base < C && base + 16 >= C) soff[C] = o;      // the thread holding the last chunk
+}
+
+// Largest x in [lo, hi) with a[x] <= p, given a[lo] <= p.
+__device__ __forceinline__ uint32_t upper_index(const uint32_t* a, uint32_t lo, uint32_t hi, uint32_t p) {
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] <= p) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256) k_unpack_expand(const uint32_t* soff, const uint32_t* sbase, uint32_t C,
+                                                       uint32_t N, const uint32_t* tvalues, uint32_t* out_values) {
+    __shared__ uint32_t s_off[TILE + 1];
+    __shared__ uint32_t s_base[TILE];
+    __shared__ uint32_t s_x[2];
+    const uint32_t p0 = blockIdx.x * TILE;
+    const uint32_t p1 = p0 + TILE < N ? p0 + TILE : N;
+    if (threadIdx.x < 2) s_x[threadIdx.x] = upper_index(soff, 0, C, threadIdx.x == 0 ? p0 : p1 - 1);
+    __syncthreads();
+    const uint32_t x0 = s_x[0], nx = s_x[1] - x0 + 1;     // chunks covering [p0, p1): at most TILE
+    for (uint32_t i = threadIdx.x; i < nx; i += blockDim.x) {
+        s_off[i] = soff[x0 + i];
+        s_base[i] = sbase[x0 + i];
+    }
+    s_off[nx] = 0xFFFFFFFFu;                               // sentinel for the walk below
+    __syncthreads();
+    // thread t writes the 16 consecutive outputs q .. q + 15: one search, then
+    // a walk (measured faster than lane-consecutive outputs, which need a
+    // search each)
+    const uint32_t q = p0 + threadIdx.x * SCAN_ITEMS;
+    if (q >= p1) return;
+    uint32_t x = upper_index(s_off, 0, nx, q);
+    uint32_t nxt = s_off[x + 1];
+    uint32_t o[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const uint32_t p = q + i;
+        while (p >= nxt) { x++; nxt = s_off[x + 1]; }
+        o[i] = p < p1 ? tvalues[s_base[x] + (p - s_off[x])] : 0u;
+    }
+    if (q + 16 <= p1) {
+        uint4* d = reinterpret_cast<uint4*>(out_values + q);
+#pragma unroll
+        for (int i = 0; i < 4; i++) d[i] = make_uint4(o[4 * i], o[4 * i + 1], o[4 * i + 2], o[4 * i + 3]);
+    } else {
+        for (int i = 0; i < 16; i++)
+            if (q + i < p1) out_values[q + i] = o[i];
+    }
+}
+
 // --------------------------------------------------------- radix sorting --
 __global__ void __launch_bounds__(256) k_radix_upsweep(const uint32_t* keys, uint64_t n, uint32_t shift,
                                                        uint32_t* hist, uint32_t ntiles) {
@@ -213,11 +541,15 @@ __global__ void __launch_bounds__(256) k_radix_upsweep(const uint32_t* keys, uin
     __syncthreads();
     const uint32_t w = wave_id(), lane = lane_id();
     const uint64_t tb = (uint64_t)blockIdx.x * TILE + (uint64_t)w * 1024;
-#pragma unroll 4
+    uint32_t k[16];      // all loads issued before the first use
+#pragma unroll
     for (uint32_t r = 0; r < 16; r++) {
-        uint64_t p = tb + r * 64 + lane;
-        if (p < n) atomicAdd(&cnt[w][(keys[p] >> shift) & 0xFFu], 1u);
+        const uint64_t p = tb + r * 64 + lane;
+        k[r] = p < n ? keys[p] : 0u;
     }
+#pragma unroll
+    for (uint32_t r = 0; r < 16; r++)
+        if (tb + r * 64 + lane < n) atomicAdd(&cnt[w][(k[r] >> shift) & 0xFFu], 1u);
     __syncthreads();
     const uint32_t d = threadIdx.x;
     uint32_t s = 0;
@@ -242,6 +574,9 @@ __global__ void __launch_bounds__(256) k_radix_rowscan(uint32_t* hist, uint32_t 
     if (threadIdx.x == 0) digit_totals[blockIdx.x] = total;
 }
 
+// ROWSCAN: hist holds the upsweep's raw tile counts and each block sums its
+// digit rows itself (for small tile counts, in place of k_radix_rowscan).
+template <bool ROWSCAN>
 __global__ void __launch_bounds__(256) k_radix_downsweep(const uint32_t* keys, const uint32_t* vals, uint64_t n,
                                                          uint32_t shift, const uint32_t* hist, uint32_t ntiles,
                                                          const uint32_t* digit_totals, uint32_t* okeys,
@@ -255,7 +590,18 @@ __global__ void __launch_bounds__(256) k_radix_downsweep(const uint32_t* keys, c
     const uint32_t tid = threadIdx.x, w = wave_id(), lane = lane_id();
     for (uint32_t i = tid; i < WAVES * 256; i += blockDim.x) (&whist[0][0])[i] = 0;
     // global base of each digit for this tile
-    {
+    if (ROWSCAN) {
+        const uint32_t* row = hist + (size_t)tid * ntiles;
+        uint32_t before = 0, dt = 0;
+        for (uint32_t i = 0; i < ntiles; i++) {
+            const uint32_t c = row[i];
+            before += i < blockIdx.x ? c : 0u;
+            dt += c;
+        }
+        uint32_t t;
+        const uint32_t pre = block_exclusive_add(dt, lds, &t);
+        gbase[tid] = pre + before;
+    } else {
         uint32_t t;
         uint32_t dt = digit_totals[tid];
         uint32_t pre = block_exclusive_add(dt, lds, &t);
@@ -266,12 +612,16 @@ __global__ void __launch_bounds__(256) k_radix_downsweep(const uint32_t* keys, c
     const uint64_t lt = (1ull << lane) - 1ull;
     uint32_t key[16], val[16], rank[16];
 #pragma unroll
+    for (uint32_t r = 0; r < 16; r++) {      // all loads issued before the ranking
+        const uint64_t p = tb + r * 64 + lane;
+        key[r] = p < n ? keys[p] : 0u;
+        val[r] = p < n ? vals[p] : 0u;
+    }
+#pragma unroll
     for (uint32_t r = 0; r < 16; r++) {
         uint64_t p = tb + r * 64 + lane;
         bool valid = p < n;
-        uint32_t k = valid ? keys[p] : 0u;
-        key[r] = k;
-        val[r] = valid ? vals[p] : 0u;
+        uint32_t k = key[r];
         uint32_t d = (k >> shift) & 0xFFu;
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -427,6 +777,41 @@ hipError_t run_compress(const uint32_t* keys, uint64_t n, uint32_t* bases, uint3
     return hipGetLastError();
 }
 
+hipError_t run_pack(const uint32_t* flags, const uint32_t* keys, const uint32_t* values, uint64_t n,
+                    uint32_t* tvalues, uint32_t* bases, uint32_t* ckeys, uint32_t* cvalues, uint32_t* d_totals,
+                    ScanScratch& sc, hipStream_t s) {
+    const uint32_t nt = tiles_for(n);
+    if (nt > MAX_TILES) return hipErrorInvalidValue;
+    if (nt == 0) {
+        launch_fill_u32(d_totals, 0, 2, s);
+        launch_fill_u32(bases, 0, 1, s);
+        return hipGetLastError();
+    }
+    uint4* meta = sc.hist.as<uint4>();
+    k_pack_tiles<<<nt, 256, 0, s>>>(flags, keys, n, meta);
+    k_pack_scan<<<1, 1024, 0, s>>>(meta, nt, d_totals, bases);
+    k_pack_write<<<nt, 256, 0, s>>>(flags, keys, values, n, meta, tvalues, bases, ckeys, cvalues);
+    return hipGetLastError();
+}
+
+hipError_t run_unpack(const uint32_t* bases, const uint32_t* sorted_cvalues, const uint32_t* tvalues, uint64_t C,
+                      uint64_t N, uint32_t* soff, uint32_t* sbase, uint32_t* out_values, ScanScratch& sc,
+                      hipStream_t s) {
+    const uint32_t nt = tiles_for(C);
+    if (nt > MAX_TILES) return hipErrorInvalidValue;
+    if (nt == 0 || N == 0) return hipSuccess;
+    uint32_t* tiles = sc.tile_counts.as<uint32_t>();
+    k_unpack_sizes<<<nt, 256, 0, s>>>(sorted_cvalues, bases, C, tiles);
+    k_scan_tiles<<<1, 1024, 0, s>>>(tiles, nt, sc.total.as<uint32_t>());
+    k_unpack_offsets<<<nt, 256, 0, s>>>(sorted_cvalues, bases, C, tiles, soff, sbase);
+    k_unpack_expand<<<tiles_for(N), 256, 0, s>>>(soff, sbase, (uint32_t)C, (uint32_t)N, tvalues, out_values);
+    return hipGetLastError();
+}
+
+// Up to this many tiles the downsweep sums the digit rows itself (saves a launch
+// per digit; each block reads 256 x tiles counts).
+constexpr uint32_t ROWSCAN_IN_DOWNSWEEP = 64;
+
 hipError_t run_radix_sort(const uint32_t* keys, const uint32_t* values, uint64_t n, uint32_t bits,
                           uint32_t* out_keys, uint32_t* out_values, ScanScratch& sc, hipStream_t s) {
     if (n == 0) return hipSuccess;
@@ -442,9 +827,14 @@ hipError_t run_radix_sort(const uint32_t* keys, const uint32_t* values, uint64_t
         uint32_t dst = ((passes - 1 - pass) & 1u) ? 0u : 1u;
         uint32_t shift = 8 * pass;
         k_radix_upsweep<<<nt, 256, 0, s>>>(ik, n, shift, sc.hist.as<uint32_t>(), nt);
-        k_radix_rowscan<<<256, 256, 0, s>>>(sc.hist.as<uint32_t>(), nt, sc.digit_totals.as<uint32_t>());
-        k_radix_downsweep<<<nt, 256, 0, s>>>(ik, iv, n, shift, sc.hist.as<uint32_t>(), nt,
-                                             sc.digit_totals.as<uint32_t>(), bk[dst], bv[dst]);
+        if (nt <= ROWSCAN_IN_DOWNSWEEP) {
+            k_radix_downsweep<true><<<nt, 256, 0, s>>>(ik, iv, n, shift, sc.hist.as<uint32_t>(), nt, nullptr,
+                                                       bk[dst], bv[dst]);
+        } else {
+            k_radix_rowscan<<<256, 256, 0, s>>>(sc.hist.as<uint32_t>(), nt, sc.digit_totals.as<uint32_t>());
+            k_radix_downsweep<false><<<nt, 256, 0, s>>>(ik, iv, n, shift, sc.hist.as<uint32_t>(), nt,
+                                                        sc.digit_totals.as<uint32_t>(), bk[dst], bv[dst]);
+        }
         ik = bk[dst];
         iv = bv[dst];
     }

@@ -423,7 +423,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
     __shared__ uint16_t s_pair[PCAP];          // survivor << 5 | leaf
     __shared__ float4 s_lsph[BRANCH];          // leaf nodes of the top node
     __shared__ float4 s_lcone[BRANCH];
-    __shared__ float4 s_laux[BRANCH];          // (cos, tan, lmax, r / cos)
+    __shared__ float4 s_laux[BRANCH];          // (cos, tan, lmax, rcos): node_bound
     __shared__ uint8_t s_used[64];             // survivor of the round appears in a pair
     const uint32_t lane = lane_id();
     const uint32_t item = blockIdx.x;
@@ -502,7 +502,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
                     e = s_pair[ci];
                     const uint32_t b = e & 31u;
                     const float4 la = s_laux[b];
-                    lp = sphere_node(s_lsph[b], s_lcone[b], s_qsph[c0 + (e >> 5)], la.x, la.y);
+                    lp = sphere_node_fast(s_lsph[b], s_lcone[b], s_qsph[c0 + (e >> 5)], la.x, la.y, la.w);
                 }
                 const uint64_t pm = __ballot(lp);
                 if (lp) {     // in place: entries c.. were read above, writes land below c + 64
@@ -625,7 +625,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
             if (done) break;             // every ray of the wave is occluded: stop queueing
             // ---- stage A
             bool p = false;
-            if (ink[k]) p = sphere_node(tS, tC, tsk[k], tA.x, tA.y);
+            if (ink[k]) p = sphere_node_fast(tS, tC, tsk[k], tA.x, tA.y, t_bound.y);
             const uint64_t m = __ballot(p);
             if (count) { n_top += (uint64_t)__popcll(__ballot(ink[k])); n_surv += (uint64_t)__popcll(m); }
             if (m == 0ull) continue;

@@ -232,11 +232,28 @@ __device__ inline uint32_t key_oct(const HashParams& h, v3 o, v3 d, uint32_t typ
 }
 
 // -------------------------------------------------------------- geometry --
+__device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// Correctly rounded 1/x.  For |x| in [2^-125, 2^126) (exponent field 2..252)
+// one Newton step on v_rcp_f32, y = y0 + y0 (1 - x y0) in two FMAs, equals the
+// IEEE quotient for every input (an exhaustive device check over all such
+// floats, rsh_selfcheck_rcp, is part of the test suite); other inputs, zeros,
+// denormals, infinities and NaN take the IEEE division.
+__device__ __forceinline__ float rcp_rn(float x) {
+    const uint32_t ex = (__float_as_uint(x) >> 23) & 0xFFu;
+    if (ex - 2u <= 250u) {
+        const float y0 = hw_rcp(x);
+        return __builtin_fmaf(__builtin_fmaf(-x, y0, 1.0f), y0, y0);
+    }
+    return 1.0f / x;
+}
+
 // RayTracer.cu:257-280 (reciprocal correctly rounded instead of __fdividef)
 __device__ __forceinline__ float ray_triangle(v3 o, v3 d, v3 v0, v3 e1, v3 e2) {
     v3 tvec = o - v0;
     v3 pvec = cross(d, e2);
-    float det = 1.0f / dot(e1, pvec);
+    float det = rcp_rn(dot(e1, pvec));
     float u = dot(tvec, pvec) * det;
     if (u < 0.0f || u > 1.0f) return -1.0f;
     v3 qvec = cross(tvec, e1);
@@ -255,6 +272,42 @@ __device__ __forceinline__ bool sphere_node(float4 sphere, float4 cone, float4 t
     return (pr * tangent + (sphere.w + tri.w) / cosine) >= len(s2t - proj);
 }
 
+// 1/cos from v_rcp_f32 for sphere_node_fast and the cluster test, or NaN for
+// a cosine outside [1e-30, 1e30] (there the estimate is not trusted).
+__device__ __forceinline__ float node_rcos(float cosine) {
+    const float a = fabsf(cosine);
+    return (a >= 1e-30f && a <= 1e30f) ? hw_rcp(cosine) : __int_as_float(0x7FC00000);
+}
+
+// sphere_node with the same result, decided from hardware estimates when
+// they are clearly on one side.  With rcos = 1/cos and the square root from
+// v_rcp_f32 / v_sqrt_f32 (each within 1 ulp), the estimated sides
+// lhs' = pr tan + (r + rt) rcos and rhs' = |v| differ from the correctly
+// rounded lhs and rhs by at most 2^-21 (|pr tan| + |(r + rt) rcos|) and
+// 2^-22 |v| (plus denormal ulps for values below 1e-19).  When
+// |lhs' - rhs'| exceeds M = 2^-18 (|pr tan| + |(r + rt) rcos| + |v|) + 2^-60,
+// four times those bounds, lhs' >= rhs' decides as lhs >= rhs would;
+// otherwise, and whenever a value is NaN or infinite (rcos NaN included),
+// the exact expression of sphere_node decides.
+__device__ __forceinline__ bool sphere_node_fast(float4 sphere, float4 cone, float4 tri, float cosine, float tangent,
+                                                 float rcos) {
+    if (cone.w == HALF_PI_F) return true;
+    const v3 cd = xyz(cone);
+    const v3 s2t = xyz(tri) - xyz(sphere);
+    const float pr = dot(s2t, cd);
+    const v3 v = s2t - pr * cd;
+    const float dd = dot(v, v);
+    const float a = pr * tangent;
+    const float R = sphere.w + tri.w;
+    const float q = R * rcos;
+    const float rs = hw_sqrt(dd);
+    const float diff = (a + q) - rs;
+    const float M = 0x1p-18f * (fabsf(a) + fabsf(q) + rs) + 0x1p-60f;
+    if (diff > M) return true;
+    if (diff < -M) return false;
+    return (a + R / cosine) >= sqrtf(dd);
+}
+
 // Conservative cluster pre-test for sphere_node.  With J = I - cd cd^T the
 // triangle test passes iff f(c) = |J (c - s)| - ((c - s).cd) tan <= (r + rt)/cos.
 // f is Lipschitz with constant L = sqrt(1 + (tan k)^2 + 2 tan k |1 - k^2|),
@@ -267,22 +320,19 @@ __device__ __forceinline__ bool sphere_node(float4 sphere, float4 cone, float4 t
 // evaluation, is ~1e-6 of those magnitudes, far inside E.  Nodes outside the
 // argument's assumptions (the HALF_PI shortcut, cos <= 0, tan < 0,
 // non-finite values, |cd| far from 1) never reject; a NaN anywhere passes.
-// node_bound returns (max(L, 1/cos) slightly inflated, r/cos), with -1 in the
-// first for a node the argument does not cover; cluster_may_pass then never
-// rejects.
-__device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
-__device__ __forceinline__ float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
-
+// node_bound returns (max(L, 1/cos) slightly inflated, node_rcos(cos)), with
+// -1 in the first for a node the argument does not cover (cluster_may_pass
+// then never rejects).
 __device__ __forceinline__ float2 node_bound(float4 sphere, float4 cone, float cosine, float tangent) {
     const float k = hw_sqrt(dot(xyz(cone), xyz(cone)));
     const float tk = tangent * k;
     const float L = hw_sqrt(1.0f + tk * tk + 2.0f * tk * fabsf(1.0f - k * k));
-    const float rc = hw_rcp(cosine);
+    const float rc = node_rcos(cosine);
     const float lmax = fmaxf(L, rc) * 1.00001f;
-    const bool ok = cone.w != HALF_PI_F && cosine > 0.0f && tangent >= 0.0f && k > 0.5f && k < 1.4f &&
+    const bool ok = cone.w != HALF_PI_F && cosine >= 1e-30f && rc == rc && tangent >= 0.0f && k > 0.5f && k < 1.4f &&
                     isfinite(cosine) && isfinite(tangent) && isfinite(lmax) && isfinite(sphere.w) &&
                     isfinite(sphere.x) && isfinite(sphere.y) && isfinite(sphere.z);
-    return make_float2(ok ? lmax : -1.0f, sphere.w * rc);
+    return make_float2(ok ? lmax : -1.0f, rc);
 }
 
 // bound = node_bound(sphere, cone, cos, tan)
@@ -295,7 +345,7 @@ __device__ __forceinline__ bool cluster_may_pass(float4 sphere, float4 cone, flo
     const v3 v = s2q - pr * cd;
     const float f = hw_sqrt(dot(v, v)) - pr * tangent;
     const float rho = cl.w;
-    const float allow = bound.y + rho * lmax;
+    const float allow = sphere.w * bound.y + rho * lmax;
     const float E = 1e-4f * (hw_sqrt(dot(s2q, s2q)) + rho + (fabsf(pr) + rho) * tangent + (sphere.w + rho) * lmax) +
                     1e-4f;
     return !(f > allow + E);

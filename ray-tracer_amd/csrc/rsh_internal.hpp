@@ -80,6 +80,19 @@ hipError_t run_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, Sca
 hipError_t run_compress(const uint32_t* keys, uint64_t n, uint32_t* bases, uint32_t* sizes,
                         uint32_t* ckeys, uint32_t* cvalues, uint32_t* d_total, ScanScratch& sc,
                         hipStream_t s);
+// Trimming and compression in one walk (the engine's path): the live slots'
+// values in order -> tvalues, one chunk per run of equal live keys -> bases
+// (first live position; bases[C] = N), ckeys, cvalues (chunk index).
+// d_totals[0] = N, d_totals[1] = C.  bases holds n + 1 entries.
+hipError_t run_pack(const uint32_t* flags, const uint32_t* keys, const uint32_t* values, uint64_t n,
+                    uint32_t* tvalues, uint32_t* bases, uint32_t* ckeys, uint32_t* cvalues, uint32_t* d_totals,
+                    ScanScratch& sc, hipStream_t s);
+// Decompression of run_pack's chunks in sorted order: out_values receives
+// tvalues[bases[v] .. bases[v + 1]) for v = sorted_cvalues[0 .. C) in turn
+// (N values in all).  soff (C + 1 entries) and sbase (C) are scratch.
+hipError_t run_unpack(const uint32_t* bases, const uint32_t* sorted_cvalues, const uint32_t* tvalues, uint64_t C,
+                      uint64_t N, uint32_t* soff, uint32_t* sbase, uint32_t* out_values, ScanScratch& sc,
+                      hipStream_t s);
 // Stable LSD radix sort of (keys, values) over the low `bits` bits; result
 // in out_keys / out_values (inputs preserved).
 hipError_t run_radix_sort(const uint32_t* keys, const uint32_t* values, uint64_t n, uint32_t bits,

@@ -104,7 +104,7 @@ NATIVE_SYMBOLS = ["rsh_engine_create", "rsh_engine_destroy", "rsh_last_error", "
                   "rsh_device_alloc", "rsh_device_free", "rsh_memcpy_htod", "rsh_memcpy_dtoh",
                   "rsh_memset_d32", "rsh_device_synchronize", "rsh_device_count",
                   "rsh_compat_configure", "rsh_compat_set_stream", "rsh_compat_set_exit_on_error",
-                  "rsh_compat_status", "rsh_compat_error", "rsh_compat_reset_status"]
+                  "rsh_compat_status", "rsh_compat_error", "rsh_compat_reset_status", "rsh_selfcheck_rcp"]
 
 _lib = None
 
@@ -163,6 +163,8 @@ def lib():
     L.rsh_compat_set_exit_on_error.argtypes = [C.c_int]
     L.rsh_compat_status.restype = C.c_int
     L.rsh_compat_error.restype = C.c_char_p
+    L.rsh_selfcheck_rcp.restype = C.c_int
+    L.rsh_selfcheck_rcp.argtypes = [C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     for name, args in COMPAT_SIGNATURES.items():
         fn = getattr(L, name)
         fn.restype = None
@@ -174,6 +176,15 @@ def lib():
                                                                    "bindFragmentNormalArray") else [PV, C.c_uint]
     _lib = L
     return L
+
+
+def selfcheck_rcp(device=0):
+    """(tested, mismatches) of the device reciprocal check (rsh_selfcheck_rcp)."""
+    t, m = C.c_uint64(0), C.c_uint64(0)
+    rc = lib().rsh_selfcheck_rcp(device, C.byref(t), C.byref(m))
+    if rc != 0:
+        raise RshError(f"rsh_selfcheck_rcp failed with HIP error {rc}")
+    return t.value, m.value
 
 
 # ------------------------------------------------------------ device memory --

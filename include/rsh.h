@@ -91,6 +91,10 @@ typedef struct rsh_frame_config {
     uint32_t traversal_chunk;       /* 64-ray kernel: candidate clusters (64 consecutive
                                        triangles of one object) per work item; 0 = sized
                                        per pass                                           */
+    uint32_t traversal_prefilter;   /* 64-ray kernel: test each ray's line against the
+                                       triangle's bounding sphere before the exact test;
+                                       0 default (on), 1 off, 2 on.  Hit records are the
+                                       same either way.                                    */
 } rsh_frame_config;
 
 #define RSH_MAX_PASSES 16
@@ -117,6 +121,8 @@ typedef struct rsh_frame_stats {
     uint64_t pass_leaf_steps[RSH_MAX_PASSES];    /* 64-ray kernel: wave steps of leaf tests (bit 1) */
     uint64_t pass_ray_steps[RSH_MAX_PASSES];     /* 64-ray kernel: wave steps of ray tests (bit 1)  */
     uint32_t graph_replayed;                     /* 1: the call replayed a captured graph (bit 3) */
+    uint64_t pass_mt_tests[RSH_MAX_PASSES];      /* Moller-Trumbore tests run after the ray pre-test
+                                                    (bit 1; equals pass_ray_tests without it)   */
 } rsh_frame_stats;
 
 typedef struct rsh_engine rsh_engine;

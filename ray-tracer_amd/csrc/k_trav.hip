@@ -396,27 +396,35 @@ __global__ void __launch_bounds__(256) k_item_desc(TravParams P, const uint32_t*
 //  B  Per round of up to SCH queued survivors, the (survivor, leaf)
 //     candidates allowed by the masks are listed and tested 64 per step;
 //     passing pairs are compacted in place.
-//  C  The Moller-Trumbore vectors of the survivors that appear in a pair are
-//     loaded, then the pair list is walked BRANCH pairs per step, lane =
-//     pair p0 + lane / LEAF, ray lane % LEAF of the pair's leaf; closest hits
-//     merge per ray in LDS (atomicMin), shadow rays set a flag.
+//  C  The pair list is walked BRANCH pairs per step, lane = pair p0 + lane /
+//     LEAF, ray lane % LEAF of the pair's leaf.  With the pre-test (PRE, the
+//     default) each ray's line is first tested against the survivor's bounding
+//     sphere (ray_may_hit_sphere); the (survivor, ray) tests that may hit go to
+//     an LDS ring and run the Moller-Trumbore test 64 at a time, the vectors
+//     loaded from memory per test.  Without it, the vectors of the survivors in
+//     a pair are staged in LDS and every (pair, ray) runs the exact test.
+//     Closest hits merge per ray in LDS (atomicMin), shadow rays set a flag.
 // The set of (leaf, triangle) pairs and ray tests is the reference's; only
 // the order differs, and any-hit / closest-hit are order independent.
 // One wave per workgroup: items differ widely in length, and a workgroup's LDS
 // is held until its last wave ends, so a finished item must not wait for
 // others (measured: 2x faster than four-wave workgroups on the 1080p atrium).
-// The register budget is capped at 128 VGPRs for four waves per SIMD; the
-// 10 KB of LDS per wave allows sixteen waves per CU.
-template <int LEAF>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_traverse64(TravParams P) {
+// With the pre-test the kernel needs 7.3 KB of LDS and under 102 VGPRs per
+// wave, so five waves per SIMD (twenty per CU) fit; without it, four.  The
+// pre-test rejects most tests of spread-out secondary rays (99% on the atrium's
+// second bounce) and about 60% of primary ones, and measured faster on every
+// pass of the 1080p atrium frame.
+template <int LEAF, bool PRE>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 4))) k_traverse64(TravParams P) {
     constexpr uint32_t BRANCH = 64 / LEAF;
+    constexpr uint32_t TQ = 128;                  // ring of pre-tested (survivor, ray) tests
     constexpr uint32_t QCAP = 128;                // survivor queue
     constexpr uint32_t PCAP = 512;                // (survivor, leaf) candidate / pair list
     constexpr uint32_t SCH = PCAP / BRANCH < 64 ? PCAP / BRANCH : 64;   // survivors per B+C round
     __shared__ float4 s_qsph[QCAP];
     __shared__ uint32_t s_qidx[QCAP];
     __shared__ uint32_t s_qmask[QCAP];         // leaves the survivor's cluster may pass
-    __shared__ float4 s_mt[64][3];
+    __shared__ float4 s_mt[PRE ? 1 : 64][3];      // Moller-Trumbore vectors of a round's survivors
     __shared__ float4 s_ray0[64];              // (o + d*eps, tmax)
     __shared__ float4 s_ray1[64];              // (d, valid)
     __shared__ unsigned long long s_best[64];  // closest record / shadow flag per ray
@@ -424,7 +432,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
     __shared__ float4 s_lsph[BRANCH];          // leaf nodes of the top node
     __shared__ float4 s_lcone[BRANCH];
     __shared__ float4 s_laux[BRANCH];          // (cos, tan, lmax, rcos): node_bound
-    __shared__ uint8_t s_used[64];             // survivor of the round appears in a pair
+    __shared__ uint8_t s_used[PRE ? 1 : 64];   // survivor of the round appears in a pair
+    __shared__ uint16_t s_tq[PRE ? TQ : 1];
     const uint32_t lane = lane_id();
     const uint32_t item = blockIdx.x;
     if (item >= P.items) return;
@@ -466,7 +475,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
     const float2 tA = P.aux[gt];
     const uint64_t alive = __ballot(valid);
     const bool count = P.counters != nullptr;
-    uint64_t n_top = 0, n_surv = 0, n_leaf = 0, n_tests = 0, n_bsteps = 0, n_csteps = 0;
+    uint64_t n_top = 0, n_surv = 0, n_leaf = 0, n_tests = 0, n_bsteps = 0, n_csteps = 0, n_mt = 0, n_psteps = 0;
     bool done = false;
     uint32_t qlen = 0;
     __builtin_amdgcn_wave_barrier();
@@ -491,7 +500,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
                 lm &= lm - 1u;
                 s_pair[o++] = (uint16_t)((lane << 5) | b);
             }
-            s_used[lane] = 0;
+            if constexpr (!PRE) s_used[lane] = 0;
             __builtin_amdgcn_wave_barrier();
             uint32_t np_ = 0;
             for (uint32_t c = 0; c < ncand_; c += 64) {
@@ -507,39 +516,93 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
                 const uint64_t pm = __ballot(lp);
                 if (lp) {     // in place: entries c.. were read above, writes land below c + 64
                     s_pair[np_ + (uint32_t)__popcll(pm & ((1ull << lane) - 1ull))] = (uint16_t)e;
-                    s_used[e >> 5] = 1;
+                    if constexpr (!PRE) s_used[e >> 5] = 1;
                 }
                 np_ += (uint32_t)__popcll(pm);
                 if (count) { n_leaf += (uint64_t)__popcll(pm); n_bsteps++; }
             }
             __builtin_amdgcn_wave_barrier();
             if (np_ == 0) continue;
-            if (lane < n && s_used[lane]) {
-                size_t t = s_qidx[c0 + lane];
-                if (t >= P.T) t = 0;     // queue entries are < T by construction
-                s_mt[lane][0] = P.tri_mt[3 * t];
-                s_mt[lane][1] = P.tri_mt[3 * t + 1];
-                s_mt[lane][2] = P.tri_mt[3 * t + 2];
+            if constexpr (!PRE) {
+                if (lane < n && s_used[lane]) {
+                    size_t t = s_qidx[c0 + lane];
+                    if (t >= P.T) t = 0;     // queue entries are < T by construction
+                    s_mt[lane][0] = P.tri_mt[3 * t];
+                    s_mt[lane][1] = P.tri_mt[3 * t + 1];
+                    s_mt[lane][2] = P.tri_mt[3 * t + 2];
+                }
+                __builtin_amdgcn_wave_barrier();
             }
-            __builtin_amdgcn_wave_barrier();
-            for (uint32_t p0 = 0; p0 < np_; p0 += BRANCH) {
-                const uint32_t pi = p0 + lane / LEAF;
-                if (count) n_csteps++;
-                if (pi < np_) {
-                    const uint32_t e = s_pair[pi];
-                    const uint32_t sv = e >> 5;
-                    const uint32_t ray = (e & 31u) * LEAF + lane % LEAF;
-                    const float4 r0 = s_ray0[ray], r1 = s_ray1[ray];
-                    const bool live = r1.w != 0.0f && (P.kind != 1 || s_best[ray] == 0ull);
-                    if (live) {
-                        const float4 A = s_mt[sv][0], B = s_mt[sv][1], Cc = s_mt[sv][2];
-                        const float th = ray_triangle(xyz(r0), xyz(r1), xyz(A), xyz(B), xyz(Cc));
-                        if (count) n_tests++;
-                        if (P.kind == 1) {
-                            if (th > EPS && th < r0.w) s_best[ray] = 1ull;
-                        } else if (th > EPS) {
-                            const unsigned long long rec = pack_hit(th, s_qidx[c0 + sv], P.hit_mode);
-                            if (rec < s_best[ray]) atomicMin(&s_best[ray], rec);
+            // exact test of survivor sv against ray `ray` (a valid ray), merged per ray
+            auto exact = [&](uint32_t sv, uint32_t ray, float4 r0, float4 r1) {
+                float4 A, B, Cc;
+                if constexpr (PRE) {     // few tests pass the pre-test: vectors straight from memory
+                    size_t t = s_qidx[c0 + sv];
+                    if (t >= P.T) t = 0;     // queue entries are < T by construction
+                    A = P.tri_mt[3 * t]; B = P.tri_mt[3 * t + 1]; Cc = P.tri_mt[3 * t + 2];
+                } else {
+                    A = s_mt[sv][0]; B = s_mt[sv][1]; Cc = s_mt[sv][2];
+                }
+                const float th = ray_triangle(xyz(r0), xyz(r1), xyz(A), xyz(B), xyz(Cc));
+                if (count) n_mt++;
+                if (P.kind == 1) {
+                    if (th > EPS && th < r0.w) s_best[ray] = 1ull;
+                } else if (th > EPS) {
+                    const unsigned long long rec = pack_hit(th, s_qidx[c0 + sv], P.hit_mode);
+                    if (rec < s_best[ray]) atomicMin(&s_best[ray], rec);
+                }
+            };
+            if constexpr (PRE) {
+                // C1: each ray's line against the survivor's bounding sphere, BRANCH
+                // pairs per step; tests that may hit queue in the ring s_tq
+                // (survivor << 6 | ray), and C2 runs the exact test 64 at a time.
+                uint32_t qh = 0, qt = 0;
+                auto c2 = [&](uint32_t cnt) {
+                    if (count) n_csteps++;
+                    if (lane < cnt) {
+                        const uint32_t code = s_tq[(qh + lane) & (TQ - 1)];
+                        const uint32_t sv = code >> 6, ray = code & 63u;
+                        if (P.kind != 1 || s_best[ray] == 0ull) exact(sv, ray, s_ray0[ray], s_ray1[ray]);
+                    }
+                    qh += cnt;
+                    __builtin_amdgcn_wave_barrier();
+                };
+                for (uint32_t p0 = 0; p0 < np_; p0 += BRANCH) {
+                    const uint32_t pi = p0 + lane / LEAF;
+                    bool keep = false;
+                    uint32_t code = 0;
+                    if (count) n_psteps++;
+                    if (pi < np_) {
+                        const uint32_t e = s_pair[pi];
+                        const uint32_t sv = e >> 5;
+                        const uint32_t ray = (e & 31u) * LEAF + lane % LEAF;
+                        const float4 r0 = s_ray0[ray], r1 = s_ray1[ray];
+                        if (r1.w != 0.0f && (P.kind != 1 || s_best[ray] == 0ull)) {
+                            if (count) n_tests++;
+                            keep = ray_may_hit_sphere(xyz(r0), xyz(r1), s_qsph[c0 + sv]);
+                            code = (sv << 6) | ray;
+                        }
+                    }
+                    const uint64_t km = __ballot(keep);
+                    if (keep) s_tq[(qt + (uint32_t)__popcll(km & ((1ull << lane) - 1ull))) & (TQ - 1)] = (uint16_t)code;
+                    qt += (uint32_t)__popcll(km);
+                    __builtin_amdgcn_wave_barrier();
+                    if (qt - qh >= 64) c2(64);
+                }
+                while (qt != qh) c2(qt - qh < 64 ? qt - qh : 64);
+            } else {
+                for (uint32_t p0 = 0; p0 < np_; p0 += BRANCH) {
+                    const uint32_t pi = p0 + lane / LEAF;
+                    if (count) n_csteps++;
+                    if (pi < np_) {
+                        const uint32_t e = s_pair[pi];
+                        const uint32_t sv = e >> 5;
+                        const uint32_t ray = (e & 31u) * LEAF + lane % LEAF;
+                        const float4 r0 = s_ray0[ray], r1 = s_ray1[ray];
+                        const bool live = r1.w != 0.0f && (P.kind != 1 || s_best[ray] == 0ull);
+                        if (live) {
+                            if (count) n_tests++;
+                            exact(sv, ray, r0, r1);
                         }
                     }
                 }
@@ -665,8 +728,11 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
         else if (best != (P.hit_mode == 0 ? MISS64 : (unsigned long long)MISS32)) atomicMin(&P.closest[slot], best);
     }
     if (count) {
-        unsigned long long tsum = n_tests;
-        for (int dlt = 32; dlt > 0; dlt >>= 1) tsum += __shfl_xor(tsum, dlt, 64);
+        unsigned long long tsum = n_tests, msum = n_mt;
+        for (int dlt = 32; dlt > 0; dlt >>= 1) {
+            tsum += __shfl_xor(tsum, dlt, 64);
+            msum += __shfl_xor(msum, dlt, 64);
+        }
         if (lane == 0) {
             unsigned long long* cs = P.counters + (size_t)(item % COUNTER_SLOTS) * COUNTERS_PER_SLOT;
             atomicAdd(&cs[CNT_LEAF_PAIRS], (unsigned long long)n_leaf);
@@ -675,7 +741,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
             atomicAdd(&cs[CNT_TRI_TESTS], (unsigned long long)n_top);
             atomicAdd(&cs[CNT_CLUSTER_TESTS], (unsigned long long)n_cl);
             atomicAdd(&cs[CNT_B_STEPS], (unsigned long long)n_bsteps);
-            atomicAdd(&cs[CNT_C_STEPS], (unsigned long long)n_csteps);
+            atomicAdd(&cs[CNT_C_STEPS], (unsigned long long)(n_csteps + n_psteps));
+            atomicAdd(&cs[CNT_MT_TESTS], msum);
         }
     }
 }
@@ -840,6 +907,12 @@ void launch_item_desc(const TravParams& p, const uint32_t* nitems, uint4* desc, 
     if (ntop) k_item_desc<<<(ntop + 3) / 4, 256, 0, s>>>(p, nitems, desc);
 }
 
+template <int LEAF, bool PRE>
+hipError_t launch64(const TravParams& p, uint32_t blocks, hipStream_t s) {
+    k_traverse64<LEAF, PRE><<<blocks, 64, 0, s>>>(p);
+    return hipGetLastError();
+}
+
 hipError_t launch_traverse(const TravParams& p, hipStream_t s) {
     const uint32_t top = p.level_count[p.depth - 1];
     if (uses_items(p)) {
@@ -850,11 +923,11 @@ hipError_t launch_traverse(const TravParams& p, hipStream_t s) {
         // one wave per workgroup: a finished item releases its LDS at once
         const uint32_t blocks = p.items;
         switch (p.leaf) {
-            case 2: k_traverse64<2><<<blocks, 64, 0, s>>>(p); return hipGetLastError();
-            case 4: k_traverse64<4><<<blocks, 64, 0, s>>>(p); return hipGetLastError();
-            case 8: k_traverse64<8><<<blocks, 64, 0, s>>>(p); return hipGetLastError();
-            case 16: k_traverse64<16><<<blocks, 64, 0, s>>>(p); return hipGetLastError();
-            case 32: k_traverse64<32><<<blocks, 64, 0, s>>>(p); return hipGetLastError();
+            case 2: return p.prefilter ? launch64<2, true>(p, blocks, s) : launch64<2, false>(p, blocks, s);
+            case 4: return p.prefilter ? launch64<4, true>(p, blocks, s) : launch64<4, false>(p, blocks, s);
+            case 8: return p.prefilter ? launch64<8, true>(p, blocks, s) : launch64<8, false>(p, blocks, s);
+            case 16: return p.prefilter ? launch64<16, true>(p, blocks, s) : launch64<16, false>(p, blocks, s);
+            case 32: return p.prefilter ? launch64<32, true>(p, blocks, s) : launch64<32, false>(p, blocks, s);
             default: return hipErrorInvalidValue;
         }
     }

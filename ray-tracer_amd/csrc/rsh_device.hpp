@@ -308,6 +308,19 @@ __device__ __forceinline__ bool sphere_node_fast(float4 sphere, float4 cone, flo
     return (a + R / cosine) >= sqrtf(dd);
 }
 
+// Ray pre-test before ray_triangle: false only when the ray's line (origin o,
+// direction d, any length) passes the sphere (center c, radius r) that holds
+// the triangle at a distance beyond r by a margin of 1e-4 (|c - o|_1 + r) +
+// 1e-6.  ray_triangle's FP32 evaluation can accept a point off the triangle
+// only within ~1e-6 of those magnitudes (its rounding is relative to |o - v0|,
+// |d| and the edges), so a rejected ray would be a miss there too.  NaN passes.
+__device__ __forceinline__ bool ray_may_hit_sphere(v3 o, v3 d, float4 sph) {
+    const v3 w = xyz(sph) - o;
+    const v3 x = cross(w, d);
+    const float rr = sph.w + 1e-4f * (fabsf(w.x) + fabsf(w.y) + fabsf(w.z) + sph.w) + 1e-6f;
+    return !(dot(x, x) > rr * rr * dot(d, d));
+}
+
 // Conservative cluster pre-test for sphere_node.  With J = I - cd cd^T the
 // triangle test passes iff f(c) = |J (c - s)| - ((c - s).cd) tan <= (r + rt)/cos.
 // f is Lipschitz with constant L = sqrt(1 + (tan k)^2 + 2 tan k |1 - k^2|),

@@ -69,7 +69,7 @@ struct GBufView {
 // the host) so that the per-wave atomics do not serialise on one address.
 // Each slot holds COUNTERS_PER_SLOT values (see CounterIndex).
 constexpr uint32_t COUNTER_SLOTS = 64;
-constexpr uint32_t COUNTERS_PER_SLOT = 8;
+constexpr uint32_t COUNTERS_PER_SLOT = 16;
 enum CounterIndex : uint32_t {
     CNT_TOP_PAIRS = 0,     // (top node, triangle) candidate pairs of the reference's culling
     CNT_LEAF_PAIRS = 1,    // (leaf, triangle) pairs passing the leaf test
@@ -78,7 +78,8 @@ enum CounterIndex : uint32_t {
     CNT_TRI_TESTS = 4,     // top-level triangle-sphere tests actually run
     CNT_CLUSTER_TESTS = 5, // cluster pre-tests run
     CNT_C_STEPS = 6,       // 64-ray kernel: wave steps of the ray-test stage (lane use = tests / 64 steps)
-    CNT_B_STEPS = 7        // 64-ray kernel: wave steps of the leaf-test stage
+    CNT_B_STEPS = 7,       // 64-ray kernel: wave steps of the leaf-test stage
+    CNT_MT_TESTS = 8       // 64-ray kernel: Moller-Trumbore tests run after the ray pre-test
 };
 // Triangles per cluster of the traversal's cluster level.
 constexpr uint32_t CLUSTER = 64;
@@ -122,6 +123,7 @@ struct TravParams {
     const float4* cl_sphere;      // [clusters] conservative sphere over the member triangle spheres
     const uint32_t* obj_cl_base;  // [K] first cluster of each object (RAH: also the list's obj_cum)
     uint32_t n_clusters;
+    uint32_t prefilter;           // 64-ray kernel: ray-line / triangle-sphere test before the exact one
 };
 
 // Work items of one launch of the 64-ray kernel: 64 lanes each, so the grid

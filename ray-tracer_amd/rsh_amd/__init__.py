@@ -47,7 +47,8 @@ class FrameConfigDesc(C.Structure):
                 ("bounce_shadows", u32), ("primary_rays", u32), ("cell_bits", u32), ("dir_bits", u32),
                 ("key_layout", u32), ("scene_min", f32 * 3), ("cell_scale", f32 * 3),
                 ("segment_shift", u32), ("initial_radius", f32), ("initial_spread", f32),
-                ("traversal_kernel", u32), ("traversal_splits", u32), ("traversal_chunk", u32)]
+                ("traversal_kernel", u32), ("traversal_splits", u32), ("traversal_chunk", u32),
+                ("traversal_prefilter", u32)]
 
 
 class FrameStats(C.Structure):
@@ -59,7 +60,8 @@ class FrameStats(C.Structure):
                 ("pass_splits", u32 * MAX_PASSES), ("pass_items", u32 * MAX_PASSES),
                 ("pass_chunk", u32 * MAX_PASSES), ("pass_tri_tests", u64 * MAX_PASSES),
                 ("pass_cluster_tests", u64 * MAX_PASSES), ("pass_leaf_steps", u64 * MAX_PASSES),
-                ("pass_ray_steps", u64 * MAX_PASSES), ("graph_replayed", u32)]
+                ("pass_ray_steps", u64 * MAX_PASSES), ("graph_replayed", u32),
+                ("pass_mt_tests", u64 * MAX_PASSES)]
 
 
 class Float3(C.Structure):
@@ -392,7 +394,7 @@ class Engine:
                      traverse_ms=st.pass_traverse_ms[i], splits=st.pass_splits[i], items=st.pass_items[i],
                      chunk=st.pass_chunk[i], tri_tests=st.pass_tri_tests[i],
                      cluster_tests=st.pass_cluster_tests[i], leaf_steps=st.pass_leaf_steps[i],
-                     ray_steps=st.pass_ray_steps[i])
+                     ray_steps=st.pass_ray_steps[i], mt_tests=st.pass_mt_tests[i])
                 for i in range(st.pass_count)]
 
     def close(self):

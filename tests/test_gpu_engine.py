@@ -68,6 +68,10 @@ def test_config0_default(engine, oracle, rsh, w):
     dict(traversal_splits=4, depth=3, branch=4),
     dict(key_layout=1),
     dict(key_layout=2),                      # bit-interleaved cell / direction key
+    dict(traversal_prefilter=1),             # without the ray / triangle-sphere pre-test
+    dict(traversal_prefilter=1, bounces=2, bounce_shadows=1),
+    dict(traversal_prefilter=1, leaf=4, branch=16, bounces=2, bounce_shadows=1),
+    dict(traversal_prefilter=1, traversal_chunk=1, hit_mode=1),
     dict(triangle_batch=512),
     dict(bounces=2, bounce_shadows=1),
 ])
@@ -106,6 +110,8 @@ def test_config2_full_scene_low_res(engine, oracle, rsh, atrium):
 @pytest.mark.parametrize("variant", [
     dict(),
     dict(algorithm=1),
+    dict(traversal_prefilter=1),
+    dict(traversal_prefilter=1, bounces=2, bounce_shadows=1),
     dict(traversal_chunk=1, hit_mode=1),
     dict(depth=3, branch=4),                 # generic kernel
     dict(leaf=16, branch=4, bounces=2, bounce_shadows=1),
@@ -130,6 +136,7 @@ def test_traversal_counts_match_oracle(engine, oracle, rsh, variant):
         else:
             assert (g["pairs_leaf"], g["ray_tests"]) == (o["pairs_leaf"], o["ray_tests"]), (g, o)
         assert g["tri_tests"] <= g["pairs_top"]
+        assert g["mt_tests"] <= g["ray_tests"]
 
 
 def test_config2_counts_match_oracle(engine, oracle, rsh):

@@ -434,6 +434,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5
     __shared__ float4 s_laux[BRANCH];          // (cos, tan, lmax, rcos): node_bound
     __shared__ uint8_t s_used[PRE ? 1 : 64];   // survivor of the round appears in a pair
     __shared__ uint16_t s_tq[PRE ? TQ : 1];
+    __shared__ uint8_t s_grp[64];                 // passing clusters of a group, in order
+    __shared__ uint32_t s_gm[64];                 // leaf mask of each cluster of a group
     const uint32_t lane = lane_id();
     const uint32_t item = blockIdx.x;
     if (item >= P.items) return;
@@ -637,9 +639,34 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5
         rem -= take;
     };
     if (rem) load_entry(j, desc.z);
-    // Next block of one cluster's triangles [tb, tl] (cluster cid) that passed
-    // the cluster pre-test; clusters are pre-tested 64 at a time, one per lane.
-    auto next_block = [&](uint32_t& tb, uint32_t& tl, uint32_t& cid) -> bool {
+    // Leaf masks of the clusters in gm (bits = local cluster index in the group
+    // at g0): each passing cluster is tested against the BRANCH leaves
+    // (conservative, as the cluster pre-test), CPS clusters per wave step.
+    // Returns the clusters that may pass some leaf; their masks are in s_gm.
+    auto leaf_masks = [&](uint64_t gm) -> uint64_t {
+        constexpr uint32_t CPS = 64 / BRANCH;
+        constexpr uint32_t LMASK = BRANCH >= 32 ? 0xFFFFFFFFu : (1u << BRANCH) - 1u;
+        const uint32_t g = (uint32_t)__popcll(gm);
+        if ((gm >> lane) & 1ull) s_grp[__popcll(gm & ((1ull << lane) - 1ull))] = (uint8_t)lane;
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t leaf = lane % BRANCH;
+        const bool leaf_ok = top * BRANCH + leaf < P.level_count[0];
+        const float4 ls = s_lsph[leaf], lc = s_lcone[leaf], la = s_laux[leaf];
+        for (uint32_t j0 = 0; j0 < g; j0 += CPS) {
+            const uint32_t ci = j0 + lane / BRANCH;
+            bool lp = false;
+            if (ci < g && leaf_ok)
+                lp = cluster_may_pass(ls, lc, la.y, make_float2(la.z, la.w), P.cl_sphere[e_cbase + g0 + s_grp[ci]]);
+            const uint64_t bm = __ballot(lp);
+            if (lane < CPS && j0 + lane < g) s_gm[s_grp[j0 + lane]] = (uint32_t)(bm >> (lane * BRANCH)) & LMASK;
+        }
+        __builtin_amdgcn_wave_barrier();
+        return __ballot(((gm >> lane) & 1ull) && s_gm[lane] != 0u);
+    };
+    // Next block of one cluster's triangles [tb, tl] that passed the cluster
+    // pre-test and may pass some leaf (cm: those leaves); clusters are
+    // pre-tested 64 at a time, one per lane.
+    auto next_block = [&](uint32_t& tb, uint32_t& tl, uint32_t& cm) -> bool {
         while (gmask == 0ull) {
             if (lc_cur >= lc_end) {
                 if (rem == 0) return false;     // the item's cluster count is exact
@@ -650,15 +677,18 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5
             g0 = lc_cur;
             const uint32_t n = lc_end - lc_cur < 64 ? lc_end - lc_cur : 64;
             lc_cur += n;
-            if (e_ncl == 1) { gmask = 1ull; break; }   // the object test already stands for it
-            bool cp = false;
-            if (lane < n) cp = cluster_may_pass(tS, tC, tA.y, t_bound, P.cl_sphere[e_cbase + g0 + lane]);
-            gmask = __ballot(cp);
-            if (count) n_cl += n;
+            uint64_t gm = 1ull;                 // one cluster: the object test already stands for it
+            if (e_ncl != 1) {
+                bool cp = false;
+                if (lane < n) cp = cluster_may_pass(tS, tC, tA.y, t_bound, P.cl_sphere[e_cbase + g0 + lane]);
+                gm = __ballot(cp);
+                if (count) n_cl += n;
+            }
+            if (gm) gmask = leaf_masks(gm);
         }
         const uint32_t i = (uint32_t)__builtin_ctzll(gmask);
         gmask &= gmask - 1ull;
-        cid = e_cbase + g0 + i;
+        cm = s_gm[i];
         tb = e_first + (g0 + i) * CLUSTER;
         tl = tb + (CLUSTER - 1) < e_last ? tb + (CLUSTER - 1) : e_last;
         return true;
@@ -669,17 +699,17 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5
     constexpr int NB = 4;
     bool have = rem != 0 || lc_cur < lc_end;
     while (have && !done) {
-        uint32_t tbk[NB], cidk[NB];
+        uint32_t tbk[NB], cmk[NB];
         bool ink[NB];
         float4 tsk[NB];
 #pragma unroll
         for (int k = 0; k < NB; k++) {
-            uint32_t tb = 0, tl = 0, cid = 0;
-            have = have && next_block(tb, tl, cid);
+            uint32_t tb = 0, tl = 0, cm = 0;
+            have = have && next_block(tb, tl, cm);
             const uint32_t t = tb + lane;
             ink[k] = have && t <= tl && !(t >= e_exlo && t < e_exhi);
             tbk[k] = tb;
-            cidk[k] = cid;
+            cmk[k] = cm;
             tsk[k] = make_float4(0, 0, 0, 0);
             if (ink[k]) tsk[k] = P.tri_sphere[t];
         }
@@ -692,17 +722,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5
             const uint64_t m = __ballot(p);
             if (count) { n_top += (uint64_t)__popcll(__ballot(ink[k])); n_surv += (uint64_t)__popcll(m); }
             if (m == 0ull) continue;
-            // leaves the block's cluster may pass (conservative, as the cluster pre-test);
-            // a cluster that no leaf may pass contributes no pair
-            bool lpass = false;
-            if (lane < BRANCH) {
-                const float4 la = s_laux[lane];
-                lpass = top * BRANCH + lane < P.level_count[0] &&
-                        cluster_may_pass(s_lsph[lane], s_lcone[lane], la.y, make_float2(la.z, la.w),
-                                         P.cl_sphere[cidk[k]]);
-            }
-            const uint32_t cm = (uint32_t)__ballot(lpass);
-            if (cm == 0u) continue;
+            const uint32_t cm = cmk[k];      // leaves the block's cluster may pass (nonzero)
             const uint32_t add = (uint32_t)__popcll(m);
             if (qlen + add > QCAP) flush();   // never taken: qlen < 64 here
             if (done) break;

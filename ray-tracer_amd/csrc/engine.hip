@@ -49,7 +49,7 @@ struct rsh_engine {
     DevBuf rvalues, soff, rec0, rec1, nodes, aux;
     DevBuf closest, shadow, pixels, counters, dcounts;
     // work list of the 64-ray traversal kernel
-    DevBuf w_npass, w_cand, w_nitems, w_off, w_first, w_list, w_cum, w_items;
+    DevBuf w_npass, w_cand, w_nitems, w_off, w_first, w_list, w_cum, w_items, w_table;
     DevBuf views_d;                    // view table of a batch (more than one view)
     uint64_t scene_epoch = 0;          // counts scene loads
 
@@ -229,20 +229,40 @@ rsh_status build_items(rsh_engine* e, const rsh_frame_config& cfg, TravParams& p
         launch_item_count(cand, chunk, ntop, nitems, s);
         HIP_OK(run_exclusive_scan(nitems, u32p(e->w_first), (uint64_t)ntop + 1, e->sc, s));
     }
-    for (; !e->replaying;) {
-        launch_item_count(cand, chunk, ntop, nitems, s);
-        HIP_OK(run_exclusive_scan(nitems, u32p(e->w_first), (uint64_t)ntop + 1, e->sc, s));
+    if (!e->replaying) {
+        // One read-back sizes the pass: the list total and, from the table of
+        // item totals per chunk size, the chunk (shrunk from ITEM_CHUNK until the
+        // pass has ITEM_TARGET items) and the item total.
+        const bool tabled = chunk <= ITEM_TABLE;
+        unsigned long long ht[ITEM_TABLE] = {};
+        uint32_t* first = u32p(e->w_first);
+        if (tabled) {
+            HIP_OK(e->w_table.reserve(ITEM_TABLE * sizeof(unsigned long long)));
+            launch_item_table(cand, ntop, e->w_table.as<unsigned long long>(), s);
+            HIP_OK(hipMemcpyAsync(ht, e->w_table.p, sizeof(ht), hipMemcpyDeviceToHost, s));
+        } else {
+            launch_item_count(cand, chunk, ntop, nitems, s);
+            HIP_OK(run_exclusive_scan(nitems, first, (uint64_t)ntop + 1, e->sc, s));
+            HIP_OK(hipMemcpyAsync(tot + 1, first + ntop, 4, hipMemcpyDeviceToHost, s));
+        }
         if (p.algorithm != 1) HIP_OK(hipMemcpyAsync(tot, u32p(e->w_off) + ntop, 4, hipMemcpyDeviceToHost, s));
-        HIP_OK(hipMemcpyAsync(tot + 1, u32p(e->w_first) + ntop, 4, hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
-        if (cfg.traversal_chunk || tot[1] >= ITEM_TARGET || chunk <= ITEM_CHUNK_MIN || tot[1] == 0) break;
-        const uint64_t want = (uint64_t)chunk * tot[1] / ITEM_TARGET;
-        chunk = want < ITEM_CHUNK_MIN ? ITEM_CHUNK_MIN : (uint32_t)want;
-    }
-    if (tot[1] > MAX_TRAVERSE_ITEMS) {
-        e->err = "traversal work list of " + std::to_string(tot[1]) + " items exceeds " +
-                 std::to_string(MAX_TRAVERSE_ITEMS) + " (raise traversal_chunk)";
-        return RSH_CAPACITY;
+        if (tabled) {
+            if (!cfg.traversal_chunk) {
+                while (ht[chunk - 1] < ITEM_TARGET && chunk > ITEM_CHUNK_MIN && ht[chunk - 1] != 0) {
+                    const uint64_t want = (uint64_t)chunk * ht[chunk - 1] / ITEM_TARGET;
+                    chunk = want < ITEM_CHUNK_MIN ? ITEM_CHUNK_MIN : (uint32_t)want;
+                }
+            }
+            if (ht[chunk - 1] > MAX_TRAVERSE_ITEMS) {
+                e->err = "traversal work list of " + std::to_string(ht[chunk - 1]) + " items exceeds " +
+                         std::to_string(MAX_TRAVERSE_ITEMS) + " (raise traversal_chunk)";
+                return RSH_CAPACITY;
+            }
+            tot[1] = (uint32_t)ht[chunk - 1];
+            launch_item_count(cand, chunk, ntop, nitems, s);
+            HIP_OK(run_exclusive_scan(nitems, first, (uint64_t)ntop + 1, e->sc, s));
+        }
     }
     HIP_OK(e->w_list.reserve((size_t)tot[0] * 16 + 16));
     HIP_OK(e->w_cum.reserve((size_t)tot[0] * 4 + 4));
@@ -433,7 +453,7 @@ void rsh_engine_destroy(rsh_engine* e) {
                       &e->ckeys, &e->cvalues, &e->skeys, &e->svalues, &e->rvalues, &e->soff,
                       &e->rec0, &e->rec1, &e->nodes, &e->aux, &e->closest, &e->shadow, &e->pixels, &e->counters,
                       &e->dcounts, &e->w_npass, &e->w_cand, &e->w_nitems, &e->w_off, &e->w_first, &e->w_list,
-                      &e->w_cum, &e->w_items, &e->views_d})
+                      &e->w_cum, &e->w_items, &e->w_table, &e->views_d})
         b->release();
     if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
     if (e->graph) (void)hipGraphDestroy(e->graph);

@@ -355,6 +355,22 @@ __global__ void __launch_bounds__(256) k_item_count(const uint32_t* cand, uint32
     nitems[i] = i == ntop ? 0u : (uint32_t)(((uint64_t)cand[i] + chunk - 1) / chunk);
 }
 
+// Item totals for every chunk size 1..ITEM_TABLE: table[c - 1] = sum over top
+// nodes of ceil(cand / c).  One wave per slice of TABLE_SLICE top nodes, lane
+// c - 1 sums chunk size c in a register; one read-back of the table sizes a pass.
+constexpr uint32_t TABLE_SLICE = 256;
+__global__ void __launch_bounds__(64) k_item_table(const uint32_t* cand, uint32_t ntop, unsigned long long* table) {
+    const uint32_t c = (threadIdx.x & 63u) + 1u;
+    const uint32_t t0 = blockIdx.x * TABLE_SLICE;
+    const uint32_t t1 = t0 + TABLE_SLICE < ntop ? t0 + TABLE_SLICE : ntop;
+    unsigned long long sum = 0ull;
+    for (uint32_t t = t0; t < t1; t++) {
+        const uint32_t n = cand[t];
+        sum += (n + c - 1u) / c;
+    }
+    if (sum) atomicAdd(&table[c - 1u], sum);
+}
+
 // Work-item descriptors, one wave per top node, one lane per item:
 //   x top node, y absolute list entry holding the item's first cluster,
 //   z first cluster within that entry, w clusters of the item | (1 << 31)
@@ -916,6 +932,12 @@ void launch_cull_count(const TravParams& p, uint32_t* npass, uint32_t* cand, hip
 void launch_cull_list(const TravParams& p, uint4* obj_ent, uint32_t* obj_cum, hipStream_t s) {
     const uint32_t top = p.level_count[1];
     if (top && p.algorithm != 1) k_cull<true><<<(top + 3) / 4, 256, 0, s>>>(p, nullptr, nullptr, obj_ent, obj_cum);
+}
+
+void launch_item_table(const uint32_t* cand, uint32_t ntop, unsigned long long* table, hipStream_t s) {
+    static_assert(ITEM_TABLE == 64, "one lane per chunk size");
+    launch_fill_u64(table, 0ull, ITEM_TABLE, s);
+    if (ntop) k_item_table<<<(ntop + TABLE_SLICE - 1) / TABLE_SLICE, 64, 0, s>>>(cand, ntop, table);
 }
 
 void launch_item_count(const uint32_t* cand, uint32_t chunk, uint32_t ntop, uint32_t* nitems, hipStream_t s) {

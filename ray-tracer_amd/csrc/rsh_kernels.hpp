@@ -145,6 +145,9 @@ inline bool uses_items(const TravParams& p) {
 
 void launch_cull_count(const TravParams& p, uint32_t* npass, uint32_t* cand, hipStream_t s);
 void launch_cull_list(const TravParams& p, uint4* obj_ent, uint32_t* obj_cum, hipStream_t s);
+// Item totals per chunk size: table[c - 1] = sum of ceil(cand / c), c = 1..ITEM_TABLE.
+constexpr uint32_t ITEM_TABLE = 64;
+void launch_item_table(const uint32_t* cand, uint32_t ntop, unsigned long long* table, hipStream_t s);
 void launch_item_count(const uint32_t* cand, uint32_t chunk, uint32_t ntop, uint32_t* nitems, hipStream_t s);
 void launch_item_desc(const TravParams& p, const uint32_t* nitems, uint4* desc, hipStream_t s);
 

@@ -91,10 +91,16 @@ typedef struct rsh_frame_config {
     uint32_t traversal_chunk;       /* 64-ray kernel: candidate clusters (64 consecutive
                                        triangles of one object) per work item; 0 = sized
                                        per pass                                           */
-    uint32_t traversal_prefilter;   /* 64-ray kernel: test each ray's line against the
-                                       triangle's bounding sphere before the exact test;
-                                       0 default (on), 1 off, 2 on.  Hit records are the
-                                       same either way.                                    */
+    uint32_t traversal_prefilter;   /* 64-ray kernel: ray pre-tests before the exact test.
+                                       1 none; 2 each ray's line against the triangle's
+                                       bounding sphere; 3 that and, per passing cluster,
+                                       the rays' lines against the cluster's sphere (for
+                                       spread-out rays); 0 automatic: 2 or 3 per pass,
+                                       3 when over a quarter of the top nodes' cones are
+                                       wider than tan 0.1.  Hit records
+                                       are the same in every mode; with mode 3 and stats
+                                       counters (flag bit 1) the cluster masks are off so
+                                       that the counts stay the reference's.             */
 } rsh_frame_config;
 
 #define RSH_MAX_PASSES 16
@@ -123,6 +129,8 @@ typedef struct rsh_frame_stats {
     uint32_t graph_replayed;                     /* 1: the call replayed a captured graph (bit 3) */
     uint64_t pass_mt_tests[RSH_MAX_PASSES];      /* Moller-Trumbore tests run after the ray pre-test
                                                     (bit 1; equals pass_ray_tests without it)   */
+    uint32_t pass_mode[RSH_MAX_PASSES];          /* 64-ray kernel mode used: 0 exact tests only,
+                                                    1 triangle pre-test, 2 and cluster masks   */
 } rsh_frame_stats;
 
 typedef struct rsh_engine rsh_engine;

@@ -56,7 +56,7 @@ struct rsh_engine {
     // Graph replay (render flag bit 3).  An eager frame records every pass's
     // sizes (the values the host otherwise reads back); a second run replays
     // those sizes without read-backs or synchronisation under stream capture.
-    struct PassPlan { uint32_t N = 0, C = 0, chunk = 0, list_total = 0, items = 0; };
+    struct PassPlan { uint32_t N = 0, C = 0, chunk = 0, list_total = 0, items = 0, mode = 0; };
     std::vector<PassPlan> plan;
     bool recording = false, replaying = false;
     size_t plan_pos = 0;
@@ -208,6 +208,7 @@ struct PassOut {
     uint32_t N = 0, C = 0, splits = 0, items = 0, chunk = 0;
     uint64_t top = 0, leaf = 0, tests = 0, surv = 0, tri_tests = 0, cl_tests = 0, b_steps = 0, c_steps = 0;
     uint64_t mt_tests = 0;
+    uint32_t mode = 0;     // traversal mode (TravParams::prefilter)
 };
 
 // Work list of the 64-ray kernel (see uses_items): cull, size, scan, fill.
@@ -225,7 +226,7 @@ rsh_status build_items(rsh_engine* e, const rsh_frame_config& cfg, TravParams& p
     uint32_t tot[2] = {0, 0};   // list entries, items
     if (e->replaying) {
         const rsh_engine::PassPlan& pp = e->plan[e->plan_pos - 1];
-        chunk = pp.chunk; tot[0] = pp.list_total; tot[1] = pp.items;
+        chunk = pp.chunk; tot[0] = pp.list_total; tot[1] = pp.items; p.prefilter = pp.mode;
         launch_item_count(cand, chunk, ntop, nitems, s);
         HIP_OK(run_exclusive_scan(nitems, u32p(e->w_first), (uint64_t)ntop + 1, e->sc, s));
     }
@@ -234,19 +235,24 @@ rsh_status build_items(rsh_engine* e, const rsh_frame_config& cfg, TravParams& p
         // item totals per chunk size, the chunk (shrunk from ITEM_CHUNK until the
         // pass has ITEM_TARGET items) and the item total.
         const bool tabled = chunk <= ITEM_TABLE;
-        unsigned long long ht[ITEM_TABLE] = {};
+        unsigned long long ht[ITEM_TABLE + 1] = {};
         uint32_t* first = u32p(e->w_first);
-        if (tabled) {
-            HIP_OK(e->w_table.reserve(ITEM_TABLE * sizeof(unsigned long long)));
-            launch_item_table(cand, ntop, e->w_table.as<unsigned long long>(), s);
-            HIP_OK(hipMemcpyAsync(ht, e->w_table.p, sizeof(ht), hipMemcpyDeviceToHost, s));
-        } else {
+        HIP_OK(e->w_table.reserve((ITEM_TABLE + 1) * sizeof(unsigned long long)));
+        launch_item_table(cand, p.aux + p.level_offset[1], ntop, e->w_table.as<unsigned long long>(), s);
+        HIP_OK(hipMemcpyAsync(ht, e->w_table.p, sizeof(ht), hipMemcpyDeviceToHost, s));
+        if (!tabled) {
             launch_item_count(cand, chunk, ntop, nitems, s);
             HIP_OK(run_exclusive_scan(nitems, first, (uint64_t)ntop + 1, e->sc, s));
             HIP_OK(hipMemcpyAsync(tot + 1, first + ntop, 4, hipMemcpyDeviceToHost, s));
         }
         if (p.algorithm != 1) HIP_OK(hipMemcpyAsync(tot, u32p(e->w_off) + ntop, 4, hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
+        // Traversal mode: the cluster ray masks pay when many top nodes are wide
+        // (spread-out secondary rays) and cost where rays are coherent.  More than
+        // a quarter of the nodes with tan(spread) > WIDE_TAN picks the best mode
+        // for every pass of the 1080p atrium frame (primary and first shadow pass
+        // without, every bounce with).
+        if (cfg.traversal_prefilter == 0) p.prefilter = 4 * ht[ITEM_TABLE] > ntop ? 2u : 1u;
         if (tabled) {
             if (!cfg.traversal_chunk) {
                 while (ht[chunk - 1] < ITEM_TARGET && chunk > ITEM_CHUNK_MIN && ht[chunk - 1] != 0) {
@@ -274,10 +280,10 @@ rsh_status build_items(rsh_engine* e, const rsh_frame_config& cfg, TravParams& p
     launch_cull_list(p, e->w_list.as<uint4>(), u32p(e->w_cum), s);     // writes at p.obj_off
     launch_item_desc(p, nitems, e->w_items.as<uint4>(), s);
     HIP_OK(hipGetLastError());
-    out.items = tot[1]; out.chunk = chunk;
+    out.items = tot[1]; out.chunk = chunk; out.mode = p.prefilter;
     if (e->recording) {
         rsh_engine::PassPlan& pp = e->plan.back();
-        pp.chunk = chunk; pp.list_total = tot[0]; pp.items = tot[1];
+        pp.chunk = chunk; pp.list_total = tot[0]; pp.items = tot[1]; pp.mode = p.prefilter;
     }
     return RSH_OK;
 }
@@ -359,9 +365,10 @@ rsh_status trace_pass(rsh_engine* e, const rsh_frame_config& cfg, const float* r
     p.closest = u64p(e->closest); p.shadow = u32p(e->shadow);
     p.counters = counting ? u64p(e->counters) : nullptr;
     p.force_generic = cfg.traversal_kernel == 1 ? 1u : 0u;
-    // The ray / triangle-sphere pre-test (default on): measured faster on every pass
-    // of the 1080p atrium frame, primary included.
-    p.prefilter = cfg.traversal_prefilter == 1 ? 0u : 1u;
+    // Traversal mode: 0 exact tests only, 1 ray / triangle-sphere pre-test, 2 that
+    // and ray / cluster-sphere masks.  Automatic (0): 1 or 2, picked per pass from
+    // the spread of its top nodes when the work list is sized.
+    p.prefilter = cfg.traversal_prefilter == 1 ? 0u : cfg.traversal_prefilter == 3 ? 2u : 1u;
     p.splits = 1;
     if (counting) launch_fill_u64(u64p(e->counters), 0ull, COUNTER_SLOTS * COUNTERS_PER_SLOT, s);
     if (uses_items(p)) {
@@ -409,6 +416,7 @@ rsh_status record(rsh_engine* e, rsh_frame_stats* st, uint32_t kind, uint32_t S,
         st->pass_top_survivors[i] = po.surv;
         st->pass_tri_tests[i] = po.tri_tests; st->pass_cluster_tests[i] = po.cl_tests;
         st->pass_leaf_steps[i] = po.b_steps; st->pass_ray_steps[i] = po.c_steps; st->pass_mt_tests[i] = po.mt_tests;
+        st->pass_mode[i] = po.mode;
         st->rays_total += po.N;
     }
     e->pass_index++;
@@ -867,6 +875,7 @@ rsh_status rsh_engine_trace(rsh_engine* e, const float* rays, const float* tmax,
         st->pass_top_survivors[i] = po.surv; st->pass_items[i] = po.items; st->pass_chunk[i] = po.chunk;
         st->pass_splits[i] = po.splits; st->pass_tri_tests[i] = po.tri_tests; st->pass_cluster_tests[i] = po.cl_tests;
         st->pass_leaf_steps[i] = po.b_steps; st->pass_ray_steps[i] = po.c_steps; st->pass_mt_tests[i] = po.mt_tests;
+        st->pass_mode[i] = po.mode;
         st->rays_total = po.N;
     }
     HIP_OK(hipMemcpyAsync(out, kind == 1 ? e->shadow.p : e->closest.p, (size_t)n * (kind == 1 ? 4 : 8),

@@ -356,11 +356,14 @@ __global__ void __launch_bounds__(256) k_item_count(const uint32_t* cand, uint32
 }
 
 // Item totals for every chunk size 1..ITEM_TABLE: table[c - 1] = sum over top
-// nodes of ceil(cand / c).  One wave per slice of TABLE_SLICE top nodes, lane
-// c - 1 sums chunk size c in a register; one read-back of the table sizes a pass.
+// nodes of ceil(cand / c); table[ITEM_TABLE] = top nodes whose cone is wide
+// (tangent of the spread above WIDE_TAN, or not finite).  One wave per slice
+// of TABLE_SLICE top nodes, lane c - 1 sums chunk size c in a register; one
+// read-back of the table sizes a pass and picks its traversal mode.
 constexpr uint32_t TABLE_SLICE = 256;
-__global__ void __launch_bounds__(64) k_item_table(const uint32_t* cand, uint32_t ntop, unsigned long long* table) {
-    const uint32_t c = (threadIdx.x & 63u) + 1u;
+__global__ void __launch_bounds__(64) k_item_table(const uint32_t* cand, const float2* top_aux, uint32_t ntop,
+                                                   unsigned long long* table) {
+    const uint32_t lane = threadIdx.x & 63u, c = lane + 1u;
     const uint32_t t0 = blockIdx.x * TABLE_SLICE;
     const uint32_t t1 = t0 + TABLE_SLICE < ntop ? t0 + TABLE_SLICE : ntop;
     unsigned long long sum = 0ull;
@@ -369,6 +372,10 @@ __global__ void __launch_bounds__(64) k_item_table(const uint32_t* cand, uint32_
         sum += (n + c - 1u) / c;
     }
     if (sum) atomicAdd(&table[c - 1u], sum);
+    uint32_t wide = 0;
+    for (uint32_t t = t0 + lane; t < t1; t += 64u) wide += !(top_aux[t].y <= WIDE_TAN) ? 1u : 0u;
+    for (int d = 32; d > 0; d >>= 1) wide += __shfl_xor(wide, d, 64);
+    if (lane == 0 && wide) atomicAdd(&table[ITEM_TABLE], (unsigned long long)wide);
 }
 
 // Work-item descriptors, one wave per top node, one lane per item:
@@ -413,27 +420,34 @@ __global__ void __launch_bounds__(256) k_item_desc(TravParams P, const uint32_t*
 //     candidates allowed by the masks are listed and tested 64 per step;
 //     passing pairs are compacted in place.
 //  C  The pair list is walked BRANCH pairs per step, lane = pair p0 + lane /
-//     LEAF, ray lane % LEAF of the pair's leaf.  With the pre-test (PRE, the
-//     default) each ray's line is first tested against the survivor's bounding
-//     sphere (ray_may_hit_sphere); the (survivor, ray) tests that may hit go to
-//     an LDS ring and run the Moller-Trumbore test 64 at a time, the vectors
-//     loaded from memory per test.  Without it, the vectors of the survivors in
-//     a pair are staged in LDS and every (pair, ray) runs the exact test.
-//     Closest hits merge per ray in LDS (atomicMin), shadow rays set a flag.
+//     LEAF, ray lane % LEAF of the pair's leaf (MODE 2: one pair per lane,
+//     its candidate rays listed one per round).  With the pre-test (MODE >= 1)
+//     each ray's line is first tested against the survivor's bounding sphere
+//     (ray_may_hit_sphere); the (survivor, ray) tests that may hit go to an LDS
+//     ring and run the Moller-Trumbore test 64 at a time, the vectors loaded
+//     from memory per test.  Without it, the vectors of the survivors in a pair
+//     are staged in LDS and every (pair, ray) runs the exact test.  Closest
+//     hits merge per ray in LDS (atomicMin), shadow rays set a flag.
 // The set of (leaf, triangle) pairs and ray tests is the reference's; only
 // the order differs, and any-hit / closest-hit are order independent.
 // One wave per workgroup: items differ widely in length, and a workgroup's LDS
 // is held until its last wave ends, so a finished item must not wait for
 // others (measured: 2x faster than four-wave workgroups on the 1080p atrium).
-// With the pre-test the kernel needs 7.3 KB of LDS and under 102 VGPRs per
-// wave, so five waves per SIMD (twenty per CU) fit; without it, four.  The
-// pre-test rejects most tests of spread-out secondary rays (99% on the atrium's
-// second bounce) and about 60% of primary ones, and measured faster on every
-// pass of the 1080p atrium frame.
-template <int LEAF, bool PRE>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5 : 4))) k_traverse64(TravParams P) {
+// MODE 0 runs the exact test on every (pair, ray); MODE 1 first tests each
+// ray's line against the triangle's bounding sphere (rejects 99% of the tests
+// of the atrium's second bounce and about 60% of primary ones); MODE 2 also
+// computes, per passing cluster, which of the 64 rays may meet the cluster's
+// sphere and lists only those (pair, ray) candidates.  MODE 2 pays for
+// spread-out secondary rays (2x on the atrium's bounces) and costs on coherent
+// ones; the engine picks the mode per pass.  Modes 1 and 2 use under 102 VGPRs
+// and 7.6 / 8.9 KB of LDS per wave (five waves per SIMD); mode 0, four.
+template <int LEAF, int MODE>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MODE >= 1 ? 5 : 4))) k_traverse64(TravParams P) {
+    constexpr bool PRE = MODE >= 1;               // ray / triangle-sphere pre-test
+    constexpr bool RM = MODE == 2;                // and ray / cluster-sphere masks
     constexpr uint32_t BRANCH = 64 / LEAF;
-    constexpr uint32_t TQ = 128;                  // ring of pre-tested (survivor, ray) tests
+    constexpr uint32_t TQ = 128;                  // rings of (survivor, ray) tests
+    constexpr unsigned long long LRAYS = LEAF >= 64 ? ~0ull : (1ull << LEAF) - 1ull;   // one leaf's rays
     constexpr uint32_t QCAP = 128;                // survivor queue
     constexpr uint32_t PCAP = 512;                // (survivor, leaf) candidate / pair list
     constexpr uint32_t SCH = PCAP / BRANCH < 64 ? PCAP / BRANCH : 64;   // survivors per B+C round
@@ -449,7 +463,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5
     __shared__ float4 s_lcone[BRANCH];
     __shared__ float4 s_laux[BRANCH];          // (cos, tan, lmax, rcos): node_bound
     __shared__ uint8_t s_used[PRE ? 1 : 64];   // survivor of the round appears in a pair
-    __shared__ uint16_t s_tq[PRE ? TQ : 1];
+    __shared__ uint16_t s_tq[PRE ? TQ : 1];       // (survivor, ray) tests
+    __shared__ uint16_t s_tq2[RM ? TQ : 1];       // RM: candidates that passed the ray pre-test
+    __shared__ unsigned long long s_qrm[RM ? QCAP : 1];   // RM: rays that may meet the survivor's cluster
     __shared__ uint8_t s_grp[64];                 // passing clusters of a group, in order
     __shared__ uint32_t s_gm[64];                 // leaf mask of each cluster of a group
     const uint32_t lane = lane_id();
@@ -570,7 +586,71 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5
                     if (rec < s_best[ray]) atomicMin(&s_best[ray], rec);
                 }
             };
-            if constexpr (PRE) {
+            if constexpr (RM) {
+                // C0: the (pair, ray) candidates -- rays of the pair's leaf that are
+                // live and may meet the survivor's cluster -- are listed in the ring
+                // s_tq, one pair per lane and one ray per lane and round;
+                // C1: each candidate ray's line against the survivor's bounding
+                // sphere, 64 per step, passing ones to the ring s_tq2;
+                // C2: the exact test, 64 per step.
+                const uint64_t lt = (1ull << lane) - 1ull;
+                uint32_t qh = 0, qt = 0, rh = 0, rt = 0;
+                auto c2 = [&](uint32_t cnt) {
+                    if (count) n_csteps++;
+                    if (lane < cnt) {
+                        const uint32_t code = s_tq2[(rh + lane) & (TQ - 1)];
+                        const uint32_t sv = code >> 6, ray = code & 63u;
+                        if (P.kind != 1 || s_best[ray] == 0ull) exact(sv, ray, s_ray0[ray], s_ray1[ray]);
+                    }
+                    rh += cnt;
+                    __builtin_amdgcn_wave_barrier();
+                };
+                auto c1 = [&](uint32_t cnt) {
+                    if (count) n_psteps++;
+                    bool keep = false;
+                    uint32_t code = 0;
+                    if (lane < cnt) {
+                        code = s_tq[(qh + lane) & (TQ - 1)];
+                        const uint32_t sv = code >> 6, ray = code & 63u;
+                        if (P.kind != 1 || s_best[ray] == 0ull)
+                            keep = ray_may_hit_sphere(xyz(s_ray0[ray]), xyz(s_ray1[ray]), s_qsph[c0 + sv]);
+                    }
+                    const uint64_t km = __ballot(keep);
+                    if (keep) s_tq2[(rt + (uint32_t)__popcll(km & lt)) & (TQ - 1)] = (uint16_t)code;
+                    rt += (uint32_t)__popcll(km);
+                    qh += cnt;
+                    __builtin_amdgcn_wave_barrier();
+                    if (rt - rh >= 64) c2(64);
+                };
+                uint64_t live = alive;
+                if (P.kind == 1) live &= ~__ballot(s_best[lane] != 0ull);
+                for (uint32_t p0 = 0; p0 < np_; p0 += 64) {
+                    const uint32_t pi = p0 + lane;
+                    uint32_t rb = 0, sv = 0, b = 0;
+                    if (pi < np_) {
+                        const uint32_t e = s_pair[pi];
+                        sv = e >> 5;
+                        b = e & 31u;
+                        const uint32_t lv = (uint32_t)((live >> (b * LEAF)) & LRAYS);
+                        if (count) n_tests += (uint64_t)__popc(lv);
+                        rb = (uint32_t)(s_qrm[c0 + sv] >> (b * LEAF)) & lv;
+                    }
+                    for (;;) {
+                        const uint64_t hm = __ballot(rb != 0u);
+                        if (hm == 0ull) break;
+                        if (rb) {
+                            const uint32_t r = (uint32_t)__builtin_ctz(rb);
+                            rb &= rb - 1u;
+                            s_tq[(qt + (uint32_t)__popcll(hm & lt)) & (TQ - 1)] = (uint16_t)((sv << 6) | (b * LEAF + r));
+                        }
+                        qt += (uint32_t)__popcll(hm);
+                        __builtin_amdgcn_wave_barrier();
+                        if (qt - qh >= 64) c1(64);
+                    }
+                }
+                while (qt != qh) c1(qt - qh < 64 ? qt - qh : 64);
+                while (rt != rh) c2(rt - rh < 64 ? rt - rh : 64);
+            } else if constexpr (PRE) {
                 // C1: each ray's line against the survivor's bounding sphere, BRANCH
                 // pairs per step; tests that may hit queue in the ring s_tq
                 // (survivor << 6 | ray), and C2 runs the exact test 64 at a time.
@@ -682,7 +762,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5
     // Next block of one cluster's triangles [tb, tl] that passed the cluster
     // pre-test and may pass some leaf (cm: those leaves); clusters are
     // pre-tested 64 at a time, one per lane.
-    auto next_block = [&](uint32_t& tb, uint32_t& tl, uint32_t& cm) -> bool {
+    auto next_block = [&](uint32_t& tb, uint32_t& tl, uint32_t& cm, uint32_t& cid) -> bool {
         while (gmask == 0ull) {
             if (lc_cur >= lc_end) {
                 if (rem == 0) return false;     // the item's cluster count is exact
@@ -705,6 +785,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5
         const uint32_t i = (uint32_t)__builtin_ctzll(gmask);
         gmask &= gmask - 1ull;
         cm = s_gm[i];
+        cid = e_cbase + g0 + i;
         tb = e_first + (g0 + i) * CLUSTER;
         tl = tb + (CLUSTER - 1) < e_last ? tb + (CLUSTER - 1) : e_last;
         return true;
@@ -715,17 +796,18 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5
     constexpr int NB = 4;
     bool have = rem != 0 || lc_cur < lc_end;
     while (have && !done) {
-        uint32_t tbk[NB], cmk[NB];
+        uint32_t tbk[NB], cmk[NB], cidk[NB];
         bool ink[NB];
         float4 tsk[NB];
 #pragma unroll
         for (int k = 0; k < NB; k++) {
-            uint32_t tb = 0, tl = 0, cm = 0;
-            have = have && next_block(tb, tl, cm);
+            uint32_t tb = 0, tl = 0, cm = 0, cid = 0;
+            have = have && next_block(tb, tl, cm, cid);
             const uint32_t t = tb + lane;
             ink[k] = have && t <= tl && !(t >= e_exlo && t < e_exhi);
             tbk[k] = tb;
             cmk[k] = cm;
+            cidk[k] = cid;
             tsk[k] = make_float4(0, 0, 0, 0);
             if (ink[k]) tsk[k] = P.tri_sphere[t];
         }
@@ -738,7 +820,25 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5
             const uint64_t m = __ballot(p);
             if (count) { n_top += (uint64_t)__popcll(__ballot(ink[k])); n_surv += (uint64_t)__popcll(m); }
             if (m == 0ull) continue;
-            const uint32_t cm = cmk[k];      // leaves the block's cluster may pass (nonzero)
+            uint32_t cm = cmk[k];            // leaves the block's cluster may pass (nonzero)
+            uint64_t rm = ~0ull;
+            if constexpr (RM) {
+                // Rays whose line may meet the cluster's sphere (ray_may_hit_sphere
+                // holds for the cluster as for each of its triangles); leaves with
+                // none of them are dropped.  Counting runs keep every ray and leaf,
+                // so the counts stay the reference's.
+                if (!count) {
+                    const float4 cs = P.cl_sphere[cidk[k]];
+                    rm = __ballot(((alive >> lane) & 1ull) &&
+                                  ray_may_hit_sphere(xyz(s_ray0[lane]), xyz(s_ray1[lane]), cs));
+                    uint32_t lr = 0;
+#pragma unroll
+                    for (uint32_t b = 0; b < BRANCH; b++)
+                        lr |= (((rm >> (b * LEAF)) & LRAYS) != 0ull ? 1u : 0u) << b;
+                    cm &= lr;
+                    if (cm == 0u) continue;
+                }
+            }
             const uint32_t add = (uint32_t)__popcll(m);
             if (qlen + add > QCAP) flush();   // never taken: qlen < 64 here
             if (done) break;
@@ -747,6 +847,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PRE ? 5
                 s_qsph[r] = tsk[k];
                 s_qidx[r] = tbk[k] + lane;
                 s_qmask[r] = cm;
+                if constexpr (RM) s_qrm[r] = rm;
             }
             qlen += add;
             __builtin_amdgcn_wave_barrier();
@@ -934,10 +1035,11 @@ void launch_cull_list(const TravParams& p, uint4* obj_ent, uint32_t* obj_cum, hi
     if (top && p.algorithm != 1) k_cull<true><<<(top + 3) / 4, 256, 0, s>>>(p, nullptr, nullptr, obj_ent, obj_cum);
 }
 
-void launch_item_table(const uint32_t* cand, uint32_t ntop, unsigned long long* table, hipStream_t s) {
+void launch_item_table(const uint32_t* cand, const float2* top_aux, uint32_t ntop, unsigned long long* table,
+                       hipStream_t s) {
     static_assert(ITEM_TABLE == 64, "one lane per chunk size");
-    launch_fill_u64(table, 0ull, ITEM_TABLE, s);
-    if (ntop) k_item_table<<<(ntop + TABLE_SLICE - 1) / TABLE_SLICE, 64, 0, s>>>(cand, ntop, table);
+    launch_fill_u64(table, 0ull, ITEM_TABLE + 1, s);
+    if (ntop) k_item_table<<<(ntop + TABLE_SLICE - 1) / TABLE_SLICE, 64, 0, s>>>(cand, top_aux, ntop, table);
 }
 
 void launch_item_count(const uint32_t* cand, uint32_t chunk, uint32_t ntop, uint32_t* nitems, hipStream_t s) {
@@ -949,9 +1051,14 @@ void launch_item_desc(const TravParams& p, const uint32_t* nitems, uint4* desc, 
     if (ntop) k_item_desc<<<(ntop + 3) / 4, 256, 0, s>>>(p, nitems, desc);
 }
 
-template <int LEAF, bool PRE>
+template <int LEAF>
 hipError_t launch64(const TravParams& p, uint32_t blocks, hipStream_t s) {
-    k_traverse64<LEAF, PRE><<<blocks, 64, 0, s>>>(p);
+    switch (p.prefilter) {
+        case 0: k_traverse64<LEAF, 0><<<blocks, 64, 0, s>>>(p); break;
+        case 1: k_traverse64<LEAF, 1><<<blocks, 64, 0, s>>>(p); break;
+        case 2: k_traverse64<LEAF, 2><<<blocks, 64, 0, s>>>(p); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
@@ -965,11 +1072,11 @@ hipError_t launch_traverse(const TravParams& p, hipStream_t s) {
         // one wave per workgroup: a finished item releases its LDS at once
         const uint32_t blocks = p.items;
         switch (p.leaf) {
-            case 2: return p.prefilter ? launch64<2, true>(p, blocks, s) : launch64<2, false>(p, blocks, s);
-            case 4: return p.prefilter ? launch64<4, true>(p, blocks, s) : launch64<4, false>(p, blocks, s);
-            case 8: return p.prefilter ? launch64<8, true>(p, blocks, s) : launch64<8, false>(p, blocks, s);
-            case 16: return p.prefilter ? launch64<16, true>(p, blocks, s) : launch64<16, false>(p, blocks, s);
-            case 32: return p.prefilter ? launch64<32, true>(p, blocks, s) : launch64<32, false>(p, blocks, s);
+            case 2: return launch64<2>(p, blocks, s);
+            case 4: return launch64<4>(p, blocks, s);
+            case 8: return launch64<8>(p, blocks, s);
+            case 16: return launch64<16>(p, blocks, s);
+            case 32: return launch64<32>(p, blocks, s);
             default: return hipErrorInvalidValue;
         }
     }

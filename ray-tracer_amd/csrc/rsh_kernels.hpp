@@ -123,7 +123,8 @@ struct TravParams {
     const float4* cl_sphere;      // [clusters] conservative sphere over the member triangle spheres
     const uint32_t* obj_cl_base;  // [K] first cluster of each object (RAH: also the list's obj_cum)
     uint32_t n_clusters;
-    uint32_t prefilter;           // 64-ray kernel: ray-line / triangle-sphere test before the exact one
+    uint32_t prefilter;           // 64-ray kernel: 0 exact tests only, 1 ray-line / triangle-sphere
+                                  // pre-test, 2 that and ray / cluster-sphere masks
 };
 
 // Work items of one launch of the 64-ray kernel: 64 lanes each, so the grid
@@ -145,9 +146,12 @@ inline bool uses_items(const TravParams& p) {
 
 void launch_cull_count(const TravParams& p, uint32_t* npass, uint32_t* cand, hipStream_t s);
 void launch_cull_list(const TravParams& p, uint4* obj_ent, uint32_t* obj_cum, hipStream_t s);
-// Item totals per chunk size: table[c - 1] = sum of ceil(cand / c), c = 1..ITEM_TABLE.
+// Item totals per chunk size: table[c - 1] = sum of ceil(cand / c), c = 1..ITEM_TABLE;
+// table[ITEM_TABLE] = top nodes with a wide cone (spread tangent > WIDE_TAN).
 constexpr uint32_t ITEM_TABLE = 64;
-void launch_item_table(const uint32_t* cand, uint32_t ntop, unsigned long long* table, hipStream_t s);
+constexpr float WIDE_TAN = 0.1f;
+void launch_item_table(const uint32_t* cand, const float2* top_aux, uint32_t ntop, unsigned long long* table,
+                       hipStream_t s);
 void launch_item_count(const uint32_t* cand, uint32_t chunk, uint32_t ntop, uint32_t* nitems, hipStream_t s);
 void launch_item_desc(const TravParams& p, const uint32_t* nitems, uint4* desc, hipStream_t s);
 

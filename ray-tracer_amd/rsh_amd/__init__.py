@@ -61,7 +61,7 @@ class FrameStats(C.Structure):
                 ("pass_chunk", u32 * MAX_PASSES), ("pass_tri_tests", u64 * MAX_PASSES),
                 ("pass_cluster_tests", u64 * MAX_PASSES), ("pass_leaf_steps", u64 * MAX_PASSES),
                 ("pass_ray_steps", u64 * MAX_PASSES), ("graph_replayed", u32),
-                ("pass_mt_tests", u64 * MAX_PASSES)]
+                ("pass_mt_tests", u64 * MAX_PASSES), ("pass_mode", u32 * MAX_PASSES)]
 
 
 class Float3(C.Structure):
@@ -394,7 +394,7 @@ class Engine:
                      traverse_ms=st.pass_traverse_ms[i], splits=st.pass_splits[i], items=st.pass_items[i],
                      chunk=st.pass_chunk[i], tri_tests=st.pass_tri_tests[i],
                      cluster_tests=st.pass_cluster_tests[i], leaf_steps=st.pass_leaf_steps[i],
-                     ray_steps=st.pass_ray_steps[i], mt_tests=st.pass_mt_tests[i])
+                     ray_steps=st.pass_ray_steps[i], mt_tests=st.pass_mt_tests[i], mode=st.pass_mode[i])
                 for i in range(st.pass_count)]
 
     def close(self):

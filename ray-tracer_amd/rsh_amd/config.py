@@ -49,7 +49,7 @@ class FrameConfig:
     traversal_kernel: int = 0       # engine-only: 0 automatic, 1 generic kernel
     traversal_splits: int = 0       # engine-only, generic kernel: waves per top node, 0 = fill the GPU
     traversal_chunk: int = 0        # engine-only, 64-ray kernel: candidate clusters per work item, 0 = per pass
-    traversal_prefilter: int = 0    # engine-only, 64-ray kernel: ray / triangle-sphere pre-test, 0 default (on), 1 off, 2 on
+    traversal_prefilter: int = 0    # engine-only, 64-ray kernel: 0 auto, 1 no pre-test, 2 triangle, 3 + cluster masks
 
     def with_scene_bounds(self, scene: Scene) -> "FrameConfig":
         """Cell grid over the scene AABB: 2^cell_bits cells per axis."""

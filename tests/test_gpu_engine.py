@@ -68,10 +68,16 @@ def test_config0_default(engine, oracle, rsh, w):
     dict(traversal_splits=4, depth=3, branch=4),
     dict(key_layout=1),
     dict(key_layout=2),                      # bit-interleaved cell / direction key
-    dict(traversal_prefilter=1),             # without the ray / triangle-sphere pre-test
+    dict(traversal_prefilter=1),             # without the ray pre-tests
     dict(traversal_prefilter=1, bounces=2, bounce_shadows=1),
     dict(traversal_prefilter=1, leaf=4, branch=16, bounces=2, bounce_shadows=1),
     dict(traversal_prefilter=1, traversal_chunk=1, hit_mode=1),
+    dict(traversal_prefilter=2, bounces=2, bounce_shadows=1),   # triangle pre-test only
+    dict(traversal_prefilter=3),             # and cluster ray masks on every pass
+    dict(traversal_prefilter=3, bounces=2, bounce_shadows=1),
+    dict(traversal_prefilter=3, leaf=4, branch=16, bounces=2, bounce_shadows=1),
+    dict(traversal_prefilter=3, leaf=32, branch=2),
+    dict(traversal_prefilter=3, traversal_chunk=1, hit_mode=1),
     dict(triangle_batch=512),
     dict(bounces=2, bounce_shadows=1),
 ])
@@ -112,6 +118,7 @@ def test_config2_full_scene_low_res(engine, oracle, rsh, atrium):
     dict(algorithm=1),
     dict(traversal_prefilter=1),
     dict(traversal_prefilter=1, bounces=2, bounce_shadows=1),
+    dict(traversal_prefilter=3, bounces=2, bounce_shadows=1),   # counting keeps every ray
     dict(traversal_chunk=1, hit_mode=1),
     dict(depth=3, branch=4),                 # generic kernel
     dict(leaf=16, branch=4, bounces=2, bounce_shadows=1),

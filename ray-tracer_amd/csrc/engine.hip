@@ -49,7 +49,7 @@ struct rsh_engine {
     DevBuf rvalues, soff, rec0, rec1, nodes, aux;
     DevBuf closest, shadow, pixels, counters, dcounts;
     // work list of the 64-ray traversal kernel
-    DevBuf w_npass, w_cand, w_nitems, w_off, w_first, w_list, w_cum, w_items, w_table;
+    DevBuf w_npass, w_cand, w_nitems, w_off, w_first, w_list, w_cum, w_items, w_table, w_pmask;
     DevBuf views_d;                    // view table of a batch (more than one view)
     uint64_t scene_epoch = 0;          // counts scene loads
 
@@ -220,15 +220,16 @@ rsh_status build_items(rsh_engine* e, const rsh_frame_config& cfg, TravParams& p
     uint32_t* npass = u32p(e->w_npass);
     uint32_t* cand = u32p(e->w_cand);
     uint32_t* nitems = u32p(e->w_nitems);
-    launch_cull_count(p, npass, cand, s);
+    HIP_OK(e->w_pmask.reserve(((size_t)ntop + 1) * ((e->K + 63) / 64) * 8 + 8));
+    unsigned long long* pmask = e->w_pmask.as<unsigned long long>();
+    launch_cull_count(p, npass, cand, pmask, s);
     if (p.algorithm != 1) HIP_OK(run_exclusive_scan(npass, u32p(e->w_off), (uint64_t)ntop + 1, e->sc, s));
     uint32_t chunk = cfg.traversal_chunk ? cfg.traversal_chunk : ITEM_CHUNK;
     uint32_t tot[2] = {0, 0};   // list entries, items
     if (e->replaying) {
         const rsh_engine::PassPlan& pp = e->plan[e->plan_pos - 1];
         chunk = pp.chunk; tot[0] = pp.list_total; tot[1] = pp.items; p.prefilter = pp.mode;
-        launch_item_count(cand, chunk, ntop, nitems, s);
-        HIP_OK(run_exclusive_scan(nitems, u32p(e->w_first), (uint64_t)ntop + 1, e->sc, s));
+        HIP_OK(run_item_scan(cand, ntop, chunk, nitems, u32p(e->w_first), e->sc, s));
     }
     if (!e->replaying) {
         // One read-back sizes the pass: the list total and, from the table of
@@ -241,8 +242,7 @@ rsh_status build_items(rsh_engine* e, const rsh_frame_config& cfg, TravParams& p
         launch_item_table(cand, p.aux + p.level_offset[1], ntop, e->w_table.as<unsigned long long>(), s);
         HIP_OK(hipMemcpyAsync(ht, e->w_table.p, sizeof(ht), hipMemcpyDeviceToHost, s));
         if (!tabled) {
-            launch_item_count(cand, chunk, ntop, nitems, s);
-            HIP_OK(run_exclusive_scan(nitems, first, (uint64_t)ntop + 1, e->sc, s));
+            HIP_OK(run_item_scan(cand, ntop, chunk, nitems, first, e->sc, s));
             HIP_OK(hipMemcpyAsync(tot + 1, first + ntop, 4, hipMemcpyDeviceToHost, s));
         }
         if (p.algorithm != 1) HIP_OK(hipMemcpyAsync(tot, u32p(e->w_off) + ntop, 4, hipMemcpyDeviceToHost, s));
@@ -266,8 +266,7 @@ rsh_status build_items(rsh_engine* e, const rsh_frame_config& cfg, TravParams& p
                 return RSH_CAPACITY;
             }
             tot[1] = (uint32_t)ht[chunk - 1];
-            launch_item_count(cand, chunk, ntop, nitems, s);
-            HIP_OK(run_exclusive_scan(nitems, first, (uint64_t)ntop + 1, e->sc, s));
+            HIP_OK(run_item_scan(cand, ntop, chunk, nitems, first, e->sc, s));
         }
     }
     HIP_OK(e->w_list.reserve((size_t)tot[0] * 16 + 16));
@@ -277,8 +276,8 @@ rsh_status build_items(rsh_engine* e, const rsh_frame_config& cfg, TravParams& p
     p.obj_off = u32p(e->w_off); p.obj_ent = e->w_list.as<uint4>(); p.obj_cum = u32p(e->w_cum);
     if (p.algorithm == 1) { p.obj_ent = e->all_list.as<uint4>(); p.obj_cum = u32p(e->cl_base); }
     p.chunk = chunk; p.items = tot[1];
-    launch_cull_list(p, e->w_list.as<uint4>(), u32p(e->w_cum), s);     // writes at p.obj_off
-    launch_item_desc(p, nitems, e->w_items.as<uint4>(), s);
+    launch_cull_list(p, e->w_list.as<uint4>(), u32p(e->w_cum), pmask, s);     // writes at p.obj_off
+    launch_item_desc(p, e->w_items.as<uint4>(), s);
     HIP_OK(hipGetLastError());
     out.items = tot[1]; out.chunk = chunk; out.mode = p.prefilter;
     if (e->recording) {
@@ -301,8 +300,10 @@ rsh_status trace_pass(rsh_engine* e, const rsh_frame_config& cfg, const float* r
         // trimming and compression in one walk over the slots (the compress
         // stage time is part of the trim stage's)
         StageTimer t(e, ST_TRIM, timing);
+        const unsigned long long miss = cfg.hit_mode == 0 ? MISS64 : (unsigned long long)MISS32;
         HIP_OK(run_pack(u32p(e->flags), u32p(e->keys), u32p(e->values), S, u32p(e->tvalues), u32p(e->bases),
-                        u32p(e->ckeys), u32p(e->cvalues), dc, e->sc, s));
+                        u32p(e->ckeys), u32p(e->cvalues), dc, kind == 0 ? u64p(e->closest) : nullptr, miss,
+                        kind == 1 ? u32p(e->shadow) : nullptr, e->sc, s));
     }
     uint32_t hc[2] = {0, 0};
     if (e->replaying) {
@@ -315,8 +316,6 @@ rsh_status trace_pass(rsh_engine* e, const rsh_frame_config& cfg, const float* r
     }
     const uint32_t N = hc[0];
     out.N = N;
-    if (kind == 0) launch_fill_u64(u64p(e->closest), cfg.hit_mode == 0 ? MISS64 : (unsigned long long)MISS32, S, s);
-    else launch_fill_u32(u32p(e->shadow), 0u, S, s);
     if (N == 0) return RSH_OK;
     const uint32_t* sorted_values = u32p(e->tvalues);
     if (cfg.algorithm == RSH_CRSH) {
@@ -461,7 +460,7 @@ void rsh_engine_destroy(rsh_engine* e) {
                       &e->ckeys, &e->cvalues, &e->skeys, &e->svalues, &e->rvalues, &e->soff,
                       &e->rec0, &e->rec1, &e->nodes, &e->aux, &e->closest, &e->shadow, &e->pixels, &e->counters,
                       &e->dcounts, &e->w_npass, &e->w_cand, &e->w_nitems, &e->w_off, &e->w_first, &e->w_list,
-                      &e->w_cum, &e->w_items, &e->w_table, &e->views_d})
+                      &e->w_cum, &e->w_items, &e->w_table, &e->w_pmask, &e->views_d})
         b->release();
     if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
     if (e->graph) (void)hipGraphDestroy(e->graph);

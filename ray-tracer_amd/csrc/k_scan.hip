@@ -285,6 +285,33 @@ __device__ __forceinline__ void load16_live(const uint32_t* p, uint64_t base, ui
     }
 }
 
+// Exclusive scan of n <= SMALL_SCAN values in one workgroup of 1024 threads
+// (thread t scans a contiguous run); out[n - 1] closes the scan when the last
+// input is 0.  MODE 1 scans item counts ceil(in[i] / chunk) for i < n - 1 and 0
+// for the last entry (the work-list scan of the traversal).
+template <int MODE>
+__global__ void __launch_bounds__(1024) k_scan_block(const uint32_t* in, uint32_t n, uint32_t chunk, uint32_t* out,
+                                                     uint32_t* total) {
+    __shared__ uint32_t red[16];
+    const uint32_t per = (n + 1023u) / 1024u;
+    const uint32_t b = threadIdx.x * per;
+    const uint32_t e = b + per < n ? b + per : n;
+    auto val = [&](uint32_t i) -> uint32_t {
+        if (MODE == 1) return i + 1u < n ? (in[i] + chunk - 1u) / chunk : 0u;
+        return in[i];
+    };
+    uint32_t sum = 0;
+    for (uint32_t i = b; i < e; i++) sum += val(i);
+    uint32_t tot;
+    uint32_t run = block_excl_sum(sum, red, &tot);
+    for (uint32_t i = b; i < e; i++) {
+        const uint32_t v = val(i);
+        out[i] = run;
+        run += v;
+    }
+    if (threadIdx.x == 0 && total) *total = tot;
+}
+
 // A thread's 16 slots: live mask, heads after the thread's first live key,
 // the index and key of that first live key, and the last live key.
 struct PackThread {
@@ -378,7 +405,8 @@ __global__ void __launch_bounds__(1024) k_pack_scan(uint4* meta, uint32_t ntiles
 __global__ void __launch_bounds__(256) k_pack_write(const uint32_t* flags, const uint32_t* keys,
                                                     const uint32_t* values, uint64_t n, const uint4* meta,
                                                     uint32_t* tvalues, uint32_t* bases, uint32_t* ckeys,
-                                                    uint32_t* cvalues) {
+                                                    uint32_t* cvalues, unsigned long long* clear64,
+                                                    unsigned long long miss, uint32_t* clear32) {
     __shared__ uint32_t sa[TILE];
     __shared__ uint32_t sb[TILE];
     __shared__ LastKey lk[WAVES];
@@ -389,6 +417,25 @@ __global__ void __launch_bounds__(256) k_pack_write(const uint32_t* flags, const
     load16(flags, base, n, f, 1u);
     load16_live(keys, base, n, f, k);
     load16_live(values, base, n, f, v);
+    // the pass's per-slot results start as misses (closest) or clear (shadow)
+    if (clear64) {
+        if (base + 16 <= n) {
+            ulonglong2* q = reinterpret_cast<ulonglong2*>(clear64 + base);
+#pragma unroll
+            for (int i = 0; i < 8; i++) q[i] = make_ulonglong2(miss, miss);
+        } else {
+            for (int i = 0; i < 16; i++) if (base + i < n) clear64[base + i] = miss;
+        }
+    }
+    if (clear32) {
+        if (base + 16 <= n) {
+            uint4* q = reinterpret_cast<uint4*>(clear32 + base);
+#pragma unroll
+            for (int i = 0; i < 4; i++) q[i] = make_uint4(0u, 0u, 0u, 0u);
+        } else {
+            for (int i = 0; i < 16; i++) if (base + i < n) clear32[base + i] = 0u;
+        }
+    }
     PackThread t = pack_thread(f, k, base, n);
     LastKey blk;
     LastKey carry = block_last_key(t.last, lk, &blk);
@@ -699,6 +746,13 @@ __global__ void __launch_bounds__(256) k_expand(const uint32_t* bases, const uin
     }
 }
 
+__global__ void __launch_bounds__(256) k_item_counts(const uint32_t* cand, uint32_t chunk, uint32_t ntop,
+                                                     uint32_t* counts) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i > ntop) return;
+    counts[i] = i == ntop ? 0u : (uint32_t)(((uint64_t)cand[i] + chunk - 1) / chunk);
+}
+
 inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap = 1u << 20) {
     uint64_t g = (n + block - 1) / block;
     if (g == 0) g = 1;
@@ -749,6 +803,10 @@ static hipError_t run_scan(const uint32_t* in, uint32_t* out, uint64_t n, ScanSc
     uint32_t nt = tiles_for(n);
     if (nt > MAX_TILES) return hipErrorInvalidValue;
     if (nt == 0) return hipSuccess;
+    if (!inclusive && n <= SMALL_SCAN) {     // one launch instead of three
+        k_scan_block<0><<<1, 1024, 0, s>>>(in, (uint32_t)n, 1u, out, sc.total.as<uint32_t>());
+        return hipGetLastError();
+    }
     uint32_t* tiles = sc.tile_counts.as<uint32_t>();
     k_tile_total<1><<<nt, 256, 0, s>>>(in, n, tiles);
     k_scan_tiles<<<1, 1024, 0, s>>>(tiles, nt, sc.total.as<uint32_t>());
@@ -762,6 +820,17 @@ hipError_t run_inclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, Sca
 }
 hipError_t run_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, ScanScratch& sc, hipStream_t s) {
     return run_scan(in, out, n, sc, s, false);
+}
+
+hipError_t run_item_scan(const uint32_t* cand, uint32_t ntop, uint32_t chunk, uint32_t* counts, uint32_t* first,
+                         ScanScratch& sc, hipStream_t s) {
+    if (chunk == 0) return hipErrorInvalidValue;
+    if ((uint64_t)ntop + 1 <= SMALL_SCAN) {
+        k_scan_block<1><<<1, 1024, 0, s>>>(cand, ntop + 1, chunk, first, nullptr);
+        return hipGetLastError();
+    }
+    k_item_counts<<<ntop / 256 + 1, 256, 0, s>>>(cand, chunk, ntop, counts);
+    return run_scan(counts, first, (uint64_t)ntop + 1, sc, s, false);
 }
 
 hipError_t run_compress(const uint32_t* keys, uint64_t n, uint32_t* bases, uint32_t* sizes, uint32_t* ckeys,
@@ -779,7 +848,8 @@ hipError_t run_compress(const uint32_t* keys, uint64_t n, uint32_t* bases, uint3
 
 hipError_t run_pack(const uint32_t* flags, const uint32_t* keys, const uint32_t* values, uint64_t n,
                     uint32_t* tvalues, uint32_t* bases, uint32_t* ckeys, uint32_t* cvalues, uint32_t* d_totals,
-                    ScanScratch& sc, hipStream_t s) {
+                    unsigned long long* clear64, unsigned long long miss, uint32_t* clear32, ScanScratch& sc,
+                    hipStream_t s) {
     const uint32_t nt = tiles_for(n);
     if (nt > MAX_TILES) return hipErrorInvalidValue;
     if (nt == 0) {
@@ -790,7 +860,8 @@ hipError_t run_pack(const uint32_t* flags, const uint32_t* keys, const uint32_t*
     uint4* meta = sc.hist.as<uint4>();
     k_pack_tiles<<<nt, 256, 0, s>>>(flags, keys, n, meta);
     k_pack_scan<<<1, 1024, 0, s>>>(meta, nt, d_totals, bases);
-    k_pack_write<<<nt, 256, 0, s>>>(flags, keys, values, n, meta, tvalues, bases, ckeys, cvalues);
+    k_pack_write<<<nt, 256, 0, s>>>(flags, keys, values, n, meta, tvalues, bases, ckeys, cvalues, clear64, miss,
+                                    clear32);
     return hipGetLastError();
 }
 

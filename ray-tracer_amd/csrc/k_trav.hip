@@ -290,9 +290,12 @@ __device__ __forceinline__ uint32_t wave_incl_add(uint32_t x) {
     return x;
 }
 
+// The counting pass stores each node's passing objects as 64-bit masks
+// (pmask, ceil(K / 64) words per node); the list pass reads them back instead
+// of repeating the sphere tests.
 template <bool LIST>
 __global__ void __launch_bounds__(256) k_cull(TravParams P, uint32_t* npass, uint32_t* cand, uint4* obj_ent,
-                                              uint32_t* obj_cum) {
+                                              uint32_t* obj_cum, unsigned long long* pmask) {
     const uint32_t lane = lane_id();
     const uint32_t top = blockIdx.x * 4 + (threadIdx.x >> 6);
     const uint32_t ntop = P.level_count[1];
@@ -314,14 +317,21 @@ __global__ void __launch_bounds__(256) k_cull(TravParams P, uint32_t* npass, uin
     const float4 tS = P.nodes[2 * (size_t)gt], tC = P.nodes[2 * (size_t)gt + 1];
     const float2 tA = P.aux[gt];
     const size_t base = LIST ? P.obj_off[top] : 0;
+    const uint32_t words = (P.K + 63) / 64;
+    unsigned long long* pm = pmask + (size_t)top * words;
+    const float rcos = node_rcos(tA.x);
     uint32_t n = 0, cum = 0;
     unsigned long long pairs = 0;
     for (uint32_t ob = 0; ob < P.K; ob += 64) {
         const uint32_t oi = ob + lane;
         bool po = false;
         uint32_t ncl = 0, npair = 0;
+        if (LIST) {
+            po = (pm[ob / 64] >> lane) & 1ull;
+        } else if (oi < P.K) {
+            po = sphere_node_fast(tS, tC, P.obj_sphere[oi], tA.x, tA.y, rcos);
+        }
         if (oi < P.K) {
-            po = sphere_node(tS, tC, P.obj_sphere[oi], tA.x, tA.y);
             if (po) {
                 const uint4 r = P.obj_range[oi];
                 ncl = (r.y - r.x + CLUSTER) / CLUSTER;
@@ -329,6 +339,7 @@ __global__ void __launch_bounds__(256) k_cull(TravParams P, uint32_t* npass, uin
             }
         }
         const uint64_t m = __ballot(po);
+        if (!LIST && lane == 0) pm[ob / 64] = m;
         const uint32_t incl = wave_incl_add(ncl);
         if (LIST && po) {
             const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
@@ -363,12 +374,15 @@ __global__ void __launch_bounds__(256) k_item_count(const uint32_t* cand, uint32
 constexpr uint32_t TABLE_SLICE = 256;
 __global__ void __launch_bounds__(64) k_item_table(const uint32_t* cand, const float2* top_aux, uint32_t ntop,
                                                    unsigned long long* table) {
+    __shared__ uint32_t s_cand[TABLE_SLICE];
     const uint32_t lane = threadIdx.x & 63u, c = lane + 1u;
     const uint32_t t0 = blockIdx.x * TABLE_SLICE;
     const uint32_t t1 = t0 + TABLE_SLICE < ntop ? t0 + TABLE_SLICE : ntop;
+    for (uint32_t t = t0 + lane; t < t1; t += 64u) s_cand[t - t0] = cand[t];     // coalesced
+    __builtin_amdgcn_wave_barrier();
     unsigned long long sum = 0ull;
-    for (uint32_t t = t0; t < t1; t++) {
-        const uint32_t n = cand[t];
+    for (uint32_t t = 0; t < t1 - t0; t++) {
+        const uint32_t n = s_cand[t];
         sum += (n + c - 1u) / c;
     }
     if (sum) atomicAdd(&table[c - 1u], sum);
@@ -384,11 +398,11 @@ __global__ void __launch_bounds__(64) k_item_table(const uint32_t* cand, const f
 //   when the item is the node's only one (no merge needed).
 // The entry is found by binary search of the node's candidate offsets (RAH:
 // the shared list of all objects with the cluster bases as offsets).
-__global__ void __launch_bounds__(256) k_item_desc(TravParams P, const uint32_t* nitems, uint4* desc) {
+__global__ void __launch_bounds__(256) k_item_desc(TravParams P, uint4* desc) {
     const uint32_t top = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (top >= P.level_count[1]) return;
     const bool rah = P.algorithm == 1;
-    const uint32_t f = P.item_first[top], k = nitems[top], ncand = P.cand[top];
+    const uint32_t f = P.item_first[top], k = P.item_first[top + 1] - f, ncand = P.cand[top];
     const uint32_t lbase = rah ? 0u : P.obj_off[top];
     const uint32_t np = rah ? P.K : P.obj_off[top + 1] - lbase;
     for (uint32_t c = lane_id(); c < k; c += 64) {
@@ -1025,14 +1039,17 @@ void launch_hierarchy(const float4* rec0, const float4* rec1, const float* rays,
                                                            counts[k], branch, mode);
 }
 
-void launch_cull_count(const TravParams& p, uint32_t* npass, uint32_t* cand, hipStream_t s) {
+void launch_cull_count(const TravParams& p, uint32_t* npass, uint32_t* cand, unsigned long long* pmask,
+                       hipStream_t s) {
     const uint32_t top = p.level_count[1];
-    k_cull<false><<<top / 4 + 1, 256, 0, s>>>(p, npass, cand, nullptr, nullptr);
+    k_cull<false><<<top / 4 + 1, 256, 0, s>>>(p, npass, cand, nullptr, nullptr, pmask);
 }
 
-void launch_cull_list(const TravParams& p, uint4* obj_ent, uint32_t* obj_cum, hipStream_t s) {
+void launch_cull_list(const TravParams& p, uint4* obj_ent, uint32_t* obj_cum, unsigned long long* pmask,
+                      hipStream_t s) {
     const uint32_t top = p.level_count[1];
-    if (top && p.algorithm != 1) k_cull<true><<<(top + 3) / 4, 256, 0, s>>>(p, nullptr, nullptr, obj_ent, obj_cum);
+    if (top && p.algorithm != 1)
+        k_cull<true><<<(top + 3) / 4, 256, 0, s>>>(p, nullptr, nullptr, obj_ent, obj_cum, pmask);
 }
 
 void launch_item_table(const uint32_t* cand, const float2* top_aux, uint32_t ntop, unsigned long long* table,
@@ -1046,9 +1063,9 @@ void launch_item_count(const uint32_t* cand, uint32_t chunk, uint32_t ntop, uint
     k_item_count<<<ntop / 256 + 1, 256, 0, s>>>(cand, chunk, ntop, nitems);
 }
 
-void launch_item_desc(const TravParams& p, const uint32_t* nitems, uint4* desc, hipStream_t s) {
+void launch_item_desc(const TravParams& p, uint4* desc, hipStream_t s) {
     const uint32_t ntop = p.level_count[1];
-    if (ntop) k_item_desc<<<(ntop + 3) / 4, 256, 0, s>>>(p, nitems, desc);
+    if (ntop) k_item_desc<<<(ntop + 3) / 4, 256, 0, s>>>(p, desc);
 }
 
 template <int LEAF>

@@ -18,6 +18,8 @@ constexpr uint32_t SCAN_BLOCK = 256;
 constexpr uint32_t SCAN_ITEMS = 16;
 constexpr uint32_t TILE = SCAN_BLOCK * SCAN_ITEMS;
 constexpr uint32_t MAX_TILES = 16384;
+// Exclusive scans up to this length run in one workgroup.
+constexpr uint32_t SMALL_SCAN = 131072;
 
 inline uint32_t tiles_for(uint64_t n) { return (uint32_t)((n + TILE - 1) / TILE); }
 
@@ -84,9 +86,16 @@ hipError_t run_compress(const uint32_t* keys, uint64_t n, uint32_t* bases, uint3
 // values in order -> tvalues, one chunk per run of equal live keys -> bases
 // (first live position; bases[C] = N), ckeys, cvalues (chunk index).
 // d_totals[0] = N, d_totals[1] = C.  bases holds n + 1 entries.
+// clear64 (if set) receives `miss` in every slot, clear32 (if set) 0: the pass's
+// per-slot results start cleared.
 hipError_t run_pack(const uint32_t* flags, const uint32_t* keys, const uint32_t* values, uint64_t n,
                     uint32_t* tvalues, uint32_t* bases, uint32_t* ckeys, uint32_t* cvalues, uint32_t* d_totals,
-                    ScanScratch& sc, hipStream_t s);
+                    unsigned long long* clear64, unsigned long long miss, uint32_t* clear32, ScanScratch& sc,
+                    hipStream_t s);
+// Exclusive scan of the work-item counts ceil(cand[t] / chunk), t < ntop, with
+// first[ntop] = their total (counts is scratch of ntop + 1 entries).
+hipError_t run_item_scan(const uint32_t* cand, uint32_t ntop, uint32_t chunk, uint32_t* counts, uint32_t* first,
+                         ScanScratch& sc, hipStream_t s);
 // Decompression of run_pack's chunks in sorted order: out_values receives
 // tvalues[bases[v] .. bases[v + 1]) for v = sorted_cvalues[0 .. C) in turn
 // (N values in all).  soff (C + 1 entries) and sbase (C) are scratch.

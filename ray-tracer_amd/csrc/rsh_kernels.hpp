@@ -144,8 +144,11 @@ inline bool uses_items(const TravParams& p) {
            (p.leaf == 2 || p.leaf == 4 || p.leaf == 8 || p.leaf == 16 || p.leaf == 32);
 }
 
-void launch_cull_count(const TravParams& p, uint32_t* npass, uint32_t* cand, hipStream_t s);
-void launch_cull_list(const TravParams& p, uint4* obj_ent, uint32_t* obj_cum, hipStream_t s);
+// pmask: ceil(K / 64) words per top node, written by the count pass, read by the list pass
+void launch_cull_count(const TravParams& p, uint32_t* npass, uint32_t* cand, unsigned long long* pmask,
+                       hipStream_t s);
+void launch_cull_list(const TravParams& p, uint4* obj_ent, uint32_t* obj_cum, unsigned long long* pmask,
+                      hipStream_t s);
 // Item totals per chunk size: table[c - 1] = sum of ceil(cand / c), c = 1..ITEM_TABLE;
 // table[ITEM_TABLE] = top nodes with a wide cone (spread tangent > WIDE_TAN).
 constexpr uint32_t ITEM_TABLE = 64;
@@ -153,7 +156,7 @@ constexpr float WIDE_TAN = 0.1f;
 void launch_item_table(const uint32_t* cand, const float2* top_aux, uint32_t ntop, unsigned long long* table,
                        hipStream_t s);
 void launch_item_count(const uint32_t* cand, uint32_t chunk, uint32_t ntop, uint32_t* nitems, hipStream_t s);
-void launch_item_desc(const TravParams& p, const uint32_t* nitems, uint4* desc, hipStream_t s);
+void launch_item_desc(const TravParams& p, uint4* desc, hipStream_t s);
 
 // --- rays ---
 void launch_create_primary(const Views& V, uint32_t P, KeyCfg kc, float* rays, uint32_t* flags, uint32_t* keys,

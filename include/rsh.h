@@ -118,7 +118,7 @@ typedef struct rsh_frame_stats {
                                                 hierarchy, traversal, shading (summed) */
     uint32_t rays_total;
     uint64_t pass_top_survivors[RSH_MAX_PASSES]; /* top-level (node, triangle) pairs passing */
-    float pass_traverse_ms[RSH_MAX_PASSES];      /* traversal kernel time per pass (bit 2)   */
+    float pass_traverse_ms[RSH_MAX_PASSES];      /* traversal kernel time per pass (bit 2 or 4) */
     uint32_t pass_splits[RSH_MAX_PASSES];        /* generic kernel: waves per top node       */
     uint32_t pass_items[RSH_MAX_PASSES];         /* 64-ray kernel: work items launched       */
     uint32_t pass_chunk[RSH_MAX_PASSES];         /* 64-ray kernel: candidate clusters per item */
@@ -148,6 +148,8 @@ rsh_status rsh_engine_load_scene(rsh_engine* e, const rsh_scene_desc* scene);
  *   bit 0  keep every pass's per-slot hit records for rsh_engine_read_pass
  *   bit 1  collect traversal counters (pairs / tests) into the stats
  *   bit 2  time stages with HIP events into stats->stage_ms
+ *   bit 4  time only the traversal kernel launches (stats->pass_traverse_ms and
+ *          stage_ms[6]), two events per pass instead of two per stage
  *   bit 3  graph replay (alone, without bits 0-2): the first call for a given
  *          set of views, configuration and scene renders eagerly and captures
  *          a HIP graph of the frame (every pass's sizes fixed to the eager

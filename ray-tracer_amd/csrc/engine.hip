@@ -295,6 +295,7 @@ rsh_status trace_pass(rsh_engine* e, const rsh_frame_config& cfg, const float* r
     out = PassOut{};
     hipStream_t s = e->stream;
     const bool timing = flags_mask & 4u, counting = flags_mask & 2u;
+    const bool trav_timing = timing || (flags_mask & 16u);     // bit 4: traversal launches only
     uint32_t* dc = u32p(e->dcounts);
     {
         // trimming and compression in one walk over the slots (the compress
@@ -388,7 +389,7 @@ rsh_status trace_pass(rsh_engine* e, const rsh_frame_config& cfg, const float* r
         out.splits = p.splits;
     }
     {
-        StageTimer t(e, ST_PASS_TRAVERSE + (int)e->pass_index, timing);
+        StageTimer t(e, ST_PASS_TRAVERSE + (int)e->pass_index, trav_timing);
         HIP_OK(launch_traverse(p, s));
     }
     if (counting) {
@@ -704,7 +705,7 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
     HIP_OK(hipGetLastError());
     if (e->replaying) return RSH_OK;     // under stream capture: no synchronisation
     HIP_OK(hipStreamSynchronize(s));
-    if (st && timing) {
+    if (st && (timing || (fl & 16u))) {
         for (auto& t : e->timed) {
             float ms = 0.0f;
             if (hipEventElapsedTime(&ms, t.second.first, t.second.second) != hipSuccess) continue;
@@ -744,7 +745,7 @@ void drop_graph(rsh_engine* e) {
 rsh_status render(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rsh_frame_config& cfg, uint32_t fl,
                   rsh_frame_stats* st) {
     if (!(fl & 8u)) return render_frame(e, hv, nv, cfg, fl, st);
-    if (fl & 7u) { e->err = "render: graph replay keeps no passes, counters or timers (flag bits 0-2)"; return RSH_INVALID_ARG; }
+    if (fl & 23u) { e->err = "render: graph replay keeps no passes, counters or timers (flag bits 0-2, 4)"; return RSH_INVALID_ARG; }
     if (e->stream == nullptr) { e->err = "render: graph replay needs a non-null stream"; return RSH_INVALID_ARG; }
     std::vector<uint8_t> sig(sizeof(cfg) + sizeof(e->scene_epoch) + (size_t)nv * sizeof(ViewDesc));
     std::memcpy(sig.data(), &cfg, sizeof(cfg));
@@ -880,7 +881,7 @@ rsh_status rsh_engine_trace(rsh_engine* e, const float* rays, const float* tmax,
     HIP_OK(hipMemcpyAsync(out, kind == 1 ? e->shadow.p : e->closest.p, (size_t)n * (kind == 1 ? 4 : 8),
                           hipMemcpyDeviceToDevice, s));
     HIP_OK(hipStreamSynchronize(s));
-    if (st && timing) {
+    if (st && (timing || (fl & 16u))) {
         for (auto& t : e->timed) {
             float ms = 0.0f;
             if (hipEventElapsedTime(&ms, t.second.first, t.second.second) != hipSuccess) continue;

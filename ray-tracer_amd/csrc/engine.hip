@@ -48,6 +48,7 @@ struct rsh_engine {
     DevBuf rays_a, rays_b, flags, keys, values, tvalues, bases, ckeys, cvalues, skeys, svalues;
     DevBuf rvalues, soff, rec0, rec1, nodes, aux;
     DevBuf closest, shadow, pixels, counters, dcounts;
+    uint32_t* hstage = nullptr;        // page-locked host words for the per-pass read-backs
     // work list of the 64-ray traversal kernel
     DevBuf w_npass, w_cand, w_nitems, w_off, w_first, w_list, w_cum, w_items, w_table, w_pmask;
     DevBuf views_d;                    // view table of a batch (more than one view)
@@ -145,6 +146,7 @@ constexpr uint64_t TRAVERSE_TARGET_WAVES = 16384;
 // atrium frame: 32768 -> 10.1 ms of traversal, 131072 -> 9.3, 262144 -> 9.3,
 // 524288 -> 9.6 (smaller items cost more per-item set-up).
 constexpr uint32_t ITEM_CHUNK = 64;
+constexpr size_t HSTAGE_WORDS = 2 + 2 * (ITEM_TABLE + 1);   // list / item totals, then the item table
 constexpr uint32_t ITEM_CHUNK_MIN = 4;
 constexpr uint32_t ITEM_TARGET = 131072;
 
@@ -240,13 +242,16 @@ rsh_status build_items(rsh_engine* e, const rsh_frame_config& cfg, TravParams& p
         uint32_t* first = u32p(e->w_first);
         HIP_OK(e->w_table.reserve((ITEM_TABLE + 1) * sizeof(unsigned long long)));
         launch_item_table(cand, p.aux + p.level_offset[1], ntop, e->w_table.as<unsigned long long>(), s);
-        HIP_OK(hipMemcpyAsync(ht, e->w_table.p, sizeof(ht), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(e->hstage + 2, e->w_table.p, sizeof(ht), hipMemcpyDeviceToHost, s));
         if (!tabled) {
             HIP_OK(run_item_scan(cand, ntop, chunk, nitems, first, e->sc, s));
-            HIP_OK(hipMemcpyAsync(tot + 1, first + ntop, 4, hipMemcpyDeviceToHost, s));
+            HIP_OK(hipMemcpyAsync(e->hstage + 1, first + ntop, 4, hipMemcpyDeviceToHost, s));
         }
-        if (p.algorithm != 1) HIP_OK(hipMemcpyAsync(tot, u32p(e->w_off) + ntop, 4, hipMemcpyDeviceToHost, s));
+        if (p.algorithm != 1) HIP_OK(hipMemcpyAsync(e->hstage, u32p(e->w_off) + ntop, 4, hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
+        if (p.algorithm != 1) tot[0] = e->hstage[0];
+        if (!tabled) tot[1] = e->hstage[1];
+        std::memcpy(ht, e->hstage + 2, sizeof(ht));
         // Traversal mode: the cluster ray masks pay when many top nodes are wide
         // (spread-out secondary rays) and cost where rays are coherent.  More than
         // a quarter of the nodes with tan(spread) > WIDE_TAN picks the best mode
@@ -311,8 +316,9 @@ rsh_status trace_pass(rsh_engine* e, const rsh_frame_config& cfg, const float* r
         const rsh_engine::PassPlan& pp = e->plan[e->plan_pos++];
         hc[0] = pp.N; hc[1] = pp.C;
     } else {
-        HIP_OK(hipMemcpyAsync(hc, dc, 8, hipMemcpyDeviceToHost, s));
+        HIP_OK(hipMemcpyAsync(e->hstage, dc, 8, hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
+        hc[0] = e->hstage[0]; hc[1] = e->hstage[1];
         if (e->recording) { e->plan.emplace_back(); e->plan.back().N = hc[0]; e->plan.back().C = hc[1]; }
     }
     const uint32_t N = hc[0];
@@ -446,6 +452,13 @@ rsh_status rsh_engine_create(rsh_engine** out, int device) {
     e->device = device;
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) { delete e; return RSH_HIP_ERROR; }
     e->own_stream = true;
+    // read-backs of a few words per pass land in page-locked memory (a direct copy,
+    // no staging through pageable memory)
+    if (hipHostMalloc(reinterpret_cast<void**>(&e->hstage), HSTAGE_WORDS * 4, hipHostMallocDefault) != hipSuccess) {
+        (void)hipStreamDestroy(e->stream);
+        delete e;
+        return RSH_HIP_ERROR;
+    }
     *out = e;
     return RSH_OK;
 }
@@ -454,6 +467,7 @@ void rsh_engine_destroy(rsh_engine* e) {
     if (!e) return;
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->hstage) (void)hipHostFree(e->hstage);
     for (DevBuf* b : {&e->tri_pos, &e->tri_nrm, &e->tri_mat, &e->mat_dif, &e->mat_spec, &e->light_pos, &e->light_col,
                       &e->light_int, &e->tri_sphere, &e->tri_mt, &e->obj_sphere, &e->obj_range, &e->cl_sphere,
                       &e->cl_base, &e->all_list, &e->rays_a,

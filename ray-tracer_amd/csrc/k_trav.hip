@@ -366,11 +366,12 @@ __global__ void __launch_bounds__(256) k_item_count(const uint32_t* cand, uint32
     nitems[i] = i == ntop ? 0u : (uint32_t)(((uint64_t)cand[i] + chunk - 1) / chunk);
 }
 
-// Item totals for every chunk size 1..ITEM_TABLE: table[c - 1] = sum over top
-// nodes of ceil(cand / c); table[ITEM_TABLE] = top nodes whose cone is wide
-// (tangent of the spread above WIDE_TAN, or not finite).  One wave per slice
-// of TABLE_SLICE top nodes, lane c - 1 sums chunk size c in a register; one
-// read-back of the table sizes a pass and picks its traversal mode.
+// Item totals per chunk size: table[c - 1] = sum over top nodes of ceil(cand /
+// c) for c = 1..ITEM_TABLE, table[ITEM_TABLE + c - 1] the same for chunk 2c;
+// table[2 ITEM_TABLE] = top nodes whose cone is wide (tangent of the spread
+// above WIDE_TAN, or not finite).  One wave per slice of TABLE_SLICE top nodes,
+// lane c - 1 sums chunk sizes c and 2c in registers; one read-back of the
+// table sizes a pass and picks its traversal mode.
 constexpr uint32_t TABLE_SLICE = 256;
 __global__ void __launch_bounds__(64) k_item_table(const uint32_t* cand, const float2* top_aux, uint32_t ntop,
                                                    unsigned long long* table) {
@@ -380,16 +381,18 @@ __global__ void __launch_bounds__(64) k_item_table(const uint32_t* cand, const f
     const uint32_t t1 = t0 + TABLE_SLICE < ntop ? t0 + TABLE_SLICE : ntop;
     for (uint32_t t = t0 + lane; t < t1; t += 64u) s_cand[t - t0] = cand[t];     // coalesced
     __builtin_amdgcn_wave_barrier();
-    unsigned long long sum = 0ull;
+    unsigned long long sum = 0ull, sum2 = 0ull;
     for (uint32_t t = 0; t < t1 - t0; t++) {
         const uint32_t n = s_cand[t];
         sum += (n + c - 1u) / c;
+        sum2 += (n + 2u * c - 1u) / (2u * c);
     }
     if (sum) atomicAdd(&table[c - 1u], sum);
+    if (sum2) atomicAdd(&table[ITEM_TABLE + c - 1u], sum2);
     uint32_t wide = 0;
     for (uint32_t t = t0 + lane; t < t1; t += 64u) wide += !(top_aux[t].y <= WIDE_TAN) ? 1u : 0u;
     for (int d = 32; d > 0; d >>= 1) wide += __shfl_xor(wide, d, 64);
-    if (lane == 0 && wide) atomicAdd(&table[ITEM_TABLE], (unsigned long long)wide);
+    if (lane == 0 && wide) atomicAdd(&table[2 * ITEM_TABLE], (unsigned long long)wide);
 }
 
 // Work-item descriptors, one wave per top node, one lane per item:
@@ -1055,7 +1058,7 @@ void launch_cull_list(const TravParams& p, uint4* obj_ent, uint32_t* obj_cum, un
 void launch_item_table(const uint32_t* cand, const float2* top_aux, uint32_t ntop, unsigned long long* table,
                        hipStream_t s) {
     static_assert(ITEM_TABLE == 64, "one lane per chunk size");
-    launch_fill_u64(table, 0ull, ITEM_TABLE + 1, s);
+    launch_fill_u64(table, 0ull, 2 * ITEM_TABLE + 1, s);
     if (ntop) k_item_table<<<(ntop + TABLE_SLICE - 1) / TABLE_SLICE, 64, 0, s>>>(cand, top_aux, ntop, table);
 }
 

@@ -259,7 +259,7 @@ def main():
     avg_launch_ms = trav_ms / trav_launches if trav_launches else 0.0
     flops_launch = flops_frame / launches_per_frame if launches_per_frame else 0.0
     achieved = flops_launch / (avg_launch_ms * 1e-3) / 1e12 if avg_launch_ms > 0 else 0.0
-    roof = {"bound": "valu", "kernel": "k_traverse64<8, true> (fused traversal + intersection, one launch per pass)",
+    roof = {"bound": "valu", "kernel": "k_traverse64<8, MODE> (fused traversal + intersection, one launch per pass)",
             "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic_per_launch(),
             "avg_launch_ms": avg_launch_ms, "launches_per_frame": launches_per_frame,

@@ -369,7 +369,8 @@ __global__ void __launch_bounds__(256) k_item_count(const uint32_t* cand, uint32
 // Item totals per chunk size: table[c - 1] = sum over top nodes of ceil(cand /
 // c) for c = 1..ITEM_TABLE, table[ITEM_TABLE + c - 1] the same for chunk 2c;
 // table[2 ITEM_TABLE] = top nodes whose cone is wide (tangent of the spread
-// above WIDE_TAN, or not finite).  One wave per slice of TABLE_SLICE top nodes,
+// outside [0, WIDE_TAN]: a spread clamped to pi/2 has a large negative float
+// tangent; or not finite).  One wave per slice of TABLE_SLICE top nodes,
 // lane c - 1 sums chunk sizes c and 2c in registers; one read-back of the
 // table sizes a pass and picks its traversal mode.
 constexpr uint32_t TABLE_SLICE = 256;
@@ -390,7 +391,7 @@ __global__ void __launch_bounds__(64) k_item_table(const uint32_t* cand, const f
     if (sum) atomicAdd(&table[c - 1u], sum);
     if (sum2) atomicAdd(&table[ITEM_TABLE + c - 1u], sum2);
     uint32_t wide = 0;
-    for (uint32_t t = t0 + lane; t < t1; t += 64u) wide += !(top_aux[t].y <= WIDE_TAN) ? 1u : 0u;
+    for (uint32_t t = t0 + lane; t < t1; t += 64u) wide += !(top_aux[t].y >= 0.0f && top_aux[t].y <= WIDE_TAN) ? 1u : 0u;
     for (int d = 32; d > 0; d >>= 1) wide += __shfl_xor(wide, d, 64);
     if (lane == 0 && wide) atomicAdd(&table[2 * ITEM_TABLE], (unsigned long long)wide);
 }

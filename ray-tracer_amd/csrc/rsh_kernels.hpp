@@ -151,7 +151,7 @@ void launch_cull_list(const TravParams& p, uint4* obj_ent, uint32_t* obj_cum, un
                       hipStream_t s);
 // Item totals per chunk size: table[c - 1] = sum of ceil(cand / c), c = 1..ITEM_TABLE;
 // table[ITEM_TABLE + c - 1] the same for chunk 2c; table[2 ITEM_TABLE] = top nodes
-// with a wide cone (spread tangent > WIDE_TAN).
+// with a wide cone (spread tangent outside [0, WIDE_TAN]).
 constexpr uint32_t ITEM_TABLE = 64;
 constexpr float WIDE_TAN = 0.1f;
 void launch_item_table(const uint32_t* cand, const float2* top_aux, uint32_t ntop, unsigned long long* table,

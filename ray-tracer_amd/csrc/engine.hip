@@ -37,7 +37,7 @@ struct rsh_engine {
     DevBuf tri_sphere, tri_mt, obj_sphere, obj_range;
     // cluster level: CLUSTER consecutive triangles of one object per cluster
     uint32_t NC = 0;
-    DevBuf cl_sphere, cl_base, all_list;   // float4[NC], uint32[K] first cluster, uint4[K] all objects as
+    DevBuf cl_sphere, sub_sphere, sub_use, cl_base, all_list;   // float4[NC], uint32[K] first cluster, uint4[K] all objects as
                                            // list entries (first, last, first cluster, 0) for RAH
     std::vector<float> host_spheres;   // float3[2K]
     uint32_t ranges_tb = 0xFFFFFFFFu;  // triangle batch the exclusion ranges were built for
@@ -370,7 +370,7 @@ rsh_status trace_pass(rsh_engine* e, const rsh_frame_config& cfg, const float* r
     p.rec0 = f4p(e->rec0); p.rec1 = f4p(e->rec1); p.N = N;
     p.tri_sphere = f4p(e->tri_sphere); p.tri_mt = f4p(e->tri_mt); p.T = e->T;
     p.obj_sphere = f4p(e->obj_sphere); p.obj_range = e->obj_range.as<uint4>(); p.K = e->K;
-    p.cl_sphere = f4p(e->cl_sphere); p.obj_cl_base = u32p(e->cl_base); p.n_clusters = e->NC;
+    p.cl_sphere = f4p(e->cl_sphere); p.sub_sphere = f4p(e->sub_sphere); p.sub_use = e->sub_use.as<uint8_t>(); p.obj_cl_base = u32p(e->cl_base); p.n_clusters = e->NC;
     p.algorithm = cfg.algorithm; p.kind = kind; p.hit_mode = cfg.hit_mode;
     p.closest = u64p(e->closest); p.shadow = u32p(e->shadow);
     p.counters = counting ? u64p(e->counters) : nullptr;
@@ -473,7 +473,7 @@ void rsh_engine_destroy(rsh_engine* e) {
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     if (e->hstage) (void)hipHostFree(e->hstage);
     for (DevBuf* b : {&e->tri_pos, &e->tri_nrm, &e->tri_mat, &e->mat_dif, &e->mat_spec, &e->light_pos, &e->light_col,
-                      &e->light_int, &e->tri_sphere, &e->tri_mt, &e->obj_sphere, &e->obj_range, &e->cl_sphere,
+                      &e->light_int, &e->tri_sphere, &e->tri_mt, &e->obj_sphere, &e->obj_range, &e->cl_sphere, &e->sub_sphere, &e->sub_use,
                       &e->cl_base, &e->all_list, &e->rays_a,
                       &e->rays_b, &e->flags, &e->keys, &e->values, &e->tvalues, &e->bases,
                       &e->ckeys, &e->cvalues, &e->skeys, &e->svalues, &e->rvalues, &e->soff,
@@ -566,7 +566,17 @@ rsh_status rsh_engine_load_scene(rsh_engine* e, const rsh_scene_desc* s) {
         }
     }
     e->NC = nc;
+    // sub-clusters of each cluster; missing ones (short clusters) get an empty
+    // range, are never tested and hold a NaN sphere
+    std::vector<uint32_t> srange((size_t)nc * SUB_PER_CLUSTER * 2);
+    for (uint32_t c = 0; c < nc; c++)
+        for (uint32_t k = 0; k < SUB_PER_CLUSTER; k++) {
+            const uint32_t a = crange[2 * c] + k * SUBCLUSTER, b = crange[2 * c + 1];
+            srange[2 * ((size_t)c * SUB_PER_CLUSTER + k)] = a <= b ? a : 1u;
+            srange[2 * ((size_t)c * SUB_PER_CLUSTER + k) + 1] = a <= b ? (a + SUBCLUSTER - 1 < b ? a + SUBCLUSTER - 1 : b) : 0u;
+        }
     HIP_OK(e->cl_sphere.reserve((size_t)nc * 16)); HIP_OK(e->cl_base.reserve((size_t)K * 4));
+    HIP_OK(e->sub_sphere.reserve((size_t)nc * SUB_PER_CLUSTER * 16)); HIP_OK(e->sub_use.reserve((size_t)nc + 4));
     HIP_OK(e->all_list.reserve((size_t)K * 16));
     DevBuf crange_d;
     HIP_OK(crange_d.reserve(crange.size() * 4));
@@ -574,6 +584,12 @@ rsh_status rsh_engine_load_scene(rsh_engine* e, const rsh_scene_desc* s) {
     HIP_OK(hipMemcpyAsync(e->cl_base.p, base.data(), (size_t)K * 4, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(e->all_list.p, ident.data(), (size_t)K * 16, hipMemcpyHostToDevice, st));
     launch_cluster_build(f4p(e->tri_sphere), crange_d.as<uint2>(), nc, f4p(e->cl_sphere), st);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(st));
+    HIP_OK(crange_d.reserve(srange.size() * 4));
+    HIP_OK(hipMemcpyAsync(crange_d.p, srange.data(), srange.size() * 4, hipMemcpyHostToDevice, st));
+    launch_cluster_build(f4p(e->tri_sphere), crange_d.as<uint2>(), nc * SUB_PER_CLUSTER, f4p(e->sub_sphere), st);
+    launch_sub_use(f4p(e->cl_sphere), f4p(e->sub_sphere), nc, e->sub_use.as<uint8_t>(), st);
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(st));
     crange_d.release();

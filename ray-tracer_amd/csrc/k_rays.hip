@@ -287,6 +287,20 @@ __global__ void __launch_bounds__(256) k_cluster_build(const float4* tri_sphere,
     }
 }
 
+// Sub-clusters worth a ray mask: bit k of sub_use[c] is set when sub-cluster k
+// of cluster c has a radius below SUB_RATIO of the cluster's (a smaller sphere
+// meets fewer of the rays that meet the cluster's).
+__global__ void __launch_bounds__(256) k_sub_use(const float4* cl_sphere, const float4* sub_sphere, uint32_t nc,
+                                                 uint8_t* sub_use) {
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        const float rc = cl_sphere[c].w;
+        uint32_t bits = 0;
+        for (uint32_t k = 0; k < SUB_PER_CLUSTER; k++)
+            bits |= sub_sphere[(size_t)c * SUB_PER_CLUSTER + k].w < SUB_RATIO * rc ? 1u << k : 0u;
+        sub_use[c] = (uint8_t)bits;
+    }
+}
+
 // RayTracer.cu:448-511
 __global__ void __launch_bounds__(256) k_update_vertex(const float* model, const float* normal, uint32_t n,
                                                        const float4* pos_in, const float4* nrm_in,
@@ -422,6 +436,10 @@ void launch_shade_reflection(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, 
 void launch_cluster_build(const float4* tri_sphere, const uint2* range, uint32_t n, float4* cl_sphere,
                           hipStream_t s) {
     if (n) k_cluster_build<<<grid_for(n, 256), 256, 0, s>>>(tri_sphere, range, n, cl_sphere);
+}
+
+void launch_sub_use(const float4* cl_sphere, const float4* sub_sphere, uint32_t nc, uint8_t* sub_use, hipStream_t s) {
+    if (nc) k_sub_use<<<grid_for(nc, 256), 256, 0, s>>>(cl_sphere, sub_sphere, nc, sub_use);
 }
 
 void launch_triangle_prep(const float4* tri_pos, uint32_t T, float4* tri_sphere, float4* tri_mt, hipStream_t s) {

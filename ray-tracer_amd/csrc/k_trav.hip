@@ -469,6 +469,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MODE >=
     constexpr uint32_t QCAP = 128;                // survivor queue
     constexpr uint32_t PCAP = 512;                // (survivor, leaf) candidate / pair list
     constexpr uint32_t SCH = PCAP / BRANCH < 64 ? PCAP / BRANCH : 64;   // survivors per B+C round
+    constexpr uint32_t SUB_MIN = 64;              // RM: survivors x rays that earn a sub-cluster mask
     __shared__ float4 s_qsph[QCAP];
     __shared__ uint32_t s_qidx[QCAP];
     __shared__ uint32_t s_qmask[QCAP];         // leaves the survivor's cluster may pass
@@ -835,26 +836,54 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MODE >=
             // ---- stage A
             bool p = false;
             if (ink[k]) p = sphere_node_fast(tS, tC, tsk[k], tA.x, tA.y, t_bound.y);
-            const uint64_t m = __ballot(p);
+            uint64_t m = __ballot(p);
             if (count) { n_top += (uint64_t)__popcll(__ballot(ink[k])); n_surv += (uint64_t)__popcll(m); }
             if (m == 0ull) continue;
-            uint32_t cm = cmk[k];            // leaves the block's cluster may pass (nonzero)
+            uint32_t cm = cmk[k];            // leaves the block's cluster (RM: the lane's survivor) may pass
             uint64_t rm = ~0ull;
             if constexpr (RM) {
-                // Rays whose line may meet the cluster's sphere (ray_may_hit_sphere
-                // holds for the cluster as for each of its triangles); leaves with
-                // none of them are dropped.  Counting runs keep every ray and leaf,
-                // so the counts stay the reference's.
+                // Rays whose line may meet the cluster's sphere, then, per
+                // sub-cluster holding survivors, its sphere (the argument behind
+                // ray_may_hit_sphere holds for any sphere that holds the
+                // triangle); a survivor keeps the rays of its sub-cluster and the
+                // leaves holding one of them.  Counting runs keep every ray and
+                // leaf, so the counts stay the reference's.
                 if (!count) {
-                    const float4 cs = P.cl_sphere[cidk[k]];
-                    rm = __ballot(((alive >> lane) & 1ull) &&
-                                  ray_may_hit_sphere(xyz(s_ray0[lane]), xyz(s_ray1[lane]), cs));
-                    uint32_t lr = 0;
+                    auto leaves_of = [&](uint64_t r) {
+                        uint32_t lr = 0;
 #pragma unroll
-                    for (uint32_t b = 0; b < BRANCH; b++)
-                        lr |= (((rm >> (b * LEAF)) & LRAYS) != 0ull ? 1u : 0u) << b;
-                    cm &= lr;
+                        for (uint32_t b = 0; b < BRANCH; b++)
+                            lr |= (((r >> (b * LEAF)) & LRAYS) != 0ull ? 1u : 0u) << b;
+                        return lr;
+                    };
+                    const v3 ro = xyz(s_ray0[lane]), rd = xyz(s_ray1[lane]);
+                    const uint32_t use = P.sub_use[cidk[k]];
+                    rm = __ballot(((alive >> lane) & 1ull) && ray_may_hit_sphere(ro, rd, P.cl_sphere[cidk[k]]));
+                    cm &= leaves_of(rm);
                     if (cm == 0u) continue;
+                    // sub-clusters worth a mask: small enough (sub_use) and survivors
+                    // x rays at least SUB_MIN
+                    const uint32_t nrc = (uint32_t)__popcll(rm);
+                    uint32_t subs = 0;
+#pragma unroll
+                    for (uint32_t k8 = 0; k8 < SUB_PER_CLUSTER; k8++)
+                        subs |= (uint32_t)__popcll((m >> (k8 * SUBCLUSTER)) & ((1ull << SUBCLUSTER) - 1ull)) * nrc >= SUB_MIN
+                                    ? 1u << k8 : 0u;
+                    subs &= use;
+                    if (subs) {
+                        const bool rin = (rm >> lane) & 1ull;
+                        const float4* ssp = P.sub_sphere + (size_t)cidk[k] * SUB_PER_CLUSTER;
+                        do {
+                            const uint32_t k8 = (uint32_t)__builtin_ctz(subs);
+                            subs &= subs - 1u;
+                            const uint64_t rk = __ballot(rin && ray_may_hit_sphere(ro, rd, ssp[k8]));
+                            if (lane / SUBCLUSTER == k8) rm = rk;
+                        } while (subs);
+                        cm &= leaves_of(rm);
+                        p = p && cm != 0u;
+                        m = __ballot(p);
+                        if (m == 0ull) continue;
+                    }
                 }
             }
             const uint32_t add = (uint32_t)__popcll(m);

@@ -83,6 +83,14 @@ enum CounterIndex : uint32_t {
 };
 // Triangles per cluster of the traversal's cluster level.
 constexpr uint32_t CLUSTER = 64;
+// Sub-clusters: SUBCLUSTER consecutive triangles of a cluster, each with its own
+// sphere (the cluster's sub-clusters at SUB_PER_CLUSTER * cluster + k).
+constexpr uint32_t SUBCLUSTER = 8;
+constexpr uint32_t SUB_PER_CLUSTER = CLUSTER / SUBCLUSTER;
+static_assert(SUB_PER_CLUSTER <= 8, "sub-cluster flags are one byte per cluster");
+// A sub-cluster gets a ray mask of its own only when its radius is below
+// SUB_RATIO of its cluster's.
+constexpr float SUB_RATIO = 0.7f;
 
 struct TravParams {
     const float4* nodes;          // hierarchy, sphere at 2g, cone at 2g+1
@@ -121,6 +129,8 @@ struct TravParams {
     uint32_t items;
     // cluster level: clusters of CLUSTER consecutive triangles of one object
     const float4* cl_sphere;      // [clusters] conservative sphere over the member triangle spheres
+    const float4* sub_sphere;     // [clusters * SUB_PER_CLUSTER] the same per sub-cluster
+    const uint8_t* sub_use;       // [clusters] sub-clusters worth a ray mask (bit k: sub-cluster k)
     const uint32_t* obj_cl_base;  // [K] first cluster of each object (RAH: also the list's obj_cum)
     uint32_t n_clusters;
     uint32_t prefilter;           // 64-ray kernel: 0 exact tests only, 1 ray-line / triangle-sphere
@@ -179,6 +189,7 @@ void launch_shade_reflection(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, 
                              const unsigned long long* rec, uint32_t hit_mode, const uint32_t* sflags,
                              uint32_t* pixels, hipStream_t s);
 void launch_triangle_prep(const float4* tri_pos, uint32_t T, float4* tri_sphere, float4* tri_mt, hipStream_t s);
+void launch_sub_use(const float4* cl_sphere, const float4* sub_sphere, uint32_t nc, uint8_t* sub_use, hipStream_t s);
 void launch_cluster_build(const float4* tri_sphere, const uint2* range, uint32_t n, float4* cl_sphere,
                           hipStream_t s);
 // reference-only stages

@@ -255,10 +255,10 @@ rsh_status build_items(rsh_engine* e, const rsh_frame_config& cfg, TravParams& p
         std::memcpy(ht, e->hstage + 2, sizeof(ht));
         // Traversal mode: the cluster ray masks pay when many top nodes are wide
         // (spread-out secondary rays) and cost where rays are coherent.  More than
-        // a quarter of the nodes with tan(spread) outside [0, WIDE_TAN] picks the best mode
-        // for every pass of the 1080p atrium frame (primary and first shadow pass
-        // without, every bounce with).
-        if (cfg.traversal_prefilter == 0) p.prefilter = 4 * ht[2 * ITEM_TABLE] > ntop ? 2u : 1u;
+        // a sixteenth of the nodes with tan(spread) outside [0, WIDE_TAN] picks the
+        // best mode for every pass of the 1080p atrium frame (primary and first
+        // shadow pass without, every later pass with).
+        if (cfg.traversal_prefilter == 0) p.prefilter = 16 * ht[2 * ITEM_TABLE] > ntop ? 2u : 1u;
         if (tabled) {
             if (!cfg.traversal_chunk) {
                 // coherent passes with plenty of items take longer items (less

@@ -231,7 +231,7 @@ rsh_status build_items(rsh_engine* e, const rsh_frame_config& cfg, TravParams& p
     if (e->replaying) {
         const rsh_engine::PassPlan& pp = e->plan[e->plan_pos - 1];
         chunk = pp.chunk; tot[0] = pp.list_total; tot[1] = pp.items; p.prefilter = pp.mode;
-        HIP_OK(run_item_scan(cand, ntop, chunk, nitems, u32p(e->w_first), e->sc, s));
+        HIP_OK(run_item_scan(cand, ntop, chunk, e->NC, nitems, u32p(e->w_first), e->sc, s));
     }
     if (!e->replaying) {
         // One read-back sizes the pass: the list total and, from the table of
@@ -242,10 +242,10 @@ rsh_status build_items(rsh_engine* e, const rsh_frame_config& cfg, TravParams& p
         auto items_at = [&](uint32_t c) { return c <= ITEM_TABLE ? ht[c - 1] : ht[ITEM_TABLE + c / 2 - 1]; };
         uint32_t* first = u32p(e->w_first);
         HIP_OK(e->w_table.reserve((2 * ITEM_TABLE + 1) * sizeof(unsigned long long)));
-        launch_item_table(cand, p.aux + p.level_offset[1], ntop, e->w_table.as<unsigned long long>(), s);
+        launch_item_table(cand, p.aux + p.level_offset[1], ntop, e->NC, e->w_table.as<unsigned long long>(), s);
         HIP_OK(hipMemcpyAsync(e->hstage + 2, e->w_table.p, sizeof(ht), hipMemcpyDeviceToHost, s));
         if (!tabled) {
-            HIP_OK(run_item_scan(cand, ntop, chunk, nitems, first, e->sc, s));
+            HIP_OK(run_item_scan(cand, ntop, chunk, e->NC, nitems, first, e->sc, s));
             HIP_OK(hipMemcpyAsync(e->hstage + 1, first + ntop, 4, hipMemcpyDeviceToHost, s));
         }
         if (p.algorithm != 1) HIP_OK(hipMemcpyAsync(e->hstage, u32p(e->w_off) + ntop, 4, hipMemcpyDeviceToHost, s));
@@ -275,7 +275,7 @@ rsh_status build_items(rsh_engine* e, const rsh_frame_config& cfg, TravParams& p
                 return RSH_CAPACITY;
             }
             tot[1] = (uint32_t)items_at(chunk);
-            HIP_OK(run_item_scan(cand, ntop, chunk, nitems, first, e->sc, s));
+            HIP_OK(run_item_scan(cand, ntop, chunk, e->NC, nitems, first, e->sc, s));
         }
     }
     HIP_OK(e->w_list.reserve((size_t)tot[0] * 16 + 16));

@@ -285,29 +285,58 @@ __device__ __forceinline__ void load16_live(const uint32_t* p, uint64_t base, ui
     }
 }
 
-// Exclusive scan of n <= SMALL_SCAN values in one workgroup of 1024 threads
-// (thread t scans a contiguous run); out[n - 1] closes the scan when the last
-// input is 0.  MODE 1 scans item counts ceil(in[i] / chunk) for i < n - 1 and 0
-// for the last entry (the work-list scan of the traversal).
+// Exclusive scan of n <= SMALL_SCAN values in one workgroup of 1024 threads:
+// wave w scans a contiguous run of `wper` values (a multiple of 256), 256 per
+// step with lane l holding 4 consecutive ones, so every access is coalesced
+// (one workgroup sits on one CU, whose address unit would take a cycle per
+// distinct cache line of a scattered access).  Pass 1 sums each wave's run,
+// the run offsets come from one block-wide sum, pass 2 scans and writes.
+// out[n - 1] closes the scan when the last input is 0.  MODE 1 scans item
+// counts ceil(in[i] / chunk) for i < n - 1 and 0 for the last entry (the
+// work-list scan of the traversal).  Each value is read before it is written
+// by the same lane, so in == out is allowed.
+// MODE 1 divides by multiply-high when `fast` (every count + chunk - 1 within
+// div_hi's bound).
 template <int MODE>
-__global__ void __launch_bounds__(1024) k_scan_block(const uint32_t* in, uint32_t n, uint32_t chunk, uint32_t* out,
-                                                     uint32_t* total) {
+__global__ void __launch_bounds__(1024) k_scan_block(const uint32_t* in, uint32_t n, uint32_t chunk, bool fast,
+                                                     uint32_t* out, uint32_t* total) {
     __shared__ uint32_t red[16];
-    const uint32_t per = (n + 1023u) / 1024u;
-    const uint32_t b = threadIdx.x * per;
-    const uint32_t e = b + per < n ? b + per : n;
+    const uint32_t lane = lane_id(), w = wave_id();
+    const uint32_t wper = ((n + 16u * 256u - 1u) / (16u * 256u)) * 256u;
+    const uint32_t wb = w * wper;
+    const uint32_t we = wb + wper < n ? wb + wper : n;
+    const uint32_t mc = div_hi_magic(chunk);
     auto val = [&](uint32_t i) -> uint32_t {
-        if (MODE == 1) return i + 1u < n ? (in[i] + chunk - 1u) / chunk : 0u;
+        if (i >= we) return 0u;
+        if (MODE == 1) {
+            if (i + 1u >= n) return 0u;
+            const uint32_t x = in[i] + chunk - 1u;
+            return fast ? div_hi(x, chunk, mc) : x / chunk;
+        }
         return in[i];
     };
     uint32_t sum = 0;
-    for (uint32_t i = b; i < e; i++) sum += val(i);
+#pragma unroll 4
+    for (uint32_t i0 = wb + 4u * lane; i0 < we; i0 += 256u)
+        sum += val(i0) + val(i0 + 1u) + val(i0 + 2u) + val(i0 + 3u);
     uint32_t tot;
-    uint32_t run = block_excl_sum(sum, red, &tot);
-    for (uint32_t i = b; i < e; i++) {
-        const uint32_t v = val(i);
-        out[i] = run;
-        run += v;
+    const uint32_t wsum = wave_inclusive_add(sum);       // lane 63: the wave's run total
+    uint32_t carry = block_excl_sum(lane == 63 ? wsum : 0u, red, &tot);
+    carry = __shfl(carry, 63, 64);
+    for (uint32_t s0 = wb; s0 < we; s0 += 256u) {
+        const uint32_t i = s0 + 4u * lane;
+        const uint32_t v0 = val(i), v1 = val(i + 1u), v2 = val(i + 2u), v3 = val(i + 3u);
+        const uint32_t ls = v0 + v1 + v2 + v3;
+        const uint32_t inc = wave_inclusive_add(ls);
+        uint32_t run = carry + inc - ls;
+        if (i < we) out[i] = run;
+        run += v0;
+        if (i + 1u < we) out[i + 1u] = run;
+        run += v1;
+        if (i + 2u < we) out[i + 2u] = run;
+        run += v2;
+        if (i + 3u < we) out[i + 3u] = run;
+        carry += __shfl(inc, 63, 64);
     }
     if (threadIdx.x == 0 && total) *total = tot;
 }
@@ -804,7 +833,7 @@ static hipError_t run_scan(const uint32_t* in, uint32_t* out, uint64_t n, ScanSc
     if (nt > MAX_TILES) return hipErrorInvalidValue;
     if (nt == 0) return hipSuccess;
     if (!inclusive && n <= SMALL_SCAN) {     // one launch instead of three
-        k_scan_block<0><<<1, 1024, 0, s>>>(in, (uint32_t)n, 1u, out, sc.total.as<uint32_t>());
+        k_scan_block<0><<<1, 1024, 0, s>>>(in, (uint32_t)n, 1u, true, out, sc.total.as<uint32_t>());
         return hipGetLastError();
     }
     uint32_t* tiles = sc.tile_counts.as<uint32_t>();
@@ -822,11 +851,12 @@ hipError_t run_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, Sca
     return run_scan(in, out, n, sc, s, false);
 }
 
-hipError_t run_item_scan(const uint32_t* cand, uint32_t ntop, uint32_t chunk, uint32_t* counts, uint32_t* first,
-                         ScanScratch& sc, hipStream_t s) {
+hipError_t run_item_scan(const uint32_t* cand, uint32_t ntop, uint32_t chunk, uint32_t max_cand, uint32_t* counts,
+                         uint32_t* first, ScanScratch& sc, hipStream_t s) {
     if (chunk == 0) return hipErrorInvalidValue;
     if ((uint64_t)ntop + 1 <= SMALL_SCAN) {
-        k_scan_block<1><<<1, 1024, 0, s>>>(cand, ntop + 1, chunk, first, nullptr);
+        const bool fast = div_hi_ok((uint64_t)max_cand + chunk - 1, chunk);
+        k_scan_block<1><<<1, 1024, 0, s>>>(cand, ntop + 1, chunk, fast, first, nullptr);
         return hipGetLastError();
     }
     k_item_counts<<<ntop / 256 + 1, 256, 0, s>>>(cand, chunk, ntop, counts);

@@ -80,15 +80,35 @@ __global__ void __launch_bounds__(256) k_level0(const float4* rec0, const float4
             }
         };
         v3 o, d;
-        fetch(j * leaf, o, d);
-        float4 sphere = f4(o, r0);
-        float4 cone = f4(mode == 0 ? d : normalize(d), s0);
-        for (uint32_t i = 1; i < leaf; i++) {
-            uint32_t p = j * leaf + i;
-            if (p >= N) break;
-            fetch(p, o, d);
-            sphere = sphere_union(sphere, f4(o, r0));
-            cone = cone0(cone, d, s0, mode);
+        float4 sphere, cone;
+        if (FROM_RECORDS && leaf == 8) {
+            // the eight records' loads issued together, then the same fold
+            float4 a[8], b[8];
+#pragma unroll
+            for (uint32_t i = 0; i < 8; i++) {
+                const uint32_t p = j * 8 + i;
+                a[i] = p < N ? rec0[p] : make_float4(0, 0, 0, 0);
+                b[i] = p < N ? rec1[p] : make_float4(0, 0, 0, 0);
+            }
+            sphere = f4(xyz(a[0]), r0);
+            cone = f4(mode == 0 ? xyz(b[0]) : normalize(xyz(b[0])), s0);
+#pragma unroll
+            for (uint32_t i = 1; i < 8; i++) {
+                if (j * 8 + i >= N) break;
+                sphere = sphere_union(sphere, f4(xyz(a[i]), r0));
+                cone = cone0(cone, xyz(b[i]), s0, mode);
+            }
+        } else {
+            fetch(j * leaf, o, d);
+            sphere = f4(o, r0);
+            cone = f4(mode == 0 ? d : normalize(d), s0);
+            for (uint32_t i = 1; i < leaf; i++) {
+                uint32_t p = j * leaf + i;
+                if (p >= N) break;
+                fetch(p, o, d);
+                sphere = sphere_union(sphere, f4(o, r0));
+                cone = cone0(cone, d, s0, mode);
+            }
         }
         nodes[2 * (size_t)j] = sphere;
         nodes[2 * (size_t)j + 1] = cone;
@@ -321,41 +341,51 @@ __global__ void __launch_bounds__(256) k_cull(TravParams P, uint32_t* npass, uin
     unsigned long long* pm = pmask + (size_t)top * words;
     const float rcos = node_rcos(tA.x);
     uint32_t n = 0, cum = 0;
+    if (LIST) {
+        // only the words of the count pass's mask with a passing object
+        for (uint32_t ob = 0; ob < P.K; ob += 64) {
+            const uint64_t m = pm[ob / 64];
+            if (m == 0ull) continue;
+            const uint32_t oi = ob + lane;
+            const bool po = (m >> lane) & 1ull;
+            uint4 r = make_uint4(0u, 0u, 0u, 0u);
+            uint32_t ncl = 0;
+            if (po) {
+                r = P.obj_range[oi];
+                ncl = (r.y - r.x + CLUSTER) / CLUSTER;
+            }
+            const uint32_t incl = wave_incl_add(ncl);
+            if (po) {
+                const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                obj_ent[base + n + rank] = make_uint4(r.x, r.y, P.obj_cl_base[oi], r.z);
+                obj_cum[base + n + rank] = cum + incl - ncl;
+            }
+            n += (uint32_t)__popcll(m);
+            cum += __shfl(incl, 63, 64);
+        }
+        return;
+    }
+    // count pass: per-lane sums, one reduction at the end
     unsigned long long pairs = 0;
     for (uint32_t ob = 0; ob < P.K; ob += 64) {
         const uint32_t oi = ob + lane;
         bool po = false;
-        uint32_t ncl = 0, npair = 0;
-        if (LIST) {
-            po = (pm[ob / 64] >> lane) & 1ull;
-        } else if (oi < P.K) {
-            po = sphere_node_fast(tS, tC, P.obj_sphere[oi], tA.x, tA.y, rcos);
-        }
-        if (oi < P.K) {
-            if (po) {
-                const uint4 r = P.obj_range[oi];
-                ncl = (r.y - r.x + CLUSTER) / CLUSTER;
-                npair = (r.y - r.x + 1) - (r.w - r.z);
-            }
+        if (oi < P.K) po = sphere_node_fast(tS, tC, P.obj_sphere[oi], tA.x, tA.y, rcos);
+        if (po) {
+            const uint4 r = P.obj_range[oi];
+            cum += (r.y - r.x + CLUSTER) / CLUSTER;
+            pairs += (r.y - r.x + 1) - (r.w - r.z);
         }
         const uint64_t m = __ballot(po);
-        if (!LIST && lane == 0) pm[ob / 64] = m;
-        const uint32_t incl = wave_incl_add(ncl);
-        if (LIST && po) {
-            const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-            const uint4 r = P.obj_range[oi];
-            obj_ent[base + n + rank] = make_uint4(r.x, r.y, P.obj_cl_base[oi], r.z);
-            obj_cum[base + n + rank] = cum + incl - ncl;
-        }
+        if (lane == 0) pm[ob / 64] = m;
         n += (uint32_t)__popcll(m);
-        cum += __shfl(incl, 63, 64);
-        pairs += npair;
     }
-    if (!LIST && cs) {
+    for (int dlt = 32; dlt > 0; dlt >>= 1) cum += __shfl_xor(cum, dlt, 64);
+    if (cs) {
         for (int dlt = 32; dlt > 0; dlt >>= 1) pairs += __shfl_xor(pairs, dlt, 64);
         if (lane == 0) atomicAdd(&cs[CNT_TOP_PAIRS], pairs);
     }
-    if (!LIST && lane == 0) { npass[top] = n; cand[top] = cum; }
+    if (lane == 0) { npass[top] = n; cand[top] = cum; }
 }
 
 // Work items per top node for a chunk size; entry n_top is 0.
@@ -375,18 +405,38 @@ __global__ void __launch_bounds__(256) k_item_count(const uint32_t* cand, uint32
 // table sizes a pass and picks its traversal mode.
 constexpr uint32_t TABLE_SLICE = 256;
 __global__ void __launch_bounds__(64) k_item_table(const uint32_t* cand, const float2* top_aux, uint32_t ntop,
-                                                   unsigned long long* table) {
-    __shared__ uint32_t s_cand[TABLE_SLICE];
+                                                   unsigned long long* table, bool fast) {
+    __shared__ uint4 s_cand4[TABLE_SLICE / 4];
+    uint32_t* s_cand = reinterpret_cast<uint32_t*>(s_cand4);
     const uint32_t lane = threadIdx.x & 63u, c = lane + 1u;
     const uint32_t t0 = blockIdx.x * TABLE_SLICE;
     const uint32_t t1 = t0 + TABLE_SLICE < ntop ? t0 + TABLE_SLICE : ntop;
-    for (uint32_t t = t0 + lane; t < t1; t += 64u) s_cand[t - t0] = cand[t];     // coalesced
+    for (uint32_t t = lane; t < TABLE_SLICE; t += 64u) s_cand[t] = t0 + t < t1 ? cand[t0 + t] : 0u;   // coalesced
     __builtin_amdgcn_wave_barrier();
+    // four candidates per LDS read (the zero padding adds nothing)
+    const uint32_t q1 = (t1 - t0 + 3u) / 4u;
+    uint32_t lo = 0, lo2 = 0;                  // per-read partial sums stay below 2^32
     unsigned long long sum = 0ull, sum2 = 0ull;
-    for (uint32_t t = 0; t < t1 - t0; t++) {
-        const uint32_t n = s_cand[t];
-        sum += (n + c - 1u) / c;
-        sum2 += (n + 2u * c - 1u) / (2u * c);
+    if (fast) {       // every n + 2c - 1 within div_hi's bound for 2c <= 128
+        const uint32_t m1 = div_hi_magic(c), m2 = div_hi_magic(2u * c);
+        auto f1 = [&](uint32_t n) { return div_hi(n + c - 1u, c, m1); };
+        auto f2 = [&](uint32_t n) { return div_hi(n + 2u * c - 1u, 2u * c, m2); };
+#pragma unroll 4
+        for (uint32_t q = 0; q < q1; q++) {
+            const uint4 v = s_cand4[q];
+            lo = f1(v.x) + f1(v.y);
+            lo2 = f2(v.x) + f2(v.y);
+            sum += lo; sum2 += lo2;
+            lo = f1(v.z) + f1(v.w);
+            lo2 = f2(v.z) + f2(v.w);
+            sum += lo; sum2 += lo2;
+        }
+    } else {
+        for (uint32_t t = 0; t < t1 - t0; t++) {
+            const uint32_t n = s_cand[t];
+            sum += (n + c - 1u) / c;
+            sum2 += (n + 2u * c - 1u) / (2u * c);
+        }
     }
     if (sum) atomicAdd(&table[c - 1u], sum);
     if (sum2) atomicAdd(&table[ITEM_TABLE + c - 1u], sum2);
@@ -1085,11 +1135,12 @@ void launch_cull_list(const TravParams& p, uint4* obj_ent, uint32_t* obj_cum, un
         k_cull<true><<<(top + 3) / 4, 256, 0, s>>>(p, nullptr, nullptr, obj_ent, obj_cum, pmask);
 }
 
-void launch_item_table(const uint32_t* cand, const float2* top_aux, uint32_t ntop, unsigned long long* table,
-                       hipStream_t s) {
+void launch_item_table(const uint32_t* cand, const float2* top_aux, uint32_t ntop, uint32_t max_cand,
+                       unsigned long long* table, hipStream_t s) {
     static_assert(ITEM_TABLE == 64, "one lane per chunk size");
     launch_fill_u64(table, 0ull, 2 * ITEM_TABLE + 1, s);
-    if (ntop) k_item_table<<<(ntop + TABLE_SLICE - 1) / TABLE_SLICE, 64, 0, s>>>(cand, top_aux, ntop, table);
+    const bool fast = div_hi_ok((uint64_t)max_cand + 2 * ITEM_TABLE - 1, 2 * ITEM_TABLE);
+    if (ntop) k_item_table<<<(ntop + TABLE_SLICE - 1) / TABLE_SLICE, 64, 0, s>>>(cand, top_aux, ntop, table, fast);
 }
 
 void launch_item_count(const uint32_t* cand, uint32_t chunk, uint32_t ntop, uint32_t* nitems, hipStream_t s) {

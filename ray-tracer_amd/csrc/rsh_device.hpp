@@ -173,6 +173,23 @@ __device__ __forceinline__ float atan2_diag(float v) {
     return (__float_as_uint(v) >> 31) ? -3.1415927410125732f : 0.0f;
 }
 
+// ------------------------------------------------------ integer division --
+// Division by a divisor c fixed over many numerators: floor(x / c) is the high
+// word of x * m, m = ceil(2^32 / c) (c >= 2).  With m c = 2^32 + e, 0 <= e < c,
+// and x = q c + r: x m / 2^32 = q + r / c + x e / (c 2^32), whose floor is q
+// while r + x e / 2^32 < c, which x (c - 1) < 2^32 guarantees.  Callers check
+// that bound for their largest numerator once, on the host (div_hi_ok), and
+// pass the verdict to the kernel, so no lane takes a per-value branch.
+__host__ __device__ inline bool div_hi_ok(uint64_t max_x, uint32_t c) {
+    return c <= 1u || max_x <= 0xFFFFFFFFull / (c - 1u);
+}
+__device__ __forceinline__ uint32_t div_hi_magic(uint32_t c) {
+    return c <= 1u ? 0u : (uint32_t)((0x100000000ull + c - 1ull) / c);
+}
+__device__ __forceinline__ uint32_t div_hi(uint32_t x, uint32_t c, uint32_t m) {
+    return c == 1u ? x : __umulhi(x, m);
+}
+
 // ------------------------------------------------------------------ keys --
 // RayTracer.cu:147-168 CreateShadowRayIndex
 __device__ inline uint32_t shadow_key_ref(v3 d, uint32_t l) {

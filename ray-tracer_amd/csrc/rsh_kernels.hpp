@@ -164,8 +164,9 @@ void launch_cull_list(const TravParams& p, uint4* obj_ent, uint32_t* obj_cum, un
 // with a wide cone (spread tangent outside [0, WIDE_TAN]).
 constexpr uint32_t ITEM_TABLE = 64;
 constexpr float WIDE_TAN = 0.1f;
-void launch_item_table(const uint32_t* cand, const float2* top_aux, uint32_t ntop, unsigned long long* table,
-                       hipStream_t s);
+// max_cand: an upper bound of every cand entry (the scene's cluster count)
+void launch_item_table(const uint32_t* cand, const float2* top_aux, uint32_t ntop, uint32_t max_cand,
+                       unsigned long long* table, hipStream_t s);
 void launch_item_count(const uint32_t* cand, uint32_t chunk, uint32_t ntop, uint32_t* nitems, hipStream_t s);
 void launch_item_desc(const TravParams& p, uint4* desc, hipStream_t s);
 

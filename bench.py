@@ -19,8 +19,10 @@ Beside the headline it measures, on the same GPU:
 
 Prints one JSON line.  "roofline" describes the dominant kernel, the fused
 traversal (k_traverse64, one launch per pass): its launch durations come from
-HIP events recorded on the engine's stream around every launch of the timed
-frames; its algorithmic FLOPs from one extra frame with the traversal
+HIP events recorded around every traversal launch of three frames rendered
+with serial passes (the timed frames run the shadow passes concurrently with
+the reflection chain, so their launches share the GPU); its algorithmic FLOPs
+from one extra frame with the traversal
 counters on (pair and test counts are deterministic); its HBM traffic from the
 rocprofv3 FETCH_SIZE / WRITE_SIZE summary committed under profiles/.
 "cpu_baseline" is the C++ oracle on a band of rows of the same frame.
@@ -189,7 +191,7 @@ def batching_leg(rsh_amd, eng, scene2):
         parts = eng.split_views(eng.pixels(), views)
         same = sum(int((a == b).sum()) for a, b in zip(parts, imgs)) / out["pixels"]
         graph_ms, brays = autotune.time_batch(eng, views, cfg, reps=3)
-        st = eng.render_views(views, cfg, flags=eng.TIMING)
+        st = eng.render_views(views, cfg, flags=eng.TIMING | eng.SERIAL)
         stages = {n: st.stage_ms[i] for i, n in enumerate(rsh_amd.STAGES)}
         bp = [{"slots": st.pass_slots[i], "rays": st.pass_rays[i], "chunks": st.pass_chunks[i]}
               for i in range(st.pass_count)]
@@ -229,21 +231,26 @@ def main():
     t_all = time.perf_counter()
     for _ in range(a.steps):
         t0 = time.perf_counter()
-        # HIP events around each traversal launch only (the roofline's kernel time)
-        st = eng.render(cam, cfg, flags=eng.TRAV_TIMING)
+        # the frame as served: shadow passes concurrent with the reflection chain
+        st = eng.render(cam, cfg)
         times.append(time.perf_counter() - t0)
         rays.append(st.rays_total)
+    rsh_amd.lib().rsh_device_synchronize()
+    total = time.perf_counter() - t_all
+    value = sum(rays) / total / 1e6
+    # The traversal kernel's launch time (the roofline): HIP events around each
+    # traversal launch only, in frames whose passes run one at a time, so no
+    # other kernel shares the GPU with the timed launch.
+    for _ in range(3):
+        st = eng.render(cam, cfg, flags=eng.TRAV_TIMING | eng.SERIAL)
         for i in range(st.pass_count):
             if st.pass_rays[i]:
                 trav_ms += st.pass_traverse_ms[i]
                 trav_launches += 1
-    rsh_amd.lib().rsh_device_synchronize()
-    total = time.perf_counter() - t_all
-    value = sum(rays) / total / 1e6
     # Stage breakdown: events around every stage (their own overhead stays out
-    # of the timed frames).
+    # of the timed frames), serial passes as well.
     for _ in range(3):
-        st = eng.render(cam, cfg, flags=eng.TIMING)
+        st = eng.render(cam, cfg, flags=eng.TIMING | eng.SERIAL)
         for i in range(len(stage_sum)):
             stage_sum[i] += st.stage_ms[i]
     stages = {name: stage_sum[i] / 3 for i, name in enumerate(rsh_amd.STAGES)}

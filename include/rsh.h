@@ -144,12 +144,18 @@ rsh_status rsh_engine_set_stream(rsh_engine* e, void* hip_stream);
 /* Upload a scene (host arrays) and precompute per-triangle data. */
 rsh_status rsh_engine_load_scene(rsh_engine* e, const rsh_scene_desc* scene);
 
-/* Render one frame into the engine's pixel buffer.  `flags`:
+/* Render one frame into the engine's pixel buffer.  The shadow passes run on
+ * a second stream of the engine (and, eagerly, a host thread of their own),
+ * concurrently with the primary / reflection passes on the engine's stream;
+ * the call returns when both are done.  `flags`:
  *   bit 0  keep every pass's per-slot hit records for rsh_engine_read_pass
  *   bit 1  collect traversal counters (pairs / tests) into the stats
  *   bit 2  time stages with HIP events into stats->stage_ms
  *   bit 4  time only the traversal kernel launches (stats->pass_traverse_ms and
  *          stage_ms[6]), two events per pass instead of two per stage
+ *   bit 5  serial passes: each shadow pass is issued as soon as its hits are,
+ *          one pass at a time (the timings of bits 2 and 4 are then free of
+ *          the other chain's kernels)
  *   bit 3  graph replay (alone, without bits 0-2): the first call for a given
  *          set of views, configuration and scene renders eagerly and captures
  *          a HIP graph of the frame (every pass's sizes fixed to the eager

@@ -15,9 +15,12 @@
 //    arrays;
 //  * the OpenGL G-buffer is produced by a primary-ray pass through the same
 //    pipeline.
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/rsh.h"
@@ -25,9 +28,42 @@
 
 using namespace rsh;
 
+// Graph replay (render flag bit 3).  An eager frame records every pass's
+// sizes (the values the host otherwise reads back); a later run replays
+// those sizes without read-backs or synchronisation under stream capture.
+struct PassPlan { uint32_t N = 0, C = 0, chunk = 0, list_total = 0, items = 0, mode = 0; };
+
+// One pass pipeline's stream and scratch.  A frame runs its shadow passes on a
+// second lane, concurrently with the primary / reflection chain of the first,
+// so one chain's latency-bound steps (read-backs, one-workgroup scans, the
+// tail of a traversal launch) overlap the other's work.
+struct Lane {
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+    size_t cap_slots = 0;
+    DevBuf flags, keys, values, tvalues, bases, ckeys, cvalues, skeys, svalues;
+    DevBuf rvalues, soff, rec0, rec1, nodes, aux, counters, dcounts;
+    uint32_t* hstage = nullptr;        // page-locked host words for the per-pass read-backs
+    // work list of the 64-ray traversal kernel
+    DevBuf w_npass, w_cand, w_nitems, w_off, w_first, w_list, w_cum, w_items, w_table, w_pmask;
+    ScanScratch sc;
+    std::vector<PassPlan> plan;        // sizes of this lane's passes (recorded / replayed in order)
+    size_t plan_pos = 0;
+    // timing (flag bits 2 and 4)
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> timed;
+    size_t ev_used = 0;
+    std::vector<DevBuf*> bufs() {
+        return {&flags, &keys, &values, &tvalues, &bases, &ckeys, &cvalues, &skeys, &svalues, &rvalues, &soff,
+                &rec0, &rec1, &nodes, &aux, &counters, &dcounts, &w_npass, &w_cand, &w_nitems, &w_off, &w_first,
+                &w_list, &w_cum, &w_items, &w_table, &w_pmask};
+    }
+};
+
 struct rsh_engine {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;      // the caller's stream (lane 0)
     bool own_stream = false;
     std::string err;
 
@@ -42,42 +78,28 @@ struct rsh_engine {
     std::vector<float> host_spheres;   // float3[2K]
     uint32_t ranges_tb = 0xFFFFFFFFu;  // triangle batch the exclusion ranges were built for
 
-    // frame buffers
+    // frame buffers: closest hits and the bounce G-buffer alternate between
+    // consecutive reflection passes (a pass's shading reads them while the
+    // next pass traces)
     uint32_t W = 0, H = 0, Lmax = 1;
     size_t cap_slots = 0;
-    DevBuf rays_a, rays_b, flags, keys, values, tvalues, bases, ckeys, cvalues, skeys, svalues;
-    DevBuf rvalues, soff, rec0, rec1, nodes, aux;
-    DevBuf closest, shadow, pixels, counters, dcounts;
-    uint32_t* hstage = nullptr;        // page-locked host words for the per-pass read-backs
-    // work list of the 64-ray traversal kernel
-    DevBuf w_npass, w_cand, w_nitems, w_off, w_first, w_list, w_cum, w_items, w_table, w_pmask;
+    DevBuf rays_a, rays_b, closest[2], shadow, pixels;
     DevBuf views_d;                    // view table of a batch (more than one view)
     uint64_t scene_epoch = 0;          // counts scene loads
+    Lane lane[2];
+    std::vector<hipEvent_t> sync_ev;   // cross-lane dependencies of a frame
 
-    // Graph replay (render flag bit 3).  An eager frame records every pass's
-    // sizes (the values the host otherwise reads back); a second run replays
-    // those sizes without read-backs or synchronisation under stream capture.
-    struct PassPlan { uint32_t N = 0, C = 0, chunk = 0, list_total = 0, items = 0, mode = 0; };
-    std::vector<PassPlan> plan;
     bool recording = false, replaying = false;
-    size_t plan_pos = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     std::vector<uint8_t> gsig;         // views + config + scene epoch the graph was captured for
     uint64_t gepoch = 0;               // allocation epoch at capture
     rsh_frame_stats gstats{};          // statistics of the captured frame
-    DevBuf g0[4], gb[4];
-    ScanScratch sc;
+    DevBuf g0[4], gb[2][4];
 
-    // kept passes (flag bit 0)
-    struct Kept { DevBuf buf; uint32_t kind; uint32_t slots; };
+    // kept passes (flag bit 0), by pass index
+    struct Kept { DevBuf buf; uint32_t kind = 0; uint32_t slots = 0; bool set = false; };
     std::vector<Kept> kept;
-
-    // timing (flag bit 2)
-    std::vector<hipEvent_t> ev_pool;
-    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> timed;
-    size_t ev_used = 0;
-    uint32_t pass_index = 0;   // passes traced so far in the current frame
 };
 
 namespace {
@@ -92,6 +114,17 @@ bool fail(rsh_engine* e, const std::string& msg) {
         hipError_t _e = (call);                                                            \
         if (_e != hipSuccess) {                                                            \
             e->err = std::string(#call) + ": " + hipGetErrorString(_e);                    \
+            return RSH_HIP_ERROR;                                                          \
+        }                                                                                  \
+    } while (0)
+
+// The same inside a lane's pipeline: the error goes to the lane (a frame's two
+// lanes may fail on their own threads) and the frame copies it out.
+#define LANE_OK(call)                                                                      \
+    do {                                                                                   \
+        hipError_t _e = (call);                                                            \
+        if (_e != hipSuccess) {                                                            \
+            L.err = std::string(#call) + ": " + hipGetErrorString(_e);                     \
             return RSH_HIP_ERROR;                                                          \
         }                                                                                  \
     } while (0)
@@ -111,27 +144,27 @@ SceneView scene_view(rsh_engine* e) {
 }
 
 struct StageTimer {
-    rsh_engine* e;
+    Lane& L;
     int stage;
     bool on;
     hipEvent_t a = nullptr, b = nullptr;
-    StageTimer(rsh_engine* en, int st, bool enabled) : e(en), stage(st), on(enabled) {
+    StageTimer(Lane& ln, int st, bool enabled) : L(ln), stage(st), on(enabled) {
         if (!on) return;
-        if (e->ev_used + 2 > e->ev_pool.size()) {
+        if (L.ev_used + 2 > L.ev_pool.size()) {
             for (int i = 0; i < 64; i++) {
                 hipEvent_t ev;
-                if (hipEventCreate(&ev) == hipSuccess) e->ev_pool.push_back(ev);
+                if (hipEventCreate(&ev) == hipSuccess) L.ev_pool.push_back(ev);
             }
         }
-        if (e->ev_used + 2 > e->ev_pool.size()) { on = false; return; }
-        a = e->ev_pool[e->ev_used++];
-        b = e->ev_pool[e->ev_used++];
-        (void)hipEventRecord(a, e->stream);
+        if (L.ev_used + 2 > L.ev_pool.size()) { on = false; return; }
+        a = L.ev_pool[L.ev_used++];
+        b = L.ev_pool[L.ev_used++];
+        (void)hipEventRecord(a, L.stream);
     }
     ~StageTimer() {
         if (!on) return;
-        (void)hipEventRecord(b, e->stream);
-        e->timed.push_back({stage, {a, b}});
+        (void)hipEventRecord(b, L.stream);
+        L.timed.push_back({stage, {a, b}});
     }
 };
 
@@ -150,24 +183,67 @@ constexpr size_t HSTAGE_WORDS = 2 + 2 * (2 * ITEM_TABLE + 1);   // list / item t
 constexpr uint32_t ITEM_CHUNK_MIN = 4;
 constexpr uint32_t ITEM_TARGET = 131072;
 
+// Lane 1's stream and page-locked words, created on first use; lane 0 runs
+// on the engine's stream.
+rsh_status ensure_lanes(rsh_engine* e) {
+    e->lane[0].stream = e->stream;
+    Lane& L = e->lane[1];
+    if (!L.stream) {
+        HIP_OK(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+        L.own_stream = true;
+    }
+    for (Lane& l : e->lane)
+        if (!l.hstage) HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&l.hstage), HSTAGE_WORDS * 4, hipHostMallocDefault));
+    while (e->sync_ev.size() < 2 * (RSH_MAX_PASSES + 2)) {
+        hipEvent_t ev;
+        HIP_OK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        e->sync_ev.push_back(ev);
+    }
+    return RSH_OK;
+}
+
+rsh_status ensure_lane(rsh_engine* e, Lane& L, size_t S) {
+    if (S <= L.cap_slots) return RSH_OK;
+    const size_t u = S * 4;
+    for (DevBuf* b : {&L.flags, &L.keys, &L.values, &L.tvalues, &L.ckeys, &L.cvalues, &L.skeys, &L.svalues,
+                      &L.rvalues})
+        HIP_OK(b->reserve(u));
+    HIP_OK(L.bases.reserve(u + 4));   // bases[C] = N closes the last chunk
+    HIP_OK(L.soff.reserve(u + 4));
+    HIP_OK(L.rec0.reserve(S * 16)); HIP_OK(L.rec1.reserve(S * 16));
+    HIP_OK(L.counters.reserve(COUNTER_SLOTS * COUNTERS_PER_SLOT * 8)); HIP_OK(L.dcounts.reserve(64));
+    HIP_OK(L.sc.reserve(S));
+    L.cap_slots = S;
+    return RSH_OK;
+}
+
 rsh_status ensure_frame(rsh_engine* e, uint32_t W, uint32_t H, uint32_t Lmax) {
     size_t S = (size_t)W * H * Lmax, P = (size_t)W * H;
     e->W = W; e->H = H; e->Lmax = Lmax;
+    rsh_status rs = ensure_lanes(e);
+    if (rs != RSH_OK) return rs;
+    for (Lane& L : e->lane)
+        if ((rs = ensure_lane(e, L, S)) != RSH_OK) return rs;
     if (S <= e->cap_slots) return RSH_OK;
-    size_t u = S * 4;
     HIP_OK(e->rays_a.reserve(S * 24)); HIP_OK(e->rays_b.reserve(S * 24));
-    for (DevBuf* b : {&e->flags, &e->keys, &e->values, &e->tvalues, &e->ckeys, &e->cvalues, &e->skeys, &e->svalues,
-                      &e->rvalues, &e->shadow})
-        HIP_OK(b->reserve(u));
-    HIP_OK(e->bases.reserve(u + 4));   // bases[C] = N closes the last chunk
-    HIP_OK(e->soff.reserve(u + 4));
-    HIP_OK(e->closest.reserve(S * 8));
-    HIP_OK(e->rec0.reserve(S * 16)); HIP_OK(e->rec1.reserve(S * 16));
+    HIP_OK(e->shadow.reserve(S * 4));
+    HIP_OK(e->closest[0].reserve(S * 8)); HIP_OK(e->closest[1].reserve(S * 8));
     HIP_OK(e->pixels.reserve(P * 4));
-    for (int i = 0; i < 4; i++) { HIP_OK(e->g0[i].reserve(P * 16)); HIP_OK(e->gb[i].reserve(P * 16)); }
-    HIP_OK(e->counters.reserve(COUNTER_SLOTS * COUNTERS_PER_SLOT * 8)); HIP_OK(e->dcounts.reserve(64));
-    HIP_OK(e->sc.reserve(S));
+    for (int i = 0; i < 4; i++) {
+        HIP_OK(e->g0[i].reserve(P * 16)); HIP_OK(e->gb[0][i].reserve(P * 16)); HIP_OK(e->gb[1][i].reserve(P * 16));
+    }
     e->cap_slots = S;
+    return RSH_OK;
+}
+
+// Hierarchy levels of N rays on both lanes.
+rsh_status ensure_nodes(rsh_engine* e, uint32_t N, const rsh_frame_config& cfg) {
+    uint32_t cnt[4], off[4];
+    const size_t total = level_sizes(N, cfg.leaf, cfg.branch, cfg.depth, cnt, off);
+    for (Lane& L : e->lane) {
+        HIP_OK(L.nodes.reserve(total * 32 + 64));
+        HIP_OK(L.aux.reserve(total * 8 + 64));
+    }
     return RSH_OK;
 }
 
@@ -214,24 +290,24 @@ struct PassOut {
 };
 
 // Work list of the 64-ray kernel (see uses_items): cull, size, scan, fill.
-rsh_status build_items(rsh_engine* e, const rsh_frame_config& cfg, TravParams& p, PassOut& out) {
-    hipStream_t s = e->stream;
+rsh_status build_items(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, TravParams& p, PassOut& out) {
+    hipStream_t s = L.stream;
     const uint32_t ntop = p.level_count[1];
     const size_t nb = ((size_t)ntop + 1) * 4;
-    for (DevBuf* b : {&e->w_npass, &e->w_cand, &e->w_nitems, &e->w_off, &e->w_first}) HIP_OK(b->reserve(nb));
-    uint32_t* npass = u32p(e->w_npass);
-    uint32_t* cand = u32p(e->w_cand);
-    uint32_t* nitems = u32p(e->w_nitems);
-    HIP_OK(e->w_pmask.reserve(((size_t)ntop + 1) * ((e->K + 63) / 64) * 8 + 8));
-    unsigned long long* pmask = e->w_pmask.as<unsigned long long>();
+    for (DevBuf* b : {&L.w_npass, &L.w_cand, &L.w_nitems, &L.w_off, &L.w_first}) LANE_OK(b->reserve(nb));
+    uint32_t* npass = u32p(L.w_npass);
+    uint32_t* cand = u32p(L.w_cand);
+    uint32_t* nitems = u32p(L.w_nitems);
+    LANE_OK(L.w_pmask.reserve(((size_t)ntop + 1) * ((e->K + 63) / 64) * 8 + 8));
+    unsigned long long* pmask = L.w_pmask.as<unsigned long long>();
     launch_cull_count(p, npass, cand, pmask, s);
-    if (p.algorithm != 1) HIP_OK(run_exclusive_scan(npass, u32p(e->w_off), (uint64_t)ntop + 1, e->sc, s));
+    if (p.algorithm != 1) LANE_OK(run_exclusive_scan(npass, u32p(L.w_off), (uint64_t)ntop + 1, L.sc, s));
     uint32_t chunk = cfg.traversal_chunk ? cfg.traversal_chunk : ITEM_CHUNK;
     uint32_t tot[2] = {0, 0};   // list entries, items
     if (e->replaying) {
-        const rsh_engine::PassPlan& pp = e->plan[e->plan_pos - 1];
+        const PassPlan& pp = L.plan[L.plan_pos - 1];
         chunk = pp.chunk; tot[0] = pp.list_total; tot[1] = pp.items; p.prefilter = pp.mode;
-        HIP_OK(run_item_scan(cand, ntop, chunk, e->NC, nitems, u32p(e->w_first), e->sc, s));
+        LANE_OK(run_item_scan(cand, ntop, chunk, e->NC, nitems, u32p(L.w_first), L.sc, s));
     }
     if (!e->replaying) {
         // One read-back sizes the pass: the list total and, from the table of
@@ -240,19 +316,19 @@ rsh_status build_items(rsh_engine* e, const rsh_frame_config& cfg, TravParams& p
         const bool tabled = chunk <= ITEM_TABLE || (chunk <= 2 * ITEM_TABLE && chunk % 2 == 0);
         unsigned long long ht[2 * ITEM_TABLE + 1] = {};
         auto items_at = [&](uint32_t c) { return c <= ITEM_TABLE ? ht[c - 1] : ht[ITEM_TABLE + c / 2 - 1]; };
-        uint32_t* first = u32p(e->w_first);
-        HIP_OK(e->w_table.reserve((2 * ITEM_TABLE + 1) * sizeof(unsigned long long)));
-        launch_item_table(cand, p.aux + p.level_offset[1], ntop, e->NC, e->w_table.as<unsigned long long>(), s);
-        HIP_OK(hipMemcpyAsync(e->hstage + 2, e->w_table.p, sizeof(ht), hipMemcpyDeviceToHost, s));
+        uint32_t* first = u32p(L.w_first);
+        LANE_OK(L.w_table.reserve((2 * ITEM_TABLE + 1) * sizeof(unsigned long long)));
+        launch_item_table(cand, p.aux + p.level_offset[1], ntop, e->NC, L.w_table.as<unsigned long long>(), s);
+        LANE_OK(hipMemcpyAsync(L.hstage + 2, L.w_table.p, sizeof(ht), hipMemcpyDeviceToHost, s));
         if (!tabled) {
-            HIP_OK(run_item_scan(cand, ntop, chunk, e->NC, nitems, first, e->sc, s));
-            HIP_OK(hipMemcpyAsync(e->hstage + 1, first + ntop, 4, hipMemcpyDeviceToHost, s));
+            LANE_OK(run_item_scan(cand, ntop, chunk, e->NC, nitems, first, L.sc, s));
+            LANE_OK(hipMemcpyAsync(L.hstage + 1, first + ntop, 4, hipMemcpyDeviceToHost, s));
         }
-        if (p.algorithm != 1) HIP_OK(hipMemcpyAsync(e->hstage, u32p(e->w_off) + ntop, 4, hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
-        if (p.algorithm != 1) tot[0] = e->hstage[0];
-        if (!tabled) tot[1] = e->hstage[1];
-        std::memcpy(ht, e->hstage + 2, sizeof(ht));
+        if (p.algorithm != 1) LANE_OK(hipMemcpyAsync(L.hstage, u32p(L.w_off) + ntop, 4, hipMemcpyDeviceToHost, s));
+        LANE_OK(hipStreamSynchronize(s));
+        if (p.algorithm != 1) tot[0] = L.hstage[0];
+        if (!tabled) tot[1] = L.hstage[1];
+        std::memcpy(ht, L.hstage + 2, sizeof(ht));
         // Traversal mode: the cluster ray masks pay when many top nodes are wide
         // (spread-out secondary rays) and cost where rays are coherent.  More than
         // a sixteenth of the nodes with tan(spread) outside [0, WIDE_TAN] picks the
@@ -270,27 +346,27 @@ rsh_status build_items(rsh_engine* e, const rsh_frame_config& cfg, TravParams& p
                 }
             }
             if (items_at(chunk) > MAX_TRAVERSE_ITEMS) {
-                e->err = "traversal work list of " + std::to_string(items_at(chunk)) + " items exceeds " +
+                L.err = "traversal work list of " + std::to_string(items_at(chunk)) + " items exceeds " +
                          std::to_string(MAX_TRAVERSE_ITEMS) + " (raise traversal_chunk)";
                 return RSH_CAPACITY;
             }
             tot[1] = (uint32_t)items_at(chunk);
-            HIP_OK(run_item_scan(cand, ntop, chunk, e->NC, nitems, first, e->sc, s));
+            LANE_OK(run_item_scan(cand, ntop, chunk, e->NC, nitems, first, L.sc, s));
         }
     }
-    HIP_OK(e->w_list.reserve((size_t)tot[0] * 16 + 16));
-    HIP_OK(e->w_cum.reserve((size_t)tot[0] * 4 + 4));
-    HIP_OK(e->w_items.reserve((size_t)tot[1] * 16 + 16));
-    p.item_desc = e->w_items.as<uint4>(); p.item_first = u32p(e->w_first); p.cand = cand;
-    p.obj_off = u32p(e->w_off); p.obj_ent = e->w_list.as<uint4>(); p.obj_cum = u32p(e->w_cum);
+    LANE_OK(L.w_list.reserve((size_t)tot[0] * 16 + 16));
+    LANE_OK(L.w_cum.reserve((size_t)tot[0] * 4 + 4));
+    LANE_OK(L.w_items.reserve((size_t)tot[1] * 16 + 16));
+    p.item_desc = L.w_items.as<uint4>(); p.item_first = u32p(L.w_first); p.cand = cand;
+    p.obj_off = u32p(L.w_off); p.obj_ent = L.w_list.as<uint4>(); p.obj_cum = u32p(L.w_cum);
     if (p.algorithm == 1) { p.obj_ent = e->all_list.as<uint4>(); p.obj_cum = u32p(e->cl_base); }
     p.chunk = chunk; p.items = tot[1];
-    launch_cull_list(p, e->w_list.as<uint4>(), u32p(e->w_cum), pmask, s);     // writes at p.obj_off
-    launch_item_desc(p, e->w_items.as<uint4>(), s);
-    HIP_OK(hipGetLastError());
+    launch_cull_list(p, L.w_list.as<uint4>(), u32p(L.w_cum), pmask, s);     // writes at p.obj_off
+    launch_item_desc(p, L.w_items.as<uint4>(), s);
+    LANE_OK(hipGetLastError());
     out.items = tot[1]; out.chunk = chunk; out.mode = p.prefilter;
     if (e->recording) {
-        rsh_engine::PassPlan& pp = e->plan.back();
+        PassPlan& pp = L.plan.back();
         pp.chunk = chunk; pp.list_total = tot[0]; pp.items = tot[1]; pp.mode = p.prefilter;
     }
     return RSH_OK;
@@ -298,93 +374,94 @@ rsh_status build_items(rsh_engine* e, const rsh_frame_config& cfg, TravParams& p
 
 // One ray batch: trim -> (compress -> sort -> decompress) -> gather ->
 // hierarchy -> fused traversal + intersection.  kind 0: closest hit into
-// e->closest; kind 1: shadow flags into e->shadow.
-rsh_status trace_pass(rsh_engine* e, const rsh_frame_config& cfg, const float* rays, uint32_t S, uint32_t kind,
-                      const float4* shadow_pos, uint32_t flags_mask, PassOut& out, const float* tmax = nullptr) {
+// closest; kind 1: shadow flags into e->shadow.
+rsh_status trace_pass(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, const float* rays, uint32_t S,
+                      uint32_t kind, const float4* shadow_pos, uint32_t flags_mask, PassOut& out, uint32_t pass_idx,
+                      DevBuf& closest, const float* tmax = nullptr) {
     out = PassOut{};
-    hipStream_t s = e->stream;
+    hipStream_t s = L.stream;
     const bool timing = flags_mask & 4u, counting = flags_mask & 2u;
     const bool trav_timing = timing || (flags_mask & 16u);     // bit 4: traversal launches only
-    uint32_t* dc = u32p(e->dcounts);
+    uint32_t* dc = u32p(L.dcounts);
     {
         // trimming and compression in one walk over the slots (the compress
         // stage time is part of the trim stage's)
-        StageTimer t(e, ST_TRIM, timing);
+        StageTimer t(L, ST_TRIM, timing);
         const unsigned long long miss = cfg.hit_mode == 0 ? MISS64 : (unsigned long long)MISS32;
-        HIP_OK(run_pack(u32p(e->flags), u32p(e->keys), u32p(e->values), S, u32p(e->tvalues), u32p(e->bases),
-                        u32p(e->ckeys), u32p(e->cvalues), dc, kind == 0 ? u64p(e->closest) : nullptr, miss,
-                        kind == 1 ? u32p(e->shadow) : nullptr, e->sc, s));
+        LANE_OK(run_pack(u32p(L.flags), u32p(L.keys), u32p(L.values), S, u32p(L.tvalues), u32p(L.bases),
+                        u32p(L.ckeys), u32p(L.cvalues), dc, kind == 0 ? u64p(closest) : nullptr, miss,
+                        kind == 1 ? u32p(e->shadow) : nullptr, L.sc, s));
     }
     uint32_t hc[2] = {0, 0};
     if (e->replaying) {
-        const rsh_engine::PassPlan& pp = e->plan[e->plan_pos++];
+        const PassPlan& pp = L.plan[L.plan_pos++];
         hc[0] = pp.N; hc[1] = pp.C;
     } else {
-        HIP_OK(hipMemcpyAsync(e->hstage, dc, 8, hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
-        hc[0] = e->hstage[0]; hc[1] = e->hstage[1];
-        if (e->recording) { e->plan.emplace_back(); e->plan.back().N = hc[0]; e->plan.back().C = hc[1]; }
+        LANE_OK(hipMemcpyAsync(L.hstage, dc, 8, hipMemcpyDeviceToHost, s));
+        LANE_OK(hipStreamSynchronize(s));
+        hc[0] = L.hstage[0]; hc[1] = L.hstage[1];
+        if (e->recording) { L.plan.emplace_back(); L.plan.back().N = hc[0]; L.plan.back().C = hc[1]; }
     }
     const uint32_t N = hc[0];
     out.N = N;
     if (N == 0) return RSH_OK;
-    const uint32_t* sorted_values = u32p(e->tvalues);
+    const uint32_t* sorted_values = u32p(L.tvalues);
     if (cfg.algorithm == RSH_CRSH) {
         const uint32_t C = hc[1];
         out.C = C;
         {
-            StageTimer t(e, ST_SORT, timing);
-            HIP_OK(run_radix_sort(u32p(e->ckeys), u32p(e->cvalues), C, key_bits(cfg), u32p(e->skeys),
-                                  u32p(e->svalues), e->sc, s));
+            StageTimer t(L, ST_SORT, timing);
+            LANE_OK(run_radix_sort(u32p(L.ckeys), u32p(L.cvalues), C, key_bits(cfg), u32p(L.skeys),
+                                  u32p(L.svalues), L.sc, s));
         }
         {
-            StageTimer t(e, ST_DECOMPRESS, timing);
+            StageTimer t(L, ST_DECOMPRESS, timing);
             // the sorted keys are not needed after the sort: their buffer holds the chunk bases
-            HIP_OK(run_unpack(u32p(e->bases), u32p(e->svalues), u32p(e->tvalues), C, N, u32p(e->soff),
-                              u32p(e->skeys), u32p(e->rvalues), e->sc, s));
+            LANE_OK(run_unpack(u32p(L.bases), u32p(L.svalues), u32p(L.tvalues), C, N, u32p(L.soff),
+                              u32p(L.skeys), u32p(L.rvalues), L.sc, s));
         }
-        sorted_values = u32p(e->rvalues);
+        sorted_values = u32p(L.rvalues);
     } else {
         out.C = N;
     }
     uint32_t counts[4] = {0, 0, 0, 0}, offs[4] = {0, 0, 0, 0};
     level_sizes(N, cfg.leaf, cfg.branch, cfg.depth, counts, offs);
     {
-        StageTimer t(e, ST_HIER, timing);
+        StageTimer t(L, ST_HIER, timing);
         launch_gather(rays, sorted_values, N, kind == 1 ? shadow_pos : nullptr, kind == 1 ? tmax : nullptr, e->W,
-                      e->Lmax, f4p(e->rec0),
-                      f4p(e->rec1), s);
-        launch_hierarchy(f4p(e->rec0), f4p(e->rec1), nullptr, nullptr, N, cfg.leaf, cfg.branch, cfg.depth,
-                         cfg.initial_radius, cfg.initial_spread, cfg.cone_mode, f4p(e->nodes), e->aux.as<float2>(),
+                      e->Lmax, f4p(L.rec0),
+                      f4p(L.rec1), s);
+        launch_hierarchy(f4p(L.rec0), f4p(L.rec1), nullptr, nullptr, N, cfg.leaf, cfg.branch, cfg.depth,
+                         cfg.initial_radius, cfg.initial_spread, cfg.cone_mode, f4p(L.nodes), L.aux.as<float2>(),
                          counts, offs, s);
-        HIP_OK(hipGetLastError());
+        LANE_OK(hipGetLastError());
     }
     TravParams p{};
-    p.nodes = f4p(e->nodes); p.aux = e->aux.as<float2>();
+    p.nodes = f4p(L.nodes); p.aux = L.aux.as<float2>();
     for (int k = 0; k < 4; k++) { p.level_offset[k] = offs[k]; p.level_count[k] = counts[k]; }
     p.depth = cfg.depth; p.leaf = cfg.leaf; p.branch = cfg.branch;
     uint32_t rpt = cfg.leaf;
     for (uint32_t k = 1; k < cfg.depth; k++) rpt *= cfg.branch;
     p.rays_per_top = rpt;
     p.waves_per_top = (rpt + 63) / 64;
-    p.rec0 = f4p(e->rec0); p.rec1 = f4p(e->rec1); p.N = N;
+    p.rec0 = f4p(L.rec0); p.rec1 = f4p(L.rec1); p.N = N;
     p.tri_sphere = f4p(e->tri_sphere); p.tri_mt = f4p(e->tri_mt); p.T = e->T;
     p.obj_sphere = f4p(e->obj_sphere); p.obj_range = e->obj_range.as<uint4>(); p.K = e->K;
     p.cl_sphere = f4p(e->cl_sphere); p.sub_sphere = f4p(e->sub_sphere); p.sub_use = e->sub_use.as<uint8_t>(); p.obj_cl_base = u32p(e->cl_base); p.n_clusters = e->NC;
     p.algorithm = cfg.algorithm; p.kind = kind; p.hit_mode = cfg.hit_mode;
-    p.closest = u64p(e->closest); p.shadow = u32p(e->shadow);
-    p.counters = counting ? u64p(e->counters) : nullptr;
+    p.closest = u64p(closest); p.shadow = u32p(e->shadow);
+    p.counters = counting ? u64p(L.counters) : nullptr;
     p.force_generic = cfg.traversal_kernel == 1 ? 1u : 0u;
     // Traversal mode: 0 exact tests only, 1 ray / triangle-sphere pre-test, 2 that
     // and ray / cluster-sphere masks.  Automatic (0): 1 or 2, picked per pass from
     // the spread of its top nodes when the work list is sized.
     p.prefilter = cfg.traversal_prefilter == 1 ? 0u : cfg.traversal_prefilter == 3 ? 2u : 1u;
     p.splits = 1;
-    if (counting) launch_fill_u64(u64p(e->counters), 0ull, COUNTER_SLOTS * COUNTERS_PER_SLOT, s);
+    if (counting) launch_fill_u64(u64p(L.counters), 0ull, COUNTER_SLOTS * COUNTERS_PER_SLOT, s);
     if (uses_items(p)) {
         // cull + work list; its time (host read-backs included) counts as traversal
-        StageTimer t(e, ST_TRAVERSE, timing);
-        rsh_status r = build_items(e, cfg, p, out);
+        StageTimer t(L, ST_TRAVERSE, timing);
+        rsh_status r = build_items(e, L, cfg, p, out);
         if (r != RSH_OK) return r;
     } else {
         // Enough waves to fill the GPU: passes with few top nodes split each top
@@ -399,13 +476,13 @@ rsh_status trace_pass(rsh_engine* e, const rsh_frame_config& cfg, const float* r
         out.splits = p.splits;
     }
     {
-        StageTimer t(e, ST_PASS_TRAVERSE + (int)e->pass_index, trav_timing);
-        HIP_OK(launch_traverse(p, s));
+        StageTimer t(L, ST_PASS_TRAVERSE + (int)pass_idx, trav_timing);
+        LANE_OK(launch_traverse(p, s));
     }
     if (counting) {
         unsigned long long c[COUNTER_SLOTS * COUNTERS_PER_SLOT];
-        HIP_OK(hipMemcpyAsync(c, e->counters.p, sizeof(c), hipMemcpyDeviceToHost, s));
-        HIP_OK(hipStreamSynchronize(s));
+        LANE_OK(hipMemcpyAsync(c, L.counters.p, sizeof(c), hipMemcpyDeviceToHost, s));
+        LANE_OK(hipStreamSynchronize(s));
         unsigned long long sum[COUNTERS_PER_SLOT] = {};
         for (uint32_t k = 0; k < COUNTER_SLOTS; k++)
             for (uint32_t i = 0; i < COUNTERS_PER_SLOT; i++) sum[i] += c[(size_t)k * COUNTERS_PER_SLOT + i];
@@ -416,9 +493,13 @@ rsh_status trace_pass(rsh_engine* e, const rsh_frame_config& cfg, const float* r
     return RSH_OK;
 }
 
-rsh_status record(rsh_engine* e, rsh_frame_stats* st, uint32_t kind, uint32_t S, const PassOut& po, uint32_t fl) {
-    if (st && st->pass_count < RSH_MAX_PASSES) {
-        uint32_t i = st->pass_count++;
+// Statistics of pass `idx` (its own slots of st, so the two lanes never write
+// the same field) and, with flag bit 0, a copy of its results on its lane.
+// kind: 0 primary, 1 shadow, 2 reflection.
+rsh_status record(rsh_engine* e, Lane& L, rsh_frame_stats* st, uint32_t idx, uint32_t kind, uint32_t S,
+                  const PassOut& po, uint32_t fl, const DevBuf& closest) {
+    if (st && idx < RSH_MAX_PASSES) {
+        const uint32_t i = idx;
         st->pass_splits[i] = po.splits;
         st->pass_items[i] = po.items; st->pass_chunk[i] = po.chunk;
         st->pass_kind[i] = kind; st->pass_slots[i] = S; st->pass_rays[i] = po.N; st->pass_chunks[i] = po.C;
@@ -427,17 +508,14 @@ rsh_status record(rsh_engine* e, rsh_frame_stats* st, uint32_t kind, uint32_t S,
         st->pass_tri_tests[i] = po.tri_tests; st->pass_cluster_tests[i] = po.cl_tests;
         st->pass_leaf_steps[i] = po.b_steps; st->pass_ray_steps[i] = po.c_steps; st->pass_mt_tests[i] = po.mt_tests;
         st->pass_mode[i] = po.mode;
-        st->rays_total += po.N;
     }
-    e->pass_index++;
-    if (fl & 1u) {
-        rsh_engine::Kept k;
-        k.kind = kind; k.slots = S;
+    if ((fl & 1u) && idx < e->kept.size()) {
+        rsh_engine::Kept& k = e->kept[idx];
+        k.kind = kind; k.slots = S; k.set = true;
         size_t bytes = kind == 1 ? (size_t)S * 4 : (size_t)S * 8;
-        HIP_OK(k.buf.reserve(bytes));
-        HIP_OK(hipMemcpyAsync(k.buf.p, kind == 1 ? e->shadow.p : e->closest.p, bytes, hipMemcpyDeviceToDevice,
-                              e->stream));
-        e->kept.push_back(k);
+        LANE_OK(k.buf.reserve(bytes));
+        LANE_OK(hipMemcpyAsync(k.buf.p, kind == 1 ? e->shadow.p : closest.p, bytes, hipMemcpyDeviceToDevice,
+                               L.stream));
     }
     return RSH_OK;
 }
@@ -456,13 +534,6 @@ rsh_status rsh_engine_create(rsh_engine** out, int device) {
     e->device = device;
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) { delete e; return RSH_HIP_ERROR; }
     e->own_stream = true;
-    // read-backs of a few words per pass land in page-locked memory (a direct copy,
-    // no staging through pageable memory)
-    if (hipHostMalloc(reinterpret_cast<void**>(&e->hstage), HSTAGE_WORDS * 4, hipHostMallocDefault) != hipSuccess) {
-        (void)hipStreamDestroy(e->stream);
-        delete e;
-        return RSH_HIP_ERROR;
-    }
     *out = e;
     return RSH_OK;
 }
@@ -471,22 +542,24 @@ void rsh_engine_destroy(rsh_engine* e) {
     if (!e) return;
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
-    if (e->hstage) (void)hipHostFree(e->hstage);
+    for (Lane& L : e->lane) {
+        if (L.stream) (void)hipStreamSynchronize(L.stream);
+        for (DevBuf* b : L.bufs()) b->release();
+        L.sc.release();
+        if (L.hstage) (void)hipHostFree(L.hstage);
+        for (hipEvent_t ev : L.ev_pool) (void)hipEventDestroy(ev);
+        if (L.own_stream && L.stream) (void)hipStreamDestroy(L.stream);
+    }
+    for (hipEvent_t ev : e->sync_ev) (void)hipEventDestroy(ev);
     for (DevBuf* b : {&e->tri_pos, &e->tri_nrm, &e->tri_mat, &e->mat_dif, &e->mat_spec, &e->light_pos, &e->light_col,
-                      &e->light_int, &e->tri_sphere, &e->tri_mt, &e->obj_sphere, &e->obj_range, &e->cl_sphere, &e->sub_sphere, &e->sub_use,
-                      &e->cl_base, &e->all_list, &e->rays_a,
-                      &e->rays_b, &e->flags, &e->keys, &e->values, &e->tvalues, &e->bases,
-                      &e->ckeys, &e->cvalues, &e->skeys, &e->svalues, &e->rvalues, &e->soff,
-                      &e->rec0, &e->rec1, &e->nodes, &e->aux, &e->closest, &e->shadow, &e->pixels, &e->counters,
-                      &e->dcounts, &e->w_npass, &e->w_cand, &e->w_nitems, &e->w_off, &e->w_first, &e->w_list,
-                      &e->w_cum, &e->w_items, &e->w_table, &e->w_pmask, &e->views_d})
+                      &e->light_int, &e->tri_sphere, &e->tri_mt, &e->obj_sphere, &e->obj_range, &e->cl_sphere,
+                      &e->sub_sphere, &e->sub_use, &e->cl_base, &e->all_list, &e->rays_a, &e->rays_b, &e->closest[0],
+                      &e->closest[1], &e->shadow, &e->pixels, &e->views_d})
         b->release();
     if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
-    for (int i = 0; i < 4; i++) { e->g0[i].release(); e->gb[i].release(); }
+    for (int i = 0; i < 4; i++) { e->g0[i].release(); e->gb[0][i].release(); e->gb[1][i].release(); }
     for (auto& k : e->kept) k.buf.release();
-    e->sc.release();
-    for (hipEvent_t ev : e->ev_pool) (void)hipEventDestroy(ev);
     if (e->own_stream && e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
@@ -640,12 +713,7 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
     if (rs != RSH_OK) return rs;
     rs = build_ranges(e, cfg.triangle_batch);
     if (rs != RSH_OK) return rs;
-    {
-        uint32_t cnt[4], off[4];
-        size_t total = level_sizes((uint32_t)S64, cfg.leaf, cfg.branch, cfg.depth, cnt, off);
-        HIP_OK(e->nodes.reserve(total * 32 + 64));
-        HIP_OK(e->aux.reserve(total * 8 + 64));
-    }
+    if ((rs = ensure_nodes(e, (uint32_t)S64, cfg)) != RSH_OK) return rs;
     Views V{};
     if (nv == 1) {
         V.n = 0; V.one = hv[0];
@@ -659,97 +727,225 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
     if (!e->replaying) {
         for (auto& k : e->kept) k.buf.release();
         e->kept.clear();
+        if (fl & 1u) e->kept.resize(RSH_MAX_PASSES);
     }
-    e->timed.clear();
-    e->ev_used = 0;
-    e->pass_index = 0;
+    for (Lane& l : e->lane) { l.timed.clear(); l.ev_used = 0; l.err.clear(); }
     if (st) std::memset(st, 0, sizeof(*st));
-    hipStream_t s = e->stream;
+    Lane& A = e->lane[0];
+    Lane& B = e->lane[1];
+    hipStream_t sa = A.stream, sb = B.stream;
     const uint32_t W = width, H = height, Lm = e->L, P = W * H, S = W * H * Lm;
     const KeyCfg kc = key_cfg(cfg);
     const SceneView sv = scene_view(e);
     const bool timing = fl & 4u;
-    GBufView G0 = view(e, e->g0), GB = view(e, e->gb);
-    launch_fill_u32(u32p(e->pixels), 0u, P, s);
+    const bool bs = cfg.bounce_shadows != 0;
+    const uint32_t bounces = cfg.bounces < (RSH_MAX_PASSES - 2) / 2 ? cfg.bounces : (RSH_MAX_PASSES - 2) / 2;
+    GBufView G0 = view(e, e->g0), GB[2] = {view(e, e->gb[0]), view(e, e->gb[1])};
+    float* rays_a = e->rays_a.as<float>();
+    float* rays_b = e->rays_b.as<float>();
+    // cross-lane events: evG[k] G-buffer k written on lane 0 (k = 0: primary
+    // hits, k = b: bounce b), evS[k] shadow pass of G-buffer k done on lane 1
+    hipEvent_t* evG = e->sync_ev.data();
+    hipEvent_t* evS = e->sync_ev.data() + RSH_MAX_PASSES + 2;
+    // Pass indices (statistics, kept results): primary 0, shadow of the primary
+    // hits 1, then per bounce b its reflection pass and, with bounce shadows,
+    // the shadow pass of its hits.
+    auto idx_refl = [&](uint32_t b) { return 2u + (b - 1u) * (bs ? 2u : 1u); };
+    auto idx_shadow = [&](uint32_t k) { return k == 0 ? 1u : idx_refl(k) + 1u; };
+    uint32_t last_idx = 0;
+    // Lane 1 runs on a thread of its own for eager frames (its read-backs then
+    // block only that thread); under capture, or with flag bit 5, each shadow
+    // pass is issued as soon as its G-buffer is (the order of a serial frame).
+    const bool threaded = !e->replaying && !(fl & 32u);
+    struct Shared {
+        std::mutex m;
+        std::condition_variable cv;
+        uint32_t g_ready = 0, s_done = 0;
+        bool a_end = false, b_end = false;
+        rsh_status b_rs = RSH_OK;
+        PassOut sh[RSH_MAX_PASSES];
+    } X;
 
-    // primary pass -> G-buffer
-    {
-        StageTimer t(e, ST_CREATE, timing);
-        launch_create_primary(V, P, kc, e->rays_b.as<float>(), u32p(e->flags), u32p(e->keys), u32p(e->values), s);
-    }
-    PassOut po;
-    rs = trace_pass(e, cfg, e->rays_b.as<float>(), P, 0, nullptr, fl, po);
-    if (rs != RSH_OK) return rs;
-    if ((rs = record(e, st, 0, P, po, fl)) != RSH_OK) return rs;
-    {
-        StageTimer t(e, ST_SHADE, timing);
-        launch_surface(e->rays_b.as<float>(), u64p(e->closest), cfg.hit_mode, W, H, 1, 1, sv, G0, s);
-    }
-    // shadow pass + primary shading
-    {
-        StageTimer t(e, ST_CREATE, timing);
-        launch_create_shadow(G0.pos, G0.nrm, W, H, Lm, f4p(e->light_pos), kc, V, e->rays_b.as<float>(),
-                             u32p(e->flags), u32p(e->keys), u32p(e->values), s);
-    }
-    rs = trace_pass(e, cfg, e->rays_b.as<float>(), S, 1, G0.pos, fl, po);
-    if (rs != RSH_OK) return rs;
-    if ((rs = record(e, st, 1, S, po, fl)) != RSH_OK) return rs;
-    if (po.N > 0) {
-        StageTimer t(e, ST_SHADE, timing);
-        launch_shade_primary(G0, W, H, Lm, sv.lights, V, u32p(e->shadow), u32p(e->pixels), s);
-    }
-    // mirror bounces
-    for (uint32_t b = 1; b <= cfg.bounces; b++) {
+    auto shadow_step = [&](uint32_t k) -> rsh_status {
+        Lane& L = B;
+        const GBufView& G = k == 0 ? G0 : GB[k % 2];
+        LANE_OK(hipStreamWaitEvent(sb, evG[k], 0));
         {
-            StageTimer t(e, ST_CREATE, timing);
-            if (b == 1)
-                launch_create_reflection(G0.pos, G0.nrm, G0.dif, W, H, Lm, V, cfg.reflect_gate, 1, kc,
-                                         e->rays_a.as<float>(), u32p(e->flags), u32p(e->keys), u32p(e->values), s);
-            else
-                launch_create_bounce(GB, W, H, Lm, u64p(e->closest), cfg.hit_mode, cfg.reflect_gate, kc, V,
-                                     e->rays_a.as<float>(), u32p(e->flags), u32p(e->keys), u32p(e->values), s);
+            StageTimer t(B, ST_CREATE, timing);
+            launch_create_shadow(G.pos, G.nrm, W, H, Lm, f4p(e->light_pos), kc, V, rays_b, u32p(B.flags),
+                                 u32p(B.keys), u32p(B.values), sb);
         }
-        rs = trace_pass(e, cfg, e->rays_a.as<float>(), S, 0, nullptr, fl, po);
-        if (rs != RSH_OK) return rs;
-        if ((rs = record(e, st, 2, S, po, fl)) != RSH_OK) return rs;
-        if (po.N == 0) break;
-        {
-            StageTimer t(e, ST_SHADE, timing);
-            launch_surface(e->rays_a.as<float>(), u64p(e->closest), cfg.hit_mode, W, H, Lm, 0, sv, GB, s);
+        PassOut po;
+        rsh_status r = trace_pass(e, B, cfg, rays_b, S, 1, G.pos, fl, po, idx_shadow(k), e->closest[0]);
+        if (r != RSH_OK) return r;
+        if ((r = record(e, B, st, idx_shadow(k), 1, S, po, fl, e->closest[0])) != RSH_OK) return r;
+        LANE_OK(hipEventRecord(evS[k], sb));
+        X.sh[k] = po;
+        return RSH_OK;
+    };
+    auto b_loop = [&]() {
+        rsh_status r = RSH_OK;
+        for (uint32_t k = 0;; k++) {
+            {
+                std::unique_lock<std::mutex> lk(X.m);
+                X.cv.wait(lk, [&] { return X.g_ready > k || X.a_end; });
+                if (X.g_ready <= k) break;     // lane 0 ended without G-buffer k
+            }
+            r = shadow_step(k);
+            std::lock_guard<std::mutex> lk(X.m);
+            if (r != RSH_OK) break;
+            X.s_done = k + 1;
+            X.cv.notify_all();
+        }
+        std::lock_guard<std::mutex> lk(X.m);
+        X.b_rs = r;
+        X.b_end = true;
+        X.cv.notify_all();
+    };
+    // lane 0: G-buffer k is written
+    auto publish = [&](uint32_t k) -> rsh_status {
+        Lane& L = A;
+        LANE_OK(hipEventRecord(evG[k], sa));
+        last_idx = std::max(last_idx, idx_shadow(k));
+        if (threaded) {
+            std::lock_guard<std::mutex> lk(X.m);
+            X.g_ready = k + 1;
+            X.cv.notify_all();
+            return RSH_OK;
+        }
+        const rsh_status r = shadow_step(k);
+        if (r != RSH_OK) return r;
+        X.s_done = k + 1;
+        // serial passes: lane 0 resumes once the shadow pass has ended
+        if (fl & 32u) LANE_OK(hipStreamWaitEvent(sa, evS[k], 0));
+        return RSH_OK;
+    };
+    // lane 0: wait for the shadow pass of G-buffer k
+    auto await_shadow = [&](uint32_t k, PassOut& po) -> rsh_status {
+        Lane& L = A;
+        if (threaded) {
+            std::unique_lock<std::mutex> lk(X.m);
+            X.cv.wait(lk, [&] { return X.s_done > k || X.b_end; });
+            if (X.s_done <= k) return X.b_rs != RSH_OK ? X.b_rs : RSH_INTERNAL;
+        }
+        po = X.sh[k];
+        LANE_OK(hipStreamWaitEvent(sa, evS[k], 0));
+        return RSH_OK;
+    };
+    // lane 0: shading of G-buffer k's hits, after its shadow pass
+    auto shade = [&](uint32_t k) -> rsh_status {
+        PassOut ps;
+        if (k == 0) {
+            rsh_status r = await_shadow(0, ps);
+            if (r != RSH_OK) return r;
+            if (ps.N > 0) {
+                StageTimer t(A, ST_SHADE, timing);
+                launch_shade_primary(G0, W, H, Lm, sv.lights, V, u32p(e->shadow), u32p(e->pixels), sa);
+            }
+            return RSH_OK;
         }
         const uint32_t* sflags = nullptr;
-        if (cfg.bounce_shadows) {
-            {
-                StageTimer t(e, ST_CREATE, timing);
-                launch_create_shadow(GB.pos, GB.nrm, W, H, Lm, f4p(e->light_pos), kc, V, e->rays_b.as<float>(),
-                                     u32p(e->flags), u32p(e->keys), u32p(e->values), s);
-            }
-            PassOut sp;
-            rs = trace_pass(e, cfg, e->rays_b.as<float>(), S, 1, GB.pos, fl, sp);
-            if (rs != RSH_OK) return rs;
-            if ((rs = record(e, st, 1, S, sp, fl)) != RSH_OK) return rs;
+        if (bs) {
+            rsh_status r = await_shadow(k, ps);
+            if (r != RSH_OK) return r;
             sflags = u32p(e->shadow);
         }
+        StageTimer t(A, ST_SHADE, timing);
+        launch_shade_reflection(GB[k % 2], W, H, Lm, sv.lights, V, u64p(e->closest[k % 2]), cfg.hit_mode, sflags,
+                                u32p(e->pixels), sa);
+        return RSH_OK;
+    };
+    // lane 0: primary pass, then the mirror bounces; bounce b's hits alternate
+    // between closest[b % 2] / GB[b % 2], and each G-buffer is shaded once the
+    // next reflection pass has been traced (its shadow pass runs meanwhile)
+    auto run_a = [&]() -> rsh_status {
+        Lane& L = A;
+        launch_fill_u32(u32p(e->pixels), 0u, P, sa);
         {
-            StageTimer t(e, ST_SHADE, timing);
-            launch_shade_reflection(GB, W, H, Lm, sv.lights, V, u64p(e->closest), cfg.hit_mode, sflags,
-                                    u32p(e->pixels), s);
+            StageTimer t(A, ST_CREATE, timing);
+            launch_create_primary(V, P, kc, rays_b, u32p(A.flags), u32p(A.keys), u32p(A.values), sa);
         }
-    }
-    HIP_OK(hipGetLastError());
-    if (e->replaying) return RSH_OK;     // under stream capture: no synchronisation
-    HIP_OK(hipStreamSynchronize(s));
-    if (st && (timing || (fl & 16u))) {
-        for (auto& t : e->timed) {
-            float ms = 0.0f;
-            if (hipEventElapsedTime(&ms, t.second.first, t.second.second) != hipSuccess) continue;
-            if (t.first >= ST_PASS_TRAVERSE) {
-                st->stage_ms[ST_TRAVERSE] += ms;
-                if (t.first - ST_PASS_TRAVERSE < RSH_MAX_PASSES) st->pass_traverse_ms[t.first - ST_PASS_TRAVERSE] += ms;
-            } else {
-                st->stage_ms[t.first] += ms;
+        PassOut po;
+        rsh_status r = trace_pass(e, A, cfg, rays_b, P, 0, nullptr, fl, po, 0, e->closest[0]);
+        if (r != RSH_OK) return r;
+        if ((r = record(e, A, st, 0, 0, P, po, fl, e->closest[0])) != RSH_OK) return r;
+        {
+            StageTimer t(A, ST_SHADE, timing);
+            launch_surface(rays_b, u64p(e->closest[0]), cfg.hit_mode, W, H, 1, 1, sv, G0, sa);
+        }
+        if ((r = publish(0)) != RSH_OK) return r;
+        uint32_t due = 0;      // G-buffer whose shading is due
+        for (uint32_t b = 1; b <= bounces; b++) {
+            DevBuf& cur = e->closest[b % 2];
+            {
+                StageTimer t(A, ST_CREATE, timing);
+                if (b == 1)
+                    launch_create_reflection(G0.pos, G0.nrm, G0.dif, W, H, Lm, V, cfg.reflect_gate, 1, kc, rays_a,
+                                             u32p(A.flags), u32p(A.keys), u32p(A.values), sa);
+                else
+                    launch_create_bounce(GB[(b - 1) % 2], W, H, Lm, u64p(e->closest[(b - 1) % 2]), cfg.hit_mode,
+                                         cfg.reflect_gate, kc, V, rays_a, u32p(A.flags), u32p(A.keys),
+                                         u32p(A.values), sa);
             }
+            PassOut pr;
+            if ((r = trace_pass(e, A, cfg, rays_a, S, 0, nullptr, fl, pr, idx_refl(b), cur)) != RSH_OK) return r;
+            if ((r = record(e, A, st, idx_refl(b), 2, S, pr, fl, cur)) != RSH_OK) return r;
+            last_idx = std::max(last_idx, idx_refl(b));
+            if ((r = shade(due)) != RSH_OK) return r;
+            due = 0xFFFFFFFFu;
+            if (pr.N == 0) return RSH_OK;
+            {
+                StageTimer t(A, ST_SHADE, timing);
+                launch_surface(rays_a, u64p(cur), cfg.hit_mode, W, H, Lm, 0, sv, GB[b % 2], sa);
+            }
+            if (bs && (r = publish(b)) != RSH_OK) return r;
+            due = b;
         }
+        if (due != 0xFFFFFFFFu && (r = shade(due)) != RSH_OK) return r;
+        LANE_OK(hipGetLastError());
+        return RSH_OK;
+    };
+
+    std::thread tb;
+    if (threaded) tb = std::thread(b_loop);
+    const rsh_status ra = run_a();
+    if (threaded) {
+        {
+            std::lock_guard<std::mutex> lk(X.m);
+            X.a_end = true;
+            X.cv.notify_all();
+        }
+        tb.join();
+    }
+    if (ra != RSH_OK || (threaded && X.b_rs != RSH_OK)) {
+        e->err = !A.err.empty() ? A.err : !B.err.empty() ? B.err : "render: a pass failed";
+        if (!e->replaying) (void)hipDeviceSynchronize();     // nothing of this frame left in flight
+        return ra != RSH_OK ? ra : X.b_rs;
+    }
+    if (st) {
+        st->pass_count = last_idx + 1 < RSH_MAX_PASSES ? last_idx + 1 : RSH_MAX_PASSES;
+        for (uint32_t i = 0; i < st->pass_count; i++) st->rays_total += st->pass_rays[i];
+    }
+    if (!e->replaying && (fl & 1u)) {
+        uint32_t n = 0;
+        while (n < e->kept.size() && e->kept[n].set) n++;
+        for (size_t i = n; i < e->kept.size(); i++) e->kept[i].buf.release();
+        e->kept.resize(n);
+    }
+    if (e->replaying) return RSH_OK;     // under stream capture: no synchronisation
+    HIP_OK(hipStreamSynchronize(sa));
+    if (st && (timing || (fl & 16u))) {
+        for (Lane& l : e->lane)
+            for (auto& t : l.timed) {
+                float ms = 0.0f;
+                if (hipEventElapsedTime(&ms, t.second.first, t.second.second) != hipSuccess) continue;
+                if (t.first >= ST_PASS_TRAVERSE) {
+                    st->stage_ms[ST_TRAVERSE] += ms;
+                    if (t.first - ST_PASS_TRAVERSE < RSH_MAX_PASSES) st->pass_traverse_ms[t.first - ST_PASS_TRAVERSE] += ms;
+                } else {
+                    st->stage_ms[t.first] += ms;
+                }
+            }
     }
     return RSH_OK;
 }
@@ -794,7 +990,7 @@ rsh_status render(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rsh_fram
     }
     drop_graph(e);
     rsh_frame_stats local{};
-    e->plan.clear();
+    for (Lane& l : e->lane) { l.plan.clear(); l.plan_pos = 0; }
     e->recording = true;
     rsh_status rs = render_frame(e, hv, nv, cfg, 0, &local);
     e->recording = false;
@@ -802,7 +998,7 @@ rsh_status render(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rsh_fram
     if (st) *st = local;
     const uint64_t epoch = alloc_epoch().load();
     e->replaying = true;
-    e->plan_pos = 0;
+    for (Lane& l : e->lane) l.plan_pos = 0;
     HIP_OK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
     rs = render_frame(e, hv, nv, cfg, 0, nullptr);
     hipGraph_t g = nullptr;
@@ -810,7 +1006,8 @@ rsh_status render(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rsh_fram
     e->replaying = false;
     if (rs != RSH_OK) { if (g) (void)hipGraphDestroy(g); return rs; }
     HIP_OK(ce);
-    if (alloc_epoch().load() != epoch || e->plan_pos != e->plan.size()) {
+    if (alloc_epoch().load() != epoch || e->lane[0].plan_pos != e->lane[0].plan.size() ||
+        e->lane[1].plan_pos != e->lane[1].plan.size()) {
         (void)hipGraphDestroy(g);
         e->err = "render: the captured frame did not repeat the recorded one";
         return RSH_INTERNAL;
@@ -882,26 +1079,25 @@ rsh_status rsh_engine_trace(rsh_engine* e, const float* rays, const float* tmax,
     rsh_status rs = ensure_frame(e, n, 1, 1);
     if (rs != RSH_OK) return rs;
     if ((rs = build_ranges(e, cfg.triangle_batch)) != RSH_OK) return rs;
-    {
-        uint32_t cnt[4], off[4];
-        size_t total = level_sizes(n, cfg.leaf, cfg.branch, cfg.depth, cnt, off);
-        HIP_OK(e->nodes.reserve(total * 32 + 64));
-        HIP_OK(e->aux.reserve(total * 8 + 64));
-    }
-    e->timed.clear();
-    e->ev_used = 0;
-    e->pass_index = 0;
+    if ((rs = ensure_nodes(e, n, cfg)) != RSH_OK) return rs;
+    Lane& L = e->lane[0];
+    L.timed.clear();
+    L.ev_used = 0;
+    L.err.clear();
     if (st) std::memset(st, 0, sizeof(*st));
-    hipStream_t s = e->stream;
+    hipStream_t s = L.stream;
     const bool timing = fl & 4u;
     {
-        StageTimer t(e, ST_CREATE, timing);
-        launch_query_keys(rays, segments, n, key_cfg(cfg), kind == 1 ? 1u : 0u, u32p(e->flags), u32p(e->keys),
-                          u32p(e->values), s);
+        StageTimer t(L, ST_CREATE, timing);
+        launch_query_keys(rays, segments, n, key_cfg(cfg), kind == 1 ? 1u : 0u, u32p(L.flags), u32p(L.keys),
+                          u32p(L.values), s);
         HIP_OK(hipGetLastError());
     }
     PassOut po;
-    if ((rs = trace_pass(e, cfg, rays, n, kind, nullptr, fl, po, tmax)) != RSH_OK) return rs;
+    if ((rs = trace_pass(e, L, cfg, rays, n, kind, nullptr, fl, po, 0, e->closest[0], tmax)) != RSH_OK) {
+        e->err = L.err;
+        return rs;
+    }
     if (st && st->pass_count < RSH_MAX_PASSES) {
         const uint32_t i = st->pass_count++;
         st->pass_kind[i] = kind; st->pass_slots[i] = n; st->pass_rays[i] = po.N; st->pass_chunks[i] = po.C;
@@ -912,11 +1108,11 @@ rsh_status rsh_engine_trace(rsh_engine* e, const float* rays, const float* tmax,
         st->pass_mode[i] = po.mode;
         st->rays_total = po.N;
     }
-    HIP_OK(hipMemcpyAsync(out, kind == 1 ? e->shadow.p : e->closest.p, (size_t)n * (kind == 1 ? 4 : 8),
+    HIP_OK(hipMemcpyAsync(out, kind == 1 ? e->shadow.p : e->closest[0].p, (size_t)n * (kind == 1 ? 4 : 8),
                           hipMemcpyDeviceToDevice, s));
     HIP_OK(hipStreamSynchronize(s));
     if (st && (timing || (fl & 16u))) {
-        for (auto& t : e->timed) {
+        for (auto& t : L.timed) {
             float ms = 0.0f;
             if (hipEventElapsedTime(&ms, t.second.first, t.second.second) != hipSuccess) continue;
             const int k = t.first >= ST_PASS_TRAVERSE ? ST_TRAVERSE : t.first;

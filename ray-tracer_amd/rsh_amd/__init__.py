@@ -284,7 +284,7 @@ class RshError(RuntimeError):
 class Engine:
     """Device-resident scene + frame renderer (include/rsh.h)."""
 
-    KEEP_PASSES, COUNTERS, TIMING, GRAPH, TRAV_TIMING = 1, 2, 4, 8, 16
+    KEEP_PASSES, COUNTERS, TIMING, GRAPH, TRAV_TIMING, SERIAL = 1, 2, 4, 8, 16, 32
 
     def __init__(self, device: int = 0):
         self.L = lib()

@@ -290,7 +290,8 @@ struct PassOut {
 };
 
 // Work list of the 64-ray kernel (see uses_items): cull, size, scan, fill.
-rsh_status build_items(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, TravParams& p, PassOut& out) {
+rsh_status build_items(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, TravParams& p, PassOut& out,
+                       bool camera) {
     hipStream_t s = L.stream;
     const uint32_t ntop = p.level_count[1];
     const size_t nb = ((size_t)ntop + 1) * 4;
@@ -330,11 +331,14 @@ rsh_status build_items(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, Trav
         if (!tabled) tot[1] = L.hstage[1];
         std::memcpy(ht, L.hstage + 2, sizeof(ht));
         // Traversal mode: the cluster ray masks pay when many top nodes are wide
-        // (spread-out secondary rays) and cost where rays are coherent.  More than
-        // a sixteenth of the nodes with tan(spread) outside [0, WIDE_TAN] picks the
-        // best mode for every pass of the 1080p atrium frame (primary and first
-        // shadow pass without, every later pass with).
-        if (cfg.traversal_prefilter == 0) p.prefilter = 16 * ht[2 * ITEM_TABLE] > ntop ? 2u : 1u;
+        // (spread-out secondary rays) and cost where rays are coherent.  A frame's
+        // camera rays never take them (measured faster without on the 1080p
+        // atrium frame and on both 64-view batches, whose small views leave many
+        // primary nodes above WIDE_TAN); otherwise more than a sixteenth of the
+        // nodes with tan(spread) outside [0, WIDE_TAN] picks the best mode for
+        // every pass of the 1080p atrium frame (first shadow pass without, every
+        // later pass with).
+        if (cfg.traversal_prefilter == 0) p.prefilter = !camera && 16 * ht[2 * ITEM_TABLE] > ntop ? 2u : 1u;
         if (tabled) {
             if (!cfg.traversal_chunk) {
                 // coherent passes with plenty of items take longer items (less
@@ -377,7 +381,7 @@ rsh_status build_items(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, Trav
 // closest; kind 1: shadow flags into e->shadow.
 rsh_status trace_pass(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, const float* rays, uint32_t S,
                       uint32_t kind, const float4* shadow_pos, uint32_t flags_mask, PassOut& out, uint32_t pass_idx,
-                      DevBuf& closest, const float* tmax = nullptr) {
+                      DevBuf& closest, const float* tmax = nullptr, bool camera = false) {
     out = PassOut{};
     hipStream_t s = L.stream;
     const bool timing = flags_mask & 4u, counting = flags_mask & 2u;
@@ -461,7 +465,7 @@ rsh_status trace_pass(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, const
     if (uses_items(p)) {
         // cull + work list; its time (host read-backs included) counts as traversal
         StageTimer t(L, ST_TRAVERSE, timing);
-        rsh_status r = build_items(e, L, cfg, p, out);
+        rsh_status r = build_items(e, L, cfg, p, out, camera);
         if (r != RSH_OK) return r;
     } else {
         // Enough waves to fill the GPU: passes with few top nodes split each top
@@ -866,7 +870,7 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
             launch_create_primary(V, P, kc, rays_b, u32p(A.flags), u32p(A.keys), u32p(A.values), sa);
         }
         PassOut po;
-        rsh_status r = trace_pass(e, A, cfg, rays_b, P, 0, nullptr, fl, po, 0, e->closest[0]);
+        rsh_status r = trace_pass(e, A, cfg, rays_b, P, 0, nullptr, fl, po, 0, e->closest[0], nullptr, true);
         if (r != RSH_OK) return r;
         if ((r = record(e, A, st, 0, 0, P, po, fl, e->closest[0])) != RSH_OK) return r;
         {

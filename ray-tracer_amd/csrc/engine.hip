@@ -45,6 +45,8 @@ struct Lane {
     DevBuf flags, keys, values, tvalues, bases, ckeys, cvalues, skeys, svalues;
     DevBuf rvalues, soff, rec0, rec1, nodes, aux, counters, dcounts;
     uint32_t* hstage = nullptr;        // page-locked host words for the per-pass read-backs
+    hipEvent_t rb_ev = nullptr;        // marks a read-back's completion (polled)
+    bool table_clean = false;          // w_table is zero (the item scan clears it after each read-back)
     // work list of the 64-ray traversal kernel
     DevBuf w_npass, w_cand, w_nitems, w_off, w_first, w_list, w_cum, w_items, w_table, w_pmask;
     ScanScratch sc;
@@ -173,15 +175,29 @@ constexpr int ST_PASS_TRAVERSE = 100;   // + pass index: per-pass traversal time
 constexpr uint64_t TRAVERSE_TARGET_WAVES = 16384;
 // 64-ray kernel work items: a pass starts at ITEM_CHUNK candidate clusters
 // (of CLUSTER triangles) per item and shrinks the chunk (not below
-// ITEM_CHUNK_MIN) until it has at least ITEM_TARGET items, about 25 per
-// resident wave slot (5120 on 256 CUs): item lengths vary widely, and the
-// short bounce passes otherwise end in a long tail.  Measured on the 1080p
-// atrium frame: 32768 -> 10.1 ms of traversal, 131072 -> 9.3, 262144 -> 9.3,
-// 524288 -> 9.6 (smaller items cost more per-item set-up).
+// ITEM_CHUNK_MIN) until it has at least ITEM_TARGET items, about 13 per
+// resident wave slot (5120 on 256 CUs): item lengths vary widely, and short
+// passes otherwise end in a long tail, while every item repeats the node's
+// set-up.  With the shadow passes overlapping the reflection chain (which
+// fills most tails), measured on the 1080p atrium frame: 32768 -> 7.57 ms,
+// 49152 -> 7.53, 65536 -> 7.52, 98304 -> 7.64, 131072 -> 7.81 ms per frame.
 constexpr uint32_t ITEM_CHUNK = 64;
-constexpr size_t HSTAGE_WORDS = 2 + 2 * (2 * ITEM_TABLE + 1);   // list / item totals, then the item table
+constexpr size_t HSTAGE_WORDS = 2 + 2 * ITEM_TABLE_WORDS;   // N / C or the item total, then the item table
 constexpr uint32_t ITEM_CHUNK_MIN = 4;
-constexpr uint32_t ITEM_TARGET = 131072;
+constexpr uint32_t ITEM_TARGET = 65536;
+
+// Wait for a lane's read-back copies: an event after them, polled.  A
+// read-back is a few microseconds of GPU work, and a blocking wait would add
+// the host thread's wake-up latency to every pass.
+hipError_t readback_wait(Lane& L) {
+    hipError_t r = hipEventRecord(L.rb_ev, L.stream);
+    if (r != hipSuccess) return r;
+    for (;;) {
+        r = hipEventQuery(L.rb_ev);
+        if (r != hipErrorNotReady) return r;
+        __builtin_ia32_pause();
+    }
+}
 
 // Lane 1's stream and page-locked words, created on first use; lane 0 runs
 // on the engine's stream.
@@ -192,8 +208,10 @@ rsh_status ensure_lanes(rsh_engine* e) {
         HIP_OK(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
         L.own_stream = true;
     }
-    for (Lane& l : e->lane)
+    for (Lane& l : e->lane) {
         if (!l.hstage) HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&l.hstage), HSTAGE_WORDS * 4, hipHostMallocDefault));
+        if (!l.rb_ev) HIP_OK(hipEventCreateWithFlags(&l.rb_ev, hipEventDisableTiming));
+    }
     while (e->sync_ev.size() < 2 * (RSH_MAX_PASSES + 2)) {
         hipEvent_t ev;
         HIP_OK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
@@ -315,21 +333,28 @@ rsh_status build_items(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, Trav
         // item totals per chunk size, the chunk (shrunk from ITEM_CHUNK until the
         // pass has ITEM_TARGET items) and the item total.
         const bool tabled = chunk <= ITEM_TABLE || (chunk <= 2 * ITEM_TABLE && chunk % 2 == 0);
-        unsigned long long ht[2 * ITEM_TABLE + 1] = {};
+        unsigned long long ht[ITEM_TABLE_WORDS] = {};
         auto items_at = [&](uint32_t c) { return c <= ITEM_TABLE ? ht[c - 1] : ht[ITEM_TABLE + c / 2 - 1]; };
         uint32_t* first = u32p(L.w_first);
-        LANE_OK(L.w_table.reserve((2 * ITEM_TABLE + 1) * sizeof(unsigned long long)));
-        launch_item_table(cand, p.aux + p.level_offset[1], ntop, e->NC, L.w_table.as<unsigned long long>(), s);
-        LANE_OK(hipMemcpyAsync(L.hstage + 2, L.w_table.p, sizeof(ht), hipMemcpyDeviceToHost, s));
+        LANE_OK(L.w_table.reserve(ITEM_TABLE_WORDS * sizeof(unsigned long long)));
+        unsigned long long* table = L.w_table.as<unsigned long long>();
+        if (!L.table_clean) launch_fill_u64(table, 0ull, ITEM_TABLE_WORDS, s);   // new, or a pass that failed
+        L.table_clean = false;
+        launch_item_table(cand, p.aux + p.level_offset[1], ntop, e->NC, p.algorithm != 1 ? u32p(L.w_off) + ntop : nullptr,
+                          table, s);
+        LANE_OK(hipMemcpyAsync(L.hstage + 2, table, sizeof(ht), hipMemcpyDeviceToHost, s));
         if (!tabled) {
             LANE_OK(run_item_scan(cand, ntop, chunk, e->NC, nitems, first, L.sc, s));
             LANE_OK(hipMemcpyAsync(L.hstage + 1, first + ntop, 4, hipMemcpyDeviceToHost, s));
         }
-        if (p.algorithm != 1) LANE_OK(hipMemcpyAsync(L.hstage, u32p(L.w_off) + ntop, 4, hipMemcpyDeviceToHost, s));
-        LANE_OK(hipStreamSynchronize(s));
-        if (p.algorithm != 1) tot[0] = L.hstage[0];
-        if (!tabled) tot[1] = L.hstage[1];
+        LANE_OK(readback_wait(L));
         std::memcpy(ht, L.hstage + 2, sizeof(ht));
+        if (p.algorithm != 1) tot[0] = (uint32_t)ht[2 * ITEM_TABLE + 1];
+        if (!tabled) {
+            tot[1] = L.hstage[1];
+            launch_fill_u64(table, 0ull, ITEM_TABLE_WORDS, s);    // for the next pass
+            L.table_clean = true;
+        }
         // Traversal mode: the cluster ray masks pay when many top nodes are wide
         // (spread-out secondary rays) and cost where rays are coherent.  A frame's
         // camera rays never take them (measured faster without on the 1080p
@@ -355,7 +380,9 @@ rsh_status build_items(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, Trav
                 return RSH_CAPACITY;
             }
             tot[1] = (uint32_t)items_at(chunk);
-            LANE_OK(run_item_scan(cand, ntop, chunk, e->NC, nitems, first, L.sc, s));
+            // the item scan also clears the table for the next pass
+            LANE_OK(run_item_scan(cand, ntop, chunk, e->NC, nitems, first, L.sc, s, table, ITEM_TABLE_WORDS));
+            L.table_clean = true;
         }
     }
     LANE_OK(L.w_list.reserve((size_t)tot[0] * 16 + 16));
@@ -402,7 +429,7 @@ rsh_status trace_pass(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, const
         hc[0] = pp.N; hc[1] = pp.C;
     } else {
         LANE_OK(hipMemcpyAsync(L.hstage, dc, 8, hipMemcpyDeviceToHost, s));
-        LANE_OK(hipStreamSynchronize(s));
+        LANE_OK(readback_wait(L));
         hc[0] = L.hstage[0]; hc[1] = L.hstage[1];
         if (e->recording) { L.plan.emplace_back(); L.plan.back().N = hc[0]; L.plan.back().C = hc[1]; }
     }
@@ -551,6 +578,7 @@ void rsh_engine_destroy(rsh_engine* e) {
         for (DevBuf* b : L.bufs()) b->release();
         L.sc.release();
         if (L.hstage) (void)hipHostFree(L.hstage);
+        if (L.rb_ev) (void)hipEventDestroy(L.rb_ev);
         for (hipEvent_t ev : L.ev_pool) (void)hipEventDestroy(ev);
         if (L.own_stream && L.stream) (void)hipStreamDestroy(L.stream);
     }

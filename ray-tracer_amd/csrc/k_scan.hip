@@ -299,8 +299,10 @@ __device__ __forceinline__ void load16_live(const uint32_t* p, uint64_t base, ui
 // div_hi's bound).
 template <int MODE>
 __global__ void __launch_bounds__(1024) k_scan_block(const uint32_t* in, uint32_t n, uint32_t chunk, bool fast,
-                                                     uint32_t* out, uint32_t* total) {
+                                                     uint32_t* out, uint32_t* total, unsigned long long* zero,
+                                                     uint32_t zero_n) {
     __shared__ uint32_t red[16];
+    if (zero && threadIdx.x < zero_n) zero[threadIdx.x] = 0ull;
     const uint32_t lane = lane_id(), w = wave_id();
     const uint32_t wper = ((n + 16u * 256u - 1u) / (16u * 256u)) * 256u;
     const uint32_t wb = w * wper;
@@ -833,7 +835,7 @@ static hipError_t run_scan(const uint32_t* in, uint32_t* out, uint64_t n, ScanSc
     if (nt > MAX_TILES) return hipErrorInvalidValue;
     if (nt == 0) return hipSuccess;
     if (!inclusive && n <= SMALL_SCAN) {     // one launch instead of three
-        k_scan_block<0><<<1, 1024, 0, s>>>(in, (uint32_t)n, 1u, true, out, sc.total.as<uint32_t>());
+        k_scan_block<0><<<1, 1024, 0, s>>>(in, (uint32_t)n, 1u, true, out, sc.total.as<uint32_t>(), nullptr, 0u);
         return hipGetLastError();
     }
     uint32_t* tiles = sc.tile_counts.as<uint32_t>();
@@ -852,13 +854,15 @@ hipError_t run_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, Sca
 }
 
 hipError_t run_item_scan(const uint32_t* cand, uint32_t ntop, uint32_t chunk, uint32_t max_cand, uint32_t* counts,
-                         uint32_t* first, ScanScratch& sc, hipStream_t s) {
-    if (chunk == 0) return hipErrorInvalidValue;
+                         uint32_t* first, ScanScratch& sc, hipStream_t s, unsigned long long* zero,
+                         uint32_t zero_n) {
+    if (chunk == 0 || zero_n > 1024) return hipErrorInvalidValue;
     if ((uint64_t)ntop + 1 <= SMALL_SCAN) {
         const bool fast = div_hi_ok((uint64_t)max_cand + chunk - 1, chunk);
-        k_scan_block<1><<<1, 1024, 0, s>>>(cand, ntop + 1, chunk, fast, first, nullptr);
+        k_scan_block<1><<<1, 1024, 0, s>>>(cand, ntop + 1, chunk, fast, first, nullptr, zero, zero_n);
         return hipGetLastError();
     }
+    if (zero) launch_fill_u64(zero, 0ull, zero_n, s);
     k_item_counts<<<ntop / 256 + 1, 256, 0, s>>>(cand, chunk, ntop, counts);
     return run_scan(counts, first, (uint64_t)ntop + 1, sc, s, false);
 }

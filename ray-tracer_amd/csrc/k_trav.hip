@@ -400,12 +400,14 @@ __global__ void __launch_bounds__(256) k_item_count(const uint32_t* cand, uint32
 // c) for c = 1..ITEM_TABLE, table[ITEM_TABLE + c - 1] the same for chunk 2c;
 // table[2 ITEM_TABLE] = top nodes whose cone is wide (tangent of the spread
 // outside [0, WIDE_TAN]: a spread clamped to pi/2 has a large negative float
-// tangent; or not finite).  One wave per slice of TABLE_SLICE top nodes,
+// tangent; or not finite); table[2 ITEM_TABLE + 1] = *list_total (the pass's
+// list entries, so one read-back carries everything).  The table must start
+// zeroed (the item scan clears it for the next pass).  One wave per slice of TABLE_SLICE top nodes,
 // lane c - 1 sums chunk sizes c and 2c in registers; one read-back of the
 // table sizes a pass and picks its traversal mode.
 constexpr uint32_t TABLE_SLICE = 256;
 __global__ void __launch_bounds__(64) k_item_table(const uint32_t* cand, const float2* top_aux, uint32_t ntop,
-                                                   unsigned long long* table, bool fast) {
+                                                   unsigned long long* table, bool fast, const uint32_t* list_total) {
     __shared__ uint4 s_cand4[TABLE_SLICE / 4];
     uint32_t* s_cand = reinterpret_cast<uint32_t*>(s_cand4);
     const uint32_t lane = threadIdx.x & 63u, c = lane + 1u;
@@ -444,6 +446,7 @@ __global__ void __launch_bounds__(64) k_item_table(const uint32_t* cand, const f
     for (uint32_t t = t0 + lane; t < t1; t += 64u) wide += !(top_aux[t].y >= 0.0f && top_aux[t].y <= WIDE_TAN) ? 1u : 0u;
     for (int d = 32; d > 0; d >>= 1) wide += __shfl_xor(wide, d, 64);
     if (lane == 0 && wide) atomicAdd(&table[2 * ITEM_TABLE], (unsigned long long)wide);
+    if (blockIdx.x == 0 && lane == 0) table[2 * ITEM_TABLE + 1] = list_total ? *list_total : 0u;
 }
 
 // Work-item descriptors, one wave per top node, one lane per item:
@@ -1136,11 +1139,12 @@ void launch_cull_list(const TravParams& p, uint4* obj_ent, uint32_t* obj_cum, un
 }
 
 void launch_item_table(const uint32_t* cand, const float2* top_aux, uint32_t ntop, uint32_t max_cand,
-                       unsigned long long* table, hipStream_t s) {
+                       const uint32_t* list_total, unsigned long long* table, hipStream_t s) {
     static_assert(ITEM_TABLE == 64, "one lane per chunk size");
-    launch_fill_u64(table, 0ull, 2 * ITEM_TABLE + 1, s);
     const bool fast = div_hi_ok((uint64_t)max_cand + 2 * ITEM_TABLE - 1, 2 * ITEM_TABLE);
-    if (ntop) k_item_table<<<(ntop + TABLE_SLICE - 1) / TABLE_SLICE, 64, 0, s>>>(cand, top_aux, ntop, table, fast);
+    if (ntop)
+        k_item_table<<<(ntop + TABLE_SLICE - 1) / TABLE_SLICE, 64, 0, s>>>(cand, top_aux, ntop, table, fast,
+                                                                           list_total);
 }
 
 void launch_item_count(const uint32_t* cand, uint32_t chunk, uint32_t ntop, uint32_t* nitems, hipStream_t s) {

@@ -164,9 +164,11 @@ void launch_cull_list(const TravParams& p, uint4* obj_ent, uint32_t* obj_cum, un
 // with a wide cone (spread tangent outside [0, WIDE_TAN]).
 constexpr uint32_t ITEM_TABLE = 64;
 constexpr float WIDE_TAN = 0.1f;
-// max_cand: an upper bound of every cand entry (the scene's cluster count)
+// max_cand: an upper bound of every cand entry (the scene's cluster count);
+// table: 2 ITEM_TABLE + 2 entries, zeroed beforehand; list_total may be null
 void launch_item_table(const uint32_t* cand, const float2* top_aux, uint32_t ntop, uint32_t max_cand,
-                       unsigned long long* table, hipStream_t s);
+                       const uint32_t* list_total, unsigned long long* table, hipStream_t s);
+constexpr uint32_t ITEM_TABLE_WORDS = 2 * ITEM_TABLE + 2;   // 64-bit entries
 void launch_item_count(const uint32_t* cand, uint32_t chunk, uint32_t ntop, uint32_t* nitems, hipStream_t s);
 void launch_item_desc(const TravParams& p, uint4* desc, hipStream_t s);
 

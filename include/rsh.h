@@ -144,6 +144,31 @@ rsh_status rsh_engine_set_stream(rsh_engine* e, void* hip_stream);
 /* Upload a scene (host arrays) and precompute per-triangle data. */
 rsh_status rsh_engine_load_scene(rsh_engine* e, const rsh_scene_desc* scene);
 
+/* Move the loaded scene's objects, as the reference does every frame
+ * (main.cpp:1131-1196 driving RayTracer.cu:448-570): vertex v of object k
+ * becomes model_k * (x, y, z, w) of its position as loaded (w: the object id
+ * afterwards, as the reference stores it), its normal normal_k * (nx, ny, nz, 0),
+ * normalised; object k's sphere centre model_k * (c, 1) and radius r times the
+ * largest |diagonal entry| of scale_k's upper 3x3.  Host arrays of 16 floats per
+ * object, row-major (entry (i, j) at 4i + j).  Every call starts from the scene
+ * as loaded.  The per-triangle, cluster and sub-cluster data are rebuilt on
+ * the device and captured frame graphs are dropped. */
+rsh_status rsh_engine_update_objects(rsh_engine* e, const float* model, const float* normal, const float* scale);
+
+/* Object bounding spheres for a scene (host code, no device needed): the
+ * smallest ball enclosing each object's vertices (BoundingSphere.cpp:10-62,
+ * the reference's Gaertner Miniball; here Welzl's move-to-front algorithm in
+ * double precision).  Object k spans triangles ranges[2k]..ranges[2k+1] of
+ * tri_pos (float4 per vertex, 3 per triangle).  out: float3[2K] in the
+ * reference's layout (centre | radius, first, last), the form
+ * rsh_scene_desc.object_spheres takes.  mode RSH_SPHERE_MINIMAL: centre
+ * rounded to float, radius the largest vertex distance from it, rounded up;
+ * RSH_SPHERE_REFERENCE: the reference's radius min(1e5, max(r^4, largest
+ * vertex distance)), r^2 the ball's squared radius (its quirk Q12). */
+enum { RSH_SPHERE_MINIMAL = 0, RSH_SPHERE_REFERENCE = 1 };
+rsh_status rsh_object_spheres(const float* tri_pos, uint32_t triangle_total, const uint32_t* ranges,
+                              uint32_t object_total, uint32_t mode, float* out);
+
 /* Render one frame into the engine's pixel buffer.  The shadow passes run on
  * a second stream of the engine (and, eagerly, a host thread of their own),
  * concurrently with the primary / reflection passes on the engine's stream;

@@ -51,12 +51,37 @@ class OFrameOut(C.Structure):
                 ("pass_ray_tests", u64 * 16)]
 
 
+REF_SRC = "/root/reference/projects/CUDA-Engine/source"
+REF_MINIBALL = os.path.join(HERE, "_ref", "libref_miniball.so")
+
+
 def build(quiet: bool = True) -> str:
-    """Compile the oracle with its Makefile (g++, -ffp-contract=off)."""
+    """Compile the oracle with its Makefile (g++, -ffp-contract=off), and the
+    reference's Miniball behind a C entry point (oracle/_ref) where the
+    reference sources are present."""
     r = subprocess.run(["make", "-s", "-C", HERE], capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
+    if os.path.exists(os.path.join(REF_SRC, "Miniball.hpp")):
+        r = subprocess.run(["make", "-s", "-C", HERE, "ref"], capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("reference Miniball build failed:\n" + r.stdout + r.stderr)
     return LIB_PATH
+
+
+def ref_miniball(points):
+    """(centre, squared radius) of the reference's Miniball over float32
+    points [n, 3]; None when oracle/_ref was not built."""
+    if not os.path.exists(REF_MINIBALL):
+        return None
+    L = C.CDLL(REF_MINIBALL)
+    L.ref_miniball.restype = None
+    L.ref_miniball.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    c = np.zeros(3, np.float32)
+    r2 = np.zeros(1, np.float32)
+    L.ref_miniball(p.ctypes.data, p.shape[0], c.ctypes.data, r2.ctypes.data)
+    return c, float(r2[0])
 
 
 _lib = None

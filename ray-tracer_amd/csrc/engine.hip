@@ -77,7 +77,11 @@ struct rsh_engine {
     uint32_t NC = 0;
     DevBuf cl_sphere, sub_sphere, sub_use, cl_base, all_list;   // float4[NC], uint32[K] first cluster, uint4[K] all objects as
                                            // list entries (first, last, first cluster, 0) for RAH
-    std::vector<float> host_spheres;   // float3[2K]
+    std::vector<float> host_spheres;   // float3[2K] as loaded
+    DevBuf crange, srange;             // uint2 triangle ranges of the clusters and sub-clusters
+    // the scene as loaded, kept for rsh_engine_update_objects (made on its first call)
+    bool orig_ready = false;
+    DevBuf tri_pos0, tri_nrm0, tri_obj, sph0, mats, sph_tmp;
     uint32_t ranges_tb = 0xFFFFFFFFu;  // triangle batch the exclusion ranges were built for
 
     // frame buffers: closest hits and the bounce G-buffer alternate between
@@ -298,6 +302,14 @@ KeyCfg key_cfg(const rsh_frame_config& c) {
     k.h.min_x = c.scene_min[0]; k.h.min_y = c.scene_min[1]; k.h.min_z = c.scene_min[2];
     k.h.scale_x = c.cell_scale[0]; k.h.scale_y = c.cell_scale[1]; k.h.scale_z = c.cell_scale[2];
     return k;
+}
+
+// Cluster and sub-cluster spheres and the sub-cluster flags from the triangle
+// spheres (scene load and object updates).
+void build_clusters(rsh_engine* e, hipStream_t st) {
+    launch_cluster_build(f4p(e->tri_sphere), e->crange.as<uint2>(), e->NC, f4p(e->cl_sphere), st);
+    launch_cluster_build(f4p(e->tri_sphere), e->srange.as<uint2>(), e->NC * SUB_PER_CLUSTER, f4p(e->sub_sphere), st);
+    launch_sub_use(f4p(e->cl_sphere), f4p(e->sub_sphere), e->NC, e->sub_use.as<uint8_t>(), st);
 }
 
 struct PassOut {
@@ -587,7 +599,8 @@ void rsh_engine_destroy(rsh_engine* e) {
     for (DevBuf* b : {&e->tri_pos, &e->tri_nrm, &e->tri_mat, &e->mat_dif, &e->mat_spec, &e->light_pos, &e->light_col,
                       &e->light_int, &e->tri_sphere, &e->tri_mt, &e->obj_sphere, &e->obj_range, &e->cl_sphere,
                       &e->sub_sphere, &e->sub_use, &e->cl_base, &e->all_list, &e->rays_a, &e->rays_b, &e->closest[0],
-                      &e->closest[1], &e->shadow, &e->pixels, &e->views_d})
+                      &e->closest[1], &e->shadow, &e->pixels, &e->views_d, &e->crange, &e->srange, &e->tri_pos0,
+                      &e->tri_nrm0, &e->tri_obj, &e->sph0, &e->mats, &e->sph_tmp})
         b->release();
     if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
@@ -684,21 +697,16 @@ rsh_status rsh_engine_load_scene(rsh_engine* e, const rsh_scene_desc* s) {
     HIP_OK(e->cl_sphere.reserve((size_t)nc * 16)); HIP_OK(e->cl_base.reserve((size_t)K * 4));
     HIP_OK(e->sub_sphere.reserve((size_t)nc * SUB_PER_CLUSTER * 16)); HIP_OK(e->sub_use.reserve((size_t)nc + 4));
     HIP_OK(e->all_list.reserve((size_t)K * 16));
-    DevBuf crange_d;
-    HIP_OK(crange_d.reserve(crange.size() * 4));
-    HIP_OK(hipMemcpyAsync(crange_d.p, crange.data(), crange.size() * 4, hipMemcpyHostToDevice, st));
+    HIP_OK(e->crange.reserve(crange.size() * 4));
+    HIP_OK(e->srange.reserve(srange.size() * 4));
+    HIP_OK(hipMemcpyAsync(e->crange.p, crange.data(), crange.size() * 4, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(e->srange.p, srange.data(), srange.size() * 4, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(e->cl_base.p, base.data(), (size_t)K * 4, hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(e->all_list.p, ident.data(), (size_t)K * 16, hipMemcpyHostToDevice, st));
-    launch_cluster_build(f4p(e->tri_sphere), crange_d.as<uint2>(), nc, f4p(e->cl_sphere), st);
+    build_clusters(e, st);
     HIP_OK(hipGetLastError());
     HIP_OK(hipStreamSynchronize(st));
-    HIP_OK(crange_d.reserve(srange.size() * 4));
-    HIP_OK(hipMemcpyAsync(crange_d.p, srange.data(), srange.size() * 4, hipMemcpyHostToDevice, st));
-    launch_cluster_build(f4p(e->tri_sphere), crange_d.as<uint2>(), nc * SUB_PER_CLUSTER, f4p(e->sub_sphere), st);
-    launch_sub_use(f4p(e->cl_sphere), f4p(e->sub_sphere), nc, e->sub_use.as<uint8_t>(), st);
-    HIP_OK(hipGetLastError());
-    HIP_OK(hipStreamSynchronize(st));
-    crange_d.release();
+    e->orig_ready = false;
     e->ranges_tb = 0xFFFFFFFFu;
     e->scene_epoch++;
     return RSH_OK;
@@ -1208,6 +1216,57 @@ int rsh_device_count(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;
     return n;
+}
+
+// RayTracer.cu:448-570 (UpdateVertex, UpdateBoundingSphere) as driven per
+// frame by main.cpp:1131-1196, on the engine's scene: positions and normals
+// from the scene as loaded, then the per-triangle, cluster and sub-cluster
+// data rebuilt; captured frame graphs are dropped.
+rsh_status rsh_engine_update_objects(rsh_engine* e, const float* model, const float* normal, const float* scale) {
+    if (!e || !model || !normal || !scale) return e ? (fail(e, "update_objects: null argument"), RSH_INVALID_ARG)
+                                                    : RSH_INVALID_ARG;
+    if (e->T == 0) { e->err = "update_objects: no scene loaded"; return RSH_INVALID_ARG; }
+    HIP_OK(hipSetDevice(e->device));
+    hipStream_t st = e->stream;
+    const uint32_t T = e->T, K = e->K;
+    if (!e->orig_ready) {
+        HIP_OK(e->tri_pos0.reserve((size_t)T * 48)); HIP_OK(e->tri_nrm0.reserve((size_t)T * 48));
+        HIP_OK(hipMemcpyAsync(e->tri_pos0.p, e->tri_pos.p, (size_t)T * 48, hipMemcpyDeviceToDevice, st));
+        HIP_OK(hipMemcpyAsync(e->tri_nrm0.p, e->tri_nrm.p, (size_t)T * 48, hipMemcpyDeviceToDevice, st));
+        std::vector<int32_t> ids(3 * (size_t)T);
+        std::vector<float> s8(8 * (size_t)K);     // (centre, 1), (radius, first, last, 1) per object
+        for (uint32_t k = 0; k < K; k++) {
+            const float* h = &e->host_spheres[6 * (size_t)k];
+            const uint32_t first = (uint32_t)h[4], last = (uint32_t)h[5];
+            for (uint32_t t = first; t <= last; t++) ids[3 * (size_t)t] = ids[3 * (size_t)t + 1] = ids[3 * (size_t)t + 2] = (int32_t)k;
+            float* o = &s8[8 * (size_t)k];
+            o[0] = h[0]; o[1] = h[1]; o[2] = h[2]; o[3] = 1.0f;
+            o[4] = h[3]; o[5] = h[4]; o[6] = h[5]; o[7] = 1.0f;
+        }
+        HIP_OK(e->tri_obj.reserve(ids.size() * 4)); HIP_OK(e->sph0.reserve(s8.size() * 4));
+        HIP_OK(hipMemcpyAsync(e->tri_obj.p, ids.data(), ids.size() * 4, hipMemcpyHostToDevice, st));
+        HIP_OK(hipMemcpyAsync(e->sph0.p, s8.data(), s8.size() * 4, hipMemcpyHostToDevice, st));
+        HIP_OK(hipStreamSynchronize(st));     // the host vectors go out of scope
+        e->orig_ready = true;
+    }
+    const size_t mb = (size_t)K * 64;
+    HIP_OK(e->mats.reserve(3 * mb)); HIP_OK(e->sph_tmp.reserve((size_t)K * 24));
+    float* dm = e->mats.as<float>();
+    HIP_OK(hipMemcpyAsync(dm, model, mb, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(dm + 16 * (size_t)K, normal, mb, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(dm + 32 * (size_t)K, scale, mb, hipMemcpyHostToDevice, st));
+    launch_update_vertex(dm, dm + 16 * (size_t)K, 3 * T, f4p(e->tri_pos0), f4p(e->tri_nrm0), e->tri_obj.as<int32_t>(),
+                         f4p(e->tri_pos), f4p(e->tri_nrm), st);
+    // the reference's translation matrix is the model matrix (main.cpp:1172-1179)
+    launch_update_sphere(dm, dm + 32 * (size_t)K, K, f4p(e->sph0), e->sph_tmp.as<float>(), st);
+    HIP_OK(hipMemcpy2DAsync(e->obj_sphere.p, 16, e->sph_tmp.p, 24, 16, K, hipMemcpyDeviceToDevice, st));
+    launch_triangle_prep(f4p(e->tri_pos), T, f4p(e->tri_sphere), f4p(e->tri_mt), st);
+    build_clusters(e, st);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(st));
+    drop_graph(e);
+    e->scene_epoch++;
+    return RSH_OK;
 }
 
 }  // extern "C"

@@ -106,7 +106,8 @@ NATIVE_SYMBOLS = ["rsh_engine_create", "rsh_engine_destroy", "rsh_last_error", "
                   "rsh_device_alloc", "rsh_device_free", "rsh_memcpy_htod", "rsh_memcpy_dtoh",
                   "rsh_memset_d32", "rsh_device_synchronize", "rsh_device_count",
                   "rsh_compat_configure", "rsh_compat_set_stream", "rsh_compat_set_exit_on_error",
-                  "rsh_compat_status", "rsh_compat_error", "rsh_compat_reset_status", "rsh_selfcheck_rcp"]
+                  "rsh_compat_status", "rsh_compat_error", "rsh_compat_reset_status", "rsh_selfcheck_rcp",
+                  "rsh_object_spheres", "rsh_engine_update_objects"]
 
 _lib = None
 
@@ -187,6 +188,28 @@ def selfcheck_rcp(device=0):
     if rc != 0:
         raise RshError(f"rsh_selfcheck_rcp failed with HIP error {rc}")
     return t.value, m.value
+
+
+SPHERE_MINIMAL, SPHERE_REFERENCE = 0, 1
+
+
+def object_spheres(tri_pos, ranges, mode=SPHERE_MINIMAL):
+    """Object bounding spheres (rsh_object_spheres, host code): tri_pos float32
+    [3T, 4] (or [T, 3, 4]), ranges uint32 [K, 2] (first, last triangle) ->
+    float32 [K, 6] (centre | radius, first, last), the smallest ball around
+    each object's vertices (SPHERE_MINIMAL) or with the reference's radius
+    (SPHERE_REFERENCE)."""
+    pos = np.ascontiguousarray(tri_pos, np.float32).reshape(-1, 4)
+    rg = np.ascontiguousarray(ranges, np.uint32).reshape(-1, 2)
+    out = np.empty((rg.shape[0], 6), np.float32)
+    L = lib()
+    L.rsh_object_spheres.restype = C.c_int
+    L.rsh_object_spheres.argtypes = [PV, u32, PV, u32, u32, PV]
+    rc = L.rsh_object_spheres(pos.ctypes.data, pos.shape[0] // 3, rg.ctypes.data, rg.shape[0], mode,
+                              out.ctypes.data)
+    if rc != 0:
+        raise RshError(f"rsh_object_spheres failed ({rc})")
+    return out
 
 
 # ------------------------------------------------------------ device memory --
@@ -303,6 +326,17 @@ class Engine:
     def load_scene(self, scene):
         self._scene = scene
         self._check(self.L.rsh_engine_load_scene(self.h, C.byref(scene_desc(scene))), "load_scene")
+
+    def update_objects(self, model, normal, scale):
+        """Move the loaded scene's objects (rsh_engine_update_objects): float32
+        [K, 16] row-major model, normal and scale matrices, applied to the scene
+        as loaded."""
+        K = self._scene.object_total
+        m, n, s = (np.ascontiguousarray(a, np.float32).reshape(K, 16) for a in (model, normal, scale))
+        self.L.rsh_engine_update_objects.restype = C.c_int
+        self.L.rsh_engine_update_objects.argtypes = [PV, PV, PV, PV]
+        self._check(self.L.rsh_engine_update_objects(self.h, m.ctypes.data, n.ctypes.data, s.ctypes.data),
+                    "update_objects")
 
     def render(self, camera, cfg, flags=0) -> FrameStats:
         self.cfg = cfg

@@ -186,3 +186,65 @@ def test_invalid_arguments_are_rejected(engine, rsh):
     broken.object_spheres[1, 4] += 1.0        # gap in the object tiling
     with pytest.raises(rsh.RshError):
         engine.load_scene(broken)
+
+
+def _object_transforms(scene, seed):
+    """Model / normal / scale matrices per object (row-major 4x4): the room
+    stays, every other object turns about its sphere's centre, scales and
+    moves a little (the reference's per-frame update, main.cpp:1131-1196)."""
+    rng = np.random.default_rng(seed)
+    K = scene.object_total
+    model, normal, scale = (np.tile(np.eye(4), (K, 1, 1)) for _ in range(3))
+    for k in range(1, K):
+        c = scene.object_spheres[k, :3].astype(np.float64)
+        a = rng.normal(size=3)
+        a /= np.linalg.norm(a)
+        th = rng.uniform(0.2, np.pi)
+        X = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+        R = np.eye(3) + np.sin(th) * X + (1 - np.cos(th)) * X @ X
+        s = rng.uniform(0.8, 1.2)
+        M3 = R * s
+        model[k, :3, :3] = M3
+        model[k, :3, 3] = c + rng.uniform(-3.0, 3.0, 3) - M3 @ c
+        normal[k, :3, :3] = np.linalg.inv(M3).T
+        scale[k] = np.diag([s, s, s, 1.0])
+    return [m.astype(np.float32).reshape(K, 16) for m in (model, normal, scale)]
+
+
+def _moved_scene(oracle, scene, model, normal, scale):
+    """The scene after the oracle's update wrappers (RayTracer.cu:448-570)."""
+    T, K = scene.triangle_total, scene.object_total
+    obj = np.repeat(scene.triangle_object(), 3).astype(np.int32)
+    sph4 = np.zeros((2 * K, 4), np.float32)
+    sph4[0::2, :3] = scene.object_spheres[:, :3]
+    sph4[0::2, 3] = 1.0
+    sph4[1::2, :3] = scene.object_spheres[:, 3:6]
+    sph4[1::2, 3] = 1.0
+    oracle.call("oracle_bind_original", scene.tri_pos, scene.tri_nrm, obj, sph4)
+    pos, nrm = np.empty_like(scene.tri_pos), np.empty_like(scene.tri_nrm)
+    oracle.call("oracle_TriangleUpdateWrapper", model, normal, T, pos, nrm)
+    sph = np.empty((K, 6), np.float32)
+    oracle.call("oracle_BoundingSphereUpdateWrapper", model, scale, K, sph)
+    return dataclasses.replace(scene, tri_pos=pos, tri_nrm=nrm, object_spheres=sph)
+
+
+def test_moved_objects_match_oracle(engine, oracle, rsh):
+    # two updates in a row, each applied to the scene as loaded
+    scene, cam = c0(rsh, 96)
+    cfg = dataclasses.replace(rsh.config.config0(96, 96), bounces=2, bounce_shadows=1)
+    engine.load_scene(scene)
+    for seed in (1, 2):
+        model, normal, scale = _object_transforms(scene, seed)
+        engine.update_objects(model, normal, scale)
+        engine.render(cam, cfg, flags=engine.KEEP_PASSES)
+        ofr = oracle.render(_moved_scene(oracle, scene, model, normal, scale), cam, cfg, threads=0)
+        print(seed, compare_frames(engine, ofr, cfg, strict=True), file=sys.stderr)
+
+
+def test_minimal_object_spheres_match_oracle(engine, oracle, rsh):
+    # scene with the smallest enclosing balls as object spheres (rsh_object_spheres)
+    scene = rsh.scenes.config1_scene(boxes=300)
+    rg = scene.object_spheres[:, 4:6].astype(np.uint32)
+    tight = dataclasses.replace(scene, object_spheres=rsh.object_spheres(scene.tri_pos, rg))
+    cam = rsh.scenes.config1_camera(128, 128)
+    run_case(engine, oracle, rsh, tight, cam, rsh.config.config1(128, 128))

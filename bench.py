@@ -61,6 +61,7 @@ def parse():
     ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the CPU sample (0: automatic)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-views leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--no-update", action="store_true", help="skip the object-update leg")
     ap.add_argument("--serve-seconds", type=float, default=6.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="copy MIB MiB device-to-device 3 times with a plain copy kernel and exit")
@@ -163,6 +164,65 @@ def hsc_figure(passes, stages, digit_passes):
     gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     return {"achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS,
             "bytes": int(nbytes), "ms": ms, "kind": "effective (reference stage bytes / time)"}
+
+
+def update_leg(rsh_amd, eng, scene, cam, cfg, reps=5):
+    """Object bounding spheres and per-frame object updates on the atrium:
+    rsh_object_spheres (host, the smallest enclosing balls of all objects),
+    rsh_engine_update_objects (every object but the first turned, scaled and
+    moved; positions, normals and spheres recomputed from the loaded scene,
+    the derived triangle / cluster data rebuilt), and the frame rate with the
+    smallest balls as object spheres, and with objects moving every frame."""
+    import dataclasses
+    import numpy as np
+    rg = scene.object_spheres[:, 4:6].astype(np.uint32)
+    t0 = time.perf_counter()
+    tight = rsh_amd.object_spheres(scene.tri_pos, rg)
+    spheres_ms = 1e3 * (time.perf_counter() - t0)
+    K, T = scene.object_total, scene.triangle_total
+    rng = np.random.default_rng(7)
+    mats = []
+    for _ in range(reps):
+        model, normal, scale = (np.tile(np.eye(4), (K, 1, 1)) for _ in range(3))
+        for k in range(1, K):
+            c = scene.object_spheres[k, :3].astype(np.float64)
+            th = rng.uniform(-0.05, 0.05)
+            R = np.array([[np.cos(th), 0, np.sin(th)], [0, 1, 0], [-np.sin(th), 0, np.cos(th)]])
+            model[k, :3, :3] = R
+            model[k, :3, 3] = c + rng.uniform(-0.2, 0.2, 3) - R @ c
+            normal[k, :3, :3] = R
+        mats.append([m.astype(np.float32).reshape(K, 16) for m in (model, normal, scale)])
+    times = []
+    for m in mats:
+        t0 = time.perf_counter()
+        eng.update_objects(*m)
+        times.append(time.perf_counter() - t0)
+    update_ms = 1e3 * statistics.median(times)
+    # frames of a scene whose objects move every frame (update + render)
+    rays = 0
+    t0 = time.perf_counter()
+    for m in mats:
+        eng.update_objects(*m)
+        rays += eng.render(cam, cfg).rays_total
+    moving = rays / (time.perf_counter() - t0) / 1e6
+    # the same frame with the smallest balls as object spheres
+    eng.load_scene(dataclasses.replace(scene, object_spheres=tight))
+    eng.render(cam, cfg)
+    rays = 0
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        rays += eng.render(cam, cfg).rays_total
+    tight_mrays = rays / (time.perf_counter() - t0) / 1e6
+    eng.load_scene(scene)
+    # algorithmic bytes of the update kernels: vertices (position + normal in
+    # and out, object id), triangles (positions in, sphere and edge vectors out)
+    upd_bytes = 3 * T * (16 + 16 + 16 + 16 + 4) + T * (48 + 16 + 48)
+    return {"object_spheres_ms": spheres_ms, "objects": K, "triangles": T,
+            "update_ms": update_ms, "update_bytes": upd_bytes,
+            "update_GBps": upd_bytes / (update_ms * 1e-3) / 1e9,
+            "moving_objects_mrays": moving, "smallest_ball_spheres_mrays": tight_mrays,
+            "sphere_radius_sum_ratio": float(tight[:, 3].sum() / scene.object_spheres[:, 3].sum()),
+            "note": "update_ms is the host call (three 16-float matrices per object uploaded, kernels, sync)"}
 
 
 def batching_leg(rsh_amd, eng, scene2):
@@ -275,6 +335,9 @@ def main():
     hsc["workload"] = "the timed atrium frames (8 passes, %d rays per frame)" % int(statistics.median(rays))
 
     progress(f"frames: {value:.1f} Mrays/s")
+    dynamic = None if a.no_update else update_leg(rsh_amd, eng, scene, cam, cfg)
+    if dynamic:
+        progress("object-update leg done")
     if not a.no_batch:
         batching = batching_leg(rsh_amd, eng, scene)
         progress("batching leg done")
@@ -302,6 +365,8 @@ def main():
         "frame_ms_median": 1e3 * statistics.median(times),
     }
     eng.close()
+    if dynamic:
+        out["object_update"] = dynamic
     if not a.no_batch:
         out["batching"] = batching
     if not a.no_serve:

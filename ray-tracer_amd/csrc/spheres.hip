@@ -24,7 +24,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
-#include <list>
+#include <thread>
 #include <vector>
 
 #include "../../include/rsh.h"
@@ -125,25 +125,57 @@ Ball ball_of(const P3* s, int n) {
 }
 
 struct Mtf {
-    std::list<P3> pts;
+    std::vector<P3> pts;
     P3 sup[4];
     int ns = 0;
-    Ball run(std::list<P3>::iterator end) {
+    // points [0, end) inside the ball with the support set on its boundary
+    Ball run(size_t end) {
         Ball B = ball_of(sup, ns);
         if (ns == 4) return B;
-        for (auto it = pts.begin(); it != end;) {
-            auto next = std::next(it);
-            if (!B.contains(*it)) {
-                sup[ns++] = *it;
-                B = run(it);
+        for (size_t i = 0; i < end; i++) {
+            if (!B.contains(pts[i])) {
+                sup[ns++] = pts[i];
+                B = run(i);
                 ns--;
-                pts.splice(pts.begin(), pts, it);     // move to front
+                std::rotate(pts.begin(), pts.begin() + i, pts.begin() + i + 1);     // move to front
             }
-            it = next;
         }
         return B;
     }
 };
+
+// Sphere of object k (float3[2] at out).
+void object_sphere(const float* tri_pos, uint32_t first, uint32_t last, uint32_t mode, float* o) {
+    Mtf m;
+    m.pts.reserve(3 * (size_t)(last - first + 1));
+    for (uint32_t t = first; t <= last; t++)
+        for (int v = 0; v < 3; v++) {
+            const float* p = tri_pos + 4 * (3 * (size_t)t + v);
+            m.pts.push_back({p[0], p[1], p[2]});
+        }
+    const Ball B = m.run(m.pts.size());
+    float c[3] = {(float)B.c.x, (float)B.c.y, (float)B.c.z};
+    // largest vertex distance from the float centre: exact (rounded up) and
+    // as a float evaluation of |v - c| would give it (the reference's
+    // revision step), plus one ulp
+    double dmax = 0.0;
+    float fmax = 0.0f;
+    for (const P3& p : m.pts) {
+        const double dx = p.x - c[0], dy = p.y - c[1], dz = p.z - c[2];
+        dmax = std::max(dmax, std::sqrt(dx * dx + dy * dy + dz * dz));
+        const float fx = (float)p.x - c[0], fy = (float)p.y - c[1], fz = (float)p.z - c[2];
+        fmax = std::max(fmax, std::sqrt(fx * fx + fy * fy + fz * fz));
+    }
+    float r = (float)dmax;
+    if ((double)r < dmax) r = std::nextafter(r, INFINITY);
+    r = std::nextafter(std::max(r, fmax), INFINITY);
+    if (mode == 1) {
+        const float sq = (float)B.r2;
+        r = std::min(100000.0f, std::max(sq * sq, r));
+    }
+    o[0] = c[0]; o[1] = c[1]; o[2] = c[2];
+    o[3] = r; o[4] = (float)first; o[5] = (float)last;
+}
 
 }  // namespace
 
@@ -156,39 +188,17 @@ extern "C" {
 rsh_status rsh_object_spheres(const float* tri_pos, uint32_t T, const uint32_t* ranges, uint32_t K, uint32_t mode,
                               float* out) {
     if (!tri_pos || !ranges || !out || mode > 1) return RSH_INVALID_ARG;
-    for (uint32_t k = 0; k < K; k++) {
-        const uint32_t first = ranges[2 * k], last = ranges[2 * k + 1];
-        if (first > last || last >= T) return RSH_INVALID_ARG;
-        Mtf m;
-        for (uint32_t t = first; t <= last; t++)
-            for (int v = 0; v < 3; v++) {
-                const float* p = tri_pos + 4 * (3 * (size_t)t + v);
-                m.pts.push_back({p[0], p[1], p[2]});
-            }
-        const Ball B = m.run(m.pts.end());
-        float c[3] = {(float)B.c.x, (float)B.c.y, (float)B.c.z};
-        // largest vertex distance from the float centre: exact (rounded up) and
-        // as a float evaluation of |v - c| would give it (the reference's
-        // revision step), plus one ulp
-        double dmax = 0.0;
-        float fmax = 0.0f;
-        for (const P3& p : m.pts) {
-            const double dx = p.x - c[0], dy = p.y - c[1], dz = p.z - c[2];
-            dmax = std::max(dmax, std::sqrt(dx * dx + dy * dy + dz * dz));
-            const float fx = (float)p.x - c[0], fy = (float)p.y - c[1], fz = (float)p.z - c[2];
-            fmax = std::max(fmax, std::sqrt(fx * fx + fy * fy + fz * fz));
-        }
-        float r = (float)dmax;
-        if ((double)r < dmax) r = std::nextafter(r, INFINITY);
-        r = std::nextafter(std::max(r, fmax), INFINITY);
-        if (mode == 1) {
-            const float sq = (float)B.r2;
-            r = std::min(100000.0f, std::max(sq * sq, r));
-        }
-        float* o = out + 6 * (size_t)k;
-        o[0] = c[0]; o[1] = c[1]; o[2] = c[2];
-        o[3] = r; o[4] = (float)first; o[5] = (float)last;
-    }
+    for (uint32_t k = 0; k < K; k++)
+        if (ranges[2 * k] > ranges[2 * k + 1] || ranges[2 * k + 1] >= T) return RSH_INVALID_ARG;
+    // objects are independent: a few host threads take every n-th one
+    const unsigned nt = std::max(1u, std::min(16u, std::min(std::thread::hardware_concurrency(), K / 8 + 1)));
+    std::vector<std::thread> pool;
+    for (unsigned w = 0; w < nt; w++)
+        pool.emplace_back([=] {
+            for (uint32_t k = w; k < K; k += nt)
+                object_sphere(tri_pos, ranges[2 * k], ranges[2 * k + 1], mode, out + 6 * (size_t)k);
+        });
+    for (auto& t : pool) t.join();
     return RSH_OK;
 }
 

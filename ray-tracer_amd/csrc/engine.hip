@@ -332,13 +332,15 @@ rsh_status build_items(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, Trav
     LANE_OK(L.w_pmask.reserve(((size_t)ntop + 1) * ((e->K + 63) / 64) * 8 + 8));
     unsigned long long* pmask = L.w_pmask.as<unsigned long long>();
     launch_cull_count(p, npass, cand, pmask, s);
-    if (p.algorithm != 1) LANE_OK(run_exclusive_scan(npass, u32p(L.w_off), (uint64_t)ntop + 1, L.sc, s));
+    // the list offsets: exclusive scan of npass, done by the item scan's launch
+    const uint32_t* np_scan = p.algorithm != 1 ? npass : nullptr;
     uint32_t chunk = cfg.traversal_chunk ? cfg.traversal_chunk : ITEM_CHUNK;
     uint32_t tot[2] = {0, 0};   // list entries, items
     if (e->replaying) {
         const PassPlan& pp = L.plan[L.plan_pos - 1];
         chunk = pp.chunk; tot[0] = pp.list_total; tot[1] = pp.items; p.prefilter = pp.mode;
-        LANE_OK(run_item_scan(cand, ntop, chunk, e->NC, nitems, u32p(L.w_first), L.sc, s));
+        LANE_OK(run_item_scan(cand, ntop, chunk, e->NC, nitems, u32p(L.w_first), L.sc, s, nullptr, 0, np_scan,
+                              u32p(L.w_off)));
     }
     if (!e->replaying) {
         // One read-back sizes the pass: the list total and, from the table of
@@ -352,11 +354,10 @@ rsh_status build_items(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, Trav
         unsigned long long* table = L.w_table.as<unsigned long long>();
         if (!L.table_clean) launch_fill_u64(table, 0ull, ITEM_TABLE_WORDS, s);   // new, or a pass that failed
         L.table_clean = false;
-        launch_item_table(cand, p.aux + p.level_offset[1], ntop, e->NC, p.algorithm != 1 ? u32p(L.w_off) + ntop : nullptr,
-                          table, s);
+        launch_item_table(cand, p.aux + p.level_offset[1], ntop, e->NC, np_scan, table, s);
         LANE_OK(hipMemcpyAsync(L.hstage + 2, table, sizeof(ht), hipMemcpyDeviceToHost, s));
         if (!tabled) {
-            LANE_OK(run_item_scan(cand, ntop, chunk, e->NC, nitems, first, L.sc, s));
+            LANE_OK(run_item_scan(cand, ntop, chunk, e->NC, nitems, first, L.sc, s, nullptr, 0, np_scan, u32p(L.w_off)));
             LANE_OK(hipMemcpyAsync(L.hstage + 1, first + ntop, 4, hipMemcpyDeviceToHost, s));
         }
         LANE_OK(readback_wait(L));
@@ -394,7 +395,8 @@ rsh_status build_items(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, Trav
             }
             tot[1] = (uint32_t)items_at(chunk);
             // the item scan also clears the table for the next pass
-            LANE_OK(run_item_scan(cand, ntop, chunk, e->NC, nitems, first, L.sc, s, table, ITEM_TABLE_WORDS));
+            LANE_OK(run_item_scan(cand, ntop, chunk, e->NC, nitems, first, L.sc, s, table, ITEM_TABLE_WORDS, np_scan,
+                                  u32p(L.w_off)));
             L.table_clean = true;
         }
     }
@@ -405,8 +407,10 @@ rsh_status build_items(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, Trav
     p.obj_off = u32p(L.w_off); p.obj_ent = L.w_list.as<uint4>(); p.obj_cum = u32p(L.w_cum);
     if (p.algorithm == 1) { p.obj_ent = e->all_list.as<uint4>(); p.obj_cum = u32p(e->cl_base); }
     p.chunk = chunk; p.items = tot[1];
-    launch_cull_list(p, L.w_list.as<uint4>(), u32p(L.w_cum), pmask, s);     // writes at p.obj_off
-    launch_item_desc(p, L.w_items.as<uint4>(), s);
+    if (p.algorithm != 1)
+        launch_cull_items(p, L.w_list.as<uint4>(), u32p(L.w_cum), pmask, L.w_items.as<uint4>(), s);   // at p.obj_off
+    else
+        launch_item_desc(p, L.w_items.as<uint4>(), s);
     LANE_OK(hipGetLastError());
     out.items = tot[1]; out.chunk = chunk; out.mode = p.prefilter;
     if (e->recording) {

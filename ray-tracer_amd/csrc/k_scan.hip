@@ -293,16 +293,12 @@ __device__ __forceinline__ void load16_live(const uint32_t* p, uint64_t base, ui
 // the run offsets come from one block-wide sum, pass 2 scans and writes.
 // out[n - 1] closes the scan when the last input is 0.  MODE 1 scans item
 // counts ceil(in[i] / chunk) for i < n - 1 and 0 for the last entry (the
-// work-list scan of the traversal).  Each value is read before it is written
-// by the same lane, so in == out is allowed.
-// MODE 1 divides by multiply-high when `fast` (every count + chunk - 1 within
-// div_hi's bound).
+// work-list scan of the traversal), dividing by multiply-high when `fast`
+// (every count + chunk - 1 within div_hi's bound).  Each value is read before
+// it is written by the same lane, so in == out is allowed.  Returns the total.
 template <int MODE>
-__global__ void __launch_bounds__(1024) k_scan_block(const uint32_t* in, uint32_t n, uint32_t chunk, bool fast,
-                                                     uint32_t* out, uint32_t* total, unsigned long long* zero,
-                                                     uint32_t zero_n) {
-    __shared__ uint32_t red[16];
-    if (zero && threadIdx.x < zero_n) zero[threadIdx.x] = 0ull;
+__device__ uint32_t block_scan_run(const uint32_t* in, uint32_t n, uint32_t chunk, bool fast, uint32_t* out,
+                                   uint32_t* red) {
     const uint32_t lane = lane_id(), w = wave_id();
     const uint32_t wper = ((n + 16u * 256u - 1u) / (16u * 256u)) * 256u;
     const uint32_t wb = w * wper;
@@ -340,7 +336,21 @@ __global__ void __launch_bounds__(1024) k_scan_block(const uint32_t* in, uint32_
         if (i + 3u < we) out[i + 3u] = run;
         carry += __shfl(inc, 63, 64);
     }
+    return tot;
+}
+
+// One workgroup: the scan above of `in` (MODE), then, when in2 is given, a
+// plain exclusive scan of in2 (n values) into out2; zero[0, zero_n) is
+// cleared on the way.
+template <int MODE>
+__global__ void __launch_bounds__(1024) k_scan_block(const uint32_t* in, uint32_t n, uint32_t chunk, bool fast,
+                                                     uint32_t* out, uint32_t* total, unsigned long long* zero,
+                                                     uint32_t zero_n, const uint32_t* in2, uint32_t* out2) {
+    __shared__ uint32_t red[16];
+    if (zero && threadIdx.x < zero_n) zero[threadIdx.x] = 0ull;
+    const uint32_t tot = block_scan_run<MODE>(in, n, chunk, fast, out, red);
     if (threadIdx.x == 0 && total) *total = tot;
+    if (in2) (void)block_scan_run<0>(in2, n, 1u, true, out2, red);
 }
 
 // A thread's 16 slots: live mask, heads after the thread's first live key,
@@ -835,7 +845,8 @@ static hipError_t run_scan(const uint32_t* in, uint32_t* out, uint64_t n, ScanSc
     if (nt > MAX_TILES) return hipErrorInvalidValue;
     if (nt == 0) return hipSuccess;
     if (!inclusive && n <= SMALL_SCAN) {     // one launch instead of three
-        k_scan_block<0><<<1, 1024, 0, s>>>(in, (uint32_t)n, 1u, true, out, sc.total.as<uint32_t>(), nullptr, 0u);
+        k_scan_block<0><<<1, 1024, 0, s>>>(in, (uint32_t)n, 1u, true, out, sc.total.as<uint32_t>(), nullptr, 0u,
+                                            nullptr, nullptr);
         return hipGetLastError();
     }
     uint32_t* tiles = sc.tile_counts.as<uint32_t>();
@@ -855,14 +866,18 @@ hipError_t run_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, Sca
 
 hipError_t run_item_scan(const uint32_t* cand, uint32_t ntop, uint32_t chunk, uint32_t max_cand, uint32_t* counts,
                          uint32_t* first, ScanScratch& sc, hipStream_t s, unsigned long long* zero,
-                         uint32_t zero_n) {
+                         uint32_t zero_n, const uint32_t* npass, uint32_t* off) {
     if (chunk == 0 || zero_n > 1024) return hipErrorInvalidValue;
     if ((uint64_t)ntop + 1 <= SMALL_SCAN) {
         const bool fast = div_hi_ok((uint64_t)max_cand + chunk - 1, chunk);
-        k_scan_block<1><<<1, 1024, 0, s>>>(cand, ntop + 1, chunk, fast, first, nullptr, zero, zero_n);
+        k_scan_block<1><<<1, 1024, 0, s>>>(cand, ntop + 1, chunk, fast, first, nullptr, zero, zero_n, npass, off);
         return hipGetLastError();
     }
     if (zero) launch_fill_u64(zero, 0ull, zero_n, s);
+    if (npass) {
+        hipError_t e = run_exclusive_scan(npass, off, (uint64_t)ntop + 1, sc, s);
+        if (e != hipSuccess) return e;
+    }
     k_item_counts<<<ntop / 256 + 1, 256, 0, s>>>(cand, chunk, ntop, counts);
     return run_scan(counts, first, (uint64_t)ntop + 1, sc, s, false);
 }

@@ -400,14 +400,15 @@ __global__ void __launch_bounds__(256) k_item_count(const uint32_t* cand, uint32
 // c) for c = 1..ITEM_TABLE, table[ITEM_TABLE + c - 1] the same for chunk 2c;
 // table[2 ITEM_TABLE] = top nodes whose cone is wide (tangent of the spread
 // outside [0, WIDE_TAN]: a spread clamped to pi/2 has a large negative float
-// tangent; or not finite); table[2 ITEM_TABLE + 1] = *list_total (the pass's
-// list entries, so one read-back carries everything).  The table must start
-// zeroed (the item scan clears it for the next pass).  One wave per slice of TABLE_SLICE top nodes,
+// tangent; or not finite); table[2 ITEM_TABLE + 1] = the sum of npass (the
+// pass's list entries; npass may be null), so one read-back carries
+// everything.  The table must start zeroed (the item scan clears it for the
+// next pass).  One wave per slice of TABLE_SLICE top nodes,
 // lane c - 1 sums chunk sizes c and 2c in registers; one read-back of the
 // table sizes a pass and picks its traversal mode.
 constexpr uint32_t TABLE_SLICE = 256;
 __global__ void __launch_bounds__(64) k_item_table(const uint32_t* cand, const float2* top_aux, uint32_t ntop,
-                                                   unsigned long long* table, bool fast, const uint32_t* list_total) {
+                                                   unsigned long long* table, bool fast, const uint32_t* npass) {
     __shared__ uint4 s_cand4[TABLE_SLICE / 4];
     uint32_t* s_cand = reinterpret_cast<uint32_t*>(s_cand4);
     const uint32_t lane = threadIdx.x & 63u, c = lane + 1u;
@@ -446,7 +447,12 @@ __global__ void __launch_bounds__(64) k_item_table(const uint32_t* cand, const f
     for (uint32_t t = t0 + lane; t < t1; t += 64u) wide += !(top_aux[t].y >= 0.0f && top_aux[t].y <= WIDE_TAN) ? 1u : 0u;
     for (int d = 32; d > 0; d >>= 1) wide += __shfl_xor(wide, d, 64);
     if (lane == 0 && wide) atomicAdd(&table[2 * ITEM_TABLE], (unsigned long long)wide);
-    if (blockIdx.x == 0 && lane == 0) table[2 * ITEM_TABLE + 1] = list_total ? *list_total : 0u;
+    if (npass) {
+        unsigned long long np = 0ull;
+        for (uint32_t t = t0 + lane; t < t1; t += 64u) np += npass[t];
+        for (int d = 32; d > 0; d >>= 1) np += __shfl_xor(np, d, 64);
+        if (lane == 0 && np) atomicAdd(&table[2 * ITEM_TABLE + 1], np);
+    }
 }
 
 // Work-item descriptors, one wave per top node, one lane per item:
@@ -471,6 +477,55 @@ __global__ void __launch_bounds__(256) k_item_desc(TravParams P, uint4* desc) {
             if (P.obj_cum[lbase + m] <= s0) a = m; else b = m;
         }
         desc[f + c] = make_uint4(top, lbase + a, s0 - P.obj_cum[lbase + a], rem | (k == 1 ? 0x80000000u : 0u));
+    }
+}
+
+// List pass and work-item descriptors in one (CRSH): one wave per top node
+// walks the passing objects of its count-pass mask in order, writes each as a
+// list entry with the node's running cluster count before it (as k_cull<true>)
+// and, for the items whose first cluster s0 = i * chunk falls inside the
+// object's clusters [c0, c0 + ncl), the descriptor (top, entry, s0 - c0,
+// clusters | single << 31) -- the entry k_item_desc finds by binary search.
+// Every listed object has at least one cluster, so each item's start lies in
+// exactly one of them.
+__global__ void __launch_bounds__(256) k_cull_items(TravParams P, uint4* obj_ent, uint32_t* obj_cum,
+                                                    const unsigned long long* pmask, uint4* desc) {
+    const uint32_t lane = lane_id();
+    const uint32_t top = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (top >= P.level_count[1]) return;
+    const uint32_t words = (P.K + 63) / 64;
+    const unsigned long long* pm = pmask + (size_t)top * words;
+    const uint32_t base = P.obj_off[top];
+    const uint32_t f = P.item_first[top], k = P.item_first[top + 1] - f, ncand = P.cand[top];
+    const uint32_t chunk = P.chunk;
+    const uint32_t single = k == 1 ? 0x80000000u : 0u;
+    uint32_t n = 0, cum = 0;
+    for (uint32_t ob = 0; ob < P.K; ob += 64) {
+        const uint64_t m = pm[ob / 64];
+        if (m == 0ull) continue;
+        const uint32_t oi = ob + lane;
+        const bool po = (m >> lane) & 1ull;
+        uint4 r = make_uint4(0u, 0u, 0u, 0u);
+        uint32_t ncl = 0;
+        if (po) {
+            r = P.obj_range[oi];
+            ncl = (r.y - r.x + CLUSTER) / CLUSTER;
+        }
+        const uint32_t incl = wave_incl_add(ncl);
+        if (po) {
+            const uint32_t e = base + n + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            const uint32_t c0 = cum + incl - ncl;
+            obj_ent[e] = make_uint4(r.x, r.y, P.obj_cl_base[oi], r.z);
+            obj_cum[e] = c0;
+            const uint32_t i1 = (c0 + ncl + chunk - 1u) / chunk;
+            for (uint32_t i = (c0 + chunk - 1u) / chunk; i < i1; i++) {
+                const uint32_t s0 = i * chunk;
+                const uint32_t rem = ncand - s0 < chunk ? ncand - s0 : chunk;
+                desc[f + i] = make_uint4(top, e, s0 - c0, rem | single);
+            }
+        }
+        n += (uint32_t)__popcll(m);
+        cum += __shfl(incl, 63, 64);
     }
 }
 
@@ -1139,16 +1194,22 @@ void launch_cull_list(const TravParams& p, uint4* obj_ent, uint32_t* obj_cum, un
 }
 
 void launch_item_table(const uint32_t* cand, const float2* top_aux, uint32_t ntop, uint32_t max_cand,
-                       const uint32_t* list_total, unsigned long long* table, hipStream_t s) {
+                       const uint32_t* npass, unsigned long long* table, hipStream_t s) {
     static_assert(ITEM_TABLE == 64, "one lane per chunk size");
     const bool fast = div_hi_ok((uint64_t)max_cand + 2 * ITEM_TABLE - 1, 2 * ITEM_TABLE);
     if (ntop)
         k_item_table<<<(ntop + TABLE_SLICE - 1) / TABLE_SLICE, 64, 0, s>>>(cand, top_aux, ntop, table, fast,
-                                                                           list_total);
+                                                                           npass);
 }
 
 void launch_item_count(const uint32_t* cand, uint32_t chunk, uint32_t ntop, uint32_t* nitems, hipStream_t s) {
     k_item_count<<<ntop / 256 + 1, 256, 0, s>>>(cand, chunk, ntop, nitems);
+}
+
+void launch_cull_items(const TravParams& p, uint4* obj_ent, uint32_t* obj_cum, const unsigned long long* pmask,
+                       uint4* desc, hipStream_t s) {
+    const uint32_t top = p.level_count[1];
+    if (top) k_cull_items<<<(top + 3) / 4, 256, 0, s>>>(p, obj_ent, obj_cum, pmask, desc);
 }
 
 void launch_item_desc(const TravParams& p, uint4* desc, hipStream_t s) {

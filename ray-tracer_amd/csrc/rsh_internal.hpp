@@ -95,10 +95,11 @@ hipError_t run_pack(const uint32_t* flags, const uint32_t* keys, const uint32_t*
 // Exclusive scan of the work-item counts ceil(cand[t] / chunk), t < ntop, with
 // first[ntop] = their total (counts is scratch of ntop + 1 entries); max_cand
 // bounds every cand entry (the scene's cluster count).  zero[0, zero_n) is
-// cleared on the way (the item table, for the next pass).
+// cleared on the way (the item table, for the next pass); with npass, its
+// exclusive scan (ntop + 1 entries) goes to off in the same launch.
 hipError_t run_item_scan(const uint32_t* cand, uint32_t ntop, uint32_t chunk, uint32_t max_cand, uint32_t* counts,
                          uint32_t* first, ScanScratch& sc, hipStream_t s, unsigned long long* zero = nullptr,
-                         uint32_t zero_n = 0);
+                         uint32_t zero_n = 0, const uint32_t* npass = nullptr, uint32_t* off = nullptr);
 // Decompression of run_pack's chunks in sorted order: out_values receives
 // tvalues[bases[v] .. bases[v + 1]) for v = sorted_cvalues[0 .. C) in turn
 // (N values in all).  soff (C + 1 entries) and sbase (C) are scratch.

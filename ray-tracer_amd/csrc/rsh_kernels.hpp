@@ -165,11 +165,16 @@ void launch_cull_list(const TravParams& p, uint4* obj_ent, uint32_t* obj_cum, un
 constexpr uint32_t ITEM_TABLE = 64;
 constexpr float WIDE_TAN = 0.1f;
 // max_cand: an upper bound of every cand entry (the scene's cluster count);
-// table: 2 ITEM_TABLE + 2 entries, zeroed beforehand; list_total may be null
+// table: 2 ITEM_TABLE + 2 entries, zeroed beforehand; npass (summed into the
+// last entry) may be null
 void launch_item_table(const uint32_t* cand, const float2* top_aux, uint32_t ntop, uint32_t max_cand,
-                       const uint32_t* list_total, unsigned long long* table, hipStream_t s);
+                       const uint32_t* npass, unsigned long long* table, hipStream_t s);
 constexpr uint32_t ITEM_TABLE_WORDS = 2 * ITEM_TABLE + 2;   // 64-bit entries
 void launch_item_count(const uint32_t* cand, uint32_t chunk, uint32_t ntop, uint32_t* nitems, hipStream_t s);
+// CRSH: the list pass and the work-item descriptors in one launch (needs
+// obj_off, item_first and chunk in p)
+void launch_cull_items(const TravParams& p, uint4* obj_ent, uint32_t* obj_cum, const unsigned long long* pmask,
+                       uint4* desc, hipStream_t s);
 void launch_item_desc(const TravParams& p, uint4* desc, hipStream_t s);
 
 // --- rays ---

@@ -88,7 +88,7 @@ struct rsh_engine {
     // consecutive reflection passes (a pass's shading reads them while the
     // next pass traces)
     uint32_t W = 0, H = 0, Lmax = 1;
-    size_t cap_slots = 0;
+    size_t cap_slots = 0, cap_pixels = 0;   // slots (pixels x lights) and pixels the buffers hold
     DevBuf rays_a, rays_b, closest[2], shadow, pixels;
     DevBuf views_d;                    // view table of a batch (more than one view)
     uint64_t scene_epoch = 0;          // counts scene loads
@@ -246,15 +246,22 @@ rsh_status ensure_frame(rsh_engine* e, uint32_t W, uint32_t H, uint32_t Lmax) {
     if (rs != RSH_OK) return rs;
     for (Lane& L : e->lane)
         if ((rs = ensure_lane(e, L, S)) != RSH_OK) return rs;
-    if (S <= e->cap_slots) return RSH_OK;
-    HIP_OK(e->rays_a.reserve(S * 24)); HIP_OK(e->rays_b.reserve(S * 24));
-    HIP_OK(e->shadow.reserve(S * 4));
-    HIP_OK(e->closest[0].reserve(S * 8)); HIP_OK(e->closest[1].reserve(S * 8));
-    HIP_OK(e->pixels.reserve(P * 4));
-    for (int i = 0; i < 4; i++) {
-        HIP_OK(e->g0[i].reserve(P * 16)); HIP_OK(e->gb[0][i].reserve(P * 16)); HIP_OK(e->gb[1][i].reserve(P * 16));
+    // per-slot and per-pixel buffers grow separately: a scene with more lights
+    // can leave enough slots for a later frame with more pixels
+    if (S > e->cap_slots) {
+        HIP_OK(e->rays_a.reserve(S * 24)); HIP_OK(e->rays_b.reserve(S * 24));
+        HIP_OK(e->shadow.reserve(S * 4));
+        HIP_OK(e->closest[0].reserve(S * 8)); HIP_OK(e->closest[1].reserve(S * 8));
+        e->cap_slots = S;
     }
-    e->cap_slots = S;
+    if (P > e->cap_pixels) {
+        HIP_OK(e->pixels.reserve(P * 4));
+        for (int i = 0; i < 4; i++) {
+            HIP_OK(e->g0[i].reserve(P * 16)); HIP_OK(e->gb[0][i].reserve(P * 16));
+            HIP_OK(e->gb[1][i].reserve(P * 16));
+        }
+        e->cap_pixels = P;
+    }
     return RSH_OK;
 }
 

@@ -248,3 +248,24 @@ def test_minimal_object_spheres_match_oracle(engine, oracle, rsh):
     tight = dataclasses.replace(scene, object_spheres=rsh.object_spheres(scene.tri_pos, rg))
     cam = rsh.scenes.config1_camera(128, 128)
     run_case(engine, oracle, rsh, tight, cam, rsh.config.config1(128, 128))
+
+
+def test_pixel_buffers_grow_after_a_two_light_scene(rsh):
+    # A two-light scene leaves slot capacity (pixels x lights) for a later
+    # frame with more pixels: the per-pixel buffers must still grow.
+    e = rsh.Engine(0)
+    office = rsh.scenes.config1_scene(boxes=50)
+    assert office.light_total == 2
+    e.load_scene(office)
+    e.render(rsh.scenes.config1_camera(128, 128), rsh.config.config1(128, 128))
+    scene, cam = rsh.scenes.config0_scene(), rsh.scenes.config0_camera(181, 181)   # 32761 pixels <= 32768 slots
+    cfg = rsh.config.config0(181, 181)
+    e.load_scene(scene)
+    e.render(cam, cfg)
+    got = e.pixels().copy()
+    e.close()
+    fresh = rsh.Engine(0)
+    fresh.load_scene(scene)
+    fresh.render(cam, cfg)
+    np.testing.assert_array_equal(got, fresh.pixels())
+    fresh.close()

@@ -62,6 +62,7 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-views leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-update", action="store_true", help="skip the object-update leg")
+    ap.add_argument("--no-rah", action="store_true", help="skip the RAH / CRSH A/B leg")
     ap.add_argument("--serve-seconds", type=float, default=6.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="copy MIB MiB device-to-device 3 times with a plain copy kernel and exit")
@@ -225,6 +226,43 @@ def update_leg(rsh_amd, eng, scene, cam, cfg, reps=5):
             "note": "update_ms is the host call (three 16-float matrices per object uploaded, kernels, sync)"}
 
 
+def rah_leg(rsh_amd, eng, scene2, cam2, cfg2, reps=3):
+    """The reference's two algorithms on the same frames: CRSH (sorted rays,
+    object culling) against RAH (rays in slot order, trimmed only, every
+    object a candidate; main.cpp:1016-1037), on the office-scale 1024^2 frame
+    and the atrium frame.  Reports ms, Mrays/s and the share of identical
+    pixels: the two group rays into different nodes, and the reference's node
+    tests are not exactly conservative, so a few pixels can differ (each
+    algorithm matches the oracle's version of itself)."""
+    import dataclasses
+    from rsh_amd import config as cf, scenes
+    out = {}
+    office = scenes.config1_scene()
+    for name, scene, cam, cfg in (("office 1024x1024", office, scenes.config1_camera(1024, 1024),
+                                   cf.config1(1024, 1024).with_scene_bounds(office)),
+                                  ("atrium 1920x1080", scene2, cam2, cfg2)):
+        eng.load_scene(scene)
+        row, px = {}, None
+        for alg, tag in ((0, "crsh"), (1, "rah")):
+            c = dataclasses.replace(cfg, algorithm=alg)
+            eng.render(cam, c)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                st = eng.render(cam, c)
+            ms = 1e3 * (time.perf_counter() - t0) / reps
+            row[tag + "_ms"] = ms
+            row[tag + "_mrays"] = st.rays_total / ms / 1e3
+            p = eng.pixels()
+            if px is None:
+                px = p.copy()
+            else:
+                row["identical_pixels"] = float((p == px).mean())
+        row["crsh_speedup"] = row["rah_ms"] / row["crsh_ms"]
+        out[name] = row
+    eng.load_scene(scene2)
+    return out
+
+
 def batching_leg(rsh_amd, eng, scene2):
     """Plain per-view frames against the batched form on the 64-view batch."""
     import dataclasses
@@ -338,6 +376,9 @@ def main():
     dynamic = None if a.no_update else update_leg(rsh_amd, eng, scene, cam, cfg)
     if dynamic:
         progress("object-update leg done")
+    rah = None if a.no_rah else rah_leg(rsh_amd, eng, scene, cam, cfg)
+    if rah:
+        progress("RAH A/B leg done")
     if not a.no_batch:
         batching = batching_leg(rsh_amd, eng, scene)
         progress("batching leg done")
@@ -367,6 +408,8 @@ def main():
     eng.close()
     if dynamic:
         out["object_update"] = dynamic
+    if rah:
+        out["rah_vs_crsh"] = rah
     if not a.no_batch:
         out["batching"] = batching
     if not a.no_serve:

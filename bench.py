@@ -63,6 +63,8 @@ def parse():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-update", action="store_true", help="skip the object-update leg")
     ap.add_argument("--no-rah", action="store_true", help="skip the RAH / CRSH A/B leg")
+    ap.add_argument("--office", default="", metavar="OBJ",
+                    help="also time the reference's office scene (its models/office.obj) through its camera")
     ap.add_argument("--serve-seconds", type=float, default=6.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="copy MIB MiB device-to-device 3 times with a plain copy kernel and exit")
@@ -226,6 +228,42 @@ def update_leg(rsh_amd, eng, scene, cam, cfg, reps=5):
             "note": "update_ms is the host call (three 16-float matrices per object uploaded, kernels, sync)"}
 
 
+def office_leg(rsh_amd, eng, path, reps=10):
+    """The reference's thesis office scene read from its OBJ / MTL files
+    (rsh_amd.objfile) through the reference's camera and light, 1024x1024,
+    2 mirror bounces with shadow rays at every hit and reflections from every
+    covered pixel (the reference's rule): CRSH and RAH frames."""
+    import dataclasses
+    from rsh_amd import config as cf, objfile
+    sc = objfile.office_scene(path)
+    cam = objfile.office_camera(1024, 1024)
+    base = dataclasses.replace(cf.config1(1024, 1024), reflect_gate=0).with_scene_bounds(sc)
+    eng.load_scene(sc)
+    out = {"triangles": sc.triangle_total, "objects": sc.object_total}
+    for alg, tag in ((0, "crsh"), (1, "rah")):
+        c = dataclasses.replace(base, algorithm=alg)
+        eng.render(cam, c)
+        n = reps if alg == 0 else 2
+        t0 = time.perf_counter()
+        for _ in range(n):
+            st = eng.render(cam, c)
+        ms = 1e3 * (time.perf_counter() - t0) / n
+        out[tag + "_ms"], out[tag + "_mrays"], out[tag + "_rays"] = ms, st.rays_total / ms / 1e3, st.rays_total
+    # the thesis's office frame: 512x512, shadow rays only (its primary
+    # visibility is rasterised; here the primary pass is traced as well)
+    c = dataclasses.replace(cf.config1(512, 512), bounces=0).with_scene_bounds(sc)
+    cam5 = objfile.office_camera(512, 512)
+    eng.render(cam5, c)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        st = eng.render(cam5, c)
+    ms = 1e3 * (time.perf_counter() - t0) / reps
+    shadow = sum(st.pass_rays[i] for i in range(st.pass_count) if st.pass_kind[i] == 1)
+    out["thesis_512_shadow_frame"] = {"ms": ms, "shadow_rays": int(shadow), "rays": int(st.rays_total),
+                                      "mrays": st.rays_total / ms / 1e3}
+    return out
+
+
 def rah_leg(rsh_amd, eng, scene2, cam2, cfg2, reps=3):
     """The reference's two algorithms on the same frames: CRSH (sorted rays,
     object culling) against RAH (rays in slot order, trimmed only, every
@@ -379,6 +417,8 @@ def main():
     rah = None if a.no_rah else rah_leg(rsh_amd, eng, scene, cam, cfg)
     if rah:
         progress("RAH A/B leg done")
+    office = office_leg(rsh_amd, eng, a.office) if a.office else None
+    eng.load_scene(scene)
     if not a.no_batch:
         batching = batching_leg(rsh_amd, eng, scene)
         progress("batching leg done")
@@ -410,6 +450,8 @@ def main():
         out["object_update"] = dynamic
     if rah:
         out["rah_vs_crsh"] = rah
+    if office:
+        out["office_obj"] = office
     if not a.no_batch:
         out["batching"] = batching
     if not a.no_serve:

@@ -269,3 +269,37 @@ def test_pixel_buffers_grow_after_a_two_light_scene(rsh):
     fresh.render(cam, cfg)
     np.testing.assert_array_equal(got, fresh.pixels())
     fresh.close()
+
+
+def test_obj_scene_matches_oracle(engine, oracle, rsh, tmp_path):
+    # the 512 scene written as an OBJ (per-vertex normals, one material per
+    # object) and read back through rsh_amd.objfile
+    from rsh_amd import objfile
+    scene, cam = c0(rsh, 96)
+    v = scene.tri_pos.reshape(-1, 3, 4)[:, :, :3]
+    n = scene.tri_nrm.reshape(-1, 3, 4)[:, :, :3]
+    lines, mtl = ["mtllib s.mtl"], []
+    nv = 0
+    for k, (a, b) in enumerate(scene.object_spheres[:, 4:6].astype(int)):
+        lines.append(f"o obj{k}")
+        lines.append(f"usemtl m{k}")
+        kd = scene.mat_diffuse[scene.tri_mat[3 * a]]
+        ks = scene.mat_specular[scene.tri_mat[3 * a]]
+        mtl += [f"newmtl m{k}", "Kd %r %r %r" % tuple(map(float, kd[:3])), "Ks %r %r %r" % tuple(map(float, ks[:3])),
+                "Ns %r" % float(ks[3])]
+        for t in range(a, b + 1):
+            for j in range(3):
+                lines.append("v %r %r %r" % tuple(map(float, v[t, j])))
+                lines.append("vn %r %r %r" % tuple(map(float, n[t, j])))
+            lines.append("f %d//%d %d//%d %d//%d" % (nv + 1, nv + 1, nv + 2, nv + 2, nv + 3, nv + 3))
+            nv += 3
+    (tmp_path / "s.obj").write_text("\n".join(lines) + "\n")
+    (tmp_path / "s.mtl").write_text("\n".join(mtl) + "\n")
+    lights = [(tuple(map(float, scene.light_pos[i, :3])), tuple(map(float, scene.light_col[i, :3])),
+               tuple(map(float, scene.light_int[i]))) for i in range(scene.light_total)]
+    s2 = objfile.load_obj(str(tmp_path / "s.obj"), lights=lights,
+                          reflective=lambda name, kd, ks, ns: scene.mat_diffuse[scene.tri_mat[
+                              3 * int(scene.object_spheres[int(name[1:]), 4])], 3] > 0)
+    assert s2.triangle_total == scene.triangle_total and s2.object_total == scene.object_total
+    assert np.array_equal(s2.tri_pos[:, :3], scene.tri_pos[:, :3])
+    run_case(engine, oracle, rsh, s2, cam, rsh.config.config0(96, 96))

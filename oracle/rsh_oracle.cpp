@@ -603,6 +603,20 @@ void create_bounce(const oracle_config& cfg, const GBuf& g, const Batch& prev,
     }
 }
 
+// Primary rays are keyed by screen position, not by the hash (the reference
+// rasterises primary visibility, so this stage has no reference order): 8x8
+// pixel tiles in row-major order over the view, Morton order inside a tile
+// (x bit 0, y bit 0, x bit 1, ...), the view id above segment_shift.  Sorted,
+// a leaf of 8 rays is a 4x2 pixel block and a 64-ray node one tile.
+uint32_t primary_key(uint32_t x, uint32_t y, uint32_t w, uint32_t view, uint32_t segment_shift) {
+    const uint32_t lx = x & 7u, ly = y & 7u;
+    const uint32_t m = (lx & 1u) | ((ly & 1u) << 1) | ((lx & 2u) << 1) | ((ly & 2u) << 2) | ((lx & 4u) << 2) |
+                       ((ly & 4u) << 3);
+    uint32_t k = (((y >> 3) * ((w + 7u) >> 3) + (x >> 3)) << 6) | m;
+    if (segment_shift) k |= view << segment_shift;
+    return k;
+}
+
 // Primary rays (configuration stage; formula of RT:667), per view: pixel p of
 // view v is (x, y) within it, dir = D + R*(x/w - 0.5) + U*(y/h - 0.5).
 void create_primary(const oracle_config& cfg, const ViewSet& vs, Batch& b) {
@@ -619,7 +633,7 @@ void create_primary(const oracle_config& cfg, const ViewSet& vs, Batch& b) {
         f3 d = normalize((vs.D[v] + vs.R[v] * a) + vs.U[v] * c);
         b.rays[2 * pp] = eye;
         b.rays[2 * pp + 1] = d;
-        b.keys[pp] = cfg.key_mode == 0 ? reflection_key_ref(eye, d) : view_key(cfg, eye, d, 0, v);
+        b.keys[pp] = primary_key(x, y, vs.w[v], v, cfg.segment_shift);
         b.values[pp] = (uint32_t)pp;
         // Optional row band (a bounded sample of the frame for timing).
         bool in_band = cfg.row_end == 0 || (y >= cfg.row_begin && y < cfg.row_end);

@@ -432,7 +432,7 @@ rsh_status build_items(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, Trav
 // closest; kind 1: shadow flags into e->shadow.
 rsh_status trace_pass(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, const float* rays, uint32_t S,
                       uint32_t kind, const float4* shadow_pos, uint32_t flags_mask, PassOut& out, uint32_t pass_idx,
-                      DevBuf& closest, const float* tmax = nullptr, bool camera = false) {
+                      DevBuf& closest, const float* tmax = nullptr, bool camera = false, uint32_t sort_bits = 0) {
     out = PassOut{};
     hipStream_t s = L.stream;
     const bool timing = flags_mask & 4u, counting = flags_mask & 2u;
@@ -466,7 +466,7 @@ rsh_status trace_pass(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, const
         out.C = C;
         {
             StageTimer t(L, ST_SORT, timing);
-            LANE_OK(run_radix_sort(u32p(L.ckeys), u32p(L.cvalues), C, key_bits(cfg), u32p(L.skeys),
+            LANE_OK(run_radix_sort(u32p(L.ckeys), u32p(L.cvalues), C, sort_bits ? sort_bits : key_bits(cfg), u32p(L.skeys),
                                   u32p(L.svalues), L.sc, s));
         }
         {
@@ -758,6 +758,18 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
             return RSH_INVALID_ARG;
         }
     }
+    // primary keys: 8x8 tiles of a view (Morton order inside), the view id at
+    // segment_shift; their sort needs only the bits they use
+    uint64_t tile_keys = 1;
+    for (uint32_t i = 0; i < nv; i++)
+        tile_keys = std::max<uint64_t>(tile_keys, (uint64_t)((hv[i].w + 7) / 8) * ((hv[i].h + 7) / 8) * 64);
+    if (cfg.segment_shift && tile_keys > (1ull << cfg.segment_shift)) {
+        e->err = "render: a view's primary tile keys do not fit below segment_shift";
+        return RSH_INVALID_ARG;
+    }
+    auto bit_length = [](uint64_t v) { uint32_t b = 0; while (v) { b++; v >>= 1; } return b; };
+    uint32_t primary_bits = std::max<uint32_t>(1u, bit_length(tile_keys - 1));
+    if (cfg.segment_shift && nv > 1) primary_bits = std::min<uint32_t>(32u, cfg.segment_shift + bit_length(nv - 1));
     const uint64_t S64 = (uint64_t)width * height * e->L;
     if (S64 > (uint64_t)MAX_TILES * TILE || S64 > 0xFFFFFFFFull) { e->err = "render: frame exceeds slot capacity"; return RSH_CAPACITY; }
     HIP_OK(hipSetDevice(e->device));
@@ -918,7 +930,8 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
             launch_create_primary(V, P, kc, rays_b, u32p(A.flags), u32p(A.keys), u32p(A.values), sa);
         }
         PassOut po;
-        rsh_status r = trace_pass(e, A, cfg, rays_b, P, 0, nullptr, fl, po, 0, e->closest[0], nullptr, true);
+        rsh_status r = trace_pass(e, A, cfg, rays_b, P, 0, nullptr, fl, po, 0, e->closest[0], nullptr, true,
+                                  primary_bits);
         if (r != RSH_OK) return r;
         if ((r = record(e, A, st, 0, 0, P, po, fl, e->closest[0])) != RSH_OK) return r;
         {

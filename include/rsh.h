@@ -101,6 +101,11 @@ typedef struct rsh_frame_config {
                                        are the same in every mode; with mode 3 and stats
                                        counters (flag bit 1) the cluster masks are off so
                                        that the counts stay the reference's.             */
+    uint32_t soft_shadows;          /* 1: soft shadows (RayTracer.cu:1550-1603, 1680-1697,
+                                       compiled out by Constants.h:32): 16 rays per shadow
+                                       slot from a 4x4 grid about the light, each an exact
+                                       any-hit query; shading scales both light terms by
+                                       log2(1 + 2 s), s the grid weights of the open cells */
 } rsh_frame_config;
 
 #define RSH_MAX_PASSES 16

@@ -63,6 +63,7 @@ typedef struct oracle_config {
     uint32_t segment_shift;         // segment (view/request) id << shift
     float initial_radius, initial_spread;
     uint32_t row_begin, row_end;    // primary rays only for rows [begin, end); 0,0 = all
+    uint32_t soft_shadows;          // 1: 16 shadow rays per slot from a 4x4 grid about the light
 } oracle_config;
 
 // Per-frame outputs.  Every pointer may be NULL.
@@ -111,6 +112,10 @@ uint32_t oracle_shadow_key_ref(const float* o, const float* d, uint32_t light);
 uint32_t oracle_reflection_key_ref(const float* o, const float* d);
 uint32_t oracle_key_oct(const oracle_config* c, const float* o, const float* d, uint32_t type);
 float oracle_ray_triangle(const float* o, const float* d, const float* v0, const float* e1, const float* e2);
+// Soft shadows: the 16 grid points (float3 each) about `light` of a shadow
+// ray with direction d, and the light scale of an occluded-cell mask.
+void oracle_soft_grid(const float* light, const float* d, float* out48);
+float oracle_soft_factor(uint32_t occluded);
 int oracle_sphere_node(const float* sphere, const float* cone, const float* tri);
 void oracle_triangle_sphere(const float* v0, const float* v1, const float* v2, float* out4);
 void oracle_cone0(const float* cone, const float* ray, uint32_t mode, float* out4);

@@ -40,7 +40,7 @@ class OConfig(C.Structure):
                 ("bounce_shadows", u32), ("primary_rays", u32), ("cell_bits", u32), ("dir_bits", u32),
                 ("key_layout", u32), ("scene_min", f32 * 3), ("cell_scale", f32 * 3),
                 ("segment_shift", u32), ("initial_radius", f32), ("initial_spread", f32),
-                ("row_begin", u32), ("row_end", u32)]
+                ("row_begin", u32), ("row_end", u32), ("soft_shadows", u32)]
 
 
 class OFrameOut(C.Structure):
@@ -108,6 +108,9 @@ def lib():
         L.oracle_key_oct.argtypes = [C.POINTER(OConfig), PF, PF, u32]
         L.oracle_ray_triangle.restype = f32
         L.oracle_ray_triangle.argtypes = [PF] * 5
+        L.oracle_soft_grid.argtypes = [PF, PF, PF]
+        L.oracle_soft_factor.restype = f32
+        L.oracle_soft_factor.argtypes = [u32]
         L.oracle_sphere_node.restype = C.c_int
         L.oracle_sphere_node.argtypes = [PF] * 3
         L.oracle_triangle_sphere.argtypes = [PF] * 4
@@ -214,7 +217,7 @@ class Frame:
 def render(scene, camera, cfg, gbuffer=None, threads=0, keep_hits=True, keep_gbuffer=True, row_range=None) -> Frame:
     L = lib()
     W, H = cfg.width, cfg.height
-    S = W * H * max(1, scene.light_pos.shape[0])
+    S = W * H * max(1, scene.light_pos.shape[0]) * (16 if getattr(cfg, "soft_shadows", 0) else 1)
     pixels = np.zeros(W * H, np.uint32)
     out = OFrameOut()
     out.pixels = up(pixels)
@@ -258,7 +261,7 @@ def render_views(scene, views, cfg, threads=0, keep_hits=True, keep_gbuffer=True
     n = len(views)
     P = sum(int(w) * int(h) for _, w, h in views)
     W, H = (int(views[0][1]), int(views[0][2])) if n == 1 else (P, 1)
-    S = P * max(1, scene.light_pos.shape[0])
+    S = P * max(1, scene.light_pos.shape[0]) * (16 if getattr(cfg, "soft_shadows", 0) else 1)
     pixels = np.zeros(P, np.uint32)
     out = OFrameOut()
     out.pixels = up(pixels)

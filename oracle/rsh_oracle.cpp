@@ -487,7 +487,8 @@ void trace(const Scene& sc, const oracle_config& cfg, const Batch& b, uint32_t k
                     if (tmax) {
                         lim = tmax[slot];
                     } else {
-                        uint32_t x = slot % W, y = slot / (W * Lmax);
+                        const uint32_t i = cfg.soft_shadows ? slot >> 4 : slot;
+                        uint32_t x = i % W, y = i / (W * Lmax);
                         f3 P = xyz(shadow_g->pos[(size_t)x + (size_t)y * W]);
                         lim = length(o - P) - EPS * 2.0f;
                     }
@@ -528,29 +529,71 @@ uint32_t view_key(const oracle_config& cfg, f3 o, f3 d, uint32_t type, uint32_t 
 }
 
 // ---------------------------------------------------- ray creation --------
-// RT:682-740 CreateShadowRays over slots i = x + W*(l + L*y).
+// Soft shadows (RT:1550-1603, compiled out by Constants.h:32): grid point c
+// (i = c % 4, j = c / 4) about the light at offsets (a - 2, b - 2) / 8 of two
+// unnormalised plane axes, a = i + (i >= 2), b = j + (j >= 2).  The first
+// axis crosses the coordinate axis picked by RT:1556-1565's test against
+// min(|x|, max(|y|, |z|)) with the shadow direction d; the second is d x it.
+void soft_axes(f3 d, f3& a0, f3& a1) {
+    const float ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);
+    const float m = fmin_(ax, fmax_(ay, az));
+    if (ax == m) a0 = cross(f3{1.0f, 0.0f, 0.0f}, d);
+    else if (ay == m) a0 = cross(f3{0.0f, 1.0f, 0.0f}, d);
+    else a0 = cross(f3{0.0f, 0.0f, 1.0f}, d);
+    a1 = cross(d, a0);
+}
+
+f3 soft_point(f3 o, f3 a0, f3 a1, uint32_t c) {
+    const int i = (int)(c & 3u), j = (int)(c >> 2);
+    const int a = i + (i >= 2 ? 1 : 0), bb = j + (j >= 2 ? 1 : 0);
+    return (o + (a0 * (float)(a - 2)) * 0.125f) + (a1 * (float)(bb - 2)) * 0.125f;
+}
+
+// RT:1680-1692: log2(1 + 2 s), s summing, over the open cells in cell order,
+// 0.55/16 (cells 5, 6, 9, 10), 0.5375/16 (cells 0, 3, 12, 15) or 0.525/16.
+float soft_factor(uint32_t occluded) {
+    float s = 0.0f;
+    for (uint32_t c = 0; c < 16; c++) {
+        if ((occluded >> c) & 1u) continue;
+        if (c == 5 || c == 6 || c == 9 || c == 10) s += 0.55f / 16.0f;
+        else if (c == 0 || c == 3 || c == 12 || c == 15) s += 0.5375f / 16.0f;
+        else s += 0.525f / 16.0f;
+    }
+    return log2_f(1.0f + s * 2.0f);
+}
+
+// RT:682-740 CreateShadowRays over slots i = x + W*(l + L*y).  With soft
+// shadows every live slot i holds 16 rays, 16 i + c from grid point c towards
+// the slot's surface point (RT:1568-1590's interpolated rays).
 void create_shadow(const Scene& sc, const oracle_config& cfg, const GBuf& g, uint32_t Lmax, const ViewSet& vs,
                    Batch& b) {
     const uint32_t W = g.W, H = g.H;
-    b.init(W * H * Lmax);
+    const uint32_t sh = cfg.soft_shadows ? 4u : 0u;
+    b.init((W * H * Lmax) << sh);
 #pragma omp parallel for schedule(static)
-    for (int64_t ii = 0; ii < (int64_t)b.S; ii++) {
-        uint32_t i = (uint32_t)ii;
+    for (int64_t rr = 0; rr < (int64_t)b.S; rr++) {
+        const uint32_t r = (uint32_t)rr, i = r >> sh;
         uint32_t x = i % W, y = i / (W * Lmax), l = (i / W) % Lmax;
         f3 P = xyz(g.pos[(size_t)x + (size_t)y * W]);
         f3 N = normalize(xyz(g.nrm[(size_t)x + (size_t)y * W]));
-        b.flags[i] = 1;
+        b.flags[r] = 1;
         if (length(P) != 0.0f) {
             f3 o = ld3(sc.s->light_pos + 4 * l);
             f3 d = normalize(P - o);
             float diffuse = fmax_(dot(d, -N), 0.0f);
             if (diffuse > 0.0f) {
-                b.rays[2 * (size_t)i] = o;
-                b.rays[2 * (size_t)i + 1] = d;
-                b.keys[i] = cfg.key_mode == 0 ? shadow_key_ref(o, d, l)
+                if (sh) {
+                    f3 a0, a1;
+                    soft_axes(d, a0, a1);
+                    o = soft_point(o, a0, a1, r & 15u);
+                    d = normalize(P - o);
+                }
+                b.rays[2 * (size_t)r] = o;
+                b.rays[2 * (size_t)r + 1] = d;
+                b.keys[r] = cfg.key_mode == 0 ? shadow_key_ref(o, d, l)
                                               : view_key(cfg, o, d, 1, vs.view_of(x + y * W));
-                b.values[i] = i;
-                b.flags[i] = 0;
+                b.values[r] = r;
+                b.flags[r] = 0;
             }
         }
     }
@@ -675,8 +718,10 @@ void surface_from_hits(const Scene& sc, const oracle_config& cfg, const Batch& b
     }
 }
 
-// Blinn-Phong term of one light (RT:1655-1702 and RT:1847-1876).
-inline f3 light_term(const oracle_scene* s, uint32_t l, f3 P, f3 N, f3 eye, f4 Kd, f4 Ks, float scale, bool scaled) {
+// Blinn-Phong term of one light (RT:1655-1702 and RT:1847-1876); soft: both
+// components scaled last by `factor` (RT:1693-1696).
+inline f3 light_term(const oracle_scene* s, uint32_t l, f3 P, f3 N, f3 eye, f4 Kd, f4 Ks, float scale, bool scaled,
+                     bool soft = false, float factor = 1.0f) {
     f3 color{0, 0, 0};
     f3 Ld = ld3(s->light_pos + 4 * l) - P;
     float dist = length(Ld);
@@ -688,20 +733,35 @@ inline f3 light_term(const oracle_scene* s, uint32_t l, f3 P, f3 N, f3 eye, f4 K
     if (diffuse > 0.0f) {
         f3 Hv = normalize(Ld - normalize(P - eye));
         float specular = clampf(pow_f(fmax_(dot(Hv, N), 0.0f), Ks.w), 0.0f, 1.0f);
+        f3 cd, cs;
         if (scaled) {
-            color = color + ((((xyz(Kd) * lc) * diffuse) * Ix) * att) * scale;
-            color = color + ((((xyz(Ks) * lc) * specular) * Iy) * att) * scale;
+            cd = ((((xyz(Kd) * lc) * diffuse) * Ix) * att) * scale;
+            cs = ((((xyz(Ks) * lc) * specular) * Iy) * att) * scale;
         } else {
-            color = color + (((xyz(Kd) * lc) * diffuse) * Ix) * att;
-            color = color + (((xyz(Ks) * lc) * specular) * Iy) * att;
+            cd = (((xyz(Kd) * lc) * diffuse) * Ix) * att;
+            cs = (((xyz(Ks) * lc) * specular) * Iy) * att;
         }
+        if (soft) {
+            cd = cd * factor;
+            cs = cs * factor;
+        }
+        color = color + cd;
+        color = color + cs;
     }
     return color;
 }
 
 // RT:1616-1709 ColorPrimaryShadowRay
+// Soft shadows: the 16 flags of slot i as a mask of occluded cells.
+uint32_t soft_mask(const std::vector<uint64_t>& sflags, size_t i) {
+    uint32_t m = 0;
+    for (uint32_t c = 0; c < 16; c++)
+        if (sflags[16 * i + c] == MISS32) m |= 1u << c;
+    return m;
+}
+
 void shade_primary(const Scene& sc, const GBuf& g, const std::vector<uint64_t>& sflags, uint32_t Lmax,
-                   uint32_t Lt, const ViewSet& vs, uint32_t* pixels) {
+                   uint32_t Lt, const ViewSet& vs, bool soft, uint32_t* pixels) {
     const uint32_t W = g.W, H = g.H;
 #pragma omp parallel for schedule(static)
     for (int64_t pp = 0; pp < (int64_t)W * H; pp++) {
@@ -711,9 +771,16 @@ void shade_primary(const Scene& sc, const GBuf& g, const std::vector<uint64_t>& 
         if (length(P) != 0.0f) {
             const f3 eye = vs.eye[vs.view_of((uint32_t)pp)];
             for (uint32_t l = 0; l < Lt; l++) {
-                uint64_t rec = sflags[(size_t)(x + y * W * Lmax) + (size_t)W * l];
-                if (rec != MISS32)
+                const size_t slot = (size_t)(x + y * W * Lmax) + (size_t)W * l;
+                if (soft) {
+                    // RT:1644-1647 / 1680-1697: no light when every cell is occluded
+                    const uint32_t m = soft_mask(sflags, slot);
+                    if (m != 0xFFFFu)
+                        color = color + light_term(sc.s, l, P, N, eye, g.dif[pp], g.spec[pp], 1.0f, false, true,
+                                                   soft_factor(m));
+                } else if (sflags[slot] != MISS32) {
                     color = color + light_term(sc.s, l, P, N, eye, g.dif[pp], g.spec[pp], 1.0f, false);
+                }
             }
         }
         pixels[pp] = rgb_to_int(color.x * 255.0f, color.y * 255.0f, color.z * 255.0f);
@@ -736,7 +803,14 @@ void shade_reflection(const Scene& sc, const oracle_config& cfg, const GBuf& g,
         f4 Kd = g.dif[pp], Ks = g.spec[pp];
         f3 color{0, 0, 0};
         for (uint32_t l = 0; l < Lt; l++) {
-            if (sflags && (*sflags)[(size_t)slot + (size_t)W * l] == MISS32) continue;
+            const size_t sl = (size_t)slot + (size_t)W * l;
+            if (sflags && cfg.soft_shadows) {
+                const uint32_t m = soft_mask(*sflags, sl);
+                if (m != 0xFFFFu)
+                    color = color + light_term(sc.s, l, P, N, eye, Kd, Ks, 0.75f, true, true, soft_factor(m));
+                continue;
+            }
+            if (sflags && (*sflags)[sl] == MISS32) continue;
             color = color + light_term(sc.s, l, P, N, eye, Kd, Ks, 0.75f, true);
         }
         pixels[pp] += rgb_to_int(color.x * Ks.w, color.y * Ks.w, color.z * Ks.w);
@@ -875,6 +949,15 @@ uint32_t oracle_key_oct(const oracle_config* c, const float* o, const float* d, 
 float oracle_ray_triangle(const float* o, const float* d, const float* v0, const float* e1, const float* e2) {
     return ray_triangle(ld3(o), ld3(d), ld3(v0), ld3(e1), ld3(e2));
 }
+void oracle_soft_grid(const float* light, const float* d, float* out48) {
+    f3 a0, a1;
+    soft_axes(ld3(d), a0, a1);
+    for (uint32_t c = 0; c < 16; c++) {
+        const f3 p = soft_point(ld3(light), a0, a1, c);
+        out48[3 * c] = p.x; out48[3 * c + 1] = p.y; out48[3 * c + 2] = p.z;
+    }
+}
+float oracle_soft_factor(uint32_t occluded) { return soft_factor(occluded); }
 int oracle_sphere_node(const float* sphere, const float* cone, const float* tri) {
     f4 c = ld4(cone);
     return sphere_node(ld4(sphere), c, ld4(tri), cos_f(c.w), tan_f(c.w)) ? 1 : 0;
@@ -938,7 +1021,7 @@ int render_views(const oracle_scene* scene, const ViewSet& vs, const oracle_conf
         PassStats st;
         trace(sc, cfg, b, 1, &g0, Lmax, flags, st, threads);
         record_pass(out, 1, b.S, st, flags);
-        if (st.N > 0) shade_primary(sc, g0, flags, Lmax, Lt, vs, pixels.data());
+        if (st.N > 0) shade_primary(sc, g0, flags, Lmax, Lt, vs, cfg.soft_shadows != 0, pixels.data());
     }
     Batch prev;
     GBuf gprev;

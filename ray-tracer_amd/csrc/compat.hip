@@ -182,7 +182,7 @@ void ShadowRayCreationWrapper(const unsigned int W, const unsigned int H, const 
         raise(RSH_INVALID_ARG, "ShadowRayCreationWrapper: lightTotal exceeds the bound light count");
         return;
     }
-    launch_create_shadow(c.g_pos, c.g_nrm, W, H, L, c.light_pos, ref_keys(), single_view(W, H, v3{0, 0, 0}),
+    launch_create_shadow(c.g_pos, c.g_nrm, W, H, L, c.light_pos, ref_keys(), single_view(W, H, v3{0, 0, 0}), 0u,
                          reinterpret_cast<float*>(rays), flags, keys, values, c.stream);
     check(hipGetLastError(), "ShadowRayCreationWrapper");
 }
@@ -347,7 +347,7 @@ void ShadowRayColoringWrapper(const unsigned int W, const unsigned int H, const 
     GBufView g{const_cast<float4*>(c.g_pos), const_cast<float4*>(c.g_nrm), const_cast<float4*>(c.g_dif),
                const_cast<float4*>(c.g_spec)};
     launch_shade_primary(g, W, H, c.light_max, Lights{c.light_pos, c.light_col, c.light_int, L},
-                         single_view(W, H, V(eye)), sflags, pixels, c.stream);
+                         single_view(W, H, V(eye)), sflags, 0u, pixels, c.stream);
     check(hipGetLastError(), "ShadowRayColoringWrapper");
 }
 

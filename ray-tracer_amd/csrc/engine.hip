@@ -89,6 +89,7 @@ struct rsh_engine {
     // next pass traces)
     uint32_t W = 0, H = 0, Lmax = 1;
     size_t cap_slots = 0, cap_pixels = 0;   // slots (pixels x lights) and pixels the buffers hold
+    size_t cap_shadow_slots = 0;            // shadow-pass rays rays_b and shadow hold
     DevBuf rays_a, rays_b, closest[2], shadow, pixels;
     DevBuf views_d;                    // view table of a batch (more than one view)
     uint64_t scene_epoch = 0;          // counts scene loads
@@ -239,20 +240,26 @@ rsh_status ensure_lane(rsh_engine* e, Lane& L, size_t S) {
     return RSH_OK;
 }
 
-rsh_status ensure_frame(rsh_engine* e, uint32_t W, uint32_t H, uint32_t Lmax) {
-    size_t S = (size_t)W * H * Lmax, P = (size_t)W * H;
+// shadow_mult: rays per slot of the shadow passes (16 with soft shadows);
+// they run on lane 1 and write rays_b and the shadow flags.
+rsh_status ensure_frame(rsh_engine* e, uint32_t W, uint32_t H, uint32_t Lmax, uint32_t shadow_mult = 1) {
+    size_t S = (size_t)W * H * Lmax, P = (size_t)W * H, Ss = S * shadow_mult;
     e->W = W; e->H = H; e->Lmax = Lmax;
     rsh_status rs = ensure_lanes(e);
     if (rs != RSH_OK) return rs;
-    for (Lane& L : e->lane)
-        if ((rs = ensure_lane(e, L, S)) != RSH_OK) return rs;
+    if ((rs = ensure_lane(e, e->lane[0], S)) != RSH_OK) return rs;
+    if ((rs = ensure_lane(e, e->lane[1], Ss)) != RSH_OK) return rs;
     // per-slot and per-pixel buffers grow separately: a scene with more lights
     // can leave enough slots for a later frame with more pixels
     if (S > e->cap_slots) {
-        HIP_OK(e->rays_a.reserve(S * 24)); HIP_OK(e->rays_b.reserve(S * 24));
-        HIP_OK(e->shadow.reserve(S * 4));
+        HIP_OK(e->rays_a.reserve(S * 24));
         HIP_OK(e->closest[0].reserve(S * 8)); HIP_OK(e->closest[1].reserve(S * 8));
         e->cap_slots = S;
+    }
+    if (Ss > e->cap_shadow_slots) {
+        HIP_OK(e->rays_b.reserve(Ss * 24));
+        HIP_OK(e->shadow.reserve(Ss * 4));
+        e->cap_shadow_slots = Ss;
     }
     if (P > e->cap_pixels) {
         HIP_OK(e->pixels.reserve(P * 4));
@@ -500,7 +507,7 @@ rsh_status trace_pass(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, const
     {
         StageTimer t(L, ST_HIER, timing);
         launch_gather(rays, sorted_values, N, kind == 1 ? shadow_pos : nullptr, kind == 1 ? tmax : nullptr, e->W,
-                      e->Lmax, f4p(L.rec0),
+                      e->Lmax, kind == 1 && shadow_pos && cfg.soft_shadows ? 4u : 0u, f4p(L.rec0),
                       f4p(L.rec1), s);
         launch_hierarchy(f4p(L.rec0), f4p(L.rec1), nullptr, nullptr, N, cfg.leaf, cfg.branch, cfg.depth,
                          cfg.initial_radius, cfg.initial_spread, cfg.cone_mode, f4p(L.nodes), L.aux.as<float2>(),
@@ -753,7 +760,8 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
     const uint32_t width = nv == 1 ? hv[0].w : (uint32_t)P64, height = nv == 1 ? hv[0].h : 1u;
     if (width == 0 || height == 0 || cfg.leaf == 0 || cfg.branch == 0 || cfg.depth == 0 || cfg.depth > 4 ||
         cfg.key_layout > 2 ||
-        cfg.primary_rays != 1 || cfg.algorithm > 1 || cfg.key_mode > 1 || cfg.hit_mode > 1 || cfg.cone_mode > 1) {
+        cfg.primary_rays != 1 || cfg.algorithm > 1 || cfg.key_mode > 1 || cfg.hit_mode > 1 || cfg.cone_mode > 1 ||
+        cfg.soft_shadows > 1) {
         e->err = "render: invalid configuration";
         return RSH_INVALID_ARG;
     }
@@ -787,13 +795,17 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
     uint32_t primary_bits = std::max<uint32_t>(1u, bit_length(tile_keys - 1));
     if (cfg.segment_shift && nv > 1) primary_bits = std::min<uint32_t>(32u, cfg.segment_shift + bit_length(nv - 1));
     const uint64_t S64 = (uint64_t)width * height * e->L;
-    if (S64 > (uint64_t)MAX_TILES * TILE || S64 > 0xFFFFFFFFull) { e->err = "render: frame exceeds slot capacity"; return RSH_CAPACITY; }
+    const uint32_t smult = cfg.soft_shadows ? 16u : 1u;    // rays per shadow slot
+    if (S64 * smult > (uint64_t)MAX_TILES * TILE || S64 * smult > 0xFFFFFFFFull) {
+        e->err = "render: frame exceeds slot capacity";
+        return RSH_CAPACITY;
+    }
     HIP_OK(hipSetDevice(e->device));
-    rsh_status rs = ensure_frame(e, width, height, e->L);
+    rsh_status rs = ensure_frame(e, width, height, e->L, smult);
     if (rs != RSH_OK) return rs;
     rs = build_ranges(e, cfg.triangle_batch);
     if (rs != RSH_OK) return rs;
-    if ((rs = ensure_nodes(e, (uint32_t)S64, cfg)) != RSH_OK) return rs;
+    if ((rs = ensure_nodes(e, (uint32_t)(S64 * smult), cfg)) != RSH_OK) return rs;
     Views V{};
     if (nv == 1) {
         V.n = 0; V.one = hv[0];
@@ -814,7 +826,7 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
     Lane& A = e->lane[0];
     Lane& B = e->lane[1];
     hipStream_t sa = A.stream, sb = B.stream;
-    const uint32_t W = width, H = height, Lm = e->L, P = W * H, S = W * H * Lm;
+    const uint32_t W = width, H = height, Lm = e->L, P = W * H, S = W * H * Lm, Ssh = S * smult;
     const KeyCfg kc = key_cfg(cfg);
     const SceneView sv = scene_view(e);
     const bool timing = fl & 4u;
@@ -852,13 +864,13 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
         LANE_OK(hipStreamWaitEvent(sb, evG[k], 0));
         {
             StageTimer t(B, ST_CREATE, timing);
-            launch_create_shadow(G.pos, G.nrm, W, H, Lm, f4p(e->light_pos), kc, V, rays_b, u32p(B.flags),
-                                 u32p(B.keys), u32p(B.values), sb);
+            launch_create_shadow(G.pos, G.nrm, W, H, Lm, f4p(e->light_pos), kc, V, cfg.soft_shadows, rays_b,
+                                 u32p(B.flags), u32p(B.keys), u32p(B.values), sb);
         }
         PassOut po;
-        rsh_status r = trace_pass(e, B, cfg, rays_b, S, 1, G.pos, fl, po, idx_shadow(k), e->closest[0]);
+        rsh_status r = trace_pass(e, B, cfg, rays_b, Ssh, 1, G.pos, fl, po, idx_shadow(k), e->closest[0]);
         if (r != RSH_OK) return r;
-        if ((r = record(e, B, st, idx_shadow(k), 1, S, po, fl, e->closest[0])) != RSH_OK) return r;
+        if ((r = record(e, B, st, idx_shadow(k), 1, Ssh, po, fl, e->closest[0])) != RSH_OK) return r;
         LANE_OK(hipEventRecord(evS[k], sb));
         X.sh[k] = po;
         return RSH_OK;
@@ -920,7 +932,8 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
             if (r != RSH_OK) return r;
             if (ps.N > 0) {
                 StageTimer t(A, ST_SHADE, timing);
-                launch_shade_primary(G0, W, H, Lm, sv.lights, V, u32p(e->shadow), u32p(e->pixels), sa);
+                launch_shade_primary(G0, W, H, Lm, sv.lights, V, u32p(e->shadow), cfg.soft_shadows, u32p(e->pixels),
+                                     sa);
             }
             return RSH_OK;
         }
@@ -932,6 +945,7 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
         }
         StageTimer t(A, ST_SHADE, timing);
         launch_shade_reflection(GB[k % 2], W, H, Lm, sv.lights, V, u64p(e->closest[k % 2]), cfg.hit_mode, sflags,
+                                cfg.soft_shadows,
                                 u32p(e->pixels), sa);
         return RSH_OK;
     };

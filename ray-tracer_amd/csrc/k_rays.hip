@@ -101,10 +101,14 @@ __global__ void __launch_bounds__(256) k_query_keys(const float* rays, const uin
 // RayTracer.cu:682-740; slot i = x + W*(l + Lmax*y).
 __global__ void __launch_bounds__(256) k_create_shadow(const float4* gpos, const float4* gnrm, uint32_t W, uint32_t H,
                                                        uint32_t Lmax, const float4* light_pos, KeyCfg kc, Views V,
-                                                       float* rays, uint32_t* flags, uint32_t* keys,
+                                                       uint32_t soft, float* rays, uint32_t* flags, uint32_t* keys,
                                                        uint32_t* values) {
-    const uint32_t n = W * H * Lmax;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    // soft: 16 rays per slot, ray 16 i + c from grid point c about the light
+    // (RayTracer.cu:1550-1603) towards the slot's surface point
+    const uint32_t sh = soft ? 4u : 0u;
+    const uint32_t n = (W * H * Lmax) << sh;
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+        const uint32_t i = r >> sh;
         uint32_t x = i % W, y = i / (W * Lmax), l = (i / W) % Lmax;
         size_t px = (size_t)x + (size_t)y * W;
         v3 P = xyz(gpos[px]);
@@ -115,13 +119,19 @@ __global__ void __launch_bounds__(256) k_create_shadow(const float4* gpos, const
             v3 d = normalize(P - o);
             float diffuse = fmaxf(dot(d, -N), 0.0f);
             if (diffuse > 0.0f) {
-                store_ray(rays, i, o, d);
-                keys[i] = make_key(kc, o, d, 1, l, seg_of(kc, V, (uint32_t)px));
-                values[i] = i;
+                if (soft) {
+                    v3 a0, a1;
+                    soft_axes(d, a0, a1);
+                    o = soft_point(o, a0, a1, r & 15u);
+                    d = normalize(P - o);
+                }
+                store_ray(rays, r, o, d);
+                keys[r] = make_key(kc, o, d, 1, l, seg_of(kc, V, (uint32_t)px));
+                values[r] = r;
                 flag = 0;
             }
         }
-        flags[i] = flag;
+        flags[r] = flag;
     }
 }
 
@@ -219,8 +229,23 @@ __global__ void __launch_bounds__(256) k_surface(const float* rays, const unsign
 }
 
 // RayTracer.cu:1616-1709 ColorPrimaryShadowRay
+// soft: 16 flags per slot (soft shadows), shaded by their grid weights.
+__device__ inline uint32_t soft_mask(const uint32_t* sflags, size_t slot) {
+    const uint4* q = reinterpret_cast<const uint4*>(sflags + 16 * slot);
+    uint32_t m = 0;
+    for (uint32_t k = 0; k < 4; k++) {
+        const uint4 v = q[k];
+        m |= (v.x == MISS32 ? 1u : 0u) << (4 * k);
+        m |= (v.y == MISS32 ? 2u : 0u) << (4 * k);
+        m |= (v.z == MISS32 ? 4u : 0u) << (4 * k);
+        m |= (v.w == MISS32 ? 8u : 0u) << (4 * k);
+    }
+    return m;
+}
+
 __global__ void __launch_bounds__(256) k_shade_primary(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L,
-                                                       Views V, const uint32_t* sflags, uint32_t* pixels) {
+                                                       Views V, const uint32_t* sflags, uint32_t soft,
+                                                       uint32_t* pixels) {
     const uint32_t n = W * H;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
         uint32_t x = p % W, y = p / W;
@@ -230,8 +255,13 @@ __global__ void __launch_bounds__(256) k_shade_primary(GBufView g, uint32_t W, u
             const v3 eye = view_eye(V, view_of(V, p));
             float4 Kd = g.dif[p], Ks = g.spec[p];
             for (uint32_t l = 0; l < L.total; l++) {
-                if (sflags[(size_t)(x + y * W * Lmax) + (size_t)W * l] != MISS32)
-                    color = color + light_term(L, l, P, N, eye, Kd, Ks, false);
+                const size_t slot = (size_t)(x + y * W * Lmax) + (size_t)W * l;
+                if (!soft) {
+                    if (sflags[slot] != MISS32) color = color + light_term(L, l, P, N, eye, Kd, Ks, false);
+                } else {
+                    const uint32_t m = soft_mask(sflags, slot);
+                    if (m != 0xFFFFu) color = color + light_term(L, l, P, N, eye, Kd, Ks, false, true, soft_factor(m));
+                }
             }
         }
         pixels[p] = rgb_to_int(color.x * 255.0f, color.y * 255.0f, color.z * 255.0f);
@@ -242,7 +272,7 @@ __global__ void __launch_bounds__(256) k_shade_primary(GBufView g, uint32_t W, u
 // flags a light whose shadow ray is occluded is skipped.
 __global__ void __launch_bounds__(256) k_shade_reflection(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L,
                                                           Views V, const unsigned long long* rec, uint32_t hit_mode,
-                                                          const uint32_t* sflags, uint32_t* pixels) {
+                                                          const uint32_t* sflags, uint32_t soft, uint32_t* pixels) {
     const uint32_t n = W * H;
     const unsigned long long miss = hit_mode == 0 ? MISS64 : (unsigned long long)MISS32;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
@@ -254,7 +284,13 @@ __global__ void __launch_bounds__(256) k_shade_reflection(GBufView g, uint32_t W
         float4 Kd = g.dif[p], Ks = g.spec[p];
         v3 color = mk(0.0f, 0.0f, 0.0f);
         for (uint32_t l = 0; l < L.total; l++) {
-            if (sflags && sflags[(size_t)slot + (size_t)W * l] == MISS32) continue;
+            const size_t sl = (size_t)slot + (size_t)W * l;
+            if (sflags && soft) {
+                const uint32_t m = soft_mask(sflags, sl);
+                if (m != 0xFFFFu) color = color + light_term(L, l, P, N, eye, Kd, Ks, true, true, soft_factor(m));
+                continue;
+            }
+            if (sflags && sflags[sl] == MISS32) continue;
             color = color + light_term(L, l, P, N, eye, Kd, Ks, true);
         }
         pixels[p] += rgb_to_int(color.x * Ks.w, color.y * Ks.w, color.z * Ks.w);
@@ -417,10 +453,10 @@ void launch_create_primary(const Views& V, uint32_t P, KeyCfg kc, float* rays, u
     k_create_primary<<<grid_for(P, 256, 65536), 256, 0, s>>>(V, P, kc, rays, flags, keys, values);
 }
 void launch_create_shadow(const float4* gpos, const float4* gnrm, uint32_t W, uint32_t H, uint32_t Lmax,
-                          const float4* light_pos, KeyCfg kc, const Views& V, float* rays, uint32_t* flags,
-                          uint32_t* keys, uint32_t* values, hipStream_t s) {
-    k_create_shadow<<<grid_for((uint64_t)W * H * Lmax, 256, 65536), 256, 0, s>>>(gpos, gnrm, W, H, Lmax, light_pos, kc,
-                                                                                   V, rays, flags, keys, values);
+                          const float4* light_pos, KeyCfg kc, const Views& V, uint32_t soft, float* rays,
+                          uint32_t* flags, uint32_t* keys, uint32_t* values, hipStream_t s) {
+    k_create_shadow<<<grid_for((uint64_t)W * H * Lmax << (soft ? 4 : 0), 256, 65536), 256, 0, s>>>(
+        gpos, gnrm, W, H, Lmax, light_pos, kc, V, soft, rays, flags, keys, values);
 }
 void launch_create_reflection(const float4* gpos, const float4* gnrm, const float4* gdif, uint32_t W, uint32_t H,
                               uint32_t Lmax, const Views& V, uint32_t gate, uint32_t mark_gaps, KeyCfg kc,
@@ -440,14 +476,14 @@ void launch_surface(const float* rays, const unsigned long long* rec, uint32_t h
                                                                      normalize_n, sc, g);
 }
 void launch_shade_primary(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L, const Views& V,
-                          const uint32_t* sflags, uint32_t* pixels, hipStream_t s) {
-    k_shade_primary<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(g, W, H, Lmax, L, V, sflags, pixels);
+                          const uint32_t* sflags, uint32_t soft, uint32_t* pixels, hipStream_t s) {
+    k_shade_primary<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(g, W, H, Lmax, L, V, sflags, soft, pixels);
 }
 void launch_shade_reflection(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L, const Views& V,
                              const unsigned long long* rec, uint32_t hit_mode, const uint32_t* sflags,
-                             uint32_t* pixels, hipStream_t s) {
+                             uint32_t soft, uint32_t* pixels, hipStream_t s) {
     k_shade_reflection<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(g, W, H, Lmax, L, V, rec, hit_mode,
-                                                                              sflags, pixels);
+                                                                              sflags, soft, pixels);
 }
 void launch_cluster_build(const float4* tri_sphere, const uint2* range, uint32_t n, float4* cl_sphere,
                           hipStream_t s) {

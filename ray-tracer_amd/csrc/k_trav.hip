@@ -38,10 +38,11 @@ inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap = 1u << 20) {
 // Sorted ray records: rec0 = (origin, slot bits), rec1 = (direction, tmax).
 // tmax = |o - P| - 2 eps for shadow rays (RayTracer.cu:1610), P the surface
 // point of the slot's pixel, or the caller's per-slot tmax for ray queries;
-// unused for closest-hit passes.
+// unused for closest-hit passes.  slot_shift: 4 for soft shadows (16 rays per
+// pixel-light slot), else 0.
 __global__ void __launch_bounds__(256) k_gather(const float* rays, const uint32_t* sorted_values, uint32_t N,
                                                 const float4* shadow_pos, const float* tmax_in, uint32_t W,
-                                                uint32_t Lmax, float4* rec0, float4* rec1) {
+                                                uint32_t Lmax, uint32_t slot_shift, float4* rec0, float4* rec1) {
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
         uint32_t slot = sorted_values[p];
         const float2* r = reinterpret_cast<const float2*>(rays + 6 * (size_t)slot);
@@ -51,7 +52,8 @@ __global__ void __launch_bounds__(256) k_gather(const float* rays, const uint32_
         if (tmax_in) {
             tmax = tmax_in[slot];
         } else if (shadow_pos) {
-            uint32_t x = slot % W, y = slot / (W * Lmax);
+            const uint32_t i = slot >> slot_shift;
+            uint32_t x = i % W, y = i / (W * Lmax);
             v3 P = xyz(shadow_pos[(size_t)x + (size_t)y * W]);
             tmax = len(o - P) - EPS * 2.0f;
         }
@@ -1162,9 +1164,10 @@ __global__ void __launch_bounds__(256) k_hits_intersect(const float* rays, const
 
 // ================================================================ launchers
 void launch_gather(const float* rays, const uint32_t* sorted_values, uint32_t N, const float4* shadow_pos,
-                   const float* tmax, uint32_t W, uint32_t Lmax, float4* rec0, float4* rec1, hipStream_t s) {
-    if (N) k_gather<<<grid_for(N, 256, 65536), 256, 0, s>>>(rays, sorted_values, N, shadow_pos, tmax, W, Lmax, rec0,
-                                                            rec1);
+                   const float* tmax, uint32_t W, uint32_t Lmax, uint32_t slot_shift, float4* rec0, float4* rec1,
+                   hipStream_t s) {
+    if (N) k_gather<<<grid_for(N, 256, 65536), 256, 0, s>>>(rays, sorted_values, N, shadow_pos, tmax, W, Lmax,
+                                                            slot_shift, rec0, rec1);
 }
 
 void launch_hierarchy(const float4* rec0, const float4* rec1, const float* rays, const uint32_t* sorted_values,

@@ -486,9 +486,48 @@ struct Lights {
     uint32_t total;
 };
 
+// ----------------------------------------------------------- soft shadows --
+// The reference's soft shadows (RayTracer.cu:1550-1603 and 1680-1697, compiled
+// out by Constants.h:32): 16 points about the light on a 4 x 4 grid spaced by
+// 1/8 of two plane axes, the grid's centre row and column left out.  Cell
+// c = i + 4 j lies at offsets (a - 2, b - 2) with a = i + (i >= 2) and
+// b = j + (j >= 2).  The first axis is the cross product of a coordinate axis
+// with the shadow ray's direction d; the axis is picked by the reference's
+// test against min(|x|, max(|y|, |z|)), which chooses x unless |x| is the
+// largest component.  The axes are not normalised.
+__device__ inline void soft_axes(v3 d, v3& a0, v3& a1) {
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    const float m = fminf(ax, fmaxf(ay, az));
+    if (ax == m) a0 = cross(mk(1.0f, 0.0f, 0.0f), d);
+    else if (ay == m) a0 = cross(mk(0.0f, 1.0f, 0.0f), d);
+    else a0 = cross(mk(0.0f, 0.0f, 1.0f), d);
+    a1 = cross(d, a0);
+}
+
+__device__ inline v3 soft_point(v3 o, v3 a0, v3 a1, uint32_t c) {
+    const uint32_t i = c & 3u, j = c >> 2;
+    const float u = (float)((int)(i + (i >= 2u ? 1u : 0u)) - 2), w = (float)((int)(j + (j >= 2u ? 1u : 0u)) - 2);
+    return (o + (a0 * u) * 0.125f) + (a1 * w) * 0.125f;
+}
+
+// Light scale of a pixel whose grid cells in `occluded` (bit c: cell c) are
+// shadowed: log2(1 + 2 s), s the sum over open cells of 0.55/16 (the four
+// inner cells), 0.5375/16 (the corners) or 0.525/16 (the rest), in cell order.
+__device__ inline float soft_factor(uint32_t occluded) {
+    float s = 0.0f;
+    for (uint32_t c = 0; c < 16; c++) {
+        if ((occluded >> c) & 1u) continue;
+        if (c == 5 || c == 6 || c == 9 || c == 10) s += 0.55f / 16.0f;
+        else if (c == 0 || c == 3 || c == 12 || c == 15) s += 0.5375f / 16.0f;
+        else s += 0.525f / 16.0f;
+    }
+    return log2_s(1.0f + s * 2.0f);
+}
+
 // Blinn-Phong term of one light (RayTracer.cu:1655-1702, :1847-1876).
+// soft: both components are scaled by `factor` last (soft shadows).
 __device__ inline v3 light_term(const Lights& L, uint32_t l, v3 P, v3 N, v3 eye, float4 Kd, float4 Ks,
-                                bool reflection) {
+                                bool reflection, bool soft = false, float factor = 1.0f) {
     v3 color = mk(0.0f, 0.0f, 0.0f);
     v3 Ld = xyz(L.pos[l]) - P;
     float dist = len(Ld);
@@ -500,13 +539,20 @@ __device__ inline v3 light_term(const Lights& L, uint32_t l, v3 P, v3 N, v3 eye,
     if (diffuse > 0.0f) {
         v3 Hv = normalize(Ld - normalize(P - eye));
         float specular = clampf(pow_s(fmaxf(dot(Hv, N), 0.0f), Ks.w), 0.0f, 1.0f);
+        v3 cd, cs;
         if (reflection) {
-            color = color + ((((xyz(Kd) * lc) * diffuse) * I.x) * att) * 0.75f;
-            color = color + ((((xyz(Ks) * lc) * specular) * I.y) * att) * 0.75f;
+            cd = ((((xyz(Kd) * lc) * diffuse) * I.x) * att) * 0.75f;
+            cs = ((((xyz(Ks) * lc) * specular) * I.y) * att) * 0.75f;
         } else {
-            color = color + (((xyz(Kd) * lc) * diffuse) * I.x) * att;
-            color = color + (((xyz(Ks) * lc) * specular) * I.y) * att;
+            cd = (((xyz(Kd) * lc) * diffuse) * I.x) * att;
+            cs = (((xyz(Ks) * lc) * specular) * I.y) * att;
         }
+        if (soft) {
+            cd = cd * factor;
+            cs = cs * factor;
+        }
+        color = color + cd;
+        color = color + cs;
     }
     return color;
 }

@@ -181,8 +181,8 @@ void launch_item_desc(const TravParams& p, uint4* desc, hipStream_t s);
 void launch_create_primary(const Views& V, uint32_t P, KeyCfg kc, float* rays, uint32_t* flags, uint32_t* keys,
                            uint32_t* values, hipStream_t s);
 void launch_create_shadow(const float4* gpos, const float4* gnrm, uint32_t W, uint32_t H, uint32_t Lmax,
-                          const float4* light_pos, KeyCfg kc, const Views& V, float* rays, uint32_t* flags,
-                          uint32_t* keys, uint32_t* values, hipStream_t s);
+                          const float4* light_pos, KeyCfg kc, const Views& V, uint32_t soft, float* rays,
+                          uint32_t* flags, uint32_t* keys, uint32_t* values, hipStream_t s);
 void launch_create_reflection(const float4* gpos, const float4* gnrm, const float4* gdif, uint32_t W, uint32_t H,
                               uint32_t Lmax, const Views& V, uint32_t gate, uint32_t mark_gaps, KeyCfg kc,
                               float* rays, uint32_t* flags, uint32_t* keys, uint32_t* values, hipStream_t s);
@@ -192,10 +192,10 @@ void launch_create_bounce(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, con
 void launch_surface(const float* rays, const unsigned long long* rec, uint32_t hit_mode, uint32_t W, uint32_t H,
                     uint32_t slot_stride, uint32_t normalize_n, SceneView sc, GBufView g, hipStream_t s);
 void launch_shade_primary(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L, const Views& V,
-                          const uint32_t* sflags, uint32_t* pixels, hipStream_t s);
+                          const uint32_t* sflags, uint32_t soft, uint32_t* pixels, hipStream_t s);
 void launch_shade_reflection(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L, const Views& V,
                              const unsigned long long* rec, uint32_t hit_mode, const uint32_t* sflags,
-                             uint32_t* pixels, hipStream_t s);
+                             uint32_t soft, uint32_t* pixels, hipStream_t s);
 void launch_triangle_prep(const float4* tri_pos, uint32_t T, float4* tri_sphere, float4* tri_mt, hipStream_t s);
 void launch_sub_use(const float4* cl_sphere, const float4* sub_sphere, uint32_t nc, uint8_t* sub_use, hipStream_t s);
 void launch_cluster_build(const float4* tri_sphere, const uint2* range, uint32_t n, float4* cl_sphere,
@@ -212,7 +212,8 @@ void launch_color_reflection_ref(const float4* tp, const float4* tn, uint32_t W,
 
 // --- hierarchy / traversal ---
 void launch_gather(const float* rays, const uint32_t* sorted_values, uint32_t N, const float4* shadow_pos,
-                   const float* tmax, uint32_t W, uint32_t Lmax, float4* rec0, float4* rec1, hipStream_t s);
+                   const float* tmax, uint32_t W, uint32_t Lmax, uint32_t slot_shift, float4* rec0, float4* rec1,
+                   hipStream_t s);
 // Ray queries: key, value (= index) and live flag of n caller rays (float[6n]);
 // segs (optional) places a per-ray segment id at kc.h.segment_shift.
 void launch_query_keys(const float* rays, const uint32_t* segs, uint32_t n, KeyCfg kc, uint32_t type,

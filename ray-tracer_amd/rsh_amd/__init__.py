@@ -48,7 +48,7 @@ class FrameConfigDesc(C.Structure):
                 ("key_layout", u32), ("scene_min", f32 * 3), ("cell_scale", f32 * 3),
                 ("segment_shift", u32), ("initial_radius", f32), ("initial_spread", f32),
                 ("traversal_kernel", u32), ("traversal_splits", u32), ("traversal_chunk", u32),
-                ("traversal_prefilter", u32)]
+                ("traversal_prefilter", u32), ("soft_shadows", u32)]
 
 
 class FrameStats(C.Structure):

@@ -50,6 +50,7 @@ class FrameConfig:
     traversal_splits: int = 0       # engine-only, generic kernel: waves per top node, 0 = fill the GPU
     traversal_chunk: int = 0        # engine-only, 64-ray kernel: candidate clusters per work item, 0 = per pass
     traversal_prefilter: int = 0    # engine-only, 64-ray kernel: 0 auto, 1 no pre-test, 2 triangle, 3 + cluster masks
+    soft_shadows: int = 0           # 1: 16 shadow rays per slot from a 4x4 grid about the light
 
     def with_scene_bounds(self, scene: Scene) -> "FrameConfig":
         """Cell grid over the scene AABB: 2^cell_bits cells per axis."""

@@ -305,6 +305,25 @@ def test_obj_scene_matches_oracle(engine, oracle, rsh, tmp_path):
     run_case(engine, oracle, rsh, s2, cam, rsh.config.config0(96, 96))
 
 
+@pytest.mark.parametrize("case", ["c0", "c0_ref_keys", "c1_bounce_shadows"])
+def test_soft_shadows_match_oracle(engine, oracle, rsh, case):
+    # soft shadows: 16 rays per shadow slot, shading by the grid weights of
+    # the open cells; one light, reference keys, and two lights with shadow
+    # passes at every bounce
+    import dataclasses
+    if case == "c1_bounce_shadows":
+        scene, cam = rsh.scenes.config1_scene(), rsh.scenes.config1_camera(80, 64)
+        cfg = rsh.config.config1(80, 64)
+    else:
+        scene, cam = c0(rsh, 96)
+        cfg = rsh.config.config0(96, 96, key_mode=0 if case == "c0_ref_keys" else 1)
+    rep = run_case(engine, oracle, rsh, scene, cam, dataclasses.replace(cfg, soft_shadows=1))
+    st = engine.stats
+    assert all(st.pass_slots[i] == 16 * cfg.width * cfg.height * scene.light_total
+               for i in range(st.pass_count) if st.pass_kind[i] == 1)
+    assert rep["pixels_exact"]
+
+
 @pytest.mark.parametrize("key_mode,gate", [(0, 0), (1, 1)])
 def test_cornell_layout_matches_oracle(engine, oracle, rsh, tmp_path, key_mode, gate):
     # the Cornell box assembled from per-object transforms (rsh_amd.objfile)

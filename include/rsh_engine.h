@@ -67,6 +67,14 @@ void ShadowRayCreationWrapper(const unsigned int windowWidth, const unsigned int
                               const unsigned int lightTotal, rsh_float3* rayArray,
                               unsigned int* headFlagsArray, unsigned int* rayIndexKeysArray,
                               unsigned int* rayIndexValuesArray);
+/* No counterpart wrapper in the reference: launches its CreateRefractionRays
+   (RayTracer.cu:793-842, which no reference code calls) with the arguments of
+   ReflectionRayCreationWrapper; refraction index 1/1.52, the reference's
+   CreateRefractionRayIndex keys (RayTracer.cu:200-227). */
+void RefractionRayCreationWrapper(const unsigned int windowWidth, const unsigned int windowHeight,
+                                  const rsh_float3 cameraPosition, rsh_float3* rayArray,
+                                  unsigned int* headFlagsArray, unsigned int* rayIndexKeysArray,
+                                  unsigned int* rayIndexValuesArray);
 /* RayTracer.cu:2243 ReflectionRayCreationWrapper */
 void ReflectionRayCreationWrapper(const unsigned int windowWidth, const unsigned int windowHeight,
                                   const rsh_float3 cameraPosition, rsh_float3* rayArray,

@@ -110,6 +110,8 @@ float oracle_pow(float x, float y);
 uint32_t oracle_f2u_sat(float x);
 uint32_t oracle_shadow_key_ref(const float* o, const float* d, uint32_t light);
 uint32_t oracle_reflection_key_ref(const float* o, const float* d);
+uint32_t oracle_refraction_key_ref(const float* o, const float* d);
+void oracle_refract(const float* i, const float* n, float ior, float* out3);
 uint32_t oracle_key_oct(const oracle_config* c, const float* o, const float* d, uint32_t type);
 float oracle_ray_triangle(const float* o, const float* d, const float* v0, const float* e1, const float* e2);
 // Soft shadows: the 16 grid points (float3 each) about `light` of a shadow
@@ -149,6 +151,8 @@ void oracle_BoundingSphereUpdateWrapper(const float* translation, const float* s
                                         uint32_t K, float* spheres3_out);
 void oracle_ShadowRayCreationWrapper(uint32_t W, uint32_t H, uint32_t L, float* rays,
                                      uint32_t* flags, uint32_t* keys, uint32_t* values);
+void oracle_RefractionRayCreationWrapper(uint32_t W, uint32_t H, const float* eye, float* rays,
+                                         uint32_t* flags, uint32_t* keys, uint32_t* values);
 void oracle_ReflectionRayCreationWrapper(uint32_t W, uint32_t H, const float* eye, float* rays,
                                          uint32_t* flags, uint32_t* keys, uint32_t* values);
 void oracle_RayTrimmingWrapper(const uint32_t* keys, const uint32_t* values, uint32_t W,

@@ -104,6 +104,9 @@ def lib():
         L.oracle_shadow_key_ref.argtypes = [PF, PF, u32]
         L.oracle_reflection_key_ref.restype = u32
         L.oracle_reflection_key_ref.argtypes = [PF, PF]
+        L.oracle_refraction_key_ref.restype = u32
+        L.oracle_refraction_key_ref.argtypes = [PF, PF]
+        L.oracle_refract.argtypes = [PF, PF, f32, PF]
         L.oracle_key_oct.restype = u32
         L.oracle_key_oct.argtypes = [C.POINTER(OConfig), PF, PF, u32]
         L.oracle_ray_triangle.restype = f32

@@ -76,6 +76,29 @@ uint32_t reflection_key_ref(f3 o, f3 d) {
     return index;
 }
 
+// RT:200-227 CreateRefractionRayIndex: the origin's distance (7-bit field
+// shifted by 26 in a 32-bit word, so its top bits are lost), then the four
+// fields of the reflection key.
+uint32_t refraction_key_ref(f3 o, f3 d) {
+    float distance = length(o);
+    uint32_t index = clampu(f2u_sat(distance), 0u, 128u);
+    index = (index << 7) | clampu(f2u_sat((atan2_diag(o.x) + HALF_PI_F) * 20.0f), 0u, 128u);
+    index = (index << 7) | clampu(f2u_sat(acos_f(o.z / distance) * 20.0f), 0u, 128u);
+    index = (index << 7) | clampu(f2u_sat((atan2_diag(d.x) + HALF_PI_F) * 5.0f), 0u, 30u);
+    index = (index << 5) | clampu(f2u_sat(acos_f(d.z) * 5.0f), 0u, 30u);
+    index++;
+    return index;
+}
+
+// helper_math.h:1422-1431 refract, pow(x, 2) as x * x; zero on total internal
+// reflection.
+f3 refract(f3 i, f3 n, float ior) {
+    const float c = dot(i, n);
+    const float k = 1.0f - (ior * ior) * (1.0f - c * c);
+    if (k < 0.0f) return f3{0.0f, 0.0f, 0.0f};
+    return i * ior - n * (ior * c + std::sqrt(k));
+}
+
 // Octahedral key (configuration hash; no counterpart in the reference):
 //   [type:2][cell z|y|x : 3*cb][u|v : 2*db]   (layout 0)
 //   [type:2][z y x u v bits interleaved, most significant level first] (layout 2)
@@ -943,6 +966,11 @@ float oracle_pow(float x, float y) { return pow_f(x, y); }
 uint32_t oracle_f2u_sat(float x) { return f2u_sat(x); }
 uint32_t oracle_shadow_key_ref(const float* o, const float* d, uint32_t l) { return shadow_key_ref(ld3(o), ld3(d), l); }
 uint32_t oracle_reflection_key_ref(const float* o, const float* d) { return reflection_key_ref(ld3(o), ld3(d)); }
+uint32_t oracle_refraction_key_ref(const float* o, const float* d) { return refraction_key_ref(ld3(o), ld3(d)); }
+void oracle_refract(const float* i, const float* n, float ior, float* out3) {
+    const f3 r = refract(ld3(i), ld3(n), ior);
+    out3[0] = r.x; out3[1] = r.y; out3[2] = r.z;
+}
 uint32_t oracle_key_oct(const oracle_config* c, const float* o, const float* d, uint32_t type) {
     return key_oct(*c, ld3(o), ld3(d), type);
 }
@@ -1245,6 +1273,28 @@ void oracle_ReflectionRayCreationWrapper(uint32_t W, uint32_t H, const float* ey
                 f3 d = reflect(normalize(P - eye), normalize(N));
                 st3(rays + 6 * (size_t)r, P); st3(rays + 6 * (size_t)r + 3, d);
                 keys[r] = reflection_key_ref(P, d);
+                values[r] = r;
+                flags[r] = 0;
+            } else {
+                flags[r] = 1;
+            }
+        }
+}
+
+// RT:793-842 CreateRefractionRays (never launched by the reference), with the
+// arguments of the reflection wrapper.
+void oracle_RefractionRayCreationWrapper(uint32_t W, uint32_t H, const float* eyep, float* rays, uint32_t* flags,
+                                         uint32_t* keys, uint32_t* values) {
+    f3 eye = ld3(eyep);
+    for (uint32_t y = 0; y < H; y++)
+        for (uint32_t x = 0; x < W; x++) {
+            uint32_t r = x + y * W * B.light_maximum;
+            f3 P = ld3(B.g_pos + 4 * (size_t)(x + y * W));
+            f3 N = normalize(ld3(B.g_nrm + 4 * (size_t)(x + y * W)));
+            if (length(P) != 0.0f) {
+                f3 d = refract(normalize(P - eye), normalize(N), 1.0f / 1.52f);
+                st3(rays + 6 * (size_t)r, P); st3(rays + 6 * (size_t)r + 3, d);
+                keys[r] = refraction_key_ref(P, d);
                 values[r] = r;
                 flags[r] = 0;
             } else {

@@ -197,6 +197,17 @@ void ReflectionRayCreationWrapper(const unsigned int W, const unsigned int H, co
     check(hipGetLastError(), "ReflectionRayCreationWrapper");
 }
 
+// No wrapper in the reference: CreateRefractionRays (RayTracer.cu:793-842) is
+// never launched there.  Same arguments as ReflectionRayCreationWrapper.
+void RefractionRayCreationWrapper(const unsigned int W, const unsigned int H, const rsh_float3 eye, rsh_float3* rays,
+                                  unsigned int* flags, unsigned int* keys, unsigned int* values) {
+    Compat& c = C();
+    if (!need(c.g_pos, "bindFragmentPositionArray") || !need(c.g_nrm, "bindFragmentNormalArray")) return;
+    launch_create_refraction(c.g_pos, c.g_nrm, W, H, c.light_max, single_view(W, H, V(eye)), ref_keys(),
+                             reinterpret_cast<float*>(rays), flags, keys, values, c.stream);
+    check(hipGetLastError(), "RefractionRayCreationWrapper");
+}
+
 void RayTrimmingWrapper(unsigned int* keys, unsigned int* values, const unsigned int W, const unsigned int H,
                         const unsigned int L, unsigned int* flags, unsigned int*, unsigned int* tkeys,
                         unsigned int* tvalues, unsigned int* rayTotal) {

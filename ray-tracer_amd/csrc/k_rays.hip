@@ -36,10 +36,11 @@ __device__ __forceinline__ void load_ray(const float* rays, size_t slot, v3& o, 
     d = mk(b.y, c.x, c.y);
 }
 
-// type: 0 primary, 1 shadow (light l), 2 mirror
+// type: 0 primary, 1 shadow (light l), 2 mirror, 3 refraction
 // seg: the segment (view) id OR-ed in at kc.h.segment_shift when that is set.
 __device__ __forceinline__ uint32_t make_key(const KeyCfg& kc, v3 o, v3 d, uint32_t type, uint32_t l, uint32_t seg) {
-    if (kc.mode == 0) return type == 1 ? shadow_key_ref(d, l) : reflection_key_ref(o, d);
+    if (kc.mode == 0)
+        return type == 1 ? shadow_key_ref(d, l) : type == 3 ? refraction_key_ref(o, d) : reflection_key_ref(o, d);
     uint32_t k = key_oct(kc.h, o, d, type);
     if (kc.h.segment_shift) k |= seg << kc.h.segment_shift;
     return k;
@@ -160,6 +161,31 @@ __global__ void __launch_bounds__(256) k_create_reflection(const float4* gpos, c
         }
         if (mark_gaps)
             for (uint32_t j = 1; j < Lmax; j++) flags[slot + j * W] = 1;
+    }
+}
+
+// RayTracer.cu:793-842 CreateRefractionRays (no caller in the reference):
+// slot x + y*W*Lmax, origin = the surface point, direction = refract(view
+// direction, normal, 1 / 1.52) for every covered pixel.
+__global__ void __launch_bounds__(256) k_create_refraction(const float4* gpos, const float4* gnrm, uint32_t W,
+                                                           uint32_t H, uint32_t Lmax, Views V, KeyCfg kc, float* rays,
+                                                           uint32_t* flags, uint32_t* keys, uint32_t* values) {
+    const uint32_t n = W * H;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+        uint32_t x = p % W, y = p / W;
+        uint32_t slot = x + y * W * Lmax;
+        v3 P = xyz(gpos[p]);
+        v3 N = normalize(xyz(gnrm[p]));
+        if (len(P) != 0.0f) {
+            const uint32_t v = view_of(V, p);
+            v3 d = refract(normalize(P - view_eye(V, v)), normalize(N), 1.0f / 1.52f);
+            store_ray(rays, slot, P, d);
+            keys[slot] = make_key(kc, P, d, 3, 0, v);
+            values[slot] = slot;
+            flags[slot] = 0;
+        } else {
+            flags[slot] = 1;
+        }
     }
 }
 
@@ -463,6 +489,12 @@ void launch_create_reflection(const float4* gpos, const float4* gnrm, const floa
                               float* rays, uint32_t* flags, uint32_t* keys, uint32_t* values, hipStream_t s) {
     k_create_reflection<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(gpos, gnrm, gdif, W, H, Lmax, V, gate,
                                                                                mark_gaps, kc, rays, flags, keys, values);
+}
+void launch_create_refraction(const float4* gpos, const float4* gnrm, uint32_t W, uint32_t H, uint32_t Lmax,
+                              const Views& V, KeyCfg kc, float* rays, uint32_t* flags, uint32_t* keys,
+                              uint32_t* values, hipStream_t s) {
+    k_create_refraction<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(gpos, gnrm, W, H, Lmax, V, kc, rays,
+                                                                               flags, keys, values);
 }
 void launch_create_bounce(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, const unsigned long long* rec,
                           uint32_t hit_mode, uint32_t gate, KeyCfg kc, const Views& V, float* rays, uint32_t* flags,

@@ -35,6 +35,14 @@ __device__ __forceinline__ v3 cross(v3 a, v3 b) {
 __device__ __forceinline__ float len(v3 v) { return sqrtf(dot(v, v)); }
 __device__ __forceinline__ v3 normalize(v3 v) { float inv = 1.0f / sqrtf(dot(v, v)); return v * inv; }
 __device__ __forceinline__ v3 reflect(v3 i, v3 n) { return i - 2.0f * n * dot(n, i); }
+// helper_math.h:1422-1431, pow(x, 2) taken as the correctly rounded x * x;
+// (0, 0, 0) on total internal reflection.
+__device__ inline v3 refract(v3 i, v3 n, float ior) {
+    const float c = dot(i, n);
+    const float k = 1.0f - (ior * ior) * (1.0f - c * c);
+    if (k < 0.0f) return v3{0.0f, 0.0f, 0.0f};
+    return i * ior - n * (ior * c + sqrtf(k));
+}
 __device__ __forceinline__ float clampf(float f, float a, float b) { return fmaxf(a, fminf(f, b)); }
 __device__ __forceinline__ uint32_t clampu(uint32_t f, uint32_t a, uint32_t b) {
     uint32_t m = f < b ? f : b;
@@ -202,6 +210,19 @@ __device__ inline uint32_t shadow_key_ref(v3 d, uint32_t l) {
 __device__ inline uint32_t reflection_key_ref(v3 o, v3 d) {
     float distance = len(o);
     uint32_t index = clampu(f2u((atan2_diag(o.x) + HALF_PI_F) * 20.0f), 0u, 128u);
+    index = (index << 7) | clampu(f2u(acos_s(o.z / distance) * 20.0f), 0u, 128u);
+    index = (index << 7) | clampu(f2u((atan2_diag(d.x) + HALF_PI_F) * 5.0f), 0u, 30u);
+    index = (index << 5) | clampu(f2u(acos_s(d.z) * 5.0f), 0u, 30u);
+    return index + 1u;
+}
+
+// RayTracer.cu:200-227 CreateRefractionRayIndex: the distance from the origin,
+// then the reflection key's four fields; 7 + 7 + 7 + 7 + 5 bits in 32, so the
+// distance field's top bits leave the word, as in the reference.
+__device__ inline uint32_t refraction_key_ref(v3 o, v3 d) {
+    float distance = len(o);
+    uint32_t index = clampu(f2u(distance), 0u, 128u);
+    index = (index << 7) | clampu(f2u((atan2_diag(o.x) + HALF_PI_F) * 20.0f), 0u, 128u);
     index = (index << 7) | clampu(f2u(acos_s(o.z / distance) * 20.0f), 0u, 128u);
     index = (index << 7) | clampu(f2u((atan2_diag(d.x) + HALF_PI_F) * 5.0f), 0u, 30u);
     index = (index << 5) | clampu(f2u(acos_s(d.z) * 5.0f), 0u, 30u);

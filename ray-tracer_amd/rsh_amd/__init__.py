@@ -80,6 +80,7 @@ COMPAT_SIGNATURES = {
     "MemoryPreparationWrapper": [_U, _U, _U, _P, _P, _P, _P, _P, _P],
     "ShadowRayCreationWrapper": [_U, _U, _U, _P, _P, _P, _P],
     "ReflectionRayCreationWrapper": [_U, _U, _F3, _P, _P, _P, _P],
+    "RefractionRayCreationWrapper": [_U, _U, _F3, _P, _P, _P, _P],
     "RayTrimmingWrapper": [_P, _P, _U, _U, _U, _P, _P, _P, _P, PU],
     "RayCompressionWrapper": [_P, _P, _U, _P, _P, _P, _P, _P, _P, PU],
     "RaySortingWrapper": [_P, _P, _U, _P, _P],

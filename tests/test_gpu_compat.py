@@ -4,7 +4,8 @@ The test follows the reference's frame driver for CRSH and RAH
 (projects/CUDA-Engine/source/main.cpp:630-1054, 1234-1303): shadow rays from a
 G-buffer, trim, compress, sort, decompress, hierarchy, per-triangle-batch
 warm-up traversal, lower-level traversal, shadow intersection and shading,
-then two mirror bounces.  After every wrapper the device buffers are compared
+then two mirror bounces, then refraction rays (the reference's unlaunched
+CreateRefractionRays) traced for closest hits.  After every wrapper the device buffers are compared
 with the oracle's host buffers (bit-exact, including the order of the
 compacted hit records).
 """
@@ -252,6 +253,20 @@ def test_reference_frame_through_wrappers(rsh, oracle, setup, algorithm):
         oracle.call("oracle_ReflectionRayColoringWrapper", tri.host, unrm.host, W, H, scene.light_total, eye,
                     create, sflags.host, rays.host, flags.host, pk.host, pv.host, pixels.host)
         eq(pixels, None, f"reflection shading, bounce {it}")
+
+    # ---- refraction rays (RT.cu:793-842, never launched by the reference), traced for closest hits ----
+    L.RefractionRayCreationWrapper(W, H, f3(eye), rays.dev.ptr, flags.dev.ptr, pk.dev.ptr, pv.dev.ptr)
+    oracle.call("oracle_RefractionRayCreationWrapper", W, H, eye, rays.host, flags.host, pk.host, pv.host)
+    live = flags.host[:S] == 0
+    np.testing.assert_array_equal(flags.d()[:S], flags.host[:S])
+    np.testing.assert_array_equal(pk.d()[live], pk.host[live])
+    np.testing.assert_array_equal(rays.d().reshape(S, 2, 3)[live], rays.host.reshape(S, 2, 3)[live])
+    N = cast_rays(pk, pv)
+    assert N == int(live.sum()) > 0
+    L.ReflectionRayPreparationWrapper(W, H, sflags.dev.ptr)
+    oracle.call("oracle_ReflectionRayPreparationWrapper", W, H, sflags.host)
+    traverse(N, False)
+    eq(sflags, None, "refraction closest-hit records")
     assert L.rsh_compat_status() == 0, L.rsh_compat_error()
     print("compat frame ok, algorithm", algorithm, file=sys.stderr)
 

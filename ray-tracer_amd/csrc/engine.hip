@@ -439,7 +439,8 @@ rsh_status build_items(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, Trav
 // closest; kind 1: shadow flags into e->shadow.
 rsh_status trace_pass(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, const float* rays, uint32_t S,
                       uint32_t kind, const float4* shadow_pos, uint32_t flags_mask, PassOut& out, uint32_t pass_idx,
-                      DevBuf& closest, const float* tmax = nullptr, bool camera = false, uint32_t sort_bits = 0) {
+                      DevBuf& closest, const float* tmax = nullptr, bool camera = false, uint32_t sort_bits = 0,
+                      bool presorted = false) {
     out = PassOut{};
     hipStream_t s = L.stream;
     const bool timing = flags_mask & 4u, counting = flags_mask & 2u;
@@ -478,7 +479,11 @@ rsh_status trace_pass(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, const
     out.N = N;
     if (N == 0) return RSH_OK;
     const uint32_t* sorted_values = all_unique ? u32p(L.values) : u32p(L.tvalues);
-    if (cfg.algorithm == RSH_CRSH && all_unique) {
+    if (presorted) {
+        // camera rays created in key order, straight into the hierarchy
+        // records (k_create_primary_sorted): nothing to sort or gather
+        out.C = N;
+    } else if (cfg.algorithm == RSH_CRSH && all_unique) {
         out.C = N;
         StageTimer t(L, ST_SORT, timing);
         LANE_OK(run_radix_sort(u32p(L.keys), u32p(L.values), N, sort_bits ? sort_bits : key_bits(cfg), u32p(L.skeys),
@@ -506,9 +511,10 @@ rsh_status trace_pass(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, const
     level_sizes(N, cfg.leaf, cfg.branch, cfg.depth, counts, offs);
     {
         StageTimer t(L, ST_HIER, timing);
-        launch_gather(rays, sorted_values, N, kind == 1 ? shadow_pos : nullptr, kind == 1 ? tmax : nullptr, e->W,
-                      e->Lmax, kind == 1 && shadow_pos && cfg.soft_shadows ? 4u : 0u, f4p(L.rec0),
-                      f4p(L.rec1), s);
+        if (!presorted)
+            launch_gather(rays, sorted_values, N, kind == 1 ? shadow_pos : nullptr, kind == 1 ? tmax : nullptr, e->W,
+                          e->Lmax, kind == 1 && shadow_pos && cfg.soft_shadows ? 4u : 0u, f4p(L.rec0),
+                          f4p(L.rec1), s);
         launch_hierarchy(f4p(L.rec0), f4p(L.rec1), nullptr, nullptr, N, cfg.leaf, cfg.branch, cfg.depth,
                          cfg.initial_radius, cfg.initial_spread, cfg.cone_mode, f4p(L.nodes), L.aux.as<float2>(),
                          counts, offs, s);
@@ -955,13 +961,19 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
     auto run_a = [&]() -> rsh_status {
         Lane& L = A;
         launch_fill_u32(u32p(e->pixels), 0u, P, sa);
+        // CRSH camera rays come out of creation in key order when every key
+        // is unique (one view, or the view id in the key)
+        const bool presort = cfg.algorithm == RSH_CRSH && (nv == 1 || cfg.segment_shift != 0);
         {
             StageTimer t(A, ST_CREATE, timing);
-            launch_create_primary(V, P, kc, rays_b, u32p(A.flags), u32p(A.keys), u32p(A.values), sa);
+            if (presort)
+                launch_create_primary_sorted(V, P, rays_b, f4p(A.rec0), f4p(A.rec1), sa);
+            else
+                launch_create_primary(V, P, kc, rays_b, u32p(A.flags), u32p(A.keys), u32p(A.values), sa);
         }
         PassOut po;
         rsh_status r = trace_pass(e, A, cfg, rays_b, P, 0, nullptr, fl, po, 0, e->closest[0], nullptr, true,
-                                  primary_bits);
+                                  primary_bits, presort);
         if (r != RSH_OK) return r;
         if ((r = record(e, A, st, 0, 0, P, po, fl, e->closest[0])) != RSH_OK) return r;
         {

@@ -66,6 +66,14 @@ __device__ __forceinline__ uint32_t primary_key(uint32_t x, uint32_t y, uint32_t
     return k;
 }
 
+__device__ __forceinline__ v3 primary_dir(const ViewDesc& vd, uint32_t x, uint32_t y) {
+    const float a = (float)x / (float)vd.w - 0.5f;
+    const float c = (float)y / (float)vd.h - 0.5f;
+    const v3 D = mk(vd.dir[0], vd.dir[1], vd.dir[2]), R = mk(vd.right[0], vd.right[1], vd.right[2]);
+    const v3 U = mk(vd.up[0], vd.up[1], vd.up[2]);
+    return normalize((D + R * a) + U * c);
+}
+
 __global__ void __launch_bounds__(256) k_create_primary(Views V, uint32_t P, KeyCfg kc, float* rays, uint32_t* flags,
                                                         uint32_t* keys, uint32_t* values) {
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
@@ -73,16 +81,48 @@ __global__ void __launch_bounds__(256) k_create_primary(Views V, uint32_t P, Key
         const ViewDesc& vd = V.n ? V.d[v] : V.one;
         const uint32_t q = p - vd.off;
         const uint32_t x = q % vd.w, y = q / vd.w;
-        const float a = (float)x / (float)vd.w - 0.5f;
-        const float c = (float)y / (float)vd.h - 0.5f;
-        const v3 eye = mk(vd.eye[0], vd.eye[1], vd.eye[2]);
-        const v3 D = mk(vd.dir[0], vd.dir[1], vd.dir[2]), R = mk(vd.right[0], vd.right[1], vd.right[2]);
-        const v3 U = mk(vd.up[0], vd.up[1], vd.up[2]);
-        v3 d = normalize((D + R * a) + U * c);
-        store_ray(rays, p, eye, d);
+        store_ray(rays, p, mk(vd.eye[0], vd.eye[1], vd.eye[2]), primary_dir(vd, x, y));
         keys[p] = primary_key(x, y, vd.w, v, kc.h.segment_shift);
         values[p] = p;
         flags[p] = 0;
+    }
+}
+
+// The camera rays already in key order.  The keys are unique, so sorted
+// position q holds, within its view's block (views in offset order, which is
+// view-id order), the q-th pixel in tile-major, Morton-inside order.  Tile
+// rows hold 8 w pixels (the last one h_t w); in a tile row every tile left of
+// tile tx is 8 wide, so it holds 8 h_t; inside a tile of w_t x h_t valid
+// pixels the position is the rank among the Morton indices m with
+// (mx, my) < (w_t, h_t).  Writes the slot-order rays (the surface stage reads
+// them) and the sorted hierarchy records of k_gather: rec0 = (o, slot bits),
+// rec1 = (d, 0).
+__global__ void __launch_bounds__(256) k_create_primary_sorted(Views V, uint32_t P, float* rays, float4* rec0,
+                                                               float4* rec1) {
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < P; q += gridDim.x * blockDim.x) {
+        const uint32_t v = view_of(V, q);
+        const ViewDesc& vd = V.n ? V.d[v] : V.one;
+        const uint32_t w = vd.w, h = vd.h, r = q - vd.off;
+        const uint32_t ty = r / (8u * w), r1 = r - ty * 8u * w;
+        const uint32_t ht = min(8u, h - 8u * ty);
+        const uint32_t tx = r1 / (8u * ht), r2 = r1 - tx * 8u * ht;
+        const uint32_t wt = min(8u, w - 8u * tx);
+        uint32_t m = r2;
+        if (wt != 8u || ht != 8u) {
+            uint32_t seen = 0;
+            for (m = 0; m < 64u; m++) {
+                const uint32_t mx = (m & 1u) | ((m >> 1) & 2u) | ((m >> 2) & 4u);
+                const uint32_t my = ((m >> 1) & 1u) | ((m >> 2) & 2u) | ((m >> 3) & 4u);
+                if (mx < wt && my < ht && seen++ == r2) break;
+            }
+        }
+        const uint32_t x = 8u * tx + ((m & 1u) | ((m >> 1) & 2u) | ((m >> 2) & 4u));
+        const uint32_t y = 8u * ty + (((m >> 1) & 1u) | ((m >> 2) & 2u) | ((m >> 3) & 4u));
+        const uint32_t p = vd.off + x + y * w;
+        const v3 o = mk(vd.eye[0], vd.eye[1], vd.eye[2]), d = primary_dir(vd, x, y);
+        store_ray(rays, p, o, d);
+        rec0[q] = f4(o, __uint_as_float(p));
+        rec1[q] = f4(d, 0.0f);
     }
 }
 
@@ -477,6 +517,9 @@ void launch_query_keys(const float* rays, const uint32_t* segs, uint32_t n, KeyC
 void launch_create_primary(const Views& V, uint32_t P, KeyCfg kc, float* rays, uint32_t* flags, uint32_t* keys,
                            uint32_t* values, hipStream_t s) {
     k_create_primary<<<grid_for(P, 256, 65536), 256, 0, s>>>(V, P, kc, rays, flags, keys, values);
+}
+void launch_create_primary_sorted(const Views& V, uint32_t P, float* rays, float4* rec0, float4* rec1, hipStream_t s) {
+    k_create_primary_sorted<<<grid_for(P, 256, 65536), 256, 0, s>>>(V, P, rays, rec0, rec1);
 }
 void launch_create_shadow(const float4* gpos, const float4* gnrm, uint32_t W, uint32_t H, uint32_t Lmax,
                           const float4* light_pos, KeyCfg kc, const Views& V, uint32_t soft, float* rays,

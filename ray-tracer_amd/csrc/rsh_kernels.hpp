@@ -186,6 +186,7 @@ void launch_create_shadow(const float4* gpos, const float4* gnrm, uint32_t W, ui
 void launch_create_reflection(const float4* gpos, const float4* gnrm, const float4* gdif, uint32_t W, uint32_t H,
                               uint32_t Lmax, const Views& V, uint32_t gate, uint32_t mark_gaps, KeyCfg kc,
                               float* rays, uint32_t* flags, uint32_t* keys, uint32_t* values, hipStream_t s);
+void launch_create_primary_sorted(const Views& V, uint32_t P, float* rays, float4* rec0, float4* rec1, hipStream_t s);
 void launch_create_refraction(const float4* gpos, const float4* gnrm, uint32_t W, uint32_t H, uint32_t Lmax,
                               const Views& V, KeyCfg kc, float* rays, uint32_t* flags, uint32_t* keys,
                               uint32_t* values, hipStream_t s);

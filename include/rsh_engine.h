@@ -93,6 +93,11 @@ void RayCompressionWrapper(unsigned int* trimmedRayIndexKeysArray,
                            unsigned int* chunkBasesArray, unsigned int* chunkSizesArray,
                            unsigned int* chunkIndexKeysArray, unsigned int* chunkIndexValuesArray,
                            unsigned int* chunkTotal);
+/* No counterpart in the reference: the sorting stage on the low `bits` bits
+   of the keys (stable LSD radix sort, 8- or 9-bit digits), as the engine sorts
+   a frame's shadow and reflection passes (27 of their 29 key bits vary). */
+void rsh_sort_pairs(const unsigned int* keys, const unsigned int* values, const unsigned int n,
+                    const unsigned int bits, unsigned int* out_keys, unsigned int* out_values);
 /* RayTracer.cu:2385 RaySortingWrapper (CUB SortPairs replaced by the library's radix sort) */
 void RaySortingWrapper(unsigned int* chunkIndexKeysArray, unsigned int* chunkIndexValuesArray,
                        const unsigned int chunkTotal, unsigned int* sortedChunkIndexKeysArray,

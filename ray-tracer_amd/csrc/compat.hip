@@ -238,6 +238,20 @@ void RaySortingWrapper(unsigned int* ckeys, unsigned int* cvalues, const unsigne
     check(run_radix_sort(ckeys, cvalues, Cn, 32, skeys, svalues, c.sc, c.stream), "RaySortingWrapper");
 }
 
+// The sorting stage on the low `bits` key bits (stable; 8- or 9-bit digits),
+// as the engine runs it for a frame's shadow and reflection passes.
+void rsh_sort_pairs(const unsigned int* keys, const unsigned int* values, const unsigned int n,
+                    const unsigned int bits, unsigned int* out_keys, unsigned int* out_values) {
+    Compat& c = C();
+    if (bits == 0 || bits > 32) {
+        raise(RSH_INVALID_ARG, "rsh_sort_pairs: bits must be 1..32");
+        return;
+    }
+    if (!ensure_scratch(n)) return;
+    if (check(run_radix_sort(keys, values, n, bits, out_keys, out_values, c.sc, c.stream), "rsh_sort_pairs"))
+        check(hipStreamSynchronize(c.stream), "rsh_sort_pairs");
+}
+
 void RayDecompressionWrapper(unsigned int* bases, unsigned int* sizes, unsigned int*, unsigned int* scvalues,
                              unsigned int* tkeys, unsigned int* tvalues, unsigned int* skeleton, unsigned int* scan,
                              const unsigned int Cn, unsigned int* rkeys, unsigned int* rvalues) {

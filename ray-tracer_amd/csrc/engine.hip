@@ -833,6 +833,9 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
     Lane& B = e->lane[1];
     hipStream_t sa = A.stream, sb = B.stream;
     const uint32_t W = width, H = height, Lm = e->L, P = W * H, S = W * H * Lm, Ssh = S * smult;
+    // every key of a shadow or reflection pass has the same type (the top two
+    // bits of an octahedral key), so their sorts skip those bits
+    const uint32_t pass_bits = cfg.key_mode == RSH_KEY_OCT && cfg.segment_shift == 0 ? key_bits(cfg) - 2 : 0;
     const KeyCfg kc = key_cfg(cfg);
     const SceneView sv = scene_view(e);
     const bool timing = fl & 4u;
@@ -874,7 +877,8 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
                                  u32p(B.flags), u32p(B.keys), u32p(B.values), sb);
         }
         PassOut po;
-        rsh_status r = trace_pass(e, B, cfg, rays_b, Ssh, 1, G.pos, fl, po, idx_shadow(k), e->closest[0]);
+        rsh_status r = trace_pass(e, B, cfg, rays_b, Ssh, 1, G.pos, fl, po, idx_shadow(k), e->closest[0], nullptr,
+                                  false, pass_bits);
         if (r != RSH_OK) return r;
         if ((r = record(e, B, st, idx_shadow(k), 1, Ssh, po, fl, e->closest[0])) != RSH_OK) return r;
         LANE_OK(hipEventRecord(evS[k], sb));
@@ -995,7 +999,9 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
                                          u32p(A.values), sa);
             }
             PassOut pr;
-            if ((r = trace_pass(e, A, cfg, rays_a, S, 0, nullptr, fl, pr, idx_refl(b), cur)) != RSH_OK) return r;
+            if ((r = trace_pass(e, A, cfg, rays_a, S, 0, nullptr, fl, pr, idx_refl(b), cur, nullptr, false,
+                                pass_bits)) != RSH_OK)
+                return r;
             if ((r = record(e, A, st, idx_refl(b), 2, S, pr, fl, cur)) != RSH_OK) return r;
             last_idx = std::max(last_idx, idx_refl(b));
             if ((r = shade(due)) != RSH_OK) return r;

@@ -622,10 +622,16 @@ __global__ void __launch_bounds__(256) k_unpack_expand(const uint32_t* soff, con
 }
 
 // --------------------------------------------------------- radix sorting --
+// RB-bit digits (RB = 8: 256 buckets, or 9: 512 buckets, which sorts a 27-bit
+// key in three passes instead of four); DPT = digits per thread of a
+// 256-thread block in the per-digit steps, each thread owning DPT
+// consecutive digits so that a block scan keeps digit order.
+template <uint32_t RB>
 __global__ void __launch_bounds__(256) k_radix_upsweep(const uint32_t* keys, uint64_t n, uint32_t shift,
                                                        uint32_t* hist, uint32_t ntiles) {
-    __shared__ uint32_t cnt[WAVES][256];
-    for (uint32_t i = threadIdx.x; i < WAVES * 256; i += blockDim.x) (&cnt[0][0])[i] = 0;
+    constexpr uint32_t NB = 1u << RB, MASK = NB - 1u;
+    __shared__ uint32_t cnt[WAVES][NB];
+    for (uint32_t i = threadIdx.x; i < WAVES * NB; i += blockDim.x) (&cnt[0][0])[i] = 0;
     __syncthreads();
     const uint32_t w = wave_id(), lane = lane_id();
     const uint64_t tb = (uint64_t)blockIdx.x * TILE + (uint64_t)w * 1024;
@@ -637,13 +643,14 @@ __global__ void __launch_bounds__(256) k_radix_upsweep(const uint32_t* keys, uin
     }
 #pragma unroll
     for (uint32_t r = 0; r < 16; r++)
-        if (tb + r * 64 + lane < n) atomicAdd(&cnt[w][(k[r] >> shift) & 0xFFu], 1u);
+        if (tb + r * 64 + lane < n) atomicAdd(&cnt[w][(k[r] >> shift) & MASK], 1u);
     __syncthreads();
-    const uint32_t d = threadIdx.x;
-    uint32_t s = 0;
+    for (uint32_t d = threadIdx.x; d < NB; d += blockDim.x) {
+        uint32_t s = 0;
 #pragma unroll
-    for (uint32_t k = 0; k < WAVES; k++) s += cnt[k][d];
-    hist[(size_t)d * ntiles + blockIdx.x] = s;
+        for (uint32_t q = 0; q < WAVES; q++) s += cnt[q][d];
+        hist[(size_t)d * ntiles + blockIdx.x] = s;
+    }
 }
 
 // One block per digit: exclusive scan of that digit's tile counts in place.
@@ -664,36 +671,48 @@ __global__ void __launch_bounds__(256) k_radix_rowscan(uint32_t* hist, uint32_t 
 
 // ROWSCAN: hist holds the upsweep's raw tile counts and each block sums its
 // digit rows itself (for small tile counts, in place of k_radix_rowscan).
-template <bool ROWSCAN>
+template <bool ROWSCAN, uint32_t RB>
 __global__ void __launch_bounds__(256) k_radix_downsweep(const uint32_t* keys, const uint32_t* vals, uint64_t n,
                                                          uint32_t shift, const uint32_t* hist, uint32_t ntiles,
                                                          const uint32_t* digit_totals, uint32_t* okeys,
                                                          uint32_t* ovals) {
-    __shared__ uint32_t whist[WAVES][256];
-    __shared__ uint32_t tile_start[256];
-    __shared__ uint32_t gbase[256];
+    constexpr uint32_t NB = 1u << RB, MASK = NB - 1u, DPT = NB / 256u;
+    __shared__ uint32_t whist[WAVES][NB];
+    __shared__ uint32_t tile_start[NB];
+    __shared__ uint32_t gbase[NB];
     __shared__ uint32_t sk[TILE];
     __shared__ uint32_t sv[TILE];
     __shared__ uint32_t lds[WAVES];
     const uint32_t tid = threadIdx.x, w = wave_id(), lane = lane_id();
-    for (uint32_t i = tid; i < WAVES * 256; i += blockDim.x) (&whist[0][0])[i] = 0;
+    for (uint32_t i = tid; i < WAVES * NB; i += blockDim.x) (&whist[0][0])[i] = 0;
     // global base of each digit for this tile
-    if (ROWSCAN) {
-        const uint32_t* row = hist + (size_t)tid * ntiles;
-        uint32_t before = 0, dt = 0;
-        for (uint32_t i = 0; i < ntiles; i++) {
-            const uint32_t c = row[i];
-            before += i < blockIdx.x ? c : 0u;
-            dt += c;
+    {
+        uint32_t dt[DPT], before[DPT], tsum = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < DPT; j++) {
+            const uint32_t d = tid * DPT + j;
+            if (ROWSCAN) {
+                const uint32_t* row = hist + (size_t)d * ntiles;
+                uint32_t b = 0, t = 0;
+                for (uint32_t i = 0; i < ntiles; i++) {
+                    const uint32_t c = row[i];
+                    b += i < blockIdx.x ? c : 0u;
+                    t += c;
+                }
+                dt[j] = t; before[j] = b;
+            } else {
+                dt[j] = digit_totals[d];
+                before[j] = hist[(size_t)d * ntiles + blockIdx.x];
+            }
+            tsum += dt[j];
         }
         uint32_t t;
-        const uint32_t pre = block_exclusive_add(dt, lds, &t);
-        gbase[tid] = pre + before;
-    } else {
-        uint32_t t;
-        uint32_t dt = digit_totals[tid];
-        uint32_t pre = block_exclusive_add(dt, lds, &t);
-        gbase[tid] = pre + hist[(size_t)tid * ntiles + blockIdx.x];
+        uint32_t pre = block_exclusive_add(tsum, lds, &t);
+#pragma unroll
+        for (uint32_t j = 0; j < DPT; j++) {
+            gbase[tid * DPT + j] = pre + before[j];
+            pre += dt[j];
+        }
     }
     __syncthreads();
     const uint64_t tb = (uint64_t)blockIdx.x * TILE + (uint64_t)w * 1024;
@@ -710,10 +729,10 @@ __global__ void __launch_bounds__(256) k_radix_downsweep(const uint32_t* keys, c
         uint64_t p = tb + r * 64 + lane;
         bool valid = p < n;
         uint32_t k = key[r];
-        uint32_t d = (k >> shift) & 0xFFu;
+        uint32_t d = (k >> shift) & MASK;
         uint64_t peers = __ballot(valid);
 #pragma unroll
-        for (uint32_t b = 0; b < 8; b++) {
+        for (uint32_t b = 0; b < RB; b++) {
             uint64_t bal = __ballot((d >> b) & 1u);
             peers &= ((d >> b) & 1u) ? bal : ~bal;
         }
@@ -727,20 +746,30 @@ __global__ void __launch_bounds__(256) k_radix_downsweep(const uint32_t* keys, c
     __syncthreads();
     // per digit: prefix over waves and tile-local start
     {
-        const uint32_t d = tid;
-        uint32_t run = 0;
+        uint32_t run[DPT], tsum = 0;
 #pragma unroll
-        for (uint32_t k = 0; k < WAVES; k++) { uint32_t c = whist[k][d]; whist[k][d] = run; run += c; }
+        for (uint32_t j = 0; j < DPT; j++) {
+            const uint32_t d = tid * DPT + j;
+            uint32_t r = 0;
+#pragma unroll
+            for (uint32_t q = 0; q < WAVES; q++) { uint32_t c = whist[q][d]; whist[q][d] = r; r += c; }
+            run[j] = r;
+            tsum += r;
+        }
         uint32_t t;
-        uint32_t st = block_exclusive_add(run, lds, &t);
-        tile_start[d] = st;
+        uint32_t st = block_exclusive_add(tsum, lds, &t);
+#pragma unroll
+        for (uint32_t j = 0; j < DPT; j++) {
+            tile_start[tid * DPT + j] = st;
+            st += run[j];
+        }
     }
     __syncthreads();
 #pragma unroll
     for (uint32_t r = 0; r < 16; r++) {
         uint64_t p = tb + r * 64 + lane;
         if (p < n) {
-            uint32_t d = (key[r] >> shift) & 0xFFu;
+            uint32_t d = (key[r] >> shift) & MASK;
             uint32_t lp = tile_start[d] + whist[w][d] + rank[r];
             sk[lp] = key[r];
             sv[lp] = val[r];
@@ -751,7 +780,7 @@ __global__ void __launch_bounds__(256) k_radix_downsweep(const uint32_t* keys, c
     const uint32_t valid_n = (uint32_t)((n - tile0) < TILE ? (n - tile0) : TILE);
     for (uint32_t i = tid; i < valid_n; i += blockDim.x) {
         uint32_t k = sk[i];
-        uint32_t d = (k >> shift) & 0xFFu;
+        uint32_t d = (k >> shift) & MASK;
         uint32_t g = gbase[d] + (i - tile_start[d]);
         okeys[g] = k;
         ovals[g] = sv[i];
@@ -806,8 +835,8 @@ hipError_t ScanScratch::reserve(uint64_t n) {
     hipError_t e;
     if ((e = tile_counts.reserve((MAX_TILES + 1) * sizeof(uint32_t))) != hipSuccess) return e;
     uint32_t nt = tiles_for(n) + 1;
-    if ((e = hist.reserve((size_t)256 * nt * sizeof(uint32_t))) != hipSuccess) return e;
-    if ((e = digit_totals.reserve(256 * sizeof(uint32_t))) != hipSuccess) return e;
+    if ((e = hist.reserve((size_t)512 * nt * sizeof(uint32_t))) != hipSuccess) return e;    // 9-bit digits
+    if ((e = digit_totals.reserve(512 * sizeof(uint32_t))) != hipSuccess) return e;
     if ((e = total.reserve(4 * sizeof(uint32_t))) != hipSuccess) return e;
     if ((e = ping_k.reserve(n * sizeof(uint32_t))) != hipSuccess) return e;
     if ((e = ping_v.reserve(n * sizeof(uint32_t))) != hipSuccess) return e;
@@ -932,12 +961,29 @@ hipError_t run_unpack(const uint32_t* bases, const uint32_t* sorted_cvalues, con
 // per digit; each block reads 256 x tiles counts).
 constexpr uint32_t ROWSCAN_IN_DOWNSWEEP = 64;
 
+template <uint32_t RB>
+void radix_pass(const uint32_t* ik, const uint32_t* iv, uint64_t n, uint32_t shift, uint32_t nt, uint32_t* ok,
+                uint32_t* ov, ScanScratch& sc, hipStream_t s) {
+    k_radix_upsweep<RB><<<nt, 256, 0, s>>>(ik, n, shift, sc.hist.as<uint32_t>(), nt);
+    if (nt <= ROWSCAN_IN_DOWNSWEEP) {
+        k_radix_downsweep<true, RB><<<nt, 256, 0, s>>>(ik, iv, n, shift, sc.hist.as<uint32_t>(), nt, nullptr, ok, ov);
+    } else {
+        k_radix_rowscan<<<1u << RB, 256, 0, s>>>(sc.hist.as<uint32_t>(), nt, sc.digit_totals.as<uint32_t>());
+        k_radix_downsweep<false, RB><<<nt, 256, 0, s>>>(ik, iv, n, shift, sc.hist.as<uint32_t>(), nt,
+                                                        sc.digit_totals.as<uint32_t>(), ok, ov);
+    }
+}
+
+// LSD radix sort of the low `bits` key bits: 8-bit digits, or 9-bit digits
+// where they need fewer passes (17-18 and 25-27 bits).
 hipError_t run_radix_sort(const uint32_t* keys, const uint32_t* values, uint64_t n, uint32_t bits,
                           uint32_t* out_keys, uint32_t* out_values, ScanScratch& sc, hipStream_t s) {
     if (n == 0) return hipSuccess;
     uint32_t nt = tiles_for(n);
     if (bits == 0) bits = 1;
-    uint32_t passes = (bits + 7) / 8;
+    if (bits > 32) bits = 32;
+    const uint32_t rb = (bits + 8) / 9 < (bits + 7) / 8 ? 9u : 8u;
+    const uint32_t passes = (bits + rb - 1) / rb;
     // Ping-pong so that the final pass lands in out_*.
     uint32_t* bk[2] = {sc.ping_k.as<uint32_t>(), out_keys};
     uint32_t* bv[2] = {sc.ping_v.as<uint32_t>(), out_values};
@@ -945,16 +991,8 @@ hipError_t run_radix_sort(const uint32_t* keys, const uint32_t* values, uint64_t
     const uint32_t* iv = values;
     for (uint32_t pass = 0; pass < passes; pass++) {
         uint32_t dst = ((passes - 1 - pass) & 1u) ? 0u : 1u;
-        uint32_t shift = 8 * pass;
-        k_radix_upsweep<<<nt, 256, 0, s>>>(ik, n, shift, sc.hist.as<uint32_t>(), nt);
-        if (nt <= ROWSCAN_IN_DOWNSWEEP) {
-            k_radix_downsweep<true><<<nt, 256, 0, s>>>(ik, iv, n, shift, sc.hist.as<uint32_t>(), nt, nullptr,
-                                                       bk[dst], bv[dst]);
-        } else {
-            k_radix_rowscan<<<256, 256, 0, s>>>(sc.hist.as<uint32_t>(), nt, sc.digit_totals.as<uint32_t>());
-            k_radix_downsweep<false><<<nt, 256, 0, s>>>(ik, iv, n, shift, sc.hist.as<uint32_t>(), nt,
-                                                        sc.digit_totals.as<uint32_t>(), bk[dst], bv[dst]);
-        }
+        if (rb == 9) radix_pass<9>(ik, iv, n, 9 * pass, nt, bk[dst], bv[dst], sc, s);
+        else radix_pass<8>(ik, iv, n, 8 * pass, nt, bk[dst], bv[dst], sc, s);
         ik = bk[dst];
         iv = bv[dst];
     }

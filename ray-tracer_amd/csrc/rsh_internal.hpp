@@ -54,8 +54,8 @@ struct DevBuf {
 // Scratch used by scans and the radix sort.
 struct ScanScratch {
     DevBuf tile_counts;   // uint32[MAX_TILES + 1]
-    DevBuf hist;          // uint32[256 * MAX_TILES]
-    DevBuf digit_totals;  // uint32[256]
+    DevBuf hist;          // uint32[512 * MAX_TILES] (9-bit radix digits)
+    DevBuf digit_totals;  // uint32[512]
     DevBuf total;         // uint32[4]
     DevBuf ping_k, ping_v;
     hipError_t reserve(uint64_t n);

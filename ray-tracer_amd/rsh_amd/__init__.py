@@ -84,6 +84,7 @@ COMPAT_SIGNATURES = {
     "RayTrimmingWrapper": [_P, _P, _U, _U, _U, _P, _P, _P, _P, PU],
     "RayCompressionWrapper": [_P, _P, _U, _P, _P, _P, _P, _P, _P, PU],
     "RaySortingWrapper": [_P, _P, _U, _P, _P],
+    "rsh_sort_pairs": [_P, _P, _U, _U, _P, _P],
     "RayDecompressionWrapper": [_P, _P, _P, _P, _P, _P, _P, _P, _U, _P, _P],
     "HierarchyCreationWrapper": [_P, _P, _P, _U, f32, f32, _P],
     "HierarchyTraversalWarmUpWrapper": [_P, _P, _P, _U, _U, _U, _U, _P, _P, _P, _P, PU, PU],

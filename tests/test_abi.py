@@ -28,7 +28,7 @@ def test_reference_wrapper_names_are_all_present():
     """The 18 stage wrappers and 15 binders of RayTracer.cu:2042-3301."""
     names = set(declared_functions("rsh_engine.h"))
     ref = {"TriangleUpdateWrapper", "BoundingSphereUpdateWrapper", "MemoryPreparationWrapper",
-           "ShadowRayCreationWrapper", "ReflectionRayCreationWrapper", "RefractionRayCreationWrapper", "RayTrimmingWrapper", "RayCompressionWrapper",
+           "ShadowRayCreationWrapper", "ReflectionRayCreationWrapper", "RefractionRayCreationWrapper", "rsh_sort_pairs", "RayTrimmingWrapper", "RayCompressionWrapper",
            "RaySortingWrapper", "RayDecompressionWrapper", "HierarchyCreationWrapper",
            "HierarchyTraversalWarmUpWrapper", "HierarchyTraversalWrapper", "ShadowRayPreparationWrapper",
            "ShadowRayIntersectionWrapper", "ShadowRayColoringWrapper", "ReflectionRayPreparationWrapper",

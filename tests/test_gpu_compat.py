@@ -289,6 +289,27 @@ def test_sort_large_properties(rsh):
     np.testing.assert_array_equal(sv, vals[order])
 
 
+@pytest.mark.parametrize("n,bits", [(1920 * 1080, 27), (1920 * 1080, 24), (300_000, 27), (5000, 17), (77, 9),
+                                    (100_000, 32)])
+def test_sort_pairs_bits(rsh, n, bits):
+    """The sort on the low `bits` bits (9-bit digits for 17-18 and 25-27 bits):
+    stable in those bits, the bits above carried along unchanged (a frame's
+    pass keys share their type bits above the sorted ones)."""
+    L = rsh.lib()
+    rng = np.random.default_rng(bits * 7 + n % 97)
+    low = rng.integers(0, 1 << bits, n, dtype=np.uint64)
+    top = (np.uint64(2) << np.uint64(bits)) if bits <= 29 else np.uint64(0)
+    keys = ((low | top) & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    vals = np.arange(n, dtype=np.uint32)
+    dk, dv = rsh.DeviceArray.from_host(keys), rsh.DeviceArray.from_host(vals)
+    ok_, ov = rsh.DeviceArray((n,), np.uint32), rsh.DeviceArray((n,), np.uint32)
+    L.rsh_sort_pairs(dk.ptr, dv.ptr, n, bits, ok_.ptr, ov.ptr)
+    assert L.rsh_compat_status() == 0, L.rsh_compat_error()
+    order = np.argsort(keys & np.uint32((1 << bits) - 1 if bits < 32 else 0xFFFFFFFF), kind="stable")
+    np.testing.assert_array_equal(ok_.download(), keys[order])
+    np.testing.assert_array_equal(ov.download(), vals[order])
+
+
 def test_capacity_guard(rsh, setup):
     """A traversal that would overrun hierarchyHitMemoryTotal reports RSH_CAPACITY."""
     L = rsh.lib()

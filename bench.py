@@ -15,7 +15,12 @@ Beside the headline it measures, on the same GPU:
               stage, view id in the key) eager and as a replayed HIP graph;
   "serving"   Poisson-arriving ray-query requests (1k-4M rays, primary /
               shadow / diffuse 40/30/30) against 4 resident scenes at 70% of
-              the measured capacity: p50/p99 latency and Mrays/s.
+              the measured capacity: p50/p99 latency and Mrays/s;
+  "soft_shadows", "object_update", "rah_vs_crsh"
+              the atrium frame with 16 shadow rays per slot, with every
+              object moving each frame, and with the unsorted hierarchy;
+  "office_obj", "cornell_obj" (with --office / --cornell)
+              the reference's thesis scenes read from its model files.
 
 Prints one JSON line.  "roofline" describes the dominant kernel, the fused
 traversal (k_traverse64, one launch per pass): its launch durations come from

@@ -45,6 +45,13 @@ struct Lane {
     DevBuf flags, keys, values, tvalues, bases, ckeys, cvalues, skeys, svalues;
     DevBuf rvalues, soff, rec0, rec1, nodes, aux, counters, dcounts;
     uint32_t* hstage = nullptr;        // page-locked host words for the per-pass read-backs
+    uint32_t* hsize = nullptr;         // coherent page-locked words the pack scan writes: seq, N, C
+    uint32_t* hsize_dev = nullptr;     // their device address
+    uint32_t hseq = 0;                 // sequence number of the last pack
+    unsigned long long* htable = nullptr;      // coherent page-locked: seq, then the item table
+    unsigned long long* htable_dev = nullptr;  // its device address
+    uint32_t tseq = 0;                         // sequence number of the last item table
+    DevBuf w_done;                             // block counter of k_item_table (kept at zero)
     hipEvent_t rb_ev = nullptr;        // marks a read-back's completion (polled)
     bool table_clean = false;          // w_table is zero (the item scan clears it after each read-back)
     // work list of the 64-ray traversal kernel
@@ -204,6 +211,25 @@ hipError_t readback_wait(Lane& L) {
     }
 }
 
+// Wait for a kernel's sequence word `seq` in coherent host memory (the pack
+// scan's sizes, the item table).  Every 1024 polls the stream is queried: an
+// error ends the wait, and so does an idle stream without the word (then the
+// values never came: an internal error).
+template <class T>
+hipError_t wait_word(Lane& L, const T* word, uint32_t seq) {
+    for (uint32_t spin = 1;; spin++) {
+        if (__atomic_load_n(word, __ATOMIC_ACQUIRE) == (T)seq) return hipSuccess;
+        if ((spin & 1023u) == 0) {
+            const hipError_t r = hipStreamQuery(L.stream);
+            if (r == hipSuccess)
+                return __atomic_load_n(word, __ATOMIC_ACQUIRE) == (T)seq ? hipSuccess : hipErrorUnknown;
+            if (r != hipErrorNotReady) return r;
+        }
+        __builtin_ia32_pause();
+    }
+}
+hipError_t wait_sizes(Lane& L, uint32_t seq) { return wait_word(L, L.hsize, seq); }
+
 // Lane 1's stream and page-locked words, created on first use; lane 0 runs
 // on the engine's stream.
 rsh_status ensure_lanes(rsh_engine* e) {
@@ -215,6 +241,21 @@ rsh_status ensure_lanes(rsh_engine* e) {
     }
     for (Lane& l : e->lane) {
         if (!l.hstage) HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&l.hstage), HSTAGE_WORDS * 4, hipHostMallocDefault));
+        if (!l.hsize) {
+            HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&l.hsize), 64, hipHostMallocCoherent | hipHostMallocMapped));
+            std::memset(l.hsize, 0, 64);
+            l.hseq = 0;
+            HIP_OK(hipHostGetDevicePointer(reinterpret_cast<void**>(&l.hsize_dev), l.hsize, 0));
+        }
+        if (!l.htable) {
+            const size_t bytes = (1 + ITEM_TABLE_WORDS) * sizeof(unsigned long long);
+            HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&l.htable), bytes, hipHostMallocCoherent | hipHostMallocMapped));
+            std::memset(l.htable, 0, bytes);
+            l.tseq = 0;
+            HIP_OK(hipHostGetDevicePointer(reinterpret_cast<void**>(&l.htable_dev), l.htable, 0));
+            HIP_OK(l.w_done.reserve(64));
+            launch_fill_u32(u32p(l.w_done), 0u, 16, l.stream);     // stream-ordered before its first use
+        }
         if (!l.rb_ev) HIP_OK(hipEventCreateWithFlags(&l.rb_ev, hipEventDisableTiming));
     }
     while (e->sync_ev.size() < 2 * (RSH_MAX_PASSES + 2)) {
@@ -368,14 +409,24 @@ rsh_status build_items(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, Trav
         unsigned long long* table = L.w_table.as<unsigned long long>();
         if (!L.table_clean) launch_fill_u64(table, 0ull, ITEM_TABLE_WORDS, s);   // new, or a pass that failed
         L.table_clean = false;
-        launch_item_table(cand, p.aux + p.level_offset[1], ntop, e->NC, np_scan, table, s);
-        LANE_OK(hipMemcpyAsync(L.hstage + 2, table, sizeof(ht), hipMemcpyDeviceToHost, s));
+        // the tabled case reads the table from the last k_item_table block's
+        // host copy; otherwise the item scan's total is needed too
+        const bool publish = tabled && ntop > 0;
+        launch_item_table(cand, p.aux + p.level_offset[1], ntop, e->NC, np_scan, table, s,
+                          publish ? u32p(L.w_done) : nullptr, publish ? L.htable_dev : nullptr,
+                          publish ? ++L.tseq : 0u);
+        if (!publish) LANE_OK(hipMemcpyAsync(L.hstage + 2, table, sizeof(ht), hipMemcpyDeviceToHost, s));
         if (!tabled) {
             LANE_OK(run_item_scan(cand, ntop, chunk, e->NC, nitems, first, L.sc, s, nullptr, 0, np_scan, u32p(L.w_off)));
             LANE_OK(hipMemcpyAsync(L.hstage + 1, first + ntop, 4, hipMemcpyDeviceToHost, s));
         }
-        LANE_OK(readback_wait(L));
-        std::memcpy(ht, L.hstage + 2, sizeof(ht));
+        if (publish) {
+            LANE_OK(wait_word(L, reinterpret_cast<uint64_t*>(L.htable), L.tseq));
+            for (uint32_t i = 0; i < ITEM_TABLE_WORDS; i++) ht[i] = __atomic_load_n(L.htable + 1 + i, __ATOMIC_RELAXED);
+        } else {
+            LANE_OK(readback_wait(L));
+            std::memcpy(ht, L.hstage + 2, sizeof(ht));
+        }
         if (p.algorithm != 1) tot[0] = (uint32_t)ht[2 * ITEM_TABLE + 1];
         if (!tabled) {
             tot[1] = L.hstage[1];
@@ -455,12 +506,16 @@ rsh_status trace_pass(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, const
         launch_fill_u64(u64p(closest), cfg.hit_mode == 0 ? MISS64 : (unsigned long long)MISS32, S, s);
     } else {
         // trimming and compression in one walk over the slots (the compress
-        // stage time is part of the trim stage's)
+        // stage time is part of the trim stage's).  Eager passes take N and C
+        // from the pack scan's host words, published before the chunk writes,
+        // so the next launches are issued while those run.
         StageTimer t(L, ST_TRIM, timing);
         const unsigned long long miss = cfg.hit_mode == 0 ? MISS64 : (unsigned long long)MISS32;
+        const bool poll = !e->replaying && S > 0;
         LANE_OK(run_pack(u32p(L.flags), u32p(L.keys), u32p(L.values), S, u32p(L.tvalues), u32p(L.bases),
                         u32p(L.ckeys), u32p(L.cvalues), dc, kind == 0 ? u64p(closest) : nullptr, miss,
-                        kind == 1 ? u32p(e->shadow) : nullptr, L.sc, s));
+                        kind == 1 ? u32p(e->shadow) : nullptr, L.sc, s, poll ? L.hsize_dev : nullptr,
+                        poll ? ++L.hseq : 0u));
     }
     uint32_t hc[2] = {0, 0};
     if (e->replaying) {
@@ -469,10 +524,12 @@ rsh_status trace_pass(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, const
     } else if (all_unique) {
         hc[0] = hc[1] = S;
         if (e->recording) { L.plan.emplace_back(); L.plan.back().N = S; L.plan.back().C = S; }
+    } else if (S == 0) {
+        if (e->recording) { L.plan.emplace_back(); L.plan.back().N = 0; L.plan.back().C = 0; }
     } else {
-        LANE_OK(hipMemcpyAsync(L.hstage, dc, 8, hipMemcpyDeviceToHost, s));
-        LANE_OK(readback_wait(L));
-        hc[0] = L.hstage[0]; hc[1] = L.hstage[1];
+        LANE_OK(wait_sizes(L, L.hseq));
+        hc[0] = __atomic_load_n(L.hsize + 1, __ATOMIC_RELAXED);
+        hc[1] = __atomic_load_n(L.hsize + 2, __ATOMIC_RELAXED);
         if (e->recording) { L.plan.emplace_back(); L.plan.back().N = hc[0]; L.plan.back().C = hc[1]; }
     }
     const uint32_t N = hc[0];
@@ -631,6 +688,9 @@ void rsh_engine_destroy(rsh_engine* e) {
         for (DevBuf* b : L.bufs()) b->release();
         L.sc.release();
         if (L.hstage) (void)hipHostFree(L.hstage);
+        if (L.hsize) (void)hipHostFree(L.hsize);
+        if (L.htable) (void)hipHostFree(L.htable);
+        L.w_done.release();
         if (L.rb_ev) (void)hipEventDestroy(L.rb_ev);
         for (hipEvent_t ev : L.ev_pool) (void)hipEventDestroy(ev);
         if (L.own_stream && L.stream) (void)hipStreamDestroy(L.stream);

@@ -401,7 +401,8 @@ __global__ void __launch_bounds__(256) k_pack_tiles(const uint32_t* flags, const
 // Single block: meta[tile] becomes (live offset, chunk offset, carried key,
 // carried key valid); totals[0] = N live rays, totals[1] = C chunks,
 // bases[C] = N.
-__global__ void __launch_bounds__(1024) k_pack_scan(uint4* meta, uint32_t ntiles, uint32_t* totals, uint32_t* bases) {
+__global__ void __launch_bounds__(1024) k_pack_scan(uint4* meta, uint32_t ntiles, uint32_t* totals, uint32_t* bases,
+                                                     uint32_t* host, uint32_t seq) {
     __shared__ LastKey lk[16];
     __shared__ uint32_t red[16];
     constexpr uint32_t PER = MAX_TILES / 1024;      // tiles per thread, held in registers
@@ -440,7 +441,16 @@ __global__ void __launch_bounds__(1024) k_pack_scan(uint4* meta, uint32_t ntiles
         }
         lo += m[i].x;
     }
-    if (threadIdx.x == 0) { totals[0] = ntot; totals[1] = ctot; bases[ctot] = ntot; }
+    if (threadIdx.x == 0) {
+        totals[0] = ntot; totals[1] = ctot; bases[ctot] = ntot;
+        if (host) {
+            // the sizes, then the sequence word the host polls for
+            host[1] = ntot;
+            host[2] = ctot;
+            __threadfence_system();
+            __hip_atomic_store(host, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+    }
 }
 
 __global__ void __launch_bounds__(256) k_pack_write(const uint32_t* flags, const uint32_t* keys,
@@ -927,9 +937,9 @@ hipError_t run_compress(const uint32_t* keys, uint64_t n, uint32_t* bases, uint3
 hipError_t run_pack(const uint32_t* flags, const uint32_t* keys, const uint32_t* values, uint64_t n,
                     uint32_t* tvalues, uint32_t* bases, uint32_t* ckeys, uint32_t* cvalues, uint32_t* d_totals,
                     unsigned long long* clear64, unsigned long long miss, uint32_t* clear32, ScanScratch& sc,
-                    hipStream_t s) {
+                    hipStream_t s, uint32_t* host, uint32_t seq) {
     const uint32_t nt = tiles_for(n);
-    if (nt > MAX_TILES) return hipErrorInvalidValue;
+    if (nt > MAX_TILES || (nt == 0 && host)) return hipErrorInvalidValue;
     if (nt == 0) {
         launch_fill_u32(d_totals, 0, 2, s);
         launch_fill_u32(bases, 0, 1, s);
@@ -937,7 +947,7 @@ hipError_t run_pack(const uint32_t* flags, const uint32_t* keys, const uint32_t*
     }
     uint4* meta = sc.hist.as<uint4>();
     k_pack_tiles<<<nt, 256, 0, s>>>(flags, keys, n, meta);
-    k_pack_scan<<<1, 1024, 0, s>>>(meta, nt, d_totals, bases);
+    k_pack_scan<<<1, 1024, 0, s>>>(meta, nt, d_totals, bases, host, seq);
     k_pack_write<<<nt, 256, 0, s>>>(flags, keys, values, n, meta, tvalues, bases, ckeys, cvalues, clear64, miss,
                                     clear32);
     return hipGetLastError();

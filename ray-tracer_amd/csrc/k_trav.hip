@@ -409,8 +409,12 @@ __global__ void __launch_bounds__(256) k_item_count(const uint32_t* cand, uint32
 // lane c - 1 sums chunk sizes c and 2c in registers; one read-back of the
 // table sizes a pass and picks its traversal mode.
 constexpr uint32_t TABLE_SLICE = 256;
+// host (if set, device-visible coherent host memory): the block that finishes
+// last (counted in *done, which it resets) copies the table to host[1..] and
+// then writes `seq` to host[0], so the host reads the table without a copy.
 __global__ void __launch_bounds__(64) k_item_table(const uint32_t* cand, const float2* top_aux, uint32_t ntop,
-                                                   unsigned long long* table, bool fast, const uint32_t* npass) {
+                                                   unsigned long long* table, bool fast, const uint32_t* npass,
+                                                   uint32_t* done, unsigned long long* host, uint32_t seq) {
     __shared__ uint4 s_cand4[TABLE_SLICE / 4];
     uint32_t* s_cand = reinterpret_cast<uint32_t*>(s_cand4);
     const uint32_t lane = threadIdx.x & 63u, c = lane + 1u;
@@ -454,6 +458,23 @@ __global__ void __launch_bounds__(64) k_item_table(const uint32_t* cand, const f
         for (uint32_t t = t0 + lane; t < t1; t += 64u) np += npass[t];
         for (int d = 32; d > 0; d >>= 1) np += __shfl_xor(np, d, 64);
         if (lane == 0 && np) atomicAdd(&table[2 * ITEM_TABLE + 1], np);
+    }
+    if (host) {
+        __threadfence();
+        uint32_t prev = 0;
+        if (lane == 0) prev = atomicAdd(done, 1u);
+        prev = __shfl(prev, 0, 64);
+        if (prev == gridDim.x - 1u) {
+            __threadfence();
+            for (uint32_t i = lane; i < ITEM_TABLE_WORDS; i += 64u)
+                host[1 + i] = __hip_atomic_load(&table[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __threadfence_system();
+            __builtin_amdgcn_wave_barrier();
+            if (lane == 0) {
+                *done = 0u;
+                __hip_atomic_store(host, (unsigned long long)seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+        }
     }
 }
 
@@ -1197,12 +1218,13 @@ void launch_cull_list(const TravParams& p, uint4* obj_ent, uint32_t* obj_cum, un
 }
 
 void launch_item_table(const uint32_t* cand, const float2* top_aux, uint32_t ntop, uint32_t max_cand,
-                       const uint32_t* npass, unsigned long long* table, hipStream_t s) {
+                       const uint32_t* npass, unsigned long long* table, hipStream_t s, uint32_t* done,
+                       unsigned long long* host, uint32_t seq) {
     static_assert(ITEM_TABLE == 64, "one lane per chunk size");
     const bool fast = div_hi_ok((uint64_t)max_cand + 2 * ITEM_TABLE - 1, 2 * ITEM_TABLE);
     if (ntop)
         k_item_table<<<(ntop + TABLE_SLICE - 1) / TABLE_SLICE, 64, 0, s>>>(cand, top_aux, ntop, table, fast,
-                                                                           npass);
+                                                                           npass, done, host, seq);
 }
 
 void launch_item_count(const uint32_t* cand, uint32_t chunk, uint32_t ntop, uint32_t* nitems, hipStream_t s) {

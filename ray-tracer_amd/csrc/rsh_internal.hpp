@@ -87,11 +87,13 @@ hipError_t run_compress(const uint32_t* keys, uint64_t n, uint32_t* bases, uint3
 // (first live position; bases[C] = N), ckeys, cvalues (chunk index).
 // d_totals[0] = N, d_totals[1] = C.  bases holds n + 1 entries.
 // clear64 (if set) receives `miss` in every slot, clear32 (if set) 0: the pass's
-// per-slot results start cleared.
+// per-slot results start cleared.  host (if set, device-visible coherent
+// host memory) receives N and C in host[1], host[2] and then `seq` in host[0]
+// as soon as they are known, ahead of the chunk writes.
 hipError_t run_pack(const uint32_t* flags, const uint32_t* keys, const uint32_t* values, uint64_t n,
                     uint32_t* tvalues, uint32_t* bases, uint32_t* ckeys, uint32_t* cvalues, uint32_t* d_totals,
                     unsigned long long* clear64, unsigned long long miss, uint32_t* clear32, ScanScratch& sc,
-                    hipStream_t s);
+                    hipStream_t s, uint32_t* host = nullptr, uint32_t seq = 0);
 // Exclusive scan of the work-item counts ceil(cand[t] / chunk), t < ntop, with
 // first[ntop] = their total (counts is scratch of ntop + 1 entries); max_cand
 // bounds every cand entry (the scene's cluster count).  zero[0, zero_n) is

@@ -166,9 +166,11 @@ constexpr uint32_t ITEM_TABLE = 64;
 constexpr float WIDE_TAN = 0.1f;
 // max_cand: an upper bound of every cand entry (the scene's cluster count);
 // table: 2 ITEM_TABLE + 2 entries, zeroed beforehand; npass (summed into the
-// last entry) may be null
+// last entry) may be null.  host / done / seq: see k_item_table (the table
+// published to host memory by the last block; done a zeroed device word).
 void launch_item_table(const uint32_t* cand, const float2* top_aux, uint32_t ntop, uint32_t max_cand,
-                       const uint32_t* npass, unsigned long long* table, hipStream_t s);
+                       const uint32_t* npass, unsigned long long* table, hipStream_t s, uint32_t* done = nullptr,
+                       unsigned long long* host = nullptr, uint32_t seq = 0);
 constexpr uint32_t ITEM_TABLE_WORDS = 2 * ITEM_TABLE + 2;   // 64-bit entries
 void launch_item_count(const uint32_t* cand, uint32_t chunk, uint32_t ntop, uint32_t* nitems, hipStream_t s);
 // CRSH: the list pass and the work-item descriptors in one launch (needs

@@ -37,7 +37,8 @@ def random_views(rsh, seed, n, lo, hi):
     return views
 
 
-@pytest.mark.parametrize("variant", [dict(), dict(algorithm=1), dict(bounces=2, bounce_shadows=1), dict(leaf=4, branch=16)])
+@pytest.mark.parametrize("variant", [dict(), dict(algorithm=1), dict(bounces=2, bounce_shadows=1), dict(leaf=4, branch=16),
+                                     dict(soft_shadows=1), dict(bounces=2, bounce_shadows=1, soft_shadows=1)])
 def test_batch_matches_oracle(engine, oracle, rsh, variant):
     scene = rsh.scenes.config0_scene()
     views = random_views(rsh, 11, 7, 24, 80)
@@ -100,6 +101,27 @@ def test_graph_replay_matches_eager(engine, rsh):
     for i in range(3):
         engine.render(small, dataclasses.replace(cfg, width=32, height=32, segment_shift=0))   # eager work between
         engine.render_views(views, cfg)                               # eager batch: no reallocation
+        np.testing.assert_array_equal(engine.pixels(), eager)
+        st = engine.render_views(views, cfg, flags=engine.GRAPH)
+        assert st.graph_replayed == 1, i
+        np.testing.assert_array_equal(engine.pixels(), eager, err_msg="replay %d" % i)
+
+
+def test_graph_replay_soft_shadows(engine, rsh):
+    """Replays of a captured soft-shadow batch (16 shadow rays per slot) equal
+    its eager frame, with other eager work between the replays."""
+    scene = rsh.scenes.config0_scene()
+    views = random_views(rsh, 15, 5, 40, 96)
+    cfg = batch_cfg(rsh, scene, rsh.config.config0(64, 64), soft_shadows=1, bounces=2, bounce_shadows=1)
+    engine.load_scene(scene)
+    engine.render_views(views, cfg)
+    eager = engine.pixels().copy()
+    st = engine.render_views(views, cfg, flags=engine.GRAPH)
+    assert st.graph_replayed == 0
+    np.testing.assert_array_equal(engine.pixels(), eager)
+    for i in range(3):
+        engine.render(rsh.scenes.config0_camera(32, 32), rsh.config.config0(32, 32).with_scene_bounds(scene))
+        engine.render_views(views, cfg)
         np.testing.assert_array_equal(engine.pixels(), eager)
         st = engine.render_views(views, cfg, flags=engine.GRAPH)
         assert st.graph_replayed == 1, i

@@ -123,6 +123,15 @@ void oracle_triangle_sphere(const float* v0, const float* v1, const float* v2, f
 void oracle_cone0(const float* cone, const float* ray, uint32_t mode, float* out4);
 void oracle_coneN(const float* c1, const float* c2, float* out4);
 void oracle_sphere_union(const float* s1, const float* s2, float* out4);
+// Shading terms: RgbToInt (RT:100-107) and the Blinn-Phong colour of one
+// light (RT:1655-1702; scaled = the reflection variant's x scale, RT:1863-1866).
+uint32_t oracle_rgb_to_int(float r, float g, float b);
+void oracle_light_term(const float* light_pos4, const float* light_col4, const float* light_int2, const float* P,
+                       const float* N, const float* eye, const float* Kd4, const float* Ks4, float scale, int scaled,
+                       float* out3);
+// Key of camera ray (x, y) of a view w pixels wide: 8x8 tiles in row order,
+// Morton order inside a tile, the view id at segment_shift (0: none).
+uint32_t oracle_primary_key(uint32_t x, uint32_t y, uint32_t w, uint32_t view, uint32_t segment_shift);
 
 // Traversal candidate sets (tests pin the closed form against the literal
 // flag/scan transcription).  Returns the number of leaf pairs written as

@@ -120,6 +120,12 @@ def lib():
         L.oracle_cone0.argtypes = [PF, PF, u32, PF]
         L.oracle_coneN.argtypes = [PF, PF, PF]
         L.oracle_sphere_union.argtypes = [PF, PF, PF]
+        L.oracle_rgb_to_int.restype = u32
+        L.oracle_rgb_to_int.argtypes = [f32, f32, f32]
+        L.oracle_primary_key.restype = u32
+        L.oracle_primary_key.argtypes = [u32] * 5
+        L.oracle_light_term.restype = None
+        L.oracle_light_term.argtypes = [PF] * 8 + [f32, C.c_int, PF]
         L.oracle_render.restype = C.c_int
         L.oracle_render.argtypes = [C.POINTER(OScene), C.POINTER(OCamera), C.POINTER(OConfig), PF,
                                     C.POINTER(OFrameOut), C.c_int]

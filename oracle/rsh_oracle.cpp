@@ -8,10 +8,15 @@
 // Citations: RT = projects/CUDA-Engine/source/RayTracer.cu,
 //            MAIN = projects/CUDA-Engine/source/main.cpp.
 //
-// How each part is pinned (checks live in tests/test_oracle_*.py):
+// How each part is pinned (checks live in tests/test_oracle_*.py; each check
+// is an independent float64 / closed-form / brute-force model, and the
+// shading, surface and ray-creation checks also run mutated models to show
+// that a plausible mistake would fail them):
 //   keys            closed forms of the atan2(v, v) bins + numpy float32
 //                   evaluation of the same expressions; octahedral keys against
-//                   a numpy restatement and round-trip decode bounds
+//                   a numpy restatement and round-trip decode bounds; camera-ray
+//                   keys against an explicit bit interleave and the 4x2-leaf /
+//                   8x8-node grouping they promise
 //   trim            == stable copy_if of live slots
 //   compress/sort/  == run-length encoding / std::stable_sort /
 //   decompress         stable sort of (key, value) by key
@@ -23,8 +28,18 @@
 //   intersection    Moller-Trumbore known answers (centroid, edges, parallel,
 //                   behind origin); brute-force any/closest hit over all
 //                   triangles agrees where culling is conservative
-//   shading         single-light closed forms (N.L = 1, d = 0 -> att = 1/0.75)
-// Every function above has at least one of these checks; none is "unpinned".
+//   ray creation    shadow / reflection wrappers and the frame's camera,
+//                   shadow, reflection and bounce rays against float64
+//                   (tests/test_oracle_shading.py)
+//   surface, shading G-buffer against a float64 brute-force surface stage;
+//                   RgbToInt known answers (clamp, truncation, NaN, channel
+//                   order); the Blinn-Phong term against float64 and closed
+//                   forms (N.L = 1 with the eye at the light: att =
+//                   1/((0.75 + 0.0005 d) + 0.00005 d^2), specular 1); frame
+//                   pixels, including the packed-integer add x Ns of two
+//                   bounces with bounce shadows, against float64
+//   object update   UpdateVertex / UpdateBoundingSphere against numpy 4x4
+//                   products on random transforms
 #include "oracle_api.h"
 #include "oracle_math.h"
 
@@ -998,6 +1013,20 @@ void oracle_cone0(const float* cone, const float* ray, uint32_t mode, float* out
 }
 void oracle_coneN(const float* c1, const float* c2, float* out4) { st4(out4, coneN(ld4(c1), ld4(c2), 0)); }
 void oracle_sphere_union(const float* s1, const float* s2, float* out4) { st4(out4, sphere_union(ld4(s1), ld4(s2))); }
+uint32_t oracle_rgb_to_int(float r, float g, float b) { return rgb_to_int(r, g, b); }
+uint32_t oracle_primary_key(uint32_t x, uint32_t y, uint32_t w, uint32_t view, uint32_t segment_shift) {
+    return primary_key(x, y, w, view, segment_shift);
+}
+void oracle_light_term(const float* light_pos4, const float* light_col4, const float* light_int2, const float* P,
+                       const float* N, const float* eye, const float* Kd4, const float* Ks4, float scale, int scaled,
+                       float* out3) {
+    oracle_scene one{};
+    one.light_total = 1;
+    one.light_pos = light_pos4;
+    one.light_col = light_col4;
+    one.light_int = light_int2;
+    st3(out3, light_term(&one, 0, ld3(P), ld3(N), ld3(eye), ld4(Kd4), ld4(Ks4), scale, scaled != 0));
+}
 
 }  // extern "C"
 

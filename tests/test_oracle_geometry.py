@@ -117,6 +117,26 @@ def test_sphere_union_contains_both(oracle):
             assert np.linalg.norm(s[:3].astype(np.float64) - u[:3]) + s[3] <= u[3] * (1 + 1e-5) + 1e-4
 
 
+def test_sphere_union_closed_forms(oracle):
+    """CreateHierarchySphere (RT:427-446): a sphere holding the other is
+    returned unchanged; two points at distance D give their midpoint and
+    radius D/2, and two spheres of equal radius r the minimal D/2 + r."""
+    rng = np.random.default_rng(23)
+    for _ in range(500):
+        a, b = rng.uniform(0, 100, 3), rng.uniform(0, 100, 3)
+        D = np.linalg.norm(a - b)
+        u = union(oracle, "oracle_sphere_union", np.r_[a, 0.0], np.r_[b, 0.0])
+        np.testing.assert_allclose(u[:3], (a.astype(np.float32) + b.astype(np.float32)) / 2, rtol=1e-6, atol=1e-5)
+        assert u[3] == pytest.approx(D / 2, rel=1e-5, abs=1e-5)
+        r = rng.uniform(0.1, 10)
+        u = union(oracle, "oracle_sphere_union", np.r_[a, r], np.r_[b, r])
+        assert u[3] == pytest.approx(D / 2 + r, rel=1e-5, abs=1e-5)
+        big = np.r_[a, D + 5.0].astype(np.float32)
+        small = np.r_[b, 2.0].astype(np.float32)
+        assert np.array_equal(union(oracle, "oracle_sphere_union", big, small), big)
+        assert np.array_equal(union(oracle, "oracle_sphere_union", small, big), big)
+
+
 def angle(a, b):
     a = a / np.linalg.norm(a)
     b = b / np.linalg.norm(b)

@@ -65,6 +65,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-rows", type=int, default=0, help="rows of the CPU sample (0: automatic)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-views leg")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner run in the batching leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--no-update", action="store_true", help="skip the object-update leg")
     ap.add_argument("--no-rah", action="store_true", help="skip the RAH / CRSH A/B leg")
@@ -77,6 +78,15 @@ def parse():
     ap.add_argument("--serve-seconds", type=float, default=6.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="copy MIB MiB device-to-device 3 times with a plain copy kernel and exit")
+    ap.add_argument("--hsc-profile", default="", choices=["", "frame", "office", "atrium"],
+                    help="render only this workload --steps times (for a rocprofv3 kernel trace or counter run "
+                         "of the hash, sort and compress kernels); write its pass sizes as JSON to --out")
+    ap.add_argument("--hsc-summary", nargs=2, default=None, metavar=("KERNEL_STATS_CSV", "SIZES_JSON"),
+                    help="(no GPU) summarise a rocprofv3 --stats kernel_stats.csv of a --hsc-profile run")
+    ap.add_argument("--fetch-csv", default="", help="with --hsc-summary: FETCH_SIZE counter_collection.csv")
+    ap.add_argument("--write-csv", default="", help="with --hsc-summary: WRITE_SIZE counter_collection.csv")
+    ap.add_argument("--fetch-scale", type=float, default=2.0,
+                    help="FETCH_SIZE calibration: bytes read per counted byte on a plain copy")
     ap.add_argument("--out", default="")
     return ap.parse_args()
 
@@ -134,48 +144,190 @@ def cpu_baseline(scene, cam, cfg, rows):
         t0 = time.perf_counter()
         fr = pyoracle.render(scene, cam, cfg, threads=th, keep_hits=False, keep_gbuffer=False, row_range=(y0, y0 + r))
         dt = time.perf_counter() - t0
-        return y0, sum(p["rays"] for p in fr.passes), dt
+        return y0, sum(p["rays"] for p in fr.passes), dt, fr.stage_ms
 
-    y0, rays, dt = band(rows, threads)
+    y0, rays, dt, stage_ms = band(rows, threads)
     r1 = max(4, rows // max(1, threads))
-    y1, rays1, dt1 = band(r1, 1)
+    y1, rays1, dt1, stage_ms1 = band(r1, 1)
     return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
             "cpu": cpu_model(),
             "sample": f"rows {y0}-{y0 + rows} of the 1920x1080 atrium frame (all passes), "
                       f"{rays} rays in {dt:.1f} s",
+            "stage_ms": stage_ms,
+            "stage_clock": "std::chrono::steady_clock per stage inside the oracle (TestManager.cpp:284-309's "
+                           "per-stage timers); compare with stage_ms of the GPU frame",
             "one_core": {"value": rays1 / dt1 / 1e6, "cores": 1,
-                         "sample": f"rows {y1}-{y1 + r1}, {rays1} rays in {dt1:.1f} s"}}
+                         "sample": f"rows {y1}-{y1 + r1}, {rays1} rays in {dt1:.1f} s", "stage_ms": stage_ms1}}
 
 
-def sort_passes(cfg):
-    """8-bit digit passes of the radix sort over the key width of cfg."""
-    bits = 32 if cfg.key_mode == 0 or cfg.segment_shift else min(32, cfg.hash_bits())
-    return (bits + 7) // 8
+def sort_digit_passes(cfg):
+    """Radix digit passes the engine's sort runs on a shadow or reflection
+    pass (run_radix_sort): octahedral keys without a view id sort the bits
+    below their common type bits, reference keys and keys with a view id all
+    32; 9-bit digits where they save a pass (27 bits: three passes)."""
+    if cfg.key_mode == 1 and not cfg.segment_shift:
+        bits = min(32, cfg.hash_bits()) - 2
+    else:
+        bits = 32
+    rb = 9 if -(-bits // 9) < -(-bits // 8) else 8
+    return -(-bits // rb)
 
 
-def hsc_bytes(first, S, N, C, digit_passes):
+def hsc_bytes(p, first, cfg, digit_passes):
     """Algorithmic HBM bytes of one pass's hash, trim, compress, sort and
-    decompress stages as the reference structures them (per stage: read its
-    inputs, write its outputs, at the reference's buffer layouts): create +
-    hash 36 B per slot for primary rays, 68 B per slot where rays come from a
-    G-buffer; trim 12 B per slot + 8 B per live ray; compress 4 B per ray + 16 B
-    per chunk; sort 4 + 16 B per chunk and digit pass; decompress 16 B per ray
-    + 16 B per chunk.  S slots, N live rays, C chunks."""
-    create = (36 if first else 68) * S
-    return create + (12 * S + 8 * N) + (4 * N + 16 * C) + C * (4 + 16 * digit_passes) + (16 * N + 16 * C)
+    decompress stages (the survey's per-stage definitions, counted only for
+    the stages the engine runs).  The camera pass of a CRSH frame creates its
+    rays in key order straight into the hierarchy records: 36 B per slot and
+    nothing else.  Other passes: create + hash 68 B per slot (from the
+    G-buffer); trim 12 B per slot + 8 B per live ray; compress 4 B per ray +
+    16 B per chunk; sort 4 + 16 B per chunk and digit pass actually run;
+    decompress 16 B per ray + 16 B per chunk.  S slots, N live rays, C chunks."""
+    S, N, C = int(p["slots"]), int(p["rays"]), int(p["chunks"])
+    if first and cfg.algorithm == 0:
+        return 36 * S
+    return 68 * S + (12 * S + 8 * N) + (4 * N + 16 * C) + C * (4 + 16 * digit_passes) + (16 * N + 16 * C)
 
 
-def hsc_figure(passes, stages, digit_passes):
-    """Hash + sort + compress effective bandwidth: the algorithmic bytes of those
-    stages over their summed time (events around the stages' kernels; trimming
-    and compression run fused, so the compress stage time is inside trim's).
-    The engine moves fewer bytes than the reference's stage structure, so this
-    is an effective figure."""
-    nbytes = sum(hsc_bytes(i == 0, p["slots"], p["rays"], p["chunks"], digit_passes) for i, p in enumerate(passes))
+def hsc_total_bytes(passes, cfg):
+    P = sort_digit_passes(cfg)
+    return sum(hsc_bytes(p, i == 0, cfg, P) for i, p in enumerate(passes))
+
+
+def hsc_figure(passes, stages, cfg, name):
+    """Hash + sort + compress bandwidth: the algorithmic bytes of those stages
+    over their summed time (HIP events around the stages' kernels, passes
+    serial; trimming and compression run fused, so the compress stage time is
+    inside trim's)."""
+    nbytes = hsc_total_bytes(passes, cfg)
     ms = stages["create"] + stages["trim"] + stages["compress"] + stages["sort"] + stages["decompress"]
     gbs = nbytes / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     return {"achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": gbs / PEAK_HBM_GBS,
-            "bytes": int(nbytes), "ms": ms, "kind": "effective (reference stage bytes / time)"}
+            "bytes": int(nbytes), "ms": ms, "digit_passes": sort_digit_passes(cfg),
+            "kind": "algorithmic bytes of the stages that run / HIP-event stage time",
+            "frac_rocprof": hsc_profile_frac(name)}
+
+
+HSC_KERNELS = ("k_create_primary_sorted", "k_create_shadow", "k_create_reflection", "k_create_bounce",
+               "k_pack_tiles", "k_pack_scan", "k_pack_write", "k_radix_upsweep", "k_radix_rowscan",
+               "k_radix_downsweep", "k_unpack_sizes", "k_scan_tiles", "k_unpack_offsets", "k_unpack_expand")
+HSC_PROFILES = {"frame": "hsc_atrium1080p_frame.txt", "office": "hsc_office_64views.txt",
+                "atrium": "hsc_atrium_64views.txt"}
+
+
+def hsc_profile_frac(name):
+    """The rocprofv3-based fraction committed under profiles/ for this
+    workload: the same algorithmic bytes over the summed kernel durations of
+    a kernel trace (None when absent)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", HSC_PROFILES[name])) as f:
+            m = re.search(r"^hsc_frac_rocprof\s*=\s*([0-9.eE+]+)", f.read(), re.M)
+        return float(m.group(1)) if m else None
+    except (OSError, KeyError):
+        return None
+
+
+def hsc_workload(name, rsh_amd):
+    """(engine, render callable, cfg) of a hash / sort / compress workload:
+    the atrium 1080p frame, or the 64-view batch of the office-scale or
+    atrium scene."""
+    import dataclasses
+    from rsh_amd import config as cf, scenes
+    eng = rsh_amd.Engine(0)
+    if name == "frame":
+        scene = scenes.config2_scene()
+        cam = scenes.config2_camera(1920, 1080)
+        cfg = cf.config2(1920, 1080).with_scene_bounds(scene)
+        eng.load_scene(scene)
+        return eng, (lambda: eng.render(cam, cfg)), cfg
+    views = scenes.batch_views()
+    scene, base = (scenes.config1_scene(), cf.config1()) if name == "office" else (scenes.config2_scene(), cf.config2())
+    cfg = dataclasses.replace(base, cell_bits=4, dir_bits=6, segment_shift=26).with_scene_bounds(scene)
+    eng.load_scene(scene)
+    return eng, (lambda: eng.render_views(views, cfg)), cfg
+
+
+def hsc_profile_run(a):
+    """--hsc-profile: the workload rendered --steps times and nothing else, so
+    that a kernel trace holds exactly those renders; pass sizes to --out."""
+    import dataclasses
+    import rsh_amd
+    eng, render, cfg = hsc_workload(a.hsc_profile, rsh_amd)
+    st = None
+    for i in range(a.steps):
+        st = render()
+        progress(f"{a.hsc_profile} render {i + 1}/{a.steps}")
+    passes = [{"kind": int(st.pass_kind[i]), "slots": int(st.pass_slots[i]), "rays": int(st.pass_rays[i]),
+               "chunks": int(st.pass_chunks[i])} for i in range(st.pass_count)]
+    out = {"workload": a.hsc_profile, "renders": a.steps, "passes": passes,
+           "cfg": {k: v for k, v in dataclasses.asdict(cfg).items() if isinstance(v, (int, float))},
+           "bytes_per_render": hsc_total_bytes(passes, cfg), "digit_passes": sort_digit_passes(cfg)}
+    eng.close()
+    line = json.dumps(out)
+    print(line)
+    if a.out:
+        with open(a.out, "w") as f:
+            f.write(line + "\n")
+
+
+def _kernel_base(name):
+    name = name.strip().strip('"')
+    for sep in ("<", "("):
+        name = name.split(sep)[0]
+    return name.strip()
+
+
+def hsc_summary(a):
+    """--hsc-summary: the hash / sort / compress kernels of a --hsc-profile
+    kernel trace, their summed durations, the algorithmic bytes of the
+    renders and the fraction of peak HBM; with counter files, FETCH_SIZE +
+    WRITE_SIZE of the same kernels against the algorithmic bytes."""
+    import csv
+    from rsh_amd import config as cf
+    stats_csv, sizes_json = a.hsc_summary
+    with open(sizes_json) as f:
+        sizes = json.loads(f.read().strip().splitlines()[-1])
+    cfg = cf.FrameConfig(**{k: v for k, v in sizes["cfg"].items() if k in cf.FrameConfig.__dataclass_fields__})
+    rows = {}
+    with open(stats_csv) as f:
+        for r in csv.DictReader(f):
+            base = _kernel_base(r["Name"])
+            if base in HSC_KERNELS:
+                d = rows.setdefault(r["Name"].strip('"').split("(")[0], [0, 0.0])
+                d[0] += int(r["Calls"])
+                d[1] += float(r["TotalDurationNs"]) / 1e3
+    total_us = sum(v[1] for v in rows.values())
+    R = sizes["renders"]
+    nbytes = R * hsc_total_bytes(sizes["passes"], cfg)
+    gbs = nbytes / (total_us * 1e-6) / 1e9
+    lines = [f"kernel{'':40s} calls    total_us   avg_us"]
+    for k, (c, t) in sorted(rows.items(), key=lambda kv: -kv[1][1]):
+        lines.append(f"{k:46s} {c:6d} {t:11.1f} {t / max(c, 1):8.2f}")
+    lines.append("")
+    lines.append(f"renders in the trace: {R}; passes per render: {len(sizes['passes'])}; "
+                 f"sort digit passes: {sizes['digit_passes']}")
+    lines.append("per pass (kind 0 camera / 1 shadow / 2 reflection): slots, live rays, chunks, algorithmic bytes")
+    P = sort_digit_passes(cfg)
+    for i, p in enumerate(sizes["passes"]):
+        lines.append(f"  {i}: kind {p['kind']}  S {p['slots']:>10d}  N {p['rays']:>10d}  C {p['chunks']:>10d}  "
+                     f"{hsc_bytes(p, i == 0, cfg, P):>13d} B")
+    lines.append(f"algorithmic bytes: {nbytes} ({R} renders)")
+    lines.append(f"summed kernel time: {total_us:.1f} us  ({total_us / R:.1f} us per render)")
+    lines.append(f"achieved: {gbs:.1f} GB/s of {PEAK_HBM_GBS:.0f}")
+    lines.append(f"hsc_frac_rocprof = {gbs / PEAK_HBM_GBS:.4f}")
+
+    def counter(path):
+        tot = 0.0
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                if _kernel_base(r.get("Kernel_Name", "")) in HSC_KERNELS:
+                    tot += float(r["Counter_Value"])
+        return tot * 1024.0
+    if a.fetch_csv and a.write_csv:
+        fb = counter(a.fetch_csv) * a.fetch_scale
+        wb = counter(a.write_csv)
+        lines.append(f"FETCH_SIZE x {a.fetch_scale:.4f}: {fb:.4e} B; WRITE_SIZE: {wb:.4e} B; "
+                     f"sum {fb + wb:.4e} B = {(fb + wb) / nbytes:.3f} x the algorithmic bytes")
+    print("\n".join(lines))
 
 
 def soft_leg(rsh_amd, eng, scene, cam, cfg, reps=5):
@@ -357,8 +509,75 @@ def rah_leg(rsh_amd, eng, scene2, cam2, cfg2, reps=3):
     return out
 
 
-def batching_leg(rsh_amd, eng, scene2):
-    """Plain per-view frames against the batched form on the 64-view batch."""
+def pass_slot_maps(views, L):
+    """Per view, the batch slot of each of its slots, by pass kind: camera
+    slots are pixels; reflection slot x + y w L of a view is pixel off + x +
+    y w of the batch (a P x 1 image); shadow slot x + w (l + L y) is batch
+    slot (off + x + y w) + P l."""
+    P = sum(w * h for _, w, h in views)
+    import numpy as np
+    maps, off = [], 0
+    for _, w, h in views:
+        q = np.arange(w * h)
+        x, y = q % w, q // w
+        m = {0: (q, off + q), 2: (x + y * w * L, off + q)}
+        sl = [x + w * (l + L * y) for l in range(L)]
+        bl = [off + q + P * l for l in range(L)]
+        m[1] = (np.concatenate(sl), np.concatenate(bl))
+        maps.append(m)
+        off += w * h
+    return maps
+
+
+def hit_agreement(eng, views, cfg, rsh_amd):
+    """Per-ray results of the batched form against the plain per-view frames,
+    pass by pass: closest-hit triangle ids (and, where they differ, the
+    relative difference of the hit distances) and shadow flags."""
+    import dataclasses
+    import numpy as np
+    L = eng._scene.light_total
+    eng.render_views(views, cfg, flags=eng.KEEP_PASSES)
+    nb = eng.stats.pass_count
+    kinds = [int(eng.stats.pass_kind[i]) for i in range(nb)]
+    batch = [eng.pass_hits(i) for i in range(nb)]
+    maps = pass_slot_maps(views, L)
+    same = [0] * nb
+    total = [0] * nb
+    worst = 0.0
+    for v, (cam, w, h) in enumerate(views):
+        eng.render(cam, dataclasses.replace(cfg, width=w, height=h), flags=eng.KEEP_PASSES)
+        for i in range(min(nb, eng.stats.pass_count)):
+            plain = eng.pass_hits(i)
+            ps, bs = maps[v][kinds[i]]
+            a, b = plain[ps], batch[i][bs]
+            if kinds[i] == 1:
+                same[i] += int((a == b).sum())
+            else:
+                ta, tb = a & np.uint64(0xFFFFFFFF), b & np.uint64(0xFFFFFFFF)
+                eq = ta == tb
+                same[i] += int(eq.sum())
+                d = ~eq
+                if d.any():
+                    miss = np.uint64(0xFFFFFFFFFFFFFFFF)
+                    both = d & (a != miss) & (b != miss)
+                    if (d & ~both).any():
+                        worst = float("inf")
+                    fa = (a[both] >> np.uint64(32)).astype(np.uint32).view(np.float32).astype(np.float64)
+                    fb = (b[both] >> np.uint64(32)).astype(np.uint32).view(np.float32).astype(np.float64)
+                    if both.any():
+                        worst = max(worst, float(np.max(np.abs(fa - fb) / np.maximum(np.abs(fa), 1e-30))))
+            total[i] += len(ps)
+    rays = sum(total)
+    agree = sum(same)
+    return {"slots_compared": rays, "identical": agree, "fraction": agree / rays,
+            "per_pass_differing": [t - s for s, t in zip(same, total)],
+            "max_rel_hit_distance_among_differing": worst}
+
+
+def batching_leg(rsh_amd, eng, scene2, tune=True):
+    """Plain per-view frames against the batched form on the 64-view batch:
+    throughput (eager and graph-replayed), per-ray hit agreement, the hash /
+    sort / compress bandwidth, and the autotuner's choice for the atrium."""
     import dataclasses
     from rsh_amd import autotune, config as cf, scenes
     views = scenes.batch_views()
@@ -374,11 +593,12 @@ def batching_leg(rsh_amd, eng, scene2):
         rays, imgs = 0, []
         for cam, w, h in views:
             rays += eng.render(cam, dataclasses.replace(cfg, width=w, height=h)).rays_total
-            imgs.append(eng.pixels().copy())
+            imgs.append(eng.read_pixels_pinned().copy())
         plain = time.perf_counter() - t0
         eng.render_views(views, cfg)
         t0 = time.perf_counter()
         st = eng.render_views(views, cfg)
+        eng.read_pixels_pinned()
         eager = time.perf_counter() - t0
         parts = eng.split_views(eng.pixels(), views)
         same = sum(int((a == b).sum()) for a, b in zip(parts, imgs)) / out["pixels"]
@@ -387,13 +607,29 @@ def batching_leg(rsh_amd, eng, scene2):
         stages = {n: st.stage_ms[i] for i, n in enumerate(rsh_amd.STAGES)}
         bp = [{"slots": st.pass_slots[i], "rays": st.pass_rays[i], "chunks": st.pass_chunks[i]}
               for i in range(st.pass_count)]
-        hsc = hsc_figure(bp, stages, sort_passes(cfg))
+        hsc = hsc_figure(bp, stages, cfg, name)
         out[name] = {"plain_mrays": rays / plain / 1e6, "plain_ms": plain * 1e3,
                      "batched_mrays": st.rays_total / eager / 1e6, "batched_ms": eager * 1e3,
                      "graph_mrays": brays / graph_ms / 1e3, "graph_ms": graph_ms,
                      "speedup": (brays / graph_ms / 1e3) / (rays / plain / 1e6),
                      "rays": int(brays), "identical_pixels_vs_plain": same,
+                     "hits_vs_plain": hit_agreement(eng, views, cfg, rsh_amd),
                      "hash_sort_compress": hsc}
+        progress(f"batching: {name} done")
+        if tune and name == "atrium":
+            t0 = time.perf_counter()
+            best, trials = autotune.tune(eng, scene, views, cfg, reps=3)
+            out[name]["autotune"] = {
+                "seconds": time.perf_counter() - t0,
+                "chosen": {"cell_bits": best.cell_bits, "dir_bits": best.dir_bits, "leaf": best.leaf,
+                           "branch": best.branch, "traversal_chunk": best.traversal_chunk},
+                "chosen_graph_mrays": max(t.mrays for t in trials),
+                "start_graph_mrays": trials[0].mrays,
+                "trials": [[t.cell_bits, t.dir_bits, t.leaf, t.branch, t.traversal_chunk, round(t.ms, 2),
+                            round(t.mrays, 1), round(t.same_pixels, 6)] for t in trials],
+                "trial_fields": ["cell_bits", "dir_bits", "leaf", "branch", "traversal_chunk", "graph_ms",
+                                 "mrays", "same_pixels_as_start"]}
+            progress("batching: autotune done")
     return out
 
 
@@ -406,6 +642,12 @@ def main():
     if a.copy_probe:
         copy_probe(a.copy_probe)
         return
+    if a.hsc_summary:
+        hsc_summary(a)
+        return
+    if a.hsc_profile:
+        hsc_profile_run(a)
+        return
     import rsh_amd
     from rsh_amd import config as cf, scenes
 
@@ -416,6 +658,7 @@ def main():
     eng.load_scene(scene)
     for _ in range(a.warmup):
         eng.render(cam, cfg)
+        eng.read_pixels_pinned()
     rsh_amd.lib().rsh_device_synchronize()
     times, rays = [], []
     stage_sum = [0.0] * len(rsh_amd.STAGES)
@@ -423,13 +666,32 @@ def main():
     t_all = time.perf_counter()
     for _ in range(a.steps):
         t0 = time.perf_counter()
-        # the frame as served: shadow passes concurrent with the reflection chain
+        # the frame as served: shadow passes concurrent with the reflection
+        # chain, then the image read back into page-locked host memory
         st = eng.render(cam, cfg)
+        eng.read_pixels_pinned()
         times.append(time.perf_counter() - t0)
         rays.append(st.rays_total)
     rsh_amd.lib().rsh_device_synchronize()
     total = time.perf_counter() - t_all
     value = sum(rays) / total / 1e6
+    eager_img = eng.read_pixels_pinned().copy()
+    # The same frame replayed from a captured HIP graph (valid while the
+    # camera, configuration and scene stay the same), image read back too.
+    eng.render(cam, cfg, flags=eng.GRAPH)
+    eng.render(cam, cfg, flags=eng.GRAPH)
+    graph_same = bool((eng.read_pixels_pinned() == eager_img).all())
+    g_times = []
+    g0 = time.perf_counter()
+    for _ in range(a.steps):
+        t0 = time.perf_counter()
+        st = eng.render(cam, cfg, flags=eng.GRAPH)
+        eng.read_pixels_pinned()
+        g_times.append(time.perf_counter() - t0)
+    g_total = time.perf_counter() - g0
+    graph = {"mrays": a.steps * statistics.median(rays) / g_total / 1e6, "ms_per_frame": 1e3 * g_total / a.steps,
+             "frame_ms_median": 1e3 * statistics.median(g_times), "identical_image": graph_same,
+             "note": "HIP graph of the whole frame replayed; valid while camera, configuration and scene repeat"}
     # The traversal kernel's launch time (the roofline): HIP events around each
     # traversal launch only, in frames whose passes run one at a time, so no
     # other kernel shares the GPU with the timed launch.
@@ -463,7 +725,7 @@ def main():
             "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic_per_launch(),
             "avg_launch_ms": avg_launch_ms, "launches_per_frame": launches_per_frame,
             "flops_per_launch": flops_launch}
-    hsc = hsc_figure(passes, stages, sort_passes(cfg))
+    hsc = hsc_figure(passes, stages, cfg, "frame")
     hsc["workload"] = "the timed atrium frames (8 passes, %d rays per frame)" % int(statistics.median(rays))
 
     progress(f"frames: {value:.1f} Mrays/s")
@@ -480,7 +742,7 @@ def main():
     cornell = cornell_leg(rsh_amd, eng, a.cornell) if a.cornell else None
     eng.load_scene(scene)
     if not a.no_batch:
-        batching = batching_leg(rsh_amd, eng, scene)
+        batching = batching_leg(rsh_amd, eng, scene, tune=not a.no_tune)
         progress("batching leg done")
     out = {
         "metric": "Mrays/s end-to-end at 262k tris 1080p",
@@ -499,11 +761,15 @@ def main():
                                f"{scene.object_total} objects, 3 bounces + shadows, 30% reflective",
                    "rays_per_frame": int(statistics.median(rays)),
                    "hash": "octahedral 3x5+2x6+2 bits, cell and direction bits interleaved",
-                   "hierarchy": "leaf 8, branch 8, depth 2"},
+                   "hierarchy": "leaf 8, branch 8, depth 2",
+                   "cone_union": "guarded (normalised, clamped dots, parallel and antipodal rays handled); "
+                                 "the reference's CreateHierarchyCone0 union is cone_mode 0"},
         "roofline": roof,
         "hash_sort_compress": hsc,
         "stage_ms": stages,
         "frame_ms_median": 1e3 * statistics.median(times),
+        "timed_region": "render + read-back of the 1920x1080 image into page-locked host memory",
+        "graph_replay": graph,
     }
     eng.close()
     if soft:

@@ -95,9 +95,10 @@ typedef struct rsh_frame_config {
                                        1 none; 2 each ray's line against the triangle's
                                        bounding sphere; 3 that and, per passing cluster,
                                        the rays' lines against the cluster's sphere (for
-                                       spread-out rays); 0 automatic: 2 or 3 per pass,
-                                       3 when over a quarter of the top nodes' cones are
-                                       wider than tan 0.1.  Hit records
+                                       spread-out rays); 0 automatic: 2 for a frame's
+                                       camera rays; otherwise 3 when more than a
+                                       sixteenth of the top nodes' cones have tan(spread)
+                                       outside [0, 0.1], else 2.  Hit records
                                        are the same in every mode; with mode 3 and stats
                                        counters (flag bit 1) the cluster masks are off so
                                        that the counts stay the reference's.             */
@@ -145,6 +146,12 @@ rsh_status rsh_engine_create(rsh_engine** out, int device);
 void rsh_engine_destroy(rsh_engine* e);
 const char* rsh_last_error(const rsh_engine* e);
 rsh_status rsh_engine_set_stream(rsh_engine* e, void* hip_stream);
+/* Deadline of each host wait of a frame for a pass size the device publishes
+ * (default 60000 ms).  Past it the call fails with RSH_INTERNAL and a message
+ * instead of spinning on a hung device or a blocked caller stream.  No
+ * counterpart in the reference, whose blocking cudaMemcpy read-backs
+ * (RayTracer.cu:2316,2364,2649,2720,2832) wait without a bound. */
+rsh_status rsh_engine_set_wait_timeout(rsh_engine* e, uint32_t ms);
 
 /* Upload a scene (host arrays) and precompute per-triangle data. */
 rsh_status rsh_engine_load_scene(rsh_engine* e, const rsh_scene_desc* scene);
@@ -290,6 +297,12 @@ void rsh_host_free(void* p);
  * on the hardware estimate) with IEEE division over every float in its
  * estimate range.  Returns a HIP error code; *mismatches must be 0. */
 int rsh_selfcheck_rcp(int device, uint64_t* tested, uint64_t* mismatches);
+/* Self-check of the bounded host wait (no counterpart in the reference; used
+ * by the test suite): waits with a deadline of wait_ms for a host word that is
+ * never published, on a stream a host function keeps busy for block_ms.
+ * Returns RSH_INTERNAL when the deadline ended the wait (expected for
+ * block_ms > wait_ms), RSH_HIP_ERROR when the stream went idle first. */
+rsh_status rsh_selfcheck_wait_deadline(uint32_t wait_ms, uint32_t block_ms, uint32_t* elapsed_ms);
 
 /* Device memory helpers (plain HIP calls, for callers without a HIP runtime
  * binding of their own). */

@@ -27,6 +27,9 @@
 #define RSH_ENGINE_H_
 
 #include <stdint.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
 
 #ifdef __cplusplus
 extern "C" {
@@ -168,7 +171,7 @@ void ReflectionRayIntersectionWrapper(rsh_float3* rayArray, unsigned int* sorted
 void ReflectionRayColoringWrapper(rsh_float4* trianglePositionsArray,
                                   rsh_float4* triangleNormalsArray, const unsigned int windowWidth,
                                   const unsigned int windowHeight, const unsigned int lightTotal,
-                                  const rsh_float3 cameraPosition, const int createRays,
+                                  const rsh_float3 cameraPosition, const bool createRays,
                                   unsigned int* intersectionTimeArray, rsh_float3* rayArray,
                                   unsigned int* headFlagsArray, unsigned int* rayIndexKeysArray,
                                   unsigned int* rayIndexValuesArray,

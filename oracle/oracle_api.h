@@ -82,6 +82,8 @@ typedef struct oracle_frame_out {
     uint64_t pass_pairs_top[16];    // candidate (top node, triangle) pairs tested
     uint64_t pass_pairs_leaf[16];   // surviving (leaf, triangle) pairs
     uint64_t pass_ray_tests[16];    // ray-triangle tests performed
+    double stage_ms[8];             // steady_clock ms per stage: create, trim, compress, sort,
+                                    // decompress, hierarchy, traversal, shading
 } oracle_frame_out;
 
 int oracle_render(const oracle_scene* scene, const oracle_camera* camera,

@@ -48,7 +48,7 @@ class OFrameOut(C.Structure):
                 ("gbuffer_spec", PF), ("pass_hits", PU64 * 16), ("pass_count", u32),
                 ("pass_kind", u32 * 16), ("pass_slots", u32 * 16), ("pass_rays", u32 * 16),
                 ("pass_chunks", u32 * 16), ("pass_pairs_top", u64 * 16), ("pass_pairs_leaf", u64 * 16),
-                ("pass_ray_tests", u64 * 16)]
+                ("pass_ray_tests", u64 * 16), ("stage_ms", C.c_double * 8)]
 
 
 REF_SRC = "/root/reference/projects/CUDA-Engine/source"
@@ -85,6 +85,7 @@ def ref_miniball(points):
 
 
 _lib = None
+STAGES = ("create", "trim", "compress", "sort", "decompress", "hierarchy", "traversal", "shading")
 
 
 def lib():
@@ -217,10 +218,11 @@ def call(name, *args):
 class Frame:
     """Outputs of one oracle frame."""
 
-    def __init__(self, pixels, gbuffer, passes):
+    def __init__(self, pixels, gbuffer, passes, stage_ms=None):
         self.pixels = pixels
         self.gbuffer = gbuffer
         self.passes = passes
+        self.stage_ms = stage_ms
 
 
 def render(scene, camera, cfg, gbuffer=None, threads=0, keep_hits=True, keep_gbuffer=True, row_range=None) -> Frame:
@@ -260,7 +262,7 @@ def render(scene, camera, cfg, gbuffer=None, threads=0, keep_hits=True, keep_gbu
                            chunks=out.pass_chunks[i], pairs_top=out.pass_pairs_top[i],
                            pairs_leaf=out.pass_pairs_leaf[i], ray_tests=out.pass_ray_tests[i],
                            hits=hits[i][:out.pass_slots[i]].copy() if keep_hits else None))
-    return Frame(pixels.reshape(H, W), gb, passes)
+    return Frame(pixels.reshape(H, W), gb, passes, dict(zip(STAGES, list(out.stage_ms))))
 
 
 def render_views(scene, views, cfg, threads=0, keep_hits=True, keep_gbuffer=True) -> Frame:

@@ -45,6 +45,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <climits>
 #include <cstdio>
 #include <cstdlib>
@@ -68,6 +69,25 @@ inline f3 ld3(const float* p) { return f3{p[0], p[1], p[2]}; }
 inline f4 ld4(const float* p) { return f4{p[0], p[1], p[2], p[3]}; }
 inline void st3(float* p, f3 v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
 inline void st4(float* p, f4 v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; p[3] = v.w; }
+
+// Per-stage wall time of the frame being rendered (std::chrono::steady_clock,
+// the counterpart of the reference's per-stage timers, TestManager.cpp:
+// 284-309): create, trim, compress, sort, decompress, hierarchy, traversal,
+// shading.  One frame renders at a time (the oracle is not re-entrant).
+enum { SC_CREATE, SC_TRIM, SC_COMPRESS, SC_SORT, SC_DECOMPRESS, SC_HIERARCHY, SC_TRAVERSE, SC_SHADE, SC_STAGES };
+double g_stage_ms[SC_STAGES];
+struct StageClock {
+    int stage;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    explicit StageClock(int s) : stage(s) {}
+    void lap(int next) {
+        const auto t = std::chrono::steady_clock::now();
+        g_stage_ms[stage] += std::chrono::duration<double, std::milli>(t - t0).count();
+        t0 = t;
+        stage = next;
+    }
+    ~StageClock() { lap(stage); }
+};
 
 // ------------------------------------------------------------------ keys ---
 // RT:147-168 CreateShadowRayIndex
@@ -357,15 +377,18 @@ void trim(const Batch& b, std::vector<uint32_t>& tk, std::vector<uint32_t>& tv) 
 void compress_sort_decompress(const std::vector<uint32_t>& tk, const std::vector<uint32_t>& tv,
                               std::vector<uint32_t>& sk, std::vector<uint32_t>& sv, uint32_t* chunks) {
     const uint32_t N = (uint32_t)tk.size();
+    StageClock clk(SC_COMPRESS);
     std::vector<uint32_t> bases, sizes, ckeys, cvals;
     for (uint32_t x = 0; x < N; x++)
         if (x == 0 || tk[x] != tk[x - 1]) { bases.push_back(x); ckeys.push_back(tk[x]); }
     const uint32_t C = (uint32_t)bases.size();
     sizes.resize(C);
     for (uint32_t c = 0; c < C; c++) { sizes[c] = (c + 1 < C ? bases[c + 1] : N) - bases[c]; cvals.push_back(c); }
+    clk.lap(SC_SORT);
     std::vector<uint32_t> perm(C);
     std::iota(perm.begin(), perm.end(), 0u);
     std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) { return ckeys[a] < ckeys[b]; });
+    clk.lap(SC_DECOMPRESS);
     sk.resize(N); sv.resize(N);
     uint32_t pos = 0;
     for (uint32_t i = 0; i < C; i++) {
@@ -493,15 +516,22 @@ void trace(const Scene& sc, const oracle_config& cfg, const Batch& b, uint32_t k
            int threads, const float* tmax = nullptr) {
     (void)threads;
     std::vector<uint32_t> tk, tv, sk, sv;
-    trim(b, tk, tv);
-    st.N = (uint32_t)tk.size();
-    result.assign(b.S, kind == 0 ? (cfg.hit_mode == 0 ? MISS64 : (uint64_t)MISS32) : 0ull);
+    {
+        StageClock clk(SC_TRIM);
+        trim(b, tk, tv);
+        st.N = (uint32_t)tk.size();
+        result.assign(b.S, kind == 0 ? (cfg.hit_mode == 0 ? MISS64 : (uint64_t)MISS32) : 0ull);
+    }
     if (st.N == 0) return;
     if (cfg.algorithm == 0) compress_sort_decompress(tk, tv, sk, sv, &st.C);
     else { sk = tk; sv = tv; st.C = st.N; }
     Hier h;
-    build_hierarchy(b, sv, st.N, cfg.leaf, cfg.branch, cfg.depth, cfg.initial_radius, cfg.initial_spread,
-                    cfg.cone_mode, h);
+    {
+        StageClock clk(SC_HIERARCHY);
+        build_hierarchy(b, sv, st.N, cfg.leaf, cfg.branch, cfg.depth, cfg.initial_radius, cfg.initial_spread,
+                        cfg.cone_mode, h);
+    }
+    StageClock clk(SC_TRAVERSE);
     Traverser tr{sc, h, cfg.algorithm, cfg.branch, cfg.triangle_batch};
     const uint32_t N = st.N, leaf = cfg.leaf, W = cfg.width;
     uint64_t* res = result.data();
@@ -605,6 +635,7 @@ float soft_factor(uint32_t occluded) {
 // the slot's surface point (RT:1568-1590's interpolated rays).
 void create_shadow(const Scene& sc, const oracle_config& cfg, const GBuf& g, uint32_t Lmax, const ViewSet& vs,
                    Batch& b) {
+    StageClock clk_(SC_CREATE);
     const uint32_t W = g.W, H = g.H;
     const uint32_t sh = cfg.soft_shadows ? 4u : 0u;
     b.init((W * H * Lmax) << sh);
@@ -640,6 +671,7 @@ void create_shadow(const Scene& sc, const oracle_config& cfg, const GBuf& g, uin
 // RT:742-791 CreateReflectionRays, slot x + y*W*L; gap slots (L > 1) are
 // marked empty.  With reflect_gate only materials whose diffuse.w > 0 reflect.
 void create_reflection(const oracle_config& cfg, const GBuf& g, const ViewSet& vs, uint32_t Lmax, Batch& b) {
+    StageClock clk_(SC_CREATE);
     const uint32_t W = g.W, H = g.H;
     b.init(W * H * Lmax);
 #pragma omp parallel for schedule(static)
@@ -665,6 +697,7 @@ void create_reflection(const oracle_config& cfg, const GBuf& g, const ViewSet& v
 // barycentric (unnormalised) normal; gated by material reflectivity.
 void create_bounce(const oracle_config& cfg, const GBuf& g, const Batch& prev,
                    const std::vector<uint64_t>& rec, uint32_t Lmax, const ViewSet& vs, Batch& b) {
+    StageClock clk_(SC_CREATE);
     const uint32_t W = g.W, H = g.H;
     b.init(W * H * Lmax);
 #pragma omp parallel for schedule(static)
@@ -701,6 +734,7 @@ uint32_t primary_key(uint32_t x, uint32_t y, uint32_t w, uint32_t view, uint32_t
 // Primary rays (configuration stage; formula of RT:667), per view: pixel p of
 // view v is (x, y) within it, dir = D + R*(x/w - 0.5) + U*(y/h - 0.5).
 void create_primary(const oracle_config& cfg, const ViewSet& vs, Batch& b) {
+    StageClock clk_(SC_CREATE);
     const uint32_t P = cfg.width * cfg.height;
     b.init(P);
 #pragma omp parallel for schedule(static)
@@ -727,6 +761,7 @@ void create_primary(const oracle_config& cfg, const ViewSet& vs, Batch& b) {
 // G-buffers store the normalised normal (blinn_phong_fragment_shader.glsl:72).
 void surface_from_hits(const Scene& sc, const oracle_config& cfg, const Batch& b,
                        const std::vector<uint64_t>& rec, uint32_t slot_stride, bool normalize_n, GBuf& g) {
+    StageClock clk_(SC_SHADE);
     const uint32_t W = cfg.width, H = cfg.height;
     g.init(W, H);
 #pragma omp parallel for schedule(static)
@@ -800,6 +835,7 @@ uint32_t soft_mask(const std::vector<uint64_t>& sflags, size_t i) {
 
 void shade_primary(const Scene& sc, const GBuf& g, const std::vector<uint64_t>& sflags, uint32_t Lmax,
                    uint32_t Lt, const ViewSet& vs, bool soft, uint32_t* pixels) {
+    StageClock clk_(SC_SHADE);
     const uint32_t W = g.W, H = g.H;
 #pragma omp parallel for schedule(static)
     for (int64_t pp = 0; pp < (int64_t)W * H; pp++) {
@@ -830,6 +866,7 @@ void shade_primary(const Scene& sc, const GBuf& g, const std::vector<uint64_t>& 
 void shade_reflection(const Scene& sc, const oracle_config& cfg, const GBuf& g,
                       const std::vector<uint64_t>& rec, const std::vector<uint64_t>* sflags,
                       uint32_t Lmax, uint32_t Lt, const ViewSet& vs, uint32_t* pixels) {
+    StageClock clk_(SC_SHADE);
     const uint32_t W = g.W, H = g.H;
 #pragma omp parallel for schedule(static)
     for (int64_t pp = 0; pp < (int64_t)W * H; pp++) {
@@ -1045,6 +1082,7 @@ int render_views(const oracle_scene* scene, const ViewSet& vs, const oracle_conf
     Scene sc(scene);
     const uint32_t W = cfg.width, H = cfg.height, Lt = scene->light_total, Lmax = Lt;
     if (out) out->pass_count = 0;
+    std::fill(g_stage_ms, g_stage_ms + SC_STAGES, 0.0);
     std::vector<uint32_t> pixels((size_t)W * H, 0);
     GBuf g0;
     if (cfg.primary_rays) {
@@ -1110,6 +1148,7 @@ int render_views(const oracle_scene* scene, const ViewSet& vs, const oracle_conf
         prev_rec = std::move(rec);
     }
     if (out && out->pixels) std::copy(pixels.begin(), pixels.end(), out->pixels);
+    if (out) std::copy(g_stage_ms, g_stage_ms + SC_STAGES, out->stage_ms);
     return 0;
 }
 

@@ -392,7 +392,7 @@ void ReflectionRayIntersectionWrapper(rsh_float3* rays, unsigned int*, unsigned 
 }
 
 void ReflectionRayColoringWrapper(rsh_float4* tri_pos, rsh_float4* tri_nrm, const unsigned int W, const unsigned int H,
-                                  const unsigned int L, const rsh_float3 eye, const int create, unsigned int* times,
+                                  const unsigned int L, const rsh_float3 eye, const bool create, unsigned int* times,
                                   rsh_float3* rays, unsigned int* flags, unsigned int* keys, unsigned int* values,
                                   unsigned int* pixels) {
     Compat& c = C();

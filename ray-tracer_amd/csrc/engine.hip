@@ -15,6 +15,7 @@
 //    arrays;
 //  * the OpenGL G-buffer is produced by a primary-ray pass through the same
 //    pipeline.
+#include <chrono>
 #include <condition_variable>
 #include <cstdio>
 #include <cstring>
@@ -53,6 +54,7 @@ struct Lane {
     uint32_t tseq = 0;                         // sequence number of the last item table
     DevBuf w_done;                             // block counter of k_item_table (kept at zero)
     hipEvent_t rb_ev = nullptr;        // marks a read-back's completion (polled)
+    uint32_t wait_ms = 60000;          // deadline of one host wait for a read-back or a published word
     bool table_clean = false;          // w_table is zero (the item scan clears it after each read-back)
     // work list of the 64-ray traversal kernel
     DevBuf w_npass, w_cand, w_nitems, w_off, w_first, w_list, w_cum, w_items, w_table, w_pmask;
@@ -143,6 +145,22 @@ bool fail(rsh_engine* e, const std::string& msg) {
         }                                                                                  \
     } while (0)
 
+// A bounded host wait (readback_wait, wait_word): past the deadline the pass
+// fails with RSH_INTERNAL and says so.
+#define WAIT_OK(call)                                                                      \
+    do {                                                                                   \
+        hipError_t _e = (call);                                                            \
+        if (_e == hipErrorLaunchTimeOut) {                                                 \
+            L.err = std::string(#call) + ": no result within " + std::to_string(L.wait_ms) + \
+                    " ms (device hung or stream blocked)";                                 \
+            return RSH_INTERNAL;                                                           \
+        }                                                                                  \
+        if (_e != hipSuccess) {                                                            \
+            L.err = std::string(#call) + ": " + hipGetErrorString(_e);                     \
+            return RSH_HIP_ERROR;                                                          \
+        }                                                                                  \
+    } while (0)
+
 inline float4* f4p(const DevBuf& b) { return b.as<float4>(); }
 inline uint32_t* u32p(const DevBuf& b) { return b.as<uint32_t>(); }
 inline unsigned long long* u64p(const DevBuf& b) { return b.as<unsigned long long>(); }
@@ -198,15 +216,27 @@ constexpr size_t HSTAGE_WORDS = 2 + 2 * ITEM_TABLE_WORDS;   // N / C or the item
 constexpr uint32_t ITEM_CHUNK_MIN = 4;
 constexpr uint32_t ITEM_TARGET = 65536;
 
+// Host waits are bounded: past the lane's deadline (wait_ms) a wait gives up
+// with WAIT_TIMEOUT and the pass fails with RSH_INTERNAL, instead of spinning
+// for ever on a stream that never drains (a hung device, or a caller's stream
+// blocked on outside work).
+constexpr hipError_t WAIT_TIMEOUT = hipErrorLaunchTimeOut;
+
+inline bool past_deadline(std::chrono::steady_clock::time_point t0, uint32_t ms) {
+    return std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(ms);
+}
+
 // Wait for a lane's read-back copies: an event after them, polled.  A
 // read-back is a few microseconds of GPU work, and a blocking wait would add
 // the host thread's wake-up latency to every pass.
 hipError_t readback_wait(Lane& L) {
     hipError_t r = hipEventRecord(L.rb_ev, L.stream);
     if (r != hipSuccess) return r;
-    for (;;) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t spin = 1;; spin++) {
         r = hipEventQuery(L.rb_ev);
         if (r != hipErrorNotReady) return r;
+        if ((spin & 1023u) == 0 && past_deadline(t0, L.wait_ms)) return WAIT_TIMEOUT;
         __builtin_ia32_pause();
     }
 }
@@ -214,9 +244,10 @@ hipError_t readback_wait(Lane& L) {
 // Wait for a kernel's sequence word `seq` in coherent host memory (the pack
 // scan's sizes, the item table).  Every 1024 polls the stream is queried: an
 // error ends the wait, and so does an idle stream without the word (then the
-// values never came: an internal error).
+// values never came: an internal error), and so does the deadline.
 template <class T>
 hipError_t wait_word(Lane& L, const T* word, uint32_t seq) {
+    const auto t0 = std::chrono::steady_clock::now();
     for (uint32_t spin = 1;; spin++) {
         if (__atomic_load_n(word, __ATOMIC_ACQUIRE) == (T)seq) return hipSuccess;
         if ((spin & 1023u) == 0) {
@@ -224,6 +255,7 @@ hipError_t wait_word(Lane& L, const T* word, uint32_t seq) {
             if (r == hipSuccess)
                 return __atomic_load_n(word, __ATOMIC_ACQUIRE) == (T)seq ? hipSuccess : hipErrorUnknown;
             if (r != hipErrorNotReady) return r;
+            if (past_deadline(t0, L.wait_ms)) return WAIT_TIMEOUT;
         }
         __builtin_ia32_pause();
     }
@@ -421,10 +453,10 @@ rsh_status build_items(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, Trav
             LANE_OK(hipMemcpyAsync(L.hstage + 1, first + ntop, 4, hipMemcpyDeviceToHost, s));
         }
         if (publish) {
-            LANE_OK(wait_word(L, reinterpret_cast<uint64_t*>(L.htable), L.tseq));
+            WAIT_OK(wait_word(L, reinterpret_cast<uint64_t*>(L.htable), L.tseq));
             for (uint32_t i = 0; i < ITEM_TABLE_WORDS; i++) ht[i] = __atomic_load_n(L.htable + 1 + i, __ATOMIC_RELAXED);
         } else {
-            LANE_OK(readback_wait(L));
+            WAIT_OK(readback_wait(L));
             std::memcpy(ht, L.hstage + 2, sizeof(ht));
         }
         if (p.algorithm != 1) tot[0] = (uint32_t)ht[2 * ITEM_TABLE + 1];
@@ -502,7 +534,7 @@ rsh_status trace_pass(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, const
     // sorted directly and no sizes are read back.
     const bool all_unique = camera && kind == 0;
     if (all_unique) {
-        StageTimer t(L, ST_TRIM, timing);
+        // hit records cleared: no trim or compression to time
         launch_fill_u64(u64p(closest), cfg.hit_mode == 0 ? MISS64 : (unsigned long long)MISS32, S, s);
     } else {
         // trimming and compression in one walk over the slots (the compress
@@ -527,7 +559,7 @@ rsh_status trace_pass(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, const
     } else if (S == 0) {
         if (e->recording) { L.plan.emplace_back(); L.plan.back().N = 0; L.plan.back().C = 0; }
     } else {
-        LANE_OK(wait_sizes(L, L.hseq));
+        WAIT_OK(wait_sizes(L, L.hseq));
         hc[0] = __atomic_load_n(L.hsize + 1, __ATOMIC_RELAXED);
         hc[1] = __atomic_load_n(L.hsize + 2, __ATOMIC_RELAXED);
         if (e->recording) { L.plan.emplace_back(); L.plan.back().N = hc[0]; L.plan.back().C = hc[1]; }
@@ -718,6 +750,46 @@ rsh_status rsh_engine_set_stream(rsh_engine* e, void* stream) {
     e->stream = static_cast<hipStream_t>(stream);
     e->own_stream = false;
     return RSH_OK;
+}
+
+rsh_status rsh_engine_set_wait_timeout(rsh_engine* e, uint32_t ms) {
+    if (!e || ms == 0) return e ? (fail(e, "set_wait_timeout: the deadline must be positive"), RSH_INVALID_ARG)
+                                : RSH_INVALID_ARG;
+    for (Lane& l : e->lane) l.wait_ms = ms;
+    return RSH_OK;
+}
+
+namespace {
+void sleep_host_fn(void* ms) { std::this_thread::sleep_for(std::chrono::milliseconds(*static_cast<uint32_t*>(ms))); }
+}  // namespace
+
+// The bounded wait on a word that is never published: a stream held busy by
+// a host function for block_ms, a deadline of wait_ms.  The host function
+// ends by itself, so nothing is left running.
+rsh_status rsh_selfcheck_wait_deadline(uint32_t wait_ms, uint32_t block_ms, uint32_t* elapsed_ms) {
+    if (wait_ms == 0 || !elapsed_ms) return RSH_INVALID_ARG;
+    Lane L;
+    L.wait_ms = wait_ms;
+    if (hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking) != hipSuccess) return RSH_HIP_ERROR;
+    static uint32_t block;
+    block = block_ms;
+    uint32_t* word = nullptr;
+    rsh_status st = RSH_HIP_ERROR;
+    if (hipHostMalloc(reinterpret_cast<void**>(&word), 64, hipHostMallocCoherent) == hipSuccess) {
+        word[0] = 0;
+        if (hipLaunchHostFunc(L.stream, sleep_host_fn, &block) == hipSuccess) {
+            const auto t0 = std::chrono::steady_clock::now();
+            const hipError_t r = wait_word(L, word, 1u);
+            *elapsed_ms = (uint32_t)std::chrono::duration_cast<std::chrono::milliseconds>(
+                              std::chrono::steady_clock::now() - t0).count();
+            st = r == WAIT_TIMEOUT ? RSH_INTERNAL : r == hipSuccess ? RSH_OK : RSH_HIP_ERROR;
+        }
+        (void)hipStreamSynchronize(L.stream);
+        (void)hipHostFree(word);
+    }
+    (void)hipStreamDestroy(L.stream);
+    L.stream = nullptr;
+    return st;
 }
 
 rsh_status rsh_engine_load_scene(rsh_engine* e, const rsh_scene_desc* s) {
@@ -1298,7 +1370,10 @@ rsh_status rsh_engine_trace(rsh_engine* e, const float* rays, const float* tmax,
 
 rsh_status rsh_engine_read_pixels(rsh_engine* e, uint32_t* host, size_t count) {
     if (!e || !host || count > (size_t)e->W * e->H) return RSH_INVALID_ARG;
-    HIP_OK(hipMemcpy(host, e->pixels.p, count * 4, hipMemcpyDeviceToHost));
+    // on the engine's stream, after the frame's shading (page-locked host
+    // memory from rsh_host_alloc makes this one DMA transfer)
+    HIP_OK(hipMemcpyAsync(host, e->pixels.p, count * 4, hipMemcpyDeviceToHost, e->stream));
+    HIP_OK(hipStreamSynchronize(e->stream));
     return RSH_OK;
 }
 

@@ -1001,6 +1001,16 @@ hipError_t run_radix_sort(const uint32_t* keys, const uint32_t* values, uint64_t
     uint32_t* bv[2] = {sc.ping_v.as<uint32_t>(), out_values};
     const uint32_t* ik = keys;
     const uint32_t* iv = values;
+    // In place (out == in) with an odd pass count, the first pass would scatter
+    // into the array its other blocks still read: start from a copy in the
+    // ping buffers, which the first pass does not write.
+    if ((passes & 1u) && (keys == out_keys || values == out_values || keys == out_values || values == out_keys)) {
+        hipError_t e = hipMemcpyAsync(bk[0], keys, n * 4, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(bv[0], values, n * 4, hipMemcpyDeviceToDevice, s);
+        if (e != hipSuccess) return e;
+        ik = bk[0];
+        iv = bv[0];
+    }
     for (uint32_t pass = 0; pass < passes; pass++) {
         uint32_t dst = ((passes - 1 - pass) & 1u) ? 0u : 1u;
         if (rb == 9) radix_pass<9>(ik, iv, n, 9 * pass, nt, bk[dst], bv[dst], sc, s);

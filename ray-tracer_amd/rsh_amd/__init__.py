@@ -94,7 +94,7 @@ COMPAT_SIGNATURES = {
     "ShadowRayColoringWrapper": [_U, _U, _U, _F3, _P, _P],
     "ReflectionRayPreparationWrapper": [_U, _U, _P],
     "ReflectionRayIntersectionWrapper": [_P, _P, _P, _P, _P, _U, _U, _U, _U, _U, _U, _F3, _P],
-    "ReflectionRayColoringWrapper": [_P, _P, _U, _U, _U, _F3, C.c_int, _P, _P, _P, _P, _P, _P],
+    "ReflectionRayColoringWrapper": [_P, _P, _U, _U, _U, _F3, C.c_bool, _P, _P, _P, _P, _P, _P],
 }
 BINDERS = ["bindDiffuseTextureArray", "bindSpecularTextureArray", "bindFragmentPositionArray",
            "bindFragmentNormalArray", "bindTrianglePositions", "bindTriangleNormals",
@@ -109,7 +109,8 @@ NATIVE_SYMBOLS = ["rsh_engine_create", "rsh_engine_destroy", "rsh_last_error", "
                   "rsh_memset_d32", "rsh_device_synchronize", "rsh_device_count",
                   "rsh_compat_configure", "rsh_compat_set_stream", "rsh_compat_set_exit_on_error",
                   "rsh_compat_status", "rsh_compat_error", "rsh_compat_reset_status", "rsh_selfcheck_rcp",
-                  "rsh_object_spheres", "rsh_engine_update_objects"]
+                  "rsh_object_spheres", "rsh_engine_update_objects", "rsh_engine_set_wait_timeout",
+                  "rsh_selfcheck_wait_deadline"]
 
 _lib = None
 
@@ -170,6 +171,10 @@ def lib():
     L.rsh_compat_error.restype = C.c_char_p
     L.rsh_selfcheck_rcp.restype = C.c_int
     L.rsh_selfcheck_rcp.argtypes = [C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.rsh_engine_set_wait_timeout.restype = C.c_int
+    L.rsh_engine_set_wait_timeout.argtypes = [PV, u32]
+    L.rsh_selfcheck_wait_deadline.restype = C.c_int
+    L.rsh_selfcheck_wait_deadline.argtypes = [u32, u32, C.POINTER(u32)]
     for name, args in COMPAT_SIGNATURES.items():
         fn = getattr(L, name)
         fn.restype = None
@@ -190,6 +195,15 @@ def selfcheck_rcp(device=0):
     if rc != 0:
         raise RshError(f"rsh_selfcheck_rcp failed with HIP error {rc}")
     return t.value, m.value
+
+
+def selfcheck_wait_deadline(wait_ms, block_ms):
+    """(status, elapsed ms) of the bounded host wait on a word that is never
+    published, the stream held busy by a host function for block_ms
+    (rsh_selfcheck_wait_deadline)."""
+    el = u32(0)
+    st = lib().rsh_selfcheck_wait_deadline(wait_ms, block_ms, C.byref(el))
+    return st, el.value
 
 
 SPHERE_MINIMAL, SPHERE_REFERENCE = 0, 1
@@ -409,6 +423,30 @@ class Engine:
         self._check(self.L.rsh_engine_read_pixels(self.h, out.ctypes.data_as(PU), out.size), "read_pixels")
         return out.reshape(H, W)
 
+    def read_pixels_pinned(self):
+        """The frame's pixels copied into a page-locked host buffer owned by
+        this engine (rsh_host_alloc; reused while the size fits): one DMA
+        transfer on the engine's stream.  Returns an H x W view of that buffer,
+        overwritten by the next call."""
+        W, H = self.cfg.width, self.cfg.height
+        n = W * H
+        if getattr(self, "_pinned_n", 0) < n:
+            self._free_pinned()
+            self.L.rsh_host_alloc.restype = PV
+            self.L.rsh_host_alloc.argtypes = [C.c_size_t]
+            p = self.L.rsh_host_alloc(4 * n)
+            if not p:
+                raise MemoryError("page-locked host allocation failed")
+            self._pinned, self._pinned_n = p, n
+        self._check(self.L.rsh_engine_read_pixels(self.h, C.cast(self._pinned, PU), n), "read_pixels")
+        return np.ctypeslib.as_array(C.cast(self._pinned, PU), shape=(n,)).reshape(H, W)
+
+    def _free_pinned(self):
+        if getattr(self, "_pinned", None):
+            self.L.rsh_host_free.argtypes = [PV]
+            self.L.rsh_host_free(self._pinned)
+            self._pinned, self._pinned_n = None, 0
+
     def gbuffer(self):
         n = self.cfg.width * self.cfg.height
         g = {k: np.empty((n, 4), np.float32) for k in ("dif", "spec", "pos", "nrm")}
@@ -434,6 +472,7 @@ class Engine:
                 for i in range(st.pass_count)]
 
     def close(self):
+        self._free_pinned()
         if getattr(self, "h", None):
             self.L.rsh_engine_destroy(self.h)
             self.h = None

@@ -143,3 +143,53 @@ def test_batch_rejects_view_ids_that_do_not_fit(engine, rsh):
     bad = dataclasses.replace(rsh.config.config0(64, 64), segment_shift=26).with_scene_bounds(scene)   # 29-bit hash
     with pytest.raises(rsh.RshError):
         engine.render_views(views, bad)
+
+
+@pytest.mark.parametrize("variant", [dict(), dict(bounces=2, bounce_shadows=1)])
+def test_64_views_match_oracle(engine, oracle, rsh, variant):
+    """64 views in one batch, so view ids 0..63 all occur in key bits 26-31
+    (the batching workload's view count), at small sides (8-40 px) so that
+    the oracle renders them in seconds."""
+    scene = rsh.scenes.config0_scene()
+    views = random_views(rsh, 64 + len(variant), 64, 8, 40)
+    cfg = batch_cfg(rsh, scene, rsh.config.config0(64, 64), **variant)
+    engine.load_scene(scene)
+    engine.render_views(views, cfg, flags=engine.KEEP_PASSES)
+    compare_frames(engine, oracle.render_views(scene, views, cfg, threads=0), engine.cfg)
+
+
+def test_views_at_workload_sides_match_oracle(engine, oracle, rsh):
+    """Views at the batching workload's sides (128-1024 px), fewer of them."""
+    scene = rsh.scenes.config1_scene(boxes=300)
+    rng = np.random.default_rng(65)
+    views = [(rsh.scenes.random_view(rng, s, s), s, s) for s in (128, 1024, 301)]
+    cfg = batch_cfg(rsh, scene, rsh.config.config1(64, 64))
+    engine.load_scene(scene)
+    engine.render_views(views, cfg, flags=engine.KEEP_PASSES)
+    compare_frames(engine, oracle.render_views(scene, views, cfg, threads=0), engine.cfg)
+
+
+def test_full_64_view_batch_equals_plain_frames(engine, rsh):
+    """The batching workload itself (64 square views of side 128-1024 at
+    random poses, 23.5M pixels, office-scale scene) against the plain
+    pipeline, one frame per view: pixels identical but for at most 1e-4 of
+    them, and camera-pass hit triangles likewise."""
+    scene = rsh.scenes.config1_scene()
+    views = rsh.scenes.batch_views()
+    cfg = batch_cfg(rsh, scene, rsh.config.config1(64, 64))
+    engine.load_scene(scene)
+    engine.render_views(views, cfg, flags=engine.KEEP_PASSES)
+    batch_px = engine.pixels().reshape(-1).copy()
+    batch_hits = engine.pass_hits(0)
+    bad_px = bad_hit = total = 0
+    off = 0
+    for cam, w, h in views:
+        engine.render(cam, dataclasses.replace(cfg, width=w, height=h), flags=engine.KEEP_PASSES)
+        bad_px += int((engine.pixels().reshape(-1) != batch_px[off:off + w * h]).sum())
+        a = engine.pass_hits(0) & np.uint64(0xFFFFFFFF)
+        bad_hit += int((a != (batch_hits[off:off + w * h] & np.uint64(0xFFFFFFFF))).sum())
+        total += w * h
+        off += w * h
+    assert total == 23_543_814
+    assert bad_px <= 1e-4 * total, bad_px
+    assert bad_hit <= 1e-4 * total, bad_hit

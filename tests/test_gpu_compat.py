@@ -247,7 +247,7 @@ def test_reference_frame_through_wrappers(rsh, oracle, setup, algorithm):
         oracle.call("oracle_ReflectionRayPreparationWrapper", W, H, sflags.host)
         traverse(N, False)
         eq(sflags, None, f"closest-hit records, bounce {it}")
-        L.ReflectionRayColoringWrapper(tri.dev.ptr, unrm.dev.ptr, W, H, scene.light_total, f3(eye), int(create),
+        L.ReflectionRayColoringWrapper(tri.dev.ptr, unrm.dev.ptr, W, H, scene.light_total, f3(eye), bool(create),
                                        sflags.dev.ptr, rays.dev.ptr, flags.dev.ptr, pk.dev.ptr, pv.dev.ptr,
                                        pixels.dev.ptr)
         oracle.call("oracle_ReflectionRayColoringWrapper", tri.host, unrm.host, W, H, scene.light_total, eye,
@@ -322,3 +322,23 @@ def test_capacity_guard(rsh, setup):
                                       dummy.ptr, dummy.ptr, C.byref(ht), C.byref(mem))
     assert L.rsh_compat_status() == 2
     L.rsh_compat_reset_status()
+
+
+@pytest.mark.parametrize("bits", [9, 16, 24, 27, 32])
+def test_sort_pairs_in_place(rsh, bits):
+    """rsh_sort_pairs with the outputs aliasing the inputs, as the reference's
+    buffer reuse allows: odd pass counts (9, 24 and 27 bits) start from a copy
+    instead of scattering into the array other blocks still read."""
+    L = rsh.lib()
+    L.rsh_compat_set_exit_on_error(0)
+    L.rsh_compat_reset_status()
+    n = 300_001
+    rng = np.random.default_rng(bits)
+    keys = rng.integers(0, 1 << bits, n, dtype=np.uint64).astype(np.uint32)
+    vals = np.arange(n, dtype=np.uint32)
+    dk, dv = rsh.DeviceArray.from_host(keys), rsh.DeviceArray.from_host(vals)
+    L.rsh_sort_pairs(dk.ptr, dv.ptr, n, bits, dk.ptr, dv.ptr)
+    assert L.rsh_compat_status() == 0, L.rsh_compat_error()
+    order = np.argsort(keys, kind="stable")
+    np.testing.assert_array_equal(dk.download(), keys[order])
+    np.testing.assert_array_equal(dv.download(), vals[order])

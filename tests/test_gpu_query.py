@@ -23,10 +23,23 @@ def qcfg(rsh, scene, **kw):
                                **kw).with_scene_bounds(scene)
 
 
+_SCENES = {}
+
+
+def named_scene(rsh, name):
+    if name not in _SCENES:
+        _SCENES[name] = {"config0": lambda: rsh.scenes.config0_scene(),
+                         "config1": lambda: rsh.scenes.config1_scene(boxes=200),
+                         "soup": lambda: rsh.scenes.soup_scene()}[name]()
+    return _SCENES[name]
+
+
 @pytest.mark.parametrize("mix", [queries.PRIMARY, queries.SHADOW, queries.DIFFUSE])
-@pytest.mark.parametrize("scene_name", ["config0", "config1"])
+@pytest.mark.parametrize("scene_name", ["config0", "config1", "soup"])
 def test_query_matches_oracle(engine, oracle, rsh, mix, scene_name):
-    scene = rsh.scenes.config0_scene() if scene_name == "config0" else rsh.scenes.config1_scene(boxes=200)
+    """Every query kind of the serving mix, including the serving workload's
+    100k-triangle random soup."""
+    scene = named_scene(rsh, scene_name)
     rng = np.random.default_rng(100 + mix)
     rays, tmax, kind = queries.make_query(scene, mix, 20000, rng)
     cfg = qcfg(rsh, scene)

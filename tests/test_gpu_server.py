@@ -89,3 +89,33 @@ def test_admission_under_small_budget(rsh):
     with pytest.raises(rsh.RshError):
         srv.submit(sid, np.zeros((50000, 6), np.float32), 0)          # larger than the budget
     srv.close()
+
+
+def test_alone_equals_mixed_four_scenes(rsh):
+    """The serving workload's four resident scenes (512 scene, office-scale
+    boxes, the 262k-triangle atrium, the 100k-triangle soup), requests of
+    1k-60k rays plus one of 1.2M rays on the atrium: each result is the same
+    alone and in the mix."""
+    from rsh_amd.server import Server
+    scenes = [rsh.scenes.config0_scene(), rsh.scenes.config1_scene(), rsh.scenes.config2_scene(),
+              rsh.scenes.soup_scene()]
+    srv = Server(0, hbm_budget=16 << 30, max_batch_rays=1 << 22)
+    sids = [srv.add_scene(s, scfg(rsh, s)) for s in scenes]
+    reqs = make_requests(rsh, scenes, 20, 2)
+    rng = np.random.default_rng(22)
+    big = queries.make_query(scenes[2], queries.DIFFUSE, 1_200_000, rng)
+    reqs.insert(7, (2, big[0], big[1], big[2]))
+    assert {s for s, _, _, _ in reqs} == {0, 1, 2, 3}
+    alone = []
+    for s, rays, tmax, kind in reqs:
+        res, info = srv.wait(srv.submit(sids[s], rays, kind, tmax))
+        alone.append(res.copy())
+    ids = [srv.submit(sids[s], rays, kind, tmax) for s, rays, tmax, kind in reqs]
+    merged = 0
+    for rid, (s, rays, tmax, kind), a in zip(ids, reqs, alone):
+        res, info = srv.wait(rid)
+        merged = max(merged, info.batch_requests)
+        same = (hit_ids(res, kind) == hit_ids(a, kind)).mean()
+        assert same >= 0.9999, (rid, s, len(rays), same)
+    assert merged > 1
+    srv.close()

@@ -193,7 +193,9 @@ rsh_status rsh_object_spheres(const float* tri_pos, uint32_t triangle_total, con
  *   bit 5  serial passes: each shadow pass is issued as soon as its hits are,
  *          one pass at a time (the timings of bits 2 and 4 are then free of
  *          the other chain's kernels)
- *   bit 3  graph replay (alone, without bits 0-2): the first call for a given
+ *   bit 6  keep every pass's keys, sort order, packets and hierarchy nodes for
+ *          rsh_engine_read_pass_order (a debug read-back for parity checks)
+ *   bit 3  graph replay (alone, without bits 0-2, 4, 6): the first call for a given
  *          set of views, configuration and scene renders eagerly and captures
  *          a HIP graph of the frame (every pass's sizes fixed to the eager
  *          frame's); later identical calls replay that graph with no host
@@ -244,6 +246,18 @@ rsh_status rsh_engine_read_gbuffer(rsh_engine* e, float* host_dif, float* host_s
 /* Per-slot records of pass i (closest: exact or reference packing; shadow:
  * 0 or 0xFFFFFFFF), widened to uint64.  Needs flag bit 0 at render. */
 rsh_status rsh_engine_read_pass(rsh_engine* e, uint32_t pass, uint64_t* host, size_t count);
+/* Pass i's hash / sort / compress state, kept with flag bit 6 (host buffers,
+ * any may be NULL): the slot keys and flags as ray creation wrote them
+ * (slots; none for camera rays created in key order), the sorted order
+ * (for each of the rays sorted rays, its slot: the permutation the
+ * hierarchy is built over), the packets as compression found them (chunk
+ * first positions in the trimmed array and chunk keys; CRSH passes that
+ * compress), and the hierarchy nodes (node_count float4: sphere at 2g, cone
+ * at 2g+1, level 0 first).  The reference's counterparts are the buffers
+ * its wrappers fill (RayTracer.cu:2276-2535). */
+rsh_status rsh_engine_read_pass_order(rsh_engine* e, uint32_t pass, uint32_t* keys, uint32_t* flags, size_t slots,
+                                      uint32_t* order, size_t rays, uint32_t* bases, uint32_t* ckeys, size_t chunks,
+                                      float* nodes, size_t node_count);
 /* Device pointer of the pixel buffer (uint32[W*H], RGBA8 R|G<<8|B<<16). */
 void* rsh_engine_pixels_device(rsh_engine* e);
 

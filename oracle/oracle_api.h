@@ -84,6 +84,13 @@ typedef struct oracle_frame_out {
     uint64_t pass_ray_tests[16];    // ray-triangle tests performed
     double stage_ms[8];             // steady_clock ms per stage: create, trim, compress, sort,
                                     // decompress, hierarchy, traversal, shading
+    // Optional per-pass state (capacity S slots each; nodes 8*S floats):
+    uint32_t* pass_keys[16];        // slot keys as created
+    uint32_t* pass_flags[16];       // slot flags (0 live, 1 empty)
+    uint32_t* pass_order[16];       // slot of each sorted ray (rays entries)
+    uint32_t* pass_bases[16];       // chunk first positions in the trimmed array (chunks entries)
+    uint32_t* pass_ckeys[16];       // chunk keys (chunks entries)
+    float* pass_nodes[16];          // hierarchy nodes, float4: sphere at 2g, cone at 2g+1
 } oracle_frame_out;
 
 int oracle_render(const oracle_scene* scene, const oracle_camera* camera,

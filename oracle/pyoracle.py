@@ -48,7 +48,9 @@ class OFrameOut(C.Structure):
                 ("gbuffer_spec", PF), ("pass_hits", PU64 * 16), ("pass_count", u32),
                 ("pass_kind", u32 * 16), ("pass_slots", u32 * 16), ("pass_rays", u32 * 16),
                 ("pass_chunks", u32 * 16), ("pass_pairs_top", u64 * 16), ("pass_pairs_leaf", u64 * 16),
-                ("pass_ray_tests", u64 * 16), ("stage_ms", C.c_double * 8)]
+                ("pass_ray_tests", u64 * 16), ("stage_ms", C.c_double * 8),
+                ("pass_keys", PU * 16), ("pass_flags", PU * 16), ("pass_order", PU * 16),
+                ("pass_bases", PU * 16), ("pass_ckeys", PU * 16), ("pass_nodes", PF * 16)]
 
 
 REF_SRC = "/root/reference/projects/CUDA-Engine/source"
@@ -225,13 +227,38 @@ class Frame:
         self.stage_ms = stage_ms
 
 
-def render(scene, camera, cfg, gbuffer=None, threads=0, keep_hits=True, keep_gbuffer=True, row_range=None) -> Frame:
+def _order_buffers(out, S):
+    """Per-pass keys, flags, sorted order, packets and nodes (capacity S)."""
+    bufs = []
+    for i in range(16):
+        b = {k: np.zeros(S, np.uint32) for k in ("keys", "flags", "order", "bases", "ckeys")}
+        b["nodes"] = np.zeros((2 * S, 4), np.float32)
+        out.pass_keys[i], out.pass_flags[i], out.pass_order[i] = up(b["keys"]), up(b["flags"]), up(b["order"])
+        out.pass_bases[i], out.pass_ckeys[i], out.pass_nodes[i] = up(b["bases"]), up(b["ckeys"]), fp(b["nodes"])
+        bufs.append(b)
+    return bufs
+
+
+def _attach_order(passes, bufs):
+    for p, b in zip(passes, bufs):
+        p["keys"], p["flags"] = b["keys"][:p["slots"]], b["flags"][:p["slots"]]
+        p["order"] = b["order"][:p["rays"]]
+        p["bases"], p["ckeys"] = b["bases"][:p["chunks"]], b["ckeys"][:p["chunks"]]
+        p["nodes"] = b["nodes"]
+
+
+def render(scene, camera, cfg, gbuffer=None, threads=0, keep_hits=True, keep_gbuffer=True, row_range=None,
+           keep_order=False) -> Frame:
+    """One oracle frame.  keep_order: every pass's slot keys and flags, sorted
+    order, packets (chunk bases and keys) and hierarchy nodes in
+    passes[i]["keys" / "flags" / "order" / "bases" / "ckeys" / "nodes"]."""
     L = lib()
     W, H = cfg.width, cfg.height
     S = W * H * max(1, scene.light_pos.shape[0]) * (16 if getattr(cfg, "soft_shadows", 0) else 1)
     pixels = np.zeros(W * H, np.uint32)
     out = OFrameOut()
     out.pixels = up(pixels)
+    obufs = _order_buffers(out, S) if keep_order else None
     gb = None
     if keep_gbuffer:
         gb = {k: np.zeros((W * H, 4), np.float32) for k in ("pos", "nrm", "dif", "spec")}
@@ -262,6 +289,8 @@ def render(scene, camera, cfg, gbuffer=None, threads=0, keep_hits=True, keep_gbu
                            chunks=out.pass_chunks[i], pairs_top=out.pass_pairs_top[i],
                            pairs_leaf=out.pass_pairs_leaf[i], ray_tests=out.pass_ray_tests[i],
                            hits=hits[i][:out.pass_slots[i]].copy() if keep_hits else None))
+    if obufs:
+        _attach_order(passes, obufs)
     return Frame(pixels.reshape(H, W), gb, passes, dict(zip(STAGES, list(out.stage_ms))))
 
 

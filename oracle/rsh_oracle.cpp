@@ -375,7 +375,8 @@ void trim(const Batch& b, std::vector<uint32_t>& tk, std::vector<uint32_t>& tv) 
 
 // RT:886-991 (compress), RT:2401 (stable radix sort), RT:993-1059 (decompress).
 void compress_sort_decompress(const std::vector<uint32_t>& tk, const std::vector<uint32_t>& tv,
-                              std::vector<uint32_t>& sk, std::vector<uint32_t>& sv, uint32_t* chunks) {
+                              std::vector<uint32_t>& sk, std::vector<uint32_t>& sv, uint32_t* chunks,
+                              uint32_t* out_bases = nullptr, uint32_t* out_ckeys = nullptr) {
     const uint32_t N = (uint32_t)tk.size();
     StageClock clk(SC_COMPRESS);
     std::vector<uint32_t> bases, sizes, ckeys, cvals;
@@ -384,6 +385,8 @@ void compress_sort_decompress(const std::vector<uint32_t>& tk, const std::vector
     const uint32_t C = (uint32_t)bases.size();
     sizes.resize(C);
     for (uint32_t c = 0; c < C; c++) { sizes[c] = (c + 1 < C ? bases[c + 1] : N) - bases[c]; cvals.push_back(c); }
+    if (out_bases) std::copy(bases.begin(), bases.end(), out_bases);
+    if (out_ckeys) std::copy(ckeys.begin(), ckeys.end(), out_ckeys);
     clk.lap(SC_SORT);
     std::vector<uint32_t> perm(C);
     std::iota(perm.begin(), perm.end(), 0u);
@@ -511,11 +514,24 @@ struct Traverser {
 // Build one batch's hierarchy and run traversal + intersection.
 // kind 0: closest hit (RT:1711-1774), kind 1: shadow any-hit (RT:1498-1614).
 // tmax (any hit, optional): the caller's per-slot limit instead of |o - P| - 2 eps.
+// Where a pass's keys, flags, sorted order, packets and hierarchy nodes go
+// (oracle_frame_out's pass_keys ... pass_nodes; NULL: not wanted).
+struct PassDump {
+    uint32_t* keys = nullptr;
+    uint32_t* flags = nullptr;
+    uint32_t* order = nullptr;
+    uint32_t* bases = nullptr;
+    uint32_t* ckeys = nullptr;
+    float* nodes = nullptr;
+};
+
 void trace(const Scene& sc, const oracle_config& cfg, const Batch& b, uint32_t kind,
            const GBuf* shadow_g, uint32_t Lmax, std::vector<uint64_t>& result, PassStats& st,
-           int threads, const float* tmax = nullptr) {
+           int threads, const float* tmax = nullptr, const PassDump* dump = nullptr) {
     (void)threads;
     std::vector<uint32_t> tk, tv, sk, sv;
+    if (dump && dump->keys) std::copy(b.keys.begin(), b.keys.end(), dump->keys);
+    if (dump && dump->flags) std::copy(b.flags.begin(), b.flags.end(), dump->flags);
     {
         StageClock clk(SC_TRIM);
         trim(b, tk, tv);
@@ -523,7 +539,8 @@ void trace(const Scene& sc, const oracle_config& cfg, const Batch& b, uint32_t k
         result.assign(b.S, kind == 0 ? (cfg.hit_mode == 0 ? MISS64 : (uint64_t)MISS32) : 0ull);
     }
     if (st.N == 0) return;
-    if (cfg.algorithm == 0) compress_sort_decompress(tk, tv, sk, sv, &st.C);
+    if (cfg.algorithm == 0)
+        compress_sort_decompress(tk, tv, sk, sv, &st.C, dump ? dump->bases : nullptr, dump ? dump->ckeys : nullptr);
     else { sk = tk; sv = tv; st.C = st.N; }
     Hier h;
     {
@@ -531,6 +548,9 @@ void trace(const Scene& sc, const oracle_config& cfg, const Batch& b, uint32_t k
         build_hierarchy(b, sv, st.N, cfg.leaf, cfg.branch, cfg.depth, cfg.initial_radius, cfg.initial_spread,
                         cfg.cone_mode, h);
     }
+    if (dump && dump->order) std::copy(sv.begin(), sv.end(), dump->order);
+    if (dump && dump->nodes)
+        for (size_t i = 0; i < h.nodes.size(); i++) st4(dump->nodes + 4 * i, h.nodes[i]);
     StageClock clk(SC_TRAVERSE);
     Traverser tr{sc, h, cfg.algorithm, cfg.branch, cfg.triangle_batch};
     const uint32_t N = st.N, leaf = cfg.leaf, W = cfg.width;
@@ -986,6 +1006,16 @@ void hier_from_arrays(const float* hierarchy, const uint32_t* counts, uint32_t d
     }
 }
 
+// The dump targets of the pass record_pass will record next.
+PassDump dump_for(const oracle_frame_out* out) {
+    PassDump d;
+    if (!out || out->pass_count >= 16) return d;
+    const uint32_t i = out->pass_count;
+    d.keys = out->pass_keys[i]; d.flags = out->pass_flags[i]; d.order = out->pass_order[i];
+    d.bases = out->pass_bases[i]; d.ckeys = out->pass_ckeys[i]; d.nodes = out->pass_nodes[i];
+    return d;
+}
+
 void record_pass(oracle_frame_out* out, uint32_t kind, uint32_t S, const PassStats& st,
                  const std::vector<uint64_t>& res) {
     if (!out || out->pass_count >= 16) return;
@@ -1090,7 +1120,8 @@ int render_views(const oracle_scene* scene, const ViewSet& vs, const oracle_conf
         create_primary(cfg, vs, b);
         std::vector<uint64_t> rec;
         PassStats st;
-        trace(sc, cfg, b, 0, nullptr, 1, rec, st, threads);
+        const PassDump dp = dump_for(out);
+        trace(sc, cfg, b, 0, nullptr, 1, rec, st, threads, nullptr, &dp);
         record_pass(out, 0, b.S, st, rec);
         surface_from_hits(sc, cfg, b, rec, 1, true, g0);
     } else {
@@ -1114,7 +1145,8 @@ int render_views(const oracle_scene* scene, const ViewSet& vs, const oracle_conf
         create_shadow(sc, cfg, g0, Lmax, vs, b);
         std::vector<uint64_t> flags;
         PassStats st;
-        trace(sc, cfg, b, 1, &g0, Lmax, flags, st, threads);
+        const PassDump dp = dump_for(out);
+        trace(sc, cfg, b, 1, &g0, Lmax, flags, st, threads, nullptr, &dp);
         record_pass(out, 1, b.S, st, flags);
         if (st.N > 0) shade_primary(sc, g0, flags, Lmax, Lt, vs, cfg.soft_shadows != 0, pixels.data());
     }
@@ -1127,7 +1159,8 @@ int render_views(const oracle_scene* scene, const ViewSet& vs, const oracle_conf
         else create_bounce(cfg, gprev, prev, prev_rec, Lmax, vs, b);
         std::vector<uint64_t> rec;
         PassStats st;
-        trace(sc, cfg, b, 0, nullptr, Lmax, rec, st, threads);
+        const PassDump dp = dump_for(out);
+        trace(sc, cfg, b, 0, nullptr, Lmax, rec, st, threads, nullptr, &dp);
         record_pass(out, 2, b.S, st, rec);
         if (st.N == 0) break;
         GBuf gb;
@@ -1138,7 +1171,8 @@ int render_views(const oracle_scene* scene, const ViewSet& vs, const oracle_conf
             Batch sbt;
             create_shadow(sc, cfg, gb, Lmax, vs, sbt);
             PassStats sst;
-            trace(sc, cfg, sbt, 1, &gb, Lmax, sflags, sst, threads);
+            const PassDump dps = dump_for(out);
+            trace(sc, cfg, sbt, 1, &gb, Lmax, sflags, sst, threads, nullptr, &dps);
             record_pass(out, 1, sbt.S, sst, sflags);
             have_flags = true;
         }

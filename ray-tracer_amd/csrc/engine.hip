@@ -114,7 +114,14 @@ struct rsh_engine {
     DevBuf g0[4], gb[2][4];
 
     // kept passes (flag bit 0), by pass index
-    struct Kept { DevBuf buf; uint32_t kind = 0; uint32_t slots = 0; bool set = false; };
+    struct Kept {
+        DevBuf buf; uint32_t kind = 0; uint32_t slots = 0; bool set = false;
+        // flag bit 6: the pass's slot keys and flags, sorted order (the slot of
+        // each sorted ray), chunk bases and keys, hierarchy nodes
+        DevBuf keys, flags, rec, bases, ckeys, nodes;
+        uint32_t o_slots = 0, o_rays = 0, o_chunks = 0, o_nodes = 0;
+        bool have_keys = false, have_chunks = false, order_set = false;
+    };
     std::vector<Kept> kept;
 };
 
@@ -609,6 +616,30 @@ rsh_status trace_pass(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, const
                          counts, offs, s);
         LANE_OK(hipGetLastError());
     }
+    if ((flags_mask & 64u) && pass_idx < e->kept.size()) {
+        // debug read-back of the pass's keys, order, packets and nodes
+        rsh_engine::Kept& k = e->kept[pass_idx];
+        uint32_t nn = 0;
+        for (uint32_t d = 0; d < cfg.depth; d++) nn += counts[d];
+        k.o_slots = S; k.o_rays = N; k.o_chunks = out.C; k.o_nodes = 2 * nn;
+        k.have_keys = !presorted;
+        k.have_chunks = !presorted && cfg.algorithm == RSH_CRSH && !all_unique;
+        auto keep = [&](DevBuf& dst, const void* src, size_t bytes) -> hipError_t {
+            hipError_t r = dst.reserve(bytes);
+            return r == hipSuccess ? hipMemcpyAsync(dst.p, src, bytes, hipMemcpyDeviceToDevice, s) : r;
+        };
+        if (k.have_keys) {
+            LANE_OK(keep(k.keys, L.keys.p, (size_t)S * 4));
+            LANE_OK(keep(k.flags, L.flags.p, (size_t)S * 4));
+        }
+        if (k.have_chunks) {
+            LANE_OK(keep(k.bases, L.bases.p, (size_t)out.C * 4));
+            LANE_OK(keep(k.ckeys, L.ckeys.p, (size_t)out.C * 4));
+        }
+        LANE_OK(keep(k.rec, L.rec0.p, (size_t)N * 16));
+        LANE_OK(keep(k.nodes, L.nodes.p, (size_t)2 * nn * 16));
+        k.order_set = true;
+    }
     TravParams p{};
     p.nodes = f4p(L.nodes); p.aux = L.aux.as<float2>();
     for (int k = 0; k < 4; k++) { p.level_offset[k] = offs[k]; p.level_count[k] = counts[k]; }
@@ -957,7 +988,7 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
     if (!e->replaying) {
         for (auto& k : e->kept) k.buf.release();
         e->kept.clear();
-        if (fl & 1u) e->kept.resize(RSH_MAX_PASSES);
+        if (fl & 65u) e->kept.resize(RSH_MAX_PASSES);
     }
     for (Lane& l : e->lane) { l.timed.clear(); l.ev_used = 0; l.err.clear(); }
     if (st) std::memset(st, 0, sizeof(*st));
@@ -1171,9 +1202,9 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
         st->pass_count = last_idx + 1 < RSH_MAX_PASSES ? last_idx + 1 : RSH_MAX_PASSES;
         for (uint32_t i = 0; i < st->pass_count; i++) st->rays_total += st->pass_rays[i];
     }
-    if (!e->replaying && (fl & 1u)) {
+    if (!e->replaying && (fl & 65u)) {
         uint32_t n = 0;
-        while (n < e->kept.size() && e->kept[n].set) n++;
+        while (n < e->kept.size() && (e->kept[n].set || e->kept[n].order_set)) n++;
         for (size_t i = n; i < e->kept.size(); i++) e->kept[i].buf.release();
         e->kept.resize(n);
     }
@@ -1220,7 +1251,10 @@ void drop_graph(rsh_engine* e) {
 rsh_status render(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rsh_frame_config& cfg, uint32_t fl,
                   rsh_frame_stats* st) {
     if (!(fl & 8u)) return render_frame(e, hv, nv, cfg, fl, st);
-    if (fl & 23u) { e->err = "render: graph replay keeps no passes, counters or timers (flag bits 0-2, 4)"; return RSH_INVALID_ARG; }
+    if (fl & 87u) {
+        e->err = "render: graph replay keeps no passes, orders, counters or timers (flag bits 0-2, 4, 6)";
+        return RSH_INVALID_ARG;
+    }
     if (e->stream == nullptr) { e->err = "render: graph replay needs a non-null stream"; return RSH_INVALID_ARG; }
     std::vector<uint8_t> sig(sizeof(cfg) + sizeof(e->scene_epoch) + (size_t)nv * sizeof(ViewDesc));
     std::memcpy(sig.data(), &cfg, sizeof(cfg));
@@ -1395,6 +1429,28 @@ rsh_status rsh_engine_read_pass(rsh_engine* e, uint32_t pass, uint64_t* host, si
         for (size_t i = 0; i < count; i++) host[i] = tmp[i];
     } else {
         HIP_OK(hipMemcpy(host, k.buf.p, count * 8, hipMemcpyDeviceToHost));
+    }
+    return RSH_OK;
+}
+
+rsh_status rsh_engine_read_pass_order(rsh_engine* e, uint32_t pass, uint32_t* keys, uint32_t* flags, size_t slots,
+                                      uint32_t* order, size_t rays, uint32_t* bases, uint32_t* ckeys, size_t chunks,
+                                      float* nodes, size_t node_count) {
+    if (!e || pass >= e->kept.size() || !e->kept[pass].order_set) return RSH_INVALID_ARG;
+    const auto& k = e->kept[pass];
+    if ((keys || flags) && (!k.have_keys || slots > k.o_slots)) return RSH_INVALID_ARG;
+    if ((bases || ckeys) && (!k.have_chunks || chunks > k.o_chunks)) return RSH_INVALID_ARG;
+    if ((order && rays > k.o_rays) || (nodes && node_count > k.o_nodes)) return RSH_INVALID_ARG;
+    HIP_OK(hipStreamSynchronize(e->stream));
+    if (keys) HIP_OK(hipMemcpy(keys, k.keys.p, slots * 4, hipMemcpyDeviceToHost));
+    if (flags) HIP_OK(hipMemcpy(flags, k.flags.p, slots * 4, hipMemcpyDeviceToHost));
+    if (bases) HIP_OK(hipMemcpy(bases, k.bases.p, chunks * 4, hipMemcpyDeviceToHost));
+    if (ckeys) HIP_OK(hipMemcpy(ckeys, k.ckeys.p, chunks * 4, hipMemcpyDeviceToHost));
+    if (nodes) HIP_OK(hipMemcpy(nodes, k.nodes.p, node_count * 16, hipMemcpyDeviceToHost));
+    if (order) {
+        std::vector<float> rec(rays * 4);
+        HIP_OK(hipMemcpy(rec.data(), k.rec.p, rays * 16, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < rays; i++) std::memcpy(order + i, &rec[4 * i + 3], 4);
     }
     return RSH_OK;
 }

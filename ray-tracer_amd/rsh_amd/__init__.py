@@ -110,7 +110,7 @@ NATIVE_SYMBOLS = ["rsh_engine_create", "rsh_engine_destroy", "rsh_last_error", "
                   "rsh_compat_configure", "rsh_compat_set_stream", "rsh_compat_set_exit_on_error",
                   "rsh_compat_status", "rsh_compat_error", "rsh_compat_reset_status", "rsh_selfcheck_rcp",
                   "rsh_object_spheres", "rsh_engine_update_objects", "rsh_engine_set_wait_timeout",
-                  "rsh_selfcheck_wait_deadline"]
+                  "rsh_selfcheck_wait_deadline", "rsh_engine_read_pass_order"]
 
 _lib = None
 
@@ -323,7 +323,7 @@ class RshError(RuntimeError):
 class Engine:
     """Device-resident scene + frame renderer (include/rsh.h)."""
 
-    KEEP_PASSES, COUNTERS, TIMING, GRAPH, TRAV_TIMING, SERIAL = 1, 2, 4, 8, 16, 32
+    KEEP_PASSES, COUNTERS, TIMING, GRAPH, TRAV_TIMING, SERIAL, KEEP_ORDER = 1, 2, 4, 8, 16, 32, 64
 
     def __init__(self, device: int = 0):
         self.L = lib()
@@ -458,6 +458,35 @@ class Engine:
         n = self.stats.pass_slots[i]
         out = np.empty(n, np.uint64)
         self._check(self.L.rsh_engine_read_pass(self.h, i, out.ctypes.data_as(PU64), n), "read_pass")
+        return out
+
+    def pass_order(self, i):
+        """Pass i's keys, flags, sorted order, packets and hierarchy nodes
+        (render flag KEEP_ORDER): a dict with "order" (slot of each sorted
+        ray), "nodes" (float32 [2 * nodes, 4]), and "keys" / "flags" (per slot)
+        and "bases" / "ckeys" (per chunk) where the pass has them."""
+        st = self.stats
+        S, N, Cn = int(st.pass_slots[i]), int(st.pass_rays[i]), int(st.pass_chunks[i])
+        nn = 0
+        n = (N + self.cfg.leaf - 1) // self.cfg.leaf
+        for _ in range(self.cfg.depth):
+            nn += n
+            n = (n + self.cfg.branch - 1) // self.cfg.branch
+        L = self.L
+        L.rsh_engine_read_pass_order.restype = C.c_int
+        L.rsh_engine_read_pass_order.argtypes = [PV, u32, PV, PV, C.c_size_t, PV, C.c_size_t, PV, PV, C.c_size_t,
+                                                 PV, C.c_size_t]
+        out = {"order": np.empty(N, np.uint32), "nodes": np.empty((2 * nn, 4), np.float32)}
+        self._check(L.rsh_engine_read_pass_order(self.h, i, None, None, 0, out["order"].ctypes.data, N, None, None, 0,
+                                                 out["nodes"].ctypes.data, 2 * nn), "read_pass_order")
+        keys, flags = np.empty(S, np.uint32), np.empty(S, np.uint32)
+        if L.rsh_engine_read_pass_order(self.h, i, keys.ctypes.data, flags.ctypes.data, S, None, 0, None, None, 0,
+                                        None, 0) == 0:
+            out["keys"], out["flags"] = keys, flags
+        bases, ckeys = np.empty(Cn, np.uint32), np.empty(Cn, np.uint32)
+        if L.rsh_engine_read_pass_order(self.h, i, None, None, 0, None, 0, bases.ctypes.data, ckeys.ctypes.data, Cn,
+                                        None, 0) == 0:
+            out["bases"], out["ckeys"] = bases, ckeys
         return out
 
     def passes(self):

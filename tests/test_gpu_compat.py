@@ -46,8 +46,13 @@ def setup(rsh, oracle):
     return scene, cam, ofr.gbuffer, W, H
 
 
+@pytest.mark.parametrize("alias", [False, True])
 @pytest.mark.parametrize("algorithm", [0, 1])
-def test_reference_frame_through_wrappers(rsh, oracle, setup, algorithm):
+def test_reference_frame_through_wrappers(rsh, oracle, setup, algorithm, alias):
+    """alias: the shadow flags and closest-hit records live in the chunk-key
+    buffer, as the reference's frame driver passes cudaPrimaryChunkKeysArrayDP
+    to the preparation, intersection and coloring wrappers
+    (main.cpp:674-677, 749, 823, 896) between two compressions."""
     L = rsh.lib()
     scene, cam, gb, W, H = setup
     T = scene.triangle_total
@@ -123,7 +128,7 @@ def test_reference_frame_through_wrappers(rsh, oracle, setup, algorithm):
     hits, thits = Twin(rsh, hit_mem), Twin(rsh, hit_mem)
     nodes_total = n0 + (n0 + SUB - 1) // SUB
     hier = Twin(rsh, (2 * nodes_total, 4), np.float32)
-    sflags = Twin(rsh, S)
+    sflags = ckeys if alias else Twin(rsh, S)
     pixels = Twin(rsh, W * H)
     L.MemoryPreparationWrapper(W, H, hit_mem, flags.dev.ptr, scan.dev.ptr, ckeys.dev.ptr, cvals.dev.ptr,
                                sckeys.dev.ptr, scvals.dev.ptr)

@@ -337,3 +337,46 @@ def test_cornell_layout_matches_oracle(engine, oracle, rsh, tmp_path, key_mode, 
     scene = objfile.cornell_scene(str(tmp_path), reflective=lambda k: k in (3, 6))
     cfg = dataclasses.replace(rsh.config.config1(128, 96), reflect_gate=gate, key_mode=key_mode)
     run_case(engine, oracle, rsh, scene, objfile.cornell_camera(128, 96), cfg)
+
+
+@pytest.mark.parametrize("which,variant", [("c0", dict()), ("c0", dict(key_mode=0)), ("c0", dict(algorithm=1)),
+                                           ("c0", dict(soft_shadows=1)), ("atrium", dict())])
+def test_pass_keys_order_packets_nodes(engine, oracle, rsh, which, variant):
+    """Per pass, the native engine's hash keys (every live slot), the stable
+    sort permutation (slot of each sorted ray), the packets (chunk first
+    positions and keys) and the hierarchy nodes, against the oracle bit for
+    bit: the 512 scene at 128 x 128 and a 192 x 108 slice of the
+    262k-triangle atrium frame (3 bounces with shadows)."""
+    if which == "c0":
+        scene, cam = c0(rsh, 128)
+        cfg = dataclasses.replace(rsh.config.config0(128, 128), **variant)
+    else:
+        scene, cam = rsh.scenes.config2_scene(), rsh.scenes.config2_camera(192, 108)
+        cfg = dataclasses.replace(rsh.config.config2(192, 108), **variant)
+    cfg = cfg.with_scene_bounds(scene)
+    engine.load_scene(scene)
+    engine.render(cam, cfg, flags=engine.KEEP_ORDER)
+    ofr = oracle.render(scene, cam, cfg, keep_order=True, keep_hits=False)
+    assert engine.stats.pass_count == len(ofr.passes)
+    checked_keys = checked_chunks = 0
+    for i, op in enumerate(ofr.passes):
+        assert engine.stats.pass_rays[i] == op["rays"] and engine.stats.pass_chunks[i] == op["chunks"], i
+        if op["rays"] == 0:
+            continue
+        g = engine.pass_order(i)
+        np.testing.assert_array_equal(g["order"], op["order"], err_msg=f"sort permutation, pass {i}")
+        nn = g["nodes"].shape[0]
+        np.testing.assert_array_equal(g["nodes"].view(np.uint32), op["nodes"][:nn].view(np.uint32),
+                                      err_msg=f"hierarchy nodes, pass {i}")
+        if "keys" in g:
+            np.testing.assert_array_equal(g["flags"], op["flags"], err_msg=f"slot flags, pass {i}")
+            live = op["flags"] == 0
+            np.testing.assert_array_equal(g["keys"][live], op["keys"][live], err_msg=f"keys, pass {i}")
+            checked_keys += 1
+        if "bases" in g:
+            np.testing.assert_array_equal(g["bases"], op["bases"], err_msg=f"packet bases, pass {i}")
+            np.testing.assert_array_equal(g["ckeys"], op["ckeys"], err_msg=f"packet keys, pass {i}")
+            checked_chunks += 1
+    assert checked_keys >= len(ofr.passes) - 1
+    if cfg.algorithm == 0:
+        assert checked_chunks >= len(ofr.passes) - 1

@@ -101,6 +101,19 @@ void RayCompressionWrapper(unsigned int* trimmedRayIndexKeysArray,
    a frame's shadow and reflection passes (27 of their 29 key bits vary). */
 void rsh_sort_pairs(const unsigned int* keys, const unsigned int* values, const unsigned int n,
                     const unsigned int bits, unsigned int* out_keys, unsigned int* out_values);
+/* No counterpart in the reference.  For callers that cannot allocate (code
+   under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc): rsh_sort_pairs with the sort's scratch in a file-scope device
+   array (17 MiB per GPU, held until the process ends) instead of library
+   allocations.  n <= 2^21 pairs (2,097,152: a 1920x1080 frame's slots); more
+   returns RSH_CAPACITY, bits outside 1..32 or a null pointer RSH_INVALID_ARG.
+   Asynchronous on `stream` (a hipStream_t; NULL = the null stream), no
+   synchronisation, no allocation.  Every call on one GPU shares that GPU's
+   array: calls that overlap on one GPU race.  The first call on a device looks
+   the array up (outside a graph capture).  Returns an rsh_status code
+   (include/rsh.h): 0 OK, 1 invalid argument, 2 capacity, 3 HIP error. */
+int rsh_sort_pairs_noalloc(const uint32_t* keys, const uint32_t* values, uint32_t n, uint32_t bits,
+                           uint32_t* out_keys, uint32_t* out_values, void* stream);
 /* RayTracer.cu:2385 RaySortingWrapper (CUB SortPairs replaced by the library's radix sort) */
 void RaySortingWrapper(unsigned int* chunkIndexKeysArray, unsigned int* chunkIndexValuesArray,
                        const unsigned int chunkTotal, unsigned int* sortedChunkIndexKeysArray,

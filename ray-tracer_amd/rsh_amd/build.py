@@ -20,7 +20,7 @@ LIBDIR = os.path.join(ROOT, "lib")
 OBJDIR = os.path.join(ROOT, "lib", "obj")
 LIB = os.path.join(LIBDIR, "librsh.so")
 SOURCES = ["k_scan.hip", "k_trav.hip", "k_rays.hip", "engine.hip", "compat.hip", "server.hip", "selfcheck.hip",
-           "spheres.hip"]
+           "spheres.hip", "noalloc.hip"]
 ARCH = "gfx950"
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",

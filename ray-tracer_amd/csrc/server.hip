@@ -6,10 +6,13 @@
 // hierarchy and pass scratch) and HIP stream, and answers ray queries:
 //  * submit() checks a request against the HBM budget and queues it; requests
 //    are admitted in arrival order while the budget allows;
-//  * one worker thread per scene takes every admitted request of the head
-//    request's kind (closest / any hit), up to a ray and request cap, and
-//    traces them as one batch: each request's rays get its index in the batch
-//    as the key's segment id, so the sort keeps requests apart;
+//  * one worker thread per scene takes, from every admitted request of the
+//    head request's kind (closest / any hit), its next slice of at most
+//    slice_rays rays, up to a ray and request cap, and traces them as one
+//    batch: each slice's rays get its index in the batch as the key's segment
+//    id, so the sort keeps slices apart.  A request larger than a slice stays
+//    queued with the rest of its rays, so requests that arrive meanwhile join
+//    the next batch beside its next slice instead of waiting for all of it;
 //  * results are copied back to each caller's host buffer, the request's
 //    bytes are returned to the budget (admitting waiting requests) and
 //    waiters are woken.
@@ -45,6 +48,7 @@ constexpr uint32_t MAX_BATCH_REQUESTS = 1u << (32 - SEGMENT_SHIFT);
 struct Request {
     uint64_t id = 0;
     uint32_t scene = 0, kind = 0, n = 0;
+    uint32_t taken = 0, finished = 0;   // rays handed to batches / traced
     const float* rays = nullptr;     // caller's host memory, valid until the request completes
     const float* tmax = nullptr;
     void* out = nullptr;
@@ -52,7 +56,14 @@ struct Request {
     bool admitted = false, done = false;
     rsh_status status = RSH_OK;
     Clock::time_point t_submit, t_start, t_done;
-    uint32_t batch_requests = 0;     // requests traced in the same batch
+    bool started = false;
+    uint32_t batch_requests = 0;     // most slices traced in one batch with one of this request's
+};
+
+// A slice of a request in a batch: rays [off, off + n) of the request.
+struct Slice {
+    Request* r = nullptr;
+    uint32_t off = 0, n = 0;
 };
 
 struct SceneSlot {
@@ -73,6 +84,7 @@ struct rsh_server {
     int device = 0;
     uint64_t budget = 0, admitted_bytes = 0;
     uint32_t max_batch_rays = 0;
+    uint32_t slice_rays = 1u << 20;
     std::mutex mu;
     std::condition_variable done_cv;
     std::deque<Request*> pending;                // waiting for budget, in arrival order
@@ -99,25 +111,34 @@ void admit_locked(rsh_server* s) {
     }
 }
 
-void finish_locked(rsh_server* s, std::vector<Request*>& batch, rsh_status st, SceneSlot* sc) {
+void finish_locked(rsh_server* s, std::vector<Slice>& batch, rsh_status st, SceneSlot* sc) {
     const auto now = Clock::now();
-    for (Request* r : batch) {
-        r->status = st;
-        r->done = true;
-        r->t_done = now;
-        r->batch_requests = (uint32_t)batch.size();
-        s->admitted_bytes -= r->bytes;
-        sc->requests_done++;
-        sc->rays_done += r->n;
+    for (Slice& b : batch) {
+        Request* r = b.r;
+        if (st != RSH_OK) r->status = st;
+        r->finished += b.n;
+        r->batch_requests = std::max(r->batch_requests, (uint32_t)batch.size());
+        sc->rays_done += b.n;
+        if (r->finished == r->n || st != RSH_OK) {
+            if (r->done) continue;
+            if (st != RSH_OK && r->taken < r->n) {     // a failed request takes no more slices
+                auto it = std::find(sc->queue.begin(), sc->queue.end(), r);
+                if (it != sc->queue.end()) sc->queue.erase(it);
+            }
+            r->done = true;
+            r->t_done = now;
+            s->admitted_bytes -= r->bytes;
+            sc->requests_done++;
+        }
     }
     admit_locked(s);
     s->done_cv.notify_all();
 }
 
-rsh_status trace_batch(rsh_server* s, SceneSlot* sc, std::vector<Request*>& batch) {
+rsh_status trace_batch(rsh_server* s, SceneSlot* sc, std::vector<Slice>& batch) {
     uint64_t total = 0;
-    for (Request* r : batch) total += r->n;
-    const uint32_t kind = batch[0]->kind;
+    for (const Slice& b : batch) total += b.n;
+    const uint32_t kind = batch[0].r->kind;
     if (hipSetDevice(s->device) != hipSuccess) return RSH_HIP_ERROR;
     if (sc->rays.reserve(total * 24) != hipSuccess || sc->segs.reserve(total * 4) != hipSuccess ||
         sc->out.reserve(total * 8) != hipSuccess || (kind == 1 && sc->tmax.reserve(total * 4) != hipSuccess))
@@ -127,12 +148,14 @@ rsh_status trace_batch(rsh_server* s, SceneSlot* sc, std::vector<Request*>& batc
     uint64_t off = 0;
     hipError_t he = hipSuccess;
     for (uint32_t j = 0; j < batch.size() && he == hipSuccess; j++) {
-        Request* r = batch[j];
-        he = hipMemcpyAsync(sc->rays.as<float>() + 6 * off, r->rays, (size_t)r->n * 24, hipMemcpyHostToDevice, st);
+        const Slice& b = batch[j];
+        he = hipMemcpyAsync(sc->rays.as<float>() + 6 * off, b.r->rays + 6 * (size_t)b.off, (size_t)b.n * 24,
+                            hipMemcpyHostToDevice, st);
         if (he == hipSuccess && kind == 1)
-            he = hipMemcpyAsync(sc->tmax.as<float>() + off, r->tmax, (size_t)r->n * 4, hipMemcpyHostToDevice, st);
-        if (he == hipSuccess) launch_fill_u32(sc->segs.as<uint32_t>() + off, j, r->n, st);
-        off += r->n;
+            he = hipMemcpyAsync(sc->tmax.as<float>() + off, b.r->tmax + b.off, (size_t)b.n * 4, hipMemcpyHostToDevice,
+                                st);
+        if (he == hipSuccess) launch_fill_u32(sc->segs.as<uint32_t>() + off, j, b.n, st);
+        off += b.n;
     }
     if (he == hipSuccess) he = hipStreamSynchronize(st);
     rsh_status rs = he == hipSuccess ? RSH_OK : RSH_HIP_ERROR;
@@ -142,11 +165,12 @@ rsh_status trace_batch(rsh_server* s, SceneSlot* sc, std::vector<Request*>& batc
     if (rs == RSH_OK) {
         off = 0;
         const size_t esz = kind == 1 ? 4 : 8;
-        for (Request* r : batch) {
-            he = hipMemcpyAsync(r->out, static_cast<char*>(sc->out.p) + off * esz, (size_t)r->n * esz,
-                                hipMemcpyDeviceToHost, st);
+        for (const Slice& b : batch) {
+            he = hipMemcpyAsync(static_cast<char*>(b.r->out) + (size_t)b.off * esz,
+                                static_cast<char*>(sc->out.p) + off * esz, (size_t)b.n * esz, hipMemcpyDeviceToHost,
+                                st);
             if (he != hipSuccess) break;
-            off += r->n;
+            off += b.n;
         }
         if (he == hipSuccess) he = hipStreamSynchronize(st);
         if (he != hipSuccess) rs = RSH_HIP_ERROR;
@@ -159,23 +183,27 @@ void worker_loop(rsh_server* s, SceneSlot* sc) {
     for (;;) {
         sc->cv.wait(lk, [&] { return s->stop || !sc->queue.empty(); });
         if (s->stop && sc->queue.empty()) return;
-        // the head request's kind; every queued request of that kind, in order, up to the caps
+        // the head request's kind; the next slice of every queued request of
+        // that kind, in arrival order, up to the caps
         const uint32_t kind = sc->queue.front()->kind;
-        std::vector<Request*> batch;
+        std::vector<Slice> batch;
         uint64_t rays = 0;
-        for (auto it = sc->queue.begin(); it != sc->queue.end();) {
-            Request* r = *it;
-            if (r->kind == kind && batch.size() < MAX_BATCH_REQUESTS &&
-                (batch.empty() || rays + r->n <= s->max_batch_rays)) {
-                batch.push_back(r);
-                rays += r->n;
-                it = sc->queue.erase(it);
-            } else {
-                ++it;
-            }
-        }
         const auto t0 = Clock::now();
-        for (Request* r : batch) r->t_start = t0;
+        for (auto it = sc->queue.begin(); it != sc->queue.end() && batch.size() < MAX_BATCH_REQUESTS;) {
+            Request* r = *it;
+            const uint32_t left = r->n - r->taken;
+            const uint32_t want = left < s->slice_rays ? left : s->slice_rays;
+            if (r->kind != kind || (!batch.empty() && rays + want > s->max_batch_rays)) {
+                ++it;
+                continue;
+            }
+            batch.push_back(Slice{r, r->taken, want});
+            r->taken += want;
+            rays += want;
+            if (!r->started) { r->started = true; r->t_start = t0; }
+            if (r->taken == r->n) it = sc->queue.erase(it);
+            else ++it;
+        }
         lk.unlock();
         const rsh_status st = trace_batch(s, sc, batch);
         lk.lock();
@@ -202,6 +230,13 @@ rsh_status rsh_server_create(rsh_server** out, int device, uint64_t hbm_budget_b
 }
 
 const char* rsh_server_last_error(const rsh_server* s) { return s ? s->err.c_str() : "null server"; }
+
+rsh_status rsh_server_set_slice(rsh_server* s, uint32_t slice_rays) {
+    if (!s || slice_rays == 0) return RSH_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->slice_rays = slice_rays;
+    return RSH_OK;
+}
 
 rsh_status rsh_server_add_scene(rsh_server* s, const rsh_scene_desc* scene, const rsh_frame_config* cfg,
                                 uint32_t* scene_id) {

@@ -17,6 +17,8 @@ import time
 
 import numpy as np
 
+from . import RshError
+
 HASH_SPLITS = [(4, 6), (5, 4), (3, 7), (6, 3), (2, 9)]        # 2 + 3c + 2d <= 26
 SHAPES = [(8, 8), (4, 16), (16, 4), (32, 2), (8, 4)]          # (leaf, branch); leaf * branch != 64 -> generic kernel
 CHUNKS = [64, 16, 256]                                        # candidate clusters per work item
@@ -57,7 +59,12 @@ def tune(engine, scene, views, base_cfg, reps=3, log=None):
 
     def run(cfg):
         nonlocal best, best_ms
-        ms, rays = time_batch(engine, views, cfg, reps)
+        try:
+            ms, rays = time_batch(engine, views, cfg, reps)
+        except RshError as err:      # e.g. a work list over capacity for a small chunk: not a candidate
+            if log:
+                log(f"skipped {cfg.cell_bits}/{cfg.dir_bits} {cfg.leaf}x{cfg.branch} chunk {cfg.traversal_chunk}: {err}")
+            return
         same = float((engine.pixels() == ref).mean())
         t = Trial(cfg.cell_bits, cfg.dir_bits, cfg.leaf, cfg.branch, cfg.traversal_chunk, ms, rays / ms / 1e3, same)
         trials.append(t)

@@ -46,6 +46,8 @@ def _declare(L):
     L.rsh_server_last_error.restype = C.c_char_p
     L.rsh_server_last_error.argtypes = [PV]
     L.rsh_server_destroy.argtypes = [PV]
+    L.rsh_server_set_slice.restype = C.c_int
+    L.rsh_server_set_slice.argtypes = [PV, u32]
     L.rsh_host_alloc.restype = PV
     L.rsh_host_alloc.argtypes = [C.c_size_t]
     L.rsh_host_free.argtypes = [PV]
@@ -85,6 +87,10 @@ class Server:
     def _check(self, rc, what):
         if rc != 0:
             raise RshError(f"{what} failed ({rc}): {self.L.rsh_server_last_error(self.h).decode()}")
+
+    def set_slice(self, slice_rays):
+        """Largest slice of one request per batch (rsh_server_set_slice)."""
+        self._check(self.L.rsh_server_set_slice(self.h, int(slice_rays)), "set_slice")
 
     def add_scene(self, scene, cfg) -> int:
         sid = u32()

@@ -126,9 +126,11 @@ def _run(server, sids, pools, ring, reqs, arrivals):
     return lat, np.array([d[1] for d in done]), sum(d[2] for d in done), span
 
 
-def run(duration_s=8.0, load=0.7, budget=64 << 30, max_batch_rays=16 << 20, seed=0xC0FFEE + 4, log=None):
+def run(duration_s=8.0, load=0.7, budget=64 << 30, max_batch_rays=16 << 20, slice_rays=1 << 20, seed=0xC0FFEE + 4,
+        log=None):
     scene_list = serving_scenes()
     server = Server(0, hbm_budget=budget, max_batch_rays=max_batch_rays)
+    server.set_slice(slice_rays)
     sids = [server.add_scene(s, query_config(s)) for s in scene_list]
     pools = Pools(scene_list)
     ring = OutRing(48)
@@ -160,4 +162,17 @@ def run(duration_s=8.0, load=0.7, budget=64 << 30, max_batch_rays=16 << 20, seed
         "mean_batch_requests": float(bsz.mean()), "max_batch_requests": int(bsz.max()),
         "hbm_budget_gb": budget / (1 << 30),
         "batches": int(info.batches),
+        "slice_rays": int(slice_rays),
+        "p50_ms_by_size": _by_size(reqs, lat, 50), "p99_ms_by_size": _by_size(reqs, lat, 99),
     }
+
+
+def _by_size(reqs, lat, q):
+    """Latency percentile per request-size decade (1k-10k, ..., 1M-4M)."""
+    n = np.array([r[2] for r in reqs])
+    out = {}
+    for lo, hi, name in ((1e3, 1e4, "1k-10k"), (1e4, 1e5, "10k-100k"), (1e5, 1e6, "100k-1M"), (1e6, 5e6, "1M-4M")):
+        sel = (n >= lo) & (n < hi)
+        if sel.any():
+            out[name] = float(np.percentile(lat[sel], q))
+    return out

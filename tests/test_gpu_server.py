@@ -119,3 +119,29 @@ def test_alone_equals_mixed_four_scenes(rsh):
         assert same >= 0.9999, (rid, s, len(rays), same)
     assert merged > 1
     srv.close()
+
+
+def test_sliced_requests_equal_alone(rsh):
+    """With a slice of 8k rays, a 50k-ray request is traced over several
+    batches shared with the small requests queued behind it; every result
+    equals the request traced alone in one slice."""
+    from rsh_amd.server import Server
+    scene = rsh.scenes.config1_scene(boxes=300)
+    cfg = scfg(rsh, scene)
+    srv = Server(0, hbm_budget=8 << 30, max_batch_rays=1 << 22)
+    sid = srv.add_scene(scene, cfg)
+    rng = np.random.default_rng(31)
+    reqs = [queries.make_query(scene, queries.DIFFUSE, 50_000, rng)] + \
+           [queries.make_query(scene, queries.DIFFUSE, int(n), rng) for n in rng.integers(1000, 6000, 12)]
+    alone = [srv.wait(srv.submit(sid, r, k, t))[0].copy() for r, t, k in reqs]
+    srv.set_slice(8192)
+    ids = [srv.submit(sid, r, k, t) for r, t, k in reqs]
+    infos = []
+    for rid, (r, t, k), a in zip(ids, reqs, alone):
+        res, info = srv.wait(rid)
+        infos.append(info)
+        assert (hit_ids(res, k) == hit_ids(a, k)).mean() >= 0.9999, rid
+    assert infos[0].batch_requests > 1          # the big request shared batches
+    with pytest.raises(rsh.RshError):
+        srv.set_slice(0)
+    srv.close()

@@ -87,6 +87,8 @@ def parse():
     ap.add_argument("--write-csv", default="", help="with --hsc-summary: WRITE_SIZE counter_collection.csv")
     ap.add_argument("--fetch-scale", type=float, default=2.0,
                     help="FETCH_SIZE calibration: bytes read per counted byte on a plain copy")
+    ap.add_argument("--counter-renders", type=int, default=0,
+                    help="with --fetch-csv: renders in the counter runs (default: those of SIZES_JSON)")
     ap.add_argument("--out", default="")
     return ap.parse_args()
 
@@ -209,7 +211,7 @@ def hsc_figure(passes, stages, cfg, name):
 
 HSC_KERNELS = ("k_create_primary_sorted", "k_create_shadow", "k_create_reflection", "k_create_bounce",
                "k_pack_tiles", "k_pack_scan", "k_pack_write", "k_radix_upsweep", "k_radix_rowscan",
-               "k_radix_downsweep", "k_unpack_sizes", "k_scan_tiles", "k_unpack_offsets", "k_unpack_expand")
+               "k_radix_downsweep", "k_unpack_sizes", "k_unpack_scan", "k_unpack_offsets", "k_unpack_expand")
 HSC_PROFILES = {"frame": "hsc_atrium1080p_frame.txt", "office": "hsc_office_64views.txt",
                 "atrium": "hsc_atrium_64views.txt"}
 
@@ -238,12 +240,13 @@ def hsc_workload(name, rsh_amd):
         cam = scenes.config2_camera(1920, 1080)
         cfg = cf.config2(1920, 1080).with_scene_bounds(scene)
         eng.load_scene(scene)
-        return eng, (lambda: eng.render(cam, cfg)), cfg
+        # serial passes: no kernel of the other lane shares the GPU with a timed one
+        return eng, (lambda: eng.render(cam, cfg, flags=eng.SERIAL)), cfg
     views = scenes.batch_views()
     scene, base = (scenes.config1_scene(), cf.config1()) if name == "office" else (scenes.config2_scene(), cf.config2())
     cfg = dataclasses.replace(base, cell_bits=4, dir_bits=6, segment_shift=26).with_scene_bounds(scene)
     eng.load_scene(scene)
-    return eng, (lambda: eng.render_views(views, cfg)), cfg
+    return eng, (lambda: eng.render_views(views, cfg, flags=eng.SERIAL)), cfg
 
 
 def hsc_profile_run(a):
@@ -270,10 +273,10 @@ def hsc_profile_run(a):
 
 
 def _kernel_base(name):
-    name = name.strip().strip('"')
-    for sep in ("<", "("):
-        name = name.split(sep)[0]
-    return name.strip()
+    """The kernel's own name (k_...) inside a demangled rocprofv3 name such as
+    'void rsh::(anonymous namespace)::k_radix_upsweep<9u>(unsigned int const*, ...)'."""
+    m = re.search(r"\b(k_[A-Za-z0-9_]+)", name)
+    return m.group(1) if m else name.strip()
 
 
 def hsc_summary(a):
@@ -292,7 +295,8 @@ def hsc_summary(a):
         for r in csv.DictReader(f):
             base = _kernel_base(r["Name"])
             if base in HSC_KERNELS:
-                d = rows.setdefault(r["Name"].strip('"').split("(")[0], [0, 0.0])
+                m = re.search(r"\b(k_[A-Za-z0-9_]+(<[^>]*>)?)", r["Name"])
+                d = rows.setdefault(m.group(1) if m else base, [0, 0.0])
                 d[0] += int(r["Calls"])
                 d[1] += float(r["TotalDurationNs"]) / 1e3
     total_us = sum(v[1] for v in rows.values())
@@ -323,10 +327,13 @@ def hsc_summary(a):
                     tot += float(r["Counter_Value"])
         return tot * 1024.0
     if a.fetch_csv and a.write_csv:
-        fb = counter(a.fetch_csv) * a.fetch_scale
-        wb = counter(a.write_csv)
-        lines.append(f"FETCH_SIZE x {a.fetch_scale:.4f}: {fb:.4e} B; WRITE_SIZE: {wb:.4e} B; "
-                     f"sum {fb + wb:.4e} B = {(fb + wb) / nbytes:.3f} x the algorithmic bytes")
+        rc = a.counter_renders or R
+        fb = counter(a.fetch_csv) * a.fetch_scale / rc
+        wb = counter(a.write_csv) / rc
+        per = nbytes / R
+        lines.append(f"per render ({rc} in the counter runs): FETCH_SIZE x {a.fetch_scale:.4f}: {fb:.4e} B; "
+                     f"WRITE_SIZE: {wb:.4e} B; sum {fb + wb:.4e} B = {(fb + wb) / per:.3f} x the algorithmic "
+                     f"{per:.4e} B")
     print("\n".join(lines))
 
 

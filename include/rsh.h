@@ -196,12 +196,19 @@ rsh_status rsh_object_spheres(const float* tri_pos, uint32_t triangle_total, con
  *   bit 6  keep every pass's keys, sort order, packets and hierarchy nodes for
  *          rsh_engine_read_pass_order (a debug read-back for parity checks)
  *   bit 3  graph replay (alone, without bits 0-2, 4, 6): the first call for a given
- *          set of views, configuration and scene renders eagerly and captures
- *          a HIP graph of the frame (every pass's sizes fixed to the eager
- *          frame's); later identical calls replay that graph with no host
- *          read-backs.  The graph is dropped when any device buffer of the
- *          library is reallocated, the scene is reloaded or the inputs change.
- *          Needs a non-null stream (the engine's own by default). */
+ *          set of view sizes, configuration and scene renders eagerly and
+ *          captures a HIP graph of the frame; later calls replay it with no
+ *          host read-back inside.  CRSH frames with the 64-ray traversal
+ *          (depth 2, leaf x branch = 64) keep every pass's sizes in device
+ *          memory, with launches sized by capacities from the eager frame: the
+ *          graph replays for any cameras of those sizes, and the stats carry
+ *          the replay's own ray counts.  A replay whose work list outgrows the
+ *          capacities writes nothing past them and the call renders that frame
+ *          eagerly instead.  Other configurations fix every pass's sizes to the
+ *          eager frame's and replay only for the same cameras.  The graph is
+ *          dropped when any device buffer of the library is reallocated, the
+ *          scene is reloaded or the view sizes or configuration change.  Needs
+ *          a non-null stream (the engine's own by default). */
 rsh_status rsh_engine_render(rsh_engine* e, const rsh_camera* cam, const rsh_frame_config* cfg,
                              uint32_t flags, rsh_frame_stats* stats);
 
@@ -276,9 +283,9 @@ void* rsh_engine_pixels_device(rsh_engine* e);
  * rsh_host_alloc copies fastest). */
 typedef struct rsh_server rsh_server;
 typedef struct rsh_request_info {
-    double queue_ms;         /* submit -> start of the batch that traced it */
-    double latency_ms;       /* submit -> results in the caller's buffer     */
-    uint32_t batch_requests; /* requests traced in that batch                */
+    double queue_ms;         /* submit -> start of the first batch that traced it */
+    double latency_ms;       /* submit -> results in the caller's buffer           */
+    uint32_t batch_requests; /* most request slices in one batch that traced it    */
 } rsh_request_info;
 typedef struct rsh_server_info {
     uint64_t batches, requests, rays;
@@ -299,6 +306,11 @@ rsh_status rsh_server_submit(rsh_server* s, uint32_t scene_id, uint32_t kind, co
 /* Block until the request has completed; info may be NULL.  Each request is
  * waited for once. */
 rsh_status rsh_server_wait(rsh_server* s, uint64_t request_id, rsh_request_info* info);
+/* Largest slice of one request traced in one batch (default 2^20 rays): a
+ * larger request is traced over several batches, each shared with the
+ * requests queued meanwhile, so small requests do not wait behind all of a
+ * large one.  Each ray's result does not depend on its batch. */
+rsh_status rsh_server_set_slice(rsh_server* s, uint32_t slice_rays);
 rsh_status rsh_server_stats(rsh_server* s, rsh_server_info* info);
 const char* rsh_server_last_error(const rsh_server* s);
 /* Stops the workers after the admitted requests; wait for every request first. */

@@ -58,6 +58,7 @@ struct Lane {
     bool table_clean = false;          // w_table is zero (the item scan clears it after each read-back)
     // work list of the 64-ray traversal kernel
     DevBuf w_npass, w_cand, w_nitems, w_off, w_first, w_list, w_cum, w_items, w_table, w_pmask;
+    DevBuf psz;                        // PassSizes of a device-sized pass
     ScanScratch sc;
     std::vector<PassPlan> plan;        // sizes of this lane's passes (recorded / replayed in order)
     size_t plan_pos = 0;
@@ -68,7 +69,7 @@ struct Lane {
     std::vector<DevBuf*> bufs() {
         return {&flags, &keys, &values, &tvalues, &bases, &ckeys, &cvalues, &skeys, &svalues, &rvalues, &soff,
                 &rec0, &rec1, &nodes, &aux, &counters, &dcounts, &w_npass, &w_cand, &w_nitems, &w_off, &w_first,
-                &w_list, &w_cum, &w_items, &w_table, &w_pmask};
+                &w_list, &w_cum, &w_items, &w_table, &w_pmask, &psz};
     }
 };
 
@@ -106,6 +107,15 @@ struct rsh_engine {
     std::vector<hipEvent_t> sync_ev;   // cross-lane dependencies of a frame
 
     bool recording = false, replaying = false;
+    // Graphs that replay for any camera of the same view sizes: the capture
+    // keeps the shadow and reflection passes' sizes on the device (dev_sized),
+    // sizes their launches by capacities from the recorded frame, and records
+    // each pass's (N, C) and an overflow word in frame_rec, read after a
+    // replay (hrec); an overflow re-renders the frame eagerly.
+    bool dev_sized = false;            // set while such a frame is captured
+    bool gdev = false;                 // the current graph is device-sized
+    DevBuf frame_rec;                  // uint32[2 * RSH_MAX_PASSES + 1]: (N, C) per pass, overflow
+    uint32_t* hrec = nullptr;          // page-locked copy of frame_rec
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     std::vector<uint8_t> gsig;         // views + config + scene epoch the graph was captured for
@@ -121,6 +131,10 @@ struct rsh_engine {
         DevBuf keys, flags, rec, bases, ckeys, nodes;
         uint32_t o_slots = 0, o_rays = 0, o_chunks = 0, o_nodes = 0;
         bool have_keys = false, have_chunks = false, order_set = false;
+        void release() {
+            for (DevBuf* b : {&buf, &keys, &flags, &rec, &bases, &ckeys, &nodes}) b->release();
+            set = order_set = false;
+        }
     };
     std::vector<Kept> kept;
 };
@@ -414,6 +428,27 @@ struct PassOut {
 };
 
 // Work list of the 64-ray kernel (see uses_items): cull, size, scan, fill.
+// Work-list buffers of a pass with ntop top nodes, list_total list entries
+// and items work items.
+hipError_t reserve_items(Lane& L, uint32_t ntop, uint32_t K, uint64_t list_total, uint64_t items) {
+    const size_t nb = ((size_t)ntop + 1) * 4;
+    hipError_t r = hipSuccess;
+    for (DevBuf* b : {&L.w_npass, &L.w_cand, &L.w_nitems, &L.w_off, &L.w_first})
+        if (r == hipSuccess) r = b->reserve(nb);
+    if (r == hipSuccess) r = L.w_pmask.reserve(((size_t)ntop + 1) * ((K + 63) / 64) * 8 + 8);
+    if (r == hipSuccess) r = L.w_list.reserve((size_t)list_total * 16 + 16);
+    if (r == hipSuccess) r = L.w_cum.reserve((size_t)list_total * 4 + 4);
+    if (r == hipSuccess) r = L.w_items.reserve((size_t)items * 16 + 16);
+    return r;
+}
+
+// Capacities of a device-sized pass's work list from the recorded pass.
+uint64_t list_capacity(const PassPlan& pp) { return (uint64_t)pp.list_total * 3 / 2 + 4096; }
+uint64_t items_capacity(const PassPlan& pp) {
+    const uint64_t c = (uint64_t)pp.items * 3 / 2 + 4096;
+    return c < MAX_TRAVERSE_ITEMS ? c : MAX_TRAVERSE_ITEMS;
+}
+
 rsh_status build_items(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, TravParams& p, PassOut& out,
                        bool camera) {
     hipStream_t s = L.stream;
@@ -430,7 +465,16 @@ rsh_status build_items(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, Trav
     const uint32_t* np_scan = p.algorithm != 1 ? npass : nullptr;
     uint32_t chunk = cfg.traversal_chunk ? cfg.traversal_chunk : ITEM_CHUNK;
     uint32_t tot[2] = {0, 0};   // list entries, items
-    if (e->replaying) {
+    if (e->replaying && p.ps) {
+        // device-sized: the recorded pass's chunk and mode; the list and the
+        // item total stay on the device, bounded by capacities
+        const PassPlan& pp = L.plan[L.plan_pos - 1];
+        if (pp.chunk) { chunk = pp.chunk; p.prefilter = pp.mode; }
+        tot[0] = (uint32_t)list_capacity(pp); tot[1] = (uint32_t)items_capacity(pp);
+        p.list_cap = tot[0]; p.items_cap = tot[1];
+        LANE_OK(run_item_scan(cand, ntop, chunk, e->NC, nitems, u32p(L.w_first), L.sc, s, nullptr, 0, np_scan,
+                              u32p(L.w_off), p.ps));
+    } else if (e->replaying) {
         const PassPlan& pp = L.plan[L.plan_pos - 1];
         chunk = pp.chunk; tot[0] = pp.list_total; tot[1] = pp.items; p.prefilter = pp.mode;
         LANE_OK(run_item_scan(cand, ntop, chunk, e->NC, nitems, u32p(L.w_first), L.sc, s, nullptr, 0, np_scan,
@@ -556,10 +600,21 @@ rsh_status trace_pass(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, const
                         kind == 1 ? u32p(e->shadow) : nullptr, L.sc, s, poll ? L.hsize_dev : nullptr,
                         poll ? ++L.hseq : 0u));
     }
+    // Device-sized pass (a capture for any camera): N and C stay in device
+    // memory; the launches below cover the pass's S slots.  The camera pass's
+    // sizes are fixed (every slot live, one chunk per ray), but its work list
+    // depends on the camera, so it takes the same path.
+    const bool dev = e->dev_sized && cfg.algorithm == RSH_CRSH && S > 0;
+    PassSizes* psz = dev ? L.psz.as<PassSizes>() : nullptr;
+    if (dev && all_unique)
+        launch_fixed_sizes(S, S, cfg.leaf, cfg.branch, cfg.depth, psz, u32p(e->frame_rec) + 2 * (size_t)pass_idx, s);
+    else if (dev)
+        launch_pass_sizes(dc, cfg.leaf, cfg.branch, cfg.depth, psz, u32p(e->frame_rec) + 2 * (size_t)pass_idx, s);
     uint32_t hc[2] = {0, 0};
     if (e->replaying) {
         const PassPlan& pp = L.plan[L.plan_pos++];
         hc[0] = pp.N; hc[1] = pp.C;
+        if (dev) hc[0] = hc[1] = S;       // capacities
     } else if (all_unique) {
         hc[0] = hc[1] = S;
         if (e->recording) { L.plan.emplace_back(); L.plan.back().N = S; L.plan.back().C = S; }
@@ -588,16 +643,25 @@ rsh_status trace_pass(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, const
     } else if (cfg.algorithm == RSH_CRSH) {
         const uint32_t C = hc[1];
         out.C = C;
+        const uint32_t bits = sort_bits ? sort_bits : key_bits(cfg);
         {
             StageTimer t(L, ST_SORT, timing);
-            LANE_OK(run_radix_sort(u32p(L.ckeys), u32p(L.cvalues), C, sort_bits ? sort_bits : key_bits(cfg), u32p(L.skeys),
-                                  u32p(L.svalues), L.sc, s));
+            if (dev)
+                LANE_OK(run_radix_sort_dev(u32p(L.ckeys), u32p(L.cvalues), C, &psz->C, bits, u32p(L.skeys),
+                                           u32p(L.svalues), L.sc, s));
+            else
+                LANE_OK(run_radix_sort(u32p(L.ckeys), u32p(L.cvalues), C, bits, u32p(L.skeys), u32p(L.svalues), L.sc,
+                                       s));
         }
         {
             StageTimer t(L, ST_DECOMPRESS, timing);
             // the sorted keys are not needed after the sort: their buffer holds the chunk bases
-            LANE_OK(run_unpack(u32p(L.bases), u32p(L.svalues), u32p(L.tvalues), C, N, u32p(L.soff),
-                              u32p(L.skeys), u32p(L.rvalues), L.sc, s));
+            if (dev)
+                LANE_OK(run_unpack_dev(u32p(L.bases), u32p(L.svalues), u32p(L.tvalues), C, psz, u32p(L.soff),
+                                       u32p(L.skeys), u32p(L.rvalues), L.sc, s));
+            else
+                LANE_OK(run_unpack(u32p(L.bases), u32p(L.svalues), u32p(L.tvalues), C, N, u32p(L.soff),
+                                   u32p(L.skeys), u32p(L.rvalues), L.sc, s));
         }
         sorted_values = u32p(L.rvalues);
     } else {
@@ -608,10 +672,10 @@ rsh_status trace_pass(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, const
     {
         StageTimer t(L, ST_HIER, timing);
         if (!presorted)
-            launch_gather(rays, sorted_values, N, kind == 1 ? shadow_pos : nullptr, kind == 1 ? tmax : nullptr, e->W,
-                          e->Lmax, kind == 1 && shadow_pos && cfg.soft_shadows ? 4u : 0u, f4p(L.rec0),
+            launch_gather(rays, sorted_values, N, psz, kind == 1 ? shadow_pos : nullptr, kind == 1 ? tmax : nullptr,
+                          e->W, e->Lmax, kind == 1 && shadow_pos && cfg.soft_shadows ? 4u : 0u, f4p(L.rec0),
                           f4p(L.rec1), s);
-        launch_hierarchy(f4p(L.rec0), f4p(L.rec1), nullptr, nullptr, N, cfg.leaf, cfg.branch, cfg.depth,
+        launch_hierarchy(f4p(L.rec0), f4p(L.rec1), psz, nullptr, nullptr, N, cfg.leaf, cfg.branch, cfg.depth,
                          cfg.initial_radius, cfg.initial_spread, cfg.cone_mode, f4p(L.nodes), L.aux.as<float2>(),
                          counts, offs, s);
         LANE_OK(hipGetLastError());
@@ -661,6 +725,8 @@ rsh_status trace_pass(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, const
     // the spread of its top nodes when the work list is sized.
     p.prefilter = cfg.traversal_prefilter == 1 ? 0u : cfg.traversal_prefilter == 3 ? 2u : 1u;
     p.splits = 1;
+    p.ps = psz;
+    p.overflow = dev ? u32p(e->frame_rec) + 2 * RSH_MAX_PASSES : nullptr;
     if (counting) launch_fill_u64(u64p(L.counters), 0ull, COUNTER_SLOTS * COUNTERS_PER_SLOT, s);
     if (uses_items(p)) {
         // cull + work list; its time (host read-backs included) counts as traversal
@@ -768,7 +834,9 @@ void rsh_engine_destroy(rsh_engine* e) {
     if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
     if (e->graph) (void)hipGraphDestroy(e->graph);
     for (int i = 0; i < 4; i++) { e->g0[i].release(); e->gb[0][i].release(); e->gb[1][i].release(); }
-    for (auto& k : e->kept) k.buf.release();
+    for (auto& k : e->kept) k.release();
+    e->frame_rec.release();
+    if (e->hrec) (void)hipHostFree(e->hrec);
     if (e->own_stream && e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
 }
@@ -976,9 +1044,9 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
     if (rs != RSH_OK) return rs;
     if ((rs = ensure_nodes(e, (uint32_t)(S64 * smult), cfg)) != RSH_OK) return rs;
     Views V{};
-    if (nv == 1) {
+    if (nv == 1 && !e->dev_sized) {
         V.n = 0; V.one = hv[0];
-    } else {
+    } else {      // a device table (a capture for any camera reads the cameras from it)
         if (!e->replaying) {
             HIP_OK(e->views_d.reserve((size_t)nv * sizeof(ViewDesc)));
             HIP_OK(hipMemcpyAsync(e->views_d.p, hv, (size_t)nv * sizeof(ViewDesc), hipMemcpyHostToDevice, e->stream));
@@ -986,7 +1054,7 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
         V.d = e->views_d.as<ViewDesc>(); V.n = nv;
     }
     if (!e->replaying) {
-        for (auto& k : e->kept) k.buf.release();
+        for (auto& k : e->kept) k.release();
         e->kept.clear();
         if (fl & 65u) e->kept.resize(RSH_MAX_PASSES);
     }
@@ -1105,8 +1173,9 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
             if (r != RSH_OK) return r;
             if (ps.N > 0) {
                 StageTimer t(A, ST_SHADE, timing);
+                // device-sized capture: the shadow pass's live count gates the shading
                 launch_shade_primary(G0, W, H, Lm, sv.lights, V, u32p(e->shadow), cfg.soft_shadows, u32p(e->pixels),
-                                     sa);
+                                     sa, e->dev_sized ? u32p(e->frame_rec) + 2 * (size_t)idx_shadow(0) : nullptr);
             }
             return RSH_OK;
         }
@@ -1128,6 +1197,7 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
     auto run_a = [&]() -> rsh_status {
         Lane& L = A;
         launch_fill_u32(u32p(e->pixels), 0u, P, sa);
+        if (e->dev_sized) launch_fill_u32(u32p(e->frame_rec), 0u, 2 * RSH_MAX_PASSES + 1, sa);
         // CRSH camera rays come out of creation in key order when every key
         // is unique (one view, or the view id in the key)
         const bool presort = cfg.algorithm == RSH_CRSH && (nv == 1 || cfg.segment_shift != 0);
@@ -1205,7 +1275,7 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
     if (!e->replaying && (fl & 65u)) {
         uint32_t n = 0;
         while (n < e->kept.size() && (e->kept[n].set || e->kept[n].order_set)) n++;
-        for (size_t i = n; i < e->kept.size(); i++) e->kept[i].buf.release();
+        for (size_t i = n; i < e->kept.size(); i++) e->kept[i].release();
         e->kept.resize(n);
     }
     if (e->replaying) return RSH_OK;     // under stream capture: no synchronisation
@@ -1242,12 +1312,26 @@ void drop_graph(rsh_engine* e) {
     e->gsig.clear();
 }
 
-// Flag bit 3: replay a captured graph of this exact frame when one exists
-// for the same views, configuration and scene and no device buffer has been
-// reallocated since; otherwise render eagerly (recording the pass sizes),
-// then capture a replay of the frame for the next call.  A frame's sizes
-// depend only on these inputs, so the replay repeats the eager frame's work
-// without host read-backs.
+// Shadow and reflection passes of this configuration can keep their sizes on
+// the device: CRSH with the 64-ray traversal kernel, single-block item scans.
+bool dev_sizable(const rsh_engine* e, const rsh_frame_config& cfg, uint64_t slots) {
+    if (cfg.algorithm != RSH_CRSH || cfg.depth != 2 || cfg.leaf * cfg.branch != 64 || cfg.traversal_kernel == 1)
+        return false;
+    if (cfg.leaf != 2 && cfg.leaf != 4 && cfg.leaf != 8 && cfg.leaf != 16 && cfg.leaf != 32) return false;
+    const uint64_t n0 = (slots + cfg.leaf - 1) / cfg.leaf, ntop = (n0 + cfg.branch - 1) / cfg.branch;
+    return ntop + 1 <= SMALL_SCAN && e->K > 0;
+}
+
+// Flag bit 3: replay a captured graph of the frame when one exists for the
+// same configuration, scene and view sizes (device-sized graphs: any cameras;
+// otherwise the same cameras) and no device buffer has been reallocated since;
+// otherwise render eagerly (recording the pass sizes), then capture a replay
+// for the next call.  A device-sized graph keeps the shadow and reflection
+// passes' sizes in device memory and sizes their launches by capacities from
+// the recorded frame; a replay whose work list outgrows them reports an
+// overflow and the frame is rendered eagerly instead (the next call captures
+// again).  Other graphs fix every pass's sizes to the recorded frame's, which
+// only the same cameras repeat.
 rsh_status render(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rsh_frame_config& cfg, uint32_t fl,
                   rsh_frame_stats* st) {
     if (!(fl & 8u)) return render_frame(e, hv, nv, cfg, fl, st);
@@ -1256,15 +1340,57 @@ rsh_status render(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rsh_fram
         return RSH_INVALID_ARG;
     }
     if (e->stream == nullptr) { e->err = "render: graph replay needs a non-null stream"; return RSH_INVALID_ARG; }
-    std::vector<uint8_t> sig(sizeof(cfg) + sizeof(e->scene_epoch) + (size_t)nv * sizeof(ViewDesc));
-    std::memcpy(sig.data(), &cfg, sizeof(cfg));
-    std::memcpy(sig.data() + sizeof(cfg), &e->scene_epoch, sizeof(e->scene_epoch));
-    std::memcpy(sig.data() + sizeof(cfg) + sizeof(e->scene_epoch), hv, (size_t)nv * sizeof(ViewDesc));
+    uint64_t P = 0;
+    for (uint32_t i = 0; i < nv; i++) P += (uint64_t)hv[i].w * hv[i].h;
+    const uint32_t smult = cfg.soft_shadows ? 16u : 1u;
+    const bool dev = dev_sizable(e, cfg, P * e->L * smult);
+    // signatures: configuration, scene and views; a device-sized graph's
+    // leaves the cameras out (sizes and offsets only)
+    auto signature = [&](bool cameras) {
+        std::vector<uint8_t> sg(sizeof(cfg) + sizeof(e->scene_epoch) + 4 + (size_t)nv * sizeof(ViewDesc));
+        uint8_t* q = sg.data();
+        std::memcpy(q, &cfg, sizeof(cfg)); q += sizeof(cfg);
+        std::memcpy(q, &e->scene_epoch, sizeof(e->scene_epoch)); q += sizeof(e->scene_epoch);
+        std::memcpy(q, &nv, 4); q += 4;
+        for (uint32_t i = 0; i < nv; i++, q += sizeof(ViewDesc)) {
+            ViewDesc v = hv[i];
+            if (!cameras) {
+                std::memset(v.eye, 0, sizeof(v.eye)); std::memset(v.dir, 0, sizeof(v.dir));
+                std::memset(v.right, 0, sizeof(v.right)); std::memset(v.up, 0, sizeof(v.up));
+            }
+            std::memcpy(q, &v, sizeof(ViewDesc));
+        }
+        return sg;
+    };
+    const std::vector<uint8_t> sig = signature(!(e->gexec && e->gdev));
     if (e->gexec && sig == e->gsig && alloc_epoch().load() == e->gepoch) {
         HIP_OK(hipSetDevice(e->device));
+        if (e->gdev)
+            HIP_OK(hipMemcpyAsync(e->views_d.p, hv, (size_t)nv * sizeof(ViewDesc), hipMemcpyHostToDevice, e->stream));
         HIP_OK(hipGraphLaunch(e->gexec, e->stream));
+        if (e->gdev)
+            HIP_OK(hipMemcpyAsync(e->hrec, e->frame_rec.p, (2 * RSH_MAX_PASSES + 1) * 4, hipMemcpyDeviceToHost,
+                                  e->stream));
         HIP_OK(hipStreamSynchronize(e->stream));
-        if (st) { *st = e->gstats; st->graph_replayed = 1; }
+        if (e->gdev && e->hrec[2 * RSH_MAX_PASSES]) {
+            // the work list outgrew its capacities: render this frame eagerly
+            drop_graph(e);
+            return render_frame(e, hv, nv, cfg, 0, st);
+        }
+        if (st) {
+            *st = e->gstats;
+            st->graph_replayed = 1;
+            if (e->gdev) {
+                st->rays_total = 0;
+                for (uint32_t i = 0; i < st->pass_count; i++) {
+                    if (i > 0 || cfg.algorithm != RSH_CRSH) {      // the camera pass is fixed by the view sizes
+                        st->pass_rays[i] = e->hrec[2 * i];
+                        st->pass_chunks[i] = e->hrec[2 * i + 1];
+                    }
+                    st->rays_total += st->pass_rays[i];
+                }
+            }
+        }
         return RSH_OK;
     }
     drop_graph(e);
@@ -1275,14 +1401,40 @@ rsh_status render(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rsh_fram
     e->recording = false;
     if (rs != RSH_OK) return rs;
     if (st) *st = local;
+    // every pass recorded (no early end), so a capture can issue all of them
+    const uint32_t full = 2 + cfg.bounces * (cfg.bounce_shadows ? 2u : 1u);
+    const bool use_dev = dev && local.pass_count == full && cfg.bounces < (RSH_MAX_PASSES - 2) / 2;
+    if (use_dev) {
+        // buffers the capture uses, at their capacities, before it starts
+        HIP_OK(e->frame_rec.reserve((2 * RSH_MAX_PASSES + 1) * 4));
+        if (!e->hrec) HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&e->hrec), (2 * RSH_MAX_PASSES + 1) * 4,
+                                           hipHostMallocDefault));
+        HIP_OK(e->views_d.reserve((size_t)nv * sizeof(ViewDesc)));
+        HIP_OK(hipMemcpyAsync(e->views_d.p, hv, (size_t)nv * sizeof(ViewDesc), hipMemcpyHostToDevice, e->stream));
+        for (int li = 0; li < 2; li++) {
+            Lane& L = e->lane[li];
+            const uint64_t S = li == 0 ? P * e->L : P * e->L * smult;
+            const uint64_t n0 = (S + cfg.leaf - 1) / cfg.leaf, ntop = (n0 + cfg.branch - 1) / cfg.branch;
+            uint64_t lc = 0, ic = 0;
+            for (const PassPlan& pp : L.plan) {
+                lc = std::max<uint64_t>(lc, std::max<uint64_t>(list_capacity(pp), pp.list_total));
+                ic = std::max<uint64_t>(ic, std::max<uint64_t>(items_capacity(pp), pp.items));
+            }
+            HIP_OK(L.psz.reserve(sizeof(PassSizes)));
+            HIP_OK(reserve_items(L, (uint32_t)ntop, e->K, lc, ic));
+        }
+        HIP_OK(hipStreamSynchronize(e->stream));
+    }
     const uint64_t epoch = alloc_epoch().load();
     e->replaying = true;
+    e->dev_sized = use_dev;
     for (Lane& l : e->lane) l.plan_pos = 0;
     HIP_OK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeRelaxed));
     rs = render_frame(e, hv, nv, cfg, 0, nullptr);
     hipGraph_t g = nullptr;
     const hipError_t ce = hipStreamEndCapture(e->stream, &g);
     e->replaying = false;
+    e->dev_sized = false;
     if (rs != RSH_OK) { if (g) (void)hipGraphDestroy(g); return rs; }
     HIP_OK(ce);
     if (alloc_epoch().load() != epoch || e->lane[0].plan_pos != e->lane[0].plan.size() ||
@@ -1293,7 +1445,7 @@ rsh_status render(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rsh_fram
     }
     const hipError_t ie = hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0);
     if (ie != hipSuccess) { (void)hipGraphDestroy(g); e->gexec = nullptr; HIP_OK(ie); }
-    e->graph = g; e->gsig = sig; e->gepoch = epoch; e->gstats = local;
+    e->graph = g; e->gsig = signature(!use_dev); e->gepoch = epoch; e->gstats = local; e->gdev = use_dev;
     return RSH_OK;
 }
 

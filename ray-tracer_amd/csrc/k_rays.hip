@@ -311,7 +311,10 @@ __device__ inline uint32_t soft_mask(const uint32_t* sflags, size_t slot) {
 
 __global__ void __launch_bounds__(256) k_shade_primary(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L,
                                                        Views V, const uint32_t* sflags, uint32_t soft,
-                                                       uint32_t* pixels) {
+                                                       uint32_t* pixels, const uint32_t* live) {
+    // a shadow pass with no live ray leaves the pixels as they are (the frame
+    // driver shades only after a non-empty shadow pass); live: its device count
+    if (live && *live == 0u) return;
     const uint32_t n = W * H;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
         uint32_t x = p % W, y = p / W;
@@ -551,8 +554,10 @@ void launch_surface(const float* rays, const unsigned long long* rec, uint32_t h
                                                                      normalize_n, sc, g);
 }
 void launch_shade_primary(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L, const Views& V,
-                          const uint32_t* sflags, uint32_t soft, uint32_t* pixels, hipStream_t s) {
-    k_shade_primary<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(g, W, H, Lmax, L, V, sflags, soft, pixels);
+                          const uint32_t* sflags, uint32_t soft, uint32_t* pixels, hipStream_t s,
+                          const uint32_t* live) {
+    k_shade_primary<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(g, W, H, Lmax, L, V, sflags, soft, pixels,
+                                                                           live);
 }
 void launch_shade_reflection(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L, const Views& V,
                              const unsigned long long* rec, uint32_t hit_mode, const uint32_t* sflags,

@@ -81,9 +81,20 @@ __global__ void __launch_bounds__(256) k_fill_u64(unsigned long long* p, unsigne
 // ---------------------------------------------------------- tile totals --
 // mode 0: count flags == 0 (live rays); mode 1: sum of values;
 // mode 2: count chunk heads (keys[x] != keys[x-1]); mode 3: sum of flags.
+// d_n (optional): the element count in device memory, bounded by n (the
+// capacity the grid covers); blocks past it exit.
+__device__ __forceinline__ bool device_count(const uint32_t* d_n, uint64_t& n) {
+    if (!d_n) return true;
+    const uint64_t dn = *d_n;
+    if (dn < n) n = dn;
+    return (uint64_t)blockIdx.x * TILE < n;
+}
+
 template <int MODE>
-__global__ void __launch_bounds__(256) k_tile_total(const uint32_t* in, uint64_t n, uint32_t* tile_totals) {
+__global__ void __launch_bounds__(256) k_tile_total(const uint32_t* in, uint64_t n, uint32_t* tile_totals,
+                                                    const uint32_t* d_n) {
     __shared__ uint32_t lds[WAVES];
+    if (!device_count(d_n, n)) return;
     const uint64_t base = (uint64_t)blockIdx.x * TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
     uint32_t v[16];
     load16(in, base, n, v, MODE == 0 ? 1u : 0u);
@@ -107,8 +118,7 @@ __global__ void __launch_bounds__(256) k_tile_total(const uint32_t* in, uint64_t
 
 // Single-block exclusive scan of up to MAX_TILES tile totals (in place);
 // the grand total goes to *total and to tiles[ntiles].
-__global__ void __launch_bounds__(1024) k_scan_tiles(uint32_t* tiles, uint32_t ntiles, uint32_t* total) {
-    __shared__ uint32_t lds[16];
+__device__ __forceinline__ void scan_tiles_block(uint32_t* tiles, uint32_t ntiles, uint32_t* total, uint32_t* lds) {
     const uint32_t per = (ntiles + 1023) / 1024;
     const uint32_t b = threadIdx.x * per;
     uint32_t s = 0;
@@ -124,6 +134,23 @@ __global__ void __launch_bounds__(1024) k_scan_tiles(uint32_t* tiles, uint32_t n
         if (b + i < ntiles) { uint32_t t = tiles[b + i]; tiles[b + i] = run; run += t; }
     }
     if (threadIdx.x == 0) { tiles[ntiles] = tot; if (total) *total = tot; }
+}
+
+// d_n (optional): the tiles of *d_n elements, at most ntiles.
+__global__ void __launch_bounds__(1024) k_scan_tiles(uint32_t* tiles, uint32_t ntiles, uint32_t* total,
+                                                     const uint32_t* d_n) {
+    __shared__ uint32_t lds[16];
+    if (d_n) ntiles = min(ntiles, tiles_of(*d_n));
+    scan_tiles_block(tiles, ntiles, total, lds);
+}
+
+// The decompression's scan of tile totals (its own name in kernel traces);
+// d_n: device-sized, the tiles of *d_n chunks.
+__global__ void __launch_bounds__(1024) k_unpack_scan(uint32_t* tiles, uint32_t ntiles, uint32_t* total,
+                                                      const uint32_t* d_n) {
+    __shared__ uint32_t lds[16];
+    if (d_n) ntiles = min(ntiles, tiles_of(*d_n));
+    scan_tiles_block(tiles, ntiles, total, lds);
 }
 
 // ------------------------------------------------------------- trimming --
@@ -153,8 +180,9 @@ __global__ void __launch_bounds__(256) k_trim_scatter(const uint32_t* flags, con
 // Tile-local scan writer: mode 0 inclusive, mode 1 exclusive.
 template <int MODE>
 __global__ void __launch_bounds__(256) k_scan_write(const uint32_t* in, uint64_t n, const uint32_t* tile_offsets,
-                                                    uint32_t* out) {
+                                                    uint32_t* out, const uint32_t* d_n) {
     __shared__ uint32_t lds[WAVES];
+    if (!device_count(d_n, n)) return;
     const uint64_t base = (uint64_t)blockIdx.x * TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
     uint32_t v[16];
     load16(in, base, n, v, 0u);
@@ -175,8 +203,9 @@ __global__ void __launch_bounds__(256) k_scan_write(const uint32_t* in, uint64_t
 // ----------------------------------------------------------- compression --
 __global__ void __launch_bounds__(256) k_compress_scatter(const uint32_t* keys, uint64_t n,
                                                           const uint32_t* tile_offsets, uint32_t* bases,
-                                                          uint32_t* ckeys, uint32_t* cvalues) {
+                                                          uint32_t* ckeys, uint32_t* cvalues, const uint32_t* d_n) {
     __shared__ uint32_t lds[WAVES];
+    if (!device_count(d_n, n)) return;
     const uint64_t base = (uint64_t)blockIdx.x * TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
     uint32_t k[16];
     load16(keys, base, n, k, 0u);
@@ -199,7 +228,8 @@ __global__ void __launch_bounds__(256) k_compress_scatter(const uint32_t* keys, 
 }
 
 __global__ void __launch_bounds__(256) k_chunk_sizes(const uint32_t* bases, const uint32_t* d_chunks, uint32_t n,
-                                                     uint32_t* sizes) {
+                                                     uint32_t* sizes, const uint32_t* d_n) {
+    if (d_n) n = min(n, *d_n);
     const uint32_t C = *d_chunks;
     for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < C; x += gridDim.x * blockDim.x)
         sizes[x] = (x + 1 < C ? bases[x + 1] : n) - bases[x];
@@ -345,8 +375,10 @@ __device__ uint32_t block_scan_run(const uint32_t* in, uint32_t n, uint32_t chun
 template <int MODE>
 __global__ void __launch_bounds__(1024) k_scan_block(const uint32_t* in, uint32_t n, uint32_t chunk, bool fast,
                                                      uint32_t* out, uint32_t* total, unsigned long long* zero,
-                                                     uint32_t zero_n, const uint32_t* in2, uint32_t* out2) {
+                                                     uint32_t zero_n, const uint32_t* in2, uint32_t* out2,
+                                                     const PassSizes* ps) {
     __shared__ uint32_t red[16];
+    if (ps) n = ps->ntop + 1;             // device-sized: the pass's top nodes + 1
     if (zero && threadIdx.x < zero_n) zero[threadIdx.x] = 0ull;
     const uint32_t tot = block_scan_run<MODE>(in, n, chunk, fast, out, red);
     if (threadIdx.x == 0 && total) *total = tot;
@@ -530,8 +562,12 @@ __global__ void __launch_bounds__(256) k_pack_write(const uint32_t* flags, const
 // chunks that cover the range, their offsets and bases are staged in LDS,
 // and each output finds its chunk by a binary search there.
 __global__ void __launch_bounds__(256) k_unpack_sizes(const uint32_t* sv, const uint32_t* bases, uint64_t C,
-                                                      uint32_t* tile_totals) {
+                                                      uint32_t* tile_totals, const PassSizes* ps) {
     __shared__ uint32_t red[WAVES];
+    if (ps) {
+        C = min(C, (uint64_t)ps->C);
+        if (blockIdx.x >= tiles_of((uint32_t)C)) return;
+    }
     const uint64_t base = (uint64_t)blockIdx.x * TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
     uint32_t v[16];
     load16(sv, base, C, v, 0u);
@@ -548,8 +584,12 @@ __global__ void __launch_bounds__(256) k_unpack_sizes(const uint32_t* sv, const 
 // soff[C] = N.
 __global__ void __launch_bounds__(256) k_unpack_offsets(const uint32_t* sv, const uint32_t* bases, uint64_t C,
                                                         const uint32_t* tile_offsets, uint32_t* soff,
-                                                        uint32_t* sbase) {
+                                                        uint32_t* sbase, const PassSizes* ps) {
     __shared__ uint32_t red[WAVES];
+    if (ps) {
+        C = min(C, (uint64_t)ps->C);
+        if (blockIdx.x >= tiles_of((uint32_t)C)) return;
+    }
     const uint64_t base = (uint64_t)blockIdx.x * TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
     uint32_t v[16], b0[16], sz[16];
     load16(sv, base, C, v, 0u);
@@ -592,11 +632,14 @@ __device__ __forceinline__ uint32_t upper_index(const uint32_t* a, uint32_t lo, 
 }
 
 __global__ void __launch_bounds__(256) k_unpack_expand(const uint32_t* soff, const uint32_t* sbase, uint32_t C,
-                                                       uint32_t N, const uint32_t* tvalues, uint32_t* out_values) {
+                                                       uint32_t N, const uint32_t* tvalues, uint32_t* out_values,
+                                                       const PassSizes* ps) {
     __shared__ uint32_t s_off[TILE + 1];
     __shared__ uint32_t s_base[TILE];
     __shared__ uint32_t s_x[2];
+    if (ps) { C = min(C, ps->C); N = min(N, ps->N); }
     const uint32_t p0 = blockIdx.x * TILE;
+    if (p0 >= N) return;                  // device-sized launches cover capacity
     const uint32_t p1 = p0 + TILE < N ? p0 + TILE : N;
     if (threadIdx.x < 2) s_x[threadIdx.x] = upper_index(soff, 0, C, threadIdx.x == 0 ? p0 : p1 - 1);
     __syncthreads();
@@ -638,8 +681,13 @@ __global__ void __launch_bounds__(256) k_unpack_expand(const uint32_t* soff, con
 // consecutive digits so that a block scan keeps digit order.
 template <uint32_t RB>
 __global__ void __launch_bounds__(256) k_radix_upsweep(const uint32_t* keys, uint64_t n, uint32_t shift,
-                                                       uint32_t* hist, uint32_t ntiles) {
+                                                       uint32_t* hist, uint32_t ntiles, const uint32_t* d_n) {
     constexpr uint32_t NB = 1u << RB, MASK = NB - 1u;
+    if (d_n) {                            // device-sized: n and the tile count from memory
+        n = min(n, (uint64_t)*d_n);
+        ntiles = tiles_of((uint32_t)n);
+        if (blockIdx.x >= ntiles) return;
+    }
     __shared__ uint32_t cnt[WAVES][NB];
     for (uint32_t i = threadIdx.x; i < WAVES * NB; i += blockDim.x) (&cnt[0][0])[i] = 0;
     __syncthreads();
@@ -664,8 +712,10 @@ __global__ void __launch_bounds__(256) k_radix_upsweep(const uint32_t* keys, uin
 }
 
 // One block per digit: exclusive scan of that digit's tile counts in place.
-__global__ void __launch_bounds__(256) k_radix_rowscan(uint32_t* hist, uint32_t ntiles, uint32_t* digit_totals) {
+__global__ void __launch_bounds__(256) k_radix_rowscan(uint32_t* hist, uint32_t ntiles, uint32_t* digit_totals,
+                                                       const uint32_t* d_n) {
     __shared__ uint32_t lds[WAVES];
+    if (d_n) ntiles = min(ntiles, tiles_of(*d_n));
     uint32_t* row = hist + (size_t)blockIdx.x * ntiles;
     const uint32_t per = (ntiles + 255) / 256;
     const uint32_t b = threadIdx.x * per;
@@ -685,8 +735,13 @@ template <bool ROWSCAN, uint32_t RB>
 __global__ void __launch_bounds__(256) k_radix_downsweep(const uint32_t* keys, const uint32_t* vals, uint64_t n,
                                                          uint32_t shift, const uint32_t* hist, uint32_t ntiles,
                                                          const uint32_t* digit_totals, uint32_t* okeys,
-                                                         uint32_t* ovals) {
+                                                         uint32_t* ovals, const uint32_t* d_n) {
     constexpr uint32_t NB = 1u << RB, MASK = NB - 1u, DPT = NB / 256u;
+    if (d_n) {
+        n = min(n, (uint64_t)*d_n);
+        ntiles = tiles_of((uint32_t)n);
+        if (blockIdx.x >= ntiles) return;
+    }
     __shared__ uint32_t whist[WAVES][NB];
     __shared__ uint32_t tile_start[NB];
     __shared__ uint32_t gbase[NB];
@@ -799,7 +854,8 @@ __global__ void __launch_bounds__(256) k_radix_downsweep(const uint32_t* keys, c
 
 // --------------------------------------------------------- decompression --
 __global__ void __launch_bounds__(256) k_skeleton(const uint32_t* sizes, const uint32_t* sorted_cvalues, uint64_t C,
-                                                  uint32_t* skeleton) {
+                                                  uint32_t* skeleton, const uint32_t* d_c) {
+    if (d_c && *d_c < C) C = *d_c;
     for (uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; x < C; x += (uint64_t)gridDim.x * blockDim.x)
         skeleton[x] = sizes[sorted_cvalues[x]];
 }
@@ -815,9 +871,14 @@ __device__ __forceinline__ uint32_t find_chunk(const uint32_t* scan, uint32_t C,
     return lo;
 }
 
+// d_c, d_n (optional): the chunk and ray totals in device memory, at most C
+// and N.
 __global__ void __launch_bounds__(256) k_expand(const uint32_t* bases, const uint32_t* sorted_cvalues,
                                                 const uint32_t* scan, uint32_t C, uint32_t N, const uint32_t* tk,
-                                                const uint32_t* tv, uint32_t* ok, uint32_t* ov) {
+                                                const uint32_t* tv, uint32_t* ok, uint32_t* ov, const uint32_t* d_c,
+                                                const uint32_t* d_n) {
+    if (d_c) C = min(C, *d_c);
+    if (d_n) N = min(N, *d_n);
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
         uint32_t x = find_chunk(scan, C, p);
         uint32_t idx = bases[sorted_cvalues[x]] + (p - scan[x]);
@@ -872,27 +933,32 @@ hipError_t run_trim(const uint32_t* flags, const uint32_t* keys, const uint32_t*
     if (nt > MAX_TILES) return hipErrorInvalidValue;
     if (nt == 0) { launch_fill_u32(d_total, 0, 1, s); return hipGetLastError(); }
     uint32_t* tiles = sc.tile_counts.as<uint32_t>();
-    k_tile_total<0><<<nt, 256, 0, s>>>(flags, n, tiles);
-    k_scan_tiles<<<1, 1024, 0, s>>>(tiles, nt, d_total);
+    k_tile_total<0><<<nt, 256, 0, s>>>(flags, n, tiles, nullptr);
+    k_scan_tiles<<<1, 1024, 0, s>>>(tiles, nt, d_total, nullptr);
     k_trim_scatter<<<nt, 256, 0, s>>>(flags, keys, values, n, tiles, out_keys, out_values);
     return hipGetLastError();
 }
 
+// d_n (optional): the element count in device memory, at most n (launches
+// cover n; the total lands in sc.total either way).
 static hipError_t run_scan(const uint32_t* in, uint32_t* out, uint64_t n, ScanScratch& sc, hipStream_t s,
-                           bool inclusive) {
+                           bool inclusive, const uint32_t* d_n = nullptr) {
     uint32_t nt = tiles_for(n);
     if (nt > MAX_TILES) return hipErrorInvalidValue;
-    if (nt == 0) return hipSuccess;
-    if (!inclusive && n <= SMALL_SCAN) {     // one launch instead of three
+    if (nt == 0) {
+        if (d_n) launch_fill_u32(sc.total.as<uint32_t>(), 0u, 1, s);
+        return hipGetLastError();
+    }
+    if (!inclusive && n <= SMALL_SCAN && !d_n) {     // one launch instead of three
         k_scan_block<0><<<1, 1024, 0, s>>>(in, (uint32_t)n, 1u, true, out, sc.total.as<uint32_t>(), nullptr, 0u,
-                                            nullptr, nullptr);
+                                            nullptr, nullptr, nullptr);
         return hipGetLastError();
     }
     uint32_t* tiles = sc.tile_counts.as<uint32_t>();
-    k_tile_total<1><<<nt, 256, 0, s>>>(in, n, tiles);
-    k_scan_tiles<<<1, 1024, 0, s>>>(tiles, nt, sc.total.as<uint32_t>());
-    if (inclusive) k_scan_write<0><<<nt, 256, 0, s>>>(in, n, tiles, out);
-    else k_scan_write<1><<<nt, 256, 0, s>>>(in, n, tiles, out);
+    k_tile_total<1><<<nt, 256, 0, s>>>(in, n, tiles, d_n);
+    k_scan_tiles<<<1, 1024, 0, s>>>(tiles, nt, sc.total.as<uint32_t>(), d_n);
+    if (inclusive) k_scan_write<0><<<nt, 256, 0, s>>>(in, n, tiles, out, d_n);
+    else k_scan_write<1><<<nt, 256, 0, s>>>(in, n, tiles, out, d_n);
     return hipGetLastError();
 }
 
@@ -905,13 +971,14 @@ hipError_t run_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, Sca
 
 hipError_t run_item_scan(const uint32_t* cand, uint32_t ntop, uint32_t chunk, uint32_t max_cand, uint32_t* counts,
                          uint32_t* first, ScanScratch& sc, hipStream_t s, unsigned long long* zero,
-                         uint32_t zero_n, const uint32_t* npass, uint32_t* off) {
+                         uint32_t zero_n, const uint32_t* npass, uint32_t* off, const PassSizes* ps) {
     if (chunk == 0 || zero_n > 1024) return hipErrorInvalidValue;
     if ((uint64_t)ntop + 1 <= SMALL_SCAN) {
         const bool fast = div_hi_ok((uint64_t)max_cand + chunk - 1, chunk);
-        k_scan_block<1><<<1, 1024, 0, s>>>(cand, ntop + 1, chunk, fast, first, nullptr, zero, zero_n, npass, off);
+        k_scan_block<1><<<1, 1024, 0, s>>>(cand, ntop + 1, chunk, fast, first, nullptr, zero, zero_n, npass, off, ps);
         return hipGetLastError();
     }
+    if (ps) return hipErrorInvalidValue;  // device-sized item scans are single-block (ntop < SMALL_SCAN)
     if (zero) launch_fill_u64(zero, 0ull, zero_n, s);
     if (npass) {
         hipError_t e = run_exclusive_scan(npass, off, (uint64_t)ntop + 1, sc, s);
@@ -922,15 +989,15 @@ hipError_t run_item_scan(const uint32_t* cand, uint32_t ntop, uint32_t chunk, ui
 }
 
 hipError_t run_compress(const uint32_t* keys, uint64_t n, uint32_t* bases, uint32_t* sizes, uint32_t* ckeys,
-                        uint32_t* cvalues, uint32_t* d_total, ScanScratch& sc, hipStream_t s) {
+                        uint32_t* cvalues, uint32_t* d_total, ScanScratch& sc, hipStream_t s, const uint32_t* d_n) {
     uint32_t nt = tiles_for(n);
     if (nt > MAX_TILES) return hipErrorInvalidValue;
     if (nt == 0) { launch_fill_u32(d_total, 0, 1, s); return hipGetLastError(); }
     uint32_t* tiles = sc.tile_counts.as<uint32_t>();
-    k_tile_total<2><<<nt, 256, 0, s>>>(keys, n, tiles);
-    k_scan_tiles<<<1, 1024, 0, s>>>(tiles, nt, d_total);
-    k_compress_scatter<<<nt, 256, 0, s>>>(keys, n, tiles, bases, ckeys, cvalues);
-    k_chunk_sizes<<<grid_for(n, 256, 8192), 256, 0, s>>>(bases, d_total, (uint32_t)n, sizes);
+    k_tile_total<2><<<nt, 256, 0, s>>>(keys, n, tiles, d_n);
+    k_scan_tiles<<<1, 1024, 0, s>>>(tiles, nt, d_total, d_n);
+    k_compress_scatter<<<nt, 256, 0, s>>>(keys, n, tiles, bases, ckeys, cvalues, d_n);
+    k_chunk_sizes<<<grid_for(n, 256, 8192), 256, 0, s>>>(bases, d_total, (uint32_t)n, sizes, d_n);
     return hipGetLastError();
 }
 
@@ -960,11 +1027,79 @@ hipError_t run_unpack(const uint32_t* bases, const uint32_t* sorted_cvalues, con
     if (nt > MAX_TILES) return hipErrorInvalidValue;
     if (nt == 0 || N == 0) return hipSuccess;
     uint32_t* tiles = sc.tile_counts.as<uint32_t>();
-    k_unpack_sizes<<<nt, 256, 0, s>>>(sorted_cvalues, bases, C, tiles);
-    k_scan_tiles<<<1, 1024, 0, s>>>(tiles, nt, sc.total.as<uint32_t>());
-    k_unpack_offsets<<<nt, 256, 0, s>>>(sorted_cvalues, bases, C, tiles, soff, sbase);
-    k_unpack_expand<<<tiles_for(N), 256, 0, s>>>(soff, sbase, (uint32_t)C, (uint32_t)N, tvalues, out_values);
+    k_unpack_sizes<<<nt, 256, 0, s>>>(sorted_cvalues, bases, C, tiles, nullptr);
+    k_unpack_scan<<<1, 1024, 0, s>>>(tiles, nt, sc.total.as<uint32_t>(), nullptr);
+    k_unpack_offsets<<<nt, 256, 0, s>>>(sorted_cvalues, bases, C, tiles, soff, sbase, nullptr);
+    k_unpack_expand<<<tiles_for(N), 256, 0, s>>>(soff, sbase, (uint32_t)C, (uint32_t)N, tvalues, out_values, nullptr);
     return hipGetLastError();
+}
+
+hipError_t run_unpack_dev(const uint32_t* bases, const uint32_t* sorted_cvalues, const uint32_t* tvalues,
+                          uint64_t cap, const PassSizes* ps, uint32_t* soff, uint32_t* sbase, uint32_t* out_values,
+                          ScanScratch& sc, hipStream_t s) {
+    const uint32_t nt = tiles_for(cap);
+    if (nt > MAX_TILES) return hipErrorInvalidValue;
+    if (nt == 0) return hipSuccess;
+    uint32_t* tiles = sc.tile_counts.as<uint32_t>();
+    k_unpack_sizes<<<nt, 256, 0, s>>>(sorted_cvalues, bases, cap, tiles, ps);
+    k_unpack_scan<<<1, 1024, 0, s>>>(tiles, nt, sc.total.as<uint32_t>(), &ps->C);
+    k_unpack_offsets<<<nt, 256, 0, s>>>(sorted_cvalues, bases, cap, tiles, soff, sbase, ps);
+    k_unpack_expand<<<nt, 256, 0, s>>>(soff, sbase, (uint32_t)cap, (uint32_t)cap, tvalues, out_values, ps);
+    return hipGetLastError();
+}
+
+// Hierarchy levels over q.N rays (as level_sizes on the host).
+__device__ void fill_levels(PassSizes& q, uint32_t leaf, uint32_t branch, uint32_t depth) {
+    uint32_t n = (q.N + leaf - 1) / leaf, off = 0;
+    for (uint32_t k = 0; k < 4; k++) {
+        q.count[k] = k < depth ? n : 0u;
+        q.offset[k] = off;
+        if (k < depth) { off += n; n = (n + branch - 1) / branch; }
+    }
+    q.ntop = q.count[depth - 1];
+}
+
+__global__ void k_pass_sizes(const uint32_t* totals, uint32_t leaf, uint32_t branch, uint32_t depth, PassSizes* ps,
+                             uint32_t* rec) {
+    PassSizes q{};
+    q.N = totals[0];
+    q.C = totals[1];
+    fill_levels(q, leaf, branch, depth);
+    *ps = q;
+    if (rec) { rec[0] = q.N; rec[1] = q.C; }
+}
+
+__global__ void k_fixed_sizes(uint32_t N, uint32_t C, uint32_t leaf, uint32_t branch, uint32_t depth, PassSizes* ps,
+                              uint32_t* rec) {
+    PassSizes q{};
+    q.N = N;
+    q.C = C;
+    fill_levels(q, leaf, branch, depth);
+    *ps = q;
+    if (rec) { rec[0] = N; rec[1] = C; }
+}
+
+__global__ void k_ray_sizes(const uint32_t* d_n, uint32_t n_cap, uint32_t leaf, uint32_t branch, uint32_t depth,
+                            PassSizes* ps) {
+    PassSizes q{};
+    q.N = min(*d_n, n_cap);
+    fill_levels(q, leaf, branch, depth);
+    *ps = q;
+}
+
+void launch_pass_sizes(const uint32_t* totals, uint32_t leaf, uint32_t branch, uint32_t depth, PassSizes* ps,
+                       uint32_t* rec, hipStream_t s) {
+    k_pass_sizes<<<1, 1, 0, s>>>(totals, leaf, branch, depth, ps, rec);
+}
+
+void launch_fixed_sizes(uint32_t N, uint32_t C, uint32_t leaf, uint32_t branch, uint32_t depth, PassSizes* ps,
+                        uint32_t* rec, hipStream_t s) {
+    k_fixed_sizes<<<1, 1, 0, s>>>(N, C, leaf, branch, depth, ps, rec);
+}
+
+void launch_ray_sizes(const uint32_t* d_n, uint32_t n_cap, uint32_t leaf, uint32_t branch, uint32_t depth,
+                      PassSizes* ps, hipStream_t s) {
+    k_ray_sizes<<<1, 1, 0, s>>>(d_n, n_cap, leaf, branch, depth, ps);
 }
 
 // Up to this many tiles the downsweep sums the digit rows itself (saves a launch
@@ -973,16 +1108,18 @@ hipError_t run_unpack(const uint32_t* bases, const uint32_t* sorted_cvalues, con
 // 64 0.46, 24 0.40, 8 0.38, 0 (always the separate row scan) 0.39.
 constexpr uint32_t ROWSCAN_IN_DOWNSWEEP = 8;
 
+// d_n: device-sized (n and nt are capacities; the kernels read the count)
 template <uint32_t RB>
 void radix_pass(const uint32_t* ik, const uint32_t* iv, uint64_t n, uint32_t shift, uint32_t nt, uint32_t* ok,
-                uint32_t* ov, ScanScratch& sc, hipStream_t s) {
-    k_radix_upsweep<RB><<<nt, 256, 0, s>>>(ik, n, shift, sc.hist.as<uint32_t>(), nt);
-    if (nt <= ROWSCAN_IN_DOWNSWEEP) {
-        k_radix_downsweep<true, RB><<<nt, 256, 0, s>>>(ik, iv, n, shift, sc.hist.as<uint32_t>(), nt, nullptr, ok, ov);
+                uint32_t* ov, ScanScratch& sc, hipStream_t s, const uint32_t* d_n = nullptr) {
+    k_radix_upsweep<RB><<<nt, 256, 0, s>>>(ik, n, shift, sc.hist.as<uint32_t>(), nt, d_n);
+    if (nt <= ROWSCAN_IN_DOWNSWEEP && !d_n) {
+        k_radix_downsweep<true, RB><<<nt, 256, 0, s>>>(ik, iv, n, shift, sc.hist.as<uint32_t>(), nt, nullptr, ok, ov,
+                                                       nullptr);
     } else {
-        k_radix_rowscan<<<1u << RB, 256, 0, s>>>(sc.hist.as<uint32_t>(), nt, sc.digit_totals.as<uint32_t>());
+        k_radix_rowscan<<<1u << RB, 256, 0, s>>>(sc.hist.as<uint32_t>(), nt, sc.digit_totals.as<uint32_t>(), d_n);
         k_radix_downsweep<false, RB><<<nt, 256, 0, s>>>(ik, iv, n, shift, sc.hist.as<uint32_t>(), nt,
-                                                        sc.digit_totals.as<uint32_t>(), ok, ov);
+                                                        sc.digit_totals.as<uint32_t>(), ok, ov, d_n);
     }
 }
 
@@ -1021,23 +1158,47 @@ hipError_t run_radix_sort(const uint32_t* keys, const uint32_t* values, uint64_t
     return hipGetLastError();
 }
 
+hipError_t run_radix_sort_dev(const uint32_t* keys, const uint32_t* values, uint64_t n_cap, const uint32_t* d_n,
+                              uint32_t bits, uint32_t* out_keys, uint32_t* out_values, ScanScratch& sc,
+                              hipStream_t s) {
+    if (n_cap == 0) return hipSuccess;
+    const uint32_t nt = tiles_for(n_cap);
+    if (nt > MAX_TILES) return hipErrorInvalidValue;
+    if (bits == 0) bits = 1;
+    if (bits > 32) bits = 32;
+    const uint32_t rb = (bits + 8) / 9 < (bits + 7) / 8 ? 9u : 8u;
+    const uint32_t passes = (bits + rb - 1) / rb;
+    if (keys == out_keys || values == out_values || keys == out_values || values == out_keys)
+        return hipErrorInvalidValue;
+    uint32_t* bk[2] = {sc.ping_k.as<uint32_t>(), out_keys};
+    uint32_t* bv[2] = {sc.ping_v.as<uint32_t>(), out_values};
+    const uint32_t* ik = keys;
+    const uint32_t* iv = values;
+    for (uint32_t pass = 0; pass < passes; pass++) {
+        uint32_t dst = ((passes - 1 - pass) & 1u) ? 0u : 1u;
+        if (rb == 9) radix_pass<9>(ik, iv, n_cap, 9 * pass, nt, bk[dst], bv[dst], sc, s, d_n);
+        else radix_pass<8>(ik, iv, n_cap, 8 * pass, nt, bk[dst], bv[dst], sc, s, d_n);
+        ik = bk[dst];
+        iv = bv[dst];
+    }
+    return hipGetLastError();
+}
+
 hipError_t run_decompress(const uint32_t* bases, const uint32_t* sizes, const uint32_t* sorted_cvalues,
                           const uint32_t* tkeys, const uint32_t* tvalues, uint32_t* skeleton, uint32_t* scan,
                           uint64_t C, uint64_t N, uint32_t* out_keys, uint32_t* out_values, ScanScratch& sc,
-                          hipStream_t s) {
+                          hipStream_t s, const uint32_t* d_c) {
     if (C == 0) return hipSuccess;
-    k_skeleton<<<grid_for(C, 256, 8192), 256, 0, s>>>(sizes, sorted_cvalues, C, skeleton);
-    hipError_t e = run_exclusive_scan(skeleton, scan, C, sc, s);
+    if (C > 0xFFFFFFFFull) return hipErrorInvalidValue;
+    k_skeleton<<<grid_for(C, 256, 8192), 256, 0, s>>>(sizes, sorted_cvalues, C, skeleton, d_c);
+    hipError_t e = run_scan(skeleton, scan, C, sc, s, false, d_c);
     if (e != hipSuccess) return e;
-    if (N == 0) {   // ray total not known by the caller: read the scan total
-        uint32_t t = 0;
-        if ((e = hipMemcpyAsync(&t, sc.total.p, 4, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
-        if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
-        N = t;
-        if (N == 0) return hipSuccess;
-    }
-    k_expand<<<grid_for(N, 256, 16384), 256, 0, s>>>(bases, sorted_cvalues, scan, (uint32_t)C, (uint32_t)N, tkeys,
-                                                     tvalues, out_keys, out_values);
+    // N == 0: the ray total is the scan's, read on the device; the grid
+    // strides over it
+    const uint32_t* d_n = N == 0 ? sc.total.as<uint32_t>() : nullptr;
+    k_expand<<<N == 0 ? 2048u : grid_for(N, 256, 16384), 256, 0, s>>>(
+        bases, sorted_cvalues, scan, (uint32_t)C, N == 0 ? 0xFFFFFFFFu : (uint32_t)N, tkeys, tvalues, out_keys,
+        out_values, d_c, d_n);
     return hipGetLastError();
 }
 

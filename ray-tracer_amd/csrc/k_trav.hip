@@ -41,8 +41,10 @@ inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap = 1u << 20) {
 // unused for closest-hit passes.  slot_shift: 4 for soft shadows (16 rays per
 // pixel-light slot), else 0.
 __global__ void __launch_bounds__(256) k_gather(const float* rays, const uint32_t* sorted_values, uint32_t N,
-                                                const float4* shadow_pos, const float* tmax_in, uint32_t W,
-                                                uint32_t Lmax, uint32_t slot_shift, float4* rec0, float4* rec1) {
+                                                const PassSizes* ps, const float4* shadow_pos, const float* tmax_in,
+                                                uint32_t W, uint32_t Lmax, uint32_t slot_shift, float4* rec0,
+                                                float4* rec1) {
+    if (ps) N = min(N, ps->N);
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
         uint32_t slot = sorted_values[p];
         const float2* r = reinterpret_cast<const float2*>(rays + 6 * (size_t)slot);
@@ -68,8 +70,10 @@ __global__ void __launch_bounds__(256) k_gather(const float* rays, const uint32_
 // here the node stops at the last live ray.
 template <bool FROM_RECORDS>
 __global__ void __launch_bounds__(256) k_level0(const float4* rec0, const float4* rec1, const float* rays,
-                                                const uint32_t* sorted_values, uint32_t N, uint32_t leaf,
-                                                float r0, float s0, uint32_t mode, float4* nodes, float2* aux) {
+                                                const uint32_t* sorted_values, uint32_t N, const PassSizes* ps,
+                                                uint32_t leaf, float r0, float s0, uint32_t mode, float4* nodes,
+                                                float2* aux) {
+    if (ps) N = min(N, ps->N);
     const uint32_t n0 = (N + leaf - 1) / leaf;
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n0; j += gridDim.x * blockDim.x) {
         auto fetch = [&](uint32_t p, v3& o, v3& d) {
@@ -120,7 +124,11 @@ __global__ void __launch_bounds__(256) k_level0(const float4* rec0, const float4
 
 __global__ void __launch_bounds__(256) k_levelN(float4* nodes, float2* aux, uint32_t read_off, uint32_t read_count,
                                                 uint32_t write_off, uint32_t write_count, uint32_t branch,
-                                                uint32_t mode) {
+                                                uint32_t mode, const PassSizes* ps, uint32_t level) {
+    if (ps) {       // device-sized: level `level` over level - 1
+        read_off = ps->offset[level - 1]; read_count = ps->count[level - 1];
+        write_off = ps->offset[level]; write_count = ps->count[level];
+    }
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < write_count; j += gridDim.x * blockDim.x) {
         size_t c0 = read_off + (size_t)j * branch;
         float4 sphere = nodes[2 * c0], cone = nodes[2 * c0 + 1];
@@ -137,6 +145,16 @@ __global__ void __launch_bounds__(256) k_levelN(float4* nodes, float2* aux, uint
 }
 
 // ---------------------------------------------------- fused traversal ----
+// Device-sized passes: the pass's N and level sizes from device memory into
+// the kernel's copy of the parameters.
+__device__ __forceinline__ void apply_sizes(TravParams& P) {
+    if (!P.ps) return;
+    const PassSizes q = *P.ps;
+    P.N = q.N;
+#pragma unroll
+    for (int k = 0; k < 4; k++) { P.level_count[k] = q.count[k]; P.level_offset[k] = q.offset[k]; }
+}
+
 // Object whose triangle range contains t (objects tile [0, T) in order);
 // wave-uniform binary search over the first-triangle column.
 __device__ __forceinline__ uint32_t object_of(const uint4* ranges, uint32_t K, uint32_t t) {
@@ -318,6 +336,7 @@ __device__ __forceinline__ uint32_t wave_incl_add(uint32_t x) {
 template <bool LIST>
 __global__ void __launch_bounds__(256) k_cull(TravParams P, uint32_t* npass, uint32_t* cand, uint4* obj_ent,
                                               uint32_t* obj_cum, unsigned long long* pmask) {
+    apply_sizes(P);
     const uint32_t lane = lane_id();
     const uint32_t top = blockIdx.x * 4 + (threadIdx.x >> 6);
     const uint32_t ntop = P.level_count[1];
@@ -504,18 +523,31 @@ __global__ void __launch_bounds__(256) k_item_desc(TravParams P, uint4* desc) {
 }
 
 // List pass and work-item descriptors in one (CRSH): one wave per top node
-// walks the passing objects of its count-pass mask in order, writes each as a
-// list entry with the node's running cluster count before it (as k_cull<true>)
-// and, for the items whose first cluster s0 = i * chunk falls inside the
-// object's clusters [c0, c0 + ncl), the descriptor (top, entry, s0 - c0,
-// clusters | single << 31) -- the entry k_item_desc finds by binary search.
-// Every listed object has at least one cluster, so each item's start lies in
-// exactly one of them.
+// walks the passing objects of its count-pass mask in order, 64 objects (one
+// mask word) at a time, writes each as a list entry with the node's running
+// cluster count before it (as k_cull<true>), and then the descriptors of the
+// items whose first cluster s0 = i * chunk falls inside those objects'
+// clusters, one item per lane: (top, entry, s0 - c0, clusters | single << 31)
+// for the object with the largest first cluster c0 <= s0 (a binary search
+// over the word's objects in LDS) -- the entry k_item_desc finds by binary
+// search.  Every listed object has at least one cluster, so each item's start
+// lies in exactly one of them.  (Writing the descriptors per object instead
+// left one lane writing all of a large object's items while the others
+// idled.)
 __global__ void __launch_bounds__(256) k_cull_items(TravParams P, uint4* obj_ent, uint32_t* obj_cum,
                                                     const unsigned long long* pmask, uint4* desc) {
-    const uint32_t lane = lane_id();
-    const uint32_t top = blockIdx.x * 4 + (threadIdx.x >> 6);
+    __shared__ uint32_t s_c0[4][64];   // per wave: first cluster of each passing object of the word
+    __shared__ uint32_t s_e[4][64];    // and its list entry
+    apply_sizes(P);
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    const uint32_t top = blockIdx.x * 4 + w;
     if (top >= P.level_count[1]) return;
+    // device-sized passes: a list or descriptor table over its capacity is
+    // not written (the frame reports an overflow and is rendered again)
+    if (P.ps && (P.obj_off[top + 1] > P.list_cap || P.item_first[top + 1] > P.items_cap)) {
+        if (lane == 0) atomicOr(P.overflow, 1u);
+        return;
+    }
     const uint32_t words = (P.K + 63) / 64;
     const unsigned long long* pm = pmask + (size_t)top * words;
     const uint32_t base = P.obj_off[top];
@@ -536,19 +568,31 @@ __global__ void __launch_bounds__(256) k_cull_items(TravParams P, uint4* obj_ent
         }
         const uint32_t incl = wave_incl_add(ncl);
         if (po) {
-            const uint32_t e = base + n + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            const uint32_t e = base + n + rank;
             const uint32_t c0 = cum + incl - ncl;
             obj_ent[e] = make_uint4(r.x, r.y, P.obj_cl_base[oi], r.z);
             obj_cum[e] = c0;
-            const uint32_t i1 = (c0 + ncl + chunk - 1u) / chunk;
-            for (uint32_t i = (c0 + chunk - 1u) / chunk; i < i1; i++) {
-                const uint32_t s0 = i * chunk;
-                const uint32_t rem = ncand - s0 < chunk ? ncand - s0 : chunk;
-                desc[f + i] = make_uint4(top, e, s0 - c0, rem | single);
-            }
+            s_c0[w][rank] = c0;
+            s_e[w][rank] = e;
         }
-        n += (uint32_t)__popcll(m);
-        cum += __shfl(incl, 63, 64);
+        const uint32_t cnt = (uint32_t)__popcll(m);
+        const uint32_t cum_end = cum + __shfl(incl, 63, 64);
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t i1 = (cum_end + chunk - 1u) / chunk;
+        for (uint32_t i = (cum + chunk - 1u) / chunk + lane; i < i1; i += 64u) {
+            const uint32_t s0 = i * chunk;
+            uint32_t lo = 0, hi = cnt;          // s_c0[w][lo] <= s0 < s_c0[w][hi] (ascending)
+            while (hi - lo > 1u) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_c0[w][mid] <= s0) lo = mid; else hi = mid;
+            }
+            const uint32_t rem = ncand - s0 < chunk ? ncand - s0 : chunk;
+            desc[f + i] = make_uint4(top, s_e[w][lo], s0 - s_c0[w][lo], rem | single);
+        }
+        __builtin_amdgcn_wave_barrier();      // the word's LDS entries are read before the next word's
+        n += cnt;
+        cum = cum_end;
     }
 }
 
@@ -618,8 +662,14 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MODE >=
     __shared__ unsigned long long s_qrm[RM ? QCAP : 1];   // RM: rays that may meet the survivor's cluster
     __shared__ uint8_t s_grp[64];                 // passing clusters of a group, in order
     __shared__ uint32_t s_gm[64];                 // leaf mask of each cluster of a group
+    apply_sizes(P);
     const uint32_t lane = lane_id();
     const uint32_t item = blockIdx.x;
+    if (P.ps) {     // device-sized: the item total, bounded by the capacity the grid covers
+        if (*P.overflow) return;       // a list or descriptor table was cut: this frame is rendered again
+        const uint32_t it = P.item_first[P.level_count[1]];
+        P.items = it < P.items_cap ? it : P.items_cap;
+    }
     if (item >= P.items) return;
     const uint4 desc = P.item_desc[item];
     const uint32_t top = desc.x;
@@ -1184,24 +1234,28 @@ __global__ void __launch_bounds__(256) k_hits_intersect(const float* rays, const
 }  // namespace
 
 // ================================================================ launchers
-void launch_gather(const float* rays, const uint32_t* sorted_values, uint32_t N, const float4* shadow_pos,
-                   const float* tmax, uint32_t W, uint32_t Lmax, uint32_t slot_shift, float4* rec0, float4* rec1,
-                   hipStream_t s) {
-    if (N) k_gather<<<grid_for(N, 256, 65536), 256, 0, s>>>(rays, sorted_values, N, shadow_pos, tmax, W, Lmax,
+// ps: device-sized (N is the capacity the grid covers)
+void launch_gather(const float* rays, const uint32_t* sorted_values, uint32_t N, const PassSizes* ps,
+                   const float4* shadow_pos, const float* tmax, uint32_t W, uint32_t Lmax, uint32_t slot_shift,
+                   float4* rec0, float4* rec1, hipStream_t s) {
+    if (N) k_gather<<<grid_for(N, 256, 65536), 256, 0, s>>>(rays, sorted_values, N, ps, shadow_pos, tmax, W, Lmax,
                                                             slot_shift, rec0, rec1);
 }
 
-void launch_hierarchy(const float4* rec0, const float4* rec1, const float* rays, const uint32_t* sorted_values,
-                      uint32_t N, uint32_t leaf, uint32_t branch, uint32_t depth, float r0, float s0, uint32_t mode,
-                      float4* nodes, float2* aux, const uint32_t* counts, const uint32_t* offsets, hipStream_t s) {
+// ps: device-sized (N, counts and offsets are capacities; the kernels read
+// the sizes; grids stride over them)
+void launch_hierarchy(const float4* rec0, const float4* rec1, const PassSizes* ps, const float* rays,
+                      const uint32_t* sorted_values, uint32_t N, uint32_t leaf, uint32_t branch, uint32_t depth,
+                      float r0, float s0, uint32_t mode, float4* nodes, float2* aux, const uint32_t* counts,
+                      const uint32_t* offsets, hipStream_t s) {
     if (!N) return;
-    if (rec0) k_level0<true><<<grid_for(counts[0], 256), 256, 0, s>>>(rec0, rec1, nullptr, nullptr, N, leaf, r0, s0,
-                                                                       mode, nodes, aux);
-    else k_level0<false><<<grid_for(counts[0], 256), 256, 0, s>>>(nullptr, nullptr, rays, sorted_values, N, leaf, r0,
-                                                                   s0, mode, nodes, aux);
+    if (rec0) k_level0<true><<<grid_for(counts[0], 256), 256, 0, s>>>(rec0, rec1, nullptr, nullptr, N, ps, leaf, r0,
+                                                                       s0, mode, nodes, aux);
+    else k_level0<false><<<grid_for(counts[0], 256), 256, 0, s>>>(nullptr, nullptr, rays, sorted_values, N, ps, leaf,
+                                                                   r0, s0, mode, nodes, aux);
     for (uint32_t k = 1; k < depth; k++)
         k_levelN<<<grid_for(counts[k], 256), 256, 0, s>>>(nodes, aux, offsets[k - 1], counts[k - 1], offsets[k],
-                                                           counts[k], branch, mode);
+                                                           counts[k], branch, mode, ps, k);
 }
 
 void launch_cull_count(const TravParams& p, uint32_t* npass, uint32_t* cand, unsigned long long* pmask,

@@ -23,6 +23,20 @@ constexpr uint32_t SMALL_SCAN = 131072;
 
 inline uint32_t tiles_for(uint64_t n) { return (uint32_t)((n + TILE - 1) / TILE); }
 
+// Sizes of one ray pass kept in device memory (device-sized passes): the
+// pack scan's live rays N and chunks C, and the hierarchy levels over N.
+// Kernels of such a pass read them instead of taking host counts; their
+// launches are sized by capacities (slots, or the recorded pass's sizes with
+// a margin), and blocks beyond the device sizes exit at once.
+struct PassSizes {
+    uint32_t N, C;
+    uint32_t count[4], offset[4];   // nodes per level, first node of each level
+    uint32_t ntop;                  // count[depth - 1]
+    uint32_t pad;
+};
+
+__host__ __device__ inline uint32_t tiles_of(uint32_t n) { return (n + TILE - 1) / TILE; }
+
 // Process-wide count of device (re)allocations made through DevBuf.  A
 // captured HIP graph holds raw pointers, so it stays valid only while this
 // count is unchanged.
@@ -78,10 +92,11 @@ hipError_t run_inclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, Sca
 // Exclusive scan of u32 values.
 hipError_t run_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, ScanScratch& sc, hipStream_t s);
 // Run-length compression of equal consecutive keys: chunk bases, sizes, keys,
-// values (= chunk index).  Chunk count -> *d_total.
+// values (= chunk index).  Chunk count -> *d_total.  d_n (optional): the key
+// count in device memory, at most n.
 hipError_t run_compress(const uint32_t* keys, uint64_t n, uint32_t* bases, uint32_t* sizes,
                         uint32_t* ckeys, uint32_t* cvalues, uint32_t* d_total, ScanScratch& sc,
-                        hipStream_t s);
+                        hipStream_t s, const uint32_t* d_n = nullptr);
 // Trimming and compression in one walk (the engine's path): the live slots'
 // values in order -> tvalues, one chunk per run of equal live keys -> bases
 // (first live position; bases[C] = N), ckeys, cvalues (chunk index).
@@ -101,7 +116,8 @@ hipError_t run_pack(const uint32_t* flags, const uint32_t* keys, const uint32_t*
 // exclusive scan (ntop + 1 entries) goes to off in the same launch.
 hipError_t run_item_scan(const uint32_t* cand, uint32_t ntop, uint32_t chunk, uint32_t max_cand, uint32_t* counts,
                          uint32_t* first, ScanScratch& sc, hipStream_t s, unsigned long long* zero = nullptr,
-                         uint32_t zero_n = 0, const uint32_t* npass = nullptr, uint32_t* off = nullptr);
+                         uint32_t zero_n = 0, const uint32_t* npass = nullptr, uint32_t* off = nullptr,
+                         const PassSizes* ps = nullptr);
 // Decompression of run_pack's chunks in sorted order: out_values receives
 // tvalues[bases[v] .. bases[v + 1]) for v = sorted_cvalues[0 .. C) in turn
 // (N values in all).  soff (C + 1 entries) and sbase (C) are scratch.
@@ -112,12 +128,34 @@ hipError_t run_unpack(const uint32_t* bases, const uint32_t* sorted_cvalues, con
 // in out_keys / out_values (inputs preserved).
 hipError_t run_radix_sort(const uint32_t* keys, const uint32_t* values, uint64_t n, uint32_t bits,
                           uint32_t* out_keys, uint32_t* out_values, ScanScratch& sc, hipStream_t s);
+// The same two stages with the element counts in device memory: the sort of
+// *d_n pairs (n_cap bounds it and sizes the launches; always a separate digit
+// row scan), and the decompression of ps->C chunks into ps->N rays.
+hipError_t run_radix_sort_dev(const uint32_t* keys, const uint32_t* values, uint64_t n_cap, const uint32_t* d_n,
+                              uint32_t bits, uint32_t* out_keys, uint32_t* out_values, ScanScratch& sc,
+                              hipStream_t s);
+hipError_t run_unpack_dev(const uint32_t* bases, const uint32_t* sorted_cvalues, const uint32_t* tvalues,
+                          uint64_t cap, const PassSizes* ps, uint32_t* soff, uint32_t* sbase, uint32_t* out_values,
+                          ScanScratch& sc, hipStream_t s);
+// Level sizes of a pass from its pack totals (N, C) into ps, and (N, C) into
+// rec[0..1] when rec is set (the frame's per-pass record).
+void launch_pass_sizes(const uint32_t* totals, uint32_t leaf, uint32_t branch, uint32_t depth, PassSizes* ps,
+                       uint32_t* rec, hipStream_t s);
+// The same for a pass whose sizes the host knows (a frame's camera pass).
+void launch_fixed_sizes(uint32_t N, uint32_t C, uint32_t leaf, uint32_t branch, uint32_t depth, PassSizes* ps,
+                        uint32_t* rec, hipStream_t s);
+// Level sizes of a hierarchy over min(*d_n, n_cap) rays into ps (C = 0).
+void launch_ray_sizes(const uint32_t* d_n, uint32_t n_cap, uint32_t leaf, uint32_t branch, uint32_t depth,
+                      PassSizes* ps, hipStream_t s);
 // Expand sorted chunks back to rays: for chunk x in sorted order, rays
 // trimmed[bases[sv[x]] .. + sizes[sv[x]]) in order.  `scan` receives the
 // exclusive scan of the sorted chunk sizes, `skeleton` the sizes themselves.
+// N == 0: the ray total is not known on the host and is read on the device
+// (no synchronisation).  d_c (optional): the chunk total in device memory, at
+// most C.
 hipError_t run_decompress(const uint32_t* bases, const uint32_t* sizes, const uint32_t* sorted_cvalues,
                           const uint32_t* tkeys, const uint32_t* tvalues, uint32_t* skeleton,
                           uint32_t* scan, uint64_t C, uint64_t N, uint32_t* out_keys,
-                          uint32_t* out_values, ScanScratch& sc, hipStream_t s);
+                          uint32_t* out_values, ScanScratch& sc, hipStream_t s, const uint32_t* d_c = nullptr);
 
 }  // namespace rsh

@@ -135,6 +135,13 @@ struct TravParams {
     uint32_t n_clusters;
     uint32_t prefilter;           // 64-ray kernel: 0 exact tests only, 1 ray-line / triangle-sphere
                                   // pre-test, 2 that and ray / cluster-sphere masks
+    // device-sized passes (ps set): N and the level sizes come from *ps, the
+    // item total from item_first[n_top]; level_count / N / items are then the
+    // capacities the launches are sized for, and list_cap / items_cap bound
+    // the list and descriptor writes (exceeding them sets *overflow)
+    const PassSizes* ps;
+    uint32_t list_cap, items_cap;
+    uint32_t* overflow;
 };
 
 // Work items of one launch of the 64-ray kernel: 64 lanes each, so the grid
@@ -198,7 +205,8 @@ void launch_create_bounce(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, con
 void launch_surface(const float* rays, const unsigned long long* rec, uint32_t hit_mode, uint32_t W, uint32_t H,
                     uint32_t slot_stride, uint32_t normalize_n, SceneView sc, GBufView g, hipStream_t s);
 void launch_shade_primary(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L, const Views& V,
-                          const uint32_t* sflags, uint32_t soft, uint32_t* pixels, hipStream_t s);
+                          const uint32_t* sflags, uint32_t soft, uint32_t* pixels, hipStream_t s,
+                          const uint32_t* live = nullptr);
 void launch_shade_reflection(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L, const Views& V,
                              const unsigned long long* rec, uint32_t hit_mode, const uint32_t* sflags,
                              uint32_t soft, uint32_t* pixels, hipStream_t s);
@@ -217,14 +225,16 @@ void launch_color_reflection_ref(const float4* tp, const float4* tn, uint32_t W,
                                  uint32_t* flags, uint32_t* keys, uint32_t* values, uint32_t* pixels, hipStream_t s);
 
 // --- hierarchy / traversal ---
-void launch_gather(const float* rays, const uint32_t* sorted_values, uint32_t N, const float4* shadow_pos,
+void launch_gather(const float* rays, const uint32_t* sorted_values, uint32_t N, const PassSizes* ps,
+                   const float4* shadow_pos,
                    const float* tmax, uint32_t W, uint32_t Lmax, uint32_t slot_shift, float4* rec0, float4* rec1,
                    hipStream_t s);
 // Ray queries: key, value (= index) and live flag of n caller rays (float[6n]);
 // segs (optional) places a per-ray segment id at kc.h.segment_shift.
 void launch_query_keys(const float* rays, const uint32_t* segs, uint32_t n, KeyCfg kc, uint32_t type,
                        uint32_t* flags, uint32_t* keys, uint32_t* values, hipStream_t s);
-void launch_hierarchy(const float4* rec0, const float4* rec1, const float* rays, const uint32_t* sorted_values,
+void launch_hierarchy(const float4* rec0, const float4* rec1, const PassSizes* ps, const float* rays,
+                      const uint32_t* sorted_values,
                       uint32_t N, uint32_t leaf, uint32_t branch, uint32_t depth, float r0, float s0, uint32_t mode,
                       float4* nodes, float2* aux, const uint32_t* counts, const uint32_t* offsets, hipStream_t s);
 hipError_t launch_traverse(const TravParams& p, hipStream_t s);

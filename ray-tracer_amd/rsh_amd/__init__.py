@@ -96,6 +96,29 @@ COMPAT_SIGNATURES = {
     "ReflectionRayIntersectionWrapper": [_P, _P, _P, _P, _P, _U, _U, _U, _U, _U, _U, _F3, _P],
     "ReflectionRayColoringWrapper": [_P, _P, _U, _U, _U, _F3, C.c_bool, _P, _P, _P, _P, _P, _P],
 }
+# The per-context stage API (include/rsh_ctx.h): (ctx, stream) and then the
+# wrapper's parameters, with device-count words (P) after the counts.
+CTX_STAGES = {
+    "rsh_triangle_update": "TriangleUpdateWrapper", "rsh_bounding_sphere_update": "BoundingSphereUpdateWrapper",
+    "rsh_shadow_ray_creation": "ShadowRayCreationWrapper", "rsh_reflection_ray_creation": "ReflectionRayCreationWrapper",
+    "rsh_refraction_ray_creation": "RefractionRayCreationWrapper",
+    "rsh_hierarchy_traversal_warmup": "HierarchyTraversalWarmUpWrapper",
+    "rsh_hierarchy_traversal": "HierarchyTraversalWrapper", "rsh_shadow_ray_preparation": "ShadowRayPreparationWrapper",
+    "rsh_shadow_ray_intersection": "ShadowRayIntersectionWrapper", "rsh_shadow_ray_coloring": "ShadowRayColoringWrapper",
+    "rsh_reflection_ray_preparation": "ReflectionRayPreparationWrapper",
+    "rsh_reflection_ray_intersection": "ReflectionRayIntersectionWrapper",
+    "rsh_reflection_ray_coloring": "ReflectionRayColoringWrapper",
+}
+CTX_SIGNATURES = {name: COMPAT_SIGNATURES[w] for name, w in CTX_STAGES.items()}
+CTX_SIGNATURES.update({
+    "rsh_memory_preparation": [_U, _U, _U],
+    "rsh_ray_trimming": [_P, _P, _U, _U, _U, _P, _P, _P, _P, PU, _P],
+    "rsh_ray_compression": [_P, _P, _U, _P, _P, _P, _P, _P, _P, _P, PU, _P],
+    "rsh_ray_sorting": [_P, _P, _U, _P, _P, _P],
+    "rsh_ctx_sort_pairs": [_P, _P, _U, _P, _U, _P, _P],
+    "rsh_ray_decompression": [_P, _P, _P, _P, _P, _P, _P, _P, _U, _P, _P, _P],
+    "rsh_hierarchy_creation": [_P, _P, _P, _U, _P, f32, f32, _P],
+})
 BINDERS = ["bindDiffuseTextureArray", "bindSpecularTextureArray", "bindFragmentPositionArray",
            "bindFragmentNormalArray", "bindTrianglePositions", "bindTriangleNormals",
            "bindTriangleTextureCoordinates", "bindTriangleObjectIDs", "bindTriangleMaterialIDs",
@@ -179,6 +202,22 @@ def lib():
         fn = getattr(L, name)
         fn.restype = None
         fn.argtypes = args
+    for name, args in CTX_SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = C.c_int
+        fn.argtypes = [PV, PV] + args
+    L.rsh_ctx_create.restype = C.c_int
+    L.rsh_ctx_create.argtypes = [C.c_int, C.POINTER(PV)]
+    L.rsh_ctx_destroy.restype = None
+    L.rsh_ctx_destroy.argtypes = [PV]
+    L.rsh_ctx_configure.restype = C.c_int
+    L.rsh_ctx_configure.argtypes = [PV, C.c_uint, C.c_uint, C.c_uint, C.c_int]
+    L.rsh_ctx_last_error.restype = C.c_char_p
+    L.rsh_ctx_last_error.argtypes = [PV]
+    L.rsh_set_gbuffer.restype = C.c_int
+    L.rsh_set_gbuffer.argtypes = [PV, PV, PV, PV, PV, C.c_uint, C.c_uint]
+    L.rsh_set_scene.restype = C.c_int
+    L.rsh_set_scene.argtypes = [PV, PV, PV, PV, PV, C.c_uint, PV, PV, C.c_uint, PV, C.c_uint, PV, PV, PV, C.c_uint]
     for name in BINDERS:
         fn = getattr(L, name)
         fn.restype = None
@@ -314,6 +353,41 @@ def config_desc(cfg):
         else:
             setattr(o, name, v)
     return o
+
+
+class Context:
+    """A stage context (include/rsh_ctx.h): ctx.stage(name, stream, *args)
+    calls rsh_<name>(ctx, stream, *args) and raises RshError on failure."""
+
+    def __init__(self, device=0):
+        L = lib()
+        p = PV()
+        rc = L.rsh_ctx_create(device, C.byref(p))
+        if rc != 0:
+            raise RshError(f"rsh_ctx_create({device}) failed ({rc})")
+        self.ptr = p
+
+    def _ok(self, rc, what):
+        if rc != 0:
+            raise RshError(f"{what}: {lib().rsh_ctx_last_error(self.ptr).decode()} (status {rc})")
+
+    def configure(self, light_max, depth, sub, algorithm):
+        self._ok(lib().rsh_ctx_configure(self.ptr, light_max, depth, sub, algorithm), "rsh_ctx_configure")
+
+    def set_gbuffer(self, dif, spec, pos, nrm, w, h):
+        self._ok(lib().rsh_set_gbuffer(self.ptr, dif, spec, pos, nrm, w, h), "rsh_set_gbuffer")
+
+    def set_scene(self, *args):
+        self._ok(lib().rsh_set_scene(self.ptr, *args), "rsh_set_scene")
+
+    def stage(self, name, stream, *args):
+        fn = name if name.startswith("rsh_") else "rsh_" + name
+        self._ok(getattr(lib(), fn)(self.ptr, stream, *args), fn)
+
+    def close(self):
+        if self.ptr:
+            lib().rsh_ctx_destroy(self.ptr)
+            self.ptr = None
 
 
 class RshError(RuntimeError):

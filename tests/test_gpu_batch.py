@@ -193,3 +193,30 @@ def test_full_64_view_batch_equals_plain_frames(engine, rsh):
     assert total == 23_543_814
     assert bad_px <= 1e-4 * total, bad_px
     assert bad_hit <= 1e-4 * total, bad_hit
+
+
+@pytest.mark.parametrize("variant", [dict(), dict(bounces=2, bounce_shadows=1, soft_shadows=1)])
+def test_graph_replay_follows_the_camera(engine, rsh, variant):
+    """A frame graph whose shadow and reflection passes keep their sizes in
+    device memory replays for any camera of the same size: each replay equals
+    the eager frame of its own camera (or, when the camera's work list
+    outgrows the captured capacities, is rendered eagerly instead), and the
+    returned ray counts are the camera's."""
+    scene = rsh.scenes.config1_scene(boxes=300)
+    cfg = dataclasses.replace(rsh.config.config1(96, 80), **variant).with_scene_bounds(scene)
+    engine.load_scene(scene)
+    rng = np.random.default_rng(77)
+    cams = [rsh.scenes.random_view(rng, 96, 80) for _ in range(5)]
+    st = engine.render(cams[0], cfg, flags=engine.GRAPH)          # eager frame + capture
+    assert st.graph_replayed == 0
+    replayed = 0
+    for i, cam in enumerate(cams[1:] + cams[:1]):
+        eager = engine.render(cam, cfg)
+        want = engine.pixels().copy()
+        st = engine.render(cam, cfg, flags=engine.GRAPH)
+        replayed += st.graph_replayed
+        np.testing.assert_array_equal(engine.pixels(), want, err_msg=f"camera {i}")
+        assert st.rays_total == eager.rays_total, (i, st.rays_total, eager.rays_total)
+        for k in range(eager.pass_count):
+            assert st.pass_rays[k] == eager.pass_rays[k], (i, k)
+    assert replayed >= 3

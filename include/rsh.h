@@ -274,9 +274,10 @@ void* rsh_engine_pixels_device(rsh_engine* e);
  * Each scene gets its own engine and HIP stream and one worker thread.
  * Requests (ray queries, as rsh_engine_trace) are admitted in arrival order
  * while the sum of in-flight requests' device bytes (256 per ray) stays within
- * the budget; a scene's worker merges every admitted request of the head
- * request's kind (up to max_batch_rays rays and 64 requests) into one traced
- * batch, each request's index in the batch placed above the 26-bit octahedral
+ * the budget; a scene's worker merges the next slice (rsh_server_set_slice)
+ * of every admitted request of the head request's kind (up to max_batch_rays
+ * rays and 64 slices) into one traced batch; a request's slice must fit
+ * max_batch_rays.  Each slice's index in the batch is placed above the 26-bit octahedral
  * key so the sort keeps requests apart.  Requests arriving while a batch runs
  * join the next one.  Host buffers passed to submit must stay valid until
  * rsh_server_wait returns for that request (pinned memory from

@@ -99,7 +99,7 @@ struct rsh_engine {
     // next pass traces)
     uint32_t W = 0, H = 0, Lmax = 1;
     size_t cap_slots = 0, cap_pixels = 0;   // slots (pixels x lights) and pixels the buffers hold
-    size_t cap_shadow_slots = 0;            // shadow-pass rays rays_b and shadow hold
+    size_t cap_shadow_slots = 0;            // shadow-pass flags `shadow` holds
     DevBuf rays_a, rays_b, closest[2], shadow, pixels;
     DevBuf views_d;                    // view table of a batch (more than one view)
     uint64_t scene_epoch = 0;          // counts scene loads
@@ -335,7 +335,9 @@ rsh_status ensure_lane(rsh_engine* e, Lane& L, size_t S) {
 }
 
 // shadow_mult: rays per slot of the shadow passes (16 with soft shadows);
-// they run on lane 1 and write rays_b and the shadow flags.
+// they run on lane 1 and write the shadow flags (their rays are recomputed by
+// the gather, so they have no slot-order ray buffer).  rays_b holds the camera
+// rays of a frame created in slot order (one per pixel).
 rsh_status ensure_frame(rsh_engine* e, uint32_t W, uint32_t H, uint32_t Lmax, uint32_t shadow_mult = 1) {
     size_t S = (size_t)W * H * Lmax, P = (size_t)W * H, Ss = S * shadow_mult;
     e->W = W; e->H = H; e->Lmax = Lmax;
@@ -351,11 +353,11 @@ rsh_status ensure_frame(rsh_engine* e, uint32_t W, uint32_t H, uint32_t Lmax, ui
         e->cap_slots = S;
     }
     if (Ss > e->cap_shadow_slots) {
-        HIP_OK(e->rays_b.reserve(Ss * 24));
         HIP_OK(e->shadow.reserve(Ss * 4));
         e->cap_shadow_slots = Ss;
     }
     if (P > e->cap_pixels) {
+        HIP_OK(e->rays_b.reserve(P * 24));
         HIP_OK(e->pixels.reserve(P * 4));
         for (int i = 0; i < 4; i++) {
             HIP_OK(e->g0[i].reserve(P * 16)); HIP_OK(e->gb[0][i].reserve(P * 16));
@@ -674,7 +676,7 @@ rsh_status trace_pass(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, const
         if (!presorted)
             launch_gather(rays, sorted_values, N, psz, kind == 1 ? shadow_pos : nullptr, kind == 1 ? tmax : nullptr,
                           e->W, e->Lmax, kind == 1 && shadow_pos && cfg.soft_shadows ? 4u : 0u, f4p(L.rec0),
-                          f4p(L.rec1), s);
+                          f4p(L.rec1), s, f4p(e->light_pos));
         launch_hierarchy(f4p(L.rec0), f4p(L.rec1), psz, nullptr, nullptr, N, cfg.leaf, cfg.branch, cfg.depth,
                          cfg.initial_radius, cfg.initial_spread, cfg.cone_mode, f4p(L.nodes), L.aux.as<float2>(),
                          counts, offs, s);
@@ -1104,11 +1106,12 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
         LANE_OK(hipStreamWaitEvent(sb, evG[k], 0));
         {
             StageTimer t(B, ST_CREATE, timing);
-            launch_create_shadow(G.pos, G.nrm, W, H, Lm, f4p(e->light_pos), kc, V, cfg.soft_shadows, rays_b,
+            // no slot-order rays: the gather recomputes each live ray
+            launch_create_shadow(G.pos, G.nrm, W, H, Lm, f4p(e->light_pos), kc, V, cfg.soft_shadows, nullptr,
                                  u32p(B.flags), u32p(B.keys), u32p(B.values), sb);
         }
         PassOut po;
-        rsh_status r = trace_pass(e, B, cfg, rays_b, Ssh, 1, G.pos, fl, po, idx_shadow(k), e->closest[0], nullptr,
+        rsh_status r = trace_pass(e, B, cfg, nullptr, Ssh, 1, G.pos, fl, po, idx_shadow(k), e->closest[0], nullptr,
                                   false, pass_bits);
         if (r != RSH_OK) return r;
         if ((r = record(e, B, st, idx_shadow(k), 1, Ssh, po, fl, e->closest[0])) != RSH_OK) return r;
@@ -1204,7 +1207,7 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
         {
             StageTimer t(A, ST_CREATE, timing);
             if (presort)
-                launch_create_primary_sorted(V, P, rays_b, f4p(A.rec0), f4p(A.rec1), sa);
+                launch_create_primary_sorted(V, P, f4p(A.rec0), f4p(A.rec1), sa);
             else
                 launch_create_primary(V, P, kc, rays_b, u32p(A.flags), u32p(A.keys), u32p(A.values), sa);
         }
@@ -1215,7 +1218,10 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
         if ((r = record(e, A, st, 0, 0, P, po, fl, e->closest[0])) != RSH_OK) return r;
         {
             StageTimer t(A, ST_SHADE, timing);
-            launch_surface(rays_b, u64p(e->closest[0]), cfg.hit_mode, W, H, 1, 1, sv, G0, sa);
+            // key-ordered camera rays exist only as hierarchy records: the
+            // surface stage recomputes them from the views
+            launch_surface(presort ? nullptr : rays_b, u64p(e->closest[0]), cfg.hit_mode, W, H, 1, 1, sv, G0, sa,
+                           &V);
         }
         if ((r = publish(0)) != RSH_OK) return r;
         uint32_t due = 0;      // G-buffer whose shading is due

@@ -97,11 +97,11 @@ __global__ void __launch_bounds__(256) k_create_primary(Views V, uint32_t P, Key
 // (mx, my) < (w_t, h_t).  Writes the slot-order rays (the surface stage reads
 // them) and the sorted hierarchy records of k_gather: rec0 = (o, slot bits),
 // rec1 = (d, 0).
-__global__ void __launch_bounds__(256) k_create_primary_sorted(Views V, uint32_t P, float* rays, float4* rec0,
-                                                               float4* rec1) {
+__global__ void __launch_bounds__(256) k_create_primary_sorted(Views V, uint32_t P, float4* rec0, float4* rec1) {
+    __shared__ ViewDesc s_v[VIEW_LDS];
+    const bool lds = stage_views(V, s_v);
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < P; q += gridDim.x * blockDim.x) {
-        const uint32_t v = view_of(V, q);
-        const ViewDesc& vd = V.n ? V.d[v] : V.one;
+        const ViewDesc vd = view_at(V, s_v, lds, q);
         const uint32_t w = vd.w, h = vd.h, r = q - vd.off;
         const uint32_t ty = r / (8u * w), r1 = r - ty * 8u * w;
         const uint32_t ht = min(8u, h - 8u * ty);
@@ -120,7 +120,6 @@ __global__ void __launch_bounds__(256) k_create_primary_sorted(Views V, uint32_t
         const uint32_t y = 8u * ty + (((m >> 1) & 1u) | ((m >> 2) & 2u) | ((m >> 3) & 4u));
         const uint32_t p = vd.off + x + y * w;
         const v3 o = mk(vd.eye[0], vd.eye[1], vd.eye[2]), d = primary_dir(vd, x, y);
-        store_ray(rays, p, o, d);
         rec0[q] = f4(o, __uint_as_float(p));
         rec1[q] = f4(d, 0.0f);
     }
@@ -156,17 +155,12 @@ __global__ void __launch_bounds__(256) k_create_shadow(const float4* gpos, const
         v3 N = normalize(xyz(gnrm[px]));
         uint32_t flag = 1;
         if (len(P) != 0.0f) {
-            v3 o = xyz(light_pos[l]);
-            v3 d = normalize(P - o);
-            float diffuse = fmaxf(dot(d, -N), 0.0f);
+            const v3 Lp = xyz(light_pos[l]);
+            const float diffuse = fmaxf(dot(normalize(P - Lp), -N), 0.0f);
             if (diffuse > 0.0f) {
-                if (soft) {
-                    v3 a0, a1;
-                    soft_axes(d, a0, a1);
-                    o = soft_point(o, a0, a1, r & 15u);
-                    d = normalize(P - o);
-                }
-                store_ray(rays, r, o, d);
+                v3 o, d;
+                shadow_ray(P, Lp, soft != 0u, r & 15u, o, d);
+                if (rays) store_ray(rays, r, o, d);     // null: the gather recomputes them
                 keys[r] = make_key(kc, o, d, 1, l, seg_of(kc, V, (uint32_t)px));
                 values[r] = r;
                 flag = 0;
@@ -259,9 +253,13 @@ __global__ void __launch_bounds__(256) k_create_bounce(GBufView g, uint32_t W, u
 
 // Surface record at a hit (RayTracer.cu:1807-1843): point o + d*(eps + t),
 // area-ratio barycentric normal, material of the triangle's first vertex.
+// rays == nullptr: the camera pass of views V; each hit pixel's ray is
+// recomputed as k_create_primary_sorted made it (same view, same operations).
 __global__ void __launch_bounds__(256) k_surface(const float* rays, const unsigned long long* rec, uint32_t hit_mode,
                                                  uint32_t W, uint32_t H, uint32_t slot_stride, uint32_t normalize_n,
-                                                 SceneView sc, GBufView g) {
+                                                 SceneView sc, GBufView g, Views V) {
+    __shared__ ViewDesc s_v[VIEW_LDS];
+    const bool lds = rays == nullptr && stage_views(V, s_v);
     const uint32_t n = W * H;
     const unsigned long long miss = hit_mode == 0 ? MISS64 : (unsigned long long)MISS32;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
@@ -277,7 +275,14 @@ __global__ void __launch_bounds__(256) k_surface(const float* rays, const unsign
         v3 v0 = xyz(sc.tri_pos[3 * (size_t)t]), v1 = xyz(sc.tri_pos[3 * (size_t)t + 1]);
         v3 v2 = xyz(sc.tri_pos[3 * (size_t)t + 2]);
         v3 o, d;
-        load_ray(rays, slot, o, d);
+        if (rays) {
+            load_ray(rays, slot, o, d);
+        } else {
+            const ViewDesc vd = view_at(V, s_v, lds, p);
+            const uint32_t q = p - vd.off;
+            o = mk(vd.eye[0], vd.eye[1], vd.eye[2]);
+            d = primary_dir(vd, q % vd.w, q / vd.w);
+        }
         float th = ray_triangle(o + d * EPS, d, v0, v1 - v0, v2 - v0);
         v3 P = o + d * (EPS + th);
         float aABC = len(cross(v1 - v0, v2 - v0));
@@ -521,8 +526,8 @@ void launch_create_primary(const Views& V, uint32_t P, KeyCfg kc, float* rays, u
                            uint32_t* values, hipStream_t s) {
     k_create_primary<<<grid_for(P, 256, 65536), 256, 0, s>>>(V, P, kc, rays, flags, keys, values);
 }
-void launch_create_primary_sorted(const Views& V, uint32_t P, float* rays, float4* rec0, float4* rec1, hipStream_t s) {
-    k_create_primary_sorted<<<grid_for(P, 256, 65536), 256, 0, s>>>(V, P, rays, rec0, rec1);
+void launch_create_primary_sorted(const Views& V, uint32_t P, float4* rec0, float4* rec1, hipStream_t s) {
+    k_create_primary_sorted<<<grid_for(P, 256, 65536), 256, 0, s>>>(V, P, rec0, rec1);
 }
 void launch_create_shadow(const float4* gpos, const float4* gnrm, uint32_t W, uint32_t H, uint32_t Lmax,
                           const float4* light_pos, KeyCfg kc, const Views& V, uint32_t soft, float* rays,
@@ -549,9 +554,10 @@ void launch_create_bounce(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, con
                                                                            rays, flags, keys, values);
 }
 void launch_surface(const float* rays, const unsigned long long* rec, uint32_t hit_mode, uint32_t W, uint32_t H,
-                    uint32_t slot_stride, uint32_t normalize_n, SceneView sc, GBufView g, hipStream_t s) {
+                    uint32_t slot_stride, uint32_t normalize_n, SceneView sc, GBufView g, hipStream_t s,
+                    const Views* V) {
     k_surface<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(rays, rec, hit_mode, W, H, slot_stride,
-                                                                     normalize_n, sc, g);
+                                                                     normalize_n, sc, g, V ? *V : Views{});
 }
 void launch_shade_primary(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L, const Views& V,
                           const uint32_t* sflags, uint32_t soft, uint32_t* pixels, hipStream_t s,

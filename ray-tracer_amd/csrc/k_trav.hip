@@ -40,23 +40,32 @@ inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap = 1u << 20) {
 // point of the slot's pixel, or the caller's per-slot tmax for ray queries;
 // unused for closest-hit passes.  slot_shift: 4 for soft shadows (16 rays per
 // pixel-light slot), else 0.
+// rays == nullptr (a frame's shadow pass): each ray is recomputed from its
+// slot's surface point and light as k_create_shadow made it (shadow_ray).
 __global__ void __launch_bounds__(256) k_gather(const float* rays, const uint32_t* sorted_values, uint32_t N,
                                                 const PassSizes* ps, const float4* shadow_pos, const float* tmax_in,
-                                                uint32_t W, uint32_t Lmax, uint32_t slot_shift, float4* rec0,
-                                                float4* rec1) {
+                                                const float4* light_pos, uint32_t W, uint32_t Lmax,
+                                                uint32_t slot_shift, float4* rec0, float4* rec1) {
     if (ps) N = min(N, ps->N);
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
         uint32_t slot = sorted_values[p];
-        const float2* r = reinterpret_cast<const float2*>(rays + 6 * (size_t)slot);
-        float2 a = r[0], b = r[1], c = r[2];
-        v3 o = mk(a.x, a.y, b.x), d = mk(b.y, c.x, c.y);
+        v3 o, d;
+        v3 P = mk(0.0f, 0.0f, 0.0f);
+        if (shadow_pos) {
+            const uint32_t i = slot >> slot_shift;
+            const uint32_t x = i % W, y = i / (W * Lmax);
+            P = xyz(shadow_pos[(size_t)x + (size_t)y * W]);
+            if (!rays) shadow_ray(P, xyz(light_pos[(i / W) % Lmax]), slot_shift != 0u, slot & 15u, o, d);
+        }
+        if (rays) {
+            const float2* r = reinterpret_cast<const float2*>(rays + 6 * (size_t)slot);
+            float2 a = r[0], b = r[1], c = r[2];
+            o = mk(a.x, a.y, b.x); d = mk(b.y, c.x, c.y);
+        }
         float tmax = 0.0f;
         if (tmax_in) {
             tmax = tmax_in[slot];
         } else if (shadow_pos) {
-            const uint32_t i = slot >> slot_shift;
-            uint32_t x = i % W, y = i / (W * Lmax);
-            v3 P = xyz(shadow_pos[(size_t)x + (size_t)y * W]);
             tmax = len(o - P) - EPS * 2.0f;
         }
         rec0[p] = f4(o, __uint_as_float(slot));
@@ -1237,9 +1246,9 @@ __global__ void __launch_bounds__(256) k_hits_intersect(const float* rays, const
 // ps: device-sized (N is the capacity the grid covers)
 void launch_gather(const float* rays, const uint32_t* sorted_values, uint32_t N, const PassSizes* ps,
                    const float4* shadow_pos, const float* tmax, uint32_t W, uint32_t Lmax, uint32_t slot_shift,
-                   float4* rec0, float4* rec1, hipStream_t s) {
-    if (N) k_gather<<<grid_for(N, 256, 65536), 256, 0, s>>>(rays, sorted_values, N, ps, shadow_pos, tmax, W, Lmax,
-                                                            slot_shift, rec0, rec1);
+                   float4* rec0, float4* rec1, hipStream_t s, const float4* light_pos) {
+    if (N) k_gather<<<grid_for(N, 256, 65536), 256, 0, s>>>(rays, sorted_values, N, ps, shadow_pos, tmax, light_pos,
+                                                            W, Lmax, slot_shift, rec0, rec1);
 }
 
 // ps: device-sized (N, counts and offsets are capacities; the kernels read

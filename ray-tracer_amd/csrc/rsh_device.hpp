@@ -531,6 +531,21 @@ __device__ inline v3 soft_point(v3 o, v3 a0, v3 a1, uint32_t c) {
     return (o + (a0 * u) * 0.125f) + (a1 * w) * 0.125f;
 }
 
+// The shadow ray from light position L towards surface point P
+// (RayTracer.cu:682-740); soft: from grid cell c about the light instead.
+// Ray creation and the gather of sorted shadow rays both call it, so the
+// gather recomputes exactly the ray creation made.
+__device__ inline void shadow_ray(v3 P, v3 L, bool soft, uint32_t c, v3& o, v3& d) {
+    o = L;
+    d = normalize(P - o);
+    if (soft) {
+        v3 a0, a1;
+        soft_axes(d, a0, a1);
+        o = soft_point(o, a0, a1, c);
+        d = normalize(P - o);
+    }
+}
+
 // Light scale of a pixel whose grid cells in `occluded` (bit c: cell c) are
 // shadowed: log2(1 + 2 s), s the sum over open cells of 0.55/16 (the four
 // inner cells), 0.5375/16 (the corners) or 0.525/16 (the rest), in cell order.

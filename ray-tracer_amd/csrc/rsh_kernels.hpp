@@ -41,6 +41,33 @@ __device__ __forceinline__ uint32_t view_of(const Views& V, uint32_t p) {
     }
     return a;
 }
+// Batches of up to VIEW_LDS views stage the view table in LDS once per
+// workgroup, so that a per-pixel view lookup is a search in LDS instead of a
+// chain of dependent global loads.  Call from every thread of the workgroup.
+constexpr uint32_t VIEW_LDS = 64;
+__device__ __forceinline__ bool stage_views(const Views& V, ViewDesc* s_v) {
+    const bool lds = V.n > 1 && V.n <= VIEW_LDS;
+    if (lds) {
+        const uint32_t words = V.n * (uint32_t)(sizeof(ViewDesc) / 4);
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(V.d);
+        uint32_t* dst = reinterpret_cast<uint32_t*>(s_v);
+        for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
+        __syncthreads();
+    }
+    return lds;
+}
+// The view of pixel p (s_v, lds: from stage_views).
+__device__ __forceinline__ ViewDesc view_at(const Views& V, const ViewDesc* s_v, bool lds, uint32_t p) {
+    if (lds) {
+        uint32_t a = 0, b = V.n;
+        while (b - a > 1) {
+            const uint32_t m = (a + b) >> 1;
+            if (s_v[m].off <= p) a = m; else b = m;
+        }
+        return s_v[a];
+    }
+    return V.n ? V.d[view_of(V, p)] : V.one;
+}
 __device__ __forceinline__ v3 view_eye(const Views& V, uint32_t v) {
     const float* e = V.n ? V.d[v].eye : V.one.eye;
     return mk(e[0], e[1], e[2]);
@@ -195,15 +222,18 @@ void launch_create_shadow(const float4* gpos, const float4* gnrm, uint32_t W, ui
 void launch_create_reflection(const float4* gpos, const float4* gnrm, const float4* gdif, uint32_t W, uint32_t H,
                               uint32_t Lmax, const Views& V, uint32_t gate, uint32_t mark_gaps, KeyCfg kc,
                               float* rays, uint32_t* flags, uint32_t* keys, uint32_t* values, hipStream_t s);
-void launch_create_primary_sorted(const Views& V, uint32_t P, float* rays, float4* rec0, float4* rec1, hipStream_t s);
+void launch_create_primary_sorted(const Views& V, uint32_t P, float4* rec0, float4* rec1, hipStream_t s);
 void launch_create_refraction(const float4* gpos, const float4* gnrm, uint32_t W, uint32_t H, uint32_t Lmax,
                               const Views& V, KeyCfg kc, float* rays, uint32_t* flags, uint32_t* keys,
                               uint32_t* values, hipStream_t s);
 void launch_create_bounce(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, const unsigned long long* rec,
                           uint32_t hit_mode, uint32_t gate, KeyCfg kc, const Views& V, float* rays, uint32_t* flags,
                           uint32_t* keys, uint32_t* values, hipStream_t s);
+// rays == nullptr: the camera pass of views V, whose rays are recomputed
+// from the views (k_create_primary_sorted writes no slot-order rays).
 void launch_surface(const float* rays, const unsigned long long* rec, uint32_t hit_mode, uint32_t W, uint32_t H,
-                    uint32_t slot_stride, uint32_t normalize_n, SceneView sc, GBufView g, hipStream_t s);
+                    uint32_t slot_stride, uint32_t normalize_n, SceneView sc, GBufView g, hipStream_t s,
+                    const Views* V = nullptr);
 void launch_shade_primary(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L, const Views& V,
                           const uint32_t* sflags, uint32_t soft, uint32_t* pixels, hipStream_t s,
                           const uint32_t* live = nullptr);
@@ -225,10 +255,12 @@ void launch_color_reflection_ref(const float4* tp, const float4* tn, uint32_t W,
                                  uint32_t* flags, uint32_t* keys, uint32_t* values, uint32_t* pixels, hipStream_t s);
 
 // --- hierarchy / traversal ---
+// rays == nullptr: a frame's shadow pass, each ray recomputed from its slot's
+// surface point (shadow_pos) and light (light_pos), as ray creation made it.
 void launch_gather(const float* rays, const uint32_t* sorted_values, uint32_t N, const PassSizes* ps,
                    const float4* shadow_pos,
                    const float* tmax, uint32_t W, uint32_t Lmax, uint32_t slot_shift, float4* rec0, float4* rec1,
-                   hipStream_t s);
+                   hipStream_t s, const float4* light_pos = nullptr);
 // Ray queries: key, value (= index) and live flag of n caller rays (float[6n]);
 // segs (optional) places a per-ray segment id at kc.h.segment_shift.
 void launch_query_keys(const float* rays, const uint32_t* segs, uint32_t n, KeyCfg kc, uint32_t type,

@@ -277,7 +277,9 @@ rsh_status rsh_server_submit(rsh_server* s, uint32_t scene_id, uint32_t kind, co
         return RSH_INVALID_ARG;
     std::lock_guard<std::mutex> lk(s->mu);
     if (scene_id >= s->scenes.size()) { s->err = "submit: unknown scene"; return RSH_INVALID_ARG; }
-    if (n > s->max_batch_rays) { s->err = "submit: request larger than the batch cap"; return RSH_CAPACITY; }
+    // a request is traced in slices of at most slice_rays, each within a batch
+    const uint32_t piece = n < s->slice_rays ? n : s->slice_rays;
+    if (piece > s->max_batch_rays) { s->err = "submit: request slice larger than the batch cap"; return RSH_CAPACITY; }
     const uint64_t bytes = (uint64_t)n * BYTES_PER_RAY;
     if (bytes > s->budget) { s->err = "submit: request larger than the HBM budget"; return RSH_CAPACITY; }
     if (!s->started) {

@@ -89,6 +89,17 @@ def parse():
                     help="FETCH_SIZE calibration: bytes read per counted byte on a plain copy")
     ap.add_argument("--counter-renders", type=int, default=0,
                     help="with --fetch-csv: renders in the counter runs (default: those of SIZES_JSON)")
+    ap.add_argument("--trav-summary", nargs=2, default=None, metavar=("KERNEL_STATS_CSV", "KERNEL_TRACE_CSV"),
+                    help="(no GPU) summarise a rocprofv3 kernel trace of this bench run with the same --steps and "
+                         "--warmup (and the --no-* legs off): kernel table, traversal launches per frame and their "
+                         "average over the serial frames the roofline times")
+    ap.add_argument("--trav-traffic", nargs=4, default=None,
+                    metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
+                    help="(no GPU) HBM traffic per k_traverse64 launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs "
+                         "of this bench (--steps 1 --warmup 0, frame legs only), scaled by the same counters on "
+                         "--copy-probe 512")
+    ap.add_argument("--trav-valu", default="", metavar="SQ_CSV",
+                    help="(no GPU) SQ counters per k_traverse64 launch of the first serial frame")
     ap.add_argument("--out", default="")
     return ap.parse_args()
 
@@ -277,6 +288,128 @@ def _kernel_base(name):
     'void rsh::(anonymous namespace)::k_radix_upsweep<9u>(unsigned int const*, ...)'."""
     m = re.search(r"\b(k_[A-Za-z0-9_]+)", name)
     return m.group(1) if m else name.strip()
+
+
+def trav_summary(a):
+    """--trav-summary: the kernel table of a rocprofv3 --stats run of this
+    bench (frame legs only), the k_traverse64 launches grouped by frame in
+    start order, and their average over the three serial frames with
+    traversal timers, which the roofline's avg_launch_ms times with HIP
+    events.  Frames in a run: --warmup warm-up, --steps timed, one eager
+    frame recorded for the graph, 1 + --steps graph replays, then 3 serial
+    frames with traversal timers, 3 with stage timers, 1 with counters."""
+    import csv
+    stats_csv, trace_csv = a.trav_summary
+    rows = []
+    with open(stats_csv) as f:
+        for r in csv.DictReader(f):
+            m = re.search(r"\b(k_[A-Za-z0-9_]+(<[^>]*>)?)", r["Name"])
+            rows.append((m.group(1) if m else r["Name"][:40], int(r["Calls"]), float(r["TotalDurationNs"]) / 1e3,
+                         float(r["AverageNs"]) / 1e3, float(r["MinNs"]) / 1e3, float(r["MaxNs"]) / 1e3,
+                         float(r["Percentage"])))
+    rows.sort(key=lambda t: -t[2])
+    print(f"{'kernel':28s} {'calls':>6s} {'total_us':>12s} {'avg_us':>10s} {'min_us':>10s} {'max_us':>10s} {'pct':>7s}")
+    for name, c, tot, avg, mn, mx, pct in rows[:30]:
+        print(f"{name[:28]:28s} {c:6d} {tot:12.1f} {avg:10.1f} {mn:10.1f} {mx:10.1f} {pct:7.2f}")
+    launches = []
+    with open(trace_csv) as f:
+        for r in csv.DictReader(f):
+            if "k_traverse64" in r["Kernel_Name"]:
+                launches.append((int(r["Start_Timestamp"]), (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6))
+    launches.sort()
+    per = 8
+    frames = [launches[i:i + per] for i in range(0, len(launches), per)]
+    first = a.warmup + 2 * a.steps + 2
+    print()
+    print("k_traverse64<8, MODE> per launch (ms), one row per frame, in start order")
+    for i, fr in enumerate(frames):
+        tag = "  <- serial, traversal timers" if first <= i < first + 3 else ""
+        print("  " + " ".join(f"{d:6.2f}" for _, d in fr) + f"   sum {sum(d for _, d in fr):5.2f}{tag}")
+    serial = [d for fr in frames[first:first + 3] for _, d in fr]
+    print()
+    print(f"k_traverse64 average launch: {sum(d for _, d in launches) / max(len(launches), 1):.3f} ms over "
+          f"{len(launches)} launches")
+    print(f"k_traverse64 average launch over the 3 serial frames with traversal timers: "
+          f"{sum(serial) / max(len(serial), 1):.3f} ms ({len(serial)} launches)")
+
+
+def _counter_rows(path, kernel=None):
+    """{dispatch: {counter: value}} and {dispatch: kernel name} of a
+    rocprofv3 counter_collection.csv (optionally one kernel only)."""
+    import csv
+    vals, names = {}, {}
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if kernel and kernel not in r["Kernel_Name"]:
+                continue
+            d = int(r["Dispatch_Id"])
+            vals.setdefault(d, {})[r["Counter_Name"]] = vals.get(d, {}).get(r["Counter_Name"], 0.0) + \
+                float(r["Counter_Value"])
+            names[d] = r["Kernel_Name"]
+    return vals, names
+
+
+PASS_NAMES = ["primary", "shadow1", "refl1", "shadow2", "refl2", "shadow3", "refl3", "shadow4"]
+
+
+def _first_serial_frame(dispatches, steps=1, warmup=0, per=8):
+    """The k_traverse64 dispatches of the first serial frame with traversal
+    timers of a bench run (frame order as in trav_summary), passes in order:
+    primary, then shadow and reflection passes alternating (in the timed
+    frames the two lanes' launches interleave differently)."""
+    first = warmup + 2 * steps + 2
+    return sorted(dispatches)[first * per:(first + 1) * per]
+
+
+def trav_traffic(a):
+    fetch_csv, write_csv, cf_csv, cw_csv = a.trav_traffic
+    fv, _ = _counter_rows(fetch_csv, "k_traverse64")
+    wv, _ = _counter_rows(write_csv, "k_traverse64")
+    cfv, _ = _counter_rows(cf_csv, "copyBuffer")
+    cwv, _ = _counter_rows(cw_csv, "copyBuffer")
+    copy_kib = 512 * 1024
+    big_f = [v["FETCH_SIZE"] for v in cfv.values() if v.get("FETCH_SIZE", 0) > copy_kib / 4]
+    big_w = [v["WRITE_SIZE"] for v in cwv.values() if v.get("WRITE_SIZE", 0) > copy_kib / 4]
+    fscale = copy_kib / (sum(big_f) / len(big_f))
+    wscale = copy_kib / (sum(big_w) / len(big_w))
+    print(f"copy FETCH_SIZE reads {sum(big_f) / len(big_f):.0f} KiB for {copy_kib} KiB read  -> fetch scale {fscale:.4f}")
+    print(f"copy WRITE_SIZE reads {sum(big_w) / len(big_w):.0f} KiB for {copy_kib} KiB written -> write scale "
+          f"{wscale:.4f}")
+    print()
+    fd = _first_serial_frame(fv)
+    wd = _first_serial_frame(wv)
+    print("per dispatch (KiB as counted), the first serial frame of the run (passes in order):")
+    print(f"{'pass':10s} {'FETCH_KiB':>12s} {'WRITE_KiB':>12s} {'scaled_MB':>12s}")
+    tf = tw = 0.0
+    for name, df, dw in zip(PASS_NAMES, fd, wd):
+        f_, w_ = fv[df]["FETCH_SIZE"], wv[dw]["WRITE_SIZE"]
+        tf += f_ * 1024 * fscale
+        tw += w_ * 1024 * wscale
+        print(f"{name:10s} {f_:12.0f} {w_:12.0f} {(f_ * fscale + w_ * wscale) * 1024 / 1e6:12.1f}")
+    n = max(len(fd), 1)
+    print()
+    print(f"mean per launch: fetch {tf / n:.4g} B, write {tw / n:.4g} B")
+    print(f"traffic_bytes_per_launch = {(tf + tw) / n:.6g}")
+
+
+def trav_valu(a):
+    vals, _ = _counter_rows(a.trav_valu, "k_traverse64")
+    d = _first_serial_frame(vals)
+    keys = ["SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS", "SQ_INSTS_VMEM", "SQ_WAVE_CYCLES",
+            "SQ_BUSY_CYCLES", "SQ_ACTIVE_INST_VALU"]
+    print("Counters summed over the chip per dispatch (the first serial frame of the run, passes in order).")
+    print(f"{'pass':10s} {'waves':>10s} {'VALU_insts':>12s} {'SALU_insts':>12s} {'LDS_insts':>12s} "
+          f"{'VMEM_insts':>12s} {'VALU/wave':>10s}")
+    tot = 0.0
+    for name, di in zip(PASS_NAMES, d):
+        v = vals[di]
+        w = max(v.get("SQ_WAVES", 0.0), 1.0)
+        tot += v.get("SQ_INSTS_VALU", 0.0)
+        print(f"{name:10s} {v.get('SQ_WAVES', 0):10.0f} {v.get('SQ_INSTS_VALU', 0):12.4g} "
+              f"{v.get('SQ_INSTS_SALU', 0):12.4g} {v.get('SQ_INSTS_LDS', 0):12.4g} {v.get('SQ_INSTS_VMEM', 0):12.4g} "
+              f"{v.get('SQ_INSTS_VALU', 0) / w:10.0f}")
+    print()
+    print(f"VALU instructions per frame: {tot:.4g}")
 
 
 def hsc_summary(a):
@@ -551,6 +684,7 @@ def hit_agreement(eng, views, cfg, rsh_amd):
     same = [0] * nb
     total = [0] * nb
     worst = 0.0
+    hit_vs_miss = 0
     for v, (cam, w, h) in enumerate(views):
         eng.render(cam, dataclasses.replace(cfg, width=w, height=h), flags=eng.KEEP_PASSES)
         for i in range(min(nb, eng.stats.pass_count)):
@@ -567,8 +701,7 @@ def hit_agreement(eng, views, cfg, rsh_amd):
                 if d.any():
                     miss = np.uint64(0xFFFFFFFFFFFFFFFF)
                     both = d & (a != miss) & (b != miss)
-                    if (d & ~both).any():
-                        worst = float("inf")
+                    hit_vs_miss += int((d & ~both).sum())
                     fa = (a[both] >> np.uint64(32)).astype(np.uint32).view(np.float32).astype(np.float64)
                     fb = (b[both] >> np.uint64(32)).astype(np.uint32).view(np.float32).astype(np.float64)
                     if both.any():
@@ -578,7 +711,8 @@ def hit_agreement(eng, views, cfg, rsh_amd):
     agree = sum(same)
     return {"slots_compared": rays, "identical": agree, "fraction": agree / rays,
             "per_pass_differing": [t - s for s, t in zip(same, total)],
-            "max_rel_hit_distance_among_differing": worst}
+            "max_rel_hit_distance_among_differing_hits": worst,
+            "differing_hit_vs_miss": hit_vs_miss}
 
 
 def batching_leg(rsh_amd, eng, scene2, tune=True):
@@ -629,13 +763,14 @@ def batching_leg(rsh_amd, eng, scene2, tune=True):
             out[name]["autotune"] = {
                 "seconds": time.perf_counter() - t0,
                 "chosen": {"cell_bits": best.cell_bits, "dir_bits": best.dir_bits, "leaf": best.leaf,
-                           "branch": best.branch, "traversal_chunk": best.traversal_chunk},
+                           "branch": best.branch, "traversal_chunk": best.traversal_chunk,
+                           "traversal_prefilter": best.traversal_prefilter},
                 "chosen_graph_mrays": max(t.mrays for t in trials),
                 "start_graph_mrays": trials[0].mrays,
-                "trials": [[t.cell_bits, t.dir_bits, t.leaf, t.branch, t.traversal_chunk, round(t.ms, 2),
-                            round(t.mrays, 1), round(t.same_pixels, 6)] for t in trials],
-                "trial_fields": ["cell_bits", "dir_bits", "leaf", "branch", "traversal_chunk", "graph_ms",
-                                 "mrays", "same_pixels_as_start"]}
+                "trials": [[t.cell_bits, t.dir_bits, t.leaf, t.branch, t.traversal_chunk, t.traversal_prefilter,
+                            round(t.ms, 2), round(t.mrays, 1), round(t.same_pixels, 6)] for t in trials],
+                "trial_fields": ["cell_bits", "dir_bits", "leaf", "branch", "traversal_chunk",
+                                 "traversal_prefilter", "graph_ms", "mrays", "same_pixels_as_start"]}
             progress("batching: autotune done")
     return out
 
@@ -651,6 +786,15 @@ def main():
         return
     if a.hsc_summary:
         hsc_summary(a)
+        return
+    if a.trav_summary:
+        trav_summary(a)
+        return
+    if a.trav_traffic:
+        trav_traffic(a)
+        return
+    if a.trav_valu:
+        trav_valu(a)
         return
     if a.hsc_profile:
         hsc_profile_run(a)

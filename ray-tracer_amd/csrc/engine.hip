@@ -233,6 +233,14 @@ constexpr uint64_t TRAVERSE_TARGET_WAVES = 16384;
 // fills most tails), measured on the 1080p atrium frame: 32768 -> 7.57 ms,
 // 49152 -> 7.53, 65536 -> 7.52, 98304 -> 7.64, 131072 -> 7.81 ms per frame.
 constexpr uint32_t ITEM_CHUNK = 64;
+// Shadow passes also take the longer items: alone the first shadow pass of
+// the 1080p atrium runs 0.85 ms with 64-cluster items against 1.05 ms with
+// 128, but it overlaps the reflection chain, and with longer items the frame
+// is no slower (6.02 against 6.06 ms) and the 64-view batch faster (66.3
+// against 67.8 ms).
+#ifndef RSH_SHADOW_LONG_ITEMS
+#define RSH_SHADOW_LONG_ITEMS 1
+#endif
 constexpr size_t HSTAGE_WORDS = 2 + 2 * ITEM_TABLE_WORDS;   // N / C or the item total, then the item table
 constexpr uint32_t ITEM_CHUNK_MIN = 4;
 constexpr uint32_t ITEM_TARGET = 65536;
@@ -532,7 +540,8 @@ rsh_status build_items(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, Trav
                 // passes with plenty of items take longer items (less per-item
                 // set-up: on the 1080p atrium, primary 1.98 -> 1.83 ms, first
                 // reflection pass 1.13 -> 1.05 ms)
-                if (items_at(ITEM_CHUNK) >= 2 * ITEM_TARGET) chunk = 2 * ITEM_CHUNK;
+                if (items_at(ITEM_CHUNK) >= 2 * ITEM_TARGET && (RSH_SHADOW_LONG_ITEMS || p.kind != 1))
+                    chunk = 2 * ITEM_CHUNK;
                 while (items_at(chunk) < ITEM_TARGET && chunk > ITEM_CHUNK_MIN && items_at(chunk) != 0) {
                     const uint64_t want = (uint64_t)chunk * items_at(chunk) / ITEM_TARGET;
                     chunk = want < ITEM_CHUNK_MIN ? ITEM_CHUNK_MIN : (uint32_t)want;

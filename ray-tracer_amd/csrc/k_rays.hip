@@ -45,8 +45,9 @@ __device__ __forceinline__ uint32_t make_key(const KeyCfg& kc, v3 o, v3 d, uint3
     if (kc.h.segment_shift) k |= seg << kc.h.segment_shift;
     return k;
 }
-__device__ __forceinline__ uint32_t seg_of(const KeyCfg& kc, const Views& V, uint32_t p) {
-    return kc.h.segment_shift ? view_of(V, p) : 0u;
+__device__ __forceinline__ uint32_t seg_of(const KeyCfg& kc, const Views& V, const uint32_t* s_off, bool lds,
+                                           uint32_t p) {
+    return kc.h.segment_shift ? view_index(V, s_off, lds, p) : 0u;
 }
 
 // Pixel p of view v: (x, y) within the view, dir = D + R*(x/w - 0.5) + U*(y/h - 0.5).
@@ -76,9 +77,11 @@ __device__ __forceinline__ v3 primary_dir(const ViewDesc& vd, uint32_t x, uint32
 
 __global__ void __launch_bounds__(256) k_create_primary(Views V, uint32_t P, KeyCfg kc, float* rays, uint32_t* flags,
                                                         uint32_t* keys, uint32_t* values) {
+    __shared__ uint32_t s_off[VIEW_LDS];
+    const bool lds = stage_views(V, s_off);
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < P; p += gridDim.x * blockDim.x) {
-        const uint32_t v = view_of(V, p);
-        const ViewDesc& vd = V.n ? V.d[v] : V.one;
+        const uint32_t v = view_index(V, s_off, lds, p);
+        const ViewDesc vd = V.n ? V.d[v] : V.one;
         const uint32_t q = p - vd.off;
         const uint32_t x = q % vd.w, y = q / vd.w;
         store_ray(rays, p, mk(vd.eye[0], vd.eye[1], vd.eye[2]), primary_dir(vd, x, y));
@@ -98,10 +101,10 @@ __global__ void __launch_bounds__(256) k_create_primary(Views V, uint32_t P, Key
 // them) and the sorted hierarchy records of k_gather: rec0 = (o, slot bits),
 // rec1 = (d, 0).
 __global__ void __launch_bounds__(256) k_create_primary_sorted(Views V, uint32_t P, float4* rec0, float4* rec1) {
-    __shared__ ViewDesc s_v[VIEW_LDS];
-    const bool lds = stage_views(V, s_v);
+    __shared__ uint32_t s_off[VIEW_LDS];
+    const bool lds = stage_views(V, s_off);
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < P; q += gridDim.x * blockDim.x) {
-        const ViewDesc vd = view_at(V, s_v, lds, q);
+        const ViewDesc vd = view_at(V, s_off, lds, q);
         const uint32_t w = vd.w, h = vd.h, r = q - vd.off;
         const uint32_t ty = r / (8u * w), r1 = r - ty * 8u * w;
         const uint32_t ht = min(8u, h - 8u * ty);
@@ -142,26 +145,30 @@ __global__ void __launch_bounds__(256) k_query_keys(const float* rays, const uin
 __global__ void __launch_bounds__(256) k_create_shadow(const float4* gpos, const float4* gnrm, uint32_t W, uint32_t H,
                                                        uint32_t Lmax, const float4* light_pos, KeyCfg kc, Views V,
                                                        uint32_t soft, float* rays, uint32_t* flags, uint32_t* keys,
-                                                       uint32_t* values) {
+                                                       uint32_t* values, FastDiv fw, FastDiv fl) {
     // soft: 16 rays per slot, ray 16 i + c from grid point c about the light
     // (RayTracer.cu:1550-1603) towards the slot's surface point
     const uint32_t sh = soft ? 4u : 0u;
     const uint32_t n = (W * H * Lmax) << sh;
+    __shared__ uint32_t s_off[VIEW_LDS];
+    const bool lds = stage_views(V, s_off);
     for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
         const uint32_t i = r >> sh;
-        uint32_t x = i % W, y = i / (W * Lmax), l = (i / W) % Lmax;
+        const uint32_t qw = fdiv(i, fw);          // l + Lmax y
+        const uint32_t x = i - qw * W, y = fdiv(qw, fl), l = qw - y * Lmax;
         size_t px = (size_t)x + (size_t)y * W;
         v3 P = xyz(gpos[px]);
-        v3 N = normalize(xyz(gnrm[px]));
+        const v3 Nr = xyz(gnrm[px]);
         uint32_t flag = 1;
-        if (len(P) != 0.0f) {
+        if (dot(P, P) != 0.0f) {                   // len(P) != 0 without the square root
+            const v3 N = normalize(Nr);
             const v3 Lp = xyz(light_pos[l]);
             const float diffuse = fmaxf(dot(normalize(P - Lp), -N), 0.0f);
             if (diffuse > 0.0f) {
                 v3 o, d;
                 shadow_ray(P, Lp, soft != 0u, r & 15u, o, d);
                 if (rays) store_ray(rays, r, o, d);     // null: the gather recomputes them
-                keys[r] = make_key(kc, o, d, 1, l, seg_of(kc, V, (uint32_t)px));
+                keys[r] = make_key(kc, o, d, 1, l, seg_of(kc, V, s_off, lds, (uint32_t)px));
                 values[r] = r;
                 flag = 0;
             }
@@ -175,16 +182,20 @@ __global__ void __launch_bounds__(256) k_create_shadow(const float4* gpos, const
 __global__ void __launch_bounds__(256) k_create_reflection(const float4* gpos, const float4* gnrm, const float4* gdif,
                                                            uint32_t W, uint32_t H, uint32_t Lmax, Views V,
                                                            uint32_t gate, uint32_t mark_gaps, KeyCfg kc, float* rays,
-                                                           uint32_t* flags, uint32_t* keys, uint32_t* values) {
+                                                           uint32_t* flags, uint32_t* keys, uint32_t* values,
+                                                           FastDiv fw) {
     const uint32_t n = W * H;
+    __shared__ uint32_t s_off[VIEW_LDS];
+    const bool lds = stage_views(V, s_off);
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-        uint32_t x = p % W, y = p / W;
+        const uint32_t y = fdiv(p, fw), x = p - y * W;
         uint32_t slot = x + y * W * Lmax;
         v3 P = xyz(gpos[p]);
-        v3 N = normalize(xyz(gnrm[p]));
-        bool live = len(P) != 0.0f && (!gate || gdif[p].w > 0.0f);
+        const v3 Nr = xyz(gnrm[p]);
+        bool live = dot(P, P) != 0.0f && (!gate || gdif[p].w > 0.0f);      // len(P) != 0
         if (live) {
-            const uint32_t v = view_of(V, p);
+            const v3 N = normalize(Nr);
+            const uint32_t v = view_index(V, s_off, lds, p);
             v3 d = reflect(normalize(P - view_eye(V, v)), normalize(N));
             store_ray(rays, slot, P, d);
             keys[slot] = make_key(kc, P, d, 2, 0, v);
@@ -205,13 +216,15 @@ __global__ void __launch_bounds__(256) k_create_refraction(const float4* gpos, c
                                                            uint32_t H, uint32_t Lmax, Views V, KeyCfg kc, float* rays,
                                                            uint32_t* flags, uint32_t* keys, uint32_t* values) {
     const uint32_t n = W * H;
+    __shared__ uint32_t s_off[VIEW_LDS];
+    const bool lds = stage_views(V, s_off);
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
         uint32_t x = p % W, y = p / W;
         uint32_t slot = x + y * W * Lmax;
         v3 P = xyz(gpos[p]);
         v3 N = normalize(xyz(gnrm[p]));
         if (len(P) != 0.0f) {
-            const uint32_t v = view_of(V, p);
+            const uint32_t v = view_index(V, s_off, lds, p);
             v3 d = refract(normalize(P - view_eye(V, v)), normalize(N), 1.0f / 1.52f);
             store_ray(rays, slot, P, d);
             keys[slot] = make_key(kc, P, d, 3, 0, v);
@@ -228,11 +241,13 @@ __global__ void __launch_bounds__(256) k_create_refraction(const float4* gpos, c
 __global__ void __launch_bounds__(256) k_create_bounce(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax,
                                                        const unsigned long long* rec, uint32_t hit_mode,
                                                        uint32_t gate, KeyCfg kc, Views V, float* rays,
-                                                       uint32_t* flags, uint32_t* keys, uint32_t* values) {
+                                                       uint32_t* flags, uint32_t* keys, uint32_t* values, FastDiv fw) {
     const uint32_t n = W * H;
     const unsigned long long miss = hit_mode == 0 ? MISS64 : (unsigned long long)MISS32;
+    __shared__ uint32_t s_off[VIEW_LDS];
+    const bool lds = stage_views(V, s_off);
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-        uint32_t x = p % W, y = p / W;
+        const uint32_t y = fdiv(p, fw), x = p - y * W;
         uint32_t slot = x + y * W * Lmax;
         bool live = rec[slot] != miss && (!gate || g.dif[p].w > 0.0f);
         if (live) {
@@ -241,7 +256,7 @@ __global__ void __launch_bounds__(256) k_create_bounce(GBufView g, uint32_t W, u
             v3 P = xyz(g.pos[p]), N = xyz(g.nrm[p]);
             v3 nd = reflect(d, N);
             store_ray(rays, slot, P, nd);
-            keys[slot] = make_key(kc, P, nd, 2, 0, seg_of(kc, V, p));
+            keys[slot] = make_key(kc, P, nd, 2, 0, seg_of(kc, V, s_off, lds, p));
             values[slot] = slot;
             flags[slot] = 0;
         } else {
@@ -257,13 +272,13 @@ __global__ void __launch_bounds__(256) k_create_bounce(GBufView g, uint32_t W, u
 // recomputed as k_create_primary_sorted made it (same view, same operations).
 __global__ void __launch_bounds__(256) k_surface(const float* rays, const unsigned long long* rec, uint32_t hit_mode,
                                                  uint32_t W, uint32_t H, uint32_t slot_stride, uint32_t normalize_n,
-                                                 SceneView sc, GBufView g, Views V) {
-    __shared__ ViewDesc s_v[VIEW_LDS];
-    const bool lds = rays == nullptr && stage_views(V, s_v);
+                                                 SceneView sc, GBufView g, Views V, FastDiv fw) {
+    __shared__ uint32_t s_off[VIEW_LDS];
+    const bool lds = rays == nullptr && stage_views(V, s_off);
     const uint32_t n = W * H;
     const unsigned long long miss = hit_mode == 0 ? MISS64 : (unsigned long long)MISS32;
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-        uint32_t x = p % W, y = p / W;
+        const uint32_t y = fdiv(p, fw), x = p - y * W;
         uint32_t slot = x + y * W * slot_stride;
         unsigned long long r = rec[slot];
         if (r == miss) {
@@ -278,7 +293,7 @@ __global__ void __launch_bounds__(256) k_surface(const float* rays, const unsign
         if (rays) {
             load_ray(rays, slot, o, d);
         } else {
-            const ViewDesc vd = view_at(V, s_v, lds, p);
+            const ViewDesc vd = view_at(V, s_off, lds, p);
             const uint32_t q = p - vd.off;
             o = mk(vd.eye[0], vd.eye[1], vd.eye[2]);
             d = primary_dir(vd, q % vd.w, q / vd.w);
@@ -316,17 +331,19 @@ __device__ inline uint32_t soft_mask(const uint32_t* sflags, size_t slot) {
 
 __global__ void __launch_bounds__(256) k_shade_primary(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L,
                                                        Views V, const uint32_t* sflags, uint32_t soft,
-                                                       uint32_t* pixels, const uint32_t* live) {
+                                                       uint32_t* pixels, const uint32_t* live, FastDiv fw) {
     // a shadow pass with no live ray leaves the pixels as they are (the frame
     // driver shades only after a non-empty shadow pass); live: its device count
     if (live && *live == 0u) return;
     const uint32_t n = W * H;
+    __shared__ uint32_t s_off[VIEW_LDS];
+    const bool lds = stage_views(V, s_off);
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-        uint32_t x = p % W, y = p / W;
+        const uint32_t y = fdiv(p, fw), x = p - y * W;
         v3 color = mk(0.0f, 0.0f, 0.0f);
         v3 P = xyz(g.pos[p]), N = xyz(g.nrm[p]);
-        if (len(P) != 0.0f) {
-            const v3 eye = view_eye(V, view_of(V, p));
+        if (dot(P, P) != 0.0f) {                   // len(P) != 0 without the square root
+            const v3 eye = view_eye(V, view_index(V, s_off, lds, p));
             float4 Kd = g.dif[p], Ks = g.spec[p];
             for (uint32_t l = 0; l < L.total; l++) {
                 const size_t slot = (size_t)(x + y * W * Lmax) + (size_t)W * l;
@@ -346,14 +363,17 @@ __global__ void __launch_bounds__(256) k_shade_primary(GBufView g, uint32_t W, u
 // flags a light whose shadow ray is occluded is skipped.
 __global__ void __launch_bounds__(256) k_shade_reflection(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L,
                                                           Views V, const unsigned long long* rec, uint32_t hit_mode,
-                                                          const uint32_t* sflags, uint32_t soft, uint32_t* pixels) {
+                                                          const uint32_t* sflags, uint32_t soft, uint32_t* pixels,
+                                                          FastDiv fw) {
     const uint32_t n = W * H;
     const unsigned long long miss = hit_mode == 0 ? MISS64 : (unsigned long long)MISS32;
+    __shared__ uint32_t s_off[VIEW_LDS];
+    const bool lds = stage_views(V, s_off);
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-        uint32_t x = p % W, y = p / W;
+        const uint32_t y = fdiv(p, fw), x = p - y * W;
         uint32_t slot = x + y * W * Lmax;
         if (rec[slot] == miss) continue;
-        const v3 eye = view_eye(V, view_of(V, p));
+        const v3 eye = view_eye(V, view_index(V, s_off, lds, p));
         v3 P = xyz(g.pos[p]), N = xyz(g.nrm[p]);
         float4 Kd = g.dif[p], Ks = g.spec[p];
         v3 color = mk(0.0f, 0.0f, 0.0f);
@@ -533,13 +553,14 @@ void launch_create_shadow(const float4* gpos, const float4* gnrm, uint32_t W, ui
                           const float4* light_pos, KeyCfg kc, const Views& V, uint32_t soft, float* rays,
                           uint32_t* flags, uint32_t* keys, uint32_t* values, hipStream_t s) {
     k_create_shadow<<<grid_for((uint64_t)W * H * Lmax << (soft ? 4 : 0), 256, 65536), 256, 0, s>>>(
-        gpos, gnrm, W, H, Lmax, light_pos, kc, V, soft, rays, flags, keys, values);
+        gpos, gnrm, W, H, Lmax, light_pos, kc, V, soft, rays, flags, keys, values, fast_div(W), fast_div(Lmax));
 }
 void launch_create_reflection(const float4* gpos, const float4* gnrm, const float4* gdif, uint32_t W, uint32_t H,
                               uint32_t Lmax, const Views& V, uint32_t gate, uint32_t mark_gaps, KeyCfg kc,
                               float* rays, uint32_t* flags, uint32_t* keys, uint32_t* values, hipStream_t s) {
     k_create_reflection<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(gpos, gnrm, gdif, W, H, Lmax, V, gate,
-                                                                               mark_gaps, kc, rays, flags, keys, values);
+                                                                               mark_gaps, kc, rays, flags, keys, values,
+                                                                               fast_div(W));
 }
 void launch_create_refraction(const float4* gpos, const float4* gnrm, uint32_t W, uint32_t H, uint32_t Lmax,
                               const Views& V, KeyCfg kc, float* rays, uint32_t* flags, uint32_t* keys,
@@ -551,25 +572,26 @@ void launch_create_bounce(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, con
                           uint32_t hit_mode, uint32_t gate, KeyCfg kc, const Views& V, float* rays, uint32_t* flags,
                           uint32_t* keys, uint32_t* values, hipStream_t s) {
     k_create_bounce<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(g, W, H, Lmax, rec, hit_mode, gate, kc, V,
-                                                                           rays, flags, keys, values);
+                                                                           rays, flags, keys, values, fast_div(W));
 }
 void launch_surface(const float* rays, const unsigned long long* rec, uint32_t hit_mode, uint32_t W, uint32_t H,
                     uint32_t slot_stride, uint32_t normalize_n, SceneView sc, GBufView g, hipStream_t s,
                     const Views* V) {
     k_surface<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(rays, rec, hit_mode, W, H, slot_stride,
-                                                                     normalize_n, sc, g, V ? *V : Views{});
+                                                                     normalize_n, sc, g, V ? *V : Views{},
+                                                                     fast_div(W));
 }
 void launch_shade_primary(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L, const Views& V,
                           const uint32_t* sflags, uint32_t soft, uint32_t* pixels, hipStream_t s,
                           const uint32_t* live) {
     k_shade_primary<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(g, W, H, Lmax, L, V, sflags, soft, pixels,
-                                                                           live);
+                                                                           live, fast_div(W));
 }
 void launch_shade_reflection(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, Lights L, const Views& V,
                              const unsigned long long* rec, uint32_t hit_mode, const uint32_t* sflags,
                              uint32_t soft, uint32_t* pixels, hipStream_t s) {
     k_shade_reflection<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(g, W, H, Lmax, L, V, rec, hit_mode,
-                                                                              sflags, soft, pixels);
+                                                                              sflags, soft, pixels, fast_div(W));
 }
 void launch_cluster_build(const float4* tri_sphere, const uint2* range, uint32_t n, float4* cl_sphere,
                           hipStream_t s) {

@@ -45,7 +45,8 @@ inline uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap = 1u << 20) {
 __global__ void __launch_bounds__(256) k_gather(const float* rays, const uint32_t* sorted_values, uint32_t N,
                                                 const PassSizes* ps, const float4* shadow_pos, const float* tmax_in,
                                                 const float4* light_pos, uint32_t W, uint32_t Lmax,
-                                                uint32_t slot_shift, float4* rec0, float4* rec1) {
+                                                uint32_t slot_shift, float4* rec0, float4* rec1, FastDiv fw,
+                                                FastDiv fl) {
     if (ps) N = min(N, ps->N);
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < N; p += gridDim.x * blockDim.x) {
         uint32_t slot = sorted_values[p];
@@ -53,9 +54,10 @@ __global__ void __launch_bounds__(256) k_gather(const float* rays, const uint32_
         v3 P = mk(0.0f, 0.0f, 0.0f);
         if (shadow_pos) {
             const uint32_t i = slot >> slot_shift;
-            const uint32_t x = i % W, y = i / (W * Lmax);
+            const uint32_t qw = fdiv(i, fw);              // l + Lmax y
+            const uint32_t x = i - qw * W, y = fdiv(qw, fl);
             P = xyz(shadow_pos[(size_t)x + (size_t)y * W]);
-            if (!rays) shadow_ray(P, xyz(light_pos[(i / W) % Lmax]), slot_shift != 0u, slot & 15u, o, d);
+            if (!rays) shadow_ray(P, xyz(light_pos[qw - y * Lmax]), slot_shift != 0u, slot & 15u, o, d);
         }
         if (rays) {
             const float2* r = reinterpret_cast<const float2*>(rays + 6 * (size_t)slot);
@@ -643,8 +645,24 @@ __global__ void __launch_bounds__(256) k_cull_items(TravParams P, uint4* obj_ent
 // spread-out secondary rays (2x on the atrium's bounces) and costs on coherent
 // ones; the engine picks the mode per pass.  Modes 1 and 2 use under 102 VGPRs
 // and 7.6 / 8.9 KB of LDS per wave (five waves per SIMD); mode 0, four.
+// Occupancy targets (waves per SIMD) of the pre-test modes and of mode 0,
+// and the blocks of triangle spheres loaded back to back in stage A.
+// Measured on the 1080p atrium frame and its 64-view batch (median frame /
+// graph-replayed batch): 2 blocks 6.02 ms / 66.3 ms; 1 block 6.11 / 67.9;
+// 3 blocks 6.09 / 66.9; 4 blocks 6.37 / 70.3; 6 blocks 6.84 / 76.6.  Four or
+// six waves per SIMD for the pre-test modes measured slower than five.
+#ifndef RSH_TRAV_WAVES_PRE
+#define RSH_TRAV_WAVES_PRE 5
+#endif
+#ifndef RSH_TRAV_WAVES_EXACT
+#define RSH_TRAV_WAVES_EXACT 4
+#endif
+#ifndef RSH_TRAV_NB
+#define RSH_TRAV_NB 2
+#endif
 template <int LEAF, int MODE>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MODE >= 1 ? 5 : 4))) k_traverse64(TravParams P) {
+__global__ void __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(MODE >= 1 ? RSH_TRAV_WAVES_PRE : RSH_TRAV_WAVES_EXACT))) k_traverse64(TravParams P) {
     constexpr bool PRE = MODE >= 1;               // ray / triangle-sphere pre-test
     constexpr bool RM = MODE == 2;                // and ray / cluster-sphere masks
     constexpr uint32_t BRANCH = 64 / LEAF;
@@ -1002,7 +1020,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MODE >=
 
     // NB blocks per iteration: their sphere loads are issued back to back and
     // consumed in order, so only the first load's latency is exposed.
-    constexpr int NB = 4;
+    constexpr int NB = RSH_TRAV_NB;
     bool have = rem != 0 || lc_cur < lc_end;
     while (have && !done) {
         uint32_t tbk[NB], cmk[NB], cidk[NB];
@@ -1248,7 +1266,8 @@ void launch_gather(const float* rays, const uint32_t* sorted_values, uint32_t N,
                    const float4* shadow_pos, const float* tmax, uint32_t W, uint32_t Lmax, uint32_t slot_shift,
                    float4* rec0, float4* rec1, hipStream_t s, const float4* light_pos) {
     if (N) k_gather<<<grid_for(N, 256, 65536), 256, 0, s>>>(rays, sorted_values, N, ps, shadow_pos, tmax, light_pos,
-                                                            W, Lmax, slot_shift, rec0, rec1);
+                                                            W, Lmax, slot_shift, rec0, rec1, fast_div(W),
+                                                            fast_div(Lmax));
 }
 
 // ps: device-sized (N, counts and offsets are capacities; the kernels read

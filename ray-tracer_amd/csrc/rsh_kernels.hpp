@@ -1,6 +1,7 @@
 // rsh_kernels.hpp -- kernel parameter blocks and launchers of the ray,
 // surface, shading, hierarchy and traversal stages.
 #pragma once
+#include "fastdiv.hpp"
 #include "rsh_internal.hpp"
 
 namespace rsh {
@@ -41,32 +42,31 @@ __device__ __forceinline__ uint32_t view_of(const Views& V, uint32_t p) {
     }
     return a;
 }
-// Batches of up to VIEW_LDS views stage the view table in LDS once per
-// workgroup, so that a per-pixel view lookup is a search in LDS instead of a
-// chain of dependent global loads.  Call from every thread of the workgroup.
+// Batches of up to VIEW_LDS views stage the views' pixel offsets in LDS once
+// per workgroup, so that a per-pixel view lookup is a search in LDS followed
+// by one (cached) load, instead of a chain of dependent global loads.  Call
+// from every thread of the workgroup.
 constexpr uint32_t VIEW_LDS = 64;
-__device__ __forceinline__ bool stage_views(const Views& V, ViewDesc* s_v) {
+__device__ __forceinline__ bool stage_views(const Views& V, uint32_t* s_off) {
     const bool lds = V.n > 1 && V.n <= VIEW_LDS;
     if (lds) {
-        const uint32_t words = V.n * (uint32_t)(sizeof(ViewDesc) / 4);
-        const uint32_t* src = reinterpret_cast<const uint32_t*>(V.d);
-        uint32_t* dst = reinterpret_cast<uint32_t*>(s_v);
-        for (uint32_t i = threadIdx.x; i < words; i += blockDim.x) dst[i] = src[i];
+        for (uint32_t i = threadIdx.x; i < V.n; i += blockDim.x) s_off[i] = V.d[i].off;
         __syncthreads();
     }
     return lds;
 }
-// The view of pixel p (s_v, lds: from stage_views).
-__device__ __forceinline__ ViewDesc view_at(const Views& V, const ViewDesc* s_v, bool lds, uint32_t p) {
-    if (lds) {
-        uint32_t a = 0, b = V.n;
-        while (b - a > 1) {
-            const uint32_t m = (a + b) >> 1;
-            if (s_v[m].off <= p) a = m; else b = m;
-        }
-        return s_v[a];
+// The view of pixel p (s_off, lds: from stage_views).
+__device__ __forceinline__ uint32_t view_index(const Views& V, const uint32_t* s_off, bool lds, uint32_t p) {
+    if (!lds) return view_of(V, p);
+    uint32_t a = 0, b = V.n;
+    while (b - a > 1) {
+        const uint32_t m = (a + b) >> 1;
+        if (s_off[m] <= p) a = m; else b = m;
     }
-    return V.n ? V.d[view_of(V, p)] : V.one;
+    return a;
+}
+__device__ __forceinline__ ViewDesc view_at(const Views& V, const uint32_t* s_off, bool lds, uint32_t p) {
+    return V.n ? V.d[view_index(V, s_off, lds, p)] : V.one;
 }
 __device__ __forceinline__ v3 view_eye(const Views& V, uint32_t v) {
     const float* e = V.n ? V.d[v].eye : V.one.eye;

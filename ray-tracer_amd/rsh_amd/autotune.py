@@ -2,7 +2,8 @@
 
 Searches the octahedral hash split (cell bits x direction bits, at most 26
 bits below the view id), the hierarchy shape (leaf size x branching factor,
-depth 2) and the traversal work-item size (candidate clusters per item) by
+depth 2), the traversal work-item size (candidate clusters per item) and the
+traversal mode (the engine's per-pass rule, or one mode for every pass) by
 coordinate descent: each dimension in turn, keeping the best value found so
 far for the others.  A candidate is timed as the median of graph replays of
 the whole batch (render flag bit 3), after the eager frame that captures it.
@@ -22,6 +23,7 @@ from . import RshError
 HASH_SPLITS = [(4, 6), (5, 4), (3, 7), (6, 3), (2, 9)]        # 2 + 3c + 2d <= 26
 SHAPES = [(8, 8), (4, 16), (16, 4), (32, 2), (8, 4)]          # (leaf, branch); leaf * branch != 64 -> generic kernel
 CHUNKS = [64, 16, 256]                                        # candidate clusters per work item
+PREFILTERS = [0, 2, 3]          # traversal mode: the per-pass rule, ray pre-test everywhere, + cluster masks
 
 
 @dataclasses.dataclass
@@ -31,6 +33,7 @@ class Trial:
     leaf: int
     branch: int
     traversal_chunk: int
+    traversal_prefilter: int
     ms: float
     mrays: float
     same_pixels: float
@@ -63,10 +66,12 @@ def tune(engine, scene, views, base_cfg, reps=3, log=None):
             ms, rays = time_batch(engine, views, cfg, reps)
         except RshError as err:      # e.g. a work list over capacity for a small chunk: not a candidate
             if log:
-                log(f"skipped {cfg.cell_bits}/{cfg.dir_bits} {cfg.leaf}x{cfg.branch} chunk {cfg.traversal_chunk}: {err}")
+                log(f"skipped {cfg.cell_bits}/{cfg.dir_bits} {cfg.leaf}x{cfg.branch} chunk {cfg.traversal_chunk} "
+                f"mode {cfg.traversal_prefilter}: {err}")
             return
         same = float((engine.pixels() == ref).mean())
-        t = Trial(cfg.cell_bits, cfg.dir_bits, cfg.leaf, cfg.branch, cfg.traversal_chunk, ms, rays / ms / 1e3, same)
+        t = Trial(cfg.cell_bits, cfg.dir_bits, cfg.leaf, cfg.branch, cfg.traversal_chunk, cfg.traversal_prefilter, ms,
+                  rays / ms / 1e3, same)
         trials.append(t)
         if log:
             log(t)
@@ -83,4 +88,7 @@ def tune(engine, scene, views, base_cfg, reps=3, log=None):
     for ch in CHUNKS:
         if ch != best.traversal_chunk:
             run(dataclasses.replace(best, traversal_chunk=ch))
+    for pf in PREFILTERS:
+        if pf != best.traversal_prefilter:
+            run(dataclasses.replace(best, traversal_prefilter=pf))
     return best, trials

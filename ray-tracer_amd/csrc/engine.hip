@@ -32,6 +32,16 @@ using namespace rsh;
 // Graph replay (render flag bit 3).  An eager frame records every pass's
 // sizes (the values the host otherwise reads back); a later run replays
 // those sizes without read-backs or synchronisation under stream capture.
+// Traversal mode of a frame's camera pass (TravParams::prefilter), and with
+// RSH_SHADOW_MODE1 every shadow pass in mode 1 (1080p atrium frame 6.30 against
+// 5.78 ms with the choice below).
+#ifndef RSH_CAMERA_MODE
+#define RSH_CAMERA_MODE 1u
+#endif
+#ifndef RSH_SHADOW_MODE1
+#define RSH_SHADOW_MODE1 0
+#endif
+
 struct PassPlan { uint32_t N = 0, C = 0, chunk = 0, list_total = 0, items = 0, mode = 0; };
 
 // One pass pipeline's stream and scratch.  A frame runs its shadow passes on a
@@ -240,6 +250,14 @@ constexpr uint32_t ITEM_CHUNK = 64;
 // against 67.8 ms).
 #ifndef RSH_SHADOW_LONG_ITEMS
 #define RSH_SHADOW_LONG_ITEMS 1
+#endif
+// Passes with at least RSH_LONGER_ITEMS * ITEM_TARGET items of 2 ITEM_CHUNK
+// clusters take items of 4 ITEM_CHUNK (0: never).  On the 1080p atrium (its
+// camera pass alone qualifies) and its 64-view batch (most passes do), median
+// frame / graph-replayed batch: never 5.77 / 62.0 ms, 4 5.65 / 59.9, 2 5.76 /
+// 60.0 (the first reflection pass also takes them and runs 0.97 -> 1.09 ms).
+#ifndef RSH_LONGER_ITEMS
+#define RSH_LONGER_ITEMS 4
 #endif
 constexpr size_t HSTAGE_WORDS = 2 + 2 * ITEM_TABLE_WORDS;   // N / C or the item total, then the item table
 constexpr uint32_t ITEM_CHUNK_MIN = 4;
@@ -494,9 +512,17 @@ rsh_status build_items(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, Trav
         // One read-back sizes the pass: the list total and, from the table of
         // item totals per chunk size, the chunk (shrunk from ITEM_CHUNK until the
         // pass has ITEM_TARGET items) and the item total.
-        const bool tabled = chunk <= ITEM_TABLE || (chunk <= 2 * ITEM_TABLE && chunk % 2 == 0);
+        const bool tabled = chunk <= ITEM_TABLE || (chunk <= 2 * ITEM_TABLE && chunk % 2 == 0) ||
+                            (chunk <= 4 * ITEM_TABLE && chunk % 4 == 0);
         unsigned long long ht[ITEM_TABLE_WORDS] = {};
-        auto items_at = [&](uint32_t c) { return c <= ITEM_TABLE ? ht[c - 1] : ht[ITEM_TABLE + c / 2 - 1]; };
+        // c tabled: c <= ITEM_TABLE, or even up to 2 ITEM_TABLE, or a multiple of 4 up to 4 ITEM_TABLE
+        auto items_at = [&](uint32_t c) {
+            return c <= ITEM_TABLE ? ht[c - 1] : c <= 2 * ITEM_TABLE ? ht[ITEM_TABLE + c / 2 - 1]
+                                                                      : ht[2 * ITEM_TABLE + c / 4 - 1];
+        };
+        auto tabled_below = [&](uint32_t c) {    // the largest tabled chunk size <= c
+            return c <= ITEM_TABLE ? c : c <= 2 * ITEM_TABLE ? c & ~1u : c & ~3u;
+        };
         uint32_t* first = u32p(L.w_first);
         LANE_OK(L.w_table.reserve(ITEM_TABLE_WORDS * sizeof(unsigned long long)));
         unsigned long long* table = L.w_table.as<unsigned long long>();
@@ -520,7 +546,7 @@ rsh_status build_items(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, Trav
             WAIT_OK(readback_wait(L));
             std::memcpy(ht, L.hstage + 2, sizeof(ht));
         }
-        if (p.algorithm != 1) tot[0] = (uint32_t)ht[2 * ITEM_TABLE + 1];
+        if (p.algorithm != 1) tot[0] = (uint32_t)ht[TABLE_LIST];
         if (!tabled) {
             tot[1] = L.hstage[1];
             launch_fill_u64(table, 0ull, ITEM_TABLE_WORDS, s);    // for the next pass
@@ -534,7 +560,10 @@ rsh_status build_items(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, Trav
         // nodes with tan(spread) outside [0, WIDE_TAN] picks the best mode for
         // every pass of the 1080p atrium frame (first shadow pass without, every
         // later pass with).
-        if (cfg.traversal_prefilter == 0) p.prefilter = !camera && 16 * ht[2 * ITEM_TABLE] > ntop ? 2u : 1u;
+        if (cfg.traversal_prefilter == 0)
+            p.prefilter = camera ? RSH_CAMERA_MODE
+                          : (RSH_SHADOW_MODE1 && p.kind == 1) ? 1u
+                          : 16 * ht[TABLE_WIDE] > ntop ? 2u : 1u;
         if (tabled) {
             if (!cfg.traversal_chunk) {
                 // passes with plenty of items take longer items (less per-item
@@ -542,9 +571,12 @@ rsh_status build_items(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, Trav
                 // reflection pass 1.13 -> 1.05 ms)
                 if (items_at(ITEM_CHUNK) >= 2 * ITEM_TARGET && (RSH_SHADOW_LONG_ITEMS || p.kind != 1))
                     chunk = 2 * ITEM_CHUNK;
+                if (RSH_LONGER_ITEMS && chunk == 2 * ITEM_CHUNK &&
+                    items_at(2 * ITEM_CHUNK) >= (uint64_t)RSH_LONGER_ITEMS * ITEM_TARGET)
+                    chunk = 4 * ITEM_CHUNK;
                 while (items_at(chunk) < ITEM_TARGET && chunk > ITEM_CHUNK_MIN && items_at(chunk) != 0) {
                     const uint64_t want = (uint64_t)chunk * items_at(chunk) / ITEM_TARGET;
-                    chunk = want < ITEM_CHUNK_MIN ? ITEM_CHUNK_MIN : (uint32_t)want;
+                    chunk = want < ITEM_CHUNK_MIN ? ITEM_CHUNK_MIN : tabled_below((uint32_t)want);
                 }
             }
             if (items_at(chunk) > MAX_TRAVERSE_ITEMS) {

@@ -190,11 +190,16 @@ __global__ void __launch_bounds__(256) k_create_reflection(const float4* gpos, c
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
         const uint32_t y = fdiv(p, fw), x = p - y * W;
         uint32_t slot = x + y * W * Lmax;
-        v3 P = xyz(gpos[p]);
-        const v3 Nr = xyz(gnrm[p]);
-        bool live = dot(P, P) != 0.0f && (!gate || gdif[p].w > 0.0f);      // len(P) != 0
+        // the gate first: the surface point and normal of a pixel that does
+        // not reflect are never read
+        bool live = !gate || gdif[p].w > 0.0f;
+        v3 P = mk(0.0f, 0.0f, 0.0f);
         if (live) {
-            const v3 N = normalize(Nr);
+            P = xyz(gpos[p]);
+            live = dot(P, P) != 0.0f;              // len(P) != 0 without the square root
+        }
+        if (live) {
+            const v3 N = normalize(xyz(gnrm[p]));
             const uint32_t v = view_index(V, s_off, lds, p);
             v3 d = reflect(normalize(P - view_eye(V, v)), normalize(N));
             store_ray(rays, slot, P, d);

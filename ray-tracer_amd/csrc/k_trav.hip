@@ -341,6 +341,12 @@ __device__ __forceinline__ uint32_t wave_incl_add(uint32_t x) {
     return x;
 }
 
+__device__ __forceinline__ float wave_max(float x) {
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) x = fmaxf(x, __shfl_xor(x, d, 64));
+    return x;
+}
+
 // The counting pass stores each node's passing objects as 64-bit masks
 // (pmask, ceil(K / 64) words per node); the list pass reads them back instead
 // of repeating the sphere tests.
@@ -454,40 +460,45 @@ __global__ void __launch_bounds__(64) k_item_table(const uint32_t* cand, const f
     __builtin_amdgcn_wave_barrier();
     // four candidates per LDS read (the zero padding adds nothing)
     const uint32_t q1 = (t1 - t0 + 3u) / 4u;
-    uint32_t lo = 0, lo2 = 0;                  // per-read partial sums stay below 2^32
-    unsigned long long sum = 0ull, sum2 = 0ull;
-    if (fast) {       // every n + 2c - 1 within div_hi's bound for 2c <= 128
-        const uint32_t m1 = div_hi_magic(c), m2 = div_hi_magic(2u * c);
+    uint32_t lo = 0, lo2 = 0, lo4 = 0;         // per-read partial sums stay below 2^32
+    unsigned long long sum = 0ull, sum2 = 0ull, sum4 = 0ull;
+    if (fast) {       // every n + 4c - 1 within div_hi's bound for 4c <= 256
+        const uint32_t m1 = div_hi_magic(c), m2 = div_hi_magic(2u * c), m4 = div_hi_magic(4u * c);
         auto f1 = [&](uint32_t n) { return div_hi(n + c - 1u, c, m1); };
         auto f2 = [&](uint32_t n) { return div_hi(n + 2u * c - 1u, 2u * c, m2); };
+        auto f4 = [&](uint32_t n) { return div_hi(n + 4u * c - 1u, 4u * c, m4); };
 #pragma unroll 4
         for (uint32_t q = 0; q < q1; q++) {
             const uint4 v = s_cand4[q];
             lo = f1(v.x) + f1(v.y);
             lo2 = f2(v.x) + f2(v.y);
-            sum += lo; sum2 += lo2;
+            lo4 = f4(v.x) + f4(v.y);
+            sum += lo; sum2 += lo2; sum4 += lo4;
             lo = f1(v.z) + f1(v.w);
             lo2 = f2(v.z) + f2(v.w);
-            sum += lo; sum2 += lo2;
+            lo4 = f4(v.z) + f4(v.w);
+            sum += lo; sum2 += lo2; sum4 += lo4;
         }
     } else {
         for (uint32_t t = 0; t < t1 - t0; t++) {
             const uint32_t n = s_cand[t];
             sum += (n + c - 1u) / c;
             sum2 += (n + 2u * c - 1u) / (2u * c);
+            sum4 += (n + 4u * c - 1u) / (4u * c);
         }
     }
     if (sum) atomicAdd(&table[c - 1u], sum);
     if (sum2) atomicAdd(&table[ITEM_TABLE + c - 1u], sum2);
+    if (sum4) atomicAdd(&table[2 * ITEM_TABLE + c - 1u], sum4);
     uint32_t wide = 0;
     for (uint32_t t = t0 + lane; t < t1; t += 64u) wide += !(top_aux[t].y >= 0.0f && top_aux[t].y <= WIDE_TAN) ? 1u : 0u;
     for (int d = 32; d > 0; d >>= 1) wide += __shfl_xor(wide, d, 64);
-    if (lane == 0 && wide) atomicAdd(&table[2 * ITEM_TABLE], (unsigned long long)wide);
+    if (lane == 0 && wide) atomicAdd(&table[TABLE_WIDE], (unsigned long long)wide);
     if (npass) {
         unsigned long long np = 0ull;
         for (uint32_t t = t0 + lane; t < t1; t += 64u) np += npass[t];
         for (int d = 32; d > 0; d >>= 1) np += __shfl_xor(np, d, 64);
-        if (lane == 0 && np) atomicAdd(&table[2 * ITEM_TABLE + 1], np);
+        if (lane == 0 && np) atomicAdd(&table[TABLE_LIST], np);
     }
     if (host) {
         __threadfence();
@@ -646,11 +657,21 @@ __global__ void __launch_bounds__(256) k_cull_items(TravParams P, uint4* obj_ent
 // ones; the engine picks the mode per pass.  Modes 1 and 2 use under 102 VGPRs
 // and 7.6 / 8.9 KB of LDS per wave (five waves per SIMD); mode 0, four.
 // Occupancy targets (waves per SIMD) of the pre-test modes and of mode 0,
-// and the blocks of triangle spheres loaded back to back in stage A.
+// the blocks of triangle spheres loaded back to back in stage A, and the two
+// distance prunings (RSH_TRAV_TPRUNE: each ray's pre-test also bounds the
+// triangle's parameter range by EPS and the ray's length or closest hit so
+// far; RSH_TRAV_NPRUNE: mode 1 shadow items skip clusters and triangles
+// beyond the reach of all their rays).
 // Measured on the 1080p atrium frame and its 64-view batch (median frame /
 // graph-replayed batch): 2 blocks 6.02 ms / 66.3 ms; 1 block 6.11 / 67.9;
 // 3 blocks 6.09 / 66.9; 4 blocks 6.37 / 70.3; 6 blocks 6.84 / 76.6.  Four or
 // six waves per SIMD for the pre-test modes measured slower than five.
+// Pruning, same measures: neither 6.04 / 66.3; TPRUNE 5.90 / 65.1; with
+// NPRUNE in every pass 6.05 / 65.2 (closest-hit passes lose: their reach is
+// infinite until every ray of the item has a hit), in mode 1 shadow passes
+// only 5.78 / 62.3.  Seeding a split node's items with the records its other
+// items had merged (5.99 / 65.5) and the exact mode for camera rays (6.14 /
+// 70.1) measured slower.
 #ifndef RSH_TRAV_WAVES_PRE
 #define RSH_TRAV_WAVES_PRE 5
 #endif
@@ -659,6 +680,12 @@ __global__ void __launch_bounds__(256) k_cull_items(TravParams P, uint4* obj_ent
 #endif
 #ifndef RSH_TRAV_NB
 #define RSH_TRAV_NB 2
+#endif
+#ifndef RSH_TRAV_TPRUNE
+#define RSH_TRAV_TPRUNE 1
+#endif
+#ifndef RSH_TRAV_NPRUNE
+#define RSH_TRAV_NPRUNE 1
 #endif
 template <int LEAF, int MODE>
 __global__ void __launch_bounds__(64)
@@ -740,6 +767,57 @@ __attribute__((amdgpu_waves_per_eu(MODE >= 1 ? RSH_TRAV_WAVES_PRE : RSH_TRAV_WAV
     bool done = false;
     uint32_t qlen = 0;
     __builtin_amdgcn_wave_barrier();
+
+    // Pre-test of ray `ray` (records r0, r1) against a triangle's or a
+    // cluster's bounding sphere: its line, and with RSH_TRAV_TPRUNE the part
+    // of it a hit can still use -- beyond EPS and, for a shadow ray, short of
+    // its length; for a closest-hit ray (exact 64-bit records), short of the
+    // closest hit found so far.
+    auto may_hit = [&](uint32_t ray, float4 r0, float4 r1, float4 sph) -> bool {
+        if constexpr (RSH_TRAV_TPRUNE != 0) {
+            float hi = r0.w;
+            if (P.kind != 1)
+                hi = P.hit_mode == 0 ? __uint_as_float((uint32_t)(s_best[ray] >> 32)) : __int_as_float(0x7FC00000);
+            return ray_may_hit_sphere_range(xyz(r0), xyz(r1), sph, EPS, hi);
+        } else {
+            return ray_may_hit_sphere(xyz(r0), xyz(r1), sph);
+        }
+    };
+
+    // Reach of the item (RSH_TRAV_NPRUNE, not in counting runs, whose pair
+    // and test counts stay the reference's): every live ray i starts within
+    // r_org of the top node's centre s and can still change its result only
+    // at a distance below hi_i |d_i| from its start (hi as in may_hit), so a
+    // sphere (q, rho) with |q - s| > (reach + r_org + rho) 1.001 + 1e-5,
+    // reach = max_i hi_i |d_i|, holds no triangle that changes a result (the
+    // slack covers the FP32 rounding of this test and of ray_triangle's
+    // parameter, ~1e-6 of these magnitudes).  reach is infinite while some
+    // closest-hit ray has no hit, and is recomputed after each flush.
+    const bool prune = RSH_TRAV_NPRUNE != 0 && MODE == 1 && P.kind == 1 && !count;
+    float r_org = 0.0f, dlen = 0.0f, reach = __builtin_inff();
+    if (prune) {
+        const v3 o = xyz(s_ray0[lane]), d = xyz(s_ray1[lane]);
+        dlen = valid ? len(d) : 0.0f;
+        r_org = wave_max(valid ? len(o - xyz(tS)) : 0.0f);
+    }
+    auto update_reach = [&]() {
+        if (!prune) return;
+        float v = 0.0f;        // no live ray: nothing left to reach
+        const unsigned long long b = s_best[lane];
+        if (valid && (P.kind != 1 || b == 0ull)) {
+            const float hi = P.kind == 1 ? s_ray0[lane].w
+                                         : (P.hit_mode == 0 ? __uint_as_float((uint32_t)(b >> 32)) : __builtin_inff());
+            v = hi * dlen;
+            if (!(v <= 3.0e38f)) v = __builtin_inff();    // NaN (no hit yet) or overflow
+        }
+        reach = wave_max(v);
+    };
+    auto beyond = [&](float4 sph) -> bool {
+        const v3 w = xyz(sph) - xyz(tS);
+        const float lim = (reach + r_org + sph.w) * 1.001f + 1e-5f;
+        return dot(w, w) > lim * lim;
+    };
+    update_reach();
 
     // ---- stages B + C over the queued survivors, SCH at a time:
     //  B  the (survivor, leaf) candidates allowed by each survivor's cluster
@@ -840,7 +918,7 @@ __attribute__((amdgpu_waves_per_eu(MODE >= 1 ? RSH_TRAV_WAVES_PRE : RSH_TRAV_WAV
                         code = s_tq[(qh + lane) & (TQ - 1)];
                         const uint32_t sv = code >> 6, ray = code & 63u;
                         if (P.kind != 1 || s_best[ray] == 0ull)
-                            keep = ray_may_hit_sphere(xyz(s_ray0[ray]), xyz(s_ray1[ray]), s_qsph[c0 + sv]);
+                            keep = may_hit(ray, s_ray0[ray], s_ray1[ray], s_qsph[c0 + sv]);
                     }
                     const uint64_t km = __ballot(keep);
                     if (keep) s_tq2[(rt + (uint32_t)__popcll(km & lt)) & (TQ - 1)] = (uint16_t)code;
@@ -904,7 +982,7 @@ __attribute__((amdgpu_waves_per_eu(MODE >= 1 ? RSH_TRAV_WAVES_PRE : RSH_TRAV_WAV
                         const float4 r0 = s_ray0[ray], r1 = s_ray1[ray];
                         if (r1.w != 0.0f && (P.kind != 1 || s_best[ray] == 0ull)) {
                             if (count) n_tests++;
-                            keep = ray_may_hit_sphere(xyz(r0), xyz(r1), s_qsph[c0 + sv]);
+                            keep = may_hit(ray, r0, r1, s_qsph[c0 + sv]);
                             code = (sv << 6) | ray;
                         }
                     }
@@ -936,6 +1014,7 @@ __attribute__((amdgpu_waves_per_eu(MODE >= 1 ? RSH_TRAV_WAVES_PRE : RSH_TRAV_WAV
             if (P.kind == 1 && (__ballot(s_best[lane] != 0ull) & alive) == alive) done = true;
         }
         qlen = 0;
+        update_reach();
     };
 
     // ---- cursor over this item's clusters of the node's cluster stream, from the
@@ -1003,7 +1082,10 @@ __attribute__((amdgpu_waves_per_eu(MODE >= 1 ? RSH_TRAV_WAVES_PRE : RSH_TRAV_WAV
             uint64_t gm = 1ull;                 // one cluster: the object test already stands for it
             if (e_ncl != 1) {
                 bool cp = false;
-                if (lane < n) cp = cluster_may_pass(tS, tC, tA.y, t_bound, P.cl_sphere[e_cbase + g0 + lane]);
+                if (lane < n) {
+                    const float4 cs = P.cl_sphere[e_cbase + g0 + lane];
+                    cp = cluster_may_pass(tS, tC, tA.y, t_bound, cs) && !(prune && beyond(cs));
+                }
                 gm = __ballot(cp);
                 if (count) n_cl += n;
             }
@@ -1043,7 +1125,7 @@ __attribute__((amdgpu_waves_per_eu(MODE >= 1 ? RSH_TRAV_WAVES_PRE : RSH_TRAV_WAV
             if (done) break;             // every ray of the wave is occluded: stop queueing
             // ---- stage A
             bool p = false;
-            if (ink[k]) p = sphere_node_fast(tS, tC, tsk[k], tA.x, tA.y, t_bound.y);
+            if (ink[k]) p = sphere_node_fast(tS, tC, tsk[k], tA.x, tA.y, t_bound.y) && !(prune && beyond(tsk[k]));
             uint64_t m = __ballot(p);
             if (count) { n_top += (uint64_t)__popcll(__ballot(ink[k])); n_surv += (uint64_t)__popcll(m); }
             if (m == 0ull) continue;
@@ -1064,9 +1146,9 @@ __attribute__((amdgpu_waves_per_eu(MODE >= 1 ? RSH_TRAV_WAVES_PRE : RSH_TRAV_WAV
                             lr |= (((r >> (b * LEAF)) & LRAYS) != 0ull ? 1u : 0u) << b;
                         return lr;
                     };
-                    const v3 ro = xyz(s_ray0[lane]), rd = xyz(s_ray1[lane]);
+                    const float4 ro = s_ray0[lane], rd = s_ray1[lane];
                     const uint32_t use = P.sub_use[cidk[k]];
-                    rm = __ballot(((alive >> lane) & 1ull) && ray_may_hit_sphere(ro, rd, P.cl_sphere[cidk[k]]));
+                    rm = __ballot(((alive >> lane) & 1ull) && may_hit(lane, ro, rd, P.cl_sphere[cidk[k]]));
                     cm &= leaves_of(rm);
                     if (cm == 0u) continue;
                     // sub-clusters worth a mask: small enough (sub_use) and survivors
@@ -1084,7 +1166,7 @@ __attribute__((amdgpu_waves_per_eu(MODE >= 1 ? RSH_TRAV_WAVES_PRE : RSH_TRAV_WAV
                         do {
                             const uint32_t k8 = (uint32_t)__builtin_ctz(subs);
                             subs &= subs - 1u;
-                            const uint64_t rk = __ballot(rin && ray_may_hit_sphere(ro, rd, ssp[k8]));
+                            const uint64_t rk = __ballot(rin && may_hit(lane, ro, rd, ssp[k8]));
                             if (lane / SUBCLUSTER == k8) rm = rk;
                         } while (subs);
                         cm &= leaves_of(rm);
@@ -1117,7 +1199,7 @@ __attribute__((amdgpu_waves_per_eu(MODE >= 1 ? RSH_TRAV_WAVES_PRE : RSH_TRAV_WAV
     if (valid) {
         if (P.kind == 1) { if (best != 0ull) P.shadow[slot] = MISS32; }
         else if (single) P.closest[slot] = best;
-        else if (best != (P.hit_mode == 0 ? MISS64 : (unsigned long long)MISS32)) atomicMin(&P.closest[slot], best);
+        else if (best != miss) atomicMin(&P.closest[slot], best);
     }
     if (count) {
         unsigned long long tsum = n_tests, msum = n_mt;
@@ -1303,7 +1385,7 @@ void launch_item_table(const uint32_t* cand, const float2* top_aux, uint32_t nto
                        const uint32_t* npass, unsigned long long* table, hipStream_t s, uint32_t* done,
                        unsigned long long* host, uint32_t seq) {
     static_assert(ITEM_TABLE == 64, "one lane per chunk size");
-    const bool fast = div_hi_ok((uint64_t)max_cand + 2 * ITEM_TABLE - 1, 2 * ITEM_TABLE);
+    const bool fast = div_hi_ok((uint64_t)max_cand + 4 * ITEM_TABLE - 1, 4 * ITEM_TABLE);
     if (ntop)
         k_item_table<<<(ntop + TABLE_SLICE - 1) / TABLE_SLICE, 64, 0, s>>>(cand, top_aux, ntop, table, fast,
                                                                            npass, done, host, seq);

@@ -359,6 +359,31 @@ __device__ __forceinline__ bool ray_may_hit_sphere(v3 o, v3 d, float4 sph) {
     return !(dot(x, x) > rr * rr * dot(d, d));
 }
 
+// ray_may_hit_sphere on the part of the line a hit can use: also false when
+// the sphere's parameter range along the ray, (w.d -+ r|d|) / |d|^2 with
+// w = c - o, ends before lo or starts beyond hi, with r widened by
+// 1e-3 (|w|_1 + r) + 1e-5 (ten times the line test's margin: ray_triangle's
+// parameter carries the same relative rounding as its hit point).  The
+// tests compare squares, (w.d - hi |d|^2)^2 > r'^2 |d|^2 with the left side
+// positive, so no square root is taken.  lo is the hit threshold EPS; hi the
+// shadow ray's length or the parameter of the closest hit found so far (a
+// triangle starting beyond it cannot give a smaller record).  NaN or
+// infinite bounds never reject.
+__device__ __forceinline__ bool ray_may_hit_sphere_range(v3 o, v3 d, float4 sph, float lo, float hi) {
+    const v3 w = xyz(sph) - o;
+    const v3 x = cross(w, d);
+    const float l1 = fabsf(w.x) + fabsf(w.y) + fabsf(w.z) + sph.w;
+    const float rr = sph.w + 1e-4f * l1 + 1e-6f;
+    const float dd = dot(d, d);
+    if (dot(x, x) > rr * rr * dd) return false;
+    const float rt = sph.w + 1e-3f * l1 + 1e-5f;
+    const float A = rt * rt * dd;
+    const float wd = dot(w, d);
+    const float e = wd - hi * dd;     // > rt |d|: the sphere starts beyond hi
+    const float f = lo * dd - wd;     // > rt |d|: the sphere ends before lo
+    return !((e > 0.0f && e * e > A) || (f > 0.0f && f * f > A));
+}
+
 // Conservative cluster pre-test for sphere_node.  With J = I - cd cd^T the
 // triangle test passes iff f(c) = |J (c - s)| - ((c - s).cd) tan <= (r + rt)/cos.
 // f is Lipschitz with constant L = sqrt(1 + (tan k)^2 + 2 tan k |1 - k^2|),

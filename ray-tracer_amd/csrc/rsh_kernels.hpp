@@ -194,18 +194,21 @@ void launch_cull_count(const TravParams& p, uint32_t* npass, uint32_t* cand, uns
 void launch_cull_list(const TravParams& p, uint4* obj_ent, uint32_t* obj_cum, unsigned long long* pmask,
                       hipStream_t s);
 // Item totals per chunk size: table[c - 1] = sum of ceil(cand / c), c = 1..ITEM_TABLE;
-// table[ITEM_TABLE + c - 1] the same for chunk 2c; table[2 ITEM_TABLE] = top nodes
-// with a wide cone (spread tangent outside [0, WIDE_TAN]).
+// table[ITEM_TABLE + c - 1] the same for chunk 2c, table[2 ITEM_TABLE + c - 1]
+// for chunk 4c; table[TABLE_WIDE] = top nodes with a wide cone (spread tangent
+// outside [0, WIDE_TAN]); table[TABLE_LIST] = list entries (npass summed).
 constexpr uint32_t ITEM_TABLE = 64;
+constexpr uint32_t TABLE_WIDE = 3 * ITEM_TABLE;
+constexpr uint32_t TABLE_LIST = 3 * ITEM_TABLE + 1;
 constexpr float WIDE_TAN = 0.1f;
 // max_cand: an upper bound of every cand entry (the scene's cluster count);
-// table: 2 ITEM_TABLE + 2 entries, zeroed beforehand; npass (summed into the
-// last entry) may be null.  host / done / seq: see k_item_table (the table
+// table: ITEM_TABLE_WORDS entries, zeroed beforehand; npass (summed into
+// table[TABLE_LIST]) may be null.  host / done / seq: see k_item_table (the table
 // published to host memory by the last block; done a zeroed device word).
 void launch_item_table(const uint32_t* cand, const float2* top_aux, uint32_t ntop, uint32_t max_cand,
                        const uint32_t* npass, unsigned long long* table, hipStream_t s, uint32_t* done = nullptr,
                        unsigned long long* host = nullptr, uint32_t seq = 0);
-constexpr uint32_t ITEM_TABLE_WORDS = 2 * ITEM_TABLE + 2;   // 64-bit entries
+constexpr uint32_t ITEM_TABLE_WORDS = 3 * ITEM_TABLE + 2;   // 64-bit entries
 void launch_item_count(const uint32_t* cand, uint32_t chunk, uint32_t ntop, uint32_t* nitems, hipStream_t s);
 // CRSH: the list pass and the work-item descriptors in one launch (needs
 // obj_off, item_first and chunk in p)

@@ -194,6 +194,7 @@ bool fail(rsh_engine* e, const std::string& msg) {
 
 inline float4* f4p(const DevBuf& b) { return b.as<float4>(); }
 inline uint32_t* u32p(const DevBuf& b) { return b.as<uint32_t>(); }
+inline uint8_t* u8p(const DevBuf& b) { return b.as<uint8_t>(); }
 inline unsigned long long* u64p(const DevBuf& b) { return b.as<unsigned long long>(); }
 
 GBufView view(rsh_engine* e, DevBuf* g) { return GBufView{f4p(g[0]), f4p(g[1]), f4p(g[2]), f4p(g[3])}; }
@@ -638,7 +639,8 @@ rsh_status trace_pass(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, const
         StageTimer t(L, ST_TRIM, timing);
         const unsigned long long miss = cfg.hit_mode == 0 ? MISS64 : (unsigned long long)MISS32;
         const bool poll = !e->replaying && S > 0;
-        LANE_OK(run_pack(u32p(L.flags), u32p(L.keys), u32p(L.values), S, u32p(L.tvalues), u32p(L.bases),
+        // slot flags are bytes; a live slot's value is its index (creation writes no values)
+        LANE_OK(run_pack(u8p(L.flags), u32p(L.keys), nullptr, S, u32p(L.tvalues), u32p(L.bases),
                         u32p(L.ckeys), u32p(L.cvalues), dc, kind == 0 ? u64p(closest) : nullptr, miss,
                         kind == 1 ? u32p(e->shadow) : nullptr, L.sc, s, poll ? L.hsize_dev : nullptr,
                         poll ? ++L.hseq : 0u));
@@ -737,7 +739,7 @@ rsh_status trace_pass(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, const
         };
         if (k.have_keys) {
             LANE_OK(keep(k.keys, L.keys.p, (size_t)S * 4));
-            LANE_OK(keep(k.flags, L.flags.p, (size_t)S * 4));
+            LANE_OK(keep(k.flags, L.flags.p, (size_t)S));        // one byte per slot
         }
         if (k.have_chunks) {
             LANE_OK(keep(k.bases, L.bases.p, (size_t)out.C * 4));
@@ -1149,7 +1151,7 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
             StageTimer t(B, ST_CREATE, timing);
             // no slot-order rays: the gather recomputes each live ray
             launch_create_shadow(G.pos, G.nrm, W, H, Lm, f4p(e->light_pos), kc, V, cfg.soft_shadows, nullptr,
-                                 u32p(B.flags), u32p(B.keys), u32p(B.values), sb);
+                                 u8p(B.flags), u32p(B.keys), nullptr, sb);
         }
         PassOut po;
         rsh_status r = trace_pass(e, B, cfg, nullptr, Ssh, 1, G.pos, fl, po, idx_shadow(k), e->closest[0], nullptr,
@@ -1250,7 +1252,7 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
             if (presort)
                 launch_create_primary_sorted(V, P, f4p(A.rec0), f4p(A.rec1), sa);
             else
-                launch_create_primary(V, P, kc, rays_b, u32p(A.flags), u32p(A.keys), u32p(A.values), sa);
+                launch_create_primary(V, P, kc, rays_b, u8p(A.flags), u32p(A.keys), u32p(A.values), sa);
         }
         PassOut po;
         rsh_status r = trace_pass(e, A, cfg, rays_b, P, 0, nullptr, fl, po, 0, e->closest[0], nullptr, true,
@@ -1272,11 +1274,11 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
                 StageTimer t(A, ST_CREATE, timing);
                 if (b == 1)
                     launch_create_reflection(G0.pos, G0.nrm, G0.dif, W, H, Lm, V, cfg.reflect_gate, 1, kc, rays_a,
-                                             u32p(A.flags), u32p(A.keys), u32p(A.values), sa);
+                                             u8p(A.flags), u32p(A.keys), nullptr, sa);
                 else
                     launch_create_bounce(GB[(b - 1) % 2], W, H, Lm, u64p(e->closest[(b - 1) % 2]), cfg.hit_mode,
-                                         cfg.reflect_gate, kc, V, rays_a, u32p(A.flags), u32p(A.keys),
-                                         u32p(A.values), sa);
+                                         cfg.reflect_gate, kc, V, rays_a, u8p(A.flags), u32p(A.keys), nullptr,
+                                         sa);
             }
             PassOut pr;
             if ((r = trace_pass(e, A, cfg, rays_a, S, 0, nullptr, fl, pr, idx_refl(b), cur, nullptr, false,
@@ -1567,8 +1569,8 @@ rsh_status rsh_engine_trace(rsh_engine* e, const float* rays, const float* tmax,
     const bool timing = fl & 4u;
     {
         StageTimer t(L, ST_CREATE, timing);
-        launch_query_keys(rays, segments, n, key_cfg(cfg), kind == 1 ? 1u : 0u, u32p(L.flags), u32p(L.keys),
-                          u32p(L.values), s);
+        launch_query_keys(rays, segments, n, key_cfg(cfg), kind == 1 ? 1u : 0u, u8p(L.flags), u32p(L.keys), nullptr,
+                          s);
         HIP_OK(hipGetLastError());
     }
     PassOut po;
@@ -1642,7 +1644,11 @@ rsh_status rsh_engine_read_pass_order(rsh_engine* e, uint32_t pass, uint32_t* ke
     if ((order && rays > k.o_rays) || (nodes && node_count > k.o_nodes)) return RSH_INVALID_ARG;
     HIP_OK(hipStreamSynchronize(e->stream));
     if (keys) HIP_OK(hipMemcpy(keys, k.keys.p, slots * 4, hipMemcpyDeviceToHost));
-    if (flags) HIP_OK(hipMemcpy(flags, k.flags.p, slots * 4, hipMemcpyDeviceToHost));
+    if (flags) {       // kept as one byte per slot
+        std::vector<uint8_t> fb(slots);
+        HIP_OK(hipMemcpy(fb.data(), k.flags.p, slots, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < slots; i++) flags[i] = fb[i];
+    }
     if (bases) HIP_OK(hipMemcpy(bases, k.bases.p, chunks * 4, hipMemcpyDeviceToHost));
     if (ckeys) HIP_OK(hipMemcpy(ckeys, k.ckeys.p, chunks * 4, hipMemcpyDeviceToHost));
     if (nodes) HIP_OK(hipMemcpy(nodes, k.nodes.p, node_count * 16, hipMemcpyDeviceToHost));

@@ -75,7 +75,11 @@ __device__ __forceinline__ v3 primary_dir(const ViewDesc& vd, uint32_t x, uint32
     return normalize((D + R * a) + U * c);
 }
 
-__global__ void __launch_bounds__(256) k_create_primary(Views V, uint32_t P, KeyCfg kc, float* rays, uint32_t* flags,
+// Flags: F = uint8_t for the engine's own slot flags (one byte per slot),
+// uint32_t for the stage API's buffers; values == nullptr: not written (the
+// engine's trim takes each live slot's index as its value).
+template <typename F>
+__global__ void __launch_bounds__(256) k_create_primary(Views V, uint32_t P, KeyCfg kc, float* rays, F* flags,
                                                         uint32_t* keys, uint32_t* values) {
     __shared__ uint32_t s_off[VIEW_LDS];
     const bool lds = stage_views(V, s_off);
@@ -86,7 +90,7 @@ __global__ void __launch_bounds__(256) k_create_primary(Views V, uint32_t P, Key
         const uint32_t x = q % vd.w, y = q / vd.w;
         store_ray(rays, p, mk(vd.eye[0], vd.eye[1], vd.eye[2]), primary_dir(vd, x, y));
         keys[p] = primary_key(x, y, vd.w, v, kc.h.segment_shift);
-        values[p] = p;
+        if (values) values[p] = p;
         flags[p] = 0;
     }
 }
@@ -129,22 +133,23 @@ __global__ void __launch_bounds__(256) k_create_primary_sorted(Views V, uint32_t
 }
 
 // Ray queries: the caller's rays, keyed like frame rays of the given type.
+template <typename F>
 __global__ void __launch_bounds__(256) k_query_keys(const float* rays, const uint32_t* segs, uint32_t n, KeyCfg kc,
-                                                    uint32_t type, uint32_t* flags, uint32_t* keys,
-                                                    uint32_t* values) {
+                                                    uint32_t type, F* flags, uint32_t* keys, uint32_t* values) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         v3 o, d;
         load_ray(rays, i, o, d);
         keys[i] = make_key(kc, o, d, type, 0, segs ? segs[i] : 0u);
-        values[i] = i;
+        if (values) values[i] = i;
         flags[i] = 0;
     }
 }
 
 // RayTracer.cu:682-740; slot i = x + W*(l + Lmax*y).
+template <typename F>
 __global__ void __launch_bounds__(256) k_create_shadow(const float4* gpos, const float4* gnrm, uint32_t W, uint32_t H,
                                                        uint32_t Lmax, const float4* light_pos, KeyCfg kc, Views V,
-                                                       uint32_t soft, float* rays, uint32_t* flags, uint32_t* keys,
+                                                       uint32_t soft, float* rays, F* flags, uint32_t* keys,
                                                        uint32_t* values, FastDiv fw, FastDiv fl) {
     // soft: 16 rays per slot, ray 16 i + c from grid point c about the light
     // (RayTracer.cu:1550-1603) towards the slot's surface point
@@ -169,21 +174,21 @@ __global__ void __launch_bounds__(256) k_create_shadow(const float4* gpos, const
                 shadow_ray(P, Lp, soft != 0u, r & 15u, o, d);
                 if (rays) store_ray(rays, r, o, d);     // null: the gather recomputes them
                 keys[r] = make_key(kc, o, d, 1, l, seg_of(kc, V, s_off, lds, (uint32_t)px));
-                values[r] = r;
+                if (values) values[r] = r;
                 flag = 0;
             }
         }
-        flags[r] = flag;
+        flags[r] = (F)flag;
     }
 }
 
 // RayTracer.cu:742-791; slot x + y*W*Lmax.  gate: only diffuse.w > 0 reflects.
 // mark_gaps: also flag the unused slots between rows when Lmax > 1.
+template <typename F>
 __global__ void __launch_bounds__(256) k_create_reflection(const float4* gpos, const float4* gnrm, const float4* gdif,
                                                            uint32_t W, uint32_t H, uint32_t Lmax, Views V,
                                                            uint32_t gate, uint32_t mark_gaps, KeyCfg kc, float* rays,
-                                                           uint32_t* flags, uint32_t* keys, uint32_t* values,
-                                                           FastDiv fw) {
+                                                           F* flags, uint32_t* keys, uint32_t* values, FastDiv fw) {
     const uint32_t n = W * H;
     __shared__ uint32_t s_off[VIEW_LDS];
     const bool lds = stage_views(V, s_off);
@@ -204,7 +209,7 @@ __global__ void __launch_bounds__(256) k_create_reflection(const float4* gpos, c
             v3 d = reflect(normalize(P - view_eye(V, v)), normalize(N));
             store_ray(rays, slot, P, d);
             keys[slot] = make_key(kc, P, d, 2, 0, v);
-            values[slot] = slot;
+            if (values) values[slot] = slot;
             flags[slot] = 0;
         } else {
             flags[slot] = 1;
@@ -243,10 +248,11 @@ __global__ void __launch_bounds__(256) k_create_refraction(const float4* gpos, c
 
 // Next bounce (RayTracer.cu:1995-2010): origin = hit point, direction =
 // reflect(previous direction, barycentric normal), in place.
+template <typename F>
 __global__ void __launch_bounds__(256) k_create_bounce(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax,
                                                        const unsigned long long* rec, uint32_t hit_mode,
-                                                       uint32_t gate, KeyCfg kc, Views V, float* rays,
-                                                       uint32_t* flags, uint32_t* keys, uint32_t* values, FastDiv fw) {
+                                                       uint32_t gate, KeyCfg kc, Views V, float* rays, F* flags,
+                                                       uint32_t* keys, uint32_t* values, FastDiv fw) {
     const uint32_t n = W * H;
     const unsigned long long miss = hit_mode == 0 ? MISS64 : (unsigned long long)MISS32;
     __shared__ uint32_t s_off[VIEW_LDS];
@@ -262,7 +268,7 @@ __global__ void __launch_bounds__(256) k_create_bounce(GBufView g, uint32_t W, u
             v3 nd = reflect(d, N);
             store_ray(rays, slot, P, nd);
             keys[slot] = make_key(kc, P, nd, 2, 0, seg_of(kc, V, s_off, lds, p));
-            values[slot] = slot;
+            if (values) values[slot] = slot;
             flags[slot] = 0;
         } else {
             flags[slot] = 1;
@@ -543,26 +549,30 @@ __global__ void __launch_bounds__(256) k_color_reflection_ref(const float4* tp, 
 
 }  // namespace
 
-void launch_query_keys(const float* rays, const uint32_t* segs, uint32_t n, KeyCfg kc, uint32_t type,
-                       uint32_t* flags, uint32_t* keys, uint32_t* values, hipStream_t s) {
+template <typename F>
+void launch_query_keys(const float* rays, const uint32_t* segs, uint32_t n, KeyCfg kc, uint32_t type, F* flags,
+                       uint32_t* keys, uint32_t* values, hipStream_t s) {
     if (n) k_query_keys<<<grid_for(n, 256, 65536), 256, 0, s>>>(rays, segs, n, kc, type, flags, keys, values);
 }
-void launch_create_primary(const Views& V, uint32_t P, KeyCfg kc, float* rays, uint32_t* flags, uint32_t* keys,
+template <typename F>
+void launch_create_primary(const Views& V, uint32_t P, KeyCfg kc, float* rays, F* flags, uint32_t* keys,
                            uint32_t* values, hipStream_t s) {
     k_create_primary<<<grid_for(P, 256, 65536), 256, 0, s>>>(V, P, kc, rays, flags, keys, values);
 }
 void launch_create_primary_sorted(const Views& V, uint32_t P, float4* rec0, float4* rec1, hipStream_t s) {
     k_create_primary_sorted<<<grid_for(P, 256, 65536), 256, 0, s>>>(V, P, rec0, rec1);
 }
+template <typename F>
 void launch_create_shadow(const float4* gpos, const float4* gnrm, uint32_t W, uint32_t H, uint32_t Lmax,
-                          const float4* light_pos, KeyCfg kc, const Views& V, uint32_t soft, float* rays,
-                          uint32_t* flags, uint32_t* keys, uint32_t* values, hipStream_t s) {
+                          const float4* light_pos, KeyCfg kc, const Views& V, uint32_t soft, float* rays, F* flags,
+                          uint32_t* keys, uint32_t* values, hipStream_t s) {
     k_create_shadow<<<grid_for((uint64_t)W * H * Lmax << (soft ? 4 : 0), 256, 65536), 256, 0, s>>>(
         gpos, gnrm, W, H, Lmax, light_pos, kc, V, soft, rays, flags, keys, values, fast_div(W), fast_div(Lmax));
 }
+template <typename F>
 void launch_create_reflection(const float4* gpos, const float4* gnrm, const float4* gdif, uint32_t W, uint32_t H,
                               uint32_t Lmax, const Views& V, uint32_t gate, uint32_t mark_gaps, KeyCfg kc,
-                              float* rays, uint32_t* flags, uint32_t* keys, uint32_t* values, hipStream_t s) {
+                              float* rays, F* flags, uint32_t* keys, uint32_t* values, hipStream_t s) {
     k_create_reflection<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(gpos, gnrm, gdif, W, H, Lmax, V, gate,
                                                                                mark_gaps, kc, rays, flags, keys, values,
                                                                                fast_div(W));
@@ -573,12 +583,31 @@ void launch_create_refraction(const float4* gpos, const float4* gnrm, uint32_t W
     k_create_refraction<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(gpos, gnrm, W, H, Lmax, V, kc, rays,
                                                                                flags, keys, values);
 }
+template <typename F>
 void launch_create_bounce(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, const unsigned long long* rec,
-                          uint32_t hit_mode, uint32_t gate, KeyCfg kc, const Views& V, float* rays, uint32_t* flags,
+                          uint32_t hit_mode, uint32_t gate, KeyCfg kc, const Views& V, float* rays, F* flags,
                           uint32_t* keys, uint32_t* values, hipStream_t s) {
     k_create_bounce<<<grid_for((uint64_t)W * H, 256, 65536), 256, 0, s>>>(g, W, H, Lmax, rec, hit_mode, gate, kc, V,
                                                                            rays, flags, keys, values, fast_div(W));
 }
+// the engine's byte flags and the stage API's 32-bit ones
+#define RSH_FLAG_LAUNCHERS(F)                                                                                        \
+    template void launch_query_keys<F>(const float*, const uint32_t*, uint32_t, KeyCfg, uint32_t, F*, uint32_t*,      \
+                                       uint32_t*, hipStream_t);                                                      \
+    template void launch_create_primary<F>(const Views&, uint32_t, KeyCfg, float*, F*, uint32_t*, uint32_t*,         \
+                                           hipStream_t);                                                             \
+    template void launch_create_shadow<F>(const float4*, const float4*, uint32_t, uint32_t, uint32_t, const float4*, \
+                                          KeyCfg, const Views&, uint32_t, float*, F*, uint32_t*, uint32_t*,          \
+                                          hipStream_t);                                                              \
+    template void launch_create_reflection<F>(const float4*, const float4*, const float4*, uint32_t, uint32_t,       \
+                                              uint32_t, const Views&, uint32_t, uint32_t, KeyCfg, float*, F*,        \
+                                              uint32_t*, uint32_t*, hipStream_t);                                    \
+    template void launch_create_bounce<F>(GBufView, uint32_t, uint32_t, uint32_t, const unsigned long long*,         \
+                                          uint32_t, uint32_t, KeyCfg, const Views&, float*, F*, uint32_t*,           \
+                                          uint32_t*, hipStream_t);
+RSH_FLAG_LAUNCHERS(uint8_t)
+RSH_FLAG_LAUNCHERS(uint32_t)
+#undef RSH_FLAG_LAUNCHERS
 void launch_surface(const float* rays, const unsigned long long* rec, uint32_t hit_mode, uint32_t W, uint32_t H,
                     uint32_t slot_stride, uint32_t normalize_n, SceneView sc, GBufView g, hipStream_t s,
                     const Views* V) {

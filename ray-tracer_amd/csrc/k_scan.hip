@@ -315,6 +315,19 @@ __device__ __forceinline__ void load16_live(const uint32_t* p, uint64_t base, ui
     }
 }
 
+// The 16 byte flags from base, one uint4 load (1: dead past n).
+__device__ __forceinline__ void load16_u8(const uint8_t* p, uint64_t base, uint64_t n, uint32_t v[16]) {
+    if (base + 16 <= n) {
+        const uint4 a = *reinterpret_cast<const uint4*>(p + base);
+        const uint32_t w[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+        for (int i = 0; i < 16; i++) v[i] = (w[i >> 2] >> (8 * (i & 3))) & 0xFFu;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 16; i++) v[i] = base + i < n ? (uint32_t)p[base + i] : 1u;
+    }
+}
+
 // Exclusive scan of n <= SMALL_SCAN values in one workgroup of 1024 threads:
 // wave w scans a contiguous run of `wper` values (a multiple of 256), 256 per
 // step with lane l holding 4 consecutive ones, so every access is coalesced
@@ -411,14 +424,14 @@ __device__ __forceinline__ PackThread pack_thread(const uint32_t f[16], const ui
 }
 
 // meta[tile] = (live count, heads after the tile's first live key, first live key, last live key)
-__global__ void __launch_bounds__(256) k_pack_tiles(const uint32_t* flags, const uint32_t* keys, uint64_t n,
+__global__ void __launch_bounds__(256) k_pack_tiles(const uint8_t* flags, const uint32_t* keys, uint64_t n,
                                                     uint4* meta) {
     __shared__ LastKey lk[WAVES];
     __shared__ uint32_t red[WAVES];
     __shared__ uint32_t s_first;
     const uint64_t base = (uint64_t)blockIdx.x * TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
     uint32_t f[16], k[16];
-    load16(flags, base, n, f, 1u);
+    load16_u8(flags, base, n, f);
     load16_live(keys, base, n, f, k);
     const PackThread t = pack_thread(f, k, base, n);
     LastKey blk;
@@ -485,7 +498,7 @@ __global__ void __launch_bounds__(1024) k_pack_scan(uint4* meta, uint32_t ntiles
     }
 }
 
-__global__ void __launch_bounds__(256) k_pack_write(const uint32_t* flags, const uint32_t* keys,
+__global__ void __launch_bounds__(256) k_pack_write(const uint8_t* flags, const uint32_t* keys,
                                                     const uint32_t* values, uint64_t n, const uint4* meta,
                                                     uint32_t* tvalues, uint32_t* bases, uint32_t* ckeys,
                                                     uint32_t* cvalues, unsigned long long* clear64,
@@ -497,9 +510,14 @@ __global__ void __launch_bounds__(256) k_pack_write(const uint32_t* flags, const
     const uint64_t base = (uint64_t)blockIdx.x * TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
     const uint4 m = meta[blockIdx.x];
     uint32_t f[16], k[16], v[16];
-    load16(flags, base, n, f, 1u);
+    load16_u8(flags, base, n, f);
     load16_live(keys, base, n, f, k);
-    load16_live(values, base, n, f, v);
+    if (values) {
+        load16_live(values, base, n, f, v);
+    } else {       // a live slot's value is its index
+#pragma unroll
+        for (int i = 0; i < 16; i++) v[i] = (uint32_t)(base + i);
+    }
     // the pass's per-slot results start as misses (closest) or clear (shadow)
     if (clear64) {
         if (base + 16 <= n) {
@@ -1001,7 +1019,7 @@ hipError_t run_compress(const uint32_t* keys, uint64_t n, uint32_t* bases, uint3
     return hipGetLastError();
 }
 
-hipError_t run_pack(const uint32_t* flags, const uint32_t* keys, const uint32_t* values, uint64_t n,
+hipError_t run_pack(const uint8_t* flags, const uint32_t* keys, const uint32_t* values, uint64_t n,
                     uint32_t* tvalues, uint32_t* bases, uint32_t* ckeys, uint32_t* cvalues, uint32_t* d_totals,
                     unsigned long long* clear64, unsigned long long miss, uint32_t* clear32, ScanScratch& sc,
                     hipStream_t s, uint32_t* host, uint32_t seq) {

@@ -684,6 +684,9 @@ __global__ void __launch_bounds__(256) k_cull_items(TravParams P, uint4* obj_ent
 #ifndef RSH_TRAV_TPRUNE
 #define RSH_TRAV_TPRUNE 1
 #endif
+#ifndef RSH_TRAV_SUB_MIN
+#define RSH_TRAV_SUB_MIN 64
+#endif
 #ifndef RSH_TRAV_NPRUNE
 #define RSH_TRAV_NPRUNE 1
 #endif
@@ -698,7 +701,7 @@ __attribute__((amdgpu_waves_per_eu(MODE >= 1 ? RSH_TRAV_WAVES_PRE : RSH_TRAV_WAV
     constexpr uint32_t QCAP = 128;                // survivor queue
     constexpr uint32_t PCAP = 512;                // (survivor, leaf) candidate / pair list
     constexpr uint32_t SCH = PCAP / BRANCH < 64 ? PCAP / BRANCH : 64;   // survivors per B+C round
-    constexpr uint32_t SUB_MIN = 64;              // RM: survivors x rays that earn a sub-cluster mask
+    constexpr uint32_t SUB_MIN = RSH_TRAV_SUB_MIN;   // RM: survivors x rays that earn a sub-cluster mask
     __shared__ float4 s_qsph[QCAP];
     __shared__ uint32_t s_qidx[QCAP];
     __shared__ uint32_t s_qmask[QCAP];         // leaves the survivor's cluster may pass

@@ -97,15 +97,16 @@ hipError_t run_exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, Sca
 hipError_t run_compress(const uint32_t* keys, uint64_t n, uint32_t* bases, uint32_t* sizes,
                         uint32_t* ckeys, uint32_t* cvalues, uint32_t* d_total, ScanScratch& sc,
                         hipStream_t s, const uint32_t* d_n = nullptr);
-// Trimming and compression in one walk (the engine's path): the live slots'
-// values in order -> tvalues, one chunk per run of equal live keys -> bases
+// Trimming and compression in one walk (the engine's path): flags one byte
+// per slot (0: live); the live slots' values (values == nullptr: the slot
+// index) in order -> tvalues, one chunk per run of equal live keys -> bases
 // (first live position; bases[C] = N), ckeys, cvalues (chunk index).
 // d_totals[0] = N, d_totals[1] = C.  bases holds n + 1 entries.
 // clear64 (if set) receives `miss` in every slot, clear32 (if set) 0: the pass's
 // per-slot results start cleared.  host (if set, device-visible coherent
 // host memory) receives N and C in host[1], host[2] and then `seq` in host[0]
 // as soon as they are known, ahead of the chunk writes.
-hipError_t run_pack(const uint32_t* flags, const uint32_t* keys, const uint32_t* values, uint64_t n,
+hipError_t run_pack(const uint8_t* flags, const uint32_t* keys, const uint32_t* values, uint64_t n,
                     uint32_t* tvalues, uint32_t* bases, uint32_t* ckeys, uint32_t* cvalues, uint32_t* d_totals,
                     unsigned long long* clear64, unsigned long long miss, uint32_t* clear32, ScanScratch& sc,
                     hipStream_t s, uint32_t* host = nullptr, uint32_t seq = 0);

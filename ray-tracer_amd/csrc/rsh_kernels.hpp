@@ -217,20 +217,27 @@ void launch_cull_items(const TravParams& p, uint4* obj_ent, uint32_t* obj_cum, c
 void launch_item_desc(const TravParams& p, uint4* desc, hipStream_t s);
 
 // --- rays ---
-void launch_create_primary(const Views& V, uint32_t P, KeyCfg kc, float* rays, uint32_t* flags, uint32_t* keys,
+// Ray creation: F = uint8_t (the engine's slot flags, one byte per slot) or
+// uint32_t (the stage API's); values == nullptr leaves the values unwritten
+// (the engine's trim, run_pack, takes each live slot's index as its value).
+template <typename F>
+void launch_create_primary(const Views& V, uint32_t P, KeyCfg kc, float* rays, F* flags, uint32_t* keys,
                            uint32_t* values, hipStream_t s);
+template <typename F>
 void launch_create_shadow(const float4* gpos, const float4* gnrm, uint32_t W, uint32_t H, uint32_t Lmax,
-                          const float4* light_pos, KeyCfg kc, const Views& V, uint32_t soft, float* rays,
-                          uint32_t* flags, uint32_t* keys, uint32_t* values, hipStream_t s);
+                          const float4* light_pos, KeyCfg kc, const Views& V, uint32_t soft, float* rays, F* flags,
+                          uint32_t* keys, uint32_t* values, hipStream_t s);
+template <typename F>
 void launch_create_reflection(const float4* gpos, const float4* gnrm, const float4* gdif, uint32_t W, uint32_t H,
                               uint32_t Lmax, const Views& V, uint32_t gate, uint32_t mark_gaps, KeyCfg kc,
-                              float* rays, uint32_t* flags, uint32_t* keys, uint32_t* values, hipStream_t s);
+                              float* rays, F* flags, uint32_t* keys, uint32_t* values, hipStream_t s);
 void launch_create_primary_sorted(const Views& V, uint32_t P, float4* rec0, float4* rec1, hipStream_t s);
 void launch_create_refraction(const float4* gpos, const float4* gnrm, uint32_t W, uint32_t H, uint32_t Lmax,
                               const Views& V, KeyCfg kc, float* rays, uint32_t* flags, uint32_t* keys,
                               uint32_t* values, hipStream_t s);
+template <typename F>
 void launch_create_bounce(GBufView g, uint32_t W, uint32_t H, uint32_t Lmax, const unsigned long long* rec,
-                          uint32_t hit_mode, uint32_t gate, KeyCfg kc, const Views& V, float* rays, uint32_t* flags,
+                          uint32_t hit_mode, uint32_t gate, KeyCfg kc, const Views& V, float* rays, F* flags,
                           uint32_t* keys, uint32_t* values, hipStream_t s);
 // rays == nullptr: the camera pass of views V, whose rays are recomputed
 // from the views (k_create_primary_sorted writes no slot-order rays).
@@ -266,8 +273,9 @@ void launch_gather(const float* rays, const uint32_t* sorted_values, uint32_t N,
                    hipStream_t s, const float4* light_pos = nullptr);
 // Ray queries: key, value (= index) and live flag of n caller rays (float[6n]);
 // segs (optional) places a per-ray segment id at kc.h.segment_shift.
-void launch_query_keys(const float* rays, const uint32_t* segs, uint32_t n, KeyCfg kc, uint32_t type,
-                       uint32_t* flags, uint32_t* keys, uint32_t* values, hipStream_t s);
+template <typename F>
+void launch_query_keys(const float* rays, const uint32_t* segs, uint32_t n, KeyCfg kc, uint32_t type, F* flags,
+                       uint32_t* keys, uint32_t* values, hipStream_t s);
 void launch_hierarchy(const float4* rec0, const float4* rec1, const PassSizes* ps, const float* rays,
                       const uint32_t* sorted_values,
                       uint32_t N, uint32_t leaf, uint32_t branch, uint32_t depth, float r0, float s0, uint32_t mode,

@@ -19,6 +19,8 @@
 // the same node merge closest hits with a 64-bit atomicMin.  The candidate
 // set per (leaf, triangle) is the reference's; only the order of tests
 // differs, and both any-hit and closest-hit are order independent.
+#include <type_traits>
+
 #include "rsh_internal.hpp"
 #include "rsh_kernels.hpp"
 
@@ -655,7 +657,10 @@ __global__ void __launch_bounds__(256) k_cull_items(TravParams P, uint4* obj_ent
 // sphere and lists only those (pair, ray) candidates.  MODE 2 pays for
 // spread-out secondary rays (2x on the atrium's bounces) and costs on coherent
 // ones; the engine picks the mode per pass.  Modes 1 and 2 use under 102 VGPRs
-// and 7.6 / 8.9 KB of LDS per wave (five waves per SIMD); mode 0, four.
+// and 7.0 / 6.5 KB of LDS per wave, so five waves per SIMD fit (20 per CU of
+// 160 KB; mode 2's 8.9 KB with a 128-entry survivor queue allowed 17:
+// 1080p atrium frame 5.63 -> 5.40 ms, 64-view batch 59.6 -> 57.4 ms when it
+// went to 64 entries); mode 0, four.
 // Occupancy targets (waves per SIMD) of the pre-test modes and of mode 0,
 // the blocks of triangle spheres loaded back to back in stage A, and the two
 // distance prunings (RSH_TRAV_TPRUNE: each ray's pre-test also bounds the
@@ -698,13 +703,20 @@ __attribute__((amdgpu_waves_per_eu(MODE >= 1 ? RSH_TRAV_WAVES_PRE : RSH_TRAV_WAV
     constexpr uint32_t BRANCH = 64 / LEAF;
     constexpr uint32_t TQ = 128;                  // rings of (survivor, ray) tests
     constexpr unsigned long long LRAYS = LEAF >= 64 ? ~0ull : (1ull << LEAF) - 1ull;   // one leaf's rays
-    constexpr uint32_t QCAP = 128;                // survivor queue
+    // Survivor queue.  A block adds up to 64 survivors; mode 2 keeps 64
+    // entries (flushed when full, the block's remaining survivors queued
+    // after the flush) so that its LDS stays under 8 KB per wave and 20 waves
+    // fit a CU; the other modes keep 128 and flush once 64 are queued.
+    constexpr uint32_t QCAP = RM ? 64 : 128;
+    // leaf masks, one bit per leaf of the top node
+    using LMask = typename std::conditional<BRANCH <= 8, uint8_t,
+                  typename std::conditional<BRANCH <= 16, uint16_t, uint32_t>::type>::type;
     constexpr uint32_t PCAP = 512;                // (survivor, leaf) candidate / pair list
     constexpr uint32_t SCH = PCAP / BRANCH < 64 ? PCAP / BRANCH : 64;   // survivors per B+C round
     constexpr uint32_t SUB_MIN = RSH_TRAV_SUB_MIN;   // RM: survivors x rays that earn a sub-cluster mask
     __shared__ float4 s_qsph[QCAP];
     __shared__ uint32_t s_qidx[QCAP];
-    __shared__ uint32_t s_qmask[QCAP];         // leaves the survivor's cluster may pass
+    __shared__ LMask s_qmask[QCAP];            // leaves the survivor's cluster may pass
     __shared__ float4 s_mt[PRE ? 1 : 64][3];      // Moller-Trumbore vectors of a round's survivors
     __shared__ float4 s_ray0[64];              // (o + d*eps, tmax)
     __shared__ float4 s_ray1[64];              // (d, valid)
@@ -718,7 +730,7 @@ __attribute__((amdgpu_waves_per_eu(MODE >= 1 ? RSH_TRAV_WAVES_PRE : RSH_TRAV_WAV
     __shared__ uint16_t s_tq2[RM ? TQ : 1];       // RM: candidates that passed the ray pre-test
     __shared__ unsigned long long s_qrm[RM ? QCAP : 1];   // RM: rays that may meet the survivor's cluster
     __shared__ uint8_t s_grp[64];                 // passing clusters of a group, in order
-    __shared__ uint32_t s_gm[64];                 // leaf mask of each cluster of a group
+    __shared__ LMask s_gm[64];                    // leaf mask of each cluster of a group
     apply_sizes(P);
     const uint32_t lane = lane_id();
     const uint32_t item = blockIdx.x;
@@ -1063,7 +1075,7 @@ __attribute__((amdgpu_waves_per_eu(MODE >= 1 ? RSH_TRAV_WAVES_PRE : RSH_TRAV_WAV
             if (ci < g && leaf_ok)
                 lp = cluster_may_pass(ls, lc, la.y, make_float2(la.z, la.w), P.cl_sphere[e_cbase + g0 + s_grp[ci]]);
             const uint64_t bm = __ballot(lp);
-            if (lane < CPS && j0 + lane < g) s_gm[s_grp[j0 + lane]] = (uint32_t)(bm >> (lane * BRANCH)) & LMASK;
+            if (lane < CPS && j0 + lane < g) s_gm[s_grp[j0 + lane]] = (LMask)((bm >> (lane * BRANCH)) & LMASK);
         }
         __builtin_amdgcn_wave_barrier();
         return __ballot(((gm >> lane) & 1ull) && s_gm[lane] != 0u);
@@ -1179,19 +1191,33 @@ __attribute__((amdgpu_waves_per_eu(MODE >= 1 ? RSH_TRAV_WAVES_PRE : RSH_TRAV_WAV
                     }
                 }
             }
-            const uint32_t add = (uint32_t)__popcll(m);
-            if (qlen + add > QCAP) flush();   // never taken: qlen < 64 here
-            if (done) break;
-            if (p) {
-                const uint32_t r = qlen + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-                s_qsph[r] = tsk[k];
-                s_qidx[r] = tbk[k] + lane;
-                s_qmask[r] = cm;
-                if constexpr (RM) s_qrm[r] = rm;
+            const uint32_t tot = qlen + (uint32_t)__popcll(m);
+            const uint32_t r = qlen + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            auto enqueue = [&](uint32_t at) {
+                s_qsph[at] = tsk[k];
+                s_qidx[at] = tbk[k] + lane;
+                s_qmask[at] = (LMask)cm;
+                if constexpr (RM) s_qrm[at] = rm;
+            };
+            if constexpr (QCAP >= 128) {     // qlen < 64 here: the block fits
+                if (p) enqueue(r);
+                qlen = tot;
+                __builtin_amdgcn_wave_barrier();
+                if (qlen >= 64) flush();
+            } else {
+                if (p && r < QCAP) enqueue(r);
+                __builtin_amdgcn_wave_barrier();
+                if (tot < QCAP) {
+                    qlen = tot;
+                    continue;
+                }
+                qlen = QCAP;
+                flush();
+                if (done) break;
+                if (p && r >= QCAP) enqueue(r - QCAP);
+                qlen = tot - QCAP;
+                __builtin_amdgcn_wave_barrier();
             }
-            qlen += add;
-            __builtin_amdgcn_wave_barrier();
-            if (qlen >= 64) flush();
         }
     }
     if (qlen && !done) flush();

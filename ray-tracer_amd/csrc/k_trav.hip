@@ -674,17 +674,26 @@ __global__ void __launch_bounds__(256) k_cull_items(TravParams P, uint4* obj_ent
 // Pruning, same measures: neither 6.04 / 66.3; TPRUNE 5.90 / 65.1; with
 // NPRUNE in every pass 6.05 / 65.2 (closest-hit passes lose: their reach is
 // infinite until every ray of the item has a hit), in mode 1 shadow passes
-// only 5.78 / 62.3.  Seeding a split node's items with the records its other
+// only 5.78 / 62.3.  With mode 2 at 20 waves per CU (64-entry survivor
+// queue): one block per step in mode 2 5.35 / 56.8 against two 5.44 / 57.4
+// (mode 1 keeps two); six waves per SIMD for mode 2 5.79 / 60.4; a
+// sub-cluster threshold of 16 or 32 instead of 64 within 0.5 %.  Seeding a split node's items with the records its other
 // items had merged (5.99 / 65.5) and the exact mode for camera rays (6.14 /
 // 70.1) measured slower.
 #ifndef RSH_TRAV_WAVES_PRE
 #define RSH_TRAV_WAVES_PRE 5
+#endif
+#ifndef RSH_TRAV_WAVES_MASKS
+#define RSH_TRAV_WAVES_MASKS RSH_TRAV_WAVES_PRE
 #endif
 #ifndef RSH_TRAV_WAVES_EXACT
 #define RSH_TRAV_WAVES_EXACT 4
 #endif
 #ifndef RSH_TRAV_NB
 #define RSH_TRAV_NB 2
+#endif
+#ifndef RSH_TRAV_NB_MASKS
+#define RSH_TRAV_NB_MASKS 1
 #endif
 #ifndef RSH_TRAV_TPRUNE
 #define RSH_TRAV_TPRUNE 1
@@ -697,7 +706,8 @@ __global__ void __launch_bounds__(256) k_cull_items(TravParams P, uint4* obj_ent
 #endif
 template <int LEAF, int MODE>
 __global__ void __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(MODE >= 1 ? RSH_TRAV_WAVES_PRE : RSH_TRAV_WAVES_EXACT))) k_traverse64(TravParams P) {
+__attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 ? RSH_TRAV_WAVES_PRE
+                                                                   : RSH_TRAV_WAVES_EXACT))) k_traverse64(TravParams P) {
     constexpr bool PRE = MODE >= 1;               // ray / triangle-sphere pre-test
     constexpr bool RM = MODE == 2;                // and ray / cluster-sphere masks
     constexpr uint32_t BRANCH = 64 / LEAF;
@@ -1117,7 +1127,7 @@ __attribute__((amdgpu_waves_per_eu(MODE >= 1 ? RSH_TRAV_WAVES_PRE : RSH_TRAV_WAV
 
     // NB blocks per iteration: their sphere loads are issued back to back and
     // consumed in order, so only the first load's latency is exposed.
-    constexpr int NB = RSH_TRAV_NB;
+    constexpr int NB = RM ? RSH_TRAV_NB_MASKS : RSH_TRAV_NB;
     bool have = rem != 0 || lc_cur < lc_end;
     while (have && !done) {
         uint32_t tbk[NB], cmk[NB], cidk[NB];

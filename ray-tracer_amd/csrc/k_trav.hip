@@ -349,6 +349,11 @@ __device__ __forceinline__ float wave_max(float x) {
     return x;
 }
 
+// Object j of the walk order (obj_order, or index order), j < K.
+__device__ __forceinline__ uint32_t object_at(const TravParams& P, uint32_t j) {
+    return P.obj_order ? P.obj_order[j] : j;
+}
+
 // The counting pass stores each node's passing objects as 64-bit masks
 // (pmask, ceil(K / 64) words per node); the list pass reads them back instead
 // of repeating the sphere tests.
@@ -386,8 +391,8 @@ __global__ void __launch_bounds__(256) k_cull(TravParams P, uint32_t* npass, uin
         for (uint32_t ob = 0; ob < P.K; ob += 64) {
             const uint64_t m = pm[ob / 64];
             if (m == 0ull) continue;
-            const uint32_t oi = ob + lane;
             const bool po = (m >> lane) & 1ull;
+            const uint32_t oi = po ? object_at(P, ob + lane) : 0u;
             uint4 r = make_uint4(0u, 0u, 0u, 0u);
             uint32_t ncl = 0;
             if (po) {
@@ -408,9 +413,9 @@ __global__ void __launch_bounds__(256) k_cull(TravParams P, uint32_t* npass, uin
     // count pass: per-lane sums, one reduction at the end
     unsigned long long pairs = 0;
     for (uint32_t ob = 0; ob < P.K; ob += 64) {
-        const uint32_t oi = ob + lane;
+        const uint32_t oi = ob + lane < P.K ? object_at(P, ob + lane) : 0u;
         bool po = false;
-        if (oi < P.K) po = sphere_node_fast(tS, tC, P.obj_sphere[oi], tA.x, tA.y, rcos);
+        if (ob + lane < P.K) po = sphere_node_fast(tS, tC, P.obj_sphere[oi], tA.x, tA.y, rcos);
         if (po) {
             const uint4 r = P.obj_range[oi];
             cum += (r.y - r.x + CLUSTER) / CLUSTER;
@@ -582,8 +587,8 @@ __global__ void __launch_bounds__(256) k_cull_items(TravParams P, uint4* obj_ent
     for (uint32_t ob = 0; ob < P.K; ob += 64) {
         const uint64_t m = pm[ob / 64];
         if (m == 0ull) continue;
-        const uint32_t oi = ob + lane;
         const bool po = (m >> lane) & 1ull;
+        const uint32_t oi = po ? object_at(P, ob + lane) : 0u;
         uint4 r = make_uint4(0u, 0u, 0u, 0u);
         uint32_t ncl = 0;
         if (po) {
@@ -743,16 +748,23 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 
     __shared__ LMask s_gm[64];                    // leaf mask of each cluster of a group
     apply_sizes(P);
     const uint32_t lane = lane_id();
-    const uint32_t item = blockIdx.x;
+    uint32_t item = blockIdx.x;
     if (P.ps) {     // device-sized: the item total, bounded by the capacity the grid covers
         if (*P.overflow) return;       // a list or descriptor table was cut: this frame is rendered again
         const uint32_t it = P.item_first[P.level_count[1]];
         P.items = it < P.items_cap ? it : P.items_cap;
     }
+    if (P.phase == 1) {           // block = top node: its first item
+        if (item >= P.level_count[1]) return;
+        const uint32_t f = P.item_first[item];
+        if (f >= P.item_first[item + 1] || f >= P.items) return;
+        item = f;
+    }
     if (item >= P.items) return;
     const uint4 desc = P.item_desc[item];
     const uint32_t top = desc.x;
     if (top >= P.level_count[1]) return;
+    if (P.phase == 2 && item == P.item_first[top]) return;     // ran in phase 1
     const uint32_t pos = top * 64 + lane;
     const bool valid = pos < P.N;
 
@@ -770,7 +782,10 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 
         s_ray1[lane] = f4(d, valid ? 1.0f : 0.0f);
     }
     const unsigned long long miss = P.kind == 1 ? 0ull : (P.hit_mode == 0 ? MISS64 : (unsigned long long)MISS32);
-    s_best[lane] = miss;
+    // phase 2: a split node's item starts from what phase 1 merged
+    const unsigned long long seed =
+        P.phase == 2 && P.kind != 1 && valid && (desc.w >> 31) == 0u ? P.closest[slot] : miss;
+    s_best[lane] = seed;
     if (lane < BRANCH) {
         const uint32_t bl = top * BRANCH + lane;
         float4 ls = make_float4(0, 0, 0, 0), lc = make_float4(0, 0, 1, 0), la = make_float4(0, 0, 0, 0);
@@ -799,7 +814,7 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 
     // its length; for a closest-hit ray (exact 64-bit records), short of the
     // closest hit found so far.
     auto may_hit = [&](uint32_t ray, float4 r0, float4 r1, float4 sph) -> bool {
-        if constexpr (RSH_TRAV_TPRUNE != 0) {
+        if (RSH_TRAV_TPRUNE != 0 && !P.line_only) {
             float hi = r0.w;
             if (P.kind != 1)
                 hi = P.hit_mode == 0 ? __uint_as_float((uint32_t)(s_best[ray] >> 32)) : __int_as_float(0x7FC00000);
@@ -818,7 +833,7 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 
     // slack covers the FP32 rounding of this test and of ray_triangle's
     // parameter, ~1e-6 of these magnitudes).  reach is infinite while some
     // closest-hit ray has no hit, and is recomputed after each flush.
-    const bool prune = RSH_TRAV_NPRUNE != 0 && MODE == 1 && P.kind == 1 && !count;
+    const bool prune = RSH_TRAV_NPRUNE != 0 && !count && ((MODE == 1 && P.kind == 1) || P.phase != 0);
     float r_org = 0.0f, dlen = 0.0f, reach = __builtin_inff();
     if (prune) {
         const v3 o = xyz(s_ray0[lane]), d = xyz(s_ray1[lane]);
@@ -1238,7 +1253,7 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 
     if (valid) {
         if (P.kind == 1) { if (best != 0ull) P.shadow[slot] = MISS32; }
         else if (single) P.closest[slot] = best;
-        else if (best != miss) atomicMin(&P.closest[slot], best);
+        else if (best != seed) atomicMin(&P.closest[slot], best);
     }
     if (count) {
         unsigned long long tsum = n_tests, msum = n_mt;
@@ -1463,8 +1478,9 @@ hipError_t launch_traverse(const TravParams& p, hipStream_t s) {
         if (p.chunk == 0 || !p.item_desc) return hipErrorInvalidValue;
         if (!p.obj_ent || !p.obj_cum || (p.algorithm != 1 && !p.obj_off)) return hipErrorInvalidValue;
         if (p.items > MAX_TRAVERSE_ITEMS) return hipErrorInvalidValue;
-        // one wave per workgroup: a finished item releases its LDS at once
-        const uint32_t blocks = p.items;
+        // one wave per workgroup: a finished item releases its LDS at once;
+        // phase 1: one block per top node (its first item)
+        const uint32_t blocks = p.phase == 1 ? p.level_count[1] : p.items;
         switch (p.leaf) {
             case 2: return launch64<2>(p, blocks, s);
             case 4: return launch64<4>(p, blocks, s);

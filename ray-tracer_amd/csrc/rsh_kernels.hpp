@@ -169,6 +169,16 @@ struct TravParams {
     const PassSizes* ps;
     uint32_t list_cap, items_cap;
     uint32_t* overflow;
+    // Front-to-back camera passes (64-ray kernel, closest hit, exact
+    // records): obj_order lists the objects nearest the eye first, and the
+    // cull kernels walk them in that order, so each top node's cluster stream
+    // runs front to back.  phase 1 runs every top node's first item (grid =
+    // top nodes), phase 2 the others (grid = items): they start from the
+    // records phase 1 merged and skip clusters and triangles beyond the reach
+    // of all their rays.  phase 0: every item in one launch.
+    const uint32_t* obj_order;    // [K] or null (index order)
+    uint32_t phase;
+    uint32_t line_only;           // ray pre-tests test the line only, not the part a hit can use
 };
 
 // Work items of one launch of the 64-ray kernel: 64 lanes each, so the grid

@@ -317,20 +317,19 @@ def trav_summary(a):
             if "k_traverse64" in r["Kernel_Name"]:
                 launches.append((int(r["Start_Timestamp"]), (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6))
     launches.sort()
-    per = 8
-    frames = [launches[i:i + per] for i in range(0, len(launches), per)]
-    first = a.warmup + 2 * a.steps + 2
+    counts, first = _frame_layout(len(launches), a.steps, a.warmup)
+    frames = _split_frames(launches, counts)
     print()
     print("k_traverse64<8, MODE> per launch (ms), one row per frame, in start order")
     for i, fr in enumerate(frames):
         tag = "  <- serial, traversal timers" if first <= i < first + 3 else ""
         print("  " + " ".join(f"{d:6.2f}" for _, d in fr) + f"   sum {sum(d for _, d in fr):5.2f}{tag}")
-    serial = [d for fr in frames[first:first + 3] for _, d in fr]
+    serial = [sum(d for _, d in ps) for fr in frames[first:first + 3] for ps in _passes_of(fr)]
     print()
     print(f"k_traverse64 average launch: {sum(d for _, d in launches) / max(len(launches), 1):.3f} ms over "
           f"{len(launches)} launches")
-    print(f"k_traverse64 average launch over the 3 serial frames with traversal timers: "
-          f"{sum(serial) / max(len(serial), 1):.3f} ms ({len(serial)} launches)")
+    print(f"k_traverse64 time per traced pass over the 3 serial frames with traversal timers (the camera pass's "
+          f"two launches together): {sum(serial) / max(len(serial), 1):.3f} ms ({len(serial)} passes)")
 
 
 def _counter_rows(path, kernel=None):
@@ -350,15 +349,46 @@ def _counter_rows(path, kernel=None):
 
 
 PASS_NAMES = ["primary", "shadow1", "refl1", "shadow2", "refl2", "shadow3", "refl3", "shadow4"]
+FRAME_PASSES = len(PASS_NAMES)
 
 
-def _first_serial_frame(dispatches, steps=1, warmup=0, per=8):
+def _frame_layout(n_launches, steps, warmup):
+    """k_traverse64 launches per frame of a bench run (the --no-* legs off)
+    and the index of its first serial frame with traversal timers.  Frames,
+    in order: --warmup warm-up, --steps timed, one eager frame recorded for
+    the graph, 1 + --steps graph replays, 3 serial frames with traversal
+    timers, 3 with stage timers, 1 with counters.  The camera pass runs as
+    two launches when it goes front to back (every frame but the counting
+    one), so the other frames have 8 or 9 launches alike."""
+    frames = warmup + steps + 1 + (1 + steps) + 3 + 3 + 1
+    per = (n_launches - FRAME_PASSES) // (frames - 1)
+    return [per] * (frames - 1) + [FRAME_PASSES], warmup + 2 * steps + 2
+
+
+def _split_frames(items, counts):
+    out, i = [], 0
+    for c in counts:
+        out.append(items[i:i + c])
+        i += c
+    return out
+
+
+def _passes_of(frame):
+    """A frame's launches grouped by pass: a front-to-back camera pass's two
+    launches together, then one launch per pass."""
+    if len(frame) == FRAME_PASSES + 1:
+        return [frame[0:2]] + [[x] for x in frame[2:]]
+    return [[x] for x in frame]
+
+
+def _first_serial_frame(dispatches, steps=1, warmup=0):
     """The k_traverse64 dispatches of the first serial frame with traversal
-    timers of a bench run (frame order as in trav_summary), passes in order:
-    primary, then shadow and reflection passes alternating (in the timed
-    frames the two lanes' launches interleave differently)."""
-    first = warmup + 2 * steps + 2
-    return sorted(dispatches)[first * per:(first + 1) * per]
+    timers of a bench run (frame order as in trav_summary), grouped by pass
+    in pass order: primary, then shadow and reflection passes alternating
+    (in the timed frames the two lanes' launches interleave differently)."""
+    d = sorted(dispatches)
+    counts, first = _frame_layout(len(d), steps, warmup)
+    return _passes_of(_split_frames(d, counts)[first])
 
 
 def trav_traffic(a):
@@ -378,18 +408,19 @@ def trav_traffic(a):
     print()
     fd = _first_serial_frame(fv)
     wd = _first_serial_frame(wv)
-    print("per dispatch (KiB as counted), the first serial frame of the run (passes in order):")
+    print("per pass (KiB as counted; the camera pass's two launches summed), the first serial frame of the run:")
     print(f"{'pass':10s} {'FETCH_KiB':>12s} {'WRITE_KiB':>12s} {'scaled_MB':>12s}")
     tf = tw = 0.0
     for name, df, dw in zip(PASS_NAMES, fd, wd):
-        f_, w_ = fv[df]["FETCH_SIZE"], wv[dw]["WRITE_SIZE"]
+        f_ = sum(fv[x]["FETCH_SIZE"] for x in df)
+        w_ = sum(wv[x]["WRITE_SIZE"] for x in dw)
         tf += f_ * 1024 * fscale
         tw += w_ * 1024 * wscale
         print(f"{name:10s} {f_:12.0f} {w_:12.0f} {(f_ * fscale + w_ * wscale) * 1024 / 1e6:12.1f}")
     n = max(len(fd), 1)
     print()
-    print(f"mean per launch: fetch {tf / n:.4g} B, write {tw / n:.4g} B")
-    print(f"traffic_bytes_per_launch = {(tf + tw) / n:.6g}")
+    print(f"mean per pass: fetch {tf / n:.4g} B, write {tw / n:.4g} B")
+    print(f"traffic_bytes_per_launch = {(tf + tw) / n:.6g}   (per traced pass)")
 
 
 def trav_valu(a):
@@ -397,12 +428,16 @@ def trav_valu(a):
     d = _first_serial_frame(vals)
     keys = ["SQ_WAVES", "SQ_INSTS_VALU", "SQ_INSTS_SALU", "SQ_INSTS_LDS", "SQ_INSTS_VMEM", "SQ_WAVE_CYCLES",
             "SQ_BUSY_CYCLES", "SQ_ACTIVE_INST_VALU"]
-    print("Counters summed over the chip per dispatch (the first serial frame of the run, passes in order).")
+    print("Counters summed over the chip per pass (the first serial frame of the run, passes in order; the camera "
+          "pass's two launches summed).")
     print(f"{'pass':10s} {'waves':>10s} {'VALU_insts':>12s} {'SALU_insts':>12s} {'LDS_insts':>12s} "
           f"{'VMEM_insts':>12s} {'VALU/wave':>10s}")
     tot = 0.0
-    for name, di in zip(PASS_NAMES, d):
-        v = vals[di]
+    for name, ds in zip(PASS_NAMES, d):
+        v = {}
+        for di in ds:
+            for k_, x in vals[di].items():
+                v[k_] = v.get(k_, 0.0) + x
         w = max(v.get("SQ_WAVES", 0.0), 1.0)
         tot += v.get("SQ_INSTS_VALU", 0.0)
         print(f"{name:10s} {v.get('SQ_WAVES', 0):10.0f} {v.get('SQ_INSTS_VALU', 0):12.4g} "
@@ -866,15 +901,18 @@ def main():
     n_tests = sum(p["ray_tests"] for p in passes)
     n_top = sum(p["pairs_top"] for p in passes)
     n_surv = sum(p["top_survivors"] for p in passes)
-    launches_per_frame = sum(1 for p in passes if p["rays"])
+    passes_per_frame = sum(1 for p in passes if p["rays"])
     flops_frame = FLOP_RAY_TEST * n_tests + FLOP_NODE_TEST * (n_top + cfg.branch * n_surv)
     avg_launch_ms = trav_ms / trav_launches if trav_launches else 0.0
-    flops_launch = flops_frame / launches_per_frame if launches_per_frame else 0.0
+    flops_launch = flops_frame / passes_per_frame if passes_per_frame else 0.0
     achieved = flops_launch / (avg_launch_ms * 1e-3) / 1e12 if avg_launch_ms > 0 else 0.0
-    roof = {"bound": "valu", "kernel": "k_traverse64<8, MODE> (fused traversal + intersection, one launch per pass)",
+    roof = {"bound": "valu",
+            "kernel": "k_traverse64<8, MODE> (fused traversal + intersection; one launch per pass, two for the "
+                      "front-to-back camera pass)",
             "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic_per_launch(),
-            "avg_launch_ms": avg_launch_ms, "launches_per_frame": launches_per_frame,
+            "avg_launch_ms": avg_launch_ms, "per": "traced pass (the camera pass's two launches together)",
+            "passes_per_frame": passes_per_frame,
             "flops_per_launch": flops_launch}
     hsc = hsc_figure(passes, stages, cfg, "frame")
     hsc["workload"] = "the timed atrium frames (8 passes, %d rays per frame)" % int(statistics.median(rays))

@@ -132,7 +132,9 @@ struct rsh_engine {
     uint32_t W = 0, H = 0, Lmax = 1;
     size_t cap_slots = 0, cap_pixels = 0;   // slots (pixels x lights) and pixels the buffers hold
     size_t cap_shadow_slots = 0;            // shadow-pass flags `shadow` holds
-    DevBuf rays_a, rays_b, closest[2], shadow, pixels;
+    // shadow[k % 2]: the shadow flags of G-buffer k (a frame's shadow pass k
+    // may run while G-buffer k - 1 still waits for its shading)
+    DevBuf rays_a, rays_b, closest[2], shadow[2], pixels;
     DevBuf views_d;                    // view table of a batch (more than one view)
     uint64_t scene_epoch = 0;          // counts scene loads
     Lane lane[2];
@@ -360,7 +362,7 @@ rsh_status ensure_lanes(rsh_engine* e) {
         }
         if (!l.rb_ev) HIP_OK(hipEventCreateWithFlags(&l.rb_ev, hipEventDisableTiming));
     }
-    while (e->sync_ev.size() < 2 * (RSH_MAX_PASSES + 2)) {
+    while (e->sync_ev.size() < 3 * (RSH_MAX_PASSES + 2)) {
         hipEvent_t ev;
         HIP_OK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
         e->sync_ev.push_back(ev);
@@ -402,7 +404,7 @@ rsh_status ensure_frame(rsh_engine* e, uint32_t W, uint32_t H, uint32_t Lmax, ui
         e->cap_slots = S;
     }
     if (Ss > e->cap_shadow_slots) {
-        HIP_OK(e->shadow.reserve(Ss * 4));
+        HIP_OK(e->shadow[0].reserve(Ss * 4)); HIP_OK(e->shadow[1].reserve(Ss * 4));
         e->cap_shadow_slots = Ss;
     }
     if (P > e->cap_pixels) {
@@ -636,7 +638,7 @@ rsh_status build_items(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, Trav
 
 // One ray batch: trim -> (compress -> sort -> decompress) -> gather ->
 // hierarchy -> fused traversal + intersection.  kind 0: closest hit into
-// closest; kind 1: shadow flags into e->shadow.
+// closest; kind 1: shadow flags (u32 per slot) into `closest`.
 rsh_status trace_pass(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, const float* rays, uint32_t S,
                       uint32_t kind, const float4* shadow_pos, uint32_t flags_mask, PassOut& out, uint32_t pass_idx,
                       DevBuf& closest, const float* tmax = nullptr, bool camera = false, uint32_t sort_bits = 0,
@@ -664,7 +666,7 @@ rsh_status trace_pass(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, const
         // slot flags are bytes; a live slot's value is its index (creation writes no values)
         LANE_OK(run_pack(u8p(L.flags), u32p(L.keys), nullptr, S, u32p(L.tvalues), u32p(L.bases),
                         u32p(L.ckeys), u32p(L.cvalues), dc, kind == 0 ? u64p(closest) : nullptr, miss,
-                        kind == 1 ? u32p(e->shadow) : nullptr, L.sc, s, poll ? L.hsize_dev : nullptr,
+                        kind == 1 ? u32p(closest) : nullptr, L.sc, s, poll ? L.hsize_dev : nullptr,
                         poll ? ++L.hseq : 0u));
     }
     // Device-sized pass (a capture for any camera): N and C stay in device
@@ -784,7 +786,7 @@ rsh_status trace_pass(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, const
     p.obj_sphere = f4p(e->obj_sphere); p.obj_range = e->obj_range.as<uint4>(); p.K = e->K;
     p.cl_sphere = f4p(e->cl_sphere); p.sub_sphere = f4p(e->sub_sphere); p.sub_use = e->sub_use.as<uint8_t>(); p.obj_cl_base = u32p(e->cl_base); p.n_clusters = e->NC;
     p.algorithm = cfg.algorithm; p.kind = kind; p.hit_mode = cfg.hit_mode;
-    p.closest = u64p(closest); p.shadow = u32p(e->shadow);
+    p.closest = u64p(closest); p.shadow = u32p(closest);       // by kind
     p.counters = counting ? u64p(L.counters) : nullptr;
     p.force_generic = cfg.traversal_kernel == 1 ? 1u : 0u;
     // Traversal mode: 0 exact tests only, 1 ray / triangle-sphere pre-test, 2 that
@@ -884,7 +886,7 @@ rsh_status record(rsh_engine* e, Lane& L, rsh_frame_stats* st, uint32_t idx, uin
         k.kind = kind; k.slots = S; k.set = true;
         size_t bytes = kind == 1 ? (size_t)S * 4 : (size_t)S * 8;
         LANE_OK(k.buf.reserve(bytes));
-        LANE_OK(hipMemcpyAsync(k.buf.p, kind == 1 ? e->shadow.p : closest.p, bytes, hipMemcpyDeviceToDevice,
+        LANE_OK(hipMemcpyAsync(k.buf.p, closest.p, bytes, hipMemcpyDeviceToDevice,
                                L.stream));
     }
     return RSH_OK;
@@ -930,7 +932,7 @@ void rsh_engine_destroy(rsh_engine* e) {
                       &e->light_int, &e->tri_sphere, &e->tri_mt, &e->obj_sphere, &e->obj_range, &e->obj_order,
                       &e->cl_sphere,
                       &e->sub_sphere, &e->sub_use, &e->cl_base, &e->all_list, &e->rays_a, &e->rays_b, &e->closest[0],
-                      &e->closest[1], &e->shadow, &e->pixels, &e->views_d, &e->crange, &e->srange, &e->tri_pos0,
+                      &e->closest[1], &e->shadow[0], &e->shadow[1], &e->pixels, &e->views_d, &e->crange, &e->srange, &e->tri_pos0,
                       &e->tri_nrm0, &e->tri_obj, &e->sph0, &e->mats, &e->sph_tmp})
         b->release();
     if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
@@ -1181,9 +1183,11 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
     float* rays_a = e->rays_a.as<float>();
     float* rays_b = e->rays_b.as<float>();
     // cross-lane events: evG[k] G-buffer k written on lane 0 (k = 0: primary
-    // hits, k = b: bounce b), evS[k] shadow pass of G-buffer k done on lane 1
+    // hits, k = b: bounce b), evS[k] shadow pass of G-buffer k done on lane 1,
+    // evD[k] G-buffer k shaded on lane 0 (its shadow flags free again)
     hipEvent_t* evG = e->sync_ev.data();
     hipEvent_t* evS = e->sync_ev.data() + RSH_MAX_PASSES + 2;
+    hipEvent_t* evD = e->sync_ev.data() + 2 * (RSH_MAX_PASSES + 2);
     // Pass indices (statistics, kept results): primary 0, shadow of the primary
     // hits 1, then per bounce b its reflection pass and, with bounce shadows,
     // the shadow pass of its hits.
@@ -1207,6 +1211,9 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
         Lane& L = B;
         const GBufView& G = k == 0 ? G0 : GB[k % 2];
         LANE_OK(hipStreamWaitEvent(sb, evG[k], 0));
+        // shadow[k % 2] holds G-buffer k - 2's flags until that is shaded
+        // (recorded before G-buffer k was published)
+        if (k >= 2) LANE_OK(hipStreamWaitEvent(sb, evD[k - 2], 0));
         {
             StageTimer t(B, ST_CREATE, timing);
             // no slot-order rays: the gather recomputes each live ray
@@ -1214,10 +1221,10 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
                                  u8p(B.flags), u32p(B.keys), nullptr, sb);
         }
         PassOut po;
-        rsh_status r = trace_pass(e, B, cfg, nullptr, Ssh, 1, G.pos, fl, po, idx_shadow(k), e->closest[0], nullptr,
-                                  false, pass_bits);
+        rsh_status r = trace_pass(e, B, cfg, nullptr, Ssh, 1, G.pos, fl, po, idx_shadow(k), e->shadow[k % 2],
+                                  nullptr, false, pass_bits);
         if (r != RSH_OK) return r;
-        if ((r = record(e, B, st, idx_shadow(k), 1, Ssh, po, fl, e->closest[0])) != RSH_OK) return r;
+        if ((r = record(e, B, st, idx_shadow(k), 1, Ssh, po, fl, e->shadow[k % 2])) != RSH_OK) return r;
         LANE_OK(hipEventRecord(evS[k], sb));
         X.sh[k] = po;
         return RSH_OK;
@@ -1280,7 +1287,7 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
             if (ps.N > 0) {
                 StageTimer t(A, ST_SHADE, timing);
                 // device-sized capture: the shadow pass's live count gates the shading
-                launch_shade_primary(G0, W, H, Lm, sv.lights, V, u32p(e->shadow), cfg.soft_shadows, u32p(e->pixels),
+                launch_shade_primary(G0, W, H, Lm, sv.lights, V, u32p(e->shadow[0]), cfg.soft_shadows, u32p(e->pixels),
                                      sa, e->dev_sized ? u32p(e->frame_rec) + 2 * (size_t)idx_shadow(0) : nullptr);
             }
             return RSH_OK;
@@ -1289,7 +1296,7 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
         if (bs) {
             rsh_status r = await_shadow(k, ps);
             if (r != RSH_OK) return r;
-            sflags = u32p(e->shadow);
+            sflags = u32p(e->shadow[k % 2]);
         }
         StageTimer t(A, ST_SHADE, timing);
         launch_shade_reflection(GB[k % 2], W, H, Lm, sv.lights, V, u64p(e->closest[k % 2]), cfg.hit_mode, sflags,
@@ -1298,8 +1305,8 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
         return RSH_OK;
     };
     // lane 0: primary pass, then the mirror bounces; bounce b's hits alternate
-    // between closest[b % 2] / GB[b % 2], and each G-buffer is shaded once the
-    // next reflection pass has been traced (its shadow pass runs meanwhile)
+    // between closest[b % 2] / GB[b % 2], and G-buffer k is shaded just before
+    // bounce k + 2 starts (its shadow pass runs meanwhile, beside bounce k + 1)
     auto run_a = [&]() -> rsh_status {
         Lane& L = A;
         launch_fill_u32(u32p(e->pixels), 0u, P, sa);
@@ -1327,8 +1334,23 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
                            &V);
         }
         if ((r = publish(0)) != RSH_OK) return r;
-        uint32_t due = 0;      // G-buffer whose shading is due
+        // G-buffers are shaded in order, each once its shadow pass has run
+        // and no later than bounce k + 2, which reuses G-buffer k's hit
+        // records and G-buffer (closest[k % 2], GB[k % 2]) and whose shadow
+        // pass reuses its shadow flags (shadow[k % 2], released by evD[k]):
+        // the reflection chain runs a pass ahead of the shadow passes.
+        int64_t made = 0, shaded = -1;     // last G-buffer written, shaded
+        auto shade_upto = [&](int64_t j) -> rsh_status {
+            while (shaded < j && shaded < made) {
+                const rsh_status rr = shade((uint32_t)(shaded + 1));
+                if (rr != RSH_OK) return rr;
+                shaded++;
+                LANE_OK(hipEventRecord(evD[shaded], sa));
+            }
+            return RSH_OK;
+        };
         for (uint32_t b = 1; b <= bounces; b++) {
+            if ((r = shade_upto((int64_t)b - 2)) != RSH_OK) return r;
             DevBuf& cur = e->closest[b % 2];
             {
                 StageTimer t(A, ST_CREATE, timing);
@@ -1346,17 +1368,15 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
                 return r;
             if ((r = record(e, A, st, idx_refl(b), 2, S, pr, fl, cur)) != RSH_OK) return r;
             last_idx = std::max(last_idx, idx_refl(b));
-            if ((r = shade(due)) != RSH_OK) return r;
-            due = 0xFFFFFFFFu;
-            if (pr.N == 0) return RSH_OK;
+            if (pr.N == 0) break;
             {
                 StageTimer t(A, ST_SHADE, timing);
                 launch_surface(rays_a, u64p(cur), cfg.hit_mode, W, H, Lm, 0, sv, GB[b % 2], sa);
             }
+            made = b;
             if (bs && (r = publish(b)) != RSH_OK) return r;
-            due = b;
         }
-        if (due != 0xFFFFFFFFu && (r = shade(due)) != RSH_OK) return r;
+        if ((r = shade_upto(made)) != RSH_OK) return r;
         LANE_OK(hipGetLastError());
         return RSH_OK;
     };
@@ -1648,7 +1668,7 @@ rsh_status rsh_engine_trace(rsh_engine* e, const float* rays, const float* tmax,
         st->pass_mode[i] = po.mode;
         st->rays_total = po.N;
     }
-    HIP_OK(hipMemcpyAsync(out, kind == 1 ? e->shadow.p : e->closest[0].p, (size_t)n * (kind == 1 ? 4 : 8),
+    HIP_OK(hipMemcpyAsync(out, e->closest[0].p, (size_t)n * (kind == 1 ? 4 : 8),
                           hipMemcpyDeviceToDevice, s));
     HIP_OK(hipStreamSynchronize(s));
     if (st && (timing || (fl & 16u))) {

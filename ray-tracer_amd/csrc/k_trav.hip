@@ -358,11 +358,11 @@ __device__ __forceinline__ uint32_t object_at(const TravParams& P, uint32_t j) {
 // (pmask, ceil(K / 64) words per node); the list pass reads them back instead
 // of repeating the sphere tests.
 template <bool LIST>
-__global__ void __launch_bounds__(256) k_cull(TravParams P, uint32_t* npass, uint32_t* cand, uint4* obj_ent,
-                                              uint32_t* obj_cum, unsigned long long* pmask) {
+__global__ void __launch_bounds__(64) k_cull(TravParams P, uint32_t* npass, uint32_t* cand, uint4* obj_ent,
+                                             uint32_t* obj_cum, unsigned long long* pmask) {
     apply_sizes(P);
     const uint32_t lane = lane_id();
-    const uint32_t top = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t top = blockIdx.x;
     const uint32_t ntop = P.level_count[1];
     if (top > ntop || (LIST && top == ntop)) return;
     if (top == ntop) {
@@ -532,8 +532,8 @@ __global__ void __launch_bounds__(64) k_item_table(const uint32_t* cand, const f
 //   when the item is the node's only one (no merge needed).
 // The entry is found by binary search of the node's candidate offsets (RAH:
 // the shared list of all objects with the cluster bases as offsets).
-__global__ void __launch_bounds__(256) k_item_desc(TravParams P, uint4* desc) {
-    const uint32_t top = blockIdx.x * 4 + (threadIdx.x >> 6);
+__global__ void __launch_bounds__(64) k_item_desc(TravParams P, uint4* desc) {
+    const uint32_t top = blockIdx.x;
     if (top >= P.level_count[1]) return;
     const bool rah = P.algorithm == 1;
     const uint32_t f = P.item_first[top], k = P.item_first[top + 1] - f, ncand = P.cand[top];
@@ -563,13 +563,13 @@ __global__ void __launch_bounds__(256) k_item_desc(TravParams P, uint4* desc) {
 // lies in exactly one of them.  (Writing the descriptors per object instead
 // left one lane writing all of a large object's items while the others
 // idled.)
-__global__ void __launch_bounds__(256) k_cull_items(TravParams P, uint4* obj_ent, uint32_t* obj_cum,
-                                                    const unsigned long long* pmask, uint4* desc) {
-    __shared__ uint32_t s_c0[4][64];   // per wave: first cluster of each passing object of the word
-    __shared__ uint32_t s_e[4][64];    // and its list entry
+__global__ void __launch_bounds__(64) k_cull_items(TravParams P, uint4* obj_ent, uint32_t* obj_cum,
+                                                   const unsigned long long* pmask, uint4* desc) {
+    __shared__ uint32_t s_c0[1][64];   // first cluster of each passing object of the word
+    __shared__ uint32_t s_e[1][64];    // and its list entry
     apply_sizes(P);
-    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
-    const uint32_t top = blockIdx.x * 4 + w;
+    const uint32_t lane = lane_id(), w = 0;
+    const uint32_t top = blockIdx.x;
     if (top >= P.level_count[1]) return;
     // device-sized passes: a list or descriptor table over its capacity is
     // not written (the frame reports an overflow and is rendered again)
@@ -1425,14 +1425,14 @@ void launch_hierarchy(const float4* rec0, const float4* rec1, const PassSizes* p
 void launch_cull_count(const TravParams& p, uint32_t* npass, uint32_t* cand, unsigned long long* pmask,
                        hipStream_t s) {
     const uint32_t top = p.level_count[1];
-    k_cull<false><<<top / 4 + 1, 256, 0, s>>>(p, npass, cand, nullptr, nullptr, pmask);
+    k_cull<false><<<top + 1, 64, 0, s>>>(p, npass, cand, nullptr, nullptr, pmask);
 }
 
 void launch_cull_list(const TravParams& p, uint4* obj_ent, uint32_t* obj_cum, unsigned long long* pmask,
                       hipStream_t s) {
     const uint32_t top = p.level_count[1];
     if (top && p.algorithm != 1)
-        k_cull<true><<<(top + 3) / 4, 256, 0, s>>>(p, nullptr, nullptr, obj_ent, obj_cum, pmask);
+        k_cull<true><<<top, 64, 0, s>>>(p, nullptr, nullptr, obj_ent, obj_cum, pmask);
 }
 
 void launch_item_table(const uint32_t* cand, const float2* top_aux, uint32_t ntop, uint32_t max_cand,
@@ -1452,12 +1452,12 @@ void launch_item_count(const uint32_t* cand, uint32_t chunk, uint32_t ntop, uint
 void launch_cull_items(const TravParams& p, uint4* obj_ent, uint32_t* obj_cum, const unsigned long long* pmask,
                        uint4* desc, hipStream_t s) {
     const uint32_t top = p.level_count[1];
-    if (top) k_cull_items<<<(top + 3) / 4, 256, 0, s>>>(p, obj_ent, obj_cum, pmask, desc);
+    if (top) k_cull_items<<<top, 64, 0, s>>>(p, obj_ent, obj_cum, pmask, desc);
 }
 
 void launch_item_desc(const TravParams& p, uint4* desc, hipStream_t s) {
     const uint32_t ntop = p.level_count[1];
-    if (ntop) k_item_desc<<<(ntop + 3) / 4, 256, 0, s>>>(p, desc);
+    if (ntop) k_item_desc<<<ntop, 64, 0, s>>>(p, desc);
 }
 
 template <int LEAF>

@@ -93,7 +93,11 @@ typedef struct rsh_frame_config {
                                        per pass                                           */
     uint32_t traversal_prefilter;   /* 64-ray kernel: ray pre-tests before the exact test.
                                        1 none; 2 each ray's line against the triangle's
-                                       bounding sphere; 3 that and, per passing cluster,
+                                       bounding sphere (except for a frame's camera
+                                       rays, also the sphere's parameter range against
+                                       the part of the ray a hit can still use: beyond
+                                       EPS, short of a shadow ray's length or the closest
+                                       hit so far); 3 that and, per passing cluster,
                                        the rays' lines against the cluster's sphere (for
                                        spread-out rays); 0 automatic: 2 for a frame's
                                        camera rays; otherwise 3 when more than a

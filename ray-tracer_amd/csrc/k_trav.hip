@@ -734,7 +734,7 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 
     __shared__ LMask s_qmask[QCAP];            // leaves the survivor's cluster may pass
     __shared__ float4 s_mt[PRE ? 1 : 64][3];      // Moller-Trumbore vectors of a round's survivors
     __shared__ float4 s_ray0[64];              // (o + d*eps, tmax)
-    __shared__ float4 s_ray1[64];              // (d, valid)
+    __shared__ float4 s_ray1[64];              // (d, dot(d, d)), -1 for no ray
     __shared__ unsigned long long s_best[64];  // closest record / shadow flag per ray
     __shared__ uint16_t s_pair[PCAP];          // survivor << 5 | leaf
     __shared__ float4 s_lsph[BRANCH];          // leaf nodes of the top node
@@ -779,7 +779,7 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 
             tmax = b.w;
         }
         s_ray0[lane] = f4(o + d * EPS, tmax);
-        s_ray1[lane] = f4(d, valid ? 1.0f : 0.0f);
+        s_ray1[lane] = f4(d, valid ? dot(d, d) : -1.0f);     // NaN directions never hit: no ray either
     }
     const unsigned long long miss = P.kind == 1 ? 0ull : (P.hit_mode == 0 ? MISS64 : (unsigned long long)MISS32);
     // phase 2: a split node's item starts from what phase 1 merged
@@ -818,9 +818,9 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 
             float hi = r0.w;
             if (P.kind != 1)
                 hi = P.hit_mode == 0 ? __uint_as_float((uint32_t)(s_best[ray] >> 32)) : __int_as_float(0x7FC00000);
-            return ray_may_hit_sphere_range(xyz(r0), xyz(r1), sph, EPS, hi);
+            return ray_may_hit_sphere_range(xyz(r0), xyz(r1), r1.w, sph, EPS, hi);
         } else {
-            return ray_may_hit_sphere(xyz(r0), xyz(r1), sph);
+            return ray_may_hit_sphere(xyz(r0), xyz(r1), r1.w, sph);
         }
     };
 
@@ -1020,7 +1020,7 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 
                         const uint32_t sv = e >> 5;
                         const uint32_t ray = (e & 31u) * LEAF + lane % LEAF;
                         const float4 r0 = s_ray0[ray], r1 = s_ray1[ray];
-                        if (r1.w != 0.0f && (P.kind != 1 || s_best[ray] == 0ull)) {
+                        if (r1.w >= 0.0f && (P.kind != 1 || s_best[ray] == 0ull)) {
                             if (count) n_tests++;
                             keep = may_hit(ray, r0, r1, s_qsph[c0 + sv]);
                             code = (sv << 6) | ray;
@@ -1042,7 +1042,7 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 
                         const uint32_t sv = e >> 5;
                         const uint32_t ray = (e & 31u) * LEAF + lane % LEAF;
                         const float4 r0 = s_ray0[ray], r1 = s_ray1[ray];
-                        const bool live = r1.w != 0.0f && (P.kind != 1 || s_best[ray] == 0ull);
+                        const bool live = r1.w >= 0.0f && (P.kind != 1 || s_best[ray] == 0ull);
                         if (live) {
                             if (count) n_tests++;
                             exact(sv, ray, r0, r1);

@@ -352,11 +352,12 @@ __device__ __forceinline__ bool sphere_node_fast(float4 sphere, float4 cone, flo
 // 1e-6.  ray_triangle's FP32 evaluation can accept a point off the triangle
 // only within ~1e-6 of those magnitudes (its rounding is relative to |o - v0|,
 // |d| and the edges), so a rejected ray would be a miss there too.  NaN passes.
-__device__ __forceinline__ bool ray_may_hit_sphere(v3 o, v3 d, float4 sph) {
+// dd = dot(d, d), computed once per ray.
+__device__ __forceinline__ bool ray_may_hit_sphere(v3 o, v3 d, float dd, float4 sph) {
     const v3 w = xyz(sph) - o;
     const v3 x = cross(w, d);
     const float rr = sph.w + 1e-4f * (fabsf(w.x) + fabsf(w.y) + fabsf(w.z) + sph.w) + 1e-6f;
-    return !(dot(x, x) > rr * rr * dot(d, d));
+    return !(dot(x, x) > rr * rr * dd);
 }
 
 // ray_may_hit_sphere on the part of the line a hit can use: also false when
@@ -368,13 +369,12 @@ __device__ __forceinline__ bool ray_may_hit_sphere(v3 o, v3 d, float4 sph) {
 // positive, so no square root is taken.  lo is the hit threshold EPS; hi the
 // shadow ray's length or the parameter of the closest hit found so far (a
 // triangle starting beyond it cannot give a smaller record).  NaN or
-// infinite bounds never reject.
-__device__ __forceinline__ bool ray_may_hit_sphere_range(v3 o, v3 d, float4 sph, float lo, float hi) {
+// infinite bounds never reject.  dd = dot(d, d), computed once per ray.
+__device__ __forceinline__ bool ray_may_hit_sphere_range(v3 o, v3 d, float dd, float4 sph, float lo, float hi) {
     const v3 w = xyz(sph) - o;
     const v3 x = cross(w, d);
     const float l1 = fabsf(w.x) + fabsf(w.y) + fabsf(w.z) + sph.w;
     const float rr = sph.w + 1e-4f * l1 + 1e-6f;
-    const float dd = dot(d, d);
     if (dot(x, x) > rr * rr * dd) return false;
     const float rt = sph.w + 1e-3f * l1 + 1e-5f;
     const float A = rt * rt * dd;

@@ -782,9 +782,11 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 
         s_ray1[lane] = f4(d, valid ? dot(d, d) : -1.0f);     // NaN directions never hit: no ray either
     }
     const unsigned long long miss = P.kind == 1 ? 0ull : (P.hit_mode == 0 ? MISS64 : (unsigned long long)MISS32);
-    // phase 2: a split node's item starts from what phase 1 merged
-    const unsigned long long seed =
-        P.phase == 2 && P.kind != 1 && valid && (desc.w >> 31) == 0u ? P.closest[slot] : miss;
+    // phase 2: a split node's item starts from what phase 1 merged (the
+    // closest record, or the shadow ray already occluded)
+    unsigned long long seed = miss;
+    if (P.phase == 2 && valid && (desc.w >> 31) == 0u)
+        seed = P.kind == 1 ? (P.shadow[slot] != 0u ? 1ull : 0ull) : P.closest[slot];
     s_best[lane] = seed;
     if (lane < BRANCH) {
         const uint32_t bl = top * BRANCH + lane;

@@ -251,6 +251,16 @@ rsh_status rsh_engine_trace(rsh_engine* e, const float* rays, const float* tmax,
                             uint32_t flags, rsh_frame_stats* stats);
 
 rsh_status rsh_engine_read_pixels(rsh_engine* e, uint32_t* host_rgba, size_t count);
+/* The frame's pixels copied to host memory without waiting: the copy is
+ * queued behind the frame on a copy stream of the engine and the call returns
+ * at once; rsh_engine_wait_pixels waits for it.  The next frame clears its
+ * pixels only once the copy has ended (right before its first shading), so the
+ * copy runs beside that frame's first passes.  host_rgba must stay valid, and
+ * is not to be read, until the wait returns; page-locked memory
+ * (rsh_host_alloc) makes the copy one DMA transfer.  A graph-replayed frame
+ * (flag bit 3) first waits for a pending copy. */
+rsh_status rsh_engine_read_pixels_async(rsh_engine* e, uint32_t* host_rgba, size_t count);
+rsh_status rsh_engine_wait_pixels(rsh_engine* e);
 /* Primary G-buffer: diffuse, specular, position, normal, each float4[W*H]. */
 rsh_status rsh_engine_read_gbuffer(rsh_engine* e, float* host_dif, float* host_spec,
                                    float* host_pos, float* host_nrm, size_t count);

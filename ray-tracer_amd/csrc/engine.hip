@@ -115,6 +115,11 @@ struct rsh_engine {
     int device = 0;
     hipStream_t stream = nullptr;      // the caller's stream (lane 0)
     bool own_stream = false;
+    // rsh_engine_read_pixels_async: its copy stream, the frame's end, the
+    // copy's end, and whether a copy may still run
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t frame_done = nullptr, copy_done = nullptr;
+    bool copy_pending = false;
     std::string err;
 
     // scene
@@ -926,6 +931,12 @@ void rsh_engine_destroy(rsh_engine* e) {
     if (!e) return;
     (void)hipSetDevice(e->device);
     if (e->stream) (void)hipStreamSynchronize(e->stream);
+    if (e->copy_stream) {
+        (void)hipStreamSynchronize(e->copy_stream);
+        (void)hipStreamDestroy(e->copy_stream);
+    }
+    if (e->frame_done) (void)hipEventDestroy(e->frame_done);
+    if (e->copy_done) (void)hipEventDestroy(e->copy_done);
     for (Lane& L : e->lane) {
         if (L.stream) (void)hipStreamSynchronize(L.stream);
         for (DevBuf* b : L.bufs()) b->release();
@@ -1324,7 +1335,6 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
     // bounce k + 2 starts (its shadow pass runs meanwhile, beside bounce k + 1)
     auto run_a = [&]() -> rsh_status {
         Lane& L = A;
-        launch_fill_u32(u32p(e->pixels), 0u, P, sa);
         if (e->dev_sized) launch_fill_u32(u32p(e->frame_rec), 0u, 2 * RSH_MAX_PASSES + 1, sa);
         // CRSH camera rays come out of creation in key order when every key
         // is unique (one view, or the view id in the key)
@@ -1356,6 +1366,12 @@ rsh_status render_frame(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rs
         // the reflection chain runs a pass ahead of the shadow passes.
         int64_t made = 0, shaded = -1;     // last G-buffer written, shaded
         auto shade_upto = [&](int64_t j) -> rsh_status {
+            if (shaded < 0 && j >= 0) {
+                // the pixels are cleared right before the first shading, once
+                // an asynchronous copy of the last frame's has ended
+                if (e->copy_pending) LANE_OK(hipStreamWaitEvent(sa, e->copy_done, 0));
+                launch_fill_u32(u32p(e->pixels), 0u, P, sa);
+            }
             while (shaded < j && shaded < made) {
                 const rsh_status rr = shade((uint32_t)(shaded + 1));
                 if (rr != RSH_OK) return rr;
@@ -1479,6 +1495,11 @@ bool dev_sizable(const rsh_engine* e, const rsh_frame_config& cfg, uint64_t slot
 rsh_status render(rsh_engine* e, const ViewDesc* hv, uint32_t nv, const rsh_frame_config& cfg, uint32_t fl,
                   rsh_frame_stats* st) {
     if (!(fl & 8u)) return render_frame(e, hv, nv, cfg, fl, st);
+    // a graph clears the pixels without waiting for an asynchronous copy
+    if (e->copy_pending) {
+        HIP_OK(hipEventSynchronize(e->copy_done));
+        e->copy_pending = false;
+    }
     if (fl & 87u) {
         e->err = "render: graph replay keeps no passes, orders, counters or timers (flag bits 0-2, 4, 6)";
         return RSH_INVALID_ARG;
@@ -1694,6 +1715,30 @@ rsh_status rsh_engine_trace(rsh_engine* e, const float* rays, const float* tmax,
             st->stage_ms[k] += ms;
             if (t.first >= ST_PASS_TRAVERSE) st->pass_traverse_ms[0] += ms;
         }
+    }
+    return RSH_OK;
+}
+
+rsh_status rsh_engine_read_pixels_async(rsh_engine* e, uint32_t* host, size_t count) {
+    if (!e || !host || count > (size_t)e->W * e->H) return RSH_INVALID_ARG;
+    HIP_OK(hipSetDevice(e->device));
+    if (!e->copy_stream) HIP_OK(hipStreamCreateWithFlags(&e->copy_stream, hipStreamNonBlocking));
+    if (!e->frame_done) HIP_OK(hipEventCreateWithFlags(&e->frame_done, hipEventDisableTiming));
+    if (!e->copy_done) HIP_OK(hipEventCreateWithFlags(&e->copy_done, hipEventDisableTiming));
+    // behind the frame (and any earlier copy, on the same copy stream)
+    HIP_OK(hipEventRecord(e->frame_done, e->stream));
+    HIP_OK(hipStreamWaitEvent(e->copy_stream, e->frame_done, 0));
+    HIP_OK(hipMemcpyAsync(host, e->pixels.p, count * 4, hipMemcpyDeviceToHost, e->copy_stream));
+    HIP_OK(hipEventRecord(e->copy_done, e->copy_stream));
+    e->copy_pending = true;
+    return RSH_OK;
+}
+
+rsh_status rsh_engine_wait_pixels(rsh_engine* e) {
+    if (!e) return RSH_INVALID_ARG;
+    if (e->copy_pending) {
+        HIP_OK(hipEventSynchronize(e->copy_done));
+        e->copy_pending = false;
     }
     return RSH_OK;
 }

@@ -126,7 +126,7 @@ BINDERS = ["bindDiffuseTextureArray", "bindSpecularTextureArray", "bindFragmentP
            "bindLightPositions", "bindLightColors", "bindLightIntensities"]
 NATIVE_SYMBOLS = ["rsh_engine_create", "rsh_engine_destroy", "rsh_last_error", "rsh_engine_set_stream",
                   "rsh_engine_load_scene", "rsh_engine_render", "rsh_engine_render_views", "rsh_engine_trace",
-                  "rsh_engine_read_pixels",
+                  "rsh_engine_read_pixels", "rsh_engine_read_pixels_async", "rsh_engine_wait_pixels",
                   "rsh_engine_read_gbuffer", "rsh_engine_read_pass", "rsh_engine_pixels_device",
                   "rsh_device_alloc", "rsh_device_free", "rsh_memcpy_htod", "rsh_memcpy_dtoh",
                   "rsh_memset_d32", "rsh_device_synchronize", "rsh_device_count",
@@ -170,6 +170,10 @@ def lib():
                                    C.POINTER(FrameStats)]
     L.rsh_engine_read_pixels.restype = C.c_int
     L.rsh_engine_read_pixels.argtypes = [PV, PU, C.c_size_t]
+    L.rsh_engine_read_pixels_async.restype = C.c_int
+    L.rsh_engine_read_pixels_async.argtypes = [PV, PU, C.c_size_t]
+    L.rsh_engine_wait_pixels.restype = C.c_int
+    L.rsh_engine_wait_pixels.argtypes = [PV]
     L.rsh_engine_read_gbuffer.restype = C.c_int
     L.rsh_engine_read_gbuffer.argtypes = [PV, PF, PF, PF, PF, C.c_size_t]
     L.rsh_engine_read_pass.restype = C.c_int
@@ -497,14 +501,11 @@ class Engine:
         self._check(self.L.rsh_engine_read_pixels(self.h, out.ctypes.data_as(PU), out.size), "read_pixels")
         return out.reshape(H, W)
 
-    def read_pixels_pinned(self):
-        """The frame's pixels copied into a page-locked host buffer owned by
-        this engine (rsh_host_alloc; reused while the size fits): one DMA
-        transfer on the engine's stream.  Returns an H x W view of that buffer,
-        overwritten by the next call."""
+    def _pinned_pixels(self):
         W, H = self.cfg.width, self.cfg.height
         n = W * H
         if getattr(self, "_pinned_n", 0) < n:
+            self.wait_pixels()
             self._free_pinned()
             self.L.rsh_host_alloc.restype = PV
             self.L.rsh_host_alloc.argtypes = [C.c_size_t]
@@ -512,8 +513,32 @@ class Engine:
             if not p:
                 raise MemoryError("page-locked host allocation failed")
             self._pinned, self._pinned_n = p, n
+        return n, np.ctypeslib.as_array(C.cast(self._pinned, PU), shape=(n,)).reshape(H, W)
+
+    def read_pixels_pinned(self):
+        """The frame's pixels copied into a page-locked host buffer owned by
+        this engine (rsh_host_alloc; reused while the size fits): one DMA
+        transfer on the engine's stream.  Returns an H x W view of that buffer,
+        overwritten by the next call."""
+        n, view = self._pinned_pixels()
         self._check(self.L.rsh_engine_read_pixels(self.h, C.cast(self._pinned, PU), n), "read_pixels")
-        return np.ctypeslib.as_array(C.cast(self._pinned, PU), shape=(n,)).reshape(H, W)
+        return view
+
+    def read_pixels_async(self):
+        """Queue the frame's pixels for copying into the page-locked buffer of
+        read_pixels_pinned and return at once (rsh_engine_read_pixels_async);
+        wait_pixels() returns the H x W view once the copy has ended.  The next
+        frame's passes run beside the copy."""
+        n, _ = self._pinned_pixels()
+        self._check(self.L.rsh_engine_read_pixels_async(self.h, C.cast(self._pinned, PU), n), "read_pixels_async")
+
+    def wait_pixels(self):
+        """Wait for read_pixels_async's copy; the H x W view of its buffer."""
+        self._check(self.L.rsh_engine_wait_pixels(self.h), "wait_pixels")
+        if not getattr(self, "_pinned", None):
+            return None
+        W, H = self.cfg.width, self.cfg.height
+        return np.ctypeslib.as_array(C.cast(self._pinned, PU), shape=(W * H,)).reshape(H, W)
 
     def _free_pinned(self):
         if getattr(self, "_pinned", None):
@@ -575,6 +600,8 @@ class Engine:
                 for i in range(st.pass_count)]
 
     def close(self):
+        if getattr(self, "h", None):
+            self.L.rsh_engine_wait_pixels(self.h)      # no copy left writing the page-locked buffer
         self._free_pinned()
         if getattr(self, "h", None):
             self.L.rsh_engine_destroy(self.h)

@@ -220,3 +220,29 @@ def test_graph_replay_follows_the_camera(engine, rsh, variant):
         for k in range(eager.pass_count):
             assert st.pass_rays[k] == eager.pass_rays[k], (i, k)
     assert replayed >= 3
+
+
+def test_async_pixel_readback_beside_the_next_frame(engine, rsh):
+    """rsh_engine_read_pixels_async: the copy of frame A, queued before frame
+    B renders, holds A's image (B clears its pixels only once the copy has
+    ended); the copy of B holds B's; a graph-replayed frame after an
+    asynchronous read waits for it too."""
+    scene = rsh.scenes.config1_scene(boxes=300)
+    cfg = rsh.config.config1(96, 80).with_scene_bounds(scene)
+    engine.load_scene(scene)
+    rng = np.random.default_rng(5)
+    a, b = rsh.scenes.random_view(rng, 96, 80), rsh.scenes.random_view(rng, 96, 80)
+    engine.render(a, cfg)
+    want_a = engine.pixels().copy()
+    engine.render(b, cfg)
+    want_b = engine.pixels().copy()
+    assert not np.array_equal(want_a, want_b)
+    for flags in (0, engine.GRAPH):
+        for _ in range(2):
+            engine.render(a, cfg, flags=flags)
+            engine.read_pixels_async()
+            engine.render(b, cfg, flags=flags)
+            np.testing.assert_array_equal(engine.wait_pixels(), want_a, err_msg=f"flags {flags}")
+            engine.read_pixels_async()
+            np.testing.assert_array_equal(engine.wait_pixels(), want_b, err_msg=f"flags {flags}")
+    assert engine.wait_pixels() is not None      # no copy pending: returns at once

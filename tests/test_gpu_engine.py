@@ -113,6 +113,17 @@ def test_config2_full_scene_low_res(engine, oracle, rsh, atrium):
     run_case(engine, oracle, rsh, atrium, cam, rsh.config.config2(192, 108))
 
 
+def test_config2_full_frame_matches_oracle(engine, oracle, rsh, atrium):
+    """The bench's workload at its full size: the 1920x1080 atrium frame, 3
+    bounces with shadow rays at every hit (4.45M rays), with every engine
+    path the timed frames take (front-to-back camera pass in two launches,
+    reach-pruned shadow items, cluster ray masks, 256-cluster items), against
+    the oracle pass by pass: live rays, packets, per-slot hit records, the
+    G-buffer and the image, all bit-exact."""
+    cam = rsh.scenes.config2_camera(1920, 1080)
+    run_case(engine, oracle, rsh, atrium, cam, rsh.config.config2(1920, 1080))
+
+
 @pytest.mark.parametrize("variant", [
     dict(),
     dict(algorithm=1),

@@ -35,7 +35,8 @@ using namespace rsh;
 // sizes (the values the host otherwise reads back); a later run replays
 // those sizes without read-backs or synchronisation under stream capture.
 // Camera rays' pre-tests test the line only (the range test prunes little
-// where every ray starts at the eye).
+// where every ray starts at the eye): 1 in every launch, 2 in the front-to-back
+// pass's first launch only (the second starts from the first's hits), 0 never.
 #ifndef RSH_CAMERA_LINE_ONLY
 #define RSH_CAMERA_LINE_ONLY 1
 #endif
@@ -863,6 +864,7 @@ rsh_status trace_pass(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, const
             p.phase = 1;
             LANE_OK(launch_traverse(p, s));
             p.phase = 2;
+            if (RSH_CAMERA_LINE_ONLY == 2) p.line_only = 0;
         } else if (front) {
             p.phase = 3;
         }

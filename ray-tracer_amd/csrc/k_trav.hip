@@ -661,11 +661,12 @@ __global__ void __launch_bounds__(64) k_cull_items(TravParams P, uint4* obj_ent,
 // computes, per passing cluster, which of the 64 rays may meet the cluster's
 // sphere and lists only those (pair, ray) candidates.  MODE 2 pays for
 // spread-out secondary rays (2x on the atrium's bounces) and costs on coherent
-// ones; the engine picks the mode per pass.  Modes 1 and 2 use under 102 VGPRs
-// and 7.0 / 6.5 KB of LDS per wave, so five waves per SIMD fit (20 per CU of
-// 160 KB; mode 2's 8.9 KB with a 128-entry survivor queue allowed 17:
+// ones; the engine picks the mode per pass.  Mode 1 is built for six waves per
+// SIMD (80 VGPRs, 5.6 KB of LDS with a 64-entry survivor queue, one block of
+// triangle spheres per stage-A step: 24 waves per CU of 160 KB), mode 2 for
+// five (96 VGPRs, 6.5 KB; its 8.9 KB with a 128-entry queue allowed 17 per CU:
 // 1080p atrium frame 5.63 -> 5.40 ms, 64-view batch 59.6 -> 57.4 ms when it
-// went to 64 entries); mode 0, four.
+// went to 64 entries); mode 0, four.  LDS is allocated in 512-byte granules.
 // Occupancy targets (waves per SIMD) of the pre-test modes and of mode 0,
 // the blocks of triangle spheres loaded back to back in stage A, and the two
 // distance prunings (RSH_TRAV_TPRUNE: each ray's pre-test also bounds the
@@ -675,7 +676,13 @@ __global__ void __launch_bounds__(64) k_cull_items(TravParams P, uint4* obj_ent,
 // Measured on the 1080p atrium frame and its 64-view batch (median frame /
 // graph-replayed batch): 2 blocks 6.02 ms / 66.3 ms; 1 block 6.11 / 67.9;
 // 3 blocks 6.09 / 66.9; 4 blocks 6.37 / 70.3; 6 blocks 6.84 / 76.6.  Four or
-// six waves per SIMD for the pre-test modes measured slower than five.
+// six waves per SIMD for the pre-test modes measured slower than five while
+// mode 1's 7.0 KB of LDS left room for only 22 waves per CU.  With its queue
+// at 64 entries (5.6 KB) and one block per step, six waves per SIMD fit
+// with 2 VGPRs spilled: camera pass 0.81 -> 0.72 ms, frame 5.43 -> 5.31 ms,
+// batch 57.8 -> 55.1 ms (the 64-entry queue alone at five waves 0.78 ms; six
+// waves with two blocks per step spill 25 VGPRs and lose the first shadow
+// pass, 0.85 -> 0.95 ms; seven waves with one block spill 10, 5.33 ms).
 // Pruning, same measures: neither 6.04 / 66.3; TPRUNE 5.90 / 65.1; with
 // NPRUNE in every pass 6.05 / 65.2 (closest-hit passes lose: their reach is
 // infinite until every ray of the item has a hit), in mode 1 shadow passes
@@ -686,16 +693,16 @@ __global__ void __launch_bounds__(64) k_cull_items(TravParams P, uint4* obj_ent,
 // items had merged (5.99 / 65.5) and the exact mode for camera rays (6.14 /
 // 70.1) measured slower.
 #ifndef RSH_TRAV_WAVES_PRE
-#define RSH_TRAV_WAVES_PRE 5
+#define RSH_TRAV_WAVES_PRE 6
 #endif
 #ifndef RSH_TRAV_WAVES_MASKS
-#define RSH_TRAV_WAVES_MASKS RSH_TRAV_WAVES_PRE
+#define RSH_TRAV_WAVES_MASKS 5
 #endif
 #ifndef RSH_TRAV_WAVES_EXACT
 #define RSH_TRAV_WAVES_EXACT 4
 #endif
 #ifndef RSH_TRAV_NB
-#define RSH_TRAV_NB 2
+#define RSH_TRAV_NB 1
 #endif
 #ifndef RSH_TRAV_NB_MASKS
 #define RSH_TRAV_NB_MASKS 1
@@ -709,6 +716,9 @@ __global__ void __launch_bounds__(64) k_cull_items(TravParams P, uint4* obj_ent,
 #ifndef RSH_TRAV_NPRUNE
 #define RSH_TRAV_NPRUNE 1
 #endif
+#ifndef RSH_TRAV_QCAP_PRE
+#define RSH_TRAV_QCAP_PRE 64
+#endif
 template <int LEAF, int MODE>
 __global__ void __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 ? RSH_TRAV_WAVES_PRE
@@ -718,11 +728,11 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 
     constexpr uint32_t BRANCH = 64 / LEAF;
     constexpr uint32_t TQ = 128;                  // rings of (survivor, ray) tests
     constexpr unsigned long long LRAYS = LEAF >= 64 ? ~0ull : (1ull << LEAF) - 1ull;   // one leaf's rays
-    // Survivor queue.  A block adds up to 64 survivors; mode 2 keeps 64
+    // Survivor queue.  A block adds up to 64 survivors; modes 1 and 2 keep 64
     // entries (flushed when full, the block's remaining survivors queued
-    // after the flush) so that its LDS stays under 8 KB per wave and 20 waves
-    // fit a CU; the other modes keep 128 and flush once 64 are queued.
-    constexpr uint32_t QCAP = RM ? 64 : 128;
+    // after the flush) so that their LDS stays small enough for 24 and 20
+    // waves per CU; mode 0 keeps 128 and flushes once 64 are queued.
+    constexpr uint32_t QCAP = RM ? 64 : RSH_TRAV_QCAP_PRE;
     // leaf masks, one bit per leaf of the top node
     using LMask = typename std::conditional<BRANCH <= 8, uint8_t,
                   typename std::conditional<BRANCH <= 16, uint16_t, uint32_t>::type>::type;

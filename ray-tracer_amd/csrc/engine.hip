@@ -66,6 +66,10 @@ using namespace rsh;
 #ifndef RSH_SHADOW_MODE1
 #define RSH_SHADOW_MODE1 0
 #endif
+// Mode 2 when more than 1/RSH_WIDE_DIV of a pass's top nodes are wide.
+#ifndef RSH_WIDE_DIV
+#define RSH_WIDE_DIV 16
+#endif
 
 struct PassPlan { uint32_t N = 0, C = 0, chunk = 0, list_total = 0, items = 0, mode = 0; };
 
@@ -604,7 +608,7 @@ rsh_status build_items(rsh_engine* e, Lane& L, const rsh_frame_config& cfg, Trav
         if (cfg.traversal_prefilter == 0)
             p.prefilter = camera ? RSH_CAMERA_MODE
                           : (RSH_SHADOW_MODE1 && p.kind == 1) ? 1u
-                          : 16 * ht[TABLE_WIDE] > ntop ? 2u : 1u;
+                          : RSH_WIDE_DIV * ht[TABLE_WIDE] > ntop ? 2u : 1u;
         if (tabled) {
             if (!cfg.traversal_chunk) {
                 // passes with plenty of items take longer items (less per-item

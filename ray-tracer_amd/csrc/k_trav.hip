@@ -719,6 +719,11 @@ __global__ void __launch_bounds__(64) k_cull_items(TravParams P, uint4* obj_ent,
 #ifndef RSH_TRAV_QCAP_PRE
 #define RSH_TRAV_QCAP_PRE 64
 #endif
+// Mode 2: a block's cluster ray mask before its triangle spheres are loaded
+// and tested (a block no ray may reach is skipped whole).
+#ifndef RSH_TRAV_RM_FIRST
+#define RSH_TRAV_RM_FIRST 1
+#endif
 template <int LEAF, int MODE>
 __global__ void __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 ? RSH_TRAV_WAVES_PRE
@@ -1152,6 +1157,13 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 
         return true;
     };
 
+    // leaves holding a ray of mask r
+    auto leaves_of = [&](uint64_t r) {
+        uint32_t lr = 0;
+#pragma unroll
+        for (uint32_t b = 0; b < BRANCH; b++) lr |= (((r >> (b * LEAF)) & LRAYS) != 0ull ? 1u : 0u) << b;
+        return lr;
+    };
     // NB blocks per iteration: their sphere loads are issued back to back and
     // consumed in order, so only the first load's latency is exposed.
     constexpr int NB = RM ? RSH_TRAV_NB_MASKS : RSH_TRAV_NB;
@@ -1160,12 +1172,22 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 
         uint32_t tbk[NB], cmk[NB], cidk[NB];
         bool ink[NB];
         float4 tsk[NB];
+        uint64_t rmk[RM ? NB : 1];
 #pragma unroll
         for (int k = 0; k < NB; k++) {
             uint32_t tb = 0, tl = 0, cm = 0, cid = 0;
             have = have && next_block(tb, tl, cm, cid);
+            if constexpr (RM) {
+                // rays whose line (and usable range) may meet the cluster's sphere;
+                // counting runs keep every ray, so the counts stay the reference's
+                rmk[k] = ~0ull;
+                if (RSH_TRAV_RM_FIRST && have && !count) {
+                    rmk[k] = __ballot(((alive >> lane) & 1ull) && may_hit(lane, s_ray0[lane], s_ray1[lane], P.cl_sphere[cid]));
+                    cm &= leaves_of(rmk[k]);
+                }
+            }
             const uint32_t t = tb + lane;
-            ink[k] = have && t <= tl && !(t >= e_exlo && t < e_exhi);
+            ink[k] = have && cm != 0u && t <= tl && !(t >= e_exlo && t < e_exhi);
             tbk[k] = tb;
             cmk[k] = cm;
             cidk[k] = cid;
@@ -1191,18 +1213,15 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 
                 // leaves holding one of them.  Counting runs keep every ray and
                 // leaf, so the counts stay the reference's.
                 if (!count) {
-                    auto leaves_of = [&](uint64_t r) {
-                        uint32_t lr = 0;
-#pragma unroll
-                        for (uint32_t b = 0; b < BRANCH; b++)
-                            lr |= (((r >> (b * LEAF)) & LRAYS) != 0ull ? 1u : 0u) << b;
-                        return lr;
-                    };
                     const float4 ro = s_ray0[lane], rd = s_ray1[lane];
                     const uint32_t use = P.sub_use[cidk[k]];
-                    rm = __ballot(((alive >> lane) & 1ull) && may_hit(lane, ro, rd, P.cl_sphere[cidk[k]]));
-                    cm &= leaves_of(rm);
-                    if (cm == 0u) continue;
+                    if (RSH_TRAV_RM_FIRST) {
+                        rm = rmk[k];         // cm already holds only leaves with such rays
+                    } else {
+                        rm = __ballot(((alive >> lane) & 1ull) && may_hit(lane, ro, rd, P.cl_sphere[cidk[k]]));
+                        cm &= leaves_of(rm);
+                        if (cm == 0u) continue;
+                    }
                     // sub-clusters worth a mask: small enough (sub_use) and survivors
                     // x rays at least SUB_MIN
                     const uint32_t nrc = (uint32_t)__popcll(rm);

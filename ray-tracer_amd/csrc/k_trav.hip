@@ -661,12 +661,18 @@ __global__ void __launch_bounds__(64) k_cull_items(TravParams P, uint4* obj_ent,
 // computes, per passing cluster, which of the 64 rays may meet the cluster's
 // sphere and lists only those (pair, ray) candidates.  MODE 2 pays for
 // spread-out secondary rays (2x on the atrium's bounces) and costs on coherent
-// ones; the engine picks the mode per pass.  Mode 1 is built for six waves per
-// SIMD (80 VGPRs, 5.6 KB of LDS with a 64-entry survivor queue, one block of
-// triangle spheres per stage-A step: 24 waves per CU of 160 KB), mode 2 for
-// five (96 VGPRs, 6.5 KB; its 8.9 KB with a 128-entry queue allowed 17 per CU:
-// 1080p atrium frame 5.63 -> 5.40 ms, 64-view batch 59.6 -> 57.4 ms when it
-// went to 64 entries); mode 0, four.  LDS is allocated in 512-byte granules.
+// ones; the engine picks the mode per pass.  Modes 1 and 2 are built for six
+// waves per SIMD (80 VGPRs, no spills, in the instances without counters;
+// 5.6 / 6.5 KB of LDS with 64-entry survivor queues, one block of triangle
+// spheres per stage-A step: 24 waves per CU of 160 KB; mode 2's 8.9 KB with a
+// 128-entry queue allowed 17 per CU: 1080p atrium frame 5.63 -> 5.40 ms,
+// 64-view batch 59.6 -> 57.4 ms when it went to 64 entries); mode 0, four.
+// LDS is allocated in 512-byte granules.  The counting instances (CNT) carry
+// the reference's pair and test counters; without them the others need 16
+// fewer scalar and 4 fewer vector registers, which removed every VGPR spill
+// (frame 5.22 -> 5.10 ms, batch 52.7 -> 50.9 ms) and let mode 2 reach six
+// waves per SIMD too (its passes 2 % faster; with counters it spilled 10
+// VGPRs there and lost).
 // Occupancy targets (waves per SIMD) of the pre-test modes and of mode 0,
 // the blocks of triangle spheres loaded back to back in stage A, and the two
 // distance prunings (RSH_TRAV_TPRUNE: each ray's pre-test also bounds the
@@ -696,7 +702,7 @@ __global__ void __launch_bounds__(64) k_cull_items(TravParams P, uint4* obj_ent,
 #define RSH_TRAV_WAVES_PRE 6
 #endif
 #ifndef RSH_TRAV_WAVES_MASKS
-#define RSH_TRAV_WAVES_MASKS 5
+#define RSH_TRAV_WAVES_MASKS 6
 #endif
 #ifndef RSH_TRAV_WAVES_EXACT
 #define RSH_TRAV_WAVES_EXACT 4
@@ -724,7 +730,8 @@ __global__ void __launch_bounds__(64) k_cull_items(TravParams P, uint4* obj_ent,
 #ifndef RSH_TRAV_RM_FIRST
 #define RSH_TRAV_RM_FIRST 1
 #endif
-template <int LEAF, int MODE>
+// CNT: the counting instance (P.counters set); the other carries no counters.
+template <int LEAF, int MODE, bool CNT>
 __global__ void __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 ? RSH_TRAV_WAVES_PRE
                                                                    : RSH_TRAV_WAVES_EXACT))) k_traverse64(TravParams P) {
@@ -819,7 +826,7 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 
     const float4 tS = P.nodes[2 * (size_t)gt], tC = P.nodes[2 * (size_t)gt + 1];
     const float2 tA = P.aux[gt];
     const uint64_t alive = __ballot(valid);
-    const bool count = P.counters != nullptr;
+    const bool count = CNT && P.counters != nullptr;
     uint64_t n_top = 0, n_surv = 0, n_leaf = 0, n_tests = 0, n_bsteps = 0, n_csteps = 0, n_mt = 0, n_psteps = 0;
     bool done = false;
     uint32_t qlen = 0;
@@ -1493,10 +1500,11 @@ void launch_item_desc(const TravParams& p, uint4* desc, hipStream_t s) {
 
 template <int LEAF>
 hipError_t launch64(const TravParams& p, uint32_t blocks, hipStream_t s) {
+    const bool c = p.counters != nullptr;
     switch (p.prefilter) {
-        case 0: k_traverse64<LEAF, 0><<<blocks, 64, 0, s>>>(p); break;
-        case 1: k_traverse64<LEAF, 1><<<blocks, 64, 0, s>>>(p); break;
-        case 2: k_traverse64<LEAF, 2><<<blocks, 64, 0, s>>>(p); break;
+        case 0: c ? k_traverse64<LEAF, 0, true><<<blocks, 64, 0, s>>>(p) : k_traverse64<LEAF, 0, false><<<blocks, 64, 0, s>>>(p); break;
+        case 1: c ? k_traverse64<LEAF, 1, true><<<blocks, 64, 0, s>>>(p) : k_traverse64<LEAF, 1, false><<<blocks, 64, 0, s>>>(p); break;
+        case 2: c ? k_traverse64<LEAF, 2, true><<<blocks, 64, 0, s>>>(p) : k_traverse64<LEAF, 2, false><<<blocks, 64, 0, s>>>(p); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -826,6 +826,9 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 
     const float4 tS = P.nodes[2 * (size_t)gt], tC = P.nodes[2 * (size_t)gt + 1];
     const float2 tA = P.aux[gt];
     const uint64_t alive = __ballot(valid);
+    // rays that may still change their result: a shadow ray drops out once occluded
+    uint64_t lmask = alive;
+    if (P.kind == 1) lmask &= ~__ballot(seed != 0ull);
     const bool count = CNT && P.counters != nullptr;
     uint64_t n_top = 0, n_surv = 0, n_leaf = 0, n_tests = 0, n_bsteps = 0, n_csteps = 0, n_mt = 0, n_psteps = 0;
     bool done = false;
@@ -1075,7 +1078,10 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 
                 }
             }
             __builtin_amdgcn_wave_barrier();
-            if (P.kind == 1 && (__ballot(s_best[lane] != 0ull) & alive) == alive) done = true;
+            if (P.kind == 1) {
+                lmask = alive & ~__ballot(s_best[lane] != 0ull);
+                if (lmask == 0ull) done = true;
+            }
         }
         qlen = 0;
         update_reach();
@@ -1189,7 +1195,7 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 
                 // counting runs keep every ray, so the counts stay the reference's
                 rmk[k] = ~0ull;
                 if (RSH_TRAV_RM_FIRST && have && !count) {
-                    rmk[k] = __ballot(((alive >> lane) & 1ull) && may_hit(lane, s_ray0[lane], s_ray1[lane], P.cl_sphere[cid]));
+                    rmk[k] = __ballot(((lmask >> lane) & 1ull) && may_hit(lane, s_ray0[lane], s_ray1[lane], P.cl_sphere[cid]));
                     cm &= leaves_of(rmk[k]);
                 }
             }

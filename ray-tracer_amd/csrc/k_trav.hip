@@ -529,7 +529,8 @@ __global__ void __launch_bounds__(64) k_item_table(const uint32_t* cand, const f
 // Work-item descriptors, one wave per top node, one lane per item:
 //   x top node, y absolute list entry holding the item's first cluster,
 //   z first cluster within that entry, w clusters of the item | (1 << 31)
-//   when the item is the node's only one (no merge needed).
+//   when the item is the node's only one (no merge needed) | (1 << 30) when
+//   it is the node's first.
 // The entry is found by binary search of the node's candidate offsets (RAH:
 // the shared list of all objects with the cluster bases as offsets).
 __global__ void __launch_bounds__(64) k_item_desc(TravParams P, uint4* desc) {
@@ -547,7 +548,8 @@ __global__ void __launch_bounds__(64) k_item_desc(TravParams P, uint4* desc) {
             const uint32_t m = (a + b) >> 1;
             if (P.obj_cum[lbase + m] <= s0) a = m; else b = m;
         }
-        desc[f + c] = make_uint4(top, lbase + a, s0 - P.obj_cum[lbase + a], rem | (k == 1 ? 0x80000000u : 0u));
+        desc[f + c] = make_uint4(top, lbase + a, s0 - P.obj_cum[lbase + a],
+                                 rem | (k == 1 ? 0x80000000u : 0u) | (c == 0 ? 0x40000000u : 0u));
     }
 }
 
@@ -556,7 +558,8 @@ __global__ void __launch_bounds__(64) k_item_desc(TravParams P, uint4* desc) {
 // mask word) at a time, writes each as a list entry with the node's running
 // cluster count before it (as k_cull<true>), and then the descriptors of the
 // items whose first cluster s0 = i * chunk falls inside those objects'
-// clusters, one item per lane: (top, entry, s0 - c0, clusters | single << 31)
+// clusters, one item per lane: (top, entry, s0 - c0, clusters | single << 31
+// | first << 30)
 // for the object with the largest first cluster c0 <= s0 (a binary search
 // over the word's objects in LDS) -- the entry k_item_desc finds by binary
 // search.  Every listed object has at least one cluster, so each item's start
@@ -617,7 +620,7 @@ __global__ void __launch_bounds__(64) k_cull_items(TravParams P, uint4* obj_ent,
                 if (s_c0[w][mid] <= s0) lo = mid; else hi = mid;
             }
             const uint32_t rem = ncand - s0 < chunk ? ncand - s0 : chunk;
-            desc[f + i] = make_uint4(top, s_e[w][lo], s0 - s_c0[w][lo], rem | single);
+            desc[f + i] = make_uint4(top, s_e[w][lo], s0 - s_c0[w][lo], rem | single | (i == 0 ? 0x40000000u : 0u));
         }
         __builtin_amdgcn_wave_barrier();      // the word's LDS entries are read before the next word's
         n += cnt;
@@ -786,7 +789,7 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 
     const uint4 desc = P.item_desc[item];
     const uint32_t top = desc.x;
     if (top >= P.level_count[1]) return;
-    if (P.phase == 2 && item == P.item_first[top]) return;     // ran in phase 1
+    if (P.phase == 2 && (desc.w & 0x40000000u)) return;     // the node's first item ran in phase 1
     const uint32_t pos = top * 64 + lane;
     const bool valid = pos < P.N;
 
@@ -1093,7 +1096,7 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 
     const bool single = (desc.w >> 31) != 0u;
     const float2 t_bound = node_bound(tS, tC, tA.x, tA.y);
     uint64_t n_cl = 0;
-    uint32_t rem = desc.w & 0x7FFFFFFFu;      // clusters of the item not yet taken from an entry
+    uint32_t rem = desc.w & 0x3FFFFFFFu;      // clusters of the item not yet taken from an entry
     uint32_t j = desc.y;                       // current entry (absolute index) of the object list
     uint32_t e_first = 0, e_last = 0, e_exlo = 0, e_exhi = 0, e_cbase = 0, e_ncl = 0;
     uint32_t lc_cur = 0, lc_end = 0;           // local clusters of the entry still to pre-test

@@ -145,7 +145,7 @@ struct TravParams {
     uint32_t force_generic;       // 1: use the generic kernel even for 64-ray top nodes
     uint32_t splits;              // generic kernel: waves sharing one top node's triangle range
     // work items of the 64-ray kernel: (top node, chunk of candidate clusters)
-    const uint4* item_desc;       // [items] (top, list entry, first cluster in it, clusters | single << 31)
+    const uint4* item_desc;       // [items] (top, list entry, first cluster in it, clusters | single << 31 | first << 30)
     const uint32_t* item_first;   // [n_top + 1] first item of each top node
     const uint32_t* cand;         // [n_top + 1] candidate clusters of each top node
     const uint32_t* obj_off;      // [n_top + 1] first list entry of each top node (CRSH)

@@ -733,6 +733,13 @@ __global__ void __launch_bounds__(64) k_cull_items(TravParams P, uint4* obj_ent,
 #ifndef RSH_TRAV_RM_FIRST
 #define RSH_TRAV_RM_FIRST 1
 #endif
+// Mode 1: a block whose cluster sphere no live ray's line may meet is
+// skipped before its triangle spheres are loaded, and its leaf mask keeps
+// only leaves holding such a ray (1: shadow passes; 2: every mode-1 pass;
+// 3: shadow passes and passes without a front-to-back order).
+#ifndef RSH_TRAV_SKIP_PRE
+#define RSH_TRAV_SKIP_PRE 3
+#endif
 // CNT: the counting instance (P.counters set); the other carries no counters.
 template <int LEAF, int MODE, bool CNT>
 __global__ void __launch_bounds__(64)
@@ -1201,6 +1208,10 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 
                     rmk[k] = __ballot(((lmask >> lane) & 1ull) && may_hit(lane, s_ray0[lane], s_ray1[lane], P.cl_sphere[cid]));
                     cm &= leaves_of(rmk[k]);
                 }
+            } else if constexpr (MODE == 1) {
+                if (RSH_TRAV_SKIP_PRE != 0 && have && !count &&
+                    (RSH_TRAV_SKIP_PRE == 2 || P.kind == 1 || (RSH_TRAV_SKIP_PRE == 3 && P.obj_order == nullptr)))
+                    cm &= leaves_of(__ballot(((lmask >> lane) & 1ull) && may_hit(lane, s_ray0[lane], s_ray1[lane], P.cl_sphere[cid])));
             }
             const uint32_t t = tb + lane;
             ink[k] = have && cm != 0u && t <= tl && !(t >= e_exlo && t < e_exhi);

@@ -836,9 +836,6 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 
     const float4 tS = P.nodes[2 * (size_t)gt], tC = P.nodes[2 * (size_t)gt + 1];
     const float2 tA = P.aux[gt];
     const uint64_t alive = __ballot(valid);
-    // rays that may still change their result: a shadow ray drops out once occluded
-    uint64_t lmask = alive;
-    if (P.kind == 1) lmask &= ~__ballot(seed != 0ull);
     const bool count = CNT && P.counters != nullptr;
     uint64_t n_top = 0, n_surv = 0, n_leaf = 0, n_tests = 0, n_bsteps = 0, n_csteps = 0, n_mt = 0, n_psteps = 0;
     bool done = false;
@@ -1088,10 +1085,7 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 
                 }
             }
             __builtin_amdgcn_wave_barrier();
-            if (P.kind == 1) {
-                lmask = alive & ~__ballot(s_best[lane] != 0ull);
-                if (lmask == 0ull) done = true;
-            }
+            if (P.kind == 1 && (__ballot(s_best[lane] != 0ull) & alive) == alive) done = true;
         }
         qlen = 0;
         update_reach();
@@ -1180,6 +1174,8 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 
         return true;
     };
 
+    // the lane's ray may still change its result (a shadow ray drops out once occluded)
+    auto live_lane = [&]() -> bool { return valid && (P.kind != 1 || s_best[lane] == 0ull); };
     // leaves holding a ray of mask r
     auto leaves_of = [&](uint64_t r) {
         uint32_t lr = 0;
@@ -1205,13 +1201,13 @@ __attribute__((amdgpu_waves_per_eu(MODE == 2 ? RSH_TRAV_WAVES_MASKS : MODE == 1 
                 // counting runs keep every ray, so the counts stay the reference's
                 rmk[k] = ~0ull;
                 if (RSH_TRAV_RM_FIRST && have && !count) {
-                    rmk[k] = __ballot(((lmask >> lane) & 1ull) && may_hit(lane, s_ray0[lane], s_ray1[lane], P.cl_sphere[cid]));
+                    rmk[k] = __ballot(live_lane() && may_hit(lane, s_ray0[lane], s_ray1[lane], P.cl_sphere[cid]));
                     cm &= leaves_of(rmk[k]);
                 }
             } else if constexpr (MODE == 1) {
                 if (RSH_TRAV_SKIP_PRE != 0 && have && !count &&
                     (RSH_TRAV_SKIP_PRE == 2 || P.kind == 1 || (RSH_TRAV_SKIP_PRE == 3 && P.obj_order == nullptr)))
-                    cm &= leaves_of(__ballot(((lmask >> lane) & 1ull) && may_hit(lane, s_ray0[lane], s_ray1[lane], P.cl_sphere[cid])));
+                    cm &= leaves_of(__ballot(live_lane() && may_hit(lane, s_ray0[lane], s_ray1[lane], P.cl_sphere[cid])));
             }
             const uint32_t t = tb + lane;
             ink[k] = have && cm != 0u && t <= tl && !(t >= e_exlo && t < e_exhi);

@@ -698,8 +698,10 @@ __global__ void __launch_bounds__(64) k_cull_items(TravParams P, uint4* obj_ent,
 // only 5.78 / 62.3.  With mode 2 at 20 waves per CU (64-entry survivor
 // queue): one block per step in mode 2 5.35 / 56.8 against two 5.44 / 57.4
 // (mode 1 keeps two); six waves per SIMD for mode 2 5.79 / 60.4; a
-// sub-cluster threshold of 16 or 32 instead of 64 within 0.5 %.  Seeding a split node's items with the records its other
-// items had merged (5.99 / 65.5) and the exact mode for camera rays (6.14 /
+// sub-cluster threshold of 16 or 32 instead of 64 within 0.5 % (at six waves
+// per SIMD with the block skips, 32 measured 1 % faster on the frame,
+// traversal 3.88 -> 3.81 ms, and is the default).  Seeding a split node's
+// items with the records its other items had merged (5.99 / 65.5) and the exact mode for camera rays (6.14 /
 // 70.1) measured slower.
 #ifndef RSH_TRAV_WAVES_PRE
 #define RSH_TRAV_WAVES_PRE 6
@@ -720,7 +722,7 @@ __global__ void __launch_bounds__(64) k_cull_items(TravParams P, uint4* obj_ent,
 #define RSH_TRAV_TPRUNE 1
 #endif
 #ifndef RSH_TRAV_SUB_MIN
-#define RSH_TRAV_SUB_MIN 64
+#define RSH_TRAV_SUB_MIN 32
 #endif
 #ifndef RSH_TRAV_NPRUNE
 #define RSH_TRAV_NPRUNE 1
